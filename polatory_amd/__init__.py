@@ -1,0 +1,22 @@
+"""polatory_amd: MI355X-native RBF fit/evaluate hot path behind polatory's API.
+
+Compute runs in hand-written gfx950 HIP kernels inside libpolatory_amd.so,
+reached through the C ABI declared in include/polatory_amd.h; this package is
+the host-side mirror used by tests and benchmarks.
+"""
+from . import _capi
+from .fmm import (FmmConfig, FmmEvaluator, FmmSymmetricEvaluator, direct_evaluate,
+                  make_fmm_evaluator, make_fmm_symmetric_evaluator)
+from .rbf import Biharmonic3D, CovExponential, Rbf, Triharmonic3D, make_rbf
+
+
+def device_count():
+    import ctypes
+    n = ctypes.c_int()
+    _capi.call("pam_device_count", ctypes.byref(n))
+    return n.value
+
+
+__all__ = ["Biharmonic3D", "Triharmonic3D", "CovExponential", "Rbf", "make_rbf",
+           "FmmConfig", "FmmEvaluator", "FmmSymmetricEvaluator", "make_fmm_evaluator",
+           "make_fmm_symmetric_evaluator", "direct_evaluate", "device_count"]

@@ -1,0 +1,122 @@
+"""ctypes binding of the C ABI in include/polatory_amd.h.
+
+The shared library is built in-tree (``polatory_amd/libpolatory_amd.so``) by
+``polatory_amd/build.py``. Loading fails loudly when it is missing: there is no
+CPU fallback for any entry point.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpolatory_amd.so")
+
+PAM_RBF_BH3, PAM_RBF_TH3, PAM_RBF_EXP = 0, 1, 2
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+MODE_AUTO, MODE_DIRECT, MODE_FMM = 0, 1, 2
+
+
+class PamRbf(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("params", ctypes.c_double * 2),
+                ("aniso", ctypes.c_double * 9)]
+
+
+class PamFmmConfig(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("order", ctypes.c_int),
+                ("leaf_level", ctypes.c_int), ("mode", ctypes.c_int)]
+
+
+class PamFmmStats(ctypes.Structure):
+    _fields_ = [("ms_tree", ctypes.c_double), ("ms_upward", ctypes.c_double),
+                ("ms_m2l", ctypes.c_double), ("ms_l2l", ctypes.c_double),
+                ("ms_l2p", ctypes.c_double), ("ms_p2p", ctypes.c_double),
+                ("ms_total", ctypes.c_double), ("precision", ctypes.c_int),
+                ("order", ctypes.c_int), ("leaf_level", ctypes.c_int), ("direct", ctypes.c_int),
+                ("n_src_leaves", ctypes.c_int64), ("n_trg_leaves", ctypes.c_int64)]
+
+
+class PamError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+class InvalidArgument(PamError, ValueError):
+    """std::invalid_argument in the reference API."""
+
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_D = ctypes.c_double
+_INT = ctypes.c_int
+
+# name -> argtypes (all return int status)
+_SIGS = {
+    "pam_last_error": [ctypes.c_char_p, ctypes.c_size_t],
+    "pam_device_count": [ctypes.POINTER(ctypes.c_int)],
+    "pam_fmm_evaluator_create": [ctypes.POINTER(PamRbf), _P, _P, ctypes.POINTER(_P)],
+    "pam_fmm_symmetric_evaluator_create": [ctypes.POINTER(PamRbf), _P, _P, ctypes.POINTER(_P)],
+    "pam_fmm_evaluator_destroy": [_P],
+    "pam_fmm_evaluator_set_accuracy": [_P, _D],
+    "pam_fmm_evaluator_set_source_points": [_P, _P, _I64],
+    "pam_fmm_evaluator_set_target_points": [_P, _P, _I64],
+    "pam_fmm_evaluator_set_points": [_P, _P, _I64],
+    "pam_fmm_evaluator_set_weights": [_P, _P, _I64],
+    "pam_fmm_evaluator_evaluate": [_P, _P, _I64],
+    "pam_fmm_evaluator_set_source_points_device": [_P, _P, _I64],
+    "pam_fmm_evaluator_set_target_points_device": [_P, _P, _I64],
+    "pam_fmm_evaluator_set_points_device": [_P, _P, _I64],
+    "pam_fmm_evaluator_set_weights_device": [_P, _P, _I64],
+    "pam_fmm_evaluator_evaluate_device": [_P, _P, _I64],
+    "pam_fmm_evaluator_synchronize": [_P],
+    "pam_fmm_evaluator_set_config": [_P, ctypes.POINTER(PamFmmConfig)],
+    "pam_fmm_evaluator_get_config": [_P, ctypes.POINTER(PamFmmConfig)],
+    "pam_fmm_evaluator_stats": [_P, ctypes.POINTER(PamFmmStats)],
+    "pam_fmm_evaluator_count_work": [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64)],
+    "pam_fmm_evaluator_tree_info": [_P, _INT, ctypes.POINTER(_INT), _P, ctypes.POINTER(_D),
+                                    ctypes.POINTER(_D), ctypes.POINTER(_I64), ctypes.POINTER(_I64)],
+    "pam_fmm_evaluator_export_tree": [_P, _INT, _P, _P, _P, _P],
+    "pam_fmm_evaluator_export_level": [_P, _INT, _INT, ctypes.POINTER(_I64), _P],
+    "pam_fmm_evaluator_export_p2p": [_P, _P],
+    "pam_fmm_evaluator_export_m2l": [_P, _INT, _P],
+    "pam_direct_evaluate": [ctypes.POINTER(PamRbf), _P, _I64, _P, _P, _I64, _P],
+}
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python polatory_amd/build.py` "
+                "(there is no CPU fallback)")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            f = getattr(lib, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        _lib = lib
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def last_error():
+    buf = ctypes.create_string_buffer(1024)
+    load().pam_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def check(code):
+    if code != 0:
+        msg = last_error()
+        if code == 1:
+            raise InvalidArgument(code, msg)
+        raise PamError(code, msg)
+
+
+def call(name, *args):
+    check(getattr(load(), name)(*args))

@@ -1,0 +1,36 @@
+// Host-side launchers of the device kernels (one translation unit per family).
+#pragma once
+
+#include "common.hpp"
+
+namespace pam {
+
+// ---- util.hip ---------------------------------------------------------------
+// out[i] = A * p_i for row-major n x 3 points, no contraction (bit-exact with a
+// CPU evaluation of ((A0 x + A1 y) + A2 z)).
+void launch_transform_points(const double* aniso, const double* in, int64_t n, double* out,
+                             hipStream_t s);
+// Componentwise min/max of n x 3 points; result lands in d_out[6] = {min xyz, max xyz}.
+void launch_bbox(const double* pts, int64_t n, double* d_out, DevBuf<double>& scratch,
+                 hipStream_t s);
+// Exclusive prefix sum of n int32 values (in may alias out). Returns nothing;
+// the total is out[n-1] + in[n-1] which callers read back when needed.
+void exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, DevBuf<int32_t>& scratch,
+                        hipStream_t s);
+// Stable LSD radix sort of (key, value) pairs on the low `bits` bits of key.
+void radix_sort_pairs(uint64_t* keys, int32_t* vals, int64_t n, int bits, DevBuf<uint64_t>& tk,
+                      DevBuf<int32_t>& tv, DevBuf<int32_t>& hist, DevBuf<int32_t>& scratch,
+                      hipStream_t s);
+// vals[i] = i
+void launch_iota(int32_t* vals, int64_t n, hipStream_t s);
+// out[i] = in[idx[i]] (gather) for doubles
+void launch_gather_f64(const double* in, const int32_t* idx, int64_t n, double* out, hipStream_t s);
+// FP64 direct sum: out[i] (+)= sum_j w[j] * phi(t_i - s_j) over transformed points.
+void direct_sum_f64(const RbfDesc& rbf, const double* src, const double* w, int64_t n_src,
+                    const double* trg, int64_t n_trg, double* out, bool accumulate,
+                    DevBuf<double>& scratch, hipStream_t s);
+// out[i] += sum_k c_k m_k(p_i) (monomials of degree 0..2, untransformed points).
+void launch_poly_eval(int degree, const double* pts, int64_t n, const double* d_coeffs,
+                      double* out, hipStream_t s);
+
+}  // namespace pam

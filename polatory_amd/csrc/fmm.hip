@@ -1,0 +1,664 @@
+// FMM stage kernels for gfx950.
+//   P2M / L2P : one workgroup per leaf, Lagrange weights staged in LDS, streaming
+//   M2M / L2L : separable 1-D transfers (3 passes of P^4) through LDS
+//   M2L       : gather-GEMM on MFMA (16x16x4 f32 or f64): 32 target cells of
+//               one octant x 64 operator rows per workgroup, the 189 transfer
+//               vectors of that octant reduced in registers
+//   P2P       : one workgroup per target leaf, the 27 neighbour leaves staged
+//               through LDS, targets x source-splits mapped onto 256 lanes
+#include <cmath>
+#include <type_traits>
+
+#include "fmm.hpp"
+
+namespace pam {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+// T_k(x_a) for the Chebyshev nodes x_a = cos((2a+1) pi / 2P): tk[a*P + k].
+template <class T, int P>
+__device__ __forceinline__ void init_tk(T* tk) {
+  for (int idx = threadIdx.x; idx < P * P; idx += blockDim.x) {
+    int a = idx / P, k = idx % P;
+    tk[idx] = static_cast<T>(cospi(static_cast<double>(k * (2 * a + 1)) / (2.0 * P)));
+  }
+}
+
+// Lagrange basis at the Chebyshev nodes: S_a(x) = 1/P + 2/P sum_{k>=1} T_k(x_a) T_k(x).
+template <class T, int P>
+__device__ __forceinline__ void lagrange(T xi, const T* __restrict__ tk, T* S) {
+  T t[P];
+  t[0] = T(1);
+  if constexpr (P > 1) t[1] = xi;
+#pragma unroll
+  for (int k = 2; k < P; k++) t[k] = T(2) * xi * t[k - 1] - t[k - 2];
+  const T c0 = T(1) / T(P), c1 = T(2) / T(P);
+#pragma unroll
+  for (int a = 0; a < P; a++) {
+    T sum = T(0);
+#pragma unroll
+    for (int k = 1; k < P; k++) sum += tk[a * P + k] * t[k];
+    S[a] = c0 + c1 * sum;
+  }
+}
+
+// tr[o][a][a2] = S_a((x_a2 + (2o - 1)) / 2): parent-node weight a of child node a2
+// for the child on side o of an axis. Computed in FP64.
+template <class T, int P>
+__device__ __forceinline__ void init_transfer(T* tr) {
+  for (int idx = threadIdx.x; idx < 2 * P * P; idx += blockDim.x) {
+    int o = idx / (P * P), a = (idx / P) % P, a2 = idx % P;
+    double x = (cospi((2.0 * a2 + 1.0) / (2.0 * P)) + (2.0 * o - 1.0)) / 2.0;
+    double t0 = 1.0, t1 = x, sum = 0.0;
+    for (int k = 1; k < P; k++) {
+      double tk = (k == 1) ? t1 : 2.0 * x * t1 - t0;
+      if (k >= 2) {
+        t0 = t1;
+        t1 = tk;
+      }
+      sum += cospi(static_cast<double>(k * (2 * a + 1)) / (2.0 * P)) * tk;
+    }
+    tr[idx] = static_cast<T>(1.0 / P + 2.0 / P * sum);
+  }
+}
+
+__device__ __forceinline__ uint64_t compact3(uint64_t v) {
+  v &= 0x1249249249249249ull;
+  v = (v ^ (v >> 2)) & 0x10c30c30c30c30c3ull;
+  v = (v ^ (v >> 4)) & 0x100f00f00f00f00full;
+  v = (v ^ (v >> 8)) & 0x1f0000ff0000ffull;
+  v = (v ^ (v >> 16)) & 0x1f00000000ffffull;
+  v = (v ^ (v >> 32)) & 0x1fffffull;
+  return v;
+}
+
+struct BoxArgs {
+  double lo[3];
+  double cell;   // leaf cell width
+};
+
+template <class T>
+__global__ void prepare_points_kernel(const double* __restrict__ pts, const double* __restrict__ w,
+                                      const int32_t* __restrict__ perm,
+                                      const int32_t* __restrict__ point_leaf,
+                                      const uint64_t* __restrict__ leaf_codes, int64_t n, BoxArgs b,
+                                      V4<T>* __restrict__ out) {
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int32_t j = perm[i];
+  uint64_t code = leaf_codes[point_leaf[i]];
+  double c[3] = {static_cast<double>(compact3(code >> 2)), static_cast<double>(compact3(code >> 1)),
+                 static_cast<double>(compact3(code))};
+  double inv_h = 2.0 / b.cell;
+  double xi[3];
+  for (int k = 0; k < 3; k++) {
+    double center = b.lo[k] + (c[k] + 0.5) * b.cell;
+    xi[k] = (pts[3 * j + k] - center) * inv_h;
+  }
+  V4<T> v;
+  v.x = static_cast<T>(xi[0]);
+  v.y = static_cast<T>(xi[1]);
+  v.z = static_cast<T>(xi[2]);
+  v.w = w ? static_cast<T>(w[j]) : T(0);
+  out[i] = v;
+}
+
+template <class T>
+__global__ void set_weights_kernel(const double* __restrict__ w, const int32_t* __restrict__ perm,
+                                   int64_t n, V4<T>* __restrict__ pts) {
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  pts[i].w = static_cast<T>(w[perm[i]]);
+}
+
+constexpr int kP2mChunk = 64;
+
+template <class T, int P>
+__global__ void __launch_bounds__(kThreads)
+p2m_kernel(const int32_t* __restrict__ leaf_start, const V4<T>* __restrict__ pts, T* __restrict__ m,
+           int kpad) {
+  constexpr int P3 = P * P * P;
+  constexpr int NO = (P3 + kThreads - 1) / kThreads;
+  __shared__ T tk[P * P];
+  __shared__ T sv[3][kP2mChunk][P];
+  const int leaf = blockIdx.x;
+  const int ps = leaf_start[leaf], pe = leaf_start[leaf + 1];
+  init_tk<T, P>(tk);
+  __syncthreads();
+  T acc[NO];
+#pragma unroll
+  for (int q = 0; q < NO; q++) acc[q] = T(0);
+  for (int c0 = ps; c0 < pe; c0 += kP2mChunk) {
+    int cnt = min(kP2mChunk, pe - c0);
+    if (threadIdx.x < 3 * kP2mChunk) {
+      int j = threadIdx.x % kP2mChunk, d = threadIdx.x / kP2mChunk;
+      T S[P];
+      if (j < cnt) {
+        V4<T> q = pts[c0 + j];
+        T xi = d == 0 ? q.x : (d == 1 ? q.y : q.z);
+        lagrange<T, P>(xi, tk, S);
+        if (d == 0) {
+#pragma unroll
+          for (int a = 0; a < P; a++) S[a] *= q.w;
+        }
+      } else {
+#pragma unroll
+        for (int a = 0; a < P; a++) S[a] = T(0);
+      }
+#pragma unroll
+      for (int a = 0; a < P; a++) sv[d][j][a] = S[a];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NO; q++) {
+      int o = threadIdx.x + q * kThreads;
+      if (o < P3) {
+        int a = o / (P * P), bb = (o / P) % P, g = o % P;
+        T sum = T(0);
+        for (int j = 0; j < cnt; j++) sum += sv[0][j][a] * sv[1][j][bb] * sv[2][j][g];
+        acc[q] += sum;
+      }
+    }
+    __syncthreads();
+  }
+  T* dst = m + static_cast<int64_t>(leaf) * kpad;
+#pragma unroll
+  for (int q = 0; q < NO; q++) {
+    int o = threadIdx.x + q * kThreads;
+    if (o < P3) dst[o] = acc[q];
+  }
+  for (int o = P3 + threadIdx.x; o < kpad; o += kThreads) dst[o] = T(0);
+}
+
+template <class T, int P>
+__global__ void __launch_bounds__(kThreads)
+m2m_kernel(const int32_t* __restrict__ child_start, const uint64_t* __restrict__ child_codes,
+           const T* __restrict__ mc, T* __restrict__ mp, int kpad) {
+  constexpr int P2 = P * P, P3 = P * P * P;
+  constexpr int NO = (P3 + kThreads - 1) / kThreads;
+  __shared__ T tr[2 * P2];
+  __shared__ T b0[P3], b1[P3];
+  const int parent = blockIdx.x;
+  init_transfer<T, P>(tr);
+  T acc[NO];
+#pragma unroll
+  for (int q = 0; q < NO; q++) acc[q] = T(0);
+  const int cs = child_start[parent], ce = child_start[parent + 1];
+  for (int c = cs; c < ce; c++) {
+    int oct = static_cast<int>(child_codes[c] & 7u);
+    const T* tx = tr + ((oct >> 2) & 1) * P2;
+    const T* ty = tr + ((oct >> 1) & 1) * P2;
+    const T* tz = tr + (oct & 1) * P2;
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < P3; idx += kThreads) b0[idx] = mc[static_cast<int64_t>(c) * kpad + idx];
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < P3; idx += kThreads) {  // z
+      int ab = idx / P, g = idx % P;
+      T s = T(0);
+#pragma unroll
+      for (int g2 = 0; g2 < P; g2++) s += tz[g * P + g2] * b0[ab * P + g2];
+      b1[idx] = s;
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < P3; idx += kThreads) {  // y
+      int a2 = idx / P2, bb = (idx / P) % P, g = idx % P;
+      T s = T(0);
+#pragma unroll
+      for (int b2 = 0; b2 < P; b2++) s += ty[bb * P + b2] * b1[(a2 * P + b2) * P + g];
+      b0[idx] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NO; q++) {  // x
+      int o = threadIdx.x + q * kThreads;
+      if (o < P3) {
+        int a = o / P2, bg = o % P2;
+        T s = T(0);
+#pragma unroll
+        for (int a2 = 0; a2 < P; a2++) s += tx[a * P + a2] * b0[a2 * P2 + bg];
+        acc[q] += s;
+      }
+    }
+  }
+  T* dst = mp + static_cast<int64_t>(parent) * kpad;
+#pragma unroll
+  for (int q = 0; q < NO; q++) {
+    int o = threadIdx.x + q * kThreads;
+    if (o < P3) dst[o] = acc[q];
+  }
+  for (int o = P3 + threadIdx.x; o < kpad; o += kThreads) dst[o] = T(0);
+}
+
+template <class T, int P>
+__global__ void __launch_bounds__(kThreads)
+l2l_kernel(const int32_t* __restrict__ parent_of, const uint64_t* __restrict__ codes,
+           const T* __restrict__ lp, T* __restrict__ lc, int kpad) {
+  constexpr int P2 = P * P, P3 = P * P * P;
+  __shared__ T tr[2 * P2];
+  __shared__ T b0[P3], b1[P3];
+  const int c = blockIdx.x;
+  const int parent = parent_of[c];
+  const int oct = static_cast<int>(codes[c] & 7u);
+  init_transfer<T, P>(tr);
+  for (int idx = threadIdx.x; idx < P3; idx += kThreads) b0[idx] = lp[static_cast<int64_t>(parent) * kpad + idx];
+  __syncthreads();
+  const T* tx = tr + ((oct >> 2) & 1) * P2;
+  const T* ty = tr + ((oct >> 1) & 1) * P2;
+  const T* tz = tr + (oct & 1) * P2;
+  for (int idx = threadIdx.x; idx < P3; idx += kThreads) {  // x: b1[a2][b][g] = sum_a tx[a][a2] b0[a][b][g]
+    int a2 = idx / P2, bg = idx % P2;
+    T s = T(0);
+#pragma unroll
+    for (int a = 0; a < P; a++) s += tx[a * P + a2] * b0[a * P2 + bg];
+    b1[idx] = s;
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < P3; idx += kThreads) {  // y
+    int a2 = idx / P2, b2 = (idx / P) % P, g = idx % P;
+    T s = T(0);
+#pragma unroll
+    for (int bb = 0; bb < P; bb++) s += ty[bb * P + b2] * b1[(a2 * P + bb) * P + g];
+    b0[idx] = s;
+  }
+  __syncthreads();
+  T* dst = lc + static_cast<int64_t>(c) * kpad;
+  for (int idx = threadIdx.x; idx < P3; idx += kThreads) {  // z
+    int ab = idx / P, g2 = idx % P;
+    T s = T(0);
+#pragma unroll
+    for (int g = 0; g < P; g++) s += tz[g * P + g2] * b0[ab * P + g];
+    dst[idx] += s;
+  }
+}
+
+template <class T, int P>
+__global__ void __launch_bounds__(kThreads)
+l2p_kernel(const int32_t* __restrict__ leaf_start, const V4<T>* __restrict__ pts,
+           const T* __restrict__ l, int kpad, T* __restrict__ y_far) {
+  constexpr int P2 = P * P, P3 = P * P * P;
+  __shared__ T tk[P2];
+  __shared__ T ls[P3];
+  const int leaf = blockIdx.x;
+  init_tk<T, P>(tk);
+  for (int idx = threadIdx.x; idx < P3; idx += kThreads) ls[idx] = l[static_cast<int64_t>(leaf) * kpad + idx];
+  __syncthreads();
+  const int ps = leaf_start[leaf], pe = leaf_start[leaf + 1];
+  for (int i = ps + threadIdx.x; i < pe; i += kThreads) {
+    V4<T> q = pts[i];
+    T sx[P], sy[P], sz[P];
+    lagrange<T, P>(q.x, tk, sx);
+    lagrange<T, P>(q.y, tk, sy);
+    lagrange<T, P>(q.z, tk, sz);
+    T y = T(0);
+    for (int a = 0; a < P; a++) {
+      T ya = T(0);
+#pragma unroll
+      for (int bb = 0; bb < P; bb++) {
+        T yb = T(0);
+#pragma unroll
+        for (int g = 0; g < P; g++) yb += ls[(a * P + bb) * P + g] * sz[g];
+        ya += yb * sy[bb];
+      }
+      y += ya * sx[a];
+    }
+    y_far[i] = y;
+  }
+}
+
+template <class T, int KIND>
+__global__ void __launch_bounds__(kThreads)
+p2p_kernel(const int32_t* __restrict__ tls, const int32_t* __restrict__ sls,
+           const int32_t* __restrict__ p2p, const V4<T>* __restrict__ tpts,
+           const V4<T>* __restrict__ spts, const T* __restrict__ y_far,
+           const int32_t* __restrict__ tperm, T pa, T pb, double* __restrict__ out) {
+  constexpr int CAP = 32768 / static_cast<int>(sizeof(V4<T>));
+  __shared__ V4<T> sp[CAP];
+  __shared__ int nb_start[28];
+  __shared__ int nb_src[27];
+  __shared__ T nb_off[27][3];
+  __shared__ int nb_count;
+  __shared__ T red[kThreads];
+  const int leaf = blockIdx.x;
+  const int ts = tls[leaf], te = tls[leaf + 1];
+  const int nt = te - ts;
+  if (threadIdx.x == 0) {
+    int k = 0, total = 0;
+    for (int s = 0; s < 27; s++) {
+      int sl = p2p[static_cast<int64_t>(leaf) * 27 + s];
+      if (sl < 0) continue;
+      int a = sls[sl], b = sls[sl + 1];
+      nb_src[k] = a;
+      nb_start[k] = total;
+      nb_off[k][0] = T(2 * (s / 9 - 1));
+      nb_off[k][1] = T(2 * ((s / 3) % 3 - 1));
+      nb_off[k][2] = T(2 * (s % 3 - 1));
+      total += b - a;
+      k++;
+    }
+    nb_start[k] = total;
+    nb_count = k;
+  }
+  __syncthreads();
+  const int nnb = nb_count;
+  const int total = nb_start[nnb];
+  int tpad = 1;
+  while (tpad < nt && tpad < kThreads) tpad <<= 1;
+  const int nsplit = kThreads / tpad;
+  const int ti = threadIdx.x & (tpad - 1);
+  const int si = threadIdx.x / tpad;
+  for (int c0 = 0; c0 < nt; c0 += tpad) {
+    const bool valid = c0 + ti < nt;
+    V4<T> tq{T(0), T(0), T(0), T(0)};
+    if (valid) tq = tpts[ts + c0 + ti];
+    T acc = T(0);
+    for (int cb = 0; cb < total; cb += CAP) {
+      const int cnt = min(CAP, total - cb);
+      __syncthreads();
+      for (int e = threadIdx.x; e < cnt; e += kThreads) {
+        int g = cb + e;
+        int k = 0;
+        while (k + 1 < nnb && nb_start[k + 1] <= g) k++;
+        V4<T> q = spts[nb_src[k] + (g - nb_start[k])];
+        q.x += nb_off[k][0];
+        q.y += nb_off[k][1];
+        q.z += nb_off[k][2];
+        sp[e] = q;
+      }
+      __syncthreads();
+      for (int j = si; j < cnt; j += nsplit) {
+        V4<T> q = sp[j];
+        T dx = tq.x - q.x, dy = tq.y - q.y, dz = tq.z - q.z;
+        acc += q.w * phi_t<T, KIND>(dx * dx + dy * dy + dz * dz, pa, pb);
+      }
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    if (si == 0 && valid) {
+      T s = T(0);
+      for (int k = 0; k < nsplit; k++) s += red[ti + k * tpad];
+      int i = ts + c0 + ti;
+      double far = y_far ? static_cast<double>(y_far[i]) : 0.0;
+      out[tperm[i]] = far + static_cast<double>(s);
+    }
+    __syncthreads();
+  }
+}
+
+// ---- M2L on MFMA ----
+template <class T>
+struct Mfma;
+template <>
+struct Mfma<float> {
+  using acc_t = float __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma<double> {
+  using acc_t = double __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+
+constexpr int kM2LKc = 256;  // reduction chunk staged in LDS
+
+template <class T>
+__global__ void __launch_bounds__(kThreads)
+m2l_kernel(int r, int rpad, int kpad, const T* __restrict__ a, int64_t per_v,
+           const int32_t* __restrict__ col_cell, const int32_t* __restrict__ blk_oct,
+           const int32_t* __restrict__ src_slots, const int32_t* __restrict__ vlist,
+           const T* __restrict__ in, int in_stride, T* __restrict__ out, int out_stride, T scale,
+           int accumulate) {
+  using acc_t = typename Mfma<T>::acc_t;
+  constexpr int LDB = kM2LKc + 4;
+  __shared__ V4<T> bs4[kM2LCols * LDB / 4];
+  __shared__ int cells[kM2LCols];
+  __shared__ int srcs[kM2LCols];
+  T* bs = reinterpret_cast<T*>(bs4);
+  const int blk = blockIdx.x;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int row0 = blockIdx.y * 64 + wave * 16;
+  const int oct = blk_oct ? blk_oct[blk] : 0;
+  const int nv = src_slots ? kM2LSlots : 1;
+  if (threadIdx.x < kM2LCols) cells[threadIdx.x] = col_cell[blk * kM2LCols + threadIdx.x];
+  acc_t acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+  const int col = threadIdx.x / 8, part = threadIdx.x % 8;  // staging map: 8 lanes per column
+  for (int sidx = 0; sidx < nv; sidx++) {
+    const int v = src_slots ? vlist[oct * kM2LSlots + sidx] : 0;
+    __syncthreads();
+    if (threadIdx.x < kM2LCols) {
+      int c = cells[threadIdx.x];
+      srcs[threadIdx.x] = c < 0 ? -1 : (src_slots ? src_slots[static_cast<int64_t>(c) * kM2LSlots + sidx] : c);
+    }
+    __syncthreads();
+    // Skip transfer vectors with no source in this column block.
+    bool any = false;
+    for (int k = 0; k < kM2LCols; k++) any |= srcs[k] >= 0;
+    if (!any) continue;
+    const T* av = a + static_cast<int64_t>(v) * per_v;
+    const int src = srcs[col];
+    for (int k0 = 0; k0 < kpad; k0 += kM2LKc) {
+      const int kc = min(kM2LKc, kpad - k0);
+      __syncthreads();
+      for (int k4 = part; k4 < kc / 4; k4 += 8) {
+        V4<T> val{T(0), T(0), T(0), T(0)};
+        if (src >= 0) val = *reinterpret_cast<const V4<T>*>(in + static_cast<int64_t>(src) * in_stride + k0 + 4 * k4);
+        *reinterpret_cast<V4<T>*>(bs + col * LDB + 4 * k4) = val;
+      }
+      __syncthreads();
+      if (row0 < rpad) {
+        for (int kb = 0; kb < kc / 16; kb++) {
+          const V4<T> a4 = *reinterpret_cast<const V4<T>*>(
+              av + ((static_cast<int64_t>((k0 >> 4) + kb) * rpad + row0 + (lane & 15)) << 4) + ((lane >> 4) << 2));
+          const V4<T> b0 = *reinterpret_cast<const V4<T>*>(bs + (lane & 15) * LDB + kb * 16 + ((lane >> 4) << 2));
+          const V4<T> b1 = *reinterpret_cast<const V4<T>*>(bs + (16 + (lane & 15)) * LDB + kb * 16 + ((lane >> 4) << 2));
+          acc0 = Mfma<T>::op(a4.x, b0.x, acc0);
+          acc1 = Mfma<T>::op(a4.x, b1.x, acc1);
+          acc0 = Mfma<T>::op(a4.y, b0.y, acc0);
+          acc1 = Mfma<T>::op(a4.y, b1.y, acc1);
+          acc0 = Mfma<T>::op(a4.z, b0.z, acc0);
+          acc1 = Mfma<T>::op(a4.z, b1.z, acc1);
+          acc0 = Mfma<T>::op(a4.w, b0.w, acc0);
+          acc1 = Mfma<T>::op(a4.w, b1.w, acc1);
+        }
+      }
+    }
+  }
+  // D layout (16x16): lane -> column lane%16, rows 4*(lane/16) + i.
+#pragma unroll
+  for (int ct = 0; ct < 2; ct++) {
+    const int c = cells[ct * 16 + (lane & 15)];
+    if (c < 0) continue;
+    const acc_t& acc = ct == 0 ? acc0 : acc1;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      int row = row0 + 4 * (lane >> 4) + i;
+      if (row < r) {
+        T val = acc[i] * scale;
+        T* dst = out + static_cast<int64_t>(c) * out_stride + row;
+        *dst = accumulate ? *dst + val : val;
+      }
+    }
+  }
+}
+
+template <class T>
+__global__ void build_m2l_dense_kernel(int p, int kind, double pa, double pb, int rpad, int kpad,
+                                       T* __restrict__ a) {
+  int64_t per_v = static_cast<int64_t>(rpad) * kpad;
+  int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= 343 * per_v) return;
+  int v = static_cast<int>(idx / per_v);
+  int64_t rem = idx % per_v;
+  int kb = static_cast<int>(rem / (static_cast<int64_t>(rpad) * 16));
+  int row = static_cast<int>((rem / 16) % rpad);
+  int k = kb * 16 + static_cast<int>(rem % 16);
+  int p3 = p * p * p;
+  int dx = v / 49 - 3, dy = (v / 7) % 7 - 3, dz = v % 7 - 3;
+  bool adj = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1;
+  T val = T(0);
+  if (!adj && row < p3 && k < p3) {
+    int ta = row / (p * p), tb = (row / p) % p, tc = row % p;
+    int sa = k / (p * p), sb = (k / p) % p, sc = k % p;
+    // source cell centre = target centre + 2 d (local units of the cell half width)
+    double ex = cospi((2.0 * ta + 1) / (2.0 * p)) - 2.0 * dx - cospi((2.0 * sa + 1) / (2.0 * p));
+    double ey = cospi((2.0 * tb + 1) / (2.0 * p)) - 2.0 * dy - cospi((2.0 * sb + 1) / (2.0 * p));
+    double ez = cospi((2.0 * tc + 1) / (2.0 * p)) - 2.0 * dz - cospi((2.0 * sc + 1) / (2.0 * p));
+    double r2 = ex * ex + ey * ey + ez * ez;
+    double f;
+    if (kind == kBh3) f = pa * sqrt(r2 + pb);
+    else if (kind == kTh3) { double rho = sqrt(r2 + pb); f = pa * rho * rho * rho; }
+    else f = pa * exp(pb * sqrt(r2));
+    val = static_cast<T>(f);
+  }
+  a[idx] = val;
+}
+
+template <class F>
+void dispatch_order(int p, F&& f) {
+  switch (p) {
+    case 3: f(std::integral_constant<int, 3>{}); return;
+    case 4: f(std::integral_constant<int, 4>{}); return;
+    case 5: f(std::integral_constant<int, 5>{}); return;
+    case 6: f(std::integral_constant<int, 6>{}); return;
+    case 7: f(std::integral_constant<int, 7>{}); return;
+    case 8: f(std::integral_constant<int, 8>{}); return;
+    case 9: f(std::integral_constant<int, 9>{}); return;
+    case 10: f(std::integral_constant<int, 10>{}); return;
+    case 11: f(std::integral_constant<int, 11>{}); return;
+    case 12: f(std::integral_constant<int, 12>{}); return;
+  }
+  throw_invalid("unsupported interpolation order");
+}
+
+}  // namespace
+
+template <class T>
+void launch_prepare_points(const Tree& t, const double* d_pts, const double* d_w, V4<T>* out,
+                           hipStream_t s) {
+  int64_t n = t.n_points;
+  if (n == 0) return;
+  BoxArgs b{{t.box.lo[0], t.box.lo[1], t.box.lo[2]}, t.box.width / std::ldexp(1.0, t.level)};
+  prepare_points_kernel<T><<<ceil_div(n, 256), 256, 0, s>>>(d_pts, d_w, t.perm.get(), t.point_leaf.get(),
+                                                            t.levels[t.level].codes.get(), n, b, out);
+  PAM_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_set_point_weights(const Tree& t, const double* d_w, V4<T>* pts, hipStream_t s) {
+  int64_t n = t.n_points;
+  if (n == 0) return;
+  set_weights_kernel<T><<<ceil_div(n, 256), 256, 0, s>>>(d_w, t.perm.get(), n, pts);
+  PAM_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_p2m(int p, const Tree& t, const V4<T>* pts, T* m_leaf, hipStream_t s) {
+  int64_t nl = t.n_leaves();
+  if (nl == 0) return;
+  dispatch_order(p, [&](auto P) {
+    p2m_kernel<T, decltype(P)::value><<<static_cast<unsigned>(nl), kThreads, 0, s>>>(
+        t.leaf_start.get(), pts, m_leaf, kpad_of(P));
+  });
+  PAM_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_m2m(int p, const Tree& t, int l, const T* m_child, T* m_parent, hipStream_t s) {
+  int64_t n = t.levels[l].n;
+  if (n == 0) return;
+  dispatch_order(p, [&](auto P) {
+    m2m_kernel<T, decltype(P)::value><<<static_cast<unsigned>(n), kThreads, 0, s>>>(
+        t.levels[l].child_start.get(), t.levels[l + 1].codes.get(), m_child, m_parent, kpad_of(P));
+  });
+  PAM_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_l2l(int p, const Tree& t, int l, const T* l_parent, T* l_child, hipStream_t s) {
+  int64_t n = t.levels[l].n;
+  if (n == 0) return;
+  dispatch_order(p, [&](auto P) {
+    l2l_kernel<T, decltype(P)::value><<<static_cast<unsigned>(n), kThreads, 0, s>>>(
+        t.levels[l].parent.get(), t.levels[l].codes.get(), l_parent, l_child, kpad_of(P));
+  });
+  PAM_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_l2p(int p, const Tree& t, const T* l_leaf, const V4<T>* pts, T* y_far, hipStream_t s) {
+  int64_t nl = t.n_leaves();
+  if (nl == 0) return;
+  dispatch_order(p, [&](auto P) {
+    l2p_kernel<T, decltype(P)::value><<<static_cast<unsigned>(nl), kThreads, 0, s>>>(
+        t.leaf_start.get(), pts, l_leaf, kpad_of(P), y_far);
+  });
+  PAM_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_p2p(const RbfDesc& rbf, const Tree& trg, const Tree& src, const Lists& lists,
+                const V4<T>* tpts, const V4<T>* spts, const T* y_far, double* out, hipStream_t s) {
+  int64_t nl = trg.n_leaves();
+  if (nl == 0) return;
+  double h = trg.box.width / std::ldexp(1.0, trg.level) / 2.0;
+  PhiConsts pc = phi_consts(rbf, h);
+  dispatch_kind(rbf.kind, [&](auto K) {
+    p2p_kernel<T, decltype(K)::value><<<static_cast<unsigned>(nl), kThreads, 0, s>>>(
+        trg.leaf_start.get(), src.leaf_start.get(), lists.p2p.get(), tpts, spts, y_far, trg.perm.get(),
+        static_cast<T>(pc.a), static_cast<T>(pc.b), out);
+  });
+  PAM_LAUNCH_CHECK();
+}
+
+template <class T>
+void build_m2l_dense(int p, const RbfDesc& rbf, double h, M2LOperator& op, DevBuf<T>& a,
+                     hipStream_t s) {
+  op.r = p * p * p;
+  op.rpad = rpad_of(op.r);
+  op.kpad = kpad_of(p);
+  op.per_v = static_cast<size_t>(op.rpad) * op.kpad;
+  a.resize(343 * op.per_v);
+  PhiConsts pc = phi_consts(rbf, h);
+  int64_t total = static_cast<int64_t>(343) * op.per_v;
+  build_m2l_dense_kernel<T><<<ceil_div(total, 256), 256, 0, s>>>(p, rbf.kind, pc.a, pc.b, op.rpad,
+                                                                 op.kpad, a.get());
+  PAM_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_m2l(const M2LOperator& op, const T* a, const M2LLevel& lv, const int32_t* src_slots,
+                const int32_t* vlist, const T* in, int in_stride, T* out, int out_stride, T scale,
+                bool accumulate, hipStream_t s) {
+  if (lv.n_cols == 0) return;
+  if (op.kpad % 16 != 0 || in_stride % 4 != 0) throw_runtime("m2l: bad padding");
+  dim3 grid(static_cast<unsigned>(lv.n_cols / kM2LCols), static_cast<unsigned>(op.rpad / 64));
+  m2l_kernel<T><<<grid, kThreads, 0, s>>>(op.r, op.rpad, op.kpad, a, static_cast<int64_t>(op.per_v),
+                                          lv.col_cell.get(), lv.blk_oct.get(), src_slots, vlist, in,
+                                          in_stride, out, out_stride, scale, accumulate ? 1 : 0);
+  PAM_LAUNCH_CHECK();
+}
+
+#define PAM_INSTANTIATE(T)                                                                          \
+  template void launch_prepare_points<T>(const Tree&, const double*, const double*, V4<T>*,        \
+                                         hipStream_t);                                               \
+  template void launch_set_point_weights<T>(const Tree&, const double*, V4<T>*, hipStream_t);       \
+  template void launch_p2m<T>(int, const Tree&, const V4<T>*, T*, hipStream_t);                     \
+  template void launch_m2m<T>(int, const Tree&, int, const T*, T*, hipStream_t);                    \
+  template void launch_l2l<T>(int, const Tree&, int, const T*, T*, hipStream_t);                    \
+  template void launch_l2p<T>(int, const Tree&, const T*, const V4<T>*, T*, hipStream_t);           \
+  template void launch_p2p<T>(const RbfDesc&, const Tree&, const Tree&, const Lists&, const V4<T>*, \
+                              const V4<T>*, const T*, double*, hipStream_t);                       \
+  template void build_m2l_dense<T>(int, const RbfDesc&, double, M2LOperator&, DevBuf<T>&,          \
+                                   hipStream_t);                                                    \
+  template void launch_m2l<T>(const M2LOperator&, const T*, const M2LLevel&, const int32_t*,        \
+                              const int32_t*, const T*, int, T*, int, T, bool, hipStream_t);
+
+PAM_INSTANTIATE(float)
+PAM_INSTANTIATE(double)
+
+}  // namespace pam

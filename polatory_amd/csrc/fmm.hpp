@@ -1,0 +1,73 @@
+// Interpolation FMM stage launchers (Chebyshev tensor-product interpolation of
+// order P per axis, P^3 nodes per cell). T = float (mixed form) or double.
+//
+// Storage:
+//   points in tree order as V4<T> = (xi, eta, zeta, w): coordinates local to the
+//   point's leaf, scaled so the leaf is [-1,1]^3;
+//   expansions [n_cells][kpad(P)] (zero padded), node index (a*P + b)*P + c,
+//   a along x; nodes x_a = cos((2a+1) pi / (2P)).
+#pragma once
+
+#include "common.hpp"
+#include "rbf_device.hpp"
+#include "tree.hpp"
+
+namespace pam {
+
+template <class T>
+struct alignas(4 * sizeof(T)) V4 {
+  T x, y, z, w;
+};
+
+constexpr int kMinOrder = 3;
+constexpr int kMaxOrder = 12;
+inline bool order_supported(int p) { return p >= kMinOrder && p <= kMaxOrder; }
+
+// Expansion stride: P^3 padded to 16 (the M2L reduction is blocked by 16).
+inline int kpad_of(int p) { return (p * p * p + 15) / 16 * 16; }
+// Row padding of the M2L operators: 64 rows per workgroup.
+inline int rpad_of(int r) { return (r + 63) / 64 * 64; }
+
+// M2L operator for one level in MFMA fragment order: A[v][kb][row][16] with
+// v in [0, 343), kb in [0, kpad/16), row in [0, rpad); element (row, k) of
+// transfer vector v sits at ((v * (kpad/16) + k/16) * rpad + row) * 16 + k % 16.
+struct M2LOperator {
+  int r = 0, rpad = 0, kpad = 0;
+  size_t per_v = 0;  // elements per transfer vector
+};
+
+template <class T>
+void launch_prepare_points(const Tree& t, const double* d_pts, const double* d_w, V4<T>* out,
+                           hipStream_t s);
+template <class T>
+void launch_set_point_weights(const Tree& t, const double* d_w, V4<T>* pts, hipStream_t s);
+
+template <class T>
+void launch_p2m(int p, const Tree& t, const V4<T>* pts, T* m_leaf, hipStream_t s);
+// parent level l from child level l+1
+template <class T>
+void launch_m2m(int p, const Tree& t, int l, const T* m_child, T* m_parent, hipStream_t s);
+// child level l from parent level l-1 (accumulates into l_child)
+template <class T>
+void launch_l2l(int p, const Tree& t, int l, const T* l_parent, T* l_child, hipStream_t s);
+template <class T>
+void launch_l2p(int p, const Tree& t, const T* l_leaf, const V4<T>* pts, T* y_far, hipStream_t s);
+// out[perm[i]] = y_far[i] + near field, FP64 output in input order.
+template <class T>
+void launch_p2p(const RbfDesc& rbf, const Tree& trg, const Tree& src, const Lists& lists,
+                const V4<T>* tpts, const V4<T>* spts, const T* y_far, double* out, hipStream_t s);
+
+// Builds the dense M2L operator of one level: entries phi(h * |x_a + 2v - x_b|)
+// for cells of half width h (kernel constants folded by phi_consts).
+template <class T>
+void build_m2l_dense(int p, const RbfDesc& rbf, double h, M2LOperator& op, DevBuf<T>& a,
+                     hipStream_t s);
+
+// out[col_cell][row] (+)= scale * sum_slots A_v * in[src] over the octant's 189
+// transfer vectors (src_slots != null), or out[c] = A_0 * in[c] (src_slots == null).
+template <class T>
+void launch_m2l(const M2LOperator& op, const T* a, const M2LLevel& lv, const int32_t* src_slots,
+                const int32_t* vlist, const T* in, int in_stride, T* out, int out_stride, T scale,
+                bool accumulate, hipStream_t s);
+
+}  // namespace pam

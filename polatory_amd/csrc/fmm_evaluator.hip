@@ -1,0 +1,507 @@
+// Device FMM evaluator orchestration: box, trees, lists, operator cache, the
+// upward/downward passes and the accuracy-driven configuration search.
+#include "fmm_evaluator.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <mutex>
+#include <tuple>
+
+#include "device_api.hpp"
+
+namespace pam {
+
+namespace {
+
+// ---- M2L operator cache shared by all evaluators of the process ----
+template <class T>
+struct OpEntry {
+  M2LOperator op;
+  DevBuf<T> a;
+};
+
+using OpKey = std::tuple<int, int, int, int, double, double, double>;  // dev, sizeof, P, kind, p0, p1, h
+
+template <class T>
+struct OpCache {
+  std::mutex mu;
+  std::map<OpKey, std::pair<std::shared_ptr<OpEntry<T>>, uint64_t>> map;
+  uint64_t tick = 0;
+};
+
+template <class T>
+OpCache<T>& op_cache() {
+  static OpCache<T> c;
+  return c;
+}
+
+template <class T>
+std::shared_ptr<OpEntry<T>> get_op(int p, const RbfDesc& rbf, double h, hipStream_t s) {
+  int dev = 0;
+  PAM_HIP(hipGetDevice(&dev));
+  OpKey key{dev, static_cast<int>(sizeof(T)), p, rbf.kind, rbf.p0, rbf.p1, h};
+  auto& c = op_cache<T>();
+  std::lock_guard<std::mutex> lk(c.mu);
+  auto it = c.map.find(key);
+  if (it != c.map.end()) {
+    it->second.second = ++c.tick;
+    return it->second.first;
+  }
+  constexpr size_t kMaxEntries = 24;
+  while (c.map.size() >= kMaxEntries) {
+    auto victim = c.map.begin();
+    for (auto j = c.map.begin(); j != c.map.end(); ++j)
+      if (j->second.second < victim->second.second) victim = j;
+    c.map.erase(victim);
+  }
+  auto e = std::make_shared<OpEntry<T>>();
+  build_m2l_dense<T>(p, rbf, h, e->op, e->a, s);
+  PAM_HIP(hipStreamSynchronize(s));
+  c.map[key] = {e, ++c.tick};
+  return e;
+}
+
+// Degree of homogeneity phi(l r) = l^deg phi(r), or 0 when phi is not homogeneous.
+int homogeneity(const RbfDesc& r) {
+  if ((r.kind == kBh3 || r.kind == kTh3) && r.p1 == 0.0) return r.kind == kBh3 ? 1 : 3;
+  return 0;
+}
+
+void make_box_from_bbox(const double* bmin, const double* bmax, const double* aniso, Box& box) {
+  // src/fmm/utility.hpp:19-33: corners through A, 1.01 x the largest extent.
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int c = 0; c < 8; c++) {
+    double p[3] = {(c & 1) ? bmax[0] : bmin[0], (c & 2) ? bmax[1] : bmin[1], (c & 4) ? bmax[2] : bmin[2]};
+    for (int k = 0; k < 3; k++) {
+      double t = (aniso[3 * k] * p[0] + aniso[3 * k + 1] * p[1]) + aniso[3 * k + 2] * p[2];
+      lo[k] = std::min(lo[k], t);
+      hi[k] = std::max(hi[k], t);
+    }
+  }
+  double wmax = 0.0, center[3];
+  for (int k = 0; k < 3; k++) {
+    double wk = hi[k] - lo[k];
+    wmax = std::max(wmax, wk);
+    center[k] = lo[k] + wk / 2.0;
+  }
+  double w = 1.01 * wmax;
+  if (w == 0.0) w = 1.0;
+  box.width = w;
+  for (int k = 0; k < 3; k++) box.lo[k] = center[k] - w / 2.0;
+}
+
+}  // namespace
+
+int auto_leaf_level(int64_t n_src, int64_t n_trg) {
+  double n = static_cast<double>(std::max<int64_t>(std::max(n_src, n_trg), 1));
+  int l = static_cast<int>(std::lround(std::log(n / 24.0) / std::log(8.0)));
+  return std::clamp(l, 2, 18);
+}
+
+FmmEvaluator::FmmEvaluator(const RbfDesc& rbf, const double* bbox_min, const double* bbox_max,
+                           bool symmetric)
+    : rbf_(rbf), symmetric_(symmetric), accuracy_(std::numeric_limits<double>::infinity()) {
+  for (int k = 0; k < 3; k++) {
+    user_min_[k] = bbox_min ? bbox_min[k] : INFINITY;
+    user_max_[k] = bbox_max ? bbox_max[k] : -INFINITY;
+  }
+  PAM_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  for (auto& e : ev_) PAM_HIP(hipEventCreate(&e));
+  bbox_out_.resize(12);
+}
+
+FmmEvaluator::~FmmEvaluator() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (auto& e : ev_)
+    if (e) (void)hipEventDestroy(e);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void FmmEvaluator::set_accuracy(double accuracy) {
+  if (!(accuracy >= 0.0)) throw_invalid("accuracy must be nonnegative");
+  accuracy_ = accuracy;
+  if (!fixed_config_) config_valid_ = false;
+}
+
+void FmmEvaluator::set_config(const FmmConfig& c) {
+  if (c.precision != kFp64 && c.precision != kMixed) throw_invalid("unknown precision");
+  if (!order_supported(c.order)) throw_invalid("unsupported interpolation order");
+  if (c.leaf_level != -1 && (c.leaf_level < 2 || c.leaf_level > 18)) throw_invalid("leaf level out of range");
+  FmmConfig old = cfg_;
+  cfg_ = c;
+  fixed_config_ = true;
+  config_valid_ = true;
+  if (old.leaf_level != c.leaf_level) src_tree_dirty_ = trg_tree_dirty_ = true;
+  if (old.order != c.order || old.leaf_level != c.leaf_level) invalidate_expansions();
+}
+
+void FmmEvaluator::invalidate_expansions() {
+  s32_.multipoles_valid = s64_.multipoles_valid = false;
+  s32_.spts_valid = s64_.spts_valid = false;
+  s32_.tpts_valid = s64_.tpts_valid = false;
+}
+
+void FmmEvaluator::set_source_points(const double* xyz, int64_t n, bool on_device) {
+  if (n < 0) throw_invalid("negative point count");
+  n_src_ = n;
+  src_pts_.resize(static_cast<size_t>(n) * 3);
+  const double* d_in = xyz;
+  if (!on_device) {
+    staging_.upload(xyz, static_cast<size_t>(n) * 3, stream_);
+    d_in = staging_.get();
+  }
+  launch_transform_points(rbf_.aniso, d_in, n, src_pts_.get(), stream_);
+  if (n > 0) launch_bbox(src_pts_.get(), n, bbox_out_.get(), bbox_scratch_, stream_);
+  src_tree_dirty_ = true;
+  if (symmetric_) trg_tree_dirty_ = true;
+  box_valid_ = false;
+  have_weights_ = false;
+  invalidate_expansions();
+  if (!fixed_config_) config_valid_ = false;
+  PAM_HIP(hipStreamSynchronize(stream_));
+}
+
+void FmmEvaluator::set_target_points(const double* xyz, int64_t n, bool on_device) {
+  if (symmetric_) throw_runtime("set_target_points on a symmetric evaluator");
+  if (n < 0) throw_invalid("negative point count");
+  n_trg_ = n;
+  trg_pts_.resize(static_cast<size_t>(n) * 3);
+  const double* d_in = xyz;
+  if (!on_device) {
+    staging_.upload(xyz, static_cast<size_t>(n) * 3, stream_);
+    d_in = staging_.get();
+  }
+  launch_transform_points(rbf_.aniso, d_in, n, trg_pts_.get(), stream_);
+  if (n > 0) launch_bbox(trg_pts_.get(), n, bbox_out_.get() + 6, bbox_scratch_, stream_);
+  trg_tree_dirty_ = true;
+  box_valid_ = false;
+  s32_.tpts_valid = s64_.tpts_valid = false;
+  PAM_HIP(hipStreamSynchronize(stream_));
+}
+
+void FmmEvaluator::set_weights(const double* w, int64_t n, bool on_device) {
+  if (n != n_src_) throw_invalid("weights.rows() must equal the number of source points");
+  w_.resize(static_cast<size_t>(n));
+  if (n > 0) {
+    PAM_HIP(hipMemcpyAsync(w_.get(), w, sizeof(double) * n,
+                           on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream_));
+  }
+  have_weights_ = true;
+  // Point coordinates stay; only the weights and the multipoles change.
+  if (s32_.spts_valid) launch_set_point_weights<float>(src_tree_, w_.get(), s32_.spts.get(), stream_);
+  if (s64_.spts_valid) launch_set_point_weights<double>(src_tree_, w_.get(), s64_.spts.get(), stream_);
+  s32_.multipoles_valid = s64_.multipoles_valid = false;
+  if (!on_device) PAM_HIP(hipStreamSynchronize(stream_));
+}
+
+bool FmmEvaluator::use_direct() const {
+  if (cfg_.mode == kModeDirect) return true;
+  if (cfg_.mode == kModeFmm) return false;
+  int64_t nt = symmetric_ ? n_src_ : n_trg_;
+  // Below ~2^26 pairs the FP64 direct sum is both exact and cheaper on one GPU.
+  return static_cast<double>(n_src_) * static_cast<double>(nt) < 67108864.0;
+}
+
+void FmmEvaluator::ensure_box() {
+  if (box_valid_) return;
+  double pb[12];
+  PAM_HIP(hipMemcpyAsync(pb, bbox_out_.get(), sizeof pb, hipMemcpyDeviceToHost, stream_));
+  PAM_HIP(hipStreamSynchronize(stream_));
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  auto merge = [&](const double* b) {
+    for (int k = 0; k < 3; k++) {
+      lo[k] = std::min(lo[k], b[k]);
+      hi[k] = std::max(hi[k], b[3 + k]);
+    }
+  };
+  if (n_src_ > 0) merge(pb);
+  if (!symmetric_ && n_trg_ > 0) merge(pb + 6);
+  Box box{};
+  bool user_ok = user_min_[0] <= user_max_[0] && user_min_[1] <= user_max_[1] && user_min_[2] <= user_max_[2];
+  if (user_ok) {
+    make_box_from_bbox(user_min_, user_max_, rbf_.aniso, box);
+    bool inside = true;
+    for (int k = 0; k < 3; k++)
+      if (lo[k] < box.lo[k] || hi[k] > box.lo[k] + box.width) inside = false;
+    if (!inside) user_ok = false;
+  }
+  if (!user_ok) {
+    // Points outside the given bbox (or none given): box of the transformed points.
+    double ident[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    if (!(lo[0] <= hi[0])) {
+      for (int k = 0; k < 3; k++) lo[k] = hi[k] = 0.0;
+    }
+    make_box_from_bbox(lo, hi, ident, box);
+  }
+  bool same = box.width == box_.width && box.lo[0] == box_.lo[0] && box.lo[1] == box_.lo[1] && box.lo[2] == box_.lo[2];
+  if (!same) {
+    box_ = box;
+    src_tree_dirty_ = trg_tree_dirty_ = true;
+    invalidate_expansions();
+  }
+  box_valid_ = true;
+}
+
+void FmmEvaluator::ensure_trees() {
+  ensure_box();
+  int L = cfg_.leaf_level >= 0 ? cfg_.leaf_level : auto_leaf_level(n_src_, n_trg());
+  bool rebuilt = false;
+  if (src_tree_dirty_ || src_tree_.level != L) {
+    build_tree(src_tree_, src_pts_.get(), n_src_, box_, L, ws_, stream_);
+    src_tree_dirty_ = false;
+    rebuilt = true;
+    s32_.spts_valid = s64_.spts_valid = false;
+    s32_.multipoles_valid = s64_.multipoles_valid = false;
+  }
+  if (!symmetric_ && (trg_tree_dirty_ || trg_tree_.level != L)) {
+    build_tree(trg_tree_, trg_pts_.get(), n_trg_, box_, L, ws_, stream_);
+    trg_tree_dirty_ = false;
+    rebuilt = true;
+    s32_.tpts_valid = s64_.tpts_valid = false;
+  }
+  if (symmetric_) trg_tree_dirty_ = false;
+  if (rebuilt || lists_.m2l.size() != static_cast<size_t>(L + 1)) {
+    build_lists(target_tree(), src_tree_, lists_, ws_, stream_);
+  }
+}
+
+void FmmEvaluator::prepare_trees() {
+  if (!config_valid_) choose_config();
+  ensure_trees();
+  PAM_HIP(hipStreamSynchronize(stream_));
+}
+
+template <>
+FmmState<float>& FmmEvaluator::state<float>() { return s32_; }
+template <>
+FmmState<double>& FmmEvaluator::state<double>() { return s64_; }
+
+void FmmEvaluator::record(int idx) { PAM_HIP(hipEventRecord(ev_[idx], stream_)); }
+
+template <class T>
+void FmmEvaluator::run_fmm(double* d_out) {
+  FmmState<T>& st = state<T>();
+  const int P = cfg_.order;
+  const int kp = kpad_of(P);
+  ensure_trees();
+  const Tree& tt = target_tree();
+  const int L = src_tree_.level;
+  record(1);
+  if (st.m.size() != static_cast<size_t>(L + 1)) {
+    st.m.clear();
+    st.l.clear();
+    st.m.resize(L + 1);
+    st.l.resize(L + 1);
+    st.multipoles_valid = false;
+  }
+  if (!st.spts_valid) {
+    st.spts.resize(n_src_);
+    launch_prepare_points<T>(src_tree_, src_pts_.get(), w_.get(), st.spts.get(), stream_);
+    st.spts_valid = true;
+    st.multipoles_valid = false;
+  }
+  if (!symmetric_ && !st.tpts_valid) {
+    st.tpts.resize(n_trg_);
+    launch_prepare_points<T>(trg_tree_, trg_pts_.get(), nullptr, st.tpts.get(), stream_);
+    st.tpts_valid = true;
+  }
+  const V4<T>* tpts = symmetric_ ? st.spts.get() : st.tpts.get();
+  if (!st.multipoles_valid) {
+    for (int l = 2; l <= L; l++) st.m[l].resize(static_cast<size_t>(src_tree_.levels[l].n) * kp);
+    launch_p2m<T>(P, src_tree_, st.spts.get(), st.m[L].get(), stream_);
+    for (int l = L - 1; l >= 2; l--) launch_m2m<T>(P, src_tree_, l, st.m[l + 1].get(), st.m[l].get(), stream_);
+    st.multipoles_valid = true;
+  }
+  record(2);
+  const int deg = homogeneity(rbf_);
+  const int32_t* vl = octant_vlist_device(stream_);
+  std::shared_ptr<OpEntry<T>> unit_op;
+  if (deg > 0) unit_op = get_op<T>(P, rbf_, 1.0, stream_);
+  for (int l = 2; l <= L; l++) {
+    st.l[l].resize(static_cast<size_t>(tt.levels[l].n) * kp);
+    double h = box_.width / std::ldexp(1.0, l) / 2.0;
+    std::shared_ptr<OpEntry<T>> op = deg > 0 ? unit_op : get_op<T>(P, rbf_, h, stream_);
+    T scale = static_cast<T>(deg > 0 ? std::pow(h, deg) : 1.0);
+    launch_m2l<T>(op->op, op->a.get(), lists_.m2l[l], lists_.m2l[l].src.get(), vl, st.m[l].get(), kp,
+                  st.l[l].get(), kp, scale, false, stream_);
+  }
+  record(3);
+  for (int l = 3; l <= L; l++) launch_l2l<T>(P, tt, l, st.l[l - 1].get(), st.l[l].get(), stream_);
+  record(4);
+  st.y_far.resize(static_cast<size_t>(n_trg()));
+  launch_l2p<T>(P, tt, st.l[L].get(), tpts, st.y_far.get(), stream_);
+  record(5);
+  launch_p2p<T>(rbf_, tt, src_tree_, lists_, tpts, st.spts.get(), st.y_far.get(), d_out, stream_);
+  record(6);
+}
+
+void FmmEvaluator::choose_config() {
+  FmmConfig c;
+  c.leaf_level = -1;
+  if (std::isinf(accuracy_)) {
+    c.precision = kMixed;
+    c.order = 6;
+  } else if (accuracy_ == 0.0) {
+    c.precision = kFp64;
+    c.order = 10;
+  } else {
+    // src/fmm/fmm_accuracy_estimator.hpp:74-121: try candidate configurations on
+    // up to 10,000 source points used as targets, keep the first whose L-inf
+    // error against the direct sum is within the accuracy.
+    if (!have_weights_) throw_runtime("weights must be set before a finite accuracy is tuned");
+    int64_t m = std::min<int64_t>(n_src_, 10000);
+    c.precision = kMixed;
+    c.order = 6;
+    if (m > 0) {
+      // Evenly strided sample of the source points (deterministic).
+      std::vector<double> all(static_cast<size_t>(n_src_) * 3);
+      // The raw (untransformed) points are not kept; sample the transformed ones
+      // and evaluate with an identity-anisotropy copy of the RBF.
+      PAM_HIP(hipMemcpyAsync(all.data(), src_pts_.get(), all.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+      std::vector<double> wts(static_cast<size_t>(n_src_));
+      PAM_HIP(hipMemcpyAsync(wts.data(), w_.get(), wts.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+      PAM_HIP(hipStreamSynchronize(stream_));
+      std::vector<double> smp(static_cast<size_t>(m) * 3);
+      for (int64_t i = 0; i < m; i++) {
+        int64_t j = (i * n_src_) / m;
+        for (int k = 0; k < 3; k++) smp[3 * i + k] = all[3 * j + k];
+      }
+      RbfDesc iso = rbf_;
+      for (int k = 0; k < 9; k++) iso.aniso[k] = (k % 4 == 0) ? 1.0 : 0.0;
+      ensure_box();
+      double bmin[3], bmax[3];
+      for (int k = 0; k < 3; k++) {
+        bmin[k] = box_.lo[k];
+        bmax[k] = box_.lo[k] + box_.width / 1.01;
+      }
+      FmmEvaluator probe(iso, bmin, bmax, false);
+      probe.set_source_points(all.data(), n_src_, false);
+      probe.set_weights(wts.data(), n_src_, false);
+      probe.set_target_points(smp.data(), m, false);
+      std::vector<double> exact(m), approx(m);
+      FmmConfig dc;
+      dc.mode = kModeDirect;
+      probe.set_config(dc);
+      probe.evaluate(exact.data(), m, false);
+      const int cand[][2] = {{kMixed, 4}, {kMixed, 5}, {kMixed, 6}, {kMixed, 7}, {kMixed, 8},
+                             {kFp64, 8}, {kFp64, 10}, {kFp64, 12}};
+      bool found = false;
+      for (auto& cd : cand) {
+        FmmConfig t;
+        t.precision = cd[0];
+        t.order = cd[1];
+        t.leaf_level = auto_leaf_level(n_src_, n_trg());
+        t.mode = kModeFmm;
+        probe.set_config(t);
+        probe.evaluate(approx.data(), m, false);
+        double err = 0.0;
+        for (int64_t i = 0; i < m; i++) err = std::max(err, std::abs(approx[i] - exact[i]));
+        if (err <= accuracy_) {
+          c.precision = cd[0];
+          c.order = cd[1];
+          found = true;
+          break;
+        }
+      }
+      if (!found) throw_runtime("failed to construct an evaluator that meets the desired accuracy");
+    }
+  }
+  FmmConfig old = cfg_;
+  cfg_ = c;
+  if (old.order != c.order) invalidate_expansions();
+  config_valid_ = true;
+}
+
+void FmmEvaluator::evaluate(double* out, int64_t n, bool on_device) {
+  int64_t nt = n_trg();
+  if (n != nt) throw_invalid("output size must equal the number of target points");
+  if (!have_weights_ && n_src_ > 0) throw_runtime("weights are not set");
+  if (!config_valid_) choose_config();
+  double* d_out = out;
+  DevBuf<double> tmp;
+  if (!on_device) {
+    tmp.resize(static_cast<size_t>(nt));
+    d_out = tmp.get();
+  }
+  record(0);
+  stats_ = FmmStats{};
+  if (nt > 0) {
+    if (n_src_ == 0) {
+      PAM_HIP(hipMemsetAsync(d_out, 0, sizeof(double) * nt, stream_));
+      for (int i = 1; i <= 6; i++) record(i);
+      stats_.direct = 1;
+    } else if (use_direct()) {
+      for (int i = 1; i <= 5; i++) record(i);
+      direct_sum_f64(rbf_, src_pts_.get(), w_.get(), n_src_, d_target_points(), nt, d_out, false,
+                     direct_scratch_, stream_);
+      record(6);
+      stats_.direct = 1;
+    } else if (cfg_.precision == kFp64) {
+      run_fmm<double>(d_out);
+    } else {
+      run_fmm<float>(d_out);
+    }
+  } else {
+    for (int i = 1; i <= 6; i++) record(i);
+  }
+  stats_.precision = cfg_.precision;
+  stats_.order = cfg_.order;
+  stats_.leaf_level = stats_.direct ? -1 : src_tree_.level;
+  stats_.n_src_leaves = stats_.direct ? 0 : src_tree_.n_leaves();
+  stats_.n_trg_leaves = stats_.direct ? 0 : target_tree().n_leaves();
+  stats_pending_ = true;
+  if (!on_device) {
+    tmp.download(out, static_cast<size_t>(nt), stream_);
+    PAM_HIP(hipStreamSynchronize(stream_));
+  }
+}
+
+const FmmStats& FmmEvaluator::stats() {
+  if (stats_pending_) {
+    PAM_HIP(hipEventSynchronize(ev_[6]));
+    float ms[6];
+    for (int i = 0; i < 6; i++) PAM_HIP(hipEventElapsedTime(&ms[i], ev_[i], ev_[i + 1]));
+    stats_.ms_tree = ms[0];
+    stats_.ms_upward = ms[1];
+    stats_.ms_m2l = ms[2];
+    stats_.ms_l2l = ms[3];
+    stats_.ms_l2p = ms[4];
+    stats_.ms_p2p = ms[5];
+    stats_.ms_total = ms[0] + ms[1] + ms[2] + ms[3] + ms[4] + ms[5];
+    stats_pending_ = false;
+  }
+  return stats_;
+}
+
+void FmmEvaluator::count_work(int64_t* p2p_pairs, int64_t* m2l_pairs) {
+  *p2p_pairs = 0;
+  *m2l_pairs = 0;
+  if (!config_valid_) choose_config();
+  ensure_trees();
+  PAM_HIP(hipStreamSynchronize(stream_));
+  const Tree& tt = target_tree();
+  int L = src_tree_.level;
+  int64_t nt = tt.n_leaves(), ns = src_tree_.n_leaves();
+  std::vector<int32_t> tls(nt + 1), sls(ns + 1), p2p(static_cast<size_t>(nt) * 27);
+  PAM_HIP(hipMemcpy(tls.data(), tt.leaf_start.get(), tls.size() * 4, hipMemcpyDeviceToHost));
+  PAM_HIP(hipMemcpy(sls.data(), src_tree_.leaf_start.get(), sls.size() * 4, hipMemcpyDeviceToHost));
+  PAM_HIP(hipMemcpy(p2p.data(), lists_.p2p.get(), p2p.size() * 4, hipMemcpyDeviceToHost));
+  for (int64_t t = 0; t < nt; t++) {
+    int64_t ssum = 0;
+    for (int k = 0; k < 27; k++) {
+      int32_t s = p2p[t * 27 + k];
+      if (s >= 0) ssum += sls[s + 1] - sls[s];
+    }
+    *p2p_pairs += ssum * (tls[t + 1] - tls[t]);
+  }
+  for (int l = 2; l <= L; l++) {
+    const auto& m = lists_.m2l[l];
+    std::vector<int32_t> src(static_cast<size_t>(tt.levels[l].n) * kM2LSlots);
+    if (src.empty()) continue;
+    PAM_HIP(hipMemcpy(src.data(), m.src.get(), src.size() * 4, hipMemcpyDeviceToHost));
+    for (int32_t s : src) *m2l_pairs += (s >= 0);
+  }
+}
+
+}  // namespace pam

@@ -1,0 +1,74 @@
+// Device-resident octree of one point set (uniform depth, non-empty cells only)
+// and the interaction lists between a target tree and a source tree.
+//
+// Definitions (restated by the CPU oracle for a bit-exact comparison):
+//   cell coords at level L: ix = clamp(floor((x - lo) * inv_cell), 0, 2^L - 1)
+//   Morton code: bit b of ix -> bit 3b+2, of iy -> 3b+1, of iz -> 3b
+//   points sorted stably by leaf code; cells of level l = unique(code >> 3(L-l))
+//   P2P slot (dx+1)*9+(dy+1)*3+(dz+1), separation criterion 1
+//   M2L: target t, source s at the same level with parents adjacent (or equal)
+//        and s, t not adjacent; transfer vector v = (dx+3)*49+(dy+3)*7+(dz+3)
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+
+namespace pam {
+
+struct Box {
+  double lo[3];
+  double width;
+};
+
+struct TreeLevel {
+  int64_t n = 0;
+  DevBuf<uint64_t> codes;        // [n] sorted unique cell codes
+  DevBuf<int32_t> parent;        // [n] parent index in level-1 (level >= 1)
+  DevBuf<int32_t> child_start;   // [n+1] child ranges in level+1 (level < leaf level)
+};
+
+struct Tree {
+  int level = -1;                // leaf level L
+  Box box{};
+  double inv_cell = 0.0;         // 2^L / width
+  int64_t n_points = 0;
+  DevBuf<uint64_t> point_codes;  // [n] sorted leaf codes of the points
+  DevBuf<int32_t> perm;          // [n] sorted position -> input index
+  DevBuf<int32_t> point_leaf;    // [n] sorted position -> leaf index
+  DevBuf<int32_t> leaf_start;    // [n_leaves+1]
+  std::vector<TreeLevel> levels; // [0..L]; levels[L] are the leaves
+  int64_t n_leaves() const { return level >= 0 ? levels[level].n : 0; }
+};
+
+// Scratch buffers reused across builds.
+struct TreeWorkspace {
+  DevBuf<int32_t> flags, scan, scan_scratch, hist, tv;
+  DevBuf<uint64_t> tk, tmp_codes;
+  DevBuf<int32_t> tmp_idx;
+};
+
+struct M2LLevel {
+  int64_t n_cols = 0;             // target columns, padded to 32 per octant group
+  DevBuf<int32_t> col_cell;       // [n_cols] target cell or -1
+  DevBuf<int32_t> blk_oct;        // [n_cols/32] octant of each 32-column block
+  DevBuf<int32_t> src;            // [n_trg_cells][189] source cell or -1 (octant slot order)
+};
+
+struct Lists {
+  DevBuf<int32_t> p2p;            // [n_trg_leaves][27]
+  std::vector<M2LLevel> m2l;      // index = level (entries < 2 unused)
+};
+
+constexpr int kM2LSlots = 189;
+constexpr int kM2LCols = 32;
+
+// The 189 transfer vectors valid for each octant, in increasing v order.
+const std::vector<int32_t>& octant_vlist_host();  // [8][189]
+const int32_t* octant_vlist_device(hipStream_t s);
+
+void build_tree(Tree& t, const double* d_pts, int64_t n, const Box& box, int level,
+                TreeWorkspace& ws, hipStream_t s);
+void build_lists(const Tree& trg, const Tree& src, Lists& lists, TreeWorkspace& ws, hipStream_t s);
+
+}  // namespace pam

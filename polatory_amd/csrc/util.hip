@@ -1,0 +1,361 @@
+// Generic device building blocks: point transforms, bbox reduction, prefix
+// sums, a stable LSD radix sort (4-bit digits), gathers, polynomial terms and
+// the FP64 direct O(N*M) sum.
+#include <type_traits>
+
+#include "device_api.hpp"
+#include "rbf_device.hpp"
+
+namespace pam {
+
+namespace {
+
+struct Mat3 {
+  double a[9];
+};
+
+__global__ void transform_kernel(Mat3 m, const double* __restrict__ in, int64_t n,
+                                 double* __restrict__ out) {
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double x = in[3 * i], y = in[3 * i + 1], z = in[3 * i + 2];
+  // Explicit round-to-nearest ops: no FMA contraction, so the result equals the
+  // CPU evaluation of ((A0 x + A1 y) + A2 z) bit for bit.
+  out[3 * i] = __dadd_rn(__dadd_rn(__dmul_rn(m.a[0], x), __dmul_rn(m.a[1], y)), __dmul_rn(m.a[2], z));
+  out[3 * i + 1] = __dadd_rn(__dadd_rn(__dmul_rn(m.a[3], x), __dmul_rn(m.a[4], y)), __dmul_rn(m.a[5], z));
+  out[3 * i + 2] = __dadd_rn(__dadd_rn(__dmul_rn(m.a[6], x), __dmul_rn(m.a[7], y)), __dmul_rn(m.a[8], z));
+}
+
+constexpr int kBboxThreads = 256;
+
+__global__ void bbox_partial_kernel(const double* __restrict__ pts, int64_t n,
+                                    double* __restrict__ partial) {
+  __shared__ double s[6][kBboxThreads];
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    for (int k = 0; k < 3; k++) {
+      double v = pts[3 * i + k];
+      lo[k] = fmin(lo[k], v);
+      hi[k] = fmax(hi[k], v);
+    }
+  }
+  for (int k = 0; k < 3; k++) {
+    s[k][threadIdx.x] = lo[k];
+    s[3 + k][threadIdx.x] = hi[k];
+  }
+  __syncthreads();
+  for (int off = kBboxThreads / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      for (int k = 0; k < 3; k++) {
+        s[k][threadIdx.x] = fmin(s[k][threadIdx.x], s[k][threadIdx.x + off]);
+        s[3 + k][threadIdx.x] = fmax(s[3 + k][threadIdx.x], s[3 + k][threadIdx.x + off]);
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) partial[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ void bbox_final_kernel(const double* __restrict__ partial, int nb, double* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double r[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (int b = 0; b < nb; b++) {
+    for (int k = 0; k < 3; k++) {
+      r[k] = fmin(r[k], partial[b * 6 + k]);
+      r[3 + k] = fmax(r[3 + k], partial[b * 6 + 3 + k]);
+    }
+  }
+  for (int k = 0; k < 6; k++) out[k] = r[k];
+}
+
+// ---- exclusive scan: 256 threads x 4 items per block ----
+constexpr int kScanThreads = 256;
+constexpr int kScanTile = 1024;
+
+__global__ void scan_block_kernel(const int32_t* in, int32_t* out, int64_t n, int32_t* block_sums) {
+  __shared__ int32_t s[kScanThreads];
+  int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile + threadIdx.x * 4;
+  int32_t v[4];
+  int32_t tsum = 0;
+  for (int q = 0; q < 4; q++) {
+    v[q] = (base + q < n) ? in[base + q] : 0;
+    tsum += v[q];
+  }
+  s[threadIdx.x] = tsum;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    int32_t t = (static_cast<int>(threadIdx.x) >= off) ? s[threadIdx.x - off] : 0;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  int32_t run = s[threadIdx.x] - tsum;
+  for (int q = 0; q < 4; q++) {
+    if (base + q < n) out[base + q] = run;
+    run += v[q];
+  }
+  if (threadIdx.x == kScanThreads - 1 && block_sums) block_sums[blockIdx.x] = s[threadIdx.x];
+}
+
+__global__ void scan_add_kernel(int32_t* out, int64_t n, const int32_t* offs) {
+  int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
+  int32_t o = offs[blockIdx.x];
+  for (int q = 0; q < 4; q++) {
+    int64_t i = base + threadIdx.x + q * kScanThreads;
+    if (i < n) out[i] += o;
+  }
+}
+
+void scan_rec(const int32_t* in, int32_t* out, int64_t n, int32_t* scratch, hipStream_t s) {
+  int64_t nb = (n + kScanTile - 1) / kScanTile;
+  if (nb <= 1) {
+    scan_block_kernel<<<1, kScanThreads, 0, s>>>(in, out, n, nullptr);
+    PAM_LAUNCH_CHECK();
+    return;
+  }
+  scan_block_kernel<<<static_cast<unsigned>(nb), kScanThreads, 0, s>>>(in, out, n, scratch);
+  PAM_LAUNCH_CHECK();
+  scan_rec(scratch, scratch, nb, scratch + nb, s);
+  scan_add_kernel<<<static_cast<unsigned>(nb), kScanThreads, 0, s>>>(out, n, scratch);
+  PAM_LAUNCH_CHECK();
+}
+
+// ---- radix sort: 4-bit digits, 256 threads x 8 consecutive items per tile ----
+constexpr int kRsThreads = 256;
+constexpr int kRsItems = 8;
+constexpr int kRsTile = kRsThreads * kRsItems;
+
+__global__ void radix_hist_kernel(const uint64_t* __restrict__ keys, int64_t n, int shift,
+                                  int32_t* __restrict__ hist, int nblocks) {
+  __shared__ int32_t cnt[16];
+  if (threadIdx.x < 16) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  int64_t base = static_cast<int64_t>(blockIdx.x) * kRsTile;
+  for (int q = 0; q < kRsItems; q++) {
+    int64_t i = base + threadIdx.x + q * kRsThreads;
+    if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 15u], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x < 16) hist[threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Counters for 16 digits packed as 4 x uint64 with 16-bit fields (a tile holds
+// at most 2048 items, so no field overflows).
+__global__ void radix_scatter_kernel(const uint64_t* __restrict__ kin, const int32_t* __restrict__ vin,
+                                     uint64_t* __restrict__ kout, int32_t* __restrict__ vout,
+                                     int64_t n, int shift, const int32_t* __restrict__ offs,
+                                     int nblocks) {
+  __shared__ uint64_t pk[4][kRsThreads];
+  __shared__ int32_t boff[16];
+  int64_t base = static_cast<int64_t>(blockIdx.x) * kRsTile + threadIdx.x * kRsItems;
+  uint64_t key[kRsItems];
+  int32_t val[kRsItems];
+  uint64_t c[4] = {0, 0, 0, 0};
+  for (int q = 0; q < kRsItems; q++) {
+    if (base + q < n) {
+      key[q] = kin[base + q];
+      val[q] = vin[base + q];
+      unsigned d = (key[q] >> shift) & 15u;
+      c[d >> 2] += 1ull << (16 * (d & 3));
+    }
+  }
+  for (int k = 0; k < 4; k++) pk[k][threadIdx.x] = c[k];
+  if (threadIdx.x < 16) boff[threadIdx.x] = offs[threadIdx.x * nblocks + blockIdx.x];
+  __syncthreads();
+  for (int off = 1; off < kRsThreads; off <<= 1) {
+    uint64_t t[4];
+    for (int k = 0; k < 4; k++) t[k] = (static_cast<int>(threadIdx.x) >= off) ? pk[k][threadIdx.x - off] : 0;
+    __syncthreads();
+    for (int k = 0; k < 4; k++) pk[k][threadIdx.x] += t[k];
+    __syncthreads();
+  }
+  uint64_t run[4];
+  for (int k = 0; k < 4; k++) run[k] = pk[k][threadIdx.x] - c[k];
+  for (int q = 0; q < kRsItems; q++) {
+    if (base + q < n) {
+      unsigned d = (key[q] >> shift) & 15u;
+      int64_t pos = boff[d] + static_cast<int64_t>((run[d >> 2] >> (16 * (d & 3))) & 0xFFFFull);
+      run[d >> 2] += 1ull << (16 * (d & 3));
+      kout[pos] = key[q];
+      vout[pos] = val[q];
+    }
+  }
+}
+
+__global__ void iota_kernel(int32_t* v, int64_t n) {
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = static_cast<int32_t>(i);
+}
+
+__global__ void gather_f64_kernel(const double* __restrict__ in, const int32_t* __restrict__ idx,
+                                  int64_t n, double* __restrict__ out) {
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[idx[i]];
+}
+
+// ---- FP64 direct sum: targets x source chunks, deterministic two-stage sum ----
+constexpr int kDirThreads = 256;
+
+template <int KIND>
+__global__ void __launch_bounds__(kDirThreads)
+direct_f64_kernel(double pa, double pb, const double* __restrict__ src, const double* __restrict__ w,
+                  int64_t n_src, const double* __restrict__ trg, int64_t n_trg,
+                  double* __restrict__ partial, int64_t chunk) {
+  __shared__ double4 tile[kDirThreads];
+  int64_t i = static_cast<int64_t>(blockIdx.x) * kDirThreads + threadIdx.x;
+  int64_t j0 = static_cast<int64_t>(blockIdx.y) * chunk;
+  int64_t j1 = j0 + chunk < n_src ? j0 + chunk : n_src;
+  double tx = 0, ty = 0, tz = 0;
+  if (i < n_trg) {
+    tx = trg[3 * i];
+    ty = trg[3 * i + 1];
+    tz = trg[3 * i + 2];
+  }
+  double acc = 0.0;
+  for (int64_t jt = j0; jt < j1; jt += kDirThreads) {
+    __syncthreads();
+    int64_t j = jt + threadIdx.x;
+    if (j < j1) tile[threadIdx.x] = make_double4(src[3 * j], src[3 * j + 1], src[3 * j + 2], w[j]);
+    __syncthreads();
+    int cnt = static_cast<int>((j1 - jt) < kDirThreads ? (j1 - jt) : kDirThreads);
+    for (int k = 0; k < cnt; k++) {
+      double4 q = tile[k];
+      double dx = tx - q.x, dy = ty - q.y, dz = tz - q.z;
+      acc += q.w * phi_f64<KIND>(dx * dx + dy * dy + dz * dz, pa, pb);
+    }
+  }
+  if (i < n_trg) partial[static_cast<int64_t>(blockIdx.y) * n_trg + i] = acc;
+}
+
+__global__ void reduce_partial_kernel(const double* __restrict__ partial, int nchunks, int64_t n,
+                                      double* __restrict__ out, int accumulate) {
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = 0.0;
+  for (int c = 0; c < nchunks; c++) acc += partial[c * n + i];
+  out[i] = accumulate ? out[i] + acc : acc;
+}
+
+__global__ void poly_eval_kernel(int degree, const double* __restrict__ pts, int64_t n,
+                                 const double* __restrict__ c, double* __restrict__ out) {
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
+  double acc = c[0];
+  if (degree >= 1) acc += c[1] * x + c[2] * y + c[3] * z;
+  if (degree >= 2) acc += c[4] * x * x + c[5] * x * y + c[6] * x * z + c[7] * y * y + c[8] * y * z + c[9] * z * z;
+  out[i] += acc;
+}
+
+}  // namespace
+
+void launch_transform_points(const double* aniso, const double* in, int64_t n, double* out,
+                             hipStream_t s) {
+  if (n == 0) return;
+  Mat3 m;
+  for (int i = 0; i < 9; i++) m.a[i] = aniso[i];
+  transform_kernel<<<ceil_div(n, 256), 256, 0, s>>>(m, in, n, out);
+  PAM_LAUNCH_CHECK();
+}
+
+void launch_bbox(const double* pts, int64_t n, double* d_out, DevBuf<double>& scratch,
+                 hipStream_t s) {
+  int nb = ceil_div(n, kBboxThreads);
+  if (nb > 512) nb = 512;
+  if (nb < 1) nb = 1;
+  scratch.resize(static_cast<size_t>(nb) * 6);
+  bbox_partial_kernel<<<nb, kBboxThreads, 0, s>>>(pts, n, scratch.get());
+  PAM_LAUNCH_CHECK();
+  bbox_final_kernel<<<1, 64, 0, s>>>(scratch.get(), nb, d_out);
+  PAM_LAUNCH_CHECK();
+}
+
+void exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, DevBuf<int32_t>& scratch,
+                        hipStream_t s) {
+  if (n <= 0) return;
+  int64_t need = 0;
+  for (int64_t m = n; m > 1;) {
+    m = (m + kScanTile - 1) / kScanTile;
+    need += m;
+  }
+  scratch.resize(static_cast<size_t>(need + 1));
+  scan_rec(in, out, n, scratch.get(), s);
+}
+
+void radix_sort_pairs(uint64_t* keys, int32_t* vals, int64_t n, int bits, DevBuf<uint64_t>& tk,
+                      DevBuf<int32_t>& tv, DevBuf<int32_t>& hist, DevBuf<int32_t>& scratch,
+                      hipStream_t s) {
+  if (n <= 1 || bits <= 0) return;
+  int nb = ceil_div(n, kRsTile);
+  tk.resize(static_cast<size_t>(n));
+  tv.resize(static_cast<size_t>(n));
+  hist.resize(static_cast<size_t>(16) * nb);
+  uint64_t* ka = keys;
+  int32_t* va = vals;
+  uint64_t* kb = tk.get();
+  int32_t* vb = tv.get();
+  int passes = (bits + 3) / 4;
+  for (int p = 0; p < passes; p++) {
+    int shift = 4 * p;
+    radix_hist_kernel<<<nb, kRsThreads, 0, s>>>(ka, n, shift, hist.get(), nb);
+    PAM_LAUNCH_CHECK();
+    exclusive_scan_i32(hist.get(), hist.get(), static_cast<int64_t>(16) * nb, scratch, s);
+    radix_scatter_kernel<<<nb, kRsThreads, 0, s>>>(ka, va, kb, vb, n, shift, hist.get(), nb);
+    PAM_LAUNCH_CHECK();
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  if (ka != keys) {
+    PAM_HIP(hipMemcpyAsync(keys, ka, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, s));
+    PAM_HIP(hipMemcpyAsync(vals, va, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
+  }
+}
+
+void launch_iota(int32_t* vals, int64_t n, hipStream_t s) {
+  if (n == 0) return;
+  iota_kernel<<<ceil_div(n, 256), 256, 0, s>>>(vals, n);
+  PAM_LAUNCH_CHECK();
+}
+
+void launch_gather_f64(const double* in, const int32_t* idx, int64_t n, double* out, hipStream_t s) {
+  if (n == 0) return;
+  gather_f64_kernel<<<ceil_div(n, 256), 256, 0, s>>>(in, idx, n, out);
+  PAM_LAUNCH_CHECK();
+}
+
+void direct_sum_f64(const RbfDesc& rbf, const double* src, const double* w, int64_t n_src,
+                    const double* trg, int64_t n_trg, double* out, bool accumulate,
+                    DevBuf<double>& scratch, hipStream_t s) {
+  if (n_trg == 0) return;
+  PhiConsts pc = phi_consts(rbf, 1.0);
+  int gx = ceil_div(n_trg, kDirThreads);
+  // Split sources so the grid has enough workgroups to fill 256 CUs.
+  int64_t want_chunks = (2048 + gx - 1) / gx;
+  int64_t chunk = n_src / (want_chunks > 0 ? want_chunks : 1);
+  if (chunk < 1024) chunk = 1024;
+  int nchunks = static_cast<int>(n_src > 0 ? (n_src + chunk - 1) / chunk : 1);
+  scratch.resize(static_cast<size_t>(nchunks) * n_trg);
+  if (n_src == 0) {
+    PAM_HIP(hipMemsetAsync(scratch.get(), 0, sizeof(double) * n_trg, s));
+  } else {
+    dim3 grid(gx, nchunks);
+    dispatch_kind(rbf.kind, [&](auto k) {
+      direct_f64_kernel<decltype(k)::value><<<grid, kDirThreads, 0, s>>>(pc.a, pc.b, src, w, n_src, trg,
+                                                                        n_trg, scratch.get(), chunk);
+    });
+    PAM_LAUNCH_CHECK();
+  }
+  reduce_partial_kernel<<<ceil_div(n_trg, 256), 256, 0, s>>>(scratch.get(), nchunks, n_trg, out,
+                                                             accumulate ? 1 : 0);
+  PAM_LAUNCH_CHECK();
+}
+
+void launch_poly_eval(int degree, const double* pts, int64_t n, const double* d_coeffs,
+                      double* out, hipStream_t s) {
+  if (n == 0 || degree < 0) return;
+  poly_eval_kernel<<<ceil_div(n, 256), 256, 0, s>>>(degree, pts, n, d_coeffs, out);
+  PAM_LAUNCH_CHECK();
+}
+
+}  // namespace pam

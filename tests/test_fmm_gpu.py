@@ -1,0 +1,227 @@
+"""GPU parity of the FMM seam against the CPU oracle (FP64 direct summation),
+bit-exact tree construction, and edge cases. Calls go through the C ABI."""
+import sys
+
+import numpy as np
+import pytest
+
+import polatory_amd as pa
+
+pytestmark = pytest.mark.gpu
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_err(y, ref):
+    return np.abs(y - ref).max() / np.abs(ref).max()
+
+
+def make(kind, params=None, aniso=None):
+    r = {"bh3": pa.Biharmonic3D, "th3": pa.Triharmonic3D, "exp": pa.CovExponential}[kind](
+        params if params is not None else ([1.0, 2.0] if kind == "exp" else []))
+    if aniso is not None:
+        r.set_anisotropy(aniso)
+    return r
+
+
+def orc_rbf(orc, r):
+    kind = {"bh3": orc.BH3, "th3": orc.TH3, "exp": orc.EXP}[r.short_name]
+    return orc.make_rbf(kind, r.parameters(), r.anisotropy())
+
+
+ANISO = np.array([[1.0, 0.1, 0.0], [0.0, 0.8, 0.2], [0.0, 0.0, 1.3]])
+
+
+@pytest.mark.parametrize("kind", ["bh3", "th3", "exp"])
+def test_gpu_direct_matches_oracle(orc, kind):
+    log("direct", kind)
+    rng = np.random.default_rng(11)
+    src = rng.uniform(0, 1, (3000, 3))
+    trg = rng.uniform(-0.1, 1.1, (2000, 3))
+    w = rng.normal(size=3000)
+    r = make(kind, aniso=ANISO)
+    y = pa.direct_evaluate(r, src, w, trg)
+    ref = orc.direct_rbf_sum(orc_rbf(orc, r), src, w, trg)
+    assert rel_err(y, ref) < 1e-13
+
+
+def _gpu_tree_vs_oracle(orc, ev, which, pts_raw, aniso):
+    info = ev.tree_info(which)
+    L = info["level"]
+    pts = orc.transform_points(aniso.reshape(-1), pts_raw)
+    codes = orc.morton_codes(pts, info["lo"], info["inv_cell"], L)
+    perm = orc.stable_sort_codes(codes)
+    sorted_codes = codes[perm]
+    gcodes, gperm, glc, gls = ev.export_tree(which)
+    np.testing.assert_array_equal(gperm, perm)
+    np.testing.assert_array_equal(gcodes, sorted_codes)
+    lc, ls = orc.leaves(sorted_codes)
+    np.testing.assert_array_equal(glc, lc)
+    np.testing.assert_array_equal(gls, ls)
+    for l in range(L + 1):
+        np.testing.assert_array_equal(ev.export_level(which, l), np.unique(lc >> np.uint64(3 * (L - l))))
+    return L
+
+
+@pytest.mark.parametrize("dist", ["uniform", "clustered", "aniso"])
+def test_tree_and_lists_bit_exact(orc, dist):
+    log("tree", dist)
+    rng = np.random.default_rng(12)
+    if dist == "clustered":
+        src = np.concatenate([rng.normal(0.3, 0.02, (3000, 3)), rng.uniform(0, 1, (500, 3))])
+        trg = np.concatenate([rng.normal(0.7, 0.05, (2000, 3)), rng.uniform(0, 1, (300, 3))])
+    else:
+        src = rng.uniform(0, 1, (4000, 3))
+        trg = rng.uniform(0, 1, (2500, 3))
+    aniso = ANISO if dist == "aniso" else np.eye(3)
+    r = make("bh3", aniso=aniso)
+    lo = np.minimum(src.min(0), trg.min(0))
+    hi = np.maximum(src.max(0), trg.max(0))
+    ev = pa.make_fmm_evaluator(r, (lo, hi))
+    ev.set_config(pa.FmmConfig("mixed", 4, 4, "fmm"))
+    ev.set_source_points(src)
+    ev.set_target_points(trg)
+    ev.set_weights(np.ones(len(src)))
+    # the box follows src/fmm/utility.hpp:19-33
+    c, w = orc.make_box(lo, hi, aniso)
+    info = ev.tree_info(0)
+    assert info["width"] == w
+    np.testing.assert_array_equal(info["lo"], c - w / 2.0)
+    L = _gpu_tree_vs_oracle(orc, ev, 0, src, aniso)
+    _gpu_tree_vs_oracle(orc, ev, 1, trg, aniso)
+    _, _, slc, _ = ev.export_tree(0)
+    _, _, tlc, _ = ev.export_tree(1)
+    np.testing.assert_array_equal(ev.export_p2p(), orc.p2p_lists(tlc, slc, L))
+    for l in range(2, L + 1):
+        tl = ev.export_level(1, l)
+        sl = ev.export_level(0, l)
+        np.testing.assert_array_equal(ev.export_m2l(l), orc.m2l_lists(tl, sl, l))
+
+
+# (kind, precision, order, tolerance on the relative L-inf error vs FP64 direct)
+FMM_CASES = [
+    ("bh3", "mixed", 4, 2e-4), ("bh3", "mixed", 6, 1e-5), ("bh3", "mixed", 8, 2e-6),
+    ("bh3", "fp64", 6, 1e-5), ("bh3", "fp64", 10, 1e-8),
+    ("th3", "mixed", 6, 1e-5), ("th3", "fp64", 8, 1e-7),
+    ("exp", "mixed", 6, 1e-5), ("exp", "fp64", 8, 1e-7),
+]
+
+
+@pytest.fixture(scope="module")
+def fmm_problem(orc):
+    rng = np.random.default_rng(13)
+    src = rng.uniform(0, 1, (20000, 3))
+    trg = rng.uniform(0, 1, (12000, 3))
+    w = rng.normal(size=20000)
+    refs = {}
+    for kind in ("bh3", "th3", "exp"):
+        r = make(kind)
+        refs[kind] = orc.direct_rbf_sum(orc_rbf(orc, r), src, w, trg)
+    return src, trg, w, refs
+
+
+@pytest.mark.parametrize("kind,prec,order,tol", FMM_CASES)
+def test_fmm_matches_direct(fmm_problem, kind, prec, order, tol):
+    log("fmm", kind, prec, order)
+    src, trg, w, refs = fmm_problem
+    ev = pa.make_fmm_evaluator(make(kind), ((0, 0, 0), (1, 1, 1)))
+    ev.set_config(pa.FmmConfig(prec, order, 3, "fmm"))
+    ev.set_source_points(src)
+    ev.set_weights(w)
+    ev.set_target_points(trg)
+    y = ev.evaluate()
+    e = rel_err(y, refs[kind])
+    log(f"  rel err {e:.3e} (tol {tol:.0e})")
+    assert e < tol
+
+
+@pytest.mark.parametrize("prec,order,tol", [("mixed", 6, 1e-5), ("fp64", 10, 1e-8)])
+def test_symmetric_fmm_matches_direct(orc, prec, order, tol):
+    log("symmetric", prec, order)
+    rng = np.random.default_rng(14)
+    pts = rng.uniform(0, 1, (15000, 3))
+    w = rng.normal(size=15000)
+    r = make("th3", aniso=ANISO)
+    ev = pa.make_fmm_symmetric_evaluator(r)
+    ev.set_config(pa.FmmConfig(prec, order, 3, "fmm"))
+    ev.set_points(pts)
+    ev.set_weights(w)
+    y = ev.evaluate()
+    ref = orc.direct_rbf_sum(orc_rbf(orc, r), pts, w, pts)
+    e = rel_err(y, ref)
+    log(f"  rel err {e:.3e}")
+    assert e < tol
+
+
+def test_clustered_and_ragged_leaves(orc):
+    log("clustered")
+    rng = np.random.default_rng(15)
+    src = np.concatenate([rng.normal(0.2, 0.01, (6000, 3)), rng.uniform(-1, 1, (2000, 3)),
+                          np.array([[0.5, 0.5, 0.5]] * 50)])  # coincident points
+    trg = np.concatenate([rng.normal(0.2, 0.05, (3000, 3)), rng.uniform(-1, 1, (1000, 3))])
+    w = rng.normal(size=len(src))
+    r = make("bh3")
+    ev = pa.make_fmm_evaluator(r)
+    ev.set_config(pa.FmmConfig("fp64", 8, 4, "fmm"))
+    ev.set_source_points(src)
+    ev.set_weights(w)
+    ev.set_target_points(trg)
+    y = ev.evaluate()
+    ref = orc.direct_rbf_sum(orc_rbf(orc, r), src, w, trg)
+    assert rel_err(y, ref) < 1e-6
+
+
+def test_weights_update_reuses_tree(orc):
+    log("weights update")
+    rng = np.random.default_rng(16)
+    src = rng.uniform(0, 1, (9000, 3))
+    trg = rng.uniform(0, 1, (9000, 3))
+    r = make("bh3")
+    ev = pa.make_fmm_evaluator(r)
+    ev.set_config(pa.FmmConfig("mixed", 6, 3, "fmm"))
+    ev.set_source_points(src)
+    ev.set_target_points(trg)
+    for seed in range(3):
+        w = np.random.default_rng(seed).normal(size=9000)
+        ev.set_weights(w)
+        y = ev.evaluate()
+        ref = orc.direct_rbf_sum(orc_rbf(orc, r), src, w, trg)
+        assert rel_err(y, ref) < 1e-5
+
+
+def test_small_and_empty_inputs(orc):
+    log("small/empty")
+    r = make("exp")
+    ev = pa.make_fmm_evaluator(r)
+    ev.set_source_points(np.zeros((1, 3)))
+    ev.set_weights([2.0])
+    ev.set_target_points(np.zeros((0, 3)))
+    assert ev.evaluate().shape == (0,)
+    ev.set_target_points(np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0]]))
+    y = ev.evaluate()
+    np.testing.assert_allclose(y, [2.0, 2.0 * np.exp(-1.5)], rtol=1e-14)
+    with pytest.raises(ValueError):
+        ev.set_weights([1.0, 2.0])
+
+
+def test_accuracy_search_meets_accuracy(orc):
+    log("accuracy search")
+    rng = np.random.default_rng(17)
+    src = rng.uniform(0, 1, (30000, 3))
+    trg = rng.uniform(0, 1, (8000, 3))
+    w = rng.normal(size=30000)
+    r = make("bh3")
+    ref = orc.direct_rbf_sum(orc_rbf(orc, r), src, w, trg)
+    acc = 1e-6 * np.abs(ref).max()
+    ev = pa.make_fmm_evaluator(r)
+    ev.set_source_points(src)
+    ev.set_weights(w)
+    ev.set_target_points(trg)
+    ev.set_accuracy(acc)
+    y = ev.evaluate()
+    log("  chosen", ev.config(), "err", np.abs(y - ref).max(), "acc", acc)
+    # the search checks its sample (source points); targets may land slightly above
+    assert np.abs(y - ref).max() < 3 * acc
