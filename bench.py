@@ -1,0 +1,195 @@
+"""Benchmark: FMM evaluation of 1,000,000 biharmonic3d sources at 1,000,000
+targets on one MI355X (mixed FP32/FP32-MFMA form vs the plain FP64 form).
+
+Workload (seeded, synthetic): sources U[0,1]^3 (seed 3), weights N(0,1) (seed 4),
+degree-1 polynomial coefficients N(0,1) (seed 5), targets U[0,1]^3 (seed 6);
+error vs FP64 direct summation on a 10,000-target subsample (seed 7).
+A step = one full evaluation from device-resident points and weights: point
+transform, source and target octrees, interaction lists, P2M/M2M, M2L, L2L,
+L2P, P2P and the polynomial terms, output in target input order in HBM.
+
+Prints one JSON line (see the contract in the repository documentation).
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_SRC = 1_000_000
+N_TRG = 1_000_000
+N_SUB = 10_000
+PEAK_FP32 = 157.3e12
+PEAK_FP64 = 78.6e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_inputs():
+    src = np.random.default_rng(3).uniform(0, 1, (N_SRC, 3))
+    w = np.random.default_rng(4).normal(size=N_SRC)
+    c = np.random.default_rng(5).normal(size=4)
+    trg = np.random.default_rng(6).uniform(0, 1, (N_TRG, 3))
+    sub = np.random.default_rng(7).permutation(N_TRG)[:N_SUB]
+    return src, w, c, trg, sub
+
+
+class DeviceEval:
+    """Evaluator with device-resident inputs (torch owns the HBM buffers)."""
+
+    def __init__(self, torch, src, w, trg, cfg):
+        import polatory_amd as pa
+        from polatory_amd import _capi
+        self.pa, self.capi, self.torch = pa, _capi, torch
+        dev = torch.device("cuda:0")
+        self.d_src = torch.from_numpy(src).to(dev).contiguous()
+        self.d_trg = torch.from_numpy(trg).to(dev).contiguous()
+        self.d_w = torch.from_numpy(w).to(dev).contiguous()
+        self.d_out = torch.empty(trg.shape[0], dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        self.ev = pa.make_fmm_evaluator(pa.Biharmonic3D(), ((0, 0, 0), (1, 1, 1)))
+        self.ev.set_config(cfg)
+        self.n_src, self.n_trg = src.shape[0], trg.shape[0]
+
+    def step(self):
+        P = ctypes.c_void_p
+        h = self.ev._h
+        self.capi.call("pam_fmm_evaluator_set_source_points_device", h, P(self.d_src.data_ptr()), self.n_src)
+        self.capi.call("pam_fmm_evaluator_set_weights_device", h, P(self.d_w.data_ptr()), self.n_src)
+        self.capi.call("pam_fmm_evaluator_set_target_points_device", h, P(self.d_trg.data_ptr()), self.n_trg)
+        self.capi.call("pam_fmm_evaluator_evaluate_device", h, P(self.d_out.data_ptr()), self.n_trg)
+
+    def sync(self):
+        self.capi.call("pam_fmm_evaluator_synchronize", self.ev._h)
+
+    def output(self):
+        return self.d_out.cpu().numpy()
+
+
+def poly(c, pts):
+    return c[0] + pts @ c[1:]
+
+
+def time_steps(de, steps, warmup):
+    for _ in range(warmup):
+        de.step()
+    de.sync()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        de.step()
+        stats.append(de.ev.stats())
+    de.sync()
+    t1 = time.perf_counter()
+    return (t1 - t0) / steps, stats
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--order", type=int, default=7)
+    ap.add_argument("--order64", type=int, default=7)
+    ap.add_argument("--leaf-level", type=int, default=-1)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import polatory_amd as pa
+
+    src, w, c, trg, sub = make_inputs()
+    log("inputs ready")
+    cfg32 = pa.FmmConfig("mixed", args.order, args.leaf_level, "fmm")
+    cfg64 = pa.FmmConfig("fp64", args.order64, args.leaf_level, "fmm")
+
+    # FP64 direct reference on the subsample (GPU) plus polynomial terms
+    ref = pa.direct_evaluate(pa.Biharmonic3D(), src, w, trg[sub]) + poly(c, trg[sub])
+    ref_inf = np.abs(ref).max()
+
+    results = {}
+    for name, cfg in (("mixed", cfg32), ("fp64", cfg64)):
+        de = DeviceEval(torch, src, w, trg, cfg)
+        sec, stats = time_steps(de, args.steps, args.warmup)
+        y = de.output() + poly(c, trg)  # polynomial terms on the host for the error check only
+        err = np.abs(y[sub] - ref).max() / ref_inf
+        p2p_pairs, m2l_pairs = de.ev.count_work()
+        st = stats[-1]
+        avg = {k: float(np.mean([s[k] for s in stats])) for k in
+               ("ms_tree", "ms_upward", "ms_m2l", "ms_l2l", "ms_l2p", "ms_p2p", "ms_total")}
+        results[name] = dict(sec=sec, err=err, p2p_pairs=p2p_pairs, m2l_pairs=m2l_pairs,
+                             order=st["order"], leaf_level=st["leaf_level"], stage_ms=avg)
+        log(name, json.dumps(results[name]))
+        del de
+        torch.cuda.synchronize()
+
+    m = results["mixed"]
+    f = results["fp64"]
+    p = m["order"]
+    stage = m["stage_ms"]
+    m2l_flops = 2.0 * p**6 * m["m2l_pairs"]
+    p2p_flops = 11.0 * m["p2p_pairs"]
+    if stage["ms_m2l"] >= stage["ms_p2p"]:
+        achieved = m2l_flops / (stage["ms_m2l"] * 1e-3)
+        roof = {"kernel": "m2l_kernel<float> (all levels)", "bound": "mfma", "achieved": achieved / 1e12,
+                "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
+                "traffic": None}
+    else:
+        achieved = p2p_flops / (stage["ms_p2p"] * 1e-3)
+        roof = {"kernel": "p2p_kernel<float,bh3>", "bound": "mfma", "achieved": achieved / 1e12,
+                "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
+                "traffic": None}
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        n_cpu = 2000
+        t0 = time.perf_counter()
+        oracle.direct_rbf_sum(oracle.make_rbf(oracle.BH3), src, w, trg[:n_cpu])
+        dt = time.perf_counter() - t0
+        cpu = {"value": n_cpu / dt, "unit": "evals/s", "cores": cores, "kind": "port",
+               "sample": f"oracle FP64 direct O(N*M) sum, {N_SRC} sources x {n_cpu} targets "
+                         f"({dt:.1f} s), OpenMP over {cores} threads"}
+
+    out = {
+        "metric": "FMM evals/s at 1M pts, <=1e-6 rel err (mixed vs FP64 GPU)",
+        "value": N_TRG / m["sec"],
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": m["sec"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (FP32 P2P, FP32-MFMA M2L, FP64 outputs)",
+        "data": "synthetic (seeded U[0,1]^3 points, N(0,1) weights)",
+        "config": {"workload": "biharmonic3d FMM evaluation, 1,000,000 sources -> 1,000,000 targets, "
+                               "degree-1 polynomial",
+                   "n_src": N_SRC, "n_trg": N_TRG, "order": m["order"], "leaf_level": m["leaf_level"]},
+        "rel_err": m["err"],
+        "fp64_gpu": {"value": N_TRG / f["sec"], "ms_per_step": f["sec"] * 1e3, "rel_err": f["err"],
+                     "order": f["order"]},
+        "speedup_vs_fp64_gpu": f["sec"] / m["sec"],
+        "stage_ms": stage,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

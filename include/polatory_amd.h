@@ -131,6 +131,88 @@ int pam_fmm_evaluator_export_m2l(pam_fmm_evaluator* h, int level, int32_t* slots
 int pam_direct_evaluate(const pam_rbf* rbf, const double* src, int64_t n_src, const double* w,
                         const double* trg, int64_t n_trg, double* out);
 
+
+/* ---------------------------------------------------------------------------
+ * Model (include/polatory/model.hpp:32-183): rbfs, polynomial degree, nugget.
+ * ------------------------------------------------------------------------- */
+#define PAM_POLY_DEGREE_MIN (-2) /* Model::kMinRequiredPolyDegree: cpd_order - 1 */
+typedef struct pam_model pam_model;
+/* Model(std::vector<Rbf>, poly_degree) + set_nugget (model.hpp:36-51, 135-141) */
+int pam_model_create(const pam_rbf* rbfs, int n_rbfs, int poly_degree, double nugget,
+                     pam_model** out);
+int pam_model_destroy(pam_model* m);
+/* poly_degree() and poly_basis_size() (model.hpp:126-131) */
+int pam_model_poly_degree(const pam_model* m, int* degree);
+int pam_model_poly_basis_size(const pam_model* m, int* l);
+
+/* ---------------------------------------------------------------------------
+ * Interpolant (include/polatory/interpolant.hpp:34-367): fit and evaluate.
+ * Weights are [w (n) ; c (l)] as in Interpolant::weights() (interpolant.hpp:212-216).
+ * ------------------------------------------------------------------------- */
+typedef struct pam_interpolant pam_interpolant;
+typedef struct {
+  pam_fmm_config op;        /* FGMRES operator (default: mixed FMM, direct FP64 when small) */
+  pam_fmm_config residual;  /* FP64 residual recomputation (default: FP64 FMM / direct) */
+  pam_fmm_config level;     /* RAS inter-level updates when use_level_config != 0 */
+  int use_level_config;
+  int use_preconditioner;   /* 1: multilevel RAS (ras_preconditioner.hpp), 0: none */
+  double refine_rtol;       /* inner relative residual that restarts FGMRES on the FP64 residual */
+  int verbose;              /* print the residual per iteration (solver.hpp:111-128) */
+} pam_fit_options;
+int pam_fit_options_default(pam_fit_options* opt);
+/* Interpolant(const Model&) — interpolant.hpp:43 */
+int pam_interpolant_create(const pam_model* model, pam_interpolant** out);
+int pam_interpolant_destroy(pam_interpolant* h);
+/* fit(points, values, tolerance, max_iter = 100, accuracy = inf, initial = nullptr)
+ * — interpolant.hpp:77-113. Absolute L-inf tolerance, recomputed in FP64.
+ * opt and initial may be NULL. */
+int pam_interpolant_fit(pam_interpolant* h, const double* points, int64_t n, const double* values,
+                        double tolerance, int max_iter, double accuracy, const pam_fit_options* opt,
+                        const pam_interpolant* initial);
+/* evaluate(points, accuracy = inf) — interpolant.hpp:54-56 */
+int pam_interpolant_evaluate(pam_interpolant* h, const double* points, int64_t n, double accuracy,
+                             double* out);
+int pam_interpolant_evaluate_device(pam_interpolant* h, const double* d_points, int64_t n,
+                                    double accuracy, double* d_out);
+/* Fixes the evaluation FMM configuration (NULL restores the accuracy-driven choice). */
+int pam_interpolant_set_eval_config(pam_interpolant* h, const pam_fmm_config* cfg);
+int pam_interpolant_num_centers(const pam_interpolant* h, int64_t* n);
+/* centers() — interpolant.hpp:48-52 (n x 3) */
+int pam_interpolant_centers(const pam_interpolant* h, double* out);
+/* weights() — interpolant.hpp:212-216 (n + l) */
+int pam_interpolant_weights(const pam_interpolant* h, double* out);
+/* Installs centres and weights (e.g. from a saved interpolant, interpolant.hpp:345-367). */
+int pam_interpolant_set_fitted(pam_interpolant* h, const double* centers, int64_t n,
+                               const double* weights);
+typedef struct {
+  int iterations, restarts, n_levels;
+  double residual, seconds, setup_seconds, precond_setup_seconds;
+  int64_t factor_bytes;
+} pam_fit_stats;
+int pam_interpolant_fit_stats(const pam_interpolant* h, pam_fit_stats* out);
+/* Residual after each FGMRES iteration (out may be NULL to query the length). */
+int pam_interpolant_residual_history(const pam_interpolant* h, double* out, int64_t* n);
+
+/* ---------------------------------------------------------------------------
+ * Krylov and preconditioner seams (krylov::LinearOperator users).
+ * ------------------------------------------------------------------------- */
+/* Right-preconditioned FGMRES on the device for a dense row-major operator
+ * (src/krylov/fgmres.cpp:8-26, gmres.cpp:9-50); m may be NULL (no preconditioner).
+ * Runs n_iter iterations; resid_hist receives |g_j| for j = 0..iters. */
+int pam_fgmres_dense(const double* a, const double* m, int64_t n, const double* rhs,
+                     const double* x0, int n_iter, double* x, double* resid_hist, int* iters);
+/* RasPreconditioner(model, points) — ras_preconditioner.hpp:56-189 */
+typedef struct pam_ras pam_ras;
+int pam_ras_create(const pam_model* model, const double* points, int64_t n, pam_ras** out);
+int pam_ras_destroy(pam_ras* h);
+int pam_ras_num_levels(const pam_ras* h, int* n_levels);
+/* point indices of a level (level n_levels-1 = all points; 0 = coarse grid) */
+int pam_ras_level_points(const pam_ras* h, int level, int32_t* out, int64_t* n);
+int pam_ras_num_domains(const pam_ras* h, int level, int* n);
+int pam_ras_domain(const pam_ras* h, int level, int dom, int32_t* idx, uint8_t* inner, int64_t* n);
+/* operator()(v) — ras_preconditioner.hpp:191-250 (host vectors of length n + l) */
+int pam_ras_apply(pam_ras* h, const double* v, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,6 +34,19 @@ class PamFmmStats(ctypes.Structure):
                 ("n_src_leaves", ctypes.c_int64), ("n_trg_leaves", ctypes.c_int64)]
 
 
+class PamFitOptions(ctypes.Structure):
+    _fields_ = [("op", PamFmmConfig), ("residual", PamFmmConfig), ("level", PamFmmConfig),
+                ("use_level_config", ctypes.c_int), ("use_preconditioner", ctypes.c_int),
+                ("refine_rtol", ctypes.c_double), ("verbose", ctypes.c_int)]
+
+
+class PamFitStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("restarts", ctypes.c_int), ("n_levels", ctypes.c_int),
+                ("residual", ctypes.c_double), ("seconds", ctypes.c_double),
+                ("setup_seconds", ctypes.c_double), ("precond_setup_seconds", ctypes.c_double),
+                ("factor_bytes", ctypes.c_int64)]
+
+
 class PamError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(msg)
@@ -79,6 +92,31 @@ _SIGS = {
     "pam_fmm_evaluator_export_p2p": [_P, _P],
     "pam_fmm_evaluator_export_m2l": [_P, _INT, _P],
     "pam_direct_evaluate": [ctypes.POINTER(PamRbf), _P, _I64, _P, _P, _I64, _P],
+    "pam_model_create": [_P, _INT, _INT, _D, ctypes.POINTER(_P)],
+    "pam_model_destroy": [_P],
+    "pam_model_poly_degree": [_P, ctypes.POINTER(_INT)],
+    "pam_model_poly_basis_size": [_P, ctypes.POINTER(_INT)],
+    "pam_fit_options_default": [ctypes.POINTER(PamFitOptions)],
+    "pam_interpolant_create": [_P, ctypes.POINTER(_P)],
+    "pam_interpolant_destroy": [_P],
+    "pam_interpolant_fit": [_P, _P, _I64, _P, _D, _INT, _D, ctypes.POINTER(PamFitOptions), _P],
+    "pam_interpolant_evaluate": [_P, _P, _I64, _D, _P],
+    "pam_interpolant_evaluate_device": [_P, _P, _I64, _D, _P],
+    "pam_interpolant_set_eval_config": [_P, ctypes.POINTER(PamFmmConfig)],
+    "pam_interpolant_num_centers": [_P, ctypes.POINTER(_I64)],
+    "pam_interpolant_centers": [_P, _P],
+    "pam_interpolant_weights": [_P, _P],
+    "pam_interpolant_set_fitted": [_P, _P, _I64, _P],
+    "pam_interpolant_fit_stats": [_P, ctypes.POINTER(PamFitStats)],
+    "pam_interpolant_residual_history": [_P, _P, ctypes.POINTER(_I64)],
+    "pam_fgmres_dense": [_P, _P, _I64, _P, _P, _INT, _P, _P, ctypes.POINTER(_INT)],
+    "pam_ras_create": [_P, _P, _I64, ctypes.POINTER(_P)],
+    "pam_ras_destroy": [_P],
+    "pam_ras_num_levels": [_P, ctypes.POINTER(_INT)],
+    "pam_ras_level_points": [_P, _INT, _P, ctypes.POINTER(_I64)],
+    "pam_ras_num_domains": [_P, _INT, ctypes.POINTER(_INT)],
+    "pam_ras_domain": [_P, _INT, _INT, _P, _P, ctypes.POINTER(_I64)],
+    "pam_ras_apply": [_P, _P, _P],
 }
 
 _lib = None

@@ -5,6 +5,7 @@
 #include <string>
 
 #include "../../include/polatory_amd.h"
+#include "capi_util.hpp"
 #include "device_api.hpp"
 #include "fmm_evaluator.hpp"
 
@@ -13,35 +14,18 @@ struct pam_fmm_evaluator {
   bool symmetric;
 };
 
-namespace {
+namespace pam {
 
-thread_local std::string g_last_error;
-
-template <class F>
-int guard(F&& f) {
-  try {
-    f();
-    return PAM_OK;
-  } catch (const pam::Error& e) {
-    g_last_error = e.what();
-    return e.code;
-  } catch (const std::bad_alloc&) {
-    g_last_error = "host allocation failed";
-    return PAM_ERR_OUT_OF_MEMORY;
-  } catch (const std::exception& e) {
-    g_last_error = e.what();
-    return PAM_ERR_RUNTIME;
-  } catch (...) {
-    g_last_error = "unknown error";
-    return PAM_ERR_RUNTIME;
-  }
+std::string& last_error_ref() {
+  thread_local std::string e;
+  return e;
 }
 
-pam::RbfDesc to_desc(const pam_rbf* r) {
-  if (!r) pam::throw_invalid("rbf is null");
+RbfDesc to_desc(const pam_rbf* r) {
+  if (!r) throw_invalid("rbf is null");
   if (r->kind != PAM_RBF_BH3 && r->kind != PAM_RBF_TH3 && r->kind != PAM_RBF_EXP)
-    pam::throw_invalid("unknown rbf kind");
-  pam::RbfDesc d{};
+    throw_invalid("unknown rbf kind");
+  RbfDesc d{};
   d.kind = r->kind;
   d.p0 = r->params[0];
   d.p1 = r->params[1];
@@ -49,11 +33,18 @@ pam::RbfDesc to_desc(const pam_rbf* r) {
   const double* a = d.aniso;
   double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
                a[2] * (a[3] * a[7] - a[4] * a[6]);
-  if (!(det > 0.0)) pam::throw_invalid("aniso must have a positive determinant");
-  if (d.kind == PAM_RBF_EXP && !(d.p1 > 0.0)) pam::throw_invalid("range must be positive");
-  if (!(d.p0 >= 0.0) || !(d.p1 >= 0.0)) pam::throw_invalid("rbf parameters must be nonnegative");
+  if (!(det > 0.0)) throw_invalid("aniso must have a positive determinant");
+  if (d.kind == PAM_RBF_EXP && !(d.p1 > 0.0)) throw_invalid("range must be positive");
+  if (!(d.p0 >= 0.0) || !(d.p1 >= 0.0)) throw_invalid("rbf parameters must be nonnegative");
   return d;
 }
+
+}  // namespace pam
+
+namespace {
+
+using pam::guard;
+using pam::to_desc;
 
 pam::FmmEvaluator& ev(pam_fmm_evaluator* h) {
   if (!h || !h->impl) pam::throw_invalid("null evaluator handle");
@@ -80,7 +71,7 @@ extern "C" {
 
 int pam_last_error(char* buf, size_t len) {
   if (buf && len > 0) {
-    std::strncpy(buf, g_last_error.c_str(), len - 1);
+    std::strncpy(buf, pam::last_error_ref().c_str(), len - 1);
     buf[len - 1] = '\0';
   }
   return PAM_OK;

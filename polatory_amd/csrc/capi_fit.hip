@@ -1,0 +1,346 @@
+// extern "C" entry points of the interpolation layer: model, interpolant fit /
+// evaluate, and the Krylov / RAS seams.
+#include <cmath>
+#include <limits>
+#include <memory>
+#include <vector>
+
+#include "capi_util.hpp"
+#include "interpolant.hpp"
+#include "ras.hpp"
+
+struct pam_model {
+  pam::Model m;
+};
+
+struct pam_interpolant {
+  std::unique_ptr<pam::Interpolant> impl;
+};
+
+struct pam_ras {
+  hipStream_t s = nullptr;
+  pam::Model model;
+  std::unique_ptr<pam::RasPreconditioner> impl;
+  ~pam_ras() {
+    impl.reset();
+    if (s) {
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
+    }
+  }
+};
+
+namespace {
+
+using pam::guard;
+
+pam::FmmConfig to_cfg(const pam_fmm_config& c) {
+  pam::FmmConfig r;
+  r.precision = c.precision;
+  r.order = c.order;
+  r.leaf_level = c.leaf_level;
+  r.mode = c.mode;
+  return r;
+}
+
+pam_fmm_config from_cfg(const pam::FmmConfig& c) {
+  pam_fmm_config r;
+  r.precision = c.precision;
+  r.order = c.order;
+  r.leaf_level = c.leaf_level;
+  r.mode = c.mode;
+  return r;
+}
+
+pam::FitOptions default_fit_options() {
+  pam::FitOptions o;
+  // Inner operator: mixed-precision FMM (FP32 P2P, FP32-MFMA M2L); the FP64
+  // direct sum below ~2^26 pairs.
+  o.op_cfg.precision = pam::kMixed;
+  o.op_cfg.order = 8;
+  o.op_cfg.mode = pam::kModeAuto;
+  // Final residual recomputed in FP64: FP64 FMM of order 10, direct when small.
+  o.res_cfg.precision = pam::kFp64;
+  o.res_cfg.order = 10;
+  o.res_cfg.mode = pam::kModeAuto;
+  o.level_cfg = pam::FmmConfig{};
+  return o;
+}
+
+pam::FitOptions to_opts(const pam_fit_options* o) {
+  if (!o) return default_fit_options();
+  pam::FitOptions r;
+  r.op_cfg = to_cfg(o->op);
+  r.res_cfg = to_cfg(o->residual);
+  r.level_cfg = to_cfg(o->level);
+  r.use_level_cfg = o->use_level_config != 0;
+  r.use_preconditioner = o->use_preconditioner != 0;
+  r.refine_rtol = o->refine_rtol;
+  r.verbose = o->verbose;
+  return r;
+}
+
+pam::Interpolant& ip(const pam_interpolant* h) {
+  if (!h || !h->impl) pam::throw_invalid("null interpolant handle");
+  return *h->impl;
+}
+
+// Dense row-major operator on the device (FGMRES tests).
+__global__ void dense_gemv_kernel(const double* __restrict__ a, const double* __restrict__ x, int64_t n,
+                                  double* __restrict__ y) {
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = 0.0;
+  for (int64_t j = 0; j < n; j++) acc += a[i * n + j] * x[j];
+  y[i] = acc;
+}
+
+class DenseOperator : public pam::LinearOperator {
+ public:
+  DenseOperator(const double* h_a, int64_t n, hipStream_t s) : n_(n), s_(s) {
+    a_.upload(h_a, static_cast<size_t>(n) * n, s);
+  }
+  void apply(const double* d_x, double* d_y) override {
+    dense_gemv_kernel<<<pam::ceil_div(n_, 128), 128, 0, s_>>>(a_.get(), d_x, n_, d_y);
+    PAM_LAUNCH_CHECK();
+  }
+  int64_t size() const override { return n_; }
+
+ private:
+  int64_t n_;
+  hipStream_t s_;
+  pam::DevBuf<double> a_;
+};
+
+}  // namespace
+
+extern "C" {
+
+int pam_model_create(const pam_rbf* rbfs, int n_rbfs, int poly_degree, double nugget, pam_model** out) {
+  return guard([&] {
+    if (!out) pam::throw_invalid("out is null");
+    if (n_rbfs <= 0) pam::throw_invalid("rbfs must not be empty");
+    auto m = std::make_unique<pam_model>();
+    for (int i = 0; i < n_rbfs; i++) m->m.rbfs.push_back(pam::to_desc(&rbfs[i]));
+    int lo = m->m.cpd_order() - 1;
+    m->m.poly_degree = poly_degree == PAM_POLY_DEGREE_MIN ? lo : poly_degree;
+    m->m.nugget = nugget;
+    m->m.validate();
+    *out = m.release();
+  });
+}
+
+int pam_model_destroy(pam_model* m) {
+  return guard([&] { delete m; });
+}
+
+int pam_model_poly_degree(const pam_model* m, int* degree) {
+  return guard([&] {
+    if (!m) pam::throw_invalid("null model");
+    *degree = m->m.poly_degree;
+  });
+}
+
+int pam_model_poly_basis_size(const pam_model* m, int* l) {
+  return guard([&] {
+    if (!m) pam::throw_invalid("null model");
+    *l = m->m.l();
+  });
+}
+
+int pam_fit_options_default(pam_fit_options* opt) {
+  return guard([&] {
+    auto o = default_fit_options();
+    opt->op = from_cfg(o.op_cfg);
+    opt->residual = from_cfg(o.res_cfg);
+    opt->level = from_cfg(o.level_cfg);
+    opt->use_level_config = 0;
+    opt->use_preconditioner = 1;
+    opt->refine_rtol = o.refine_rtol;
+    opt->verbose = 0;
+  });
+}
+
+int pam_interpolant_create(const pam_model* model, pam_interpolant** out) {
+  return guard([&] {
+    if (!model || !out) pam::throw_invalid("null argument");
+    auto h = std::make_unique<pam_interpolant>();
+    h->impl = std::make_unique<pam::Interpolant>(model->m);
+    *out = h.release();
+  });
+}
+
+int pam_interpolant_destroy(pam_interpolant* h) {
+  return guard([&] { delete h; });
+}
+
+int pam_interpolant_fit(pam_interpolant* h, const double* points, int64_t n, const double* values,
+                        double tolerance, int max_iter, double accuracy, const pam_fit_options* opt,
+                        const pam_interpolant* initial) {
+  return guard([&] {
+    if (n < 0) pam::throw_invalid("negative point count");
+    std::vector<double> p(points, points + 3 * n), v(values, values + n);
+    ip(h).fit(p, v, tolerance, max_iter, accuracy, to_opts(opt), initial ? &ip(initial) : nullptr);
+  });
+}
+
+int pam_interpolant_evaluate(pam_interpolant* h, const double* points, int64_t n, double accuracy,
+                             double* out) {
+  return guard([&] { ip(h).evaluate(points, n, accuracy, out, false); });
+}
+
+int pam_interpolant_evaluate_device(pam_interpolant* h, const double* d_points, int64_t n,
+                                    double accuracy, double* d_out) {
+  return guard([&] { ip(h).evaluate(d_points, n, accuracy, d_out, true); });
+}
+
+int pam_interpolant_set_eval_config(pam_interpolant* h, const pam_fmm_config* cfg) {
+  return guard([&] {
+    if (cfg) {
+      pam::FmmConfig c = to_cfg(*cfg);
+      ip(h).set_eval_config(&c);
+    } else {
+      ip(h).set_eval_config(nullptr);
+    }
+  });
+}
+
+int pam_interpolant_num_centers(const pam_interpolant* h, int64_t* n) {
+  return guard([&] { *n = static_cast<int64_t>(ip(h).centers().size() / 3); });
+}
+
+int pam_interpolant_centers(const pam_interpolant* h, double* out) {
+  return guard([&] {
+    const auto& c = ip(h).centers();
+    std::copy(c.begin(), c.end(), out);
+  });
+}
+
+int pam_interpolant_weights(const pam_interpolant* h, double* out) {
+  return guard([&] {
+    if (!ip(h).fitted()) pam::throw_runtime("the interpolant is not fitted");
+    const auto& w = ip(h).weights();
+    std::copy(w.begin(), w.end(), out);
+  });
+}
+
+int pam_interpolant_set_fitted(pam_interpolant* h, const double* centers, int64_t n, const double* weights) {
+  return guard([&] {
+    int l = ip(h).model().l();
+    std::vector<double> c(centers, centers + 3 * n), w(weights, weights + n + l);
+    ip(h).set_fitted(c, w);
+  });
+}
+
+int pam_interpolant_fit_stats(const pam_interpolant* h, pam_fit_stats* out) {
+  return guard([&] {
+    const auto& s = ip(h).fit_stats();
+    out->iterations = s.iterations;
+    out->restarts = s.restarts;
+    out->n_levels = s.n_levels;
+    out->residual = s.residual;
+    out->seconds = s.seconds;
+    out->setup_seconds = s.setup_seconds;
+    out->precond_setup_seconds = s.precond_setup_seconds;
+    out->factor_bytes = s.factor_bytes;
+  });
+}
+
+int pam_interpolant_residual_history(const pam_interpolant* h, double* out, int64_t* n) {
+  return guard([&] {
+    const auto& hist = ip(h).fit_stats().history;
+    *n = static_cast<int64_t>(hist.size());
+    if (out) std::copy(hist.begin(), hist.end(), out);
+  });
+}
+
+int pam_fgmres_dense(const double* a, const double* m, int64_t n, const double* rhs, const double* x0,
+                     int n_iter, double* x, double* resid_hist, int* iters) {
+  return guard([&] {
+    hipStream_t s;
+    PAM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    struct StreamGuard {
+      hipStream_t s;
+      ~StreamGuard() {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+      }
+    } sg{s};
+    DenseOperator op(a, n, s);
+    std::unique_ptr<DenseOperator> pc;
+    if (m) pc = std::make_unique<DenseOperator>(m, n, s);
+    pam::DevBuf<double> d_rhs, d_x0, d_x(n);
+    d_rhs.upload(rhs, n, s);
+    pam::Fgmres solver(op, d_rhs.get(), n, n_iter, s);
+    if (x0) {
+      d_x0.upload(x0, n, s);
+      solver.set_initial_solution(d_x0.get());
+    }
+    solver.set_right_preconditioner(pc.get());
+    solver.setup();
+    resid_hist[0] = solver.absolute_residual();
+    for (int i = 0; i < n_iter; i++) {
+      solver.iterate_process();
+      resid_hist[i + 1] = solver.absolute_residual();
+    }
+    solver.solution_vector(d_x.get());
+    d_x.download(x, n, s);
+    PAM_HIP(hipStreamSynchronize(s));
+    *iters = solver.iteration_count();
+  });
+}
+
+int pam_ras_create(const pam_model* model, const double* points, int64_t n, pam_ras** out) {
+  return guard([&] {
+    if (!model || !out) pam::throw_invalid("null argument");
+    auto h = std::make_unique<pam_ras>();
+    PAM_HIP(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
+    h->model = model->m;
+    std::vector<double> p(points, points + 3 * n);
+    h->impl = std::make_unique<pam::RasPreconditioner>(h->model, p, h->s, pam::RasOptions{});
+    *out = h.release();
+  });
+}
+
+int pam_ras_destroy(pam_ras* h) {
+  return guard([&] { delete h; });
+}
+
+int pam_ras_num_levels(const pam_ras* h, int* n_levels) {
+  return guard([&] { *n_levels = h->impl->n_levels(); });
+}
+
+int pam_ras_level_points(const pam_ras* h, int level, int32_t* out, int64_t* n) {
+  return guard([&] {
+    const auto& lp = h->impl->level_points();
+    if (level < 0 || level >= static_cast<int>(lp.size())) pam::throw_invalid("level out of range");
+    *n = static_cast<int64_t>(lp[level].size());
+    if (out) std::copy(lp[level].begin(), lp[level].end(), out);
+  });
+}
+
+int pam_ras_num_domains(const pam_ras* h, int level, int* n) {
+  return guard([&] { *n = static_cast<int>(h->impl->level_domains(level).size()); });
+}
+
+int pam_ras_domain(const pam_ras* h, int level, int dom, int32_t* idx, uint8_t* inner, int64_t* n) {
+  return guard([&] {
+    const auto& d = h->impl->level_domains(level).at(dom);
+    *n = static_cast<int64_t>(d.idx.size());
+    if (idx) std::copy(d.idx.begin(), d.idx.end(), idx);
+    if (inner) std::copy(d.inner.begin(), d.inner.end(), inner);
+  });
+}
+
+int pam_ras_apply(pam_ras* h, const double* v, double* out) {
+  return guard([&] {
+    int64_t m = h->impl->size();
+    pam::DevBuf<double> dv, dout(m);
+    dv.upload(v, m, h->s);
+    h->impl->apply(dv.get(), dout.get());
+    dout.download(out, m, h->s);
+    PAM_HIP(hipStreamSynchronize(h->s));
+  });
+}
+
+}  // extern "C"

@@ -426,6 +426,19 @@ m2l_kernel(int r, int rpad, int kpad, const T* __restrict__ a, int64_t per_v,
   const int nv = src_slots ? kM2LSlots : 1;
   if (threadIdx.x < kM2LCols) cells[threadIdx.x] = col_cell[blk * kM2LCols + threadIdx.x];
   acc_t acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+  // FP32 MFMA accumulators are flushed into FP64 every 64 reduction steps: the
+  // rounding of one long FP32 running sum (189 x P^3 terms) otherwise dominates
+  // the error of the whole far field.
+  double d0[4] = {0, 0, 0, 0}, d1[4] = {0, 0, 0, 0};
+  auto flush = [&]() {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      d0[i] += static_cast<double>(acc0[i]);
+      d1[i] += static_cast<double>(acc1[i]);
+      acc0[i] = T(0);
+      acc1[i] = T(0);
+    }
+  };
   const int col = threadIdx.x / 8, part = threadIdx.x % 8;  // staging map: 8 lanes per column
   for (int sidx = 0; sidx < nv; sidx++) {
     const int v = src_slots ? vlist[oct * kM2LSlots + sidx] : 0;
@@ -464,21 +477,27 @@ m2l_kernel(int r, int rpad, int kpad, const T* __restrict__ a, int64_t per_v,
           acc1 = Mfma<T>::op(a4.z, b1.z, acc1);
           acc0 = Mfma<T>::op(a4.w, b0.w, acc0);
           acc1 = Mfma<T>::op(a4.w, b1.w, acc1);
+          if constexpr (sizeof(T) == 4) {
+            if ((kb & 3) == 3) flush();
+          }
         }
+        if constexpr (sizeof(T) == 4) flush();
       }
     }
   }
-  // D layout (16x16): lane -> column lane%16, rows 4*(lane/16) + i.
+  if constexpr (sizeof(T) == 8) flush();
+  // D layout (16x16), column lane%16; register i holds row 4*(lane/16) + i for
+  // the f32 instruction and row lane/16 + 4*i for the f64 one (measured on gfx950).
 #pragma unroll
   for (int ct = 0; ct < 2; ct++) {
     const int c = cells[ct * 16 + (lane & 15)];
     if (c < 0) continue;
-    const acc_t& acc = ct == 0 ? acc0 : acc1;
+    const double* acc = ct == 0 ? d0 : d1;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      int row = row0 + 4 * (lane >> 4) + i;
+      int row = row0 + (sizeof(T) == 4 ? 4 * (lane >> 4) + i : (lane >> 4) + 4 * i);
       if (row < r) {
-        T val = acc[i] * scale;
+        T val = static_cast<T>(acc[i] * static_cast<double>(scale));
         T* dst = out + static_cast<int64_t>(c) * out_stride + row;
         *dst = accumulate ? *dst + val : val;
       }

@@ -100,13 +100,18 @@ int auto_leaf_level(int64_t n_src, int64_t n_trg) {
 }
 
 FmmEvaluator::FmmEvaluator(const RbfDesc& rbf, const double* bbox_min, const double* bbox_max,
-                           bool symmetric)
+                           bool symmetric, hipStream_t stream)
     : rbf_(rbf), symmetric_(symmetric), accuracy_(std::numeric_limits<double>::infinity()) {
   for (int k = 0; k < 3; k++) {
     user_min_[k] = bbox_min ? bbox_min[k] : INFINITY;
     user_max_[k] = bbox_max ? bbox_max[k] : -INFINITY;
   }
-  PAM_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  if (stream) {
+    stream_ = stream;
+    own_stream_ = false;
+  } else {
+    PAM_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  }
   for (auto& e : ev_) PAM_HIP(hipEventCreate(&e));
   bbox_out_.resize(12);
 }
@@ -115,7 +120,7 @@ FmmEvaluator::~FmmEvaluator() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (auto& e : ev_)
     if (e) (void)hipEventDestroy(e);
-  if (stream_) (void)hipStreamDestroy(stream_);
+  if (stream_ && own_stream_) (void)hipStreamDestroy(stream_);
 }
 
 void FmmEvaluator::set_accuracy(double accuracy) {

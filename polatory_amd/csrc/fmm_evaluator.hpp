@@ -42,7 +42,10 @@ struct FmmState {
 
 class FmmEvaluator {
  public:
-  FmmEvaluator(const RbfDesc& rbf, const double* bbox_min, const double* bbox_max, bool symmetric);
+  // stream == nullptr: the evaluator owns a stream; otherwise all work is
+  // ordered on the caller's stream (which must outlive the evaluator).
+  FmmEvaluator(const RbfDesc& rbf, const double* bbox_min, const double* bbox_max, bool symmetric,
+               hipStream_t stream = nullptr);
   ~FmmEvaluator();
   FmmEvaluator(const FmmEvaluator&) = delete;
   FmmEvaluator& operator=(const FmmEvaluator&) = delete;
@@ -87,6 +90,7 @@ class FmmEvaluator {
   bool symmetric_;
   double user_min_[3], user_max_[3];
   hipStream_t stream_ = nullptr;
+  bool own_stream_ = true;
 
   double accuracy_;
   bool fixed_config_ = false;

@@ -1,0 +1,180 @@
+// Interpolant facade (see interpolant.hpp).
+#include "interpolant.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <unordered_map>
+
+namespace pam {
+
+namespace {
+
+struct PointKey {
+  uint64_t b[3];
+  bool operator==(const PointKey& o) const { return b[0] == o.b[0] && b[1] == o.b[1] && b[2] == o.b[2]; }
+};
+struct PointHash {
+  size_t operator()(const PointKey& k) const {
+    uint64_t h = k.b[0] * 0x9e3779b97f4a7c15ull;
+    h ^= k.b[1] + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
+    h ^= k.b[2] + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
+    return static_cast<size_t>(h);
+  }
+};
+PointKey key_of(const double* p) {
+  PointKey k;
+  std::memcpy(k.b, p, sizeof k.b);
+  return k;
+}
+
+bool same_model(const Model& a, const Model& b) {
+  if (a.poly_degree != b.poly_degree || a.nugget != b.nugget || a.rbfs.size() != b.rbfs.size()) return false;
+  for (size_t i = 0; i < a.rbfs.size(); i++) {
+    const auto &x = a.rbfs[i], &y = b.rbfs[i];
+    if (x.kind != y.kind || x.p0 != y.p0 || x.p1 != y.p1) return false;
+    for (int k = 0; k < 9; k++)
+      if (x.aniso[k] != y.aniso[k]) return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+Interpolant::Interpolant(const Model& model) : model_(model) {
+  model_.validate();
+  PAM_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+}
+
+Interpolant::~Interpolant() {
+  ev_.reset();
+  if (s_) {
+    (void)hipStreamSynchronize(s_);
+    (void)hipStreamDestroy(s_);
+  }
+}
+
+void Interpolant::throw_if_not_fitted() const {
+  if (!fitted_) throw_runtime("the interpolant is not fitted");
+}
+
+std::vector<double> Interpolant::build_initial_weights(const std::vector<double>& points,
+                                                       const Interpolant& initial) const {
+  // interpolant.hpp:232-273: weights of coincident centres, tail from the initial.
+  int l = model_.l();
+  int64_t mu = static_cast<int64_t>(points.size() / 3);
+  std::vector<double> w(static_cast<size_t>(mu + l), 0.0);
+  if (!same_model(model_, initial.model_)) return w;
+  std::unordered_map<PointKey, int64_t, PointHash> ini;
+  int64_t ini_mu = static_cast<int64_t>(initial.centers_.size() / 3);
+  for (int64_t i = 0; i < ini_mu; i++) ini.emplace(key_of(&initial.centers_[3 * i]), i);
+  for (int64_t i = 0; i < mu; i++) {
+    auto it = ini.find(key_of(&points[3 * i]));
+    if (it != ini.end()) w[i] = initial.weights_[it->second];
+  }
+  for (int k = 0; k < l; k++) w[mu + k] = initial.weights_[ini_mu + k];
+  return w;
+}
+
+void Interpolant::fit(const std::vector<double>& points, const std::vector<double>& values,
+                      double tolerance, int max_iter, double accuracy, const FitOptions& opt,
+                      const Interpolant* initial) {
+  int64_t mu = static_cast<int64_t>(points.size() / 3);
+  int l = model_.l();
+  if (mu < l) throw_invalid("points.rows() must be greater than or equal to " + std::to_string(l));
+  if (static_cast<int64_t>(values.size()) != mu) throw_invalid("values.rows() must be equal to " + std::to_string(mu));
+  if (!(tolerance > 0.0)) throw_invalid("tolerance must be positive");
+  if (max_iter < 0) throw_invalid("max_iter must be nonnegative");
+  if (!(accuracy > 0.0)) throw_invalid("accuracy must be positive");
+  std::vector<double> init;
+  if (initial) {
+    if (initial->fitted_) init = build_initial_weights(points, *initial);
+    else initial = nullptr;
+  }
+  fitted_ = false;
+  ev_.reset();
+  Solver solver(model_, points, opt, s_);
+  weights_ = solver.solve(values, tolerance, max_iter, initial ? &init : nullptr, &stats_);
+  centers_ = points;
+  fitted_ = true;
+  for (int k = 0; k < 3; k++) {
+    bbox_lo_[k] = INFINITY;
+    bbox_hi_[k] = -INFINITY;
+  }
+  for (int64_t i = 0; i < mu; i++)
+    for (int k = 0; k < 3; k++) {
+      bbox_lo_[k] = std::min(bbox_lo_[k], points[3 * i + k]);
+      bbox_hi_[k] = std::max(bbox_hi_[k], points[3 * i + k]);
+    }
+}
+
+void Interpolant::set_fitted(const std::vector<double>& centers, const std::vector<double>& weights) {
+  int64_t mu = static_cast<int64_t>(centers.size() / 3);
+  if (static_cast<int64_t>(weights.size()) != mu + model_.l()) throw_invalid("weights.rows() must be centers.rows() + l");
+  centers_ = centers;
+  weights_ = weights;
+  fitted_ = true;
+  ev_.reset();
+  for (int k = 0; k < 3; k++) {
+    bbox_lo_[k] = INFINITY;
+    bbox_hi_[k] = -INFINITY;
+  }
+  for (int64_t i = 0; i < mu; i++)
+    for (int k = 0; k < 3; k++) {
+      bbox_lo_[k] = std::min(bbox_lo_[k], centers[3 * i + k]);
+      bbox_hi_[k] = std::max(bbox_hi_[k], centers[3 * i + k]);
+    }
+}
+
+void Interpolant::set_eval_config(const FmmConfig* c) {
+  have_eval_cfg_ = c != nullptr;
+  if (c) eval_cfg_ = *c;
+  ev_.reset();
+}
+
+Evaluator& Interpolant::evaluator() {
+  throw_if_not_fitted();
+  if (!ev_) {
+    int64_t mu = static_cast<int64_t>(centers_.size() / 3);
+    ev_ = std::make_unique<Evaluator>(model_, bbox_lo_, bbox_hi_, ev_acc_ > 0 ? ev_acc_ : INFINITY, s_);
+    if (have_eval_cfg_) ev_->set_fmm_config(eval_cfg_);
+    d_centers_.upload(centers_.data(), centers_.size(), s_);
+    d_weights_.upload(weights_.data(), weights_.size(), s_);
+    ev_->set_source_points(d_centers_.get(), mu);
+    ev_->set_weights(d_weights_.get());
+    PAM_HIP(hipStreamSynchronize(s_));
+  }
+  return *ev_;
+}
+
+void Interpolant::evaluate(const double* points, int64_t n, double accuracy, double* out, bool on_device) {
+  throw_if_not_fitted();
+  if (!(accuracy > 0.0)) throw_invalid("accuracy must be positive");
+  if (accuracy != ev_acc_) {
+    ev_.reset();
+    ev_acc_ = accuracy;
+  }
+  Evaluator& ev = evaluator();
+  const double* d_in = points;
+  if (!on_device) {
+    d_pts_.upload(points, static_cast<size_t>(n) * 3, s_);
+    d_in = d_pts_.get();
+  }
+  double* d_out = out;
+  if (!on_device) {
+    d_out_.resize(static_cast<size_t>(n));
+    d_out = d_out_.get();
+  }
+  ev.set_target_points(d_in, n);
+  ev.evaluate(d_out);
+  if (!on_device) {
+    PAM_HIP(hipMemcpyAsync(out, d_out, sizeof(double) * n, hipMemcpyDeviceToHost, s_));
+  }
+  PAM_HIP(hipStreamSynchronize(s_));
+}
+
+int64_t Interpolant::resident_bytes() const {
+  return static_cast<int64_t>(d_centers_.bytes() + d_weights_.bytes());
+}
+
+}  // namespace pam

@@ -1,0 +1,55 @@
+// Interpolant facade (include/polatory/interpolant.hpp:34-367) for the value
+// kernels: fit with FGMRES + RAS on the device, evaluate with the device FMM.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "solver.hpp"
+
+namespace pam {
+
+class Interpolant {
+ public:
+  explicit Interpolant(const Model& model);
+  ~Interpolant();
+  Interpolant(const Interpolant&) = delete;
+  Interpolant& operator=(const Interpolant&) = delete;
+
+  // interpolant.hpp:77-113
+  void fit(const std::vector<double>& points, const std::vector<double>& values, double tolerance,
+           int max_iter, double accuracy, const FitOptions& opt, const Interpolant* initial);
+  // interpolant.hpp:54-67 (+ polynomial terms); pointers are host or device.
+  void evaluate(const double* points, int64_t n, double accuracy, double* out, bool on_device);
+  // Installs centres and weights directly (a saved or externally fitted model).
+  void set_fitted(const std::vector<double>& centers, const std::vector<double>& weights);
+  void set_eval_config(const FmmConfig* c);
+
+  bool fitted() const { return fitted_; }
+  const Model& model() const { return model_; }
+  const std::vector<double>& centers() const { return centers_; }
+  const std::vector<double>& weights() const { return weights_; }
+  const FitStats& fit_stats() const { return stats_; }
+  hipStream_t stream() const { return s_; }
+  Evaluator& evaluator();
+  int64_t resident_bytes() const;
+
+ private:
+  std::vector<double> build_initial_weights(const std::vector<double>& points,
+                                            const Interpolant& initial) const;
+  void throw_if_not_fitted() const;
+
+  Model model_;
+  hipStream_t s_ = nullptr;
+  bool fitted_ = false;
+  std::vector<double> centers_, weights_;
+  FitStats stats_;
+  std::unique_ptr<Evaluator> ev_;
+  double ev_acc_ = -1.0;
+  bool have_eval_cfg_ = false;
+  FmmConfig eval_cfg_;
+  DevBuf<double> d_centers_, d_weights_, d_pts_, d_out_;
+  double bbox_lo_[3] = {0, 0, 0}, bbox_hi_[3] = {0, 0, 0};
+};
+
+}  // namespace pam

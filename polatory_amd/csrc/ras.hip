@@ -1,0 +1,920 @@
+// Multilevel RAS preconditioner: host domain division, batched device assembly,
+// batched FP64 Cholesky of Q^T A Q and batched subdomain solves.
+#include "ras.hpp"
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <list>
+#include <numeric>
+#include <queue>
+
+#include "device_api.hpp"
+
+namespace pam {
+
+namespace {
+
+constexpr int kT = 256;
+constexpr int kNb = 32;     // Cholesky panel width
+constexpr int kTile = 64;   // trailing-update tile
+
+struct RbfSet {
+  int n;
+  int kind[4];
+  double p0[4], p1[4];
+  double a[4][9];
+  double nugget;
+};
+
+RbfSet make_rbf_set(const Model& m) {
+  if (m.rbfs.size() > 4) throw_invalid("at most 4 rbfs per model are supported");
+  RbfSet r{};
+  r.n = static_cast<int>(m.rbfs.size());
+  for (int k = 0; k < r.n; k++) {
+    r.kind[k] = m.rbfs[k].kind;
+    r.p0[k] = m.rbfs[k].p0;
+    r.p1[k] = m.rbfs[k].p1;
+    for (int i = 0; i < 9; i++) r.a[k][i] = m.rbfs[k].aniso[i];
+  }
+  r.nugget = m.nugget;
+  return r;
+}
+
+// mat_a entry (preconditioner/mat_a.hpp:11-61): sum of rbfs + nugget on the diagonal.
+__device__ __forceinline__ double kernel_entry(const RbfSet& r, const double* __restrict__ pts, int u, int v) {
+  double dx = pts[3 * u] - pts[3 * v], dy = pts[3 * u + 1] - pts[3 * v + 1], dz = pts[3 * u + 2] - pts[3 * v + 2];
+  double s = 0.0;
+  for (int k = 0; k < r.n; k++) {
+    const double* a = r.a[k];
+    double tx = a[0] * dx + a[1] * dy + a[2] * dz;
+    double ty = a[3] * dx + a[4] * dy + a[5] * dz;
+    double tz = a[6] * dx + a[7] * dy + a[8] * dz;
+    double r2 = tx * tx + ty * ty + tz * tz;
+    if (r.kind[k] == kBh3) {
+      s += -r.p0[k] * sqrt(r2 + r.p1[k] * r.p1[k]);
+    } else if (r.kind[k] == kTh3) {
+      double rho = sqrt(r2 + r.p1[k] * r.p1[k]);
+      s += r.p0[k] * rho * rho * rho;
+    } else {
+      s += r.p0[k] * exp(-3.0 * sqrt(r2) / r.p1[k]);
+    }
+  }
+  if (u == v) s += r.nugget;
+  return s;
+}
+
+__device__ __forceinline__ void monomials(int degree, double x, double y, double z, double* m) {
+  m[0] = 1.0;
+  if (degree >= 1) { m[1] = x; m[2] = y; m[3] = z; }
+  if (degree >= 2) { m[4] = x * x; m[5] = x * y; m[6] = x * z; m[7] = y * y; m[8] = y * z; m[9] = z * z; }
+}
+
+// lag[i*l + k] = sum_j m_j(x_i) pinv[j*l + k]   (LagrangeBasis::evaluate, lagrange_basis.hpp:50-58)
+__global__ void lagrange_kernel(int degree, int l, const double* __restrict__ pts, int64_t n,
+                                const double* __restrict__ pinv, double* __restrict__ lag) {
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double m[10];
+  monomials(degree, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], m);
+  for (int k = 0; k < l; k++) {
+    double s = 0.0;
+    for (int j = 0; j < l; j++) s += m[j] * pinv[j * l + k];
+    lag[i * l + k] = s;
+  }
+}
+
+// U[a][j] = A(p_a, q_j) (l x n_d), Atl[a][b] = A(p_a, p_b)
+__global__ void assemble_u_kernel(RbfSet r, int l, const double* __restrict__ pts,
+                                  const int64_t* __restrict__ ptr, const int32_t* __restrict__ idx,
+                                  double* __restrict__ u, double* __restrict__ atl) {
+  int d = blockIdx.y;
+  int64_t p0 = ptr[d];
+  int m = static_cast<int>(ptr[d + 1] - p0);
+  int n = m - l;
+  int64_t uoff = static_cast<int64_t>(l) * (p0 - static_cast<int64_t>(d) * l);
+  int total = l * n + l * l;
+  for (int e = blockIdx.x * kT + threadIdx.x; e < total; e += gridDim.x * kT) {
+    if (e < l * n) {
+      int a = e / n, j = e % n;
+      u[uoff + e] = kernel_entry(r, pts, idx[p0 + a], idx[p0 + l + j]);
+    } else {
+      int f = e - l * n;
+      int a = f / l, b = f % l;
+      atl[static_cast<int64_t>(d) * l * l + f] = kernel_entry(r, pts, idx[p0 + a], idx[p0 + b]);
+    }
+  }
+}
+
+// Lower triangle of M = Q^T A Q (fine_grid.hpp:79-86), column-major, Q = [-Lag_inner^T ; I].
+__global__ void assemble_m_kernel(RbfSet r, int l, const double* __restrict__ pts,
+                                  const int64_t* __restrict__ ptr, const int32_t* __restrict__ idx,
+                                  const double* __restrict__ lag, const double* __restrict__ u,
+                                  const double* __restrict__ atl, const int64_t* __restrict__ mat_ptr,
+                                  double* __restrict__ mat) {
+  int d = blockIdx.y;
+  int64_t p0 = ptr[d];
+  int m = static_cast<int>(ptr[d + 1] - p0);
+  int n = m - l;
+  if (n <= 0) return;
+  int64_t uoff = static_cast<int64_t>(l) * (p0 - static_cast<int64_t>(d) * l);
+  const double* ud = u + uoff;
+  const double* ad = atl + static_cast<int64_t>(d) * l * l;
+  double* md = mat + mat_ptr[d];
+  int64_t total = static_cast<int64_t>(n) * n;
+  for (int64_t e = static_cast<int64_t>(blockIdx.x) * kT + threadIdx.x; e < total;
+       e += static_cast<int64_t>(gridDim.x) * kT) {
+    int i = static_cast<int>(e % n), j = static_cast<int>(e / n);
+    if (i < j) {
+      md[e] = 0.0;
+      continue;
+    }
+    int qi = idx[p0 + l + i], qj = idx[p0 + l + j];
+    double v = kernel_entry(r, pts, qi, qj);
+    if (l > 0) {
+      double qa[10], qb[10];
+      for (int a = 0; a < l; a++) {
+        qa[a] = -lag[static_cast<int64_t>(qi) * l + a];
+        qb[a] = -lag[static_cast<int64_t>(qj) * l + a];
+      }
+      for (int a = 0; a < l; a++) v += qa[a] * ud[a * n + j] + ud[a * n + i] * qb[a];
+      for (int a = 0; a < l; a++) {
+        double t = 0.0;
+        for (int b = 0; b < l; b++) t += ad[a * l + b] * qb[b];
+        v += qa[a] * t;
+      }
+    }
+    md[e] = v;
+  }
+}
+
+// One workgroup factors one domain: right-looking Cholesky with 32-wide panels;
+// the trailing update runs over 64x64 tiles of the lower triangle.
+__global__ void __launch_bounds__(kT)
+cholesky_kernel(const int64_t* __restrict__ ptr, int l, const int64_t* __restrict__ mat_ptr,
+                double* __restrict__ mat, int32_t* __restrict__ flag) {
+  __shared__ double dg[kNb][kNb + 1];
+  __shared__ double pi[kTile][kNb + 1];
+  __shared__ double pj[kTile][kNb + 1];
+  const int d = blockIdx.x;
+  const int n = static_cast<int>(ptr[d + 1] - ptr[d]) - l;
+  if (n <= 0) return;
+  double* a = mat + mat_ptr[d];
+  const int lda = n;
+  for (int k0 = 0; k0 < n; k0 += kNb) {
+    const int kb = min(kNb, n - k0);
+    for (int e = threadIdx.x; e < kb * kb; e += kT) {
+      int rr = e % kb, cc = e / kb;
+      dg[rr][cc] = a[static_cast<int64_t>(k0 + cc) * lda + k0 + rr];
+    }
+    __syncthreads();
+    for (int j = 0; j < kb; j++) {
+      if (threadIdx.x == 0) {
+        double v = dg[j][j];
+        if (!(v > 0.0)) {
+          atomicExch(flag, 1);
+          v = fabs(v) + 1e-300;
+        }
+        dg[j][j] = sqrt(v);
+      }
+      __syncthreads();
+      for (int rr = j + 1 + threadIdx.x; rr < kb; rr += kT) dg[rr][j] /= dg[j][j];
+      __syncthreads();
+      int w = kb - j - 1;
+      for (int e = threadIdx.x; e < w * w; e += kT) {
+        int rr = j + 1 + e / w, cc = j + 1 + e % w;
+        if (cc <= rr) dg[rr][cc] -= dg[rr][j] * dg[cc][j];
+      }
+      __syncthreads();
+    }
+    for (int e = threadIdx.x; e < kb * kb; e += kT) {
+      int rr = e % kb, cc = e / kb;
+      if (rr >= cc) a[static_cast<int64_t>(k0 + cc) * lda + k0 + rr] = dg[rr][cc];
+    }
+    // panel: rows below the diagonal block, X L11^T = A21
+    for (int i = k0 + kb + threadIdx.x; i < n; i += kT) {
+      double x[kNb];
+      for (int c = 0; c < kb; c++) {
+        double v = a[static_cast<int64_t>(k0 + c) * lda + i];
+        for (int t = 0; t < c; t++) v -= x[t] * dg[c][t];
+        x[c] = v / dg[c][c];
+        a[static_cast<int64_t>(k0 + c) * lda + i] = x[c];
+      }
+    }
+    __syncthreads();
+    // trailing update of the lower triangle: A22 -= L21 L21^T
+    const int base = k0 + kb;
+    const int rem = n - base;
+    if (rem <= 0) break;
+    const int nt = (rem + kTile - 1) / kTile;
+    const int ntiles = nt * (nt + 1) / 2;
+    const int tr = threadIdx.x % 16, tc = threadIdx.x / 16;
+    for (int t = 0; t < ntiles; t++) {
+      // t -> (I, J) with I >= J
+      int I = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) / 2.0);
+      while ((I + 1) * (I + 2) / 2 <= t) I++;
+      while (I * (I + 1) / 2 > t) I--;
+      int J = t - I * (I + 1) / 2;
+      int r0 = base + I * kTile, c0 = base + J * kTile;
+      for (int e = threadIdx.x; e < kTile * kb; e += kT) {
+        int rr = e % kTile, cc = e / kTile;
+        pi[rr][cc] = (r0 + rr < n) ? a[static_cast<int64_t>(k0 + cc) * lda + r0 + rr] : 0.0;
+        pj[rr][cc] = (c0 + rr < n) ? a[static_cast<int64_t>(k0 + cc) * lda + c0 + rr] : 0.0;
+      }
+      __syncthreads();
+      double acc[4][4] = {};
+      for (int c = 0; c < kb; c++) {
+        double x[4], y[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          x[q] = pi[tr * 4 + q][c];
+          y[q] = pj[tc * 4 + q][c];
+        }
+#pragma unroll
+        for (int p = 0; p < 4; p++)
+#pragma unroll
+          for (int q = 0; q < 4; q++) acc[p][q] += x[p] * y[q];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        int col = c0 + tc * 4 + q;
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+          int row = r0 + tr * 4 + p;
+          if (row < n && col < n && row >= col) a[static_cast<int64_t>(col) * lda + row] -= acc[p][q];
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = kT / 2; off > 0; off >>= 1) {
+    if (static_cast<int>(threadIdx.x) < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// One workgroup solves one domain (fine_grid.hpp:97-142, coarse_grid.hpp:105-131):
+// gather d, Q^T d, L L^T gamma = Q^T d, lambda = Q gamma, scatter owned entries;
+// the coarse grid also solves P_top c = d_top - A_top lambda.
+__global__ void __launch_bounds__(kT)
+ras_solve_kernel(RbfSet r, int degree, int l, const double* __restrict__ pts,
+                 const int64_t* __restrict__ ptr, const int32_t* __restrict__ idx,
+                 const uint8_t* __restrict__ inner, const double* __restrict__ lag,
+                 const int64_t* __restrict__ mat_ptr, const double* __restrict__ mat,
+                 const double* __restrict__ res, double* __restrict__ out, int64_t mu, int coarse,
+                 const double* __restrict__ pinv) {
+  extern __shared__ double sm[];
+  __shared__ double sh[kT];
+  __shared__ double lam_top[10];
+  const int d = blockIdx.x;
+  const int64_t p0 = ptr[d];
+  const int m = static_cast<int>(ptr[d + 1] - p0);
+  const int n = m - l;
+  double* vals = sm;      // [m]
+  double* y = sm + m;     // [n]
+  for (int u = threadIdx.x; u < m; u += kT) vals[u] = res[idx[p0 + u]];
+  __syncthreads();
+  if (n > 0) {
+    for (int i = threadIdx.x; i < n; i += kT) {
+      double v = vals[l + i];
+      int qi = idx[p0 + l + i];
+      for (int a = 0; a < l; a++) v -= lag[static_cast<int64_t>(qi) * l + a] * vals[a];
+      y[i] = v;
+    }
+    __syncthreads();
+    const double* L = mat + mat_ptr[d];
+    const int lda = n;
+    // forward: L y = qtd, 32-row blocks
+    for (int k0 = 0; k0 < n; k0 += kNb) {
+      int kb = min(kNb, n - k0);
+      if (threadIdx.x < 64) {  // one wavefront solves the diagonal block
+        for (int t = 0; t < kb; t++) {
+          if (static_cast<int>(threadIdx.x) == t) y[k0 + t] /= L[static_cast<int64_t>(k0 + t) * lda + k0 + t];
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+          int c = threadIdx.x;
+          if (c > t && c < kb) y[k0 + c] -= L[static_cast<int64_t>(k0 + t) * lda + k0 + c] * y[k0 + t];
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        }
+      }
+      __syncthreads();
+      for (int i = k0 + kb + threadIdx.x; i < n; i += kT) {
+        double v = y[i];
+        for (int t = 0; t < kb; t++) v -= L[static_cast<int64_t>(k0 + t) * lda + i] * y[k0 + t];
+        y[i] = v;
+      }
+      __syncthreads();
+    }
+    // backward: L^T x = y
+    for (int k0 = ((n - 1) / kNb) * kNb; k0 >= 0; k0 -= kNb) {
+      int kb = min(kNb, n - k0);
+      if (threadIdx.x < 64) {
+        for (int t = kb - 1; t >= 0; t--) {
+          if (static_cast<int>(threadIdx.x) == t) y[k0 + t] /= L[static_cast<int64_t>(k0 + t) * lda + k0 + t];
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+          int c = threadIdx.x;
+          if (c < t) y[k0 + c] -= L[static_cast<int64_t>(k0 + c) * lda + k0 + t] * y[k0 + t];
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        }
+      }
+      __syncthreads();
+      for (int i = threadIdx.x; i < k0; i += kT) {
+        double v = y[i];
+        for (int t = 0; t < kb; t++) v -= L[static_cast<int64_t>(i) * lda + k0 + t] * y[k0 + t];
+        y[i] = v;
+      }
+      __syncthreads();
+    }
+  }
+  // lambda_top[a] = sum_i Q[a][i] gamma_i = -sum_i lag[q_i][a] gamma_i
+  for (int a = 0; a < l; a++) {
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += kT) acc -= lag[static_cast<int64_t>(idx[p0 + l + i]) * l + a] * y[i];
+    double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) lam_top[a] = t;
+  }
+  __syncthreads();
+  for (int u = threadIdx.x; u < m; u += kT) {
+    if (coarse || inner[p0 + u]) {
+      double v = u < l ? lam_top[u] : y[u - l];
+      out[idx[p0 + u]] = v;
+    }
+  }
+  if (coarse && l > 0) {
+    // polynomial part: P_top c = d_top - A_top lambda
+    __shared__ double rhs_top[10];
+    for (int a = 0; a < l; a++) {
+      double acc = 0.0;
+      int pa = idx[p0 + a];
+      for (int u = threadIdx.x; u < m; u += kT) {
+        double lu = u < l ? lam_top[u] : y[u - l];
+        acc += kernel_entry(r, pts, pa, idx[p0 + u]) * lu;
+      }
+      double t = block_sum(acc, sh);
+      if (threadIdx.x == 0) rhs_top[a] = vals[a] - t;
+    }
+    __syncthreads();
+    if (threadIdx.x < static_cast<unsigned>(l)) {
+      double c = 0.0;
+      for (int b = 0; b < l; b++) c += pinv[threadIdx.x * l + b] * rhs_top[b];
+      out[mu + threadIdx.x] = c;
+    }
+  }
+}
+
+// ---- host helpers ----
+
+uint64_t splitmix64(uint64_t& s) {
+  uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+void host_monomials(int degree, const double* p, double* m) {
+  m[0] = 1.0;
+  if (degree >= 1) { m[1] = p[0]; m[2] = p[1]; m[3] = p[2]; }
+  if (degree >= 2) {
+    m[4] = p[0] * p[0]; m[5] = p[0] * p[1]; m[6] = p[0] * p[2];
+    m[7] = p[1] * p[1]; m[8] = p[1] * p[2]; m[9] = p[2] * p[2];
+  }
+}
+
+// Inverse by Gauss-Jordan with partial pivoting; returns the reciprocal
+// condition estimate 1 / (|A|_1 |A^-1|_1), 0 when singular.
+double invert(std::vector<double> a, int n, std::vector<double>& inv) {
+  inv.assign(static_cast<size_t>(n) * n, 0.0);
+  for (int i = 0; i < n; i++) inv[i * n + i] = 1.0;
+  double norm_a = 0.0;
+  for (int j = 0; j < n; j++) {
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s += std::abs(a[i * n + j]);
+    norm_a = std::max(norm_a, s);
+  }
+  for (int c = 0; c < n; c++) {
+    int p = c;
+    for (int i = c + 1; i < n; i++)
+      if (std::abs(a[i * n + c]) > std::abs(a[p * n + c])) p = i;
+    if (std::abs(a[p * n + c]) <= 1e-14 * norm_a) return 0.0;
+    if (p != c) {
+      for (int j = 0; j < n; j++) {
+        std::swap(a[p * n + j], a[c * n + j]);
+        std::swap(inv[p * n + j], inv[c * n + j]);
+      }
+    }
+    double piv = a[c * n + c];
+    for (int j = 0; j < n; j++) {
+      a[c * n + j] /= piv;
+      inv[c * n + j] /= piv;
+    }
+    for (int i = 0; i < n; i++) {
+      if (i == c) continue;
+      double f = a[i * n + c];
+      if (f == 0.0) continue;
+      for (int j = 0; j < n; j++) {
+        a[i * n + j] -= f * a[c * n + j];
+        inv[i * n + j] -= f * inv[c * n + j];
+      }
+    }
+  }
+  double norm_inv = 0.0;
+  for (int j = 0; j < n; j++) {
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s += std::abs(inv[i * n + j]);
+    norm_inv = std::max(norm_inv, s);
+  }
+  return 1.0 / (norm_a * norm_inv);
+}
+
+// UnisolventPointSet (polynomial/unisolvent_point_set.hpp:25-66): 100 random
+// trials of l distinct points, keep the best-conditioned. The random stream is
+// this library's own (splitmix64), so the chosen indices differ from polatory's.
+std::vector<int32_t> unisolvent_point_set(const std::vector<double>& pts, int degree) {
+  int l = poly_basis_size(degree);
+  int64_t n = static_cast<int64_t>(pts.size() / 3);
+  if (l == 0) return {};
+  if (n < l) throw_invalid("points.rows() must be greater than or equal to " + std::to_string(l));
+  uint64_t seed = 0x5eedu;
+  std::vector<int32_t> best;
+  double best_rcond = 0.0;
+  for (int trial = 0; trial < 100; trial++) {
+    std::vector<int32_t> set;
+    while (static_cast<int>(set.size()) < l) {
+      int32_t c = static_cast<int32_t>(splitmix64(seed) % static_cast<uint64_t>(n));
+      if (std::find(set.begin(), set.end(), c) == set.end()) set.push_back(c);
+    }
+    std::sort(set.begin(), set.end());
+    std::vector<double> p(static_cast<size_t>(l) * l), inv;
+    for (int i = 0; i < l; i++) {
+      double m[10];
+      host_monomials(degree, &pts[3 * static_cast<size_t>(set[i])], m);
+      for (int k = 0; k < l; k++) p[i * l + k] = m[k];
+    }
+    double rc = invert(p, l, inv);
+    if (rc > best_rcond) {
+      best_rcond = rc;
+      best = set;
+    }
+  }
+  if (best.empty()) throw_runtime("could not find a unisolvent set of points");
+  return best;
+}
+
+double round_half_to_even(double d) { return std::ceil((d - 0.5) / 2.0) + std::floor((d + 0.5) / 2.0); }
+
+struct Bbox3 {
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  void add(const double* p) {
+    for (int k = 0; k < 3; k++) {
+      lo[k] = std::min(lo[k], p[k]);
+      hi[k] = std::max(hi[k], p[k]);
+    }
+  }
+  std::array<int, 3> axes_by_width() const {
+    std::array<int, 3> ax{0, 1, 2};
+    std::stable_sort(ax.begin(), ax.end(), [&](int i, int j) { return hi[i] - lo[i] > hi[j] - lo[j]; });
+    return ax;
+  }
+};
+
+void sort_by_axes(std::vector<int32_t>& ids, const std::vector<double>& pts, const std::array<int, 3>& ax) {
+  std::stable_sort(ids.begin(), ids.end(), [&](int32_t a, int32_t b) {
+    for (int k : ax) {
+      double pa = pts[3 * static_cast<size_t>(a) + k], pb = pts[3 * static_cast<size_t>(b) + k];
+      if (pa != pb) return pa < pb;
+    }
+    return false;
+  });
+}
+
+}  // namespace
+
+// ---- DomainDivider ----
+
+DomainDivider::DomainDivider(const std::vector<double>& pts, const std::vector<int32_t>& point_idcs,
+                             const std::vector<int32_t>& poly_point_idcs)
+    : pts_(pts), point_idcs_(point_idcs), poly_idcs_(poly_point_idcs) {
+  constexpr double kOverlapQuota = 0.5;   // domain_divider.hpp:26
+  constexpr int64_t kMaxLeafSize = 1024;  // domain_divider.hpp:27
+  std::list<Domain> work;
+  Domain root;
+  root.idx = point_idcs;
+  root.inner.assign(point_idcs.size(), 1);
+  work.push_back(std::move(root));
+  for (auto it = work.begin(); it != work.end();) {
+    Domain& d = *it;
+    int64_t n = static_cast<int64_t>(d.idx.size());
+    if (n <= kMaxLeafSize) {
+      ++it;
+      continue;
+    }
+    // divide_domain (domain_divider.hpp:160-240): sort along the widest axes,
+    // split with an overlap of about half a leaf.
+    Bbox3 bb;
+    for (int32_t i : d.idx) bb.add(&pts_[3 * static_cast<size_t>(i)]);
+    std::vector<int32_t> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    auto ax = bb.axes_by_width();
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+      for (int k : ax) {
+        double pa = pts_[3 * static_cast<size_t>(d.idx[a]) + k], pb = pts_[3 * static_cast<size_t>(d.idx[b]) + k];
+        if (pa != pb) return pa < pb;
+      }
+      return false;
+    });
+    double q = kOverlapQuota * static_cast<double>(kMaxLeafSize) / static_cast<double>(n);
+    int64_t n_sub = static_cast<int64_t>(round_half_to_even((1.0 + q) / 2.0 * static_cast<double>(n)));
+    int64_t left_part = n - n_sub, right_part = n_sub;
+    int64_t mid = static_cast<int64_t>(round_half_to_even(static_cast<double>(left_part + right_part) / 2.0));
+    Domain left, right;
+    for (int64_t i = 0; i < right_part; i++) {
+      int32_t o = order[i];
+      left.idx.push_back(d.idx[o]);
+      left.inner.push_back(d.inner[o] && i < mid);
+    }
+    for (int64_t i = left_part; i < n; i++) {
+      int32_t o = order[i];
+      right.idx.push_back(d.idx[o]);
+      right.inner.push_back(d.inner[o] && i >= mid);
+    }
+    work.push_back(std::move(left));
+    work.push_back(std::move(right));
+    it = work.erase(it);
+  }
+  // merge_poly_points (domain.hpp:30-51)
+  for (auto& d : work) {
+    std::vector<int32_t> order(d.idx.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return d.idx[a] < d.idx[b]; });
+    Domain md;
+    std::vector<int32_t> sorted_idx;
+    std::vector<uint8_t> sorted_inner;
+    for (int32_t o : order) {
+      sorted_idx.push_back(d.idx[o]);
+      sorted_inner.push_back(d.inner[o]);
+    }
+    md.idx = poly_idcs_;
+    md.inner.assign(poly_idcs_.size(), 0);
+    for (size_t i = 0; i < poly_idcs_.size(); i++) {
+      auto jt = std::lower_bound(sorted_idx.begin(), sorted_idx.end(), poly_idcs_[i]);
+      if (jt != sorted_idx.end() && *jt == poly_idcs_[i]) {
+        size_t k = static_cast<size_t>(jt - sorted_idx.begin());
+        md.inner[i] = sorted_inner[k];
+        sorted_idx.erase(jt);
+        sorted_inner.erase(sorted_inner.begin() + static_cast<std::ptrdiff_t>(k));
+      }
+    }
+    md.idx.insert(md.idx.end(), sorted_idx.begin(), sorted_idx.end());
+    md.inner.insert(md.inner.end(), sorted_inner.begin(), sorted_inner.end());
+    domains_.push_back(std::move(md));
+  }
+}
+
+std::vector<int32_t> DomainDivider::choose_coarse_points(int64_t n_coarse) const {
+  // domain_divider.hpp:54-125: split the cluster of lowest level (largest bbox
+  // volume first) at its median until n_coarse clusters exist; take the point
+  // nearest each cluster's bbox centre.
+  struct Cluster {
+    int level;
+    std::vector<int32_t> pts;
+    double volume;
+    int32_t center;
+  };
+  auto init = [&](Cluster& c) {
+    Bbox3 bb;
+    for (int32_t i : c.pts) bb.add(&pts_[3 * static_cast<size_t>(i)]);
+    double cen[3], best = INFINITY;
+    c.volume = 1.0;
+    for (int k = 0; k < 3; k++) {
+      cen[k] = bb.lo[k] + (bb.hi[k] - bb.lo[k]) / 2.0;
+      c.volume *= bb.hi[k] - bb.lo[k];
+    }
+    c.center = c.pts.front();
+    for (int32_t i : c.pts) {
+      const double* p = &pts_[3 * static_cast<size_t>(i)];
+      double d2 = (p[0] - cen[0]) * (p[0] - cen[0]) + (p[1] - cen[1]) * (p[1] - cen[1]) + (p[2] - cen[2]) * (p[2] - cen[2]);
+      if (d2 < best) {
+        best = d2;
+        c.center = i;
+      }
+    }
+    sort_by_axes(c.pts, pts_, bb.axes_by_width());
+  };
+  auto cmp = [](const Cluster* a, const Cluster* b) {
+    if (a->level != b->level) return a->level > b->level;
+    return a->volume < b->volume;
+  };
+  std::vector<std::unique_ptr<Cluster>> store;
+  std::priority_queue<Cluster*, std::vector<Cluster*>, decltype(cmp)> q(cmp);
+  std::vector<int32_t> out(poly_idcs_);
+  auto root = std::make_unique<Cluster>();
+  root->level = 0;
+  for (int32_t i : point_idcs_)
+    if (std::find(poly_idcs_.begin(), poly_idcs_.end(), i) == poly_idcs_.end()) root->pts.push_back(i);
+  if (root->pts.empty()) return out;
+  init(*root);
+  q.push(root.get());
+  store.push_back(std::move(root));
+  int64_t current = 1;
+  while (current < n_coarse && !q.empty()) {
+    Cluster* c = q.top();
+    int64_t sz = static_cast<int64_t>(c->pts.size());
+    if (sz <= 1) break;
+    // median split (ties broken towards an even index, domain_divider.hpp:82-88)
+    int64_t mid = 0, bestd = INT64_MAX;
+    for (int64_t i = 0; i < sz; i++) {
+      int64_t dd = std::llabs(2 * i - sz);
+      if (dd < bestd || (dd == bestd && i % 2 == 0)) {
+        bestd = dd;
+        mid = i;
+      }
+    }
+    q.pop();
+    current -= 1;
+    auto left = std::make_unique<Cluster>();
+    auto right = std::make_unique<Cluster>();
+    left->level = right->level = c->level + 1;
+    left->pts.assign(c->pts.begin(), c->pts.begin() + mid);
+    right->pts.assign(c->pts.begin() + mid, c->pts.end());
+    for (auto* part : {left.get(), right.get()}) {
+      if (part->pts.empty()) continue;
+      init(*part);
+      q.push(part);
+      current += 1;
+    }
+    store.push_back(std::move(left));
+    store.push_back(std::move(right));
+  }
+  while (!q.empty()) {
+    out.push_back(q.top()->center);
+    q.pop();
+  }
+  return out;
+}
+
+// ---- RasPreconditioner ----
+
+RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<double>& points,
+                                     hipStream_t s, const RasOptions& opt)
+    : model_(model), s_(s), opt_(opt), mu_(static_cast<int64_t>(points.size() / 3)), l_(model.l()) {
+  auto t0 = std::chrono::steady_clock::now();
+  model_.validate();
+  constexpr double kFineToCoarseRatio = 10.0;  // ras_preconditioner.hpp:52
+  constexpr int64_t kNCoarsestPoints = 2048;    // ras_preconditioner.hpp:53
+  h_pts_ = points;
+  d_pts_.upload(h_pts_.data(), h_pts_.size(), s_);
+  n_levels_ = std::max(static_cast<int>(std::ceil(std::log(static_cast<double>(mu_) / kNCoarsestPoints) /
+                                                  std::log(kFineToCoarseRatio))), 0) + 1;
+  point_idcs_.resize(n_levels_);
+  std::vector<int32_t> poly_idcs;
+  if (l_ > 0) {
+    poly_idcs = unisolvent_point_set(h_pts_, model_.poly_degree);
+    std::vector<double> p(static_cast<size_t>(l_) * l_), inv;
+    for (int i = 0; i < l_; i++) {
+      double m[10];
+      host_monomials(model_.poly_degree, &h_pts_[3 * static_cast<size_t>(poly_idcs[i])], m);
+      for (int k = 0; k < l_; k++) p[i * l_ + k] = m[k];
+    }
+    if (invert(p, l_, inv) == 0.0) throw_invalid("the set of points is not unisolvent");
+    pinv_.upload(inv.data(), inv.size(), s_);
+    lag_.resize(static_cast<size_t>(mu_) * l_);
+    lagrange_kernel<<<ceil_div(mu_, kT), kT, 0, s_>>>(model_.poly_degree, l_, d_pts_.get(), mu_, pinv_.get(), lag_.get());
+    PAM_LAUNCH_CHECK();
+    auto& top = point_idcs_[n_levels_ - 1];
+    top = poly_idcs;
+    std::vector<uint8_t> is_poly(mu_, 0);
+    for (int32_t i : poly_idcs) is_poly[i] = 1;
+    for (int64_t i = 0; i < mu_; i++)
+      if (!is_poly[i]) top.push_back(static_cast<int32_t>(i));
+  } else {
+    auto& top = point_idcs_[n_levels_ - 1];
+    top.resize(mu_);
+    std::iota(top.begin(), top.end(), 0);
+    lag_.resize(1);
+    pinv_.resize(1);
+  }
+  // Domain geometry in the anisotropic frame when there is a single rbf
+  // (ras_preconditioner.hpp:105-113).
+  std::vector<double> a_pts = h_pts_;
+  if (model_.rbfs.size() == 1 && !is_identity(model_.rbfs[0].aniso)) {
+    const double* a = model_.rbfs[0].aniso;
+    for (int64_t i = 0; i < mu_; i++) {
+      const double* p = &h_pts_[3 * i];
+      for (int k = 0; k < 3; k++) a_pts[3 * i + k] = a[3 * k] * p[0] + a[3 * k + 1] * p[1] + a[3 * k + 2] * p[2];
+    }
+  }
+  flag_.resize(1);
+  flag_.zero(s_);
+  fine_.clear();
+  fine_.resize(n_levels_);
+  h_domains_.assign(n_levels_, {});
+  for (int level = n_levels_ - 1; level >= 1; level--) {
+    DomainDivider div(a_pts, point_idcs_[level], poly_idcs);
+    double finest = std::log(static_cast<double>(mu_)) / std::log(kFineToCoarseRatio);
+    double coarsest = std::log(static_cast<double>(kNCoarsestPoints)) / std::log(kFineToCoarseRatio);
+    auto n_coarse = static_cast<int64_t>(std::pow(
+        kFineToCoarseRatio, coarsest + (level - 1) * (finest - coarsest) / (n_levels_ - 1)));
+    point_idcs_[level - 1] = div.choose_coarse_points(n_coarse);
+    h_domains_[level] = std::move(div.domains());
+    setup_batch(fine_[level], h_domains_[level], false);
+  }
+  {
+    Domain cd;
+    cd.idx = point_idcs_[0];
+    cd.inner.assign(cd.idx.size(), 1);
+    h_domains_[0] = {cd};
+    setup_batch(coarse_, h_domains_[0], true);
+  }
+  int32_t flag = 0;
+  PAM_HIP(hipMemcpyAsync(&flag, flag_.get(), sizeof flag, hipMemcpyDeviceToHost, s_));
+  PAM_HIP(hipStreamSynchronize(s_));
+  if (flag) throw_runtime("a subdomain matrix Q^T A Q is not positive definite");
+  d_point_idcs_.resize(n_levels_);
+  for (int level = 0; level < n_levels_; level++)
+    d_point_idcs_[level].upload(point_idcs_[level].data(), point_idcs_[level].size(), s_);
+  res_.resize(mu_);
+  w_.resize(mu_ + l_);
+  total_.resize(mu_ + l_);
+  dot_.resize(10);
+  if (n_levels_ > 1 && l_ > 0) {
+    p_orth_.resize(static_cast<size_t>(mu_) * l_);
+    dev_poly_matrix(model_.poly_degree, d_pts_.get(), mu_, p_orth_.get(), s_);
+    dev_orthonormalize_cols(p_orth_.get(), mu_, l_, ws_, s_);
+    // ap = A P (ras_preconditioner.hpp:173-188)
+    ap_.resize(static_cast<size_t>(mu_) * l_);
+    Bbox3 bb;
+    for (int64_t i = 0; i < mu_; i++) bb.add(&h_pts_[3 * i]);
+    SymmetricEvaluator ev(model_, bb.lo, bb.hi, INFINITY, s_);
+    if (opt_.use_level_cfg) ev.set_fmm_config(opt_.level_cfg);
+    ev.set_points(d_pts_.get(), mu_);
+    DevBuf<double> wcol(static_cast<size_t>(mu_ + l_));
+    for (int k = 0; k < l_; k++) {
+      dev_fill(0.0, wcol.get(), mu_ + l_, s_);
+      dev_copy(p_orth_.get() + static_cast<int64_t>(k) * mu_, wcol.get(), mu_, s_);
+      ev.set_weights(wcol.get());
+      ev.evaluate(ap_.get() + static_cast<int64_t>(k) * mu_);
+    }
+    PAM_HIP(hipStreamSynchronize(s_));
+  }
+  PAM_HIP(hipStreamSynchronize(s_));
+  setup_s_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void RasPreconditioner::setup_batch(DomainBatch& b, const std::vector<Domain>& doms, bool coarse) {
+  (void)coarse;
+  b.n_dom = static_cast<int>(doms.size());
+  b.h_ptr.assign(b.n_dom + 1, 0);
+  std::vector<int64_t> mptr(b.n_dom + 1, 0);
+  b.max_m = 0;
+  for (int d = 0; d < b.n_dom; d++) {
+    int64_t m = static_cast<int64_t>(doms[d].idx.size());
+    if (l_ > 0 && m < l_) throw_runtime("a domain holds fewer points than the polynomial basis");
+    b.h_ptr[d + 1] = b.h_ptr[d] + m;
+    int64_t n = m - l_;
+    mptr[d + 1] = mptr[d] + n * n;
+    b.max_m = std::max<int>(b.max_m, static_cast<int>(m));
+  }
+  std::vector<int32_t> idx(b.h_ptr.back());
+  std::vector<uint8_t> inner(b.h_ptr.back());
+  for (int d = 0; d < b.n_dom; d++) {
+    std::copy(doms[d].idx.begin(), doms[d].idx.end(), idx.begin() + b.h_ptr[d]);
+    std::copy(doms[d].inner.begin(), doms[d].inner.end(), inner.begin() + b.h_ptr[d]);
+  }
+  b.idx.upload(idx.data(), idx.size(), s_);
+  b.inner.upload(inner.data(), inner.size(), s_);
+  b.ptr.upload(b.h_ptr.data(), b.h_ptr.size(), s_);
+  b.mat_ptr.upload(mptr.data(), mptr.size(), s_);
+  b.mat.resize(static_cast<size_t>(std::max<int64_t>(mptr.back(), 1)));
+  RbfSet rs = make_rbf_set(model_);
+  DevBuf<double> u(static_cast<size_t>(std::max<int64_t>(l_ * (b.h_ptr.back() - static_cast<int64_t>(b.n_dom) * l_), 1)));
+  DevBuf<double> atl(static_cast<size_t>(std::max(b.n_dom * l_ * l_, 1)));
+  if (l_ > 0) {
+    assemble_u_kernel<<<dim3(16, b.n_dom), kT, 0, s_>>>(rs, l_, d_pts_.get(), b.ptr.get(), b.idx.get(), u.get(), atl.get());
+    PAM_LAUNCH_CHECK();
+  }
+  assemble_m_kernel<<<dim3(64, b.n_dom), kT, 0, s_>>>(rs, l_, d_pts_.get(), b.ptr.get(), b.idx.get(), lag_.get(),
+                                                       u.get(), atl.get(), b.mat_ptr.get(), b.mat.get());
+  PAM_LAUNCH_CHECK();
+  cholesky_kernel<<<b.n_dom, kT, 0, s_>>>(b.ptr.get(), l_, b.mat_ptr.get(), b.mat.get(), flag_.get());
+  PAM_LAUNCH_CHECK();
+  PAM_HIP(hipStreamSynchronize(s_));  // u, atl and host vectors go out of scope
+}
+
+int64_t RasPreconditioner::factor_bytes() const {
+  int64_t b = static_cast<int64_t>(coarse_.mat.bytes());
+  for (const auto& f : fine_) b += static_cast<int64_t>(f.mat.bytes());
+  return b;
+}
+
+void RasPreconditioner::solve(int level, const double* d_res, double* d_w) {
+  // ras_preconditioner.hpp:280-297
+  dev_fill(0.0, d_w, mu_ + l_, s_);
+  DomainBatch& b = level == 0 ? coarse_ : fine_[level];
+  if (b.n_dom == 0) return;
+  size_t shmem = sizeof(double) * 2 * static_cast<size_t>(b.max_m);
+  if (shmem > 96 * 1024) throw_runtime("subdomain too large for the solve kernel");
+  RbfSet rs = make_rbf_set(model_);
+  ras_solve_kernel<<<b.n_dom, kT, shmem, s_>>>(rs, model_.poly_degree, l_, d_pts_.get(), b.ptr.get(), b.idx.get(),
+                                                b.inner.get(), lag_.get(), b.mat_ptr.get(), b.mat.get(), d_res, d_w,
+                                                mu_, level == 0 ? 1 : 0, pinv_.get());
+  PAM_LAUNCH_CHECK();
+}
+
+Evaluator& RasPreconditioner::evaluator(int src, int trg) {
+  auto key = std::make_pair(src, trg);
+  auto it = evaluators_.find(key);
+  if (it != evaluators_.end()) return *it->second;
+  auto lp = [&](int level) -> const double* {
+    auto jt = level_pts_.find(level);
+    if (jt == level_pts_.end()) {
+      DevBuf<double>& buf = level_pts_[level];
+      buf.resize(point_idcs_[level].size() * 3);
+      dev_gather_points(d_pts_.get(), d_point_idcs_[level].get(), static_cast<int64_t>(point_idcs_[level].size()),
+                        buf.get(), s_);
+      return buf.get();
+    }
+    return jt->second.get();
+  };
+  Bbox3 bb;
+  for (int64_t i = 0; i < mu_; i++) bb.add(&h_pts_[3 * i]);
+  auto ev = std::make_unique<Evaluator>(model_, bb.lo, bb.hi, INFINITY, s_);
+  if (opt_.use_level_cfg) ev->set_fmm_config(opt_.level_cfg);
+  ev->set_source_points(lp(src), static_cast<int64_t>(point_idcs_[src].size()));
+  ev->set_target_points(lp(trg), static_cast<int64_t>(point_idcs_[trg].size()));
+  Evaluator& ref = *ev;
+  evaluators_[key] = std::move(ev);
+  return ref;
+}
+
+void RasPreconditioner::update_residuals(int src, int trg, const double* d_w, double* d_res) {
+  // ras_preconditioner.hpp:299-328
+  int64_t ns = static_cast<int64_t>(point_idcs_[src].size());
+  int64_t nt = static_cast<int64_t>(point_idcs_[trg].size());
+  sw_.resize(static_cast<size_t>(ns + l_));
+  dev_gather(d_w, d_point_idcs_[src].get(), ns, sw_.get(), s_);
+  if (l_ > 0) dev_copy(d_w + mu_, sw_.get() + ns, l_, s_);
+  Evaluator& ev = evaluator(src, trg);
+  ev.set_weights(sw_.get());
+  tmp_.resize(static_cast<size_t>(nt));
+  ev.evaluate(tmp_.get());
+  dev_scatter_add(tmp_.get(), d_point_idcs_[trg].get(), nt, -1.0, d_res, s_);
+}
+
+void RasPreconditioner::orthogonalize(double* d_w, double* d_res) {
+  // ras_preconditioner.hpp:271-278
+  if (l_ == 0) return;
+  double dot[10];
+  dev_multi_dot(p_orth_.get(), mu_, l_, d_w, mu_, dot, ws_, s_);
+  PAM_HIP(hipMemcpyAsync(dot_.get(), dot, sizeof(double) * l_, hipMemcpyHostToDevice, s_));
+  dev_gemv_update(d_w, p_orth_.get(), mu_, l_, dot_.get(), mu_, -1.0, s_);
+  dev_gemv_update(d_res, ap_.get(), mu_, l_, dot_.get(), mu_, 1.0, s_);
+  PAM_HIP(hipStreamSynchronize(s_));  // dot is a host temporary
+}
+
+void RasPreconditioner::apply(const double* d_v, double* d_out) {
+  // ras_preconditioner.hpp:191-250
+  dev_copy(d_v, res_.get(), mu_, s_);
+  if (n_levels_ == 1) {
+    solve(0, res_.get(), d_out);
+    return;
+  }
+  const int n = n_levels_;
+  double* tot = total_.get();
+  dev_fill(0.0, tot, mu_ + l_, s_);
+  solve(0, res_.get(), w_.get());
+  update_residuals(0, n - 1, w_.get(), res_.get());
+  dev_axpy(1.0, w_.get(), tot, mu_ + l_, s_);
+  for (int level = 1; level < n - 1; level++) {
+    solve(level, res_.get(), w_.get());
+    update_residuals(level, n - 1, w_.get(), res_.get());
+    dev_axpy(1.0, w_.get(), tot, mu_ + l_, s_);
+    orthogonalize(tot, res_.get());
+    solve(0, res_.get(), w_.get());
+    update_residuals(0, n - 1, w_.get(), res_.get());
+    dev_axpy(1.0, w_.get(), tot, mu_ + l_, s_);
+  }
+  for (int level = n - 1; level >= 1; level--) {
+    solve(level, res_.get(), w_.get());
+    update_residuals(level, level - 1, w_.get(), res_.get());
+    dev_axpy(1.0, w_.get(), tot, mu_ + l_, s_);
+    orthogonalize(tot, res_.get());
+    solve(0, res_.get(), w_.get());
+    if (level > 1) update_residuals(0, level - 1, w_.get(), res_.get());
+    dev_axpy(1.0, w_.get(), tot, mu_ + l_, s_);
+  }
+  dev_copy(tot, d_out, mu_ + l_, s_);
+}
+
+}  // namespace pam

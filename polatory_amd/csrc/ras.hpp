@@ -1,0 +1,94 @@
+// Multilevel restricted additive Schwarz preconditioner on the device.
+// Restates preconditioner::RasPreconditioner (ras_preconditioner.hpp:56-328)
+// with DomainDivider (domain_divider.hpp:30-269), FineGrid (fine_grid.hpp:59-170)
+// and CoarseGrid (coarse_grid.hpp:40-131). Domain division runs on the host;
+// the kernel matrices Q^T A Q of all subdomains are assembled, Cholesky-factored
+// (FP64) and solved in batched device kernels, and the factors stay in HBM
+// (polatory keeps them in a temporary file, binary_cache.hpp).
+#pragma once
+
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "interpolation.hpp"
+#include "krylov.hpp"
+
+namespace pam {
+
+struct Domain {
+  std::vector<int32_t> idx;     // point indices (polynomial points first after merging)
+  std::vector<uint8_t> inner;   // 1 where this domain owns the point's solution
+};
+
+// DomainDivider (domain_divider.hpp:30-269), point multiplicity 1 (no gradient data).
+class DomainDivider {
+ public:
+  DomainDivider(const std::vector<double>& pts, const std::vector<int32_t>& point_idcs,
+                const std::vector<int32_t>& poly_point_idcs);
+  std::vector<int32_t> choose_coarse_points(int64_t n_coarse) const;
+  std::vector<Domain>& domains() { return domains_; }
+
+ private:
+  const std::vector<double>& pts_;
+  std::vector<int32_t> point_idcs_, poly_idcs_;
+  std::vector<Domain> domains_;
+};
+
+// Batched subdomain factors of one level (or the coarse grid as one domain).
+struct DomainBatch {
+  int n_dom = 0;
+  int max_m = 0;
+  std::vector<int64_t> h_ptr;            // [n_dom+1] offsets into idx
+  DevBuf<int32_t> idx;
+  DevBuf<uint8_t> inner;
+  DevBuf<int64_t> ptr, mat_ptr;          // [n_dom+1]
+  DevBuf<double> mat;                    // lower Cholesky factors, column-major
+};
+
+struct RasOptions {
+  FmmConfig level_cfg;      // inter-level residual updates (default accuracy)
+  bool use_level_cfg = false;
+};
+
+class RasPreconditioner : public LinearOperator {
+ public:
+  RasPreconditioner(const Model& model, const std::vector<double>& points, hipStream_t s,
+                    const RasOptions& opt);
+  void apply(const double* d_v, double* d_out) override;
+  int64_t size() const override { return mu_ + l_; }
+  int n_levels() const { return n_levels_; }
+  const std::vector<std::vector<int32_t>>& level_points() const { return point_idcs_; }
+  const std::vector<Domain>& level_domains(int level) const { return h_domains_.at(level); }
+  double setup_seconds() const { return setup_s_; }
+  int64_t factor_bytes() const;
+
+ private:
+  void setup_batch(DomainBatch& b, const std::vector<Domain>& doms, bool coarse);
+  void solve(int level, const double* d_res, double* d_w);
+  void update_residuals(int src, int trg, const double* d_w, double* d_res);
+  void orthogonalize(double* d_w, double* d_res);
+  Evaluator& evaluator(int src, int trg);
+
+  Model model_;
+  hipStream_t s_;
+  RasOptions opt_;
+  int64_t mu_;
+  int l_;
+  int n_levels_ = 1;
+  std::vector<double> h_pts_;
+  DevBuf<double> d_pts_, lag_, p_orth_, ap_, pinv_;
+  std::vector<std::vector<int32_t>> point_idcs_;
+  std::vector<DevBuf<int32_t>> d_point_idcs_;
+  std::vector<std::vector<Domain>> h_domains_;
+  std::vector<DomainBatch> fine_;  // index = level (1..n-1)
+  DomainBatch coarse_;
+  std::map<std::pair<int, int>, std::unique_ptr<Evaluator>> evaluators_;
+  std::map<int, DevBuf<double>> level_pts_;
+  DevBuf<double> res_, w_, total_, tmp_, sw_, dot_;
+  DevBuf<int32_t> flag_;
+  Workspace1 ws_;
+  double setup_s_ = 0.0;
+};
+
+}  // namespace pam

@@ -1,0 +1,236 @@
+// Fit operator, residual evaluator and the FGMRES solver (see solver.hpp).
+#include "solver.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <limits>
+
+#include "device_api.hpp"
+
+namespace pam {
+
+Operator::Operator(const Model& model, const double* d_pts, int64_t n, const FmmConfig& cfg, hipStream_t s)
+    : model_(model), s_(s), mu_(n), l_(model.l()), d_pts_(d_pts) {
+  model_.validate();
+  for (const auto& r : model_.rbfs) {
+    fmm_.push_back(std::make_unique<FmmEvaluator>(r, nullptr, nullptr, true, s));
+    fmm_.back()->set_config(cfg);
+    fmm_.back()->set_source_points(d_pts, n, true);
+  }
+}
+
+void Operator::apply(const double* d_x, double* d_y) {
+  // y[0:mu] = nugget w + sum_rbf A w + P c; y[mu:] = P^T w   (operator.hpp:56-82)
+  tmp_.resize(static_cast<size_t>(mu_));
+  for (size_t i = 0; i < fmm_.size(); i++) {
+    fmm_[i]->set_weights(d_x, mu_, true);
+    fmm_[i]->evaluate(i == 0 ? d_y : tmp_.get(), mu_, true);
+    if (i > 0) dev_axpy(1.0, tmp_.get(), d_y, mu_, s_);
+  }
+  if (model_.nugget != 0.0) dev_axpy(model_.nugget, d_x, d_y, mu_, s_);
+  if (l_ > 0) {
+    launch_poly_eval(model_.poly_degree, d_pts_, mu_, d_x + mu_, d_y, s_);
+    dev_poly_transpose_apply(model_.poly_degree, d_pts_, mu_, d_x, d_y + mu_, ws_, s_);
+  }
+}
+
+ResidualEvaluator::ResidualEvaluator(const Model& model, const double* d_pts, int64_t n,
+                                     const FmmConfig& full_cfg, hipStream_t s)
+    : model_(model), s_(s), mu_(n), l_(model.l()), d_pts_(d_pts), full_(model, d_pts, n, full_cfg, s) {}
+
+void ResidualEvaluator::set_values(const std::vector<double>& values) {
+  // residual_evaluator.hpp:121-151: up to 1024 sites with nonzero values.
+  // polatory shuffles with a default-seeded mt19937; this uses an evenly
+  // strided pick among the nonzero sites (deterministic, different sites).
+  constexpr int64_t kDirect = 1024;
+  values_.upload(values.data(), values.size(), s_);
+  std::vector<int32_t> nz, zero;
+  for (int64_t i = 0; i < mu_; i++) (values[i] != 0.0 ? nz : zero).push_back(static_cast<int32_t>(i));
+  std::vector<int32_t> order = nz;
+  order.insert(order.end(), zero.begin(), zero.end());
+  int64_t k = std::min<int64_t>(mu_, kDirect);
+  sample_.clear();
+  int64_t nnz = static_cast<int64_t>(nz.size());
+  if (nnz >= k) {
+    for (int64_t i = 0; i < k; i++) sample_.push_back(nz[(i * nnz) / k]);
+  } else {
+    sample_ = nz;
+    for (int64_t i = 0; static_cast<int64_t>(sample_.size()) < k; i++) sample_.push_back(zero[i]);
+  }
+  std::sort(sample_.begin(), sample_.end());
+  d_sample_.upload(sample_.data(), sample_.size(), s_);
+  int64_t ns = static_cast<int64_t>(sample_.size());
+  sample_pts_.resize(static_cast<size_t>(ns) * 3);
+  dev_gather_points(d_pts_, d_sample_.get(), ns, sample_pts_.get(), s_);
+  sample_vals_.resize(static_cast<size_t>(ns));
+  dev_gather(values_.get(), d_sample_.get(), ns, sample_vals_.get(), s_);
+  direct_.clear();
+  for (const auto& r : model_.rbfs) {
+    direct_.push_back(std::make_unique<FmmEvaluator>(r, nullptr, nullptr, false, s_));
+    FmmConfig c;
+    c.mode = kModeDirect;
+    direct_.back()->set_config(c);
+    direct_.back()->set_source_points(d_pts_, mu_, true);
+    direct_.back()->set_target_points(sample_pts_.get(), ns, true);
+  }
+  PAM_HIP(hipStreamSynchronize(s_));
+}
+
+void ResidualEvaluator::full_residual(const double* d_weights, double* d_r) {
+  y_.resize(static_cast<size_t>(mu_ + l_));
+  full_.apply(d_weights, y_.get());
+  dev_copy(values_.get(), d_r, mu_, s_);
+  dev_axpy(-1.0, y_.get(), d_r, mu_, s_);
+}
+
+Convergence ResidualEvaluator::converged(const double* d_weights, double tolerance) {
+  // residual_evaluator.hpp:56-108
+  Convergence c;
+  int64_t ns = static_cast<int64_t>(sample_.size());
+  bool exact = ns == mu_;
+  {
+    y_.resize(static_cast<size_t>(std::max<int64_t>(ns, mu_ + l_)));
+    tmp_.resize(static_cast<size_t>(ns));
+    for (size_t i = 0; i < direct_.size(); i++) {
+      direct_[i]->set_weights(d_weights, mu_, true);
+      direct_[i]->evaluate(i == 0 ? y_.get() : tmp_.get(), ns, true);
+      if (i > 0) dev_axpy(1.0, tmp_.get(), y_.get(), ns, s_);
+    }
+    if (l_ > 0) launch_poly_eval(model_.poly_degree, sample_pts_.get(), ns, d_weights + mu_, y_.get(), s_);
+    if (model_.nugget != 0.0) {
+      dev_gather(d_weights, d_sample_.get(), ns, tmp_.get(), s_);
+      dev_axpy(model_.nugget, tmp_.get(), y_.get(), ns, s_);
+    }
+    double r = dev_max_abs_diff(y_.get(), sample_vals_.get(), ns, ws_, s_);
+    c.residual = r;
+    c.exact = exact;
+    if (r > tolerance) return c;
+    if (exact) {
+      c.converged = true;
+      return c;
+    }
+  }
+  tmp_.resize(static_cast<size_t>(mu_));
+  full_residual(d_weights, tmp_.get());
+  double r = dev_max_abs_diff(tmp_.get(), nullptr, mu_, ws_, s_);
+  c.residual = r;
+  c.exact = true;
+  c.converged = r <= tolerance;
+  return c;
+}
+
+Solver::Solver(const Model& model, const std::vector<double>& points, const FitOptions& opt, hipStream_t s)
+    : model_(model), opt_(opt), s_(s), mu_(static_cast<int64_t>(points.size() / 3)), l_(model.l()) {
+  auto t0 = std::chrono::steady_clock::now();
+  model_.validate();
+  if (mu_ < l_) throw_invalid("points.rows() must be greater than or equal to " + std::to_string(l_));
+  d_pts_.upload(points.data(), points.size(), s_);
+  op_ = std::make_unique<Operator>(model_, d_pts_.get(), mu_, opt_.op_cfg, s_);
+  res_ = std::make_unique<ResidualEvaluator>(model_, d_pts_.get(), mu_, opt_.res_cfg, s_);
+  if (opt_.use_preconditioner) {
+    RasOptions ro;
+    ro.level_cfg = opt_.level_cfg;
+    ro.use_level_cfg = opt_.use_level_cfg;
+    pc_ = std::make_unique<RasPreconditioner>(model_, points, s_, ro);
+  }
+  if (l_ > 0) {
+    p_orth_.resize(static_cast<size_t>(mu_) * l_);
+    dev_poly_matrix(model_.poly_degree, d_pts_.get(), mu_, p_orth_.get(), s_);
+    dev_orthonormalize_cols(p_orth_.get(), mu_, l_, ws_, s_);
+  }
+  PAM_HIP(hipStreamSynchronize(s_));
+  setup_s_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+std::vector<double> Solver::solve(const std::vector<double>& values, double tolerance, int max_iter,
+                                  const std::vector<double>* initial, FitStats* stats) {
+  // solver.hpp:76-142, with the operator in mixed precision and FGMRES
+  // restarted on the FP64 residual when the inner solve reaches refine_rtol.
+  auto t0 = std::chrono::steady_clock::now();
+  if (static_cast<int64_t>(values.size()) != mu_) throw_invalid("values.rows() must be equal to " + std::to_string(mu_));
+  if (!(tolerance > 0.0)) throw_invalid("tolerance must be positive");
+  if (max_iter < 0) throw_invalid("max_iter must be nonnegative");
+  const int64_t m = mu_ + l_;
+  DevBuf<double> x(m), xb(m), rhs(m), dot(10);
+  dev_fill(0.0, x.get(), m, s_);
+  if (initial) {
+    if (static_cast<int64_t>(initial->size()) != m) throw_invalid("initial weights have the wrong size");
+    PAM_HIP(hipMemcpyAsync(x.get(), initial->data(), sizeof(double) * m, hipMemcpyHostToDevice, s_));
+    if (l_ > 0) {  // orthogonalize the warm start against P (solver.hpp:85-93)
+      double h[10];
+      dev_multi_dot(p_orth_.get(), mu_, l_, x.get(), mu_, h, ws_, s_);
+      PAM_HIP(hipMemcpyAsync(dot.get(), h, sizeof(double) * l_, hipMemcpyHostToDevice, s_));
+      dev_gemv_update(x.get(), p_orth_.get(), mu_, l_, dot.get(), mu_, -1.0, s_);
+    }
+    PAM_HIP(hipStreamSynchronize(s_));
+  }
+  res_->set_values(values);
+  FitStats st;
+  st.setup_seconds = setup_s_;
+  if (pc_) {
+    st.precond_setup_seconds = pc_->setup_seconds();
+    st.n_levels = pc_->n_levels();
+    st.factor_bytes = pc_->factor_bytes();
+  }
+  int total_iter = 0;
+  DevBuf<double> r(mu_);
+  bool first = true;
+  while (true) {
+    // right-hand side of this refinement cycle
+    if (first && !initial) {
+      PAM_HIP(hipMemcpyAsync(rhs.get(), values.data(), sizeof(double) * mu_, hipMemcpyHostToDevice, s_));
+      dev_fill(0.0, rhs.get() + mu_, l_, s_);
+      dev_fill(0.0, xb.get(), m, s_);
+    } else {
+      dev_copy(x.get(), xb.get(), m, s_);
+      res_->full_residual(xb.get(), rhs.get());
+      if (l_ > 0) {  // second block: 0 - P^T w
+        dev_poly_transpose_apply(model_.poly_degree, d_pts_.get(), mu_, xb.get(), rhs.get() + mu_, ws_, s_);
+        dev_scal(-1.0, rhs.get() + mu_, l_, s_);
+      }
+    }
+    first = false;
+    Fgmres solver(*op_, rhs.get(), m, std::max(max_iter - total_iter, 0), s_);
+    solver.set_right_preconditioner(pc_.get());
+    solver.setup();
+    if (solver.relative_residual() == 0.0) {
+      dev_copy(xb.get(), x.get(), m, s_);
+      break;
+    }
+    bool done = false, restart = false;
+    while (true) {
+      solver.solution_vector(x.get());
+      dev_axpy(1.0, xb.get(), x.get(), m, s_);
+      Convergence c = res_->converged(x.get(), tolerance);
+      st.history.push_back(c.residual);
+      st.residual = c.residual;
+      if (opt_.verbose)
+        std::fprintf(stderr, "%8d %s%12.6e  (inner rel %.3e)\n", total_iter + solver.iteration_count(),
+                     c.exact ? " " : "~", c.residual, solver.relative_residual());
+      if (c.converged) {
+        done = true;
+        break;
+      }
+      if (total_iter + solver.iteration_count() >= max_iter)
+        throw_runtime("reached the maximum number of iterations");
+      if (solver.iteration_count() > 0 && solver.relative_residual() < opt_.refine_rtol) {
+        restart = true;
+        break;
+      }
+      solver.iterate_process();
+    }
+    total_iter += solver.iteration_count();
+    if (done) break;
+    if (restart) st.restarts++;
+  }
+  st.iterations = total_iter;
+  std::vector<double> out(m);
+  PAM_HIP(hipMemcpyAsync(out.data(), x.get(), sizeof(double) * m, hipMemcpyDeviceToHost, s_));
+  PAM_HIP(hipStreamSynchronize(s_));
+  st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (stats) *stats = st;
+  return out;
+}
+
+}  // namespace pam

@@ -1,0 +1,108 @@
+// Fit path: the saddle-point operator, FP64 residual evaluation and the
+// FGMRES+RAS solver with mixed-precision iterative refinement.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "interpolation.hpp"
+#include "krylov.hpp"
+#include "ras.hpp"
+
+namespace pam {
+
+// interpolation/operator.hpp:56-82: [[A + nugget I, P], [P^T, 0]] x.
+class Operator : public LinearOperator {
+ public:
+  Operator(const Model& model, const double* d_pts, int64_t n, const FmmConfig& cfg, hipStream_t s);
+  void apply(const double* d_x, double* d_y) override;
+  int64_t size() const override { return mu_ + l_; }
+
+ private:
+  Model model_;
+  hipStream_t s_;
+  int64_t mu_;
+  int l_;
+  const double* d_pts_;
+  std::vector<std::unique_ptr<FmmEvaluator>> fmm_;
+  DevBuf<double> tmp_;
+  Workspace1 ws_;
+};
+
+struct Convergence {
+  bool converged = false;
+  double residual = 0.0;
+  bool exact = false;
+};
+
+// interpolation/residual_evaluator.hpp:35-151: L-inf residual at the data
+// sites, first on <= 1024 sampled sites with nonzero values (FP64 direct),
+// then at all sites in FP64.
+class ResidualEvaluator {
+ public:
+  ResidualEvaluator(const Model& model, const double* d_pts, int64_t n, const FmmConfig& full_cfg,
+                    hipStream_t s);
+  void set_values(const std::vector<double>& values);
+  Convergence converged(const double* d_weights, double tolerance);
+  // r = values - (A + nugget) w - P c at all sites (FP64), device output.
+  void full_residual(const double* d_weights, double* d_r);
+
+ private:
+  Model model_;
+  hipStream_t s_;
+  int64_t mu_;
+  int l_;
+  const double* d_pts_;
+  Operator full_;
+  std::vector<int32_t> sample_;
+  DevBuf<int32_t> d_sample_;
+  DevBuf<double> sample_pts_, sample_vals_, values_, tmp_, y_;
+  std::vector<std::unique_ptr<FmmEvaluator>> direct_;
+  Workspace1 ws_;
+};
+
+struct FitOptions {
+  FmmConfig op_cfg;       // inner operator (mixed-precision FMM by default)
+  FmmConfig res_cfg;      // FP64 residual recomputation
+  FmmConfig level_cfg;    // RAS inter-level updates
+  bool use_level_cfg = false;
+  bool use_preconditioner = true;
+  double refine_rtol = 1e-4;  // inner relative residual that triggers a refinement restart
+  int verbose = 0;
+};
+
+struct FitStats {
+  int iterations = 0;
+  int restarts = 0;
+  double residual = 0.0;
+  double seconds = 0.0;
+  double setup_seconds = 0.0;
+  double precond_setup_seconds = 0.0;
+  int n_levels = 0;
+  int64_t factor_bytes = 0;
+  std::vector<double> history;  // residual per iteration (sampled or exact)
+};
+
+// interpolation/solver.hpp:34-142 + fitter.hpp:21-26.
+class Solver {
+ public:
+  Solver(const Model& model, const std::vector<double>& points, const FitOptions& opt, hipStream_t s);
+  std::vector<double> solve(const std::vector<double>& values, double tolerance, int max_iter,
+                            const std::vector<double>* initial, FitStats* stats);
+  RasPreconditioner* preconditioner() { return pc_.get(); }
+
+ private:
+  Model model_;
+  FitOptions opt_;
+  hipStream_t s_;
+  int64_t mu_;
+  int l_;
+  DevBuf<double> d_pts_, p_orth_;
+  std::unique_ptr<Operator> op_;
+  std::unique_ptr<ResidualEvaluator> res_;
+  std::unique_ptr<RasPreconditioner> pc_;
+  Workspace1 ws_;
+  double setup_s_ = 0.0;
+};
+
+}  // namespace pam

@@ -1,0 +1,218 @@
+"""Host-side mirror of polatory's Model / Interpolant API over the C ABI.
+
+Model          include/polatory/model.hpp:32-183
+Interpolant    include/polatory/interpolant.hpp:34-367 (fit, evaluate, weights)
+RasPreconditioner (test seam) include/polatory/preconditioner/ras_preconditioner.hpp:56-250
+fgmres_dense   (test seam) src/krylov/fgmres.cpp, gmres.cpp
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import _capi
+from .fmm import FmmConfig, _points, _ptr
+from .rbf import Rbf
+
+POLY_DEGREE_MIN = -2  # Model::kMinRequiredPolyDegree
+
+
+class Model:
+    def __init__(self, rbfs, poly_degree=POLY_DEGREE_MIN, nugget=0.0):
+        if isinstance(rbfs, Rbf):
+            rbfs = [rbfs]
+        if len(rbfs) == 0:
+            raise ValueError("rbfs must not be empty")
+        self.rbfs = list(rbfs)
+        self._arr = (_capi.PamRbf * len(rbfs))(*[r.to_c() for r in rbfs])
+        h = ctypes.c_void_p()
+        _capi.call("pam_model_create", ctypes.cast(self._arr, ctypes.c_void_p), len(rbfs),
+                   int(poly_degree), float(nugget), ctypes.byref(h))
+        self._h = h
+        self._nugget = float(nugget)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _capi.load().pam_model_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def nugget(self):
+        return self._nugget
+
+    def poly_degree(self):
+        d = ctypes.c_int()
+        _capi.call("pam_model_poly_degree", self._h, ctypes.byref(d))
+        return d.value
+
+    def poly_basis_size(self):
+        l = ctypes.c_int()
+        _capi.call("pam_model_poly_basis_size", self._h, ctypes.byref(l))
+        return l.value
+
+    def cpd_order(self):
+        return max(r.cpd_order for r in self.rbfs)
+
+
+class FitOptions:
+    """Configuration of the device fit (no reference counterpart; defaults follow
+    pam_fit_options_default)."""
+
+    def __init__(self, op=None, residual=None, level=None, use_preconditioner=True,
+                 refine_rtol=None, verbose=False):
+        d = _capi.PamFitOptions()
+        _capi.call("pam_fit_options_default", ctypes.byref(d))
+        self._c = d
+        if op is not None:
+            d.op = op.to_c()
+        if residual is not None:
+            d.residual = residual.to_c()
+        if level is not None:
+            d.level = level.to_c()
+            d.use_level_config = 1
+        d.use_preconditioner = 1 if use_preconditioner else 0
+        if refine_rtol is not None:
+            d.refine_rtol = refine_rtol
+        d.verbose = 1 if verbose else 0
+
+
+class Interpolant:
+    def __init__(self, model: Model):
+        self.model = model
+        h = ctypes.c_void_p()
+        _capi.call("pam_interpolant_create", model._h, ctypes.byref(h))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _capi.load().pam_interpolant_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def fit(self, points, values, tolerance, max_iter=100, accuracy=math.inf, initial=None,
+            options: FitOptions = None):
+        p = _points(points)
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        if v.shape[0] != p.shape[0]:
+            raise ValueError(f"values.rows() must be equal to {p.shape[0]}")
+        opt = None if options is None else ctypes.byref(options._c)
+        _capi.call("pam_interpolant_fit", self._h, _ptr(p), p.shape[0], _ptr(v), float(tolerance),
+                   int(max_iter), float(accuracy), opt, None if initial is None else initial._h)
+
+    def evaluate(self, points, accuracy=math.inf):
+        p = _points(points)
+        out = np.empty(p.shape[0])
+        _capi.call("pam_interpolant_evaluate", self._h, _ptr(p), p.shape[0], float(accuracy), _ptr(out))
+        return out
+
+    def set_eval_config(self, config: FmmConfig = None):
+        if config is None:
+            _capi.call("pam_interpolant_set_eval_config", self._h, None)
+        else:
+            c = config.to_c()
+            _capi.call("pam_interpolant_set_eval_config", self._h, ctypes.byref(c))
+
+    def num_centers(self):
+        n = ctypes.c_int64()
+        _capi.call("pam_interpolant_num_centers", self._h, ctypes.byref(n))
+        return n.value
+
+    def centers(self):
+        out = np.empty((self.num_centers(), 3))
+        _capi.call("pam_interpolant_centers", self._h, _ptr(out))
+        return out
+
+    def weights(self):
+        out = np.empty(self.num_centers() + self.model.poly_basis_size())
+        _capi.call("pam_interpolant_weights", self._h, _ptr(out))
+        return out
+
+    def set_fitted(self, centers, weights):
+        c = _points(centers)
+        w = np.ascontiguousarray(weights, dtype=np.float64).reshape(-1)
+        _capi.call("pam_interpolant_set_fitted", self._h, _ptr(c), c.shape[0], _ptr(w))
+
+    def fit_stats(self):
+        s = _capi.PamFitStats()
+        _capi.call("pam_interpolant_fit_stats", self._h, ctypes.byref(s))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def residual_history(self):
+        n = ctypes.c_int64()
+        _capi.call("pam_interpolant_residual_history", self._h, None, ctypes.byref(n))
+        out = np.empty(n.value)
+        _capi.call("pam_interpolant_residual_history", self._h, _ptr(out), ctypes.byref(n))
+        return out
+
+
+class RasPreconditioner:
+    """Device multilevel RAS preconditioner (exposed for tests)."""
+
+    def __init__(self, model: Model, points):
+        self.model = model
+        p = _points(points)
+        self.n = p.shape[0]
+        h = ctypes.c_void_p()
+        _capi.call("pam_ras_create", model._h, _ptr(p), p.shape[0], ctypes.byref(h))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _capi.load().pam_ras_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def n_levels(self):
+        n = ctypes.c_int()
+        _capi.call("pam_ras_num_levels", self._h, ctypes.byref(n))
+        return n.value
+
+    def level_points(self, level):
+        n = ctypes.c_int64()
+        _capi.call("pam_ras_level_points", self._h, level, None, ctypes.byref(n))
+        out = np.empty(n.value, dtype=np.int32)
+        _capi.call("pam_ras_level_points", self._h, level, _ptr(out), ctypes.byref(n))
+        return out
+
+    def domains(self, level):
+        nd = ctypes.c_int()
+        _capi.call("pam_ras_num_domains", self._h, level, ctypes.byref(nd))
+        res = []
+        for d in range(nd.value):
+            n = ctypes.c_int64()
+            _capi.call("pam_ras_domain", self._h, level, d, None, None, ctypes.byref(n))
+            idx = np.empty(n.value, dtype=np.int32)
+            inner = np.empty(n.value, dtype=np.uint8)
+            _capi.call("pam_ras_domain", self._h, level, d, _ptr(idx), _ptr(inner), ctypes.byref(n))
+            res.append((idx, inner.astype(bool)))
+        return res
+
+    def apply(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        out = np.empty_like(v)
+        _capi.call("pam_ras_apply", self._h, _ptr(v), _ptr(out))
+        return out
+
+
+def fgmres_dense(a, rhs, n_iter, m=None, x0=None):
+    """Device FGMRES on a dense operator; returns (x, |g_j| history, iterations)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    n = a.shape[0]
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    mm = None if m is None else np.ascontiguousarray(m, dtype=np.float64)
+    x0a = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
+    x = np.empty(n)
+    hist = np.empty(n_iter + 1)
+    it = ctypes.c_int()
+    _capi.call("pam_fgmres_dense", _ptr(a), None if mm is None else _ptr(mm), n, _ptr(rhs),
+               None if x0a is None else _ptr(x0a), int(n_iter), _ptr(x), _ptr(hist), ctypes.byref(it))
+    return x, hist, it.value

@@ -1,0 +1,195 @@
+"""GPU parity of the fit path: device FGMRES against the oracle's FGMRES, RAS
+partition invariants and coarse-grid reproduction (test/preconditioner/*),
+interpolant evaluation against the FP64 direct oracle, and fits whose final
+residual, recomputed by the oracle in FP64 on the CPU, meets the tolerance
+(test/interpolation/test_fitter.cpp:51-63)."""
+import sys
+
+import numpy as np
+import pytest
+
+import polatory_amd as pa
+
+pytestmark = pytest.mark.gpu
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def orc_rbfs(orc, model):
+    out = []
+    for r in model.rbfs:
+        kind = {"bh3": orc.BH3, "th3": orc.TH3, "exp": orc.EXP}[r.short_name]
+        out.append(orc.make_rbf(kind, r.parameters(), r.anisotropy()))
+    return out
+
+
+def oracle_residual(orc, model, pts, values, weights):
+    """max |values - (A + nugget) w - P c| in FP64 on the CPU."""
+    y = orc.direct_evaluate(orc_rbfs(orc, model), model.poly_degree(), pts, weights, pts)
+    y += model.nugget() * weights[:len(pts)]
+    return np.abs(values - y).max()
+
+
+def test_fgmres_matches_oracle(orc):
+    log("fgmres")
+    rng = np.random.default_rng(21)
+    n = 100
+    b = rng.normal(size=(n, n))
+    a = b + b.T + 0.1 * n * np.eye(n)
+    m = np.linalg.inv(a) + rng.normal(size=(n, n)) * 1e-2
+    rhs = rng.normal(size=n)
+    for k, mm in ((30, m), (60, None)):
+        x, hist, it = pa.fgmres_dense(a, rhs, k, m=mm)
+        xo, ho, ito = orc.fgmres_dense(a, rhs, k, m=mm)
+        assert it == ito == k
+        np.testing.assert_allclose(hist, ho, rtol=1e-9, atol=1e-12 * np.linalg.norm(rhs))
+        np.testing.assert_allclose(x, xo, rtol=1e-9, atol=1e-9 * np.abs(xo).max())
+        true = np.linalg.norm(rhs - a @ x)
+        assert abs(hist[-1] - true) <= 1e-10 * np.linalg.norm(rhs)
+
+
+def sample_values(p):
+    return np.sin(3 * p[:, 0]) * np.cos(2 * p[:, 1]) + 0.5 * p[:, 2] ** 2
+
+
+def test_domain_divider_invariants():
+    log("ras invariants")
+    rng = np.random.default_rng(22)
+    pts = rng.uniform(0, 1, (12000, 3))
+    model = pa.Model(pa.Biharmonic3D(), 1)
+    ras = pa.RasPreconditioner(model, pts)
+    L = ras.n_levels()
+    assert L == 2  # ceil(log10(12000 / 2048)) + 1
+    poly = ras.level_points(L - 1)[:4]
+    for level in range(1, L):
+        lp = ras.level_points(level)
+        doms = ras.domains(level)
+        owner = np.zeros(len(pts), dtype=np.int64)
+        for idx, inner in doms:
+            np.testing.assert_array_equal(idx[:4], poly)  # polynomial points first
+            assert len(np.unique(idx)) == len(idx)  # no duplicates
+            assert len(idx) <= 1024 + 4
+            owner[idx[inner]] += 1
+        # every point of the level is inner in exactly one domain
+        assert np.all(owner[lp] == 1)
+        assert owner.sum() == len(lp)
+    coarse = ras.level_points(0)
+    np.testing.assert_array_equal(coarse[:4], poly)
+    # n_coarse = (Index) 10^(log10 2048) may truncate to 2047 (ras_preconditioner.hpp:130-132)
+    assert 2047 + 4 <= len(coarse) <= 2048 + 4
+
+
+@pytest.mark.parametrize("kind", ["bh3", "th3", "exp"])
+def test_coarse_grid_reproduces_data(orc, kind):
+    # test/preconditioner/test_coarse_grid.cpp:93-107: the single-level solve
+    # interpolates the data (relative error < 1e-8 against the direct evaluator).
+    log("coarse", kind)
+    rng = np.random.default_rng(23)
+    pts = rng.uniform(0, 1, (1000, 3))
+    rbf = {"bh3": pa.Biharmonic3D(), "th3": pa.Triharmonic3D(), "exp": pa.CovExponential([1.0, 0.5])}[kind]
+    model = pa.Model(rbf, pa.POLY_DEGREE_MIN if kind != "exp" else 0, nugget=0.0)
+    ras = pa.RasPreconditioner(model, pts)
+    assert ras.n_levels() == 1
+    l = model.poly_basis_size()
+    v = np.concatenate([sample_values(pts), np.zeros(l)])
+    w = ras.apply(v)
+    y = orc.direct_evaluate(orc_rbfs(orc, model), model.poly_degree(), pts, w, pts)
+    err = np.abs(y - v[:1000]).max() / np.abs(v[:1000]).max()
+    log(f"  rel err {err:.2e}")
+    assert err < 1e-8
+
+
+@pytest.mark.parametrize("kind,deg", [("bh3", 1), ("th3", 1), ("exp", 0)])
+def test_interpolant_evaluate_matches_direct(orc, kind, deg):
+    log("evaluate", kind)
+    rng = np.random.default_rng(24)
+    rbf = {"bh3": pa.Biharmonic3D(), "th3": pa.Triharmonic3D(), "exp": pa.CovExponential([1.0, 2.0])}[kind]
+    rbf.set_anisotropy(np.diag([1.0, 1.0, 1.0 / 0.3]) if kind == "exp" else np.eye(3))
+    model = pa.Model(rbf, deg)
+    l = model.poly_basis_size()
+    centers = rng.uniform(0, 1, (30000, 3))
+    w = np.concatenate([rng.normal(size=30000), rng.normal(size=l)])
+    it = pa.Interpolant(model)
+    it.set_fitted(centers, w)
+    trg = rng.uniform(0, 1, (5000, 3))
+    y = it.evaluate(trg)
+    ref = orc.direct_evaluate(orc_rbfs(orc, model), deg, centers, w, trg)
+    err = np.abs(y - ref).max() / np.abs(ref).max()
+    log(f"  rel err {err:.2e}")
+    assert err < 5e-6  # default accuracy: mixed precision, order 6
+
+
+@pytest.mark.parametrize("n,kind,nugget", [(3000, "bh3", 0.0), (10000, "bh3", 0.0),
+                                           (6000, "th3", 0.0), (5000, "exp", 0.01)])
+def test_fit_reaches_tolerance(orc, n, kind, nugget):
+    log("fit", n, kind, nugget)
+    rng = np.random.default_rng(25)
+    pts = rng.uniform(0, 1, (n, 3))
+    vals = sample_values(pts)
+    rbf = {"bh3": pa.Biharmonic3D(), "th3": pa.Triharmonic3D(), "exp": pa.CovExponential([1.0, 0.5])}[kind]
+    model = pa.Model(rbf, 1 if kind != "exp" else 0, nugget=nugget)
+    tol = 1e-4
+    it = pa.Interpolant(model)
+    it.fit(pts, vals, tol, max_iter=100)
+    st = it.fit_stats()
+    w = it.weights()
+    res = oracle_residual(orc, model, pts, vals, w)
+    log(f"  iterations {st['iterations']} restarts {st['restarts']} levels {st['n_levels']} "
+        f"residual(device) {st['residual']:.2e} residual(oracle FP64) {res:.2e} {st['seconds']:.2f}s")
+    assert res <= tol
+    # P^T w = 0 (the second block row of the saddle-point system)
+    P = np.concatenate([np.ones((n, 1)), pts], 1)[:, :model.poly_basis_size()]
+    assert np.abs(P.T @ w[:n]).max() < 1e-6 * np.abs(w[:n]).sum()
+
+
+def test_fit_with_mixed_fmm_operator_refines_to_tolerance(orc):
+    # The inner operator is the FP32/MFMA FMM; FGMRES restarts on the FP64
+    # residual (iterative refinement) until the absolute tolerance holds.
+    log("fit mixed")
+    rng = np.random.default_rng(26)
+    n = 20000
+    pts = rng.uniform(0, 1, (n, 3))
+    vals = sample_values(pts)
+    model = pa.Model(pa.Biharmonic3D(), 1)
+    opt = pa.FitOptions(op=pa.FmmConfig("mixed", 7, 3, "fmm"),
+                        residual=pa.FmmConfig("fp64", 10, -1, "direct"), verbose=True)
+    it = pa.Interpolant(model)
+    it.fit(pts, vals, 1e-5, max_iter=100, options=opt)
+    st = it.fit_stats()
+    res = oracle_residual(orc, model, pts, vals, it.weights())
+    log(f"  iterations {st['iterations']} restarts {st['restarts']} residual {res:.2e}")
+    assert res <= 1e-5
+
+
+def test_warm_start_and_max_iter(orc):
+    log("warm start")
+    rng = np.random.default_rng(27)
+    n = 4000
+    pts = rng.uniform(0, 1, (n, 3))
+    vals = sample_values(pts)
+    model = pa.Model(pa.Biharmonic3D(), 1)
+    first = pa.Interpolant(model)
+    first.fit(pts, vals, 1e-6)
+    second = pa.Interpolant(model)
+    second.fit(pts, vals, 1e-6, initial=first)
+    assert second.fit_stats()["iterations"] <= 1
+    third = pa.Interpolant(model)
+    with pytest.raises(RuntimeError, match="maximum number of iterations"):
+        third.fit(pts, vals, 1e-12, max_iter=2)
+
+
+def test_argument_errors():
+    model = pa.Model(pa.Biharmonic3D(), 1)
+    it = pa.Interpolant(model)
+    pts = np.random.default_rng(0).uniform(0, 1, (10, 3))
+    with pytest.raises(ValueError):
+        it.fit(pts, np.zeros(9), 1e-3)
+    with pytest.raises(ValueError, match="tolerance"):
+        it.fit(pts, np.zeros(10), 0.0)
+    with pytest.raises(RuntimeError, match="not fitted"):
+        it.evaluate(pts)
+    with pytest.raises(ValueError, match="greater than or equal"):
+        it.fit(pts[:3], np.zeros(3), 1e-3)
