@@ -49,7 +49,21 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}{r.stderr}")
     os.replace(tmp, LIB)
+    build_cpp_tests()
     return LIB
+
+
+def build_cpp_tests():
+    """Compiles tests/cpp/*.cpp (C++ API mirror users) against the library."""
+    tdir = os.path.join(ROOT, "tests", "cpp")
+    for src in sorted(glob.glob(os.path.join(tdir, "*.cpp"))):
+        exe = src[:-4]
+        cmd = ["g++", "-O2", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}",
+               f"-I{os.path.join(HERE, 'include')}", src, "-o", exe, f"-L{HERE}", "-lpolatory_amd",
+               "-Wl,-rpath,$ORIGIN/../../polatory_amd"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed for {src}:\n{r.stdout}{r.stderr}")
 
 
 if __name__ == "__main__":

@@ -1,0 +1,398 @@
+// Host C++ API mirroring polatory's public interface for the RBF fit/evaluate
+// hot path, implemented over the C ABI of libpolatory_amd.so (polatory_amd.h).
+//
+// Names, argument meaning and error behaviour follow the reference:
+//   polatory::rbf::{Biharmonic3D, Triharmonic3D, CovExponential}<3>
+//       include/polatory/rbf/polyharmonic_odd.hpp, cov_exponential.hpp, rbf_base.hpp
+//   polatory::Model<3>            include/polatory/model.hpp:32-183
+//   polatory::Interpolant<3>      include/polatory/interpolant.hpp:34-367
+//   polatory::fmm::FmmGenericEvaluatorBase<3>, make_fmm_evaluator,
+//   polatory::fmm::FmmGenericSymmetricEvaluatorBase<3>, make_fmm_symmetric_evaluator
+//       include/polatory/fmm/fmm_evaluator.hpp:17-106, fmm_symmetric_evaluator.hpp:15-86
+// std::invalid_argument / std::runtime_error are thrown where polatory throws them.
+// Eigen is not required: VecX and Points are small owning containers with the
+// same row-major layout (types.hpp:9-22).
+#pragma once
+
+#include <polatory_amd.h>
+
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace polatory {
+
+using Index = std::int64_t;
+
+namespace detail {
+inline void check(int rc) {
+  if (rc == PAM_OK) return;
+  char buf[1024];
+  pam_last_error(buf, sizeof buf);
+  if (rc == PAM_ERR_INVALID_ARGUMENT) throw std::invalid_argument(buf);
+  throw std::runtime_error(buf);
+}
+}  // namespace detail
+
+// Column vector of doubles (Eigen::VectorXd stand-in).
+class VecX {
+ public:
+  VecX() = default;
+  explicit VecX(Index n, double v = 0.0) : d_(static_cast<size_t>(n), v) {}
+  explicit VecX(std::vector<double> v) : d_(std::move(v)) {}
+  Index rows() const { return static_cast<Index>(d_.size()); }
+  Index size() const { return rows(); }
+  double& operator()(Index i) { return d_[static_cast<size_t>(i)]; }
+  double operator()(Index i) const { return d_[static_cast<size_t>(i)]; }
+  double* data() { return d_.data(); }
+  const double* data() const { return d_.data(); }
+  static VecX Zero(Index n) { return VecX(n); }
+
+ private:
+  std::vector<double> d_;
+};
+
+namespace geometry {
+
+// Row-major n x Dim point matrix (geometry::Points<Dim>).
+template <int Dim>
+class Points {
+ public:
+  Points() = default;
+  Points(Index rows, Index cols) : n_(rows), d_(static_cast<size_t>(rows) * Dim) {
+    if (cols != Dim) throw std::invalid_argument("Points has a fixed number of columns");
+  }
+  Index rows() const { return n_; }
+  static constexpr Index cols() { return Dim; }
+  double& operator()(Index i, Index j) { return d_[static_cast<size_t>(i * Dim + j)]; }
+  double operator()(Index i, Index j) const { return d_[static_cast<size_t>(i * Dim + j)]; }
+  double* data() { return d_.data(); }
+  const double* data() const { return d_.data(); }
+
+ private:
+  Index n_ = 0;
+  std::vector<double> d_;
+};
+
+using Points3 = Points<3>;
+
+template <int Dim>
+struct Bbox {
+  std::array<double, Dim> min_, max_;
+  Bbox() {
+    min_.fill(std::numeric_limits<double>::infinity());
+    max_.fill(-std::numeric_limits<double>::infinity());
+  }
+  Bbox(std::array<double, Dim> lo, std::array<double, Dim> hi) : min_(lo), max_(hi) {}
+  static Bbox from_points(const Points<Dim>& p) {
+    Bbox b;
+    for (Index i = 0; i < p.rows(); i++)
+      for (int k = 0; k < Dim; k++) {
+        b.min_[k] = std::min(b.min_[k], p(i, k));
+        b.max_[k] = std::max(b.max_[k], p(i, k));
+      }
+    return b;
+  }
+  Bbox convex_hull(const Bbox& o) const {
+    Bbox b;
+    for (int k = 0; k < Dim; k++) {
+      b.min_[k] = std::min(min_[k], o.min_[k]);
+      b.max_[k] = std::max(max_[k], o.max_[k]);
+    }
+    return b;
+  }
+  const std::array<double, Dim>& min() const { return min_; }
+  const std::array<double, Dim>& max() const { return max_; }
+};
+
+using Bbox3 = Bbox<3>;
+
+}  // namespace geometry
+
+template <int Dim>
+using Mat = std::array<double, Dim * Dim>;  // row-major
+
+namespace rbf {
+
+// Rbf handle (rbf/rbf.hpp:11-118) for the kernels of this library.
+template <int Dim>
+class Rbf {
+  static_assert(Dim == 3, "only 3-D kernels are provided");
+
+ public:
+  const Mat<Dim>& anisotropy() const { return aniso_; }
+  void set_anisotropy(const Mat<Dim>& a) {
+    double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+                 a[2] * (a[3] * a[7] - a[4] * a[6]);
+    if (!(det > 0.0)) throw std::invalid_argument("aniso must have a positive determinant");
+    aniso_ = a;
+  }
+  const std::vector<double>& parameters() const { return params_; }
+  int cpd_order() const { return kind_ == PAM_RBF_BH3 ? 1 : (kind_ == PAM_RBF_TH3 ? 2 : 0); }
+  std::string short_name() const { return kind_ == PAM_RBF_BH3 ? "bh3" : (kind_ == PAM_RBF_TH3 ? "th3" : "exp"); }
+  pam_rbf to_c() const {
+    pam_rbf r{};
+    r.kind = kind_;
+    r.params[0] = params_.at(0);
+    r.params[1] = params_.at(1);
+    for (int i = 0; i < 9; i++) r.aniso[i] = aniso_[i];
+    return r;
+  }
+
+ protected:
+  Rbf(int kind, std::vector<double> params) : kind_(kind), params_(std::move(params)) {
+    aniso_ = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  }
+
+ private:
+  int kind_;
+  std::vector<double> params_;
+  Mat<Dim> aniso_;
+};
+
+// polyharmonic_odd.hpp:87-99: {} -> {1, 0}, {slope} -> {slope, 0}
+inline std::vector<double> polyharmonic_params(std::vector<double> p) {
+  if (p.empty()) return {1.0, 0.0};
+  if (p.size() == 1) return {p[0], 0.0};
+  if (p.size() != 2) throw std::invalid_argument("params.size() must be 2");
+  return p;
+}
+
+template <int Dim>
+class Biharmonic3D : public Rbf<Dim> {
+ public:
+  explicit Biharmonic3D(const std::vector<double>& p = {}) : Rbf<Dim>(PAM_RBF_BH3, polyharmonic_params(p)) {}
+};
+
+template <int Dim>
+class Triharmonic3D : public Rbf<Dim> {
+ public:
+  explicit Triharmonic3D(const std::vector<double>& p = {}) : Rbf<Dim>(PAM_RBF_TH3, polyharmonic_params(p)) {}
+};
+
+template <int Dim>
+class CovExponential : public Rbf<Dim> {
+ public:
+  explicit CovExponential(const std::vector<double>& p) : Rbf<Dim>(PAM_RBF_EXP, check(p)) {}
+
+ private:
+  static std::vector<double> check(const std::vector<double>& p) {
+    if (p.size() != 2) throw std::invalid_argument("params.size() must be 2");
+    return p;
+  }
+};
+
+}  // namespace rbf
+
+template <int Dim>
+class Model {
+ public:
+  static inline const int kMinRequiredPolyDegree = PAM_POLY_DEGREE_MIN;
+
+  explicit Model(const rbf::Rbf<Dim>& r, int poly_degree = kMinRequiredPolyDegree)
+      : Model(std::vector<rbf::Rbf<Dim>>{r}, poly_degree) {}
+  explicit Model(std::vector<rbf::Rbf<Dim>> rbfs, int poly_degree = kMinRequiredPolyDegree)
+      : rbfs_(std::move(rbfs)), degree_(poly_degree) {
+    rebuild();
+  }
+  Model(const Model& o) : rbfs_(o.rbfs_), degree_(o.degree_), nugget_(o.nugget_) { rebuild(); }
+  Model& operator=(const Model&) = delete;
+  ~Model() { pam_model_destroy(h_); }
+
+  double nugget() const { return nugget_; }
+  void set_nugget(double nugget) {
+    if (!(nugget >= 0.0)) throw std::invalid_argument("nugget must be non-negative");
+    nugget_ = nugget;
+    rebuild();
+  }
+  int poly_degree() const {
+    int d = 0;
+    detail::check(pam_model_poly_degree(h_, &d));
+    return d;
+  }
+  Index poly_basis_size() const {
+    int l = 0;
+    detail::check(pam_model_poly_basis_size(h_, &l));
+    return l;
+  }
+  const std::vector<rbf::Rbf<Dim>>& rbfs() const { return rbfs_; }
+  const pam_model* handle() const { return h_; }
+
+ private:
+  void rebuild() {
+    if (h_) pam_model_destroy(h_);
+    h_ = nullptr;
+    std::vector<pam_rbf> c;
+    for (const auto& r : rbfs_) c.push_back(r.to_c());
+    detail::check(pam_model_create(c.data(), static_cast<int>(c.size()), degree_, nugget_, &h_));
+  }
+
+  std::vector<rbf::Rbf<Dim>> rbfs_;
+  int degree_;
+  double nugget_ = 0.0;
+  pam_model* h_ = nullptr;
+};
+
+template <int Dim>
+class Interpolant {
+  static constexpr double kInfinity = std::numeric_limits<double>::infinity();
+  using Points = geometry::Points<Dim>;
+
+ public:
+  explicit Interpolant(const Model<Dim>& model) : model_(model) {
+    detail::check(pam_interpolant_create(model_.handle(), &h_));
+  }
+  ~Interpolant() { pam_interpolant_destroy(h_); }
+  Interpolant(const Interpolant&) = delete;
+  Interpolant& operator=(const Interpolant&) = delete;
+
+  // interpolant.hpp:77-81
+  void fit(const Points& points, const VecX& values, double tolerance, int max_iter = 100,
+           double accuracy = kInfinity, const Interpolant* initial = nullptr) {
+    if (values.rows() != points.rows())
+      throw std::invalid_argument("values.rows() must be equal to " + std::to_string(points.rows()));
+    detail::check(pam_interpolant_fit(h_, points.data(), points.rows(), values.data(), tolerance,
+                                      max_iter, accuracy, nullptr, initial ? initial->h_ : nullptr));
+  }
+
+  // interpolant.hpp:54-56
+  VecX evaluate(const Points& points, double accuracy = kInfinity) {
+    VecX out(points.rows());
+    detail::check(pam_interpolant_evaluate(h_, points.data(), points.rows(), accuracy, out.data()));
+    return out;
+  }
+
+  Points centers() const {
+    Index n = 0;
+    detail::check(pam_interpolant_num_centers(h_, &n));
+    Points p(n, Dim);
+    detail::check(pam_interpolant_centers(h_, p.data()));
+    return p;
+  }
+
+  VecX weights() const {
+    Index n = 0;
+    detail::check(pam_interpolant_num_centers(h_, &n));
+    VecX w(n + model_.poly_basis_size());
+    detail::check(pam_interpolant_weights(h_, w.data()));
+    return w;
+  }
+
+  const Model<Dim>& model() const { return model_; }
+
+ private:
+  Model<Dim> model_;
+  pam_interpolant* h_ = nullptr;
+};
+
+namespace fmm {
+
+template <int Dim>
+class FmmGenericEvaluatorBase {
+ public:
+  virtual ~FmmGenericEvaluatorBase() = default;
+  virtual VecX evaluate() const = 0;
+  virtual void set_accuracy(double accuracy) = 0;
+  virtual void set_source_points(const geometry::Points<Dim>& points) = 0;
+  virtual void set_target_points(const geometry::Points<Dim>& points) = 0;
+  virtual void set_weights(const VecX& weights) = 0;
+};
+
+template <int Dim>
+class FmmGenericSymmetricEvaluatorBase {
+ public:
+  virtual ~FmmGenericSymmetricEvaluatorBase() = default;
+  virtual VecX evaluate() const = 0;
+  virtual void set_accuracy(double accuracy) = 0;
+  virtual void set_points(const geometry::Points<Dim>& points) = 0;
+  virtual void set_weights(const VecX& weights) = 0;
+};
+
+template <int Dim>
+using FmmGenericEvaluatorPtr = std::unique_ptr<FmmGenericEvaluatorBase<Dim>>;
+template <int Dim>
+using FmmGenericSymmetricEvaluatorPtr = std::unique_ptr<FmmGenericSymmetricEvaluatorBase<Dim>>;
+
+namespace detail {
+
+template <int Dim>
+class DeviceEvaluator final : public FmmGenericEvaluatorBase<Dim> {
+ public:
+  DeviceEvaluator(const rbf::Rbf<Dim>& r, const geometry::Bbox<Dim>& b) {
+    pam_rbf c = r.to_c();
+    polatory::detail::check(pam_fmm_evaluator_create(&c, b.min().data(), b.max().data(), &h_));
+  }
+  ~DeviceEvaluator() override { pam_fmm_evaluator_destroy(h_); }
+  VecX evaluate() const override {
+    VecX out(n_trg_);
+    polatory::detail::check(pam_fmm_evaluator_evaluate(h_, out.data(), n_trg_));
+    return out;
+  }
+  void set_accuracy(double a) override { polatory::detail::check(pam_fmm_evaluator_set_accuracy(h_, a)); }
+  void set_source_points(const geometry::Points<Dim>& p) override {
+    polatory::detail::check(pam_fmm_evaluator_set_source_points(h_, p.data(), p.rows()));
+  }
+  void set_target_points(const geometry::Points<Dim>& p) override {
+    n_trg_ = p.rows();
+    polatory::detail::check(pam_fmm_evaluator_set_target_points(h_, p.data(), p.rows()));
+  }
+  void set_weights(const VecX& w) override {
+    polatory::detail::check(pam_fmm_evaluator_set_weights(h_, w.data(), w.rows()));
+  }
+
+ private:
+  pam_fmm_evaluator* h_ = nullptr;
+  Index n_trg_ = 0;
+};
+
+template <int Dim>
+class DeviceSymmetricEvaluator final : public FmmGenericSymmetricEvaluatorBase<Dim> {
+ public:
+  DeviceSymmetricEvaluator(const rbf::Rbf<Dim>& r, const geometry::Bbox<Dim>& b) {
+    pam_rbf c = r.to_c();
+    polatory::detail::check(pam_fmm_symmetric_evaluator_create(&c, b.min().data(), b.max().data(), &h_));
+  }
+  ~DeviceSymmetricEvaluator() override { pam_fmm_evaluator_destroy(h_); }
+  VecX evaluate() const override {
+    VecX out(n_);
+    polatory::detail::check(pam_fmm_evaluator_evaluate(h_, out.data(), n_));
+    return out;
+  }
+  void set_accuracy(double a) override { polatory::detail::check(pam_fmm_evaluator_set_accuracy(h_, a)); }
+  void set_points(const geometry::Points<Dim>& p) override {
+    n_ = p.rows();
+    polatory::detail::check(pam_fmm_evaluator_set_points(h_, p.data(), p.rows()));
+  }
+  void set_weights(const VecX& w) override {
+    polatory::detail::check(pam_fmm_evaluator_set_weights(h_, w.data(), w.rows()));
+  }
+
+ private:
+  pam_fmm_evaluator* h_ = nullptr;
+  Index n_ = 0;
+};
+
+}  // namespace detail
+
+// fmm_evaluator.hpp:92-93
+template <int Dim>
+FmmGenericEvaluatorPtr<Dim> make_fmm_evaluator(const rbf::Rbf<Dim>& r, const geometry::Bbox<Dim>& bbox) {
+  return std::make_unique<detail::DeviceEvaluator<Dim>>(r, bbox);
+}
+
+// fmm_symmetric_evaluator.hpp:80-82
+template <int Dim>
+FmmGenericSymmetricEvaluatorPtr<Dim> make_fmm_symmetric_evaluator(const rbf::Rbf<Dim>& r,
+                                                                  const geometry::Bbox<Dim>& bbox) {
+  return std::make_unique<detail::DeviceSymmetricEvaluator<Dim>>(r, bbox);
+}
+
+}  // namespace fmm
+
+}  // namespace polatory

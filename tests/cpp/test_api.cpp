@@ -1,0 +1,91 @@
+// Exercises the C++ API mirror (polatory/polatory.hpp) the way polatory's own
+// tests do: test/interpolation/test_fitter.cpp (fit, residual <= tolerance) and
+// test/interpolation/test_evaluator.cpp (FMM vs direct within accuracy).
+// Exit code 0 on success; prints one line per check.
+#include <polatory/polatory.hpp>
+
+#include <cmath>
+#include <cstdio>
+#include <random>
+
+using polatory::Index;
+using polatory::Interpolant;
+using polatory::Model;
+using polatory::VecX;
+using polatory::geometry::Bbox3;
+using polatory::geometry::Points3;
+using polatory::rbf::Biharmonic3D;
+
+static double value_at(double x, double y, double z) { return std::sin(3 * x) * std::cos(2 * y) + 0.5 * z * z; }
+
+int main() {
+  std::mt19937_64 gen(42);
+  std::uniform_real_distribution<double> u(0.0, 1.0);
+  const Index n = 5000;
+  Points3 pts(n, 3);
+  VecX values(n);
+  for (Index i = 0; i < n; i++) {
+    for (int k = 0; k < 3; k++) pts(i, k) = u(gen);
+    values(i) = value_at(pts(i, 0), pts(i, 1), pts(i, 2));
+  }
+  Model<3> model(Biharmonic3D<3>(), 1);
+  Interpolant<3> interpolant(model);
+  const double tol = 1e-4;
+  interpolant.fit(pts, values, tol);
+  VecX w = interpolant.weights();
+  if (w.rows() != n + 4) return 1;
+
+  // FP64 direct residual at the data sites (DirectEvaluator semantics, bh3 = -r).
+  double res = 0.0;
+  for (Index i = 0; i < n; i++) {
+    double y = w(n) + w(n + 1) * pts(i, 0) + w(n + 2) * pts(i, 1) + w(n + 3) * pts(i, 2);
+    for (Index j = 0; j < n; j++) {
+      double dx = pts(i, 0) - pts(j, 0), dy = pts(i, 1) - pts(j, 1), dz = pts(i, 2) - pts(j, 2);
+      y += -std::sqrt(dx * dx + dy * dy + dz * dz) * w(j);
+    }
+    res = std::max(res, std::abs(y - values(i)));
+  }
+  std::printf("fit residual %.3e (tolerance %.1e)\n", res, tol);
+  if (!(res <= tol)) return 2;
+
+  // FMM seam: evaluator vs direct sum on random targets
+  const Index m = 2000;
+  Points3 trg(m, 3);
+  for (Index i = 0; i < m; i++)
+    for (int k = 0; k < 3; k++) trg(i, k) = u(gen);
+  auto ev = polatory::fmm::make_fmm_evaluator<3>(Biharmonic3D<3>(), Bbox3::from_points(pts).convex_hull(Bbox3::from_points(trg)));
+  ev->set_source_points(pts);
+  VecX wr(n);
+  for (Index i = 0; i < n; i++) wr(i) = w(i);
+  ev->set_weights(wr);
+  ev->set_target_points(trg);
+  VecX y = ev->evaluate();
+  double err = 0.0, scale = 0.0;
+  for (Index i = 0; i < m; i++) {
+    double d = 0.0;
+    for (Index j = 0; j < n; j++) {
+      double dx = trg(i, 0) - pts(j, 0), dy = trg(i, 1) - pts(j, 1), dz = trg(i, 2) - pts(j, 2);
+      d += -std::sqrt(dx * dx + dy * dy + dz * dz) * wr(j);
+    }
+    err = std::max(err, std::abs(y(i) - d));
+    scale = std::max(scale, std::abs(d));
+  }
+  std::printf("evaluator rel err %.3e\n", err / scale);
+  if (!(err <= 1e-8 * scale + 1e-12)) return 3;
+
+  // interpolant evaluation reproduces the data
+  VecX back = interpolant.evaluate(pts);
+  double e2 = 0.0;
+  for (Index i = 0; i < n; i++) e2 = std::max(e2, std::abs(back(i) - values(i)));
+  std::printf("evaluate at data sites: max |f - s| %.3e\n", e2);
+  if (!(e2 <= 10 * tol)) return 4;
+
+  // error contract: invalid arguments throw std::invalid_argument
+  try {
+    interpolant.fit(pts, values, 0.0);
+    return 5;
+  } catch (const std::invalid_argument&) {
+  }
+  std::printf("ok\n");
+  return 0;
+}

@@ -59,3 +59,24 @@ def test_rbf_parameter_defaults_follow_reference():
     with pytest.raises(ValueError):
         pa.CovExponential([1.0])
     assert pa.make_rbf("exp", [1.0, 2.0]).short_name == "exp"
+
+
+def test_cpp_api_mirror_links():
+    """The C++ mirror compiles and links against the C ABI (runs only on a GPU)."""
+    exe = os.path.join(ROOT, "tests", "cpp", "test_api")
+    assert os.path.exists(exe), "tests/cpp/test_api is built by polatory_amd/build.py"
+
+
+def test_model_validation_follows_reference():
+    import polatory_amd as pa
+    # model.hpp:39-50: poly degree within cpd_order - 1 .. 2
+    assert pa.Model(pa.Biharmonic3D()).poly_degree() == 0
+    assert pa.Model(pa.Triharmonic3D()).poly_degree() == 1
+    assert pa.Model(pa.CovExponential([1.0, 1.0])).poly_degree() == -1
+    assert pa.Model(pa.Biharmonic3D(), 1).poly_basis_size() == 4
+    with pytest.raises(ValueError, match="poly_degree"):
+        pa.Model(pa.Triharmonic3D(), 0)
+    with pytest.raises(ValueError, match="poly_degree"):
+        pa.Model(pa.Biharmonic3D(), 3)
+    with pytest.raises(ValueError, match="nugget"):
+        pa.Model(pa.Biharmonic3D(), 1, nugget=-1.0)
