@@ -98,6 +98,8 @@ typedef struct {
   int order;      /* Chebyshev points per axis, 3..12 */
   int leaf_level; /* octree leaf level 2..18, or -1 for automatic */
   int mode;       /* PAM_MODE_* */
+  double svd_tolerance; /* M2L compression cutoff sigma_r/sigma_0; 0 = dense M2L;
+                           negative = default (1e-8 for the mixed form of homogeneous kernels) */
 } pam_fmm_config;
 int pam_fmm_evaluator_set_config(pam_fmm_evaluator* h, const pam_fmm_config* cfg);
 int pam_fmm_evaluator_get_config(pam_fmm_evaluator* h, pam_fmm_config* cfg);
@@ -107,6 +109,7 @@ typedef struct {
   double ms_tree, ms_upward, ms_m2l, ms_l2l, ms_l2p, ms_p2p, ms_total;
   int precision, order, leaf_level, direct;
   int64_t n_src_leaves, n_trg_leaves;
+  int m2l_rank; /* rank of the compressed M2L, 0 when dense */
 } pam_fmm_stats;
 int pam_fmm_evaluator_stats(pam_fmm_evaluator* h, pam_fmm_stats* out);
 /* Near-field source-target pairs and M2L cell pairs of the current lists. */

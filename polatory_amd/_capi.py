@@ -22,7 +22,8 @@ class PamRbf(ctypes.Structure):
 
 class PamFmmConfig(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("order", ctypes.c_int),
-                ("leaf_level", ctypes.c_int), ("mode", ctypes.c_int)]
+                ("leaf_level", ctypes.c_int), ("mode", ctypes.c_int),
+                ("svd_tolerance", ctypes.c_double)]
 
 
 class PamFmmStats(ctypes.Structure):
@@ -31,7 +32,8 @@ class PamFmmStats(ctypes.Structure):
                 ("ms_l2p", ctypes.c_double), ("ms_p2p", ctypes.c_double),
                 ("ms_total", ctypes.c_double), ("precision", ctypes.c_int),
                 ("order", ctypes.c_int), ("leaf_level", ctypes.c_int), ("direct", ctypes.c_int),
-                ("n_src_leaves", ctypes.c_int64), ("n_trg_leaves", ctypes.c_int64)]
+                ("n_src_leaves", ctypes.c_int64), ("n_trg_leaves", ctypes.c_int64),
+                ("m2l_rank", ctypes.c_int)]
 
 
 class PamFitOptions(ctypes.Structure):

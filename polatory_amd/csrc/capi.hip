@@ -167,6 +167,7 @@ int pam_fmm_evaluator_set_config(pam_fmm_evaluator* h, const pam_fmm_config* cfg
     c.order = cfg->order;
     c.leaf_level = cfg->leaf_level;
     c.mode = cfg->mode;
+    c.svd_tol = cfg->svd_tolerance;
     ev(h).set_config(c);
   });
 }
@@ -178,6 +179,7 @@ int pam_fmm_evaluator_get_config(pam_fmm_evaluator* h, pam_fmm_config* cfg) {
     cfg->order = c.order;
     cfg->leaf_level = c.leaf_level;
     cfg->mode = c.mode;
+    cfg->svd_tolerance = c.svd_tol;
   });
 }
 
@@ -197,6 +199,7 @@ int pam_fmm_evaluator_stats(pam_fmm_evaluator* h, pam_fmm_stats* out) {
     out->direct = s.direct;
     out->n_src_leaves = s.n_src_leaves;
     out->n_trg_leaves = s.n_trg_leaves;
+    out->m2l_rank = s.m2l_rank;
   });
 }
 

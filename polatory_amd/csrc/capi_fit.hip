@@ -40,6 +40,7 @@ pam::FmmConfig to_cfg(const pam_fmm_config& c) {
   r.order = c.order;
   r.leaf_level = c.leaf_level;
   r.mode = c.mode;
+  r.svd_tol = c.svd_tolerance;
   return r;
 }
 
@@ -49,6 +50,7 @@ pam_fmm_config from_cfg(const pam::FmmConfig& c) {
   r.order = c.order;
   r.leaf_level = c.leaf_level;
   r.mode = c.mode;
+  r.svd_tolerance = c.svd_tol;
   return r;
 }
 

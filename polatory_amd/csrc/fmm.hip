@@ -8,6 +8,7 @@
 //               through LDS, targets x source-splits mapped onto 256 lanes
 #include <cmath>
 #include <type_traits>
+#include <vector>
 
 #include "fmm.hpp"
 
@@ -311,7 +312,7 @@ template <class T, int KIND>
 __global__ void __launch_bounds__(kThreads)
 p2p_kernel(const int32_t* __restrict__ tls, const int32_t* __restrict__ sls,
            const int32_t* __restrict__ p2p, const V4<T>* __restrict__ tpts,
-           const V4<T>* __restrict__ spts, const T* __restrict__ y_far,
+           const V4<T>* __restrict__ spts, const double* __restrict__ y_far,
            const int32_t* __restrict__ tperm, T pa, T pb, double* __restrict__ out) {
   constexpr int CAP = 32768 / static_cast<int>(sizeof(V4<T>));
   __shared__ V4<T> sp[CAP];
@@ -379,7 +380,7 @@ p2p_kernel(const int32_t* __restrict__ tls, const int32_t* __restrict__ sls,
       T s = T(0);
       for (int k = 0; k < nsplit; k++) s += red[ti + k * tpad];
       int i = ts + c0 + ti;
-      double far = y_far ? static_cast<double>(y_far[i]) : 0.0;
+      double far = y_far ? y_far[i] : 0.0;
       out[tperm[i]] = far + static_cast<double>(s);
     }
     __syncthreads();
@@ -406,88 +407,117 @@ struct Mfma<double> {
 
 constexpr int kM2LKc = 256;  // reduction chunk staged in LDS
 
-template <class T>
+// All levels of one M2L pass in one launch: column blocks of every level are
+// concatenated along grid.x, so the few blocks of the coarse levels run
+// beside the many blocks of the leaf level instead of after them.
+template <class TM, class TS>
 __global__ void __launch_bounds__(kThreads)
-m2l_kernel(int r, int rpad, int kpad, const T* __restrict__ a, int64_t per_v,
-           const int32_t* __restrict__ col_cell, const int32_t* __restrict__ blk_oct,
-           const int32_t* __restrict__ src_slots, const int32_t* __restrict__ vlist,
-           const T* __restrict__ in, int in_stride, T* __restrict__ out, int out_stride, T scale,
-           int accumulate) {
-  using acc_t = typename Mfma<T>::acc_t;
+m2l_batch_kernel(M2LBatch<TM, TS> b) {
+  using acc_t = typename Mfma<TM>::acc_t;
   constexpr int LDB = kM2LKc + 4;
-  __shared__ V4<T> bs4[kM2LCols * LDB / 4];
+  constexpr int NA = kM2LKc / 16;  // operator fragments per lane and chunk
+  constexpr int NB = kM2LKc / 32;  // staged multipole vectors per thread and chunk
+  __shared__ V4<TM> bs4[kM2LCols * LDB / 4];
   __shared__ int cells[kM2LCols];
   __shared__ int srcs[kM2LCols];
-  T* bs = reinterpret_cast<T*>(bs4);
-  const int blk = blockIdx.x;
+  TM* bs = reinterpret_cast<TM*>(bs4);
+  int lv = 0;
+  while (lv + 1 < b.n && static_cast<int>(blockIdx.x) >= b.blk_begin[lv + 1]) lv++;
+  const int blk = static_cast<int>(blockIdx.x) - b.blk_begin[lv];
+  const int32_t* __restrict__ src_slots = b.src_slots[lv];
+  const TS* __restrict__ in = b.in[lv];
+  const int rpad = b.rpad, kpad = b.kpad;
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int row0 = blockIdx.y * 64 + wave * 16;
-  const int oct = blk_oct ? blk_oct[blk] : 0;
+  const int oct = b.blk_oct[lv] ? b.blk_oct[lv][blk] : 0;
   const int nv = src_slots ? kM2LSlots : 1;
-  if (threadIdx.x < kM2LCols) cells[threadIdx.x] = col_cell[blk * kM2LCols + threadIdx.x];
+  if (threadIdx.x < kM2LCols) cells[threadIdx.x] = b.col_cell[lv][blk * kM2LCols + threadIdx.x];
   acc_t acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
-  // FP32 MFMA accumulators are flushed into FP64 every 64 reduction steps: the
-  // rounding of one long FP32 running sum (189 x P^3 terms) otherwise dominates
-  // the error of the whole far field.
+  // The MFMA accumulators are flushed into FP64 every 64 reduction steps (FP32)
+  // or once at the end (FP64): one long FP32 running sum over 189 x P^3 terms
+  // would otherwise dominate the error of the far field.
   double d0[4] = {0, 0, 0, 0}, d1[4] = {0, 0, 0, 0};
   auto flush = [&]() {
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       d0[i] += static_cast<double>(acc0[i]);
       d1[i] += static_cast<double>(acc1[i]);
-      acc0[i] = T(0);
-      acc1[i] = T(0);
+      acc0[i] = TM(0);
+      acc1[i] = TM(0);
     }
   };
   const int col = threadIdx.x / 8, part = threadIdx.x % 8;  // staging map: 8 lanes per column
   for (int sidx = 0; sidx < nv; sidx++) {
-    const int v = src_slots ? vlist[oct * kM2LSlots + sidx] : 0;
+    const int v = src_slots ? b.vlist[oct * kM2LSlots + sidx] : 0;
     __syncthreads();
     if (threadIdx.x < kM2LCols) {
       int c = cells[threadIdx.x];
       srcs[threadIdx.x] = c < 0 ? -1 : (src_slots ? src_slots[static_cast<int64_t>(c) * kM2LSlots + sidx] : c);
     }
     __syncthreads();
-    // Skip transfer vectors with no source in this column block.
-    bool any = false;
+    bool any = false;  // skip transfer vectors with no source in this block
     for (int k = 0; k < kM2LCols; k++) any |= srcs[k] >= 0;
     if (!any) continue;
-    const T* av = a + static_cast<int64_t>(v) * per_v;
+    const TM* av = b.a[lv] + static_cast<int64_t>(v) * b.per_v;
     const int src = srcs[col];
     for (int k0 = 0; k0 < kpad; k0 += kM2LKc) {
       const int kc = min(kM2LKc, kpad - k0);
-      __syncthreads();
-      for (int k4 = part; k4 < kc / 4; k4 += 8) {
-        V4<T> val{T(0), T(0), T(0), T(0)};
-        if (src >= 0) val = *reinterpret_cast<const V4<T>*>(in + static_cast<int64_t>(src) * in_stride + k0 + 4 * k4);
-        *reinterpret_cast<V4<T>*>(bs + col * LDB + 4 * k4) = val;
+      // Issue every global load of the chunk before waiting on any of them.
+      V4<TM> breg[NB];
+#pragma unroll
+      for (int i = 0; i < NB; i++) {
+        int k4 = part + 8 * i;
+        V4<TM> val{TM(0), TM(0), TM(0), TM(0)};
+        if (k4 < kc / 4 && src >= 0) {
+          V4<TS> t = *reinterpret_cast<const V4<TS>*>(in + static_cast<int64_t>(src) * b.in_stride + k0 + 4 * k4);
+          val.x = static_cast<TM>(t.x);
+          val.y = static_cast<TM>(t.y);
+          val.z = static_cast<TM>(t.z);
+          val.w = static_cast<TM>(t.w);
+        }
+        breg[i] = val;
+      }
+      V4<TM> areg[NA];
+#pragma unroll
+      for (int kb = 0; kb < NA; kb++) {
+        if (kb < kc / 16)
+          areg[kb] = *reinterpret_cast<const V4<TM>*>(
+              av + ((static_cast<int64_t>((k0 >> 4) + kb) * rpad + row0 + (lane & 15)) << 4) + ((lane >> 4) << 2));
       }
       __syncthreads();
-      if (row0 < rpad) {
-        for (int kb = 0; kb < kc / 16; kb++) {
-          const V4<T> a4 = *reinterpret_cast<const V4<T>*>(
-              av + ((static_cast<int64_t>((k0 >> 4) + kb) * rpad + row0 + (lane & 15)) << 4) + ((lane >> 4) << 2));
-          const V4<T> b0 = *reinterpret_cast<const V4<T>*>(bs + (lane & 15) * LDB + kb * 16 + ((lane >> 4) << 2));
-          const V4<T> b1 = *reinterpret_cast<const V4<T>*>(bs + (16 + (lane & 15)) * LDB + kb * 16 + ((lane >> 4) << 2));
-          acc0 = Mfma<T>::op(a4.x, b0.x, acc0);
-          acc1 = Mfma<T>::op(a4.x, b1.x, acc1);
-          acc0 = Mfma<T>::op(a4.y, b0.y, acc0);
-          acc1 = Mfma<T>::op(a4.y, b1.y, acc1);
-          acc0 = Mfma<T>::op(a4.z, b0.z, acc0);
-          acc1 = Mfma<T>::op(a4.z, b1.z, acc1);
-          acc0 = Mfma<T>::op(a4.w, b0.w, acc0);
-          acc1 = Mfma<T>::op(a4.w, b1.w, acc1);
-          if constexpr (sizeof(T) == 4) {
+#pragma unroll
+      for (int i = 0; i < NB; i++) {
+        int k4 = part + 8 * i;
+        if (k4 < kc / 4) *reinterpret_cast<V4<TM>*>(bs + col * LDB + 4 * k4) = breg[i];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int kb = 0; kb < NA; kb++) {
+        if (kb < kc / 16) {
+          const V4<TM> a4 = areg[kb];
+          const V4<TM> b0 = *reinterpret_cast<const V4<TM>*>(bs + (lane & 15) * LDB + kb * 16 + ((lane >> 4) << 2));
+          const V4<TM> b1 = *reinterpret_cast<const V4<TM>*>(bs + (16 + (lane & 15)) * LDB + kb * 16 + ((lane >> 4) << 2));
+          acc0 = Mfma<TM>::op(a4.x, b0.x, acc0);
+          acc1 = Mfma<TM>::op(a4.x, b1.x, acc1);
+          acc0 = Mfma<TM>::op(a4.y, b0.y, acc0);
+          acc1 = Mfma<TM>::op(a4.y, b1.y, acc1);
+          acc0 = Mfma<TM>::op(a4.z, b0.z, acc0);
+          acc1 = Mfma<TM>::op(a4.z, b1.z, acc1);
+          acc0 = Mfma<TM>::op(a4.w, b0.w, acc0);
+          acc1 = Mfma<TM>::op(a4.w, b1.w, acc1);
+          if constexpr (sizeof(TM) == 4) {
             if ((kb & 3) == 3) flush();
           }
         }
-        if constexpr (sizeof(T) == 4) flush();
       }
+      if constexpr (sizeof(TM) == 4) flush();
     }
   }
-  if constexpr (sizeof(T) == 8) flush();
+  if constexpr (sizeof(TM) == 8) flush();
   // D layout (16x16), column lane%16; register i holds row 4*(lane/16) + i for
   // the f32 instruction and row lane/16 + 4*i for the f64 one (measured on gfx950).
+  TS* __restrict__ out = b.out[lv];
+  const double scale = b.scale[lv];
 #pragma unroll
   for (int ct = 0; ct < 2; ct++) {
     const int c = cells[ct * 16 + (lane & 15)];
@@ -495,11 +525,11 @@ m2l_kernel(int r, int rpad, int kpad, const T* __restrict__ a, int64_t per_v,
     const double* acc = ct == 0 ? d0 : d1;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      int row = row0 + (sizeof(T) == 4 ? 4 * (lane >> 4) + i : (lane >> 4) + 4 * i);
-      if (row < r) {
-        T val = static_cast<T>(acc[i] * static_cast<double>(scale));
-        T* dst = out + static_cast<int64_t>(c) * out_stride + row;
-        *dst = accumulate ? *dst + val : val;
+      int row = row0 + (sizeof(TM) == 4 ? 4 * (lane >> 4) + i : (lane >> 4) + 4 * i);
+      if (row < b.r) {
+        TS val = static_cast<TS>(acc[i] * scale);
+        TS* dst = out + static_cast<int64_t>(c) * b.out_stride + row;
+        *dst = b.accumulate ? *dst + val : val;
       }
     }
   }
@@ -621,7 +651,7 @@ void launch_l2p(int p, const Tree& t, const T* l_leaf, const V4<T>* pts, T* y_fa
 
 template <class T>
 void launch_p2p(const RbfDesc& rbf, const Tree& trg, const Tree& src, const Lists& lists,
-                const V4<T>* tpts, const V4<T>* spts, const T* y_far, double* out, hipStream_t s) {
+                const V4<T>* tpts, const V4<T>* spts, const double* y_far, double* out, hipStream_t s) {
   int64_t nl = trg.n_leaves();
   if (nl == 0) return;
   double h = trg.box.width / std::ldexp(1.0, trg.level) / 2.0;
@@ -649,16 +679,38 @@ void build_m2l_dense(int p, const RbfDesc& rbf, double h, M2LOperator& op, DevBu
   PAM_LAUNCH_CHECK();
 }
 
-template <class T>
-void launch_m2l(const M2LOperator& op, const T* a, const M2LLevel& lv, const int32_t* src_slots,
-                const int32_t* vlist, const T* in, int in_stride, T* out, int out_stride, T scale,
-                bool accumulate, hipStream_t s) {
-  if (lv.n_cols == 0) return;
+template <class TM, class TS>
+void launch_m2l_batch(const M2LOperator& op, const std::vector<M2LLevelSpec<TM, TS>>& levels,
+                      const int32_t* vlist, bool accumulate, int in_stride, int out_stride, hipStream_t s) {
   if (op.kpad % 16 != 0 || in_stride % 4 != 0) throw_runtime("m2l: bad padding");
-  dim3 grid(static_cast<unsigned>(lv.n_cols / kM2LCols), static_cast<unsigned>(op.rpad / 64));
-  m2l_kernel<T><<<grid, kThreads, 0, s>>>(op.r, op.rpad, op.kpad, a, static_cast<int64_t>(op.per_v),
-                                          lv.col_cell.get(), lv.blk_oct.get(), src_slots, vlist, in,
-                                          in_stride, out, out_stride, scale, accumulate ? 1 : 0);
+  if (levels.size() > static_cast<size_t>(kMaxM2LLevels)) throw_runtime("m2l: too many levels");
+  M2LBatch<TM, TS> b{};
+  b.r = op.r;
+  b.rpad = op.rpad;
+  b.kpad = op.kpad;
+  b.per_v = static_cast<int64_t>(op.per_v);
+  b.vlist = vlist;
+  b.accumulate = accumulate ? 1 : 0;
+  b.in_stride = in_stride;
+  b.out_stride = out_stride;
+  int blocks = 0;
+  for (const auto& l : levels) {
+    if (!l.lv || l.lv->n_cols == 0) continue;
+    int k = b.n++;
+    b.blk_begin[k] = blocks;
+    b.col_cell[k] = l.lv->col_cell.get();
+    b.blk_oct[k] = l.src_slots ? l.lv->blk_oct.get() : nullptr;
+    b.src_slots[k] = l.src_slots;
+    b.in[k] = l.in;
+    b.out[k] = l.out;
+    b.a[k] = l.a;
+    b.scale[k] = l.scale;
+    blocks += static_cast<int>(l.lv->n_cols / kM2LCols);
+  }
+  b.blk_begin[b.n] = blocks;
+  if (blocks == 0) return;
+  dim3 grid(static_cast<unsigned>(blocks), static_cast<unsigned>(op.rpad / 64));
+  m2l_batch_kernel<TM, TS><<<grid, kThreads, 0, s>>>(b);
   PAM_LAUNCH_CHECK();
 }
 
@@ -671,11 +723,12 @@ void launch_m2l(const M2LOperator& op, const T* a, const M2LLevel& lv, const int
   template void launch_l2l<T>(int, const Tree&, int, const T*, T*, hipStream_t);                    \
   template void launch_l2p<T>(int, const Tree&, const T*, const V4<T>*, T*, hipStream_t);           \
   template void launch_p2p<T>(const RbfDesc&, const Tree&, const Tree&, const Lists&, const V4<T>*, \
-                              const V4<T>*, const T*, double*, hipStream_t);                       \
+                              const V4<T>*, const double*, double*, hipStream_t);                       \
   template void build_m2l_dense<T>(int, const RbfDesc&, double, M2LOperator&, DevBuf<T>&,          \
                                    hipStream_t);                                                    \
-  template void launch_m2l<T>(const M2LOperator&, const T*, const M2LLevel&, const int32_t*,        \
-                              const int32_t*, const T*, int, T*, int, T, bool, hipStream_t);
+  template void launch_m2l_batch<T, double>(const M2LOperator&,                                        \
+                                            const std::vector<M2LLevelSpec<T, double>>&, const int32_t*, \
+                                            bool, int, int, hipStream_t);
 
 PAM_INSTANTIATE(float)
 PAM_INSTANTIATE(double)

@@ -8,6 +8,8 @@
 //   a along x; nodes x_a = cos((2a+1) pi / (2P)).
 #pragma once
 
+#include <vector>
+
 #include "common.hpp"
 #include "rbf_device.hpp"
 #include "tree.hpp"
@@ -55,7 +57,7 @@ void launch_l2p(int p, const Tree& t, const T* l_leaf, const V4<T>* pts, T* y_fa
 // out[perm[i]] = y_far[i] + near field, FP64 output in input order.
 template <class T>
 void launch_p2p(const RbfDesc& rbf, const Tree& trg, const Tree& src, const Lists& lists,
-                const V4<T>* tpts, const V4<T>* spts, const T* y_far, double* out, hipStream_t s);
+                const V4<T>* tpts, const V4<T>* spts, const double* y_far, double* out, hipStream_t s);
 
 // Builds the dense M2L operator of one level: entries phi(h * |x_a + 2v - x_b|)
 // for cells of half width h (kernel constants folded by phi_consts).
@@ -63,11 +65,43 @@ template <class T>
 void build_m2l_dense(int p, const RbfDesc& rbf, double h, M2LOperator& op, DevBuf<T>& a,
                      hipStream_t s);
 
-// out[col_cell][row] (+)= scale * sum_slots A_v * in[src] over the octant's 189
-// transfer vectors (src_slots != null), or out[c] = A_0 * in[c] (src_slots == null).
-template <class T>
-void launch_m2l(const M2LOperator& op, const T* a, const M2LLevel& lv, const int32_t* src_slots,
-                const int32_t* vlist, const T* in, int in_stride, T* out, int out_stride, T scale,
-                bool accumulate, hipStream_t s);
+// One level of an M2L pass: out[col_cell][row] (+)= scale * sum_slots A_v * in[src]
+// over the octant's 189 transfer vectors (src_slots != null), or
+// out[c] = scale * A_0 * in[c] for every column cell (src_slots == null).
+// TM: MFMA arithmetic (float or double); TS: storage of in/out (expansions).
+template <class TM, class TS>
+struct M2LLevelSpec {
+  const M2LLevel* lv = nullptr;
+  const int32_t* src_slots = nullptr;
+  const TS* in = nullptr;
+  TS* out = nullptr;
+  const TM* a = nullptr;
+  double scale = 1.0;
+};
+
+constexpr int kMaxM2LLevels = 20;
+
+template <class TM, class TS>
+struct M2LBatch {
+  int n;
+  int r, rpad, kpad;
+  int64_t per_v;
+  const int32_t* vlist;
+  int accumulate;
+  int in_stride, out_stride;
+  int blk_begin[kMaxM2LLevels + 1];
+  const int32_t* col_cell[kMaxM2LLevels];
+  const int32_t* blk_oct[kMaxM2LLevels];
+  const int32_t* src_slots[kMaxM2LLevels];
+  const TS* in[kMaxM2LLevels];
+  TS* out[kMaxM2LLevels];
+  const TM* a[kMaxM2LLevels];
+  double scale[kMaxM2LLevels];
+};
+
+// All levels in one launch (they share the operator shape).
+template <class TM, class TS>
+void launch_m2l_batch(const M2LOperator& op, const std::vector<M2LLevelSpec<TM, TS>>& levels,
+                      const int32_t* vlist, bool accumulate, int in_stride, int out_stride, hipStream_t s);
 
 }  // namespace pam

@@ -9,6 +9,7 @@
 #include <tuple>
 
 #include "device_api.hpp"
+#include "m2l_compress.hpp"
 
 namespace pam {
 
@@ -62,11 +63,68 @@ std::shared_ptr<OpEntry<T>> get_op(int p, const RbfDesc& rbf, double h, hipStrea
   return e;
 }
 
+using CKey = std::tuple<int, int, int, int, double, double, double, double>;  // + tolerance
+
+template <class T>
+struct CCache {
+  std::mutex mu;
+  std::map<CKey, std::shared_ptr<CompressedM2L<T>>> map;
+};
+
+template <class T>
+std::shared_ptr<CompressedM2L<T>> get_compressed(int p, const RbfDesc& rbf, double h, double tol,
+                                                 hipStream_t s) {
+  int dev = 0;
+  PAM_HIP(hipGetDevice(&dev));
+  CKey key{dev, static_cast<int>(sizeof(T)), p, rbf.kind, rbf.p0, rbf.p1, h, tol};
+  static CCache<T> c;
+  std::lock_guard<std::mutex> lk(c.mu);
+  auto it = c.map.find(key);
+  if (it != c.map.end()) return it->second;
+  if (c.map.size() >= 16) c.map.erase(c.map.begin());
+  auto e = std::make_shared<CompressedM2L<T>>();
+  build_m2l_compressed<T>(p, rbf, h, tol, *e, s);
+  c.map[key] = e;
+  return e;
+}
+
 // Degree of homogeneity phi(l r) = l^deg phi(r), or 0 when phi is not homogeneous.
 int homogeneity(const RbfDesc& r) {
   if ((r.kind == kBh3 || r.kind == kTh3) && r.p1 == 0.0) return r.kind == kBh3 ? 1 : 3;
   return 0;
 }
+
+__global__ void identity_cols_kernel(int32_t* cols, int64_t n, int64_t n_cols) {
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n_cols) cols[i] = i < n ? static_cast<int32_t>(i) : -1;
+}
+
+void build_identity_cols(const Tree& t, std::vector<M2LLevel>& out, hipStream_t s) {
+  out.clear();
+  out.resize(t.level + 1);
+  for (int l = 2; l <= t.level; l++) {
+    int64_t n = t.levels[l].n;
+    M2LLevel& m = out[l];
+    m.n_cols = (n + kM2LCols - 1) / kM2LCols * kM2LCols;
+    m.col_cell.resize(static_cast<size_t>(std::max<int64_t>(m.n_cols, 1)));
+    if (m.n_cols > 0) {
+      identity_cols_kernel<<<ceil_div(m.n_cols, 256), 256, 0, s>>>(m.col_cell.get(), n, m.n_cols);
+      PAM_LAUNCH_CHECK();
+    }
+  }
+}
+
+}  // namespace
+
+double effective_svd_tol(const FmmConfig& c, const RbfDesc& rbf) {
+  // Compression is opt-in: the cutoff is applied to singular values obtained
+  // from G = sum K K^T, whose FP64 eigenvalues resolve sigma_r / sigma_0 only
+  // down to ~1e-8, which limits how far the rank can drop at 1e-6 accuracy.
+  (void)rbf;
+  return c.svd_tol > 0.0 ? c.svd_tol : 0.0;
+}
+
+namespace {
 
 void make_box_from_bbox(const double* bmin, const double* bmax, const double* aniso, Box& box) {
   // src/fmm/utility.hpp:19-33: corners through A, 1.01 x the largest extent.
@@ -95,7 +153,7 @@ void make_box_from_bbox(const double* bmin, const double* bmax, const double* an
 
 int auto_leaf_level(int64_t n_src, int64_t n_trg) {
   double n = static_cast<double>(std::max<int64_t>(std::max(n_src, n_trg), 1));
-  int l = static_cast<int>(std::lround(std::log(n / 24.0) / std::log(8.0)));
+  int l = static_cast<int>(std::lround(std::log(n / 128.0) / std::log(8.0)));
   return std::clamp(l, 2, 18);
 }
 
@@ -142,9 +200,9 @@ void FmmEvaluator::set_config(const FmmConfig& c) {
 }
 
 void FmmEvaluator::invalidate_expansions() {
-  s32_.multipoles_valid = s64_.multipoles_valid = false;
-  s32_.spts_valid = s64_.spts_valid = false;
-  s32_.tpts_valid = s64_.tpts_valid = false;
+  st_.multipoles_valid = false;
+  st_.s64_valid = st_.s32_valid = false;
+  st_.t64_valid = st_.t32_valid = false;
 }
 
 void FmmEvaluator::set_source_points(const double* xyz, int64_t n, bool on_device) {
@@ -181,7 +239,7 @@ void FmmEvaluator::set_target_points(const double* xyz, int64_t n, bool on_devic
   if (n > 0) launch_bbox(trg_pts_.get(), n, bbox_out_.get() + 6, bbox_scratch_, stream_);
   trg_tree_dirty_ = true;
   box_valid_ = false;
-  s32_.tpts_valid = s64_.tpts_valid = false;
+  st_.t64_valid = st_.t32_valid = false;
   PAM_HIP(hipStreamSynchronize(stream_));
 }
 
@@ -194,9 +252,9 @@ void FmmEvaluator::set_weights(const double* w, int64_t n, bool on_device) {
   }
   have_weights_ = true;
   // Point coordinates stay; only the weights and the multipoles change.
-  if (s32_.spts_valid) launch_set_point_weights<float>(src_tree_, w_.get(), s32_.spts.get(), stream_);
-  if (s64_.spts_valid) launch_set_point_weights<double>(src_tree_, w_.get(), s64_.spts.get(), stream_);
-  s32_.multipoles_valid = s64_.multipoles_valid = false;
+  if (st_.s32_valid) launch_set_point_weights<float>(src_tree_, w_.get(), st_.spts32.get(), stream_);
+  if (st_.s64_valid) launch_set_point_weights<double>(src_tree_, w_.get(), st_.spts64.get(), stream_);
+  st_.multipoles_valid = false;
   if (!on_device) PAM_HIP(hipStreamSynchronize(stream_));
 }
 
@@ -254,16 +312,18 @@ void FmmEvaluator::ensure_trees() {
   bool rebuilt = false;
   if (src_tree_dirty_ || src_tree_.level != L) {
     build_tree(src_tree_, src_pts_.get(), n_src_, box_, L, ws_, stream_);
+    build_identity_cols(src_tree_, src_cols_, stream_);
     src_tree_dirty_ = false;
     rebuilt = true;
-    s32_.spts_valid = s64_.spts_valid = false;
-    s32_.multipoles_valid = s64_.multipoles_valid = false;
+    st_.s32_valid = st_.s64_valid = false;
+    st_.multipoles_valid = false;
   }
   if (!symmetric_ && (trg_tree_dirty_ || trg_tree_.level != L)) {
     build_tree(trg_tree_, trg_pts_.get(), n_trg_, box_, L, ws_, stream_);
+    build_identity_cols(trg_tree_, trg_cols_, stream_);
     trg_tree_dirty_ = false;
     rebuilt = true;
-    s32_.tpts_valid = s64_.tpts_valid = false;
+    st_.t32_valid = st_.t64_valid = false;
   }
   if (symmetric_) trg_tree_dirty_ = false;
   if (rebuilt || lists_.m2l.size() != static_cast<size_t>(L + 1)) {
@@ -277,67 +337,130 @@ void FmmEvaluator::prepare_trees() {
   PAM_HIP(hipStreamSynchronize(stream_));
 }
 
-template <>
-FmmState<float>& FmmEvaluator::state<float>() { return s32_; }
-template <>
-FmmState<double>& FmmEvaluator::state<double>() { return s64_; }
 
 void FmmEvaluator::record(int idx) { PAM_HIP(hipEventRecord(ev_[idx], stream_)); }
 
-template <class T>
+template <class TP>
 void FmmEvaluator::run_fmm(double* d_out) {
-  FmmState<T>& st = state<T>();
+  FmmState& st = st_;
   const int P = cfg_.order;
   const int kp = kpad_of(P);
   ensure_trees();
   const Tree& tt = target_tree();
   const int L = src_tree_.level;
   record(1);
-  if (st.m.size() != static_cast<size_t>(L + 1)) {
+  if (st.m.size() != static_cast<size_t>(L + 1) || st.multipole_order != P) {
     st.m.clear();
     st.l.clear();
     st.m.resize(L + 1);
     st.l.resize(L + 1);
+    st.mt.clear();
+    st.lt.clear();
+    st.multipoles_valid = false;
+    st.multipole_order = P;
+  }
+  if (!st.s64_valid) {
+    st.spts64.resize(n_src_);
+    launch_prepare_points<double>(src_tree_, src_pts_.get(), w_.get(), st.spts64.get(), stream_);
+    st.s64_valid = true;
     st.multipoles_valid = false;
   }
-  if (!st.spts_valid) {
-    st.spts.resize(n_src_);
-    launch_prepare_points<T>(src_tree_, src_pts_.get(), w_.get(), st.spts.get(), stream_);
-    st.spts_valid = true;
-    st.multipoles_valid = false;
+  if (!symmetric_ && !st.t64_valid) {
+    st.tpts64.resize(n_trg_);
+    launch_prepare_points<double>(trg_tree_, trg_pts_.get(), nullptr, st.tpts64.get(), stream_);
+    st.t64_valid = true;
   }
-  if (!symmetric_ && !st.tpts_valid) {
-    st.tpts.resize(n_trg_);
-    launch_prepare_points<T>(trg_tree_, trg_pts_.get(), nullptr, st.tpts.get(), stream_);
-    st.tpts_valid = true;
+  const V4<double>* tpts64 = symmetric_ ? st.spts64.get() : st.tpts64.get();
+  const V4<TP>* sp2p;
+  const V4<TP>* tp2p;
+  if constexpr (sizeof(TP) == 4) {
+    if (!st.s32_valid) {
+      st.spts32.resize(n_src_);
+      launch_prepare_points<float>(src_tree_, src_pts_.get(), w_.get(), st.spts32.get(), stream_);
+      st.s32_valid = true;
+    }
+    if (!symmetric_ && !st.t32_valid) {
+      st.tpts32.resize(n_trg_);
+      launch_prepare_points<float>(trg_tree_, trg_pts_.get(), nullptr, st.tpts32.get(), stream_);
+      st.t32_valid = true;
+    }
+    sp2p = st.spts32.get();
+    tp2p = symmetric_ ? st.spts32.get() : st.tpts32.get();
+  } else {
+    sp2p = st.spts64.get();
+    tp2p = tpts64;
   }
-  const V4<T>* tpts = symmetric_ ? st.spts.get() : st.tpts.get();
   if (!st.multipoles_valid) {
     for (int l = 2; l <= L; l++) st.m[l].resize(static_cast<size_t>(src_tree_.levels[l].n) * kp);
-    launch_p2m<T>(P, src_tree_, st.spts.get(), st.m[L].get(), stream_);
-    for (int l = L - 1; l >= 2; l--) launch_m2m<T>(P, src_tree_, l, st.m[l + 1].get(), st.m[l].get(), stream_);
+    launch_p2m<double>(P, src_tree_, st.spts64.get(), st.m[L].get(), stream_);
+    for (int l = L - 1; l >= 2; l--) launch_m2m<double>(P, src_tree_, l, st.m[l + 1].get(), st.m[l].get(), stream_);
     st.multipoles_valid = true;
   }
   record(2);
   const int deg = homogeneity(rbf_);
   const int32_t* vl = octant_vlist_device(stream_);
-  std::shared_ptr<OpEntry<T>> unit_op;
-  if (deg > 0) unit_op = get_op<T>(P, rbf_, 1.0, stream_);
-  for (int l = 2; l <= L; l++) {
-    st.l[l].resize(static_cast<size_t>(tt.levels[l].n) * kp);
-    double h = box_.width / std::ldexp(1.0, l) / 2.0;
-    std::shared_ptr<OpEntry<T>> op = deg > 0 ? unit_op : get_op<T>(P, rbf_, h, stream_);
-    T scale = static_cast<T>(deg > 0 ? std::pow(h, deg) : 1.0);
-    launch_m2l<T>(op->op, op->a.get(), lists_.m2l[l], lists_.m2l[l].src.get(), vl, st.m[l].get(), kp,
-                  st.l[l].get(), kp, scale, false, stream_);
+  const double tol = effective_svd_tol(cfg_, rbf_);
+  stats_.m2l_rank = 0;
+  for (int l = 2; l <= L; l++) st.l[l].resize(static_cast<size_t>(tt.levels[l].n) * kp);
+  if (tol > 0.0) {
+    // SVD-compressed M2L: compress multipoles, transfer in rank r, expand locals.
+    if (st.mt.size() != static_cast<size_t>(L + 1)) {
+      st.mt.clear();
+      st.lt.clear();
+      st.mt.resize(L + 1);
+      st.lt.resize(L + 1);
+    }
+    std::vector<std::shared_ptr<CompressedM2L<TP>>> ops(L + 1);
+    std::shared_ptr<CompressedM2L<TP>> unit;
+    if (deg > 0) unit = get_compressed<TP>(P, rbf_, 1.0, tol, stream_);
+    std::vector<M2LLevelSpec<TP, double>> cmp, xfer, exp;
+    for (int l = 2; l <= L; l++) {
+      double h = box_.width / std::ldexp(1.0, l) / 2.0;
+      ops[l] = deg > 0 ? unit : get_compressed<TP>(P, rbf_, h, tol, stream_);
+      const int kr = ops[l]->kpad_r;
+      stats_.m2l_rank = ops[l]->r;
+      size_t nmt = static_cast<size_t>(src_tree_.levels[l].n) * kr;
+      if (st.mt[l].size() != nmt) {
+        st.mt[l].resize(nmt);
+        st.mt[l].zero(stream_);
+      }
+      size_t nlt = static_cast<size_t>(tt.levels[l].n) * kr;
+      if (st.lt[l].size() != nlt) {
+        st.lt[l].resize(nlt);
+        st.lt[l].zero(stream_);
+      }
+      double scale = deg > 0 ? std::pow(h, deg) : 1.0;
+      cmp.push_back({&src_cols_[l], nullptr, st.m[l].get(), st.mt[l].get(), ops[l]->ut.get(), 1.0});
+      xfer.push_back({&lists_.m2l[l], lists_.m2l[l].src.get(), st.mt[l].get(), st.lt[l].get(), ops[l]->c.get(), scale});
+      exp.push_back({symmetric_ ? &src_cols_[l] : &trg_cols_[l], nullptr, st.lt[l].get(), st.l[l].get(),
+                     ops[l]->u.get(), 1.0});
+    }
+    if (L >= 2) {
+      const auto& o = *ops[2];
+      launch_m2l_batch<TP, double>(o.ut_op, cmp, vl, false, kp, o.kpad_r, stream_);
+      launch_m2l_batch<TP, double>(o.c_op, xfer, vl, false, o.kpad_r, o.kpad_r, stream_);
+      launch_m2l_batch<TP, double>(o.u_op, exp, vl, false, o.kpad_r, kp, stream_);
+    }
+  } else {
+    std::vector<std::shared_ptr<OpEntry<TP>>> ops(L + 1);
+    std::shared_ptr<OpEntry<TP>> unit_op;
+    if (deg > 0) unit_op = get_op<TP>(P, rbf_, 1.0, stream_);
+    std::vector<M2LLevelSpec<TP, double>> xfer;
+    for (int l = 2; l <= L; l++) {
+      double h = box_.width / std::ldexp(1.0, l) / 2.0;
+      ops[l] = deg > 0 ? unit_op : get_op<TP>(P, rbf_, h, stream_);
+      double scale = deg > 0 ? std::pow(h, deg) : 1.0;
+      xfer.push_back({&lists_.m2l[l], lists_.m2l[l].src.get(), st.m[l].get(), st.l[l].get(), ops[l]->a.get(), scale});
+    }
+    if (L >= 2) launch_m2l_batch<TP, double>(ops[2]->op, xfer, vl, false, kp, kp, stream_);
   }
   record(3);
-  for (int l = 3; l <= L; l++) launch_l2l<T>(P, tt, l, st.l[l - 1].get(), st.l[l].get(), stream_);
+  for (int l = 3; l <= L; l++) launch_l2l<double>(P, tt, l, st.l[l - 1].get(), st.l[l].get(), stream_);
   record(4);
   st.y_far.resize(static_cast<size_t>(n_trg()));
-  launch_l2p<T>(P, tt, st.l[L].get(), tpts, st.y_far.get(), stream_);
+  launch_l2p<double>(P, tt, st.l[L].get(), tpts64, st.y_far.get(), stream_);
   record(5);
-  launch_p2p<T>(rbf_, tt, src_tree_, lists_, tpts, st.spts.get(), st.y_far.get(), d_out, stream_);
+  launch_p2p<TP>(rbf_, tt, src_tree_, lists_, tp2p, sp2p, st.y_far.get(), d_out, stream_);
   record(6);
 }
 

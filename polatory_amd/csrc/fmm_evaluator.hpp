@@ -24,20 +24,33 @@ struct FmmConfig {
   int order = 6;
   int leaf_level = -1;  // -1: chosen from the point counts
   int mode = kModeAuto;
+  // M2L compression: relative singular-value cutoff > 0 selects the
+  // SVD-compressed M2L; 0 or negative = dense M2L (the default).
+  double svd_tol = -1.0;
 };
+
+// Compression tolerance actually used for a configuration and kernel.
+double effective_svd_tol(const FmmConfig& c, const RbfDesc& rbf);
 
 struct FmmStats {
   double ms_tree = 0, ms_upward = 0, ms_m2l = 0, ms_l2l = 0, ms_l2p = 0, ms_p2p = 0, ms_total = 0;
   int precision = 0, order = 0, leaf_level = 0, direct = 0;
   int64_t n_src_leaves = 0, n_trg_leaves = 0;
+  int m2l_rank = 0;  // 0: dense
 };
 
-template <class T>
+// Expansions and the light O(N P^3) stages (P2M, M2M, L2L, L2P) are FP64 in
+// both forms; the mixed form runs P2P on FP32 copies of the points and M2L on
+// FP32 MFMA (operands rounded to FP32, FP64 accumulation).
 struct FmmState {
-  DevBuf<V4<T>> spts, tpts;
-  std::vector<DevBuf<T>> m, l;  // per level
-  DevBuf<T> y_far;
-  bool spts_valid = false, tpts_valid = false, multipoles_valid = false;
+  DevBuf<V4<double>> spts64, tpts64;
+  DevBuf<V4<float>> spts32, tpts32;
+  std::vector<DevBuf<double>> m, l;    // per level, P^3 node values
+  std::vector<DevBuf<double>> mt, lt;  // per level, compressed coefficients
+  DevBuf<double> y_far;
+  bool s64_valid = false, t64_valid = false, s32_valid = false, t32_valid = false;
+  bool multipoles_valid = false;
+  int multipole_order = 0;
 };
 
 class FmmEvaluator {
@@ -79,10 +92,8 @@ class FmmEvaluator {
   void ensure_box();
   void ensure_trees();
   bool use_direct() const;
-  template <class T>
+  template <class TP>
   void run_fmm(double* d_out);
-  template <class T>
-  FmmState<T>& state();
   void invalidate_expansions();
   void record(int idx);
 
@@ -106,11 +117,12 @@ class FmmEvaluator {
 
   Tree src_tree_, trg_tree_;
   Lists lists_;
+  // all cells of each level as M2L columns (compress / decompress transforms)
+  std::vector<M2LLevel> src_cols_, trg_cols_;
   TreeWorkspace ws_;
   DevBuf<double> direct_scratch_;
 
-  FmmState<float> s32_;
-  FmmState<double> s64_;
+  FmmState st_;
 
   hipEvent_t ev_[10] = {};
   FmmStats stats_;
@@ -118,7 +130,7 @@ class FmmEvaluator {
 };
 
 // Tree height used when no leaf level is configured: target occupancy of
-// about 24 points per leaf, clamped to [2, 18].
+// about 128 points per leaf (balances the dense M2L against P2P), clamped to [2, 18].
 int auto_leaf_level(int64_t n_src, int64_t n_trg);
 
 }  // namespace pam

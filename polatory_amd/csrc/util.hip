@@ -58,15 +58,26 @@ __global__ void bbox_partial_kernel(const double* __restrict__ pts, int64_t n,
 }
 
 __global__ void bbox_final_kernel(const double* __restrict__ partial, int nb, double* __restrict__ out) {
-  if (threadIdx.x != 0) return;
+  __shared__ double s[6][kBboxThreads];
   double r[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-  for (int b = 0; b < nb; b++) {
+  for (int b = threadIdx.x; b < nb; b += kBboxThreads) {
     for (int k = 0; k < 3; k++) {
       r[k] = fmin(r[k], partial[b * 6 + k]);
       r[3 + k] = fmax(r[3 + k], partial[b * 6 + 3 + k]);
     }
   }
-  for (int k = 0; k < 6; k++) out[k] = r[k];
+  for (int k = 0; k < 6; k++) s[k][threadIdx.x] = r[k];
+  __syncthreads();
+  for (int off = kBboxThreads / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      for (int k = 0; k < 3; k++) {
+        s[k][threadIdx.x] = fmin(s[k][threadIdx.x], s[k][threadIdx.x + off]);
+        s[3 + k][threadIdx.x] = fmax(s[3 + k][threadIdx.x], s[3 + k][threadIdx.x + off]);
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) out[threadIdx.x] = s[threadIdx.x][0];
 }
 
 // ---- exclusive scan: 256 threads x 4 items per block ----
@@ -267,7 +278,7 @@ void launch_bbox(const double* pts, int64_t n, double* d_out, DevBuf<double>& sc
   scratch.resize(static_cast<size_t>(nb) * 6);
   bbox_partial_kernel<<<nb, kBboxThreads, 0, s>>>(pts, n, scratch.get());
   PAM_LAUNCH_CHECK();
-  bbox_final_kernel<<<1, 64, 0, s>>>(scratch.get(), nb, d_out);
+  bbox_final_kernel<<<1, kBboxThreads, 0, s>>>(scratch.get(), nb, d_out);
   PAM_LAUNCH_CHECK();
 }
 

@@ -32,11 +32,12 @@ def _bbox(bbox):
 
 
 class FmmConfig:
-    def __init__(self, precision="mixed", order=6, leaf_level=-1, mode="auto"):
+    def __init__(self, precision="mixed", order=6, leaf_level=-1, mode="auto", svd_tol=-1.0):
         self.precision = precision
         self.order = order
         self.leaf_level = leaf_level
         self.mode = mode
+        self.svd_tol = svd_tol  # <0: default, 0: dense M2L, >0: compression cutoff
 
     def to_c(self):
         c = _capi.PamFmmConfig()
@@ -44,6 +45,7 @@ class FmmConfig:
         c.order = self.order
         c.leaf_level = self.leaf_level
         c.mode = {"auto": 0, "direct": 1, "fmm": 2}[self.mode]
+        c.svd_tolerance = self.svd_tol
         return c
 
 
