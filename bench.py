@@ -101,6 +101,10 @@ def main():
     ap.add_argument("--order64", type=int, default=7)
     ap.add_argument("--leaf-level", type=int, default=-1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=("both", "mixed", "fp64"), default="both",
+                    help="run one form only (profiling runs); the JSON line needs both")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-sized buffer 3 times (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -108,6 +112,15 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copied {args.copy_calibration} MiB x 3")
+        return
     import polatory_amd as pa
 
     src, w, c, trg, sub = make_inputs()
@@ -121,6 +134,8 @@ def main():
 
     results = {}
     for name, cfg in (("mixed", cfg32), ("fp64", cfg64)):
+        if args.only not in ("both", name):
+            continue
         de = DeviceEval(torch, src, w, trg, cfg)
         sec, stats = time_steps(de, args.steps, args.warmup)
         y = de.output() + poly(c, trg)  # polynomial terms on the host for the error check only
@@ -136,6 +151,8 @@ def main():
         del de
         torch.cuda.synchronize()
 
+    if args.only != "both":
+        return
     m = results["mixed"]
     f = results["fp64"]
     p = m["order"]

@@ -89,7 +89,7 @@ int pam_fmm_evaluator_synchronize(pam_fmm_evaluator* h);
 /* Explicit configuration (replaces the accuracy search; no reference counterpart:
  * polatory picks {tree height, order, d} internally, fmm_accuracy_estimator.hpp:74-121). */
 #define PAM_PRECISION_FP64 0  /* every stage in FP64 */
-#define PAM_PRECISION_MIXED 1 /* FP32 P2P/P2M/L2P and FP32-MFMA M2L, FP64 output */
+#define PAM_PRECISION_MIXED 1 /* FP32 P2P and FP32-MFMA M2L; FP64 expansions, P2M/M2M/L2L/L2P and output */
 #define PAM_MODE_AUTO 0
 #define PAM_MODE_DIRECT 1
 #define PAM_MODE_FMM 2
@@ -98,8 +98,8 @@ typedef struct {
   int order;      /* Chebyshev points per axis, 3..12 */
   int leaf_level; /* octree leaf level 2..18, or -1 for automatic */
   int mode;       /* PAM_MODE_* */
-  double svd_tolerance; /* M2L compression cutoff sigma_r/sigma_0; 0 = dense M2L;
-                           negative = default (1e-8 for the mixed form of homogeneous kernels) */
+  double svd_tolerance; /* M2L compression cutoff sigma_r/sigma_0 when > 0;
+                           0 or negative = dense M2L (the default) */
 } pam_fmm_config;
 int pam_fmm_evaluator_set_config(pam_fmm_evaluator* h, const pam_fmm_config* cfg);
 int pam_fmm_evaluator_get_config(pam_fmm_evaluator* h, pam_fmm_config* cfg);
@@ -215,6 +215,48 @@ int pam_ras_num_domains(const pam_ras* h, int level, int* n);
 int pam_ras_domain(const pam_ras* h, int level, int dom, int32_t* idx, uint8_t* inner, int64_t* n);
 /* operator()(v) — ras_preconditioner.hpp:191-250 (host vectors of length n + l) */
 int pam_ras_apply(pam_ras* h, const double* v, double* out);
+
+/* ---------------------------------------------------------------------------
+ * Serving: fitted models resident in HBM, requests admitted by HBM budget and
+ * run concurrently (one stream per slot). No reference counterpart; replaces a
+ * loop of Interpolant::evaluate calls (interpolant.hpp:54-67), each of which
+ * rebuilds the evaluator.
+ * ------------------------------------------------------------------------- */
+typedef struct pam_server pam_server;
+typedef struct {
+  int n_slots;          /* requests on the device at once (default 4) */
+  int64_t hbm_budget;   /* bytes for resident models + admitted requests; 0 = 90% of free HBM */
+  double rel_accuracy;  /* target relative accuracy of every answer (default 1e-6);
+                           0 = FP64 FMM of order 10 */
+} pam_server_options;
+int pam_server_options_default(pam_server_options* opt);
+int pam_server_create(const pam_server_options* opt, pam_server** out);
+/* Waits for the requests in flight, then frees every model. */
+int pam_server_destroy(pam_server* h);
+/* Makes a fitted interpolant's model resident. Query points must lie in
+ * [lo, hi] (united with the centres' bbox). cfg may be NULL (accuracy search). */
+int pam_server_add_model(pam_server* h, const pam_interpolant* fitted, const double lo[3],
+                         const double hi[3], const pam_fmm_config* cfg, int* model_id);
+int pam_server_model_config(pam_server* h, int model_id, pam_fmm_config* cfg, int64_t* resident_bytes);
+/* Queues an evaluation of n points (host, n x 3) into out (host, n); both must
+ * stay valid until the request completes. */
+int pam_server_submit(pam_server* h, int model_id, const double* points, int64_t n, double* out,
+                      int64_t* request_id);
+/* Blocks until the request completes and returns its status. */
+int pam_server_wait(pam_server* h, int64_t request_id);
+int pam_server_test(pam_server* h, int64_t request_id, int* done);
+int pam_server_drain(pam_server* h);
+int pam_server_request_times(pam_server* h, int64_t request_id, double* queue_ms, double* total_ms);
+typedef struct {
+  int64_t submitted, completed, failed, points;
+  double p50_ms, p99_ms, mean_ms, max_ms, mean_queue_ms;
+  double points_per_s, wall_s;
+  int max_in_flight;
+  int64_t resident_bytes, budget, peak_admitted_bytes;
+} pam_server_stats;
+int pam_server_get_stats(pam_server* h, pam_server_stats* out);
+/* Forgets completed requests and restarts the latency statistics. */
+int pam_server_reset_stats(pam_server* h);
 
 #ifdef __cplusplus
 }

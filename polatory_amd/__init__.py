@@ -10,6 +10,7 @@ from .fmm import (FmmConfig, FmmEvaluator, FmmSymmetricEvaluator, direct_evaluat
 from .interpolant import (FitOptions, Interpolant, Model, RasPreconditioner, fgmres_dense,
                           POLY_DEGREE_MIN)
 from .rbf import Biharmonic3D, CovExponential, Rbf, Triharmonic3D, make_rbf
+from .serve import Server
 
 
 def device_count():
@@ -22,4 +23,5 @@ def device_count():
 __all__ = ["Biharmonic3D", "Triharmonic3D", "CovExponential", "Rbf", "make_rbf",
            "FmmConfig", "FmmEvaluator", "FmmSymmetricEvaluator", "make_fmm_evaluator",
            "make_fmm_symmetric_evaluator", "direct_evaluate", "device_count", "Model",
-           "Interpolant", "FitOptions", "RasPreconditioner", "fgmres_dense", "POLY_DEGREE_MIN"]
+           "Interpolant", "FitOptions", "RasPreconditioner", "fgmres_dense", "POLY_DEGREE_MIN",
+           "Server"]

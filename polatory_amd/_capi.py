@@ -49,6 +49,22 @@ class PamFitStats(ctypes.Structure):
                 ("factor_bytes", ctypes.c_int64)]
 
 
+class PamServerOptions(ctypes.Structure):
+    _fields_ = [("n_slots", ctypes.c_int), ("hbm_budget", ctypes.c_int64),
+                ("rel_accuracy", ctypes.c_double)]
+
+
+class PamServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64),
+                ("failed", ctypes.c_int64), ("points", ctypes.c_int64),
+                ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double),
+                ("mean_queue_ms", ctypes.c_double), ("points_per_s", ctypes.c_double),
+                ("wall_s", ctypes.c_double), ("max_in_flight", ctypes.c_int),
+                ("resident_bytes", ctypes.c_int64), ("budget", ctypes.c_int64),
+                ("peak_admitted_bytes", ctypes.c_int64)]
+
+
 class PamError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(msg)
@@ -119,6 +135,18 @@ _SIGS = {
     "pam_ras_num_domains": [_P, _INT, ctypes.POINTER(_INT)],
     "pam_ras_domain": [_P, _INT, _INT, _P, _P, ctypes.POINTER(_I64)],
     "pam_ras_apply": [_P, _P, _P],
+    "pam_server_options_default": [ctypes.POINTER(PamServerOptions)],
+    "pam_server_create": [ctypes.POINTER(PamServerOptions), ctypes.POINTER(_P)],
+    "pam_server_destroy": [_P],
+    "pam_server_add_model": [_P, _P, _P, _P, ctypes.POINTER(PamFmmConfig), ctypes.POINTER(_INT)],
+    "pam_server_model_config": [_P, _INT, ctypes.POINTER(PamFmmConfig), ctypes.POINTER(_I64)],
+    "pam_server_submit": [_P, _INT, _P, _I64, _P, ctypes.POINTER(_I64)],
+    "pam_server_wait": [_P, _I64],
+    "pam_server_test": [_P, _I64, ctypes.POINTER(_INT)],
+    "pam_server_drain": [_P],
+    "pam_server_request_times": [_P, _I64, ctypes.POINTER(_D), ctypes.POINTER(_D)],
+    "pam_server_get_stats": [_P, ctypes.POINTER(PamServerStats)],
+    "pam_server_reset_stats": [_P],
 }
 
 _lib = None

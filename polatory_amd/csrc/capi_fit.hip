@@ -5,17 +5,8 @@
 #include <memory>
 #include <vector>
 
-#include "capi_util.hpp"
-#include "interpolant.hpp"
+#include "capi_handles.hpp"
 #include "ras.hpp"
-
-struct pam_model {
-  pam::Model m;
-};
-
-struct pam_interpolant {
-  std::unique_ptr<pam::Interpolant> impl;
-};
 
 struct pam_ras {
   hipStream_t s = nullptr;
@@ -34,25 +25,8 @@ namespace {
 
 using pam::guard;
 
-pam::FmmConfig to_cfg(const pam_fmm_config& c) {
-  pam::FmmConfig r;
-  r.precision = c.precision;
-  r.order = c.order;
-  r.leaf_level = c.leaf_level;
-  r.mode = c.mode;
-  r.svd_tol = c.svd_tolerance;
-  return r;
-}
-
-pam_fmm_config from_cfg(const pam::FmmConfig& c) {
-  pam_fmm_config r;
-  r.precision = c.precision;
-  r.order = c.order;
-  r.leaf_level = c.leaf_level;
-  r.mode = c.mode;
-  r.svd_tolerance = c.svd_tol;
-  return r;
-}
+using pam::from_cfg;
+using pam::to_cfg;
 
 pam::FitOptions default_fit_options() {
   pam::FitOptions o;

@@ -45,6 +45,25 @@ __device__ __forceinline__ void lagrange(T xi, const T* __restrict__ tk, T* S) {
   }
 }
 
+// Same values written to column `col` of s[P][ncols] with a rolled loop over
+// a, so that the compiler does not keep the P^2 table tk in registers.
+template <class T, int P, int NC>
+__device__ __forceinline__ void lagrange_lds(T xi, const T* __restrict__ tk, T (*s)[NC], int col) {
+  T t[P];
+  t[0] = T(1);
+  if constexpr (P > 1) t[1] = xi;
+#pragma unroll
+  for (int k = 2; k < P; k++) t[k] = T(2) * xi * t[k - 1] - t[k - 2];
+  const T c0 = T(1) / T(P), c1 = T(2) / T(P);
+#pragma unroll 1
+  for (int a = 0; a < P; a++) {
+    T sum = T(0);
+#pragma unroll
+    for (int k = 1; k < P; k++) sum += tk[a * P + k] * t[k];
+    s[a][col] = c0 + c1 * sum;
+  }
+}
+
 // tr[o][a][a2] = S_a((x_a2 + (2o - 1)) / 2): parent-node weight a of child node a2
 // for the child on side o of an axis. Computed in FP64.
 template <class T, int P>
@@ -281,28 +300,40 @@ l2p_kernel(const int32_t* __restrict__ leaf_start, const V4<T>* __restrict__ pts
   constexpr int P2 = P * P, P3 = P * P * P;
   __shared__ T tk[P2];
   __shared__ T ls[P3];
+  // Per-thread Lagrange factors in LDS: the loops over them stay rolled (or
+  // partly unrolled), otherwise the compiler keeps the P^2 Chebyshev table and
+  // the P^3 local coefficients in registers across points and spills.
+  __shared__ T sxs[P][kThreads];
+  __shared__ T sys[P][kThreads];
+  __shared__ T szs[P][kThreads];
+  constexpr int kUnrollB = P <= 8 ? P : 2;
   const int leaf = blockIdx.x;
   init_tk<T, P>(tk);
   for (int idx = threadIdx.x; idx < P3; idx += kThreads) ls[idx] = l[static_cast<int64_t>(leaf) * kpad + idx];
   __syncthreads();
   const int ps = leaf_start[leaf], pe = leaf_start[leaf + 1];
-  for (int i = ps + threadIdx.x; i < pe; i += kThreads) {
+  const int c = threadIdx.x;
+  for (int i = ps + c; i < pe; i += kThreads) {
     V4<T> q = pts[i];
-    T sx[P], sy[P], sz[P];
-    lagrange<T, P>(q.x, tk, sx);
-    lagrange<T, P>(q.y, tk, sy);
-    lagrange<T, P>(q.z, tk, sz);
-    T y = T(0);
-    for (int a = 0; a < P; a++) {
-      T ya = T(0);
+    lagrange_lds<T, P, kThreads>(q.x, tk, sxs, c);
+    lagrange_lds<T, P, kThreads>(q.y, tk, sys, c);
+    lagrange_lds<T, P, kThreads>(q.z, tk, szs, c);
+    T sz[P];
 #pragma unroll
+    for (int g = 0; g < P; g++) sz[g] = szs[g][c];
+    T y = T(0);
+#pragma unroll 1
+    for (int a = 0; a < P; a++) {
+      const T* la = ls + a * P2;
+      T ya = T(0);
+#pragma unroll kUnrollB
       for (int bb = 0; bb < P; bb++) {
         T yb = T(0);
 #pragma unroll
-        for (int g = 0; g < P; g++) yb += ls[(a * P + bb) * P + g] * sz[g];
-        ya += yb * sy[bb];
+        for (int g = 0; g < P; g++) yb += la[bb * P + g] * sz[g];
+        ya += yb * sys[bb][c];
       }
-      y += ya * sx[a];
+      y += ya * sxs[a][c];
     }
     y_far[i] = y;
   }

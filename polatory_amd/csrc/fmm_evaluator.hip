@@ -206,6 +206,7 @@ void FmmEvaluator::invalidate_expansions() {
 }
 
 void FmmEvaluator::set_source_points(const double* xyz, int64_t n, bool on_device) {
+  if (so_) throw_runtime("this evaluator shares another evaluator's sources");
   if (n < 0) throw_invalid("negative point count");
   n_src_ = n;
   src_pts_.resize(static_cast<size_t>(n) * 3);
@@ -244,6 +245,7 @@ void FmmEvaluator::set_target_points(const double* xyz, int64_t n, bool on_devic
 }
 
 void FmmEvaluator::set_weights(const double* w, int64_t n, bool on_device) {
+  if (so_) throw_runtime("this evaluator shares another evaluator's sources");
   if (n != n_src_) throw_invalid("weights.rows() must equal the number of source points");
   w_.resize(static_cast<size_t>(n));
   if (n > 0) {
@@ -261,9 +263,10 @@ void FmmEvaluator::set_weights(const double* w, int64_t n, bool on_device) {
 bool FmmEvaluator::use_direct() const {
   if (cfg_.mode == kModeDirect) return true;
   if (cfg_.mode == kModeFmm) return false;
+  int64_t ns = S().n_src_;
   int64_t nt = symmetric_ ? n_src_ : n_trg_;
   // Below ~2^26 pairs the FP64 direct sum is both exact and cheaper on one GPU.
-  return static_cast<double>(n_src_) * static_cast<double>(nt) < 67108864.0;
+  return static_cast<double>(ns) * static_cast<double>(nt) < 67108864.0;
 }
 
 void FmmEvaluator::ensure_box() {
@@ -271,6 +274,22 @@ void FmmEvaluator::ensure_box() {
   double pb[12];
   PAM_HIP(hipMemcpyAsync(pb, bbox_out_.get(), sizeof pb, hipMemcpyDeviceToHost, stream_));
   PAM_HIP(hipStreamSynchronize(stream_));
+  if (so_) {
+    const Box& b = so_->box_;
+    if (n_trg_ > 0) {
+      for (int k = 0; k < 3; k++)
+        if (pb[6 + k] < b.lo[k] || pb[9 + k] > b.lo[k] + b.width)
+          throw_runtime("target points lie outside the box of the shared source tree");
+    }
+    bool same = b.width == box_.width && b.lo[0] == box_.lo[0] && b.lo[1] == box_.lo[1] && b.lo[2] == box_.lo[2];
+    if (!same) {
+      box_ = b;
+      trg_tree_dirty_ = true;
+      st_.t32_valid = st_.t64_valid = false;
+    }
+    box_valid_ = true;
+    return;
+  }
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   auto merge = [&](const double* b) {
     for (int k = 0; k < 3; k++) {
@@ -308,6 +327,20 @@ void FmmEvaluator::ensure_box() {
 
 void FmmEvaluator::ensure_trees() {
   ensure_box();
+  if (so_) {
+    const Tree& stree = so_->src_tree_;
+    const int L = stree.level;
+    bool rebuilt = false;
+    if (trg_tree_dirty_ || trg_tree_.level != L) {
+      build_tree(trg_tree_, trg_pts_.get(), n_trg_, box_, L, ws_, stream_);
+      build_identity_cols(trg_tree_, trg_cols_, stream_);
+      trg_tree_dirty_ = false;
+      rebuilt = true;
+      st_.t32_valid = st_.t64_valid = false;
+    }
+    if (rebuilt || lists_.m2l.size() != static_cast<size_t>(L + 1)) build_lists(trg_tree_, stree, lists_, ws_, stream_);
+    return;
+  }
   int L = cfg_.leaf_level >= 0 ? cfg_.leaf_level : auto_leaf_level(n_src_, n_trg());
   bool rebuilt = false;
   if (src_tree_dirty_ || src_tree_.level != L) {
@@ -337,25 +370,53 @@ void FmmEvaluator::prepare_trees() {
   PAM_HIP(hipStreamSynchronize(stream_));
 }
 
+void FmmEvaluator::prepare_source() {
+  if (so_) throw_runtime("prepare_source on an evaluator that shares another's sources");
+  if (!have_weights_ && n_src_ > 0) throw_runtime("weights are not set");
+  if (!config_valid_) choose_config();
+  // Built unless the configuration forces the direct sum: whether a given
+  // target set is evaluated directly is decided per evaluation.
+  if (n_src_ > 0 && cfg_.mode != kModeDirect) {
+    ensure_trees();
+    if (cfg_.precision == kFp64) prepare_source_expansions<double>();
+    else prepare_source_expansions<float>();
+  }
+  PAM_HIP(hipStreamSynchronize(stream_));
+}
+
+void FmmEvaluator::attach_source(const FmmEvaluator* owner) {
+  if (owner == this) owner = nullptr;
+  if (symmetric_) throw_runtime("a symmetric evaluator cannot share sources");
+  if (owner && (owner->symmetric_ || owner->so_)) throw_runtime("the owner must be a non-symmetric, non-attached evaluator");
+  if (owner == so_) return;
+  so_ = owner;
+  if (owner) {
+    rbf_ = owner->rbf_;
+    cfg_ = owner->cfg_;
+    fixed_config_ = true;
+    config_valid_ = true;
+  }
+  box_valid_ = false;
+  trg_tree_dirty_ = true;
+  lists_ = Lists{};
+  st_.t32_valid = st_.t64_valid = false;
+  // Targets are transformed with the rbf's anisotropy: they must be set again.
+  n_trg_ = 0;
+}
+
 
 void FmmEvaluator::record(int idx) { PAM_HIP(hipEventRecord(ev_[idx], stream_)); }
 
 template <class TP>
-void FmmEvaluator::run_fmm(double* d_out) {
+void FmmEvaluator::prepare_source_expansions() {
   FmmState& st = st_;
   const int P = cfg_.order;
   const int kp = kpad_of(P);
-  ensure_trees();
-  const Tree& tt = target_tree();
   const int L = src_tree_.level;
-  record(1);
   if (st.m.size() != static_cast<size_t>(L + 1) || st.multipole_order != P) {
     st.m.clear();
-    st.l.clear();
     st.m.resize(L + 1);
-    st.l.resize(L + 1);
     st.mt.clear();
-    st.lt.clear();
     st.multipoles_valid = false;
     st.multipole_order = P;
   }
@@ -365,36 +426,62 @@ void FmmEvaluator::run_fmm(double* d_out) {
     st.s64_valid = true;
     st.multipoles_valid = false;
   }
-  if (!symmetric_ && !st.t64_valid) {
-    st.tpts64.resize(n_trg_);
-    launch_prepare_points<double>(trg_tree_, trg_pts_.get(), nullptr, st.tpts64.get(), stream_);
-    st.t64_valid = true;
-  }
-  const V4<double>* tpts64 = symmetric_ ? st.spts64.get() : st.tpts64.get();
-  const V4<TP>* sp2p;
-  const V4<TP>* tp2p;
   if constexpr (sizeof(TP) == 4) {
     if (!st.s32_valid) {
       st.spts32.resize(n_src_);
       launch_prepare_points<float>(src_tree_, src_pts_.get(), w_.get(), st.spts32.get(), stream_);
       st.s32_valid = true;
     }
-    if (!symmetric_ && !st.t32_valid) {
-      st.tpts32.resize(n_trg_);
-      launch_prepare_points<float>(trg_tree_, trg_pts_.get(), nullptr, st.tpts32.get(), stream_);
-      st.t32_valid = true;
-    }
-    sp2p = st.spts32.get();
-    tp2p = symmetric_ ? st.spts32.get() : st.tpts32.get();
-  } else {
-    sp2p = st.spts64.get();
-    tp2p = tpts64;
   }
   if (!st.multipoles_valid) {
     for (int l = 2; l <= L; l++) st.m[l].resize(static_cast<size_t>(src_tree_.levels[l].n) * kp);
     launch_p2m<double>(P, src_tree_, st.spts64.get(), st.m[L].get(), stream_);
     for (int l = L - 1; l >= 2; l--) launch_m2m<double>(P, src_tree_, l, st.m[l + 1].get(), st.m[l].get(), stream_);
     st.multipoles_valid = true;
+  }
+}
+
+template <class TP>
+void FmmEvaluator::run_fmm(double* d_out) {
+  FmmState& st = st_;
+  const int P = cfg_.order;
+  const int kp = kpad_of(P);
+  ensure_trees();
+  const Tree& tt = target_tree();
+  const Tree& stree = S().src_tree_;
+  const FmmState& sst = S().st_;
+  const int L = stree.level;
+  record(1);
+  if (!so_) {
+    prepare_source_expansions<TP>();
+  } else if (!sst.multipoles_valid || sst.multipole_order != P || !sst.s64_valid ||
+             (sizeof(TP) == 4 && !sst.s32_valid)) {
+    throw_runtime("the shared source side is not prepared (call prepare_source on the owner)");
+  }
+  if (st.l.size() != static_cast<size_t>(L + 1)) {
+    st.l.clear();
+    st.l.resize(L + 1);
+    st.lt.clear();
+  }
+  if (!symmetric_ && !st.t64_valid) {
+    st.tpts64.resize(n_trg_);
+    launch_prepare_points<double>(trg_tree_, trg_pts_.get(), nullptr, st.tpts64.get(), stream_);
+    st.t64_valid = true;
+  }
+  const V4<double>* tpts64 = symmetric_ ? sst.spts64.get() : st.tpts64.get();
+  const V4<TP>* sp2p;
+  const V4<TP>* tp2p;
+  if constexpr (sizeof(TP) == 4) {
+    if (!symmetric_ && !st.t32_valid) {
+      st.tpts32.resize(n_trg_);
+      launch_prepare_points<float>(trg_tree_, trg_pts_.get(), nullptr, st.tpts32.get(), stream_);
+      st.t32_valid = true;
+    }
+    sp2p = sst.spts32.get();
+    tp2p = symmetric_ ? sst.spts32.get() : st.tpts32.get();
+  } else {
+    sp2p = sst.spts64.get();
+    tp2p = tpts64;
   }
   record(2);
   const int deg = homogeneity(rbf_);
@@ -419,7 +506,7 @@ void FmmEvaluator::run_fmm(double* d_out) {
       ops[l] = deg > 0 ? unit : get_compressed<TP>(P, rbf_, h, tol, stream_);
       const int kr = ops[l]->kpad_r;
       stats_.m2l_rank = ops[l]->r;
-      size_t nmt = static_cast<size_t>(src_tree_.levels[l].n) * kr;
+      size_t nmt = static_cast<size_t>(stree.levels[l].n) * kr;
       if (st.mt[l].size() != nmt) {
         st.mt[l].resize(nmt);
         st.mt[l].zero(stream_);
@@ -430,7 +517,7 @@ void FmmEvaluator::run_fmm(double* d_out) {
         st.lt[l].zero(stream_);
       }
       double scale = deg > 0 ? std::pow(h, deg) : 1.0;
-      cmp.push_back({&src_cols_[l], nullptr, st.m[l].get(), st.mt[l].get(), ops[l]->ut.get(), 1.0});
+      cmp.push_back({&S().src_cols_[l], nullptr, sst.m[l].get(), st.mt[l].get(), ops[l]->ut.get(), 1.0});
       xfer.push_back({&lists_.m2l[l], lists_.m2l[l].src.get(), st.mt[l].get(), st.lt[l].get(), ops[l]->c.get(), scale});
       exp.push_back({symmetric_ ? &src_cols_[l] : &trg_cols_[l], nullptr, st.lt[l].get(), st.l[l].get(),
                      ops[l]->u.get(), 1.0});
@@ -450,7 +537,7 @@ void FmmEvaluator::run_fmm(double* d_out) {
       double h = box_.width / std::ldexp(1.0, l) / 2.0;
       ops[l] = deg > 0 ? unit_op : get_op<TP>(P, rbf_, h, stream_);
       double scale = deg > 0 ? std::pow(h, deg) : 1.0;
-      xfer.push_back({&lists_.m2l[l], lists_.m2l[l].src.get(), st.m[l].get(), st.l[l].get(), ops[l]->a.get(), scale});
+      xfer.push_back({&lists_.m2l[l], lists_.m2l[l].src.get(), sst.m[l].get(), st.l[l].get(), ops[l]->a.get(), scale});
     }
     if (L >= 2) launch_m2l_batch<TP, double>(ops[2]->op, xfer, vl, false, kp, kp, stream_);
   }
@@ -460,7 +547,7 @@ void FmmEvaluator::run_fmm(double* d_out) {
   st.y_far.resize(static_cast<size_t>(n_trg()));
   launch_l2p<double>(P, tt, st.l[L].get(), tpts64, st.y_far.get(), stream_);
   record(5);
-  launch_p2p<TP>(rbf_, tt, src_tree_, lists_, tp2p, sp2p, st.y_far.get(), d_out, stream_);
+  launch_p2p<TP>(rbf_, tt, stree, lists_, tp2p, sp2p, st.y_far.get(), d_out, stream_);
   record(6);
 }
 
@@ -544,7 +631,8 @@ void FmmEvaluator::choose_config() {
 void FmmEvaluator::evaluate(double* out, int64_t n, bool on_device) {
   int64_t nt = n_trg();
   if (n != nt) throw_invalid("output size must equal the number of target points");
-  if (!have_weights_ && n_src_ > 0) throw_runtime("weights are not set");
+  const FmmEvaluator& src = S();
+  if (!src.have_weights_ && src.n_src_ > 0) throw_runtime("weights are not set");
   if (!config_valid_) choose_config();
   double* d_out = out;
   DevBuf<double> tmp;
@@ -555,13 +643,13 @@ void FmmEvaluator::evaluate(double* out, int64_t n, bool on_device) {
   record(0);
   stats_ = FmmStats{};
   if (nt > 0) {
-    if (n_src_ == 0) {
+    if (src.n_src_ == 0) {
       PAM_HIP(hipMemsetAsync(d_out, 0, sizeof(double) * nt, stream_));
       for (int i = 1; i <= 6; i++) record(i);
       stats_.direct = 1;
     } else if (use_direct()) {
       for (int i = 1; i <= 5; i++) record(i);
-      direct_sum_f64(rbf_, src_pts_.get(), w_.get(), n_src_, d_target_points(), nt, d_out, false,
+      direct_sum_f64(rbf_, src.src_pts_.get(), src.w_.get(), src.n_src_, d_target_points(), nt, d_out, false,
                      direct_scratch_, stream_);
       record(6);
       stats_.direct = 1;
@@ -575,8 +663,8 @@ void FmmEvaluator::evaluate(double* out, int64_t n, bool on_device) {
   }
   stats_.precision = cfg_.precision;
   stats_.order = cfg_.order;
-  stats_.leaf_level = stats_.direct ? -1 : src_tree_.level;
-  stats_.n_src_leaves = stats_.direct ? 0 : src_tree_.n_leaves();
+  stats_.leaf_level = stats_.direct ? -1 : src.src_tree_.level;
+  stats_.n_src_leaves = stats_.direct ? 0 : src.src_tree_.n_leaves();
   stats_.n_trg_leaves = stats_.direct ? 0 : target_tree().n_leaves();
   stats_pending_ = true;
   if (!on_device) {
@@ -609,11 +697,12 @@ void FmmEvaluator::count_work(int64_t* p2p_pairs, int64_t* m2l_pairs) {
   ensure_trees();
   PAM_HIP(hipStreamSynchronize(stream_));
   const Tree& tt = target_tree();
-  int L = src_tree_.level;
-  int64_t nt = tt.n_leaves(), ns = src_tree_.n_leaves();
+  const Tree& stree = S().src_tree_;
+  int L = stree.level;
+  int64_t nt = tt.n_leaves(), ns = stree.n_leaves();
   std::vector<int32_t> tls(nt + 1), sls(ns + 1), p2p(static_cast<size_t>(nt) * 27);
   PAM_HIP(hipMemcpy(tls.data(), tt.leaf_start.get(), tls.size() * 4, hipMemcpyDeviceToHost));
-  PAM_HIP(hipMemcpy(sls.data(), src_tree_.leaf_start.get(), sls.size() * 4, hipMemcpyDeviceToHost));
+  PAM_HIP(hipMemcpy(sls.data(), stree.leaf_start.get(), sls.size() * 4, hipMemcpyDeviceToHost));
   PAM_HIP(hipMemcpy(p2p.data(), lists_.p2p.get(), p2p.size() * 4, hipMemcpyDeviceToHost));
   for (int64_t t = 0; t < nt; t++) {
     int64_t ssum = 0;

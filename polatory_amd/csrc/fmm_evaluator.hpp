@@ -72,12 +72,21 @@ class FmmEvaluator {
   void evaluate(double* out, int64_t n, bool on_device);
 
   hipStream_t stream() const { return stream_; }
+  const RbfDesc& rbf() const { return rbf_; }
   const FmmStats& stats();
   const Tree& source_tree() const { return src_tree_; }
   const Tree& target_tree() const { return symmetric_ ? src_tree_ : trg_tree_; }
   const Lists& lists() const { return lists_; }
   // Builds trees/lists for the current configuration without evaluating.
   void prepare_trees();
+  // Builds the source tree and the multipole expansions now, so that other
+  // evaluators can share them (attach_source). Synchronizes the stream.
+  void prepare_source();
+  // Evaluates against the source side (points, weights, tree, expansions, box,
+  // configuration) of a prepared evaluator instead of an own one; this
+  // evaluator then owns only its targets. The owner must stay unchanged and
+  // alive while attached. Targets must lie in the owner's box. nullptr detaches.
+  void attach_source(const FmmEvaluator* owner);
   const FmmConfig& active_config() const { return cfg_; }
   int64_t n_src() const { return n_src_; }
   int64_t n_trg() const { return symmetric_ ? n_src_ : n_trg_; }
@@ -92,6 +101,10 @@ class FmmEvaluator {
   void ensure_box();
   void ensure_trees();
   bool use_direct() const;
+  // The evaluator that holds the source side (this one unless attached).
+  const FmmEvaluator& S() const { return so_ ? *so_ : *this; }
+  template <class TP>
+  void prepare_source_expansions();
   template <class TP>
   void run_fmm(double* d_out);
   void invalidate_expansions();
@@ -123,6 +136,7 @@ class FmmEvaluator {
   DevBuf<double> direct_scratch_;
 
   FmmState st_;
+  const FmmEvaluator* so_ = nullptr;  // shared source side (attach_source)
 
   hipEvent_t ev_[10] = {};
   FmmStats stats_;

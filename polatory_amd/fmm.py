@@ -48,6 +48,15 @@ class FmmConfig:
         c.svd_tolerance = self.svd_tol
         return c
 
+    @staticmethod
+    def from_c(c):
+        return FmmConfig({0: "fp64", 1: "mixed"}[c.precision], c.order, c.leaf_level,
+                         {0: "auto", 1: "direct", 2: "fmm"}[c.mode], c.svd_tolerance)
+
+    def __repr__(self):
+        return (f"FmmConfig({self.precision!r}, order={self.order}, leaf_level={self.leaf_level}, "
+                f"mode={self.mode!r})")
+
 
 class FmmEvaluator:
     """Source-target evaluator (FmmGenericEvaluatorBase<3>)."""
