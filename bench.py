@@ -78,6 +78,129 @@ def poly(c, pts):
     return c[0] + pts @ c[1:]
 
 
+def sample_f(p):
+    """Data of the fit workloads: f(x, y, z) = sin(3x) cos(2y) + 0.5 z^2."""
+    return np.sin(3 * p[:, 0]) * np.cos(2 * p[:, 1]) + 0.5 * p[:, 2] ** 2
+
+
+def ellipsoid_points(n_surf, seed=8):
+    """333,334 area-uniform points on the ellipsoid (1, 0.7, 0.5) (rejection on the
+    sphere's image) with value 0, plus the same points offset by +-0.01 along the
+    analytic normal with values +-0.01."""
+    a = np.array([1.0, 0.7, 0.5])
+    rng = np.random.default_rng(seed)
+    gmax = max(a[1] * a[2], a[0] * a[2], a[0] * a[1])
+    got, parts = 0, []
+    while got < n_surf:
+        u = rng.normal(size=(2 * n_surf, 3))
+        u /= np.linalg.norm(u, axis=1)[:, None]
+        g = np.sqrt((a[1] * a[2] * u[:, 0]) ** 2 + (a[0] * a[2] * u[:, 1]) ** 2 + (a[0] * a[1] * u[:, 2]) ** 2)
+        keep = u[rng.uniform(size=len(u)) * gmax < g] * a
+        parts.append(keep)
+        got += len(keep)
+    p = np.concatenate(parts)[:n_surf]
+    nrm = p / a ** 2
+    nrm /= np.linalg.norm(nrm, axis=1)[:, None]
+    pts = np.concatenate([p, p + 0.01 * nrm, p - 0.01 * nrm])
+    vals = np.concatenate([np.zeros(n_surf), np.full(n_surf, 0.01), np.full(n_surf, -0.01)])
+    return pts, vals
+
+
+def fit_section(pa):
+    """1,000,002-point ellipsoid fit, biharmonic3d degree 1, absolute tolerance 1e-5:
+    mixed-precision operator (FP32 P2P, FP32-MFMA M2L, FP64 refinement) vs FP64
+    everywhere (operator and RAS level updates as FP64 FMM of the same order).
+    The final residual is recomputed in FP64 at every site by the solver."""
+    pts, vals = ellipsoid_points(333_334)
+    out = {"workload": "ellipsoid surface fit, 1,000,002 points, bh3 degree 1, tol 1e-5 absolute"}
+    for name, opt in (("mixed", pa.FitOptions()),
+                      ("fp64", pa.FitOptions(op=pa.FmmConfig("fp64", 8, -1, "auto"),
+                                             level=pa.FmmConfig("fp64", 8, -1, "auto"),
+                                             refine_rtol=0.0))):
+        ip = pa.Interpolant(pa.Model(pa.Biharmonic3D(), poly_degree=1))
+        t0 = time.perf_counter()
+        ip.fit(pts, vals, 1e-5, max_iter=100, options=opt)
+        wall = time.perf_counter() - t0
+        st = ip.fit_stats()
+        out[name] = {"wall_s": wall, "iterations": st["iterations"], "restarts": st["restarts"],
+                     "fp64_residual": st["residual"], "solve_s": st["seconds"],
+                     "precond_setup_s": st["precond_setup_seconds"], "ras_levels": st["n_levels"],
+                     "factor_gb": st["factor_bytes"] / 1e9}
+        log("fit", name, json.dumps(out[name]))
+        del ip
+    out["speedup_vs_fp64"] = out["fp64"]["wall_s"] / out["mixed"]["wall_s"]
+    return out
+
+
+def serve_section(pa, n_models, n_requests, rate):
+    """32 resident models fitted to f on N ~ log-uniform[1e4, 1e6] U[0,1]^3 sites
+    (bh3 or th3, 50/50, tol 1e-4), Poisson arrivals at `rate` req/s, each
+    request on a uniformly chosen model with log-uniform[1, 1e6] U[0,1]^3 query
+    points, 1e-6 relative accuracy, admission by HBM budget, 4 slots."""
+    rng_n = np.random.default_rng(12)
+    rng_k = np.random.default_rng(13)
+    sizes = np.exp(rng_n.uniform(np.log(1e4), np.log(1e6), n_models)).astype(int)
+    kinds = rng_k.integers(0, 2, n_models)
+    srv = pa.Server(n_slots=4, rel_accuracy=1e-6)
+    ips, mids = [], []
+    t0 = time.perf_counter()
+    for i, (n, k) in enumerate(zip(sizes, kinds)):
+        p = np.random.default_rng(1000 + i).uniform(0, 1, (n, 3))
+        rbf = pa.Biharmonic3D() if k == 0 else pa.Triharmonic3D()
+        ip = pa.Interpolant(pa.Model(rbf, poly_degree=1))
+        ip.fit(p, sample_f(p), 1e-4)
+        mids.append(srv.add_model(ip))
+        ips.append((ip, rbf))
+    t_models = time.perf_counter() - t0
+    resident = sum(srv.model_config(m)[1] for m in mids)
+    log(f"serve: {n_models} models fitted and resident in {t_models:.1f} s ({resident / 2**30:.2f} GiB)")
+    rng_a = np.random.default_rng(14)
+    rng_q = np.random.default_rng(15)
+    arrivals = np.cumsum(rng_a.exponential(1.0 / rate, n_requests))
+    models = rng_q.integers(0, n_models, n_requests)
+    counts = np.exp(rng_q.uniform(0.0, np.log(1e6), n_requests)).astype(int)
+    queries = [rng_q.uniform(0, 1, (c, 3)) for c in counts]
+    # warm up: every model once, and every slot once at the largest request size
+    # (first-use device allocations), then measure
+    for m in range(n_models):
+        srv.wait(srv.submit(mids[m], queries[0][:1]))
+    big = np.random.default_rng(16).uniform(0, 1, (1_000_000, 3))
+    for r in [srv.submit(mids[int(np.argmax(sizes))], big) for _ in range(4)]:
+        srv.wait(r)
+    srv.reset_stats()
+    rids = []
+    t_start = time.perf_counter()
+    for i in range(n_requests):
+        delay = t_start + arrivals[i] - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        rids.append(srv.submit(mids[models[i]], queries[i]))
+    outs = {}
+    check = set(range(0, n_requests, max(1, n_requests // 20)))
+    for i, r in enumerate(rids):
+        o = srv.wait(r)
+        if i in check:
+            outs[i] = o
+    st = srv.stats()
+    # accuracy of a sample of answers vs the FP64 direct sum on the GPU
+    worst = 0.0
+    for i, o in outs.items():
+        ip, rbf = ips[models[i]]
+        q = queries[i][:1000]
+        w = ip.weights()
+        n = ip.num_centers()
+        ref = pa.direct_evaluate(rbf, ip.centers(), w[:n], q) + w[n] + q @ w[n + 1:n + 4]
+        worst = max(worst, np.abs(o[:len(q)] - ref).max() / np.abs(ref).max())
+    srv.close()
+    return {"workload": f"{n_models} resident bh3/th3 models (N log-uniform 1e4..1e6), "
+                        f"{n_requests} Poisson requests at {rate:g}/s, 1..1e6 query points each",
+            "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
+            "max_ms": st["max_ms"], "mean_queue_ms": st["mean_queue_ms"],
+            "points_per_s": st["points_per_s"], "completed": st["completed"], "failed": st["failed"],
+            "max_in_flight": st["max_in_flight"], "resident_gib": resident / 2**30,
+            "max_rel_err_sampled": worst, "models_ready_s": t_models}
+
+
 def time_steps(de, steps, warmup):
     for _ in range(warmup):
         de.step()
@@ -101,6 +224,11 @@ def main():
     ap.add_argument("--order64", type=int, default=7)
     ap.add_argument("--leaf-level", type=int, default=-1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fit", action="store_true", help="skip the 1M-point fit comparison")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
+    ap.add_argument("--serve-models", type=int, default=32)
+    ap.add_argument("--serve-requests", type=int, default=1000)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--only", choices=("both", "mixed", "fp64"), default="both",
                     help="run one form only (profiling runs); the JSON line needs both")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -179,7 +307,7 @@ def main():
     if not args.no_cpu:
         import oracle
         cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        n_cpu = 2000
+        n_cpu = 40000
         t0 = time.perf_counter()
         oracle.direct_rbf_sum(oracle.make_rbf(oracle.BH3), src, w, trg[:n_cpu])
         dt = time.perf_counter() - t0
@@ -187,8 +315,12 @@ def main():
                "sample": f"oracle FP64 direct O(N*M) sum, {N_SRC} sources x {n_cpu} targets "
                          f"({dt:.1f} s), OpenMP over {cores} threads"}
 
+    fit = None if args.no_fit else fit_section(pa)
+    serve = None if args.no_serve else serve_section(pa, args.serve_models, args.serve_requests,
+                                                     args.serve_rate)
+
     out = {
-        "metric": "FMM evals/s at 1M pts, <=1e-6 rel err (mixed vs FP64 GPU)",
+        "metric": "FMM evals/s at 1M pts, <=1e-6 rel err (mixed vs FP64 GPU); fit s; serve p99 ms",
         "value": N_TRG / m["sec"],
         "unit": "evals/s",
         "n_gpus": 1,
@@ -211,6 +343,8 @@ def main():
         "stage_ms": stage,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "fit": fit,
+        "serve": serve,
     }
     print(json.dumps(out), flush=True)
 

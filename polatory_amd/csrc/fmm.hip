@@ -6,7 +6,11 @@
 //               vectors of that octant reduced in registers
 //   P2P       : one workgroup per target leaf, the 27 neighbour leaves staged
 //               through LDS, targets x source-splits mapped onto 256 lanes
+#include <array>
 #include <cmath>
+#include <map>
+#include <tuple>
+#include <algorithm>
 #include <type_traits>
 #include <vector>
 
@@ -352,6 +356,7 @@ p2p_kernel(const int32_t* __restrict__ tls, const int32_t* __restrict__ sls,
   __shared__ T nb_off[27][3];
   __shared__ int nb_count;
   __shared__ T red[kThreads];
+  __shared__ T red2[kThreads];
   const int leaf = blockIdx.x;
   const int ts = tls[leaf], te = tls[leaf + 1];
   const int nt = te - ts;
@@ -375,16 +380,21 @@ p2p_kernel(const int32_t* __restrict__ tls, const int32_t* __restrict__ sls,
   __syncthreads();
   const int nnb = nb_count;
   const int total = nb_start[nnb];
-  int tpad = 1;
-  while (tpad < nt && tpad < kThreads) tpad <<= 1;
-  const int nsplit = kThreads / tpad;
-  const int ti = threadIdx.x & (tpad - 1);
-  const int si = threadIdx.x / tpad;
-  for (int c0 = 0; c0 < nt; c0 += tpad) {
-    const bool valid = c0 + ti < nt;
-    V4<T> tq{T(0), T(0), T(0), T(0)};
-    if (valid) tq = tpts[ts + c0 + ti];
-    T acc = T(0);
+  // Every thread carries two targets (ti and ti + tp), so each source read
+  // from LDS serves two pairs; tp target slots x nsplit source splits fill
+  // the 256 lanes, and a leaf of up to 512 points takes a single pass.
+  const int half = (nt + 1) / 2;
+  int tp = 1;
+  while (tp < half && tp < kThreads) tp <<= 1;
+  const int nsplit = kThreads / tp;
+  const int ti = threadIdx.x & (tp - 1);
+  const int si = threadIdx.x / tp;
+  for (int c0 = 0; c0 < nt; c0 += 2 * tp) {
+    const bool v0 = c0 + ti < nt, v1 = c0 + tp + ti < nt;
+    V4<T> t0{T(0), T(0), T(0), T(0)}, t1{T(0), T(0), T(0), T(0)};
+    if (v0) t0 = tpts[ts + c0 + ti];
+    if (v1) t1 = tpts[ts + c0 + tp + ti];
+    T acc0 = T(0), acc1 = T(0);
     for (int cb = 0; cb < total; cb += CAP) {
       const int cnt = min(CAP, total - cb);
       __syncthreads();
@@ -401,18 +411,31 @@ p2p_kernel(const int32_t* __restrict__ tls, const int32_t* __restrict__ sls,
       __syncthreads();
       for (int j = si; j < cnt; j += nsplit) {
         V4<T> q = sp[j];
-        T dx = tq.x - q.x, dy = tq.y - q.y, dz = tq.z - q.z;
-        acc += q.w * phi_t<T, KIND>(dx * dx + dy * dy + dz * dz, pa, pb);
+        T dx = t0.x - q.x, dy = t0.y - q.y, dz = t0.z - q.z;
+        T ex = t1.x - q.x, ey = t1.y - q.y, ez = t1.z - q.z;
+        acc0 += q.w * phi_t<T, KIND>(dx * dx + dy * dy + dz * dz, pa, pb);
+        acc1 += q.w * phi_t<T, KIND>(ex * ex + ey * ey + ez * ez, pa, pb);
       }
     }
-    red[threadIdx.x] = acc;
+    red[threadIdx.x] = acc0;
+    red2[threadIdx.x] = acc1;
     __syncthreads();
-    if (si == 0 && valid) {
-      T s = T(0);
-      for (int k = 0; k < nsplit; k++) s += red[ti + k * tpad];
-      int i = ts + c0 + ti;
-      double far = y_far ? y_far[i] : 0.0;
-      out[tperm[i]] = far + static_cast<double>(s);
+    if (si == 0) {
+      T s0 = T(0), s1 = T(0);
+      for (int k = 0; k < nsplit; k++) {
+        s0 += red[ti + k * tp];
+        s1 += red2[ti + k * tp];
+      }
+      if (v0) {
+        int i = ts + c0 + ti;
+        double far = y_far ? y_far[i] : 0.0;
+        out[tperm[i]] = far + static_cast<double>(s0);
+      }
+      if (v1) {
+        int i = ts + c0 + tp + ti;
+        double far = y_far ? y_far[i] : 0.0;
+        out[tperm[i]] = far + static_cast<double>(s1);
+      }
     }
     __syncthreads();
   }
@@ -437,6 +460,7 @@ struct Mfma<double> {
 };
 
 constexpr int kM2LKc = 256;  // reduction chunk staged in LDS
+constexpr int kM2LBCols = 32;  // columns per workgroup of m2l_batch_kernel
 
 // All levels of one M2L pass in one launch: column blocks of every level are
 // concatenated along grid.x, so the few blocks of the coarse levels run
@@ -448,9 +472,9 @@ m2l_batch_kernel(M2LBatch<TM, TS> b) {
   constexpr int LDB = kM2LKc + 4;
   constexpr int NA = kM2LKc / 16;  // operator fragments per lane and chunk
   constexpr int NB = kM2LKc / 32;  // staged multipole vectors per thread and chunk
-  __shared__ V4<TM> bs4[kM2LCols * LDB / 4];
-  __shared__ int cells[kM2LCols];
-  __shared__ int srcs[kM2LCols];
+  __shared__ V4<TM> bs4[kM2LBCols * LDB / 4];
+  __shared__ int cells[kM2LBCols];
+  __shared__ int srcs[kM2LBCols];
   TM* bs = reinterpret_cast<TM*>(bs4);
   int lv = 0;
   while (lv + 1 < b.n && static_cast<int>(blockIdx.x) >= b.blk_begin[lv + 1]) lv++;
@@ -460,9 +484,9 @@ m2l_batch_kernel(M2LBatch<TM, TS> b) {
   const int rpad = b.rpad, kpad = b.kpad;
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int row0 = blockIdx.y * 64 + wave * 16;
-  const int oct = b.blk_oct[lv] ? b.blk_oct[lv][blk] : 0;
+  const int oct = b.blk_oct[lv] ? b.blk_oct[lv][blk * kM2LBCols / kM2LCols] : 0;
   const int nv = src_slots ? kM2LSlots : 1;
-  if (threadIdx.x < kM2LCols) cells[threadIdx.x] = b.col_cell[lv][blk * kM2LCols + threadIdx.x];
+  if (threadIdx.x < kM2LBCols) cells[threadIdx.x] = b.col_cell[lv][blk * kM2LBCols + threadIdx.x];
   acc_t acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
   // The MFMA accumulators are flushed into FP64 every 64 reduction steps (FP32)
   // or once at the end (FP64): one long FP32 running sum over 189 x P^3 terms
@@ -481,13 +505,13 @@ m2l_batch_kernel(M2LBatch<TM, TS> b) {
   for (int sidx = 0; sidx < nv; sidx++) {
     const int v = src_slots ? b.vlist[oct * kM2LSlots + sidx] : 0;
     __syncthreads();
-    if (threadIdx.x < kM2LCols) {
+    if (threadIdx.x < kM2LBCols) {
       int c = cells[threadIdx.x];
       srcs[threadIdx.x] = c < 0 ? -1 : (src_slots ? src_slots[static_cast<int64_t>(c) * kM2LSlots + sidx] : c);
     }
     __syncthreads();
     bool any = false;  // skip transfer vectors with no source in this block
-    for (int k = 0; k < kM2LCols; k++) any |= srcs[k] >= 0;
+    for (int k = 0; k < kM2LBCols; k++) any |= srcs[k] >= 0;
     if (!any) continue;
     const TM* av = b.a[lv] + static_cast<int64_t>(v) * b.per_v;
     const int src = srcs[col];
@@ -565,6 +589,396 @@ m2l_batch_kernel(M2LBatch<TM, TS> b) {
     }
   }
 }
+
+// ---- dense M2L on MFMA: 128 operator rows x 64 target cells per workgroup ----
+//
+// Workgroup = 4 waves in a 2 x 2 grid, each wave 64 rows x 32 cells (4 x 2
+// MFMA tiles of 16 x 16). The reduction runs over the active slots of the
+// workgroup's split (transfer vectors with at least one source cell among its
+// 64 cells) and, inside a slot, over chunks of kK operator columns. Operator
+// and multipole chunks are double-buffered in LDS: the global loads of chunk
+// t+1 are in flight while chunk t is multiplied.
+constexpr int kM2LR = 128;
+constexpr int kM2LC = kM2LCols;
+static_assert(kM2LC == 64, "the dense M2L kernel maps 64 columns onto 2 x 32-column waves");
+
+template <class TM>
+struct M2LTile {
+  static constexpr int kK = sizeof(TM) == 4 ? 32 : 16;  // reduction chunk
+  static constexpr int kKB = kK / 16;                    // 16-wide fragment blocks per chunk
+  // Row / column pitches padded by 4 elements: keeps V4 fragment reads aligned
+  // and spreads the 16 rows of a fragment over distinct LDS banks.
+  static constexpr int kAP = 16 + 4;
+  static constexpr int kBP = kK + 4;
+  static constexpr int kUpr = static_cast<int>(sizeof(TM));  // 16-byte units per 16-element row
+  static constexpr int kAUnits = kKB * kM2LR * kUpr;         // 1024
+  static constexpr int kBUnitsPerCol = kK * static_cast<int>(sizeof(TM)) / 16;  // 8
+  static_assert(kAUnits == 4 * kThreads, "4 operator loads per thread and chunk");
+  static_assert(kBUnitsPerCol == 8, "8 threads per column, 2 columns per thread");
+};
+
+template <class TM>
+struct M2LDenseArgs {
+  int n;
+  int r, rpad, kpad, nsplit;
+  int64_t per_v;
+  int64_t cells_total;
+  const int32_t* vlist;
+  const int4* sched;
+  double* partial;
+  const int32_t* col_cell[kMaxM2LLevels];
+  const int32_t* blk_oct[kMaxM2LLevels];
+  const int32_t* src_slots[kMaxM2LLevels];
+  const TM* in[kMaxM2LLevels];
+  double* out[kMaxM2LLevels];
+  const TM* a[kMaxM2LLevels];
+  double scale[kMaxM2LLevels];
+  int64_t cell_off[kMaxM2LLevels];
+};
+
+template <class TM>
+__global__ void __launch_bounds__(kThreads)
+m2l_dense_kernel(M2LDenseArgs<TM> b) {
+  using Tl = M2LTile<TM>;
+  using acc_t = typename Mfma<TM>::acc_t;
+  constexpr int kK = Tl::kK, kKB = Tl::kKB, kAP = Tl::kAP, kBP = Tl::kBP, kUpr = Tl::kUpr;
+  __shared__ __align__(16) TM as[2][kKB][kM2LR][kAP];
+  __shared__ __align__(16) TM bs[2][kM2LC][kBP];
+  __shared__ int cells[kM2LC];
+  __shared__ uint32_t mask[6];
+  __shared__ int act_v[kM2LSlots];
+  __shared__ int act_s[kM2LSlots];
+  __shared__ int nact_s;
+
+  const int4 w = b.sched[blockIdx.x];
+  if (w.x < 0) return;
+  const int lv = w.x, cb = w.y, rb = w.z, sp = w.w;
+  const int32_t* __restrict__ src_slots = b.src_slots[lv];
+  const TM* __restrict__ in = b.in[lv];
+  const int kpad = b.kpad, rpad = b.rpad;
+  const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+  const int oct = b.blk_oct[lv][cb];
+  const int s_begin = sp * kM2LSlots / b.nsplit, s_end = (sp + 1) * kM2LSlots / b.nsplit;
+
+  if (tid < kM2LC) cells[tid] = b.col_cell[lv][static_cast<int64_t>(cb) * kM2LC + tid];
+  if (tid < 6) mask[tid] = 0u;
+  __syncthreads();
+  // Active slots of this split: at least one of the 64 cells has a source there.
+  for (int idx = tid; idx < kM2LC * (s_end - s_begin); idx += kThreads) {
+    int c = idx / (s_end - s_begin), sl = s_begin + idx % (s_end - s_begin);
+    int cell = cells[c];
+    if (cell >= 0 && src_slots[static_cast<int64_t>(cell) * kM2LSlots + sl] >= 0)
+      atomicOr(&mask[sl >> 5], 1u << (sl & 31));
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int na = 0;
+    for (int sl = s_begin; sl < s_end; sl++) {
+      if (mask[sl >> 5] & (1u << (sl & 31))) {
+        act_s[na] = sl;
+        act_v[na] = b.vlist[oct * kM2LSlots + sl];
+        na++;
+      }
+    }
+    nact_s = na;
+  }
+  __syncthreads();
+  const int nact = nact_s;
+  const int nkc = (kpad + kK - 1) / kK;
+  const int total = nact * nkc;
+
+  // Loader roles: operator units tid + 256 q (q < 4); multipole columns
+  // bc0 = tid / 8 and bc1 = 32 + tid / 8, 16-byte piece bq = tid % 8.
+  const int bc0 = tid / 8, bc1 = 32 + tid / 8, bq = tid % 8;
+  const int cell0 = cells[bc0], cell1 = cells[bc1];
+  // Two register sets: the chunk two steps ahead is in flight while the next
+  // one waits to be stored into the free LDS buffer.
+  uint4 ra0[4], rb0[2], ra1[4], rb1[2];
+  int src0 = -1, src1 = -1;  // sources of the loader's columns for the slot being loaded
+
+  auto load_src = [&](int ia) {
+    int sl = act_s[ia];
+    src0 = cell0 >= 0 ? src_slots[static_cast<int64_t>(cell0) * kM2LSlots + sl] : -1;
+    src1 = cell1 >= 0 ? src_slots[static_cast<int64_t>(cell1) * kM2LSlots + sl] : -1;
+  };
+  auto load_chunk = [&](int t, uint4 (&areg)[4], uint4 (&breg)[2]) {
+    if ((t % nkc) == 0) load_src(t / nkc);
+    const int ia = t / nkc, kc = t % nkc;
+    const int k0 = kc * kK;
+    const TM* av = b.a[lv] + static_cast<int64_t>(act_v[ia]) * b.per_v;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      int u = tid + q * kThreads;
+      int seg = u / (kM2LR * kUpr), rem = u % (kM2LR * kUpr);
+      int row = rem / kUpr, pc = rem % kUpr;
+      int kb = (k0 >> 4) + seg;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (kb * 16 < kpad) {
+        const TM* g = av + ((static_cast<int64_t>(kb) * rpad + rb * kM2LR + row) << 4);
+        v = reinterpret_cast<const uint4*>(g)[pc];
+      }
+      areg[q] = v;
+    }
+    const int kel = k0 + bq * (16 / static_cast<int>(sizeof(TM)));
+    breg[0] = make_uint4(0u, 0u, 0u, 0u);
+    breg[1] = make_uint4(0u, 0u, 0u, 0u);
+    if (kel < kpad) {
+      if (src0 >= 0) breg[0] = *reinterpret_cast<const uint4*>(in + static_cast<int64_t>(src0) * kpad + kel);
+      if (src1 >= 0) breg[1] = *reinterpret_cast<const uint4*>(in + static_cast<int64_t>(src1) * kpad + kel);
+    }
+  };
+  auto store_chunk = [&](int buf, const uint4 (&areg)[4], const uint4 (&breg)[2]) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      int u = tid + q * kThreads;
+      int seg = u / (kM2LR * kUpr), rem = u % (kM2LR * kUpr);
+      int row = rem / kUpr, pc = rem % kUpr;
+      *reinterpret_cast<uint4*>(&as[buf][seg][row][pc * (16 / static_cast<int>(sizeof(TM)))]) = areg[q];
+    }
+    *reinterpret_cast<uint4*>(&bs[buf][bc0][bq * (16 / static_cast<int>(sizeof(TM)))]) = breg[0];
+    *reinterpret_cast<uint4*>(&bs[buf][bc1][bq * (16 / static_cast<int>(sizeof(TM)))]) = breg[1];
+  };
+
+  acc_t acc[4][2];
+  double dacc[4][2][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      acc[i][j] = acc_t{0, 0, 0, 0};
+#pragma unroll
+      for (int q = 0; q < 4; q++) dacc[i][j][q] = 0.0;
+    }
+  auto flush = [&]() {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          dacc[i][j][q] += static_cast<double>(acc[i][j][q]);
+          acc[i][j][q] = TM(0);
+        }
+  };
+
+  const int wr = wave >> 1, wc = wave & 1;
+  auto compute = [&](int t) {
+    const int buf = t & 1;
+    const int kc = t % nkc;
+#pragma unroll
+    for (int kb = 0; kb < kKB; kb++) {
+      if (kc * kK + kb * 16 < kpad) {
+        using V4T = V4<TM>;
+        V4T af[4], bf[2];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+          af[i] = *reinterpret_cast<const V4T*>(&as[buf][kb][wr * 64 + i * 16 + (lane & 15)][(lane >> 4) * 4]);
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+          bf[j] = *reinterpret_cast<const V4T*>(&bs[buf][wc * 32 + j * 16 + (lane & 15)][kb * 16 + (lane >> 4) * 4]);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int j = 0; j < 2; j++) {
+            acc[i][j] = Mfma<TM>::op(af[i].x, bf[j].x, acc[i][j]);
+            acc[i][j] = Mfma<TM>::op(af[i].y, bf[j].y, acc[i][j]);
+            acc[i][j] = Mfma<TM>::op(af[i].z, bf[j].z, acc[i][j]);
+            acc[i][j] = Mfma<TM>::op(af[i].w, bf[j].w, acc[i][j]);
+          }
+      }
+    }
+    // FP32 partial sums over at most 64 reduction steps are flushed into FP64.
+    if constexpr (sizeof(TM) == 4) {
+      if ((kc & 1) == 1 || kc == nkc - 1) flush();
+    }
+  };
+  // Step t: chunk t+1 sits in registers (cur), chunk t+2 is loaded into the
+  // other set (nxt) while chunk t is multiplied from LDS buffer t&1; then
+  // chunk t+1 goes to LDS buffer (t+1)&1.
+  auto step = [&](int t, uint4 (&cur_a)[4], uint4 (&cur_b)[2], uint4 (&nxt_a)[4], uint4 (&nxt_b)[2]) {
+    if (t + 2 < total) load_chunk(t + 2, nxt_a, nxt_b);
+    compute(t);
+    if (t + 1 < total) store_chunk((t + 1) & 1, cur_a, cur_b);
+    __syncthreads();
+  };
+  if (total > 0) {
+    load_chunk(0, ra1, rb1);
+    store_chunk(0, ra1, rb1);
+    if (total > 1) load_chunk(1, ra0, rb0);
+  }
+  __syncthreads();
+  for (int t = 0; t < total; t += 2) {
+    step(t, ra0, rb0, ra1, rb1);
+    if (t + 1 < total) step(t + 1, ra1, rb1, ra0, rb0);
+  }
+  if constexpr (sizeof(TM) == 8) flush();
+
+  // D layout (16 x 16), column lane % 16; register q holds row 4 (lane / 16) + q
+  // for the f32 instruction and row lane / 16 + 4 q for the f64 one.
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    const int cell = cells[wc * 32 + j * 16 + (lane & 15)];
+    if (cell < 0) continue;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        int row = rb * kM2LR + wr * 64 + i * 16 + (sizeof(TM) == 4 ? 4 * (lane >> 4) + q : (lane >> 4) + 4 * q);
+        if (row >= b.r) continue;
+        if (b.nsplit == 1) {
+          b.out[lv][static_cast<int64_t>(cell) * kpad + row] = dacc[i][j][q] * b.scale[lv];
+        } else {
+          b.partial[(static_cast<int64_t>(sp) * b.cells_total + b.cell_off[lv] + cell) * rpad + row] = dacc[i][j][q];
+        }
+      }
+  }
+}
+
+// out[cell][row] = scale * sum_s partial[s][cell][row], splits summed in order.
+struct M2LReduceArgs {
+  int n, r, rpad, kpad, nsplit;
+  int64_t cells_total;
+  const double* partial;
+  double* out[kMaxM2LLevels];
+  double scale[kMaxM2LLevels];
+  int64_t cell_off[kMaxM2LLevels + 1];
+};
+
+__global__ void m2l_reduce_kernel(M2LReduceArgs b) {
+  int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t gcell = idx / b.rpad;
+  int row = static_cast<int>(idx % b.rpad);
+  if (gcell >= b.cells_total || row >= b.r) return;
+  int lv = 0;
+  while (lv + 1 < b.n && gcell >= b.cell_off[lv + 1]) lv++;
+  double sum = 0.0;
+  for (int s = 0; s < b.nsplit; s++) sum += b.partial[(s * b.cells_total + gcell) * b.rpad + row];
+  b.out[lv][(gcell - b.cell_off[lv]) * b.kpad + row] = sum * b.scale[lv];
+}
+
+__global__ void convert_f64_f32_kernel(const double* __restrict__ in, float* __restrict__ out, int64_t n) {
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = static_cast<float>(in[i]);
+}
+
+}  // namespace
+
+void launch_convert_f64_f32(const double* in, float* out, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  convert_f64_f32_kernel<<<ceil_div(n, 256), 256, 0, s>>>(in, out, n);
+  PAM_LAUNCH_CHECK();
+}
+
+namespace {
+
+// Work items grouped by (octant, row block, split); groups dealt round-robin
+// onto the 8 XCDs (workgroup i runs on XCD i % 8), so the workgroups that
+// stream the same operator rows share that XCD's L2.
+void build_m2l_schedule(const std::vector<const M2LLevel*>& lvls, int nrowblk, M2LSchedule& sc,
+                        hipStream_t s) {
+  constexpr int kXcd = 8;
+  int64_t items = 0;
+  for (const M2LLevel* l : lvls) items += (l->n_cols / kM2LC) * nrowblk;
+  // Enough workgroups to fill 256 CUs about four times over.
+  int nsplit = items > 0 ? static_cast<int>((1024 + items - 1) / items) : 1;
+  sc.nsplit = std::clamp(nsplit, 1, 12);
+  std::map<std::tuple<int, int, int>, std::vector<std::array<int32_t, 4>>> groups;
+  for (size_t li = 0; li < lvls.size(); li++) {
+    const M2LLevel& l = *lvls[li];
+    for (int64_t cb = 0; cb < l.n_cols / kM2LC; cb++) {
+      int oct = l.h_blk_oct[cb];
+      for (int rb = 0; rb < nrowblk; rb++)
+        for (int sp = 0; sp < sc.nsplit; sp++)
+          groups[{oct, rb, sp}].push_back({static_cast<int32_t>(li), static_cast<int32_t>(cb), rb, sp});
+    }
+  }
+  std::vector<std::vector<std::array<int32_t, 4>>> per_xcd(kXcd);
+  int g = 0;
+  for (auto& kv : groups) {
+    auto& dst = per_xcd[g++ % kXcd];
+    dst.insert(dst.end(), kv.second.begin(), kv.second.end());
+  }
+  size_t len = 0;
+  for (auto& v : per_xcd) len = std::max(len, v.size());
+  sc.h.assign(len * kXcd * 4, -1);
+  for (size_t j = 0; j < len; j++)
+    for (int x = 0; x < kXcd; x++)
+      if (j < per_xcd[x].size())
+        for (int k = 0; k < 4; k++) sc.h[(j * kXcd + x) * 4 + k] = per_xcd[x][j][k];
+  sc.n_wg = static_cast<int>(len * kXcd);
+  sc.cells_total = 0;
+  for (const M2LLevel* l : lvls) sc.cells_total += l->n_cells;
+  sc.d.upload(sc.h.data(), sc.h.size(), s);
+  PAM_HIP(hipStreamSynchronize(s));  // the upload reads host memory
+  sc.valid = true;
+}
+
+}  // namespace
+
+template <class TM>
+void launch_m2l_dense(const M2LOperator& op, const std::vector<M2LDenseLevel<TM>>& levels,
+                      const int32_t* vlist, M2LSchedule& sched, DevBuf<double>& partial, hipStream_t s) {
+  if (op.rpad % kM2LR != 0) throw_runtime("M2L operator rows are not padded to 128");
+  if (op.kpad % 16 != 0) throw_runtime("M2L reduction length is not padded to 16");
+  M2LDenseArgs<TM> b{};
+  b.r = op.r;
+  b.rpad = op.rpad;
+  b.kpad = op.kpad;
+  b.per_v = static_cast<int64_t>(op.per_v);
+  b.vlist = vlist;
+  std::vector<const M2LLevel*> lvls;
+  int64_t off = 0;
+  for (const auto& l : levels) {
+    if (!l.lv || l.lv->n_cols == 0) continue;
+    if (static_cast<int64_t>(l.lv->h_blk_oct.size()) != l.lv->n_cols / kM2LC)
+      throw_runtime("M2L column blocks without octants");
+    if (b.n >= kMaxM2LLevels) throw_runtime("too many M2L levels");
+    int k = b.n++;
+    b.col_cell[k] = l.lv->col_cell.get();
+    b.blk_oct[k] = l.lv->blk_oct.get();
+    b.src_slots[k] = l.lv->src.get();
+    b.in[k] = l.in;
+    b.out[k] = l.out;
+    b.a[k] = l.a;
+    b.scale[k] = l.scale;
+    b.cell_off[k] = off;
+    off += l.lv->n_cells;
+    lvls.push_back(l.lv);
+  }
+  if (b.n == 0) return;
+  if (!sched.valid) build_m2l_schedule(lvls, op.rpad / kM2LR, sched, s);
+  if (sched.cells_total != off) throw_runtime("stale M2L schedule");
+  b.nsplit = sched.nsplit;
+  b.cells_total = sched.cells_total;
+  b.sched = reinterpret_cast<const int4*>(sched.d.get());
+  if (sched.nsplit > 1) {
+    partial.resize(static_cast<size_t>(sched.nsplit) * sched.cells_total * op.rpad);
+    b.partial = partial.get();
+  }
+  m2l_dense_kernel<TM><<<static_cast<unsigned>(sched.n_wg), kThreads, 0, s>>>(b);
+  PAM_LAUNCH_CHECK();
+  if (sched.nsplit > 1) {
+    M2LReduceArgs rb{};
+    rb.n = b.n;
+    rb.r = op.r;
+    rb.rpad = op.rpad;
+    rb.kpad = op.kpad;
+    rb.nsplit = sched.nsplit;
+    rb.cells_total = sched.cells_total;
+    rb.partial = partial.get();
+    for (int k = 0; k < b.n; k++) {
+      rb.out[k] = b.out[k];
+      rb.scale[k] = b.scale[k];
+      rb.cell_off[k] = b.cell_off[k];
+    }
+    rb.cell_off[b.n] = sched.cells_total;
+    int64_t work = sched.cells_total * op.rpad;
+    m2l_reduce_kernel<<<ceil_div(work, 256), 256, 0, s>>>(rb);
+    PAM_LAUNCH_CHECK();
+  }
+}
+
+namespace {
 
 template <class T>
 __global__ void build_m2l_dense_kernel(int p, int kind, double pa, double pb, int rpad, int kpad,
@@ -736,7 +1150,7 @@ void launch_m2l_batch(const M2LOperator& op, const std::vector<M2LLevelSpec<TM, 
     b.out[k] = l.out;
     b.a[k] = l.a;
     b.scale[k] = l.scale;
-    blocks += static_cast<int>(l.lv->n_cols / kM2LCols);
+    blocks += static_cast<int>(l.lv->n_cols / kM2LBCols);
   }
   b.blk_begin[b.n] = blocks;
   if (blocks == 0) return;
@@ -759,7 +1173,9 @@ void launch_m2l_batch(const M2LOperator& op, const std::vector<M2LLevelSpec<TM, 
                                    hipStream_t);                                                    \
   template void launch_m2l_batch<T, double>(const M2LOperator&,                                        \
                                             const std::vector<M2LLevelSpec<T, double>>&, const int32_t*, \
-                                            bool, int, int, hipStream_t);
+                                            bool, int, int, hipStream_t);                                 \
+  template void launch_m2l_dense<T>(const M2LOperator&, const std::vector<M2LDenseLevel<T>>&,       \
+                                    const int32_t*, M2LSchedule&, DevBuf<double>&, hipStream_t);
 
 PAM_INSTANTIATE(float)
 PAM_INSTANTIATE(double)

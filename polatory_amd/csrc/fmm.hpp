@@ -27,8 +27,8 @@ inline bool order_supported(int p) { return p >= kMinOrder && p <= kMaxOrder; }
 
 // Expansion stride: P^3 padded to 16 (the M2L reduction is blocked by 16).
 inline int kpad_of(int p) { return (p * p * p + 15) / 16 * 16; }
-// Row padding of the M2L operators: 64 rows per workgroup.
-inline int rpad_of(int r) { return (r + 63) / 64 * 64; }
+// Row padding of the M2L operators: 128 rows per workgroup.
+inline int rpad_of(int r) { return (r + 127) / 128 * 128; }
 
 // M2L operator for one level in MFMA fragment order: A[v][kb][row][16] with
 // v in [0, 343), kb in [0, kpad/16), row in [0, rpad); element (row, k) of
@@ -103,5 +103,35 @@ struct M2LBatch {
 template <class TM, class TS>
 void launch_m2l_batch(const M2LOperator& op, const std::vector<M2LLevelSpec<TM, TS>>& levels,
                       const int32_t* vlist, bool accumulate, int in_stride, int out_stride, hipStream_t s);
+
+// Dense M2L, all levels in one launch: out[cell][row] = scale * sum over the
+// cell's 189 slots of A_v * in[src], for the rows < r (expansion stride kpad).
+// in holds the multipoles in the MFMA type (an FP32 copy for the mixed form).
+template <class TM>
+struct M2LDenseLevel {
+  const M2LLevel* lv = nullptr;
+  const TM* in = nullptr;
+  double* out = nullptr;
+  const TM* a = nullptr;
+  double scale = 1.0;
+};
+
+// Work list of the dense M2L: one int4 (level, column block, row block, split)
+// per workgroup, ordered so that the workgroups that read the same operator
+// rows (same octant, row block and split of the slots) land on the same XCD.
+struct M2LSchedule {
+  std::vector<int32_t> h;
+  DevBuf<int32_t> d;
+  int n_wg = 0;
+  int nsplit = 1;
+  int64_t cells_total = 0;
+  bool valid = false;
+};
+
+template <class TM>
+void launch_m2l_dense(const M2LOperator& op, const std::vector<M2LDenseLevel<TM>>& levels,
+                      const int32_t* vlist, M2LSchedule& sched, DevBuf<double>& partial, hipStream_t s);
+
+void launch_convert_f64_f32(const double* in, float* out, int64_t n, hipStream_t s);
 
 }  // namespace pam

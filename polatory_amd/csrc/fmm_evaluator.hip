@@ -338,7 +338,10 @@ void FmmEvaluator::ensure_trees() {
       rebuilt = true;
       st_.t32_valid = st_.t64_valid = false;
     }
-    if (rebuilt || lists_.m2l.size() != static_cast<size_t>(L + 1)) build_lists(trg_tree_, stree, lists_, ws_, stream_);
+    if (rebuilt || lists_.m2l.size() != static_cast<size_t>(L + 1)) {
+      build_lists(trg_tree_, stree, lists_, ws_, stream_);
+      lists_gen_++;
+    }
     return;
   }
   int L = cfg_.leaf_level >= 0 ? cfg_.leaf_level : auto_leaf_level(n_src_, n_trg());
@@ -361,6 +364,7 @@ void FmmEvaluator::ensure_trees() {
   if (symmetric_) trg_tree_dirty_ = false;
   if (rebuilt || lists_.m2l.size() != static_cast<size_t>(L + 1)) {
     build_lists(target_tree(), src_tree_, lists_, ws_, stream_);
+    lists_gen_++;
   }
 }
 
@@ -399,6 +403,7 @@ void FmmEvaluator::attach_source(const FmmEvaluator* owner) {
   box_valid_ = false;
   trg_tree_dirty_ = true;
   lists_ = Lists{};
+  lists_gen_++;
   st_.t32_valid = st_.t64_valid = false;
   // Targets are transformed with the rbf's anisotropy: they must be set again.
   n_trg_ = 0;
@@ -438,6 +443,18 @@ void FmmEvaluator::prepare_source_expansions() {
     launch_p2m<double>(P, src_tree_, st.spts64.get(), st.m[L].get(), stream_);
     for (int l = L - 1; l >= 2; l--) launch_m2m<double>(P, src_tree_, l, st.m[l + 1].get(), st.m[l].get(), stream_);
     st.multipoles_valid = true;
+    st.m32_valid = false;
+  }
+  if constexpr (sizeof(TP) == 4) {
+    if (!st.m32_valid) {
+      st.m32.resize(L + 1);
+      for (int l = 2; l <= L; l++) {
+        size_t n = static_cast<size_t>(src_tree_.levels[l].n) * kp;
+        st.m32[l].resize(n);
+        launch_convert_f64_f32(st.m[l].get(), st.m32[l].get(), static_cast<int64_t>(n), stream_);
+      }
+      st.m32_valid = true;
+    }
   }
 }
 
@@ -455,7 +472,7 @@ void FmmEvaluator::run_fmm(double* d_out) {
   if (!so_) {
     prepare_source_expansions<TP>();
   } else if (!sst.multipoles_valid || sst.multipole_order != P || !sst.s64_valid ||
-             (sizeof(TP) == 4 && !sst.s32_valid)) {
+             (sizeof(TP) == 4 && (!sst.s32_valid || !sst.m32_valid))) {
     throw_runtime("the shared source side is not prepared (call prepare_source on the owner)");
   }
   if (st.l.size() != static_cast<size_t>(L + 1)) {
@@ -532,14 +549,28 @@ void FmmEvaluator::run_fmm(double* d_out) {
     std::vector<std::shared_ptr<OpEntry<TP>>> ops(L + 1);
     std::shared_ptr<OpEntry<TP>> unit_op;
     if (deg > 0) unit_op = get_op<TP>(P, rbf_, 1.0, stream_);
-    std::vector<M2LLevelSpec<TP, double>> xfer;
+    std::vector<M2LDenseLevel<TP>> xfer;
     for (int l = 2; l <= L; l++) {
       double h = box_.width / std::ldexp(1.0, l) / 2.0;
       ops[l] = deg > 0 ? unit_op : get_op<TP>(P, rbf_, h, stream_);
       double scale = deg > 0 ? std::pow(h, deg) : 1.0;
-      xfer.push_back({&lists_.m2l[l], lists_.m2l[l].src.get(), sst.m[l].get(), st.l[l].get(), ops[l]->a.get(), scale});
+      const TP* in;
+      if constexpr (sizeof(TP) == 4) in = sst.m32[l].get();
+      else in = sst.m[l].get();
+      xfer.push_back({&lists_.m2l[l], in, st.l[l].get(), ops[l]->a.get(), scale});
     }
-    if (L >= 2) launch_m2l_batch<TP, double>(ops[2]->op, xfer, vl, false, kp, kp, stream_);
+    if (L >= 2) {
+      if (st.m2l_sched_gen != lists_gen_ || st.m2l_sched.h.empty() ||
+          st.m2l_sched.nsplit < 1) {
+        st.m2l_sched.valid = false;
+      }
+      if (st.m2l_sched.valid && st.m2l_sched_rpad != ops[2]->op.rpad) st.m2l_sched.valid = false;
+      if (!st.m2l_sched.valid) {
+        st.m2l_sched_gen = lists_gen_;
+        st.m2l_sched_rpad = ops[2]->op.rpad;
+      }
+      launch_m2l_dense<TP>(ops[2]->op, xfer, vl, st.m2l_sched, st.m2l_partial, stream_);
+    }
   }
   record(3);
   for (int l = 3; l <= L; l++) launch_l2l<double>(P, tt, l, st.l[l - 1].get(), st.l[l].get(), stream_);

@@ -46,10 +46,15 @@ struct FmmState {
   DevBuf<V4<double>> spts64, tpts64;
   DevBuf<V4<float>> spts32, tpts32;
   std::vector<DevBuf<double>> m, l;    // per level, P^3 node values
+  std::vector<DevBuf<float>> m32;      // per level, FP32 copy of m (mixed-form M2L operand)
   std::vector<DevBuf<double>> mt, lt;  // per level, compressed coefficients
   DevBuf<double> y_far;
+  DevBuf<double> m2l_partial;          // split-reduction partial sums of the dense M2L
+  M2LSchedule m2l_sched;
+  uint64_t m2l_sched_gen = 0;
+  int m2l_sched_rpad = 0;
   bool s64_valid = false, t64_valid = false, s32_valid = false, t32_valid = false;
-  bool multipoles_valid = false;
+  bool multipoles_valid = false, m32_valid = false;
   int multipole_order = 0;
 };
 
@@ -137,6 +142,7 @@ class FmmEvaluator {
 
   FmmState st_;
   const FmmEvaluator* so_ = nullptr;  // shared source side (attach_source)
+  uint64_t lists_gen_ = 0;            // bumped whenever the interaction lists change
 
   hipEvent_t ev_[10] = {};
   FmmStats stats_;

@@ -3,8 +3,10 @@
 #include "ras.hpp"
 
 #include <algorithm>
+#include <parallel/algorithm>
 #include <array>
 #include <chrono>
+#include <cstdio>
 #include <cmath>
 #include <list>
 #include <numeric>
@@ -489,14 +491,61 @@ struct Bbox3 {
   }
 };
 
-void sort_by_axes(std::vector<int32_t>& ids, const std::vector<double>& pts, const std::array<int, 3>& ax) {
-  std::stable_sort(ids.begin(), ids.end(), [&](int32_t a, int32_t b) {
-    for (int k : ax) {
-      double pa = pts[3 * static_cast<size_t>(a) + k], pb = pts[3 * static_cast<size_t>(b) + k];
-      if (pa != pb) return pa < pb;
-    }
-    return false;
-  });
+// Positions of ids in lexicographic order of their coordinates along ax, ties
+// by position: the order a stable sort by coordinates gives. Keys are copied
+// into one contiguous array; large inputs use the OpenMP parallel sort
+// (deterministic, since the order is total).
+std::vector<int32_t> order_by_axes(const int32_t* ids, int64_t n, const std::vector<double>& pts,
+                                   const std::array<int, 3>& ax, bool parallel) {
+  struct Key {
+    double k[3];
+    int32_t pos;
+  };
+  std::vector<Key> keys(static_cast<size_t>(n));
+  for (int64_t i = 0; i < n; i++) {
+    const double* p = &pts[3 * static_cast<size_t>(ids[i])];
+    keys[i] = Key{{p[ax[0]], p[ax[1]], p[ax[2]]}, static_cast<int32_t>(i)};
+  }
+  auto less = [](const Key& a, const Key& b) {
+    if (a.k[0] != b.k[0]) return a.k[0] < b.k[0];
+    if (a.k[1] != b.k[1]) return a.k[1] < b.k[1];
+    if (a.k[2] != b.k[2]) return a.k[2] < b.k[2];
+    return a.pos < b.pos;
+  };
+  if (parallel && n >= (int64_t{1} << 16)) __gnu_parallel::sort(keys.begin(), keys.end(), less);
+  else std::sort(keys.begin(), keys.end(), less);
+  std::vector<int32_t> order(static_cast<size_t>(n));
+  for (int64_t i = 0; i < n; i++) order[i] = keys[i].pos;
+  return order;
+}
+
+// Like order_by_axes, but only as far as a split needs it: positions
+// [lo, hi) hold exactly the entries of those ranks, in order; the entries
+// below lo and at or above hi are the right sets in any order.
+std::vector<int32_t> order_for_split(const int32_t* ids, int64_t n, const std::vector<double>& pts,
+                                     const std::array<int, 3>& ax, int64_t lo, int64_t hi, bool parallel) {
+  if (hi - lo > n / 4) return order_by_axes(ids, n, pts, ax, parallel);
+  struct Key {
+    double k[3];
+    int32_t pos;
+  };
+  std::vector<Key> keys(static_cast<size_t>(n));
+  for (int64_t i = 0; i < n; i++) {
+    const double* p = &pts[3 * static_cast<size_t>(ids[i])];
+    keys[i] = Key{{p[ax[0]], p[ax[1]], p[ax[2]]}, static_cast<int32_t>(i)};
+  }
+  auto less = [](const Key& a, const Key& b) {
+    if (a.k[0] != b.k[0]) return a.k[0] < b.k[0];
+    if (a.k[1] != b.k[1]) return a.k[1] < b.k[1];
+    if (a.k[2] != b.k[2]) return a.k[2] < b.k[2];
+    return a.pos < b.pos;
+  };
+  std::nth_element(keys.begin(), keys.begin() + lo, keys.end(), less);
+  std::nth_element(keys.begin() + lo, keys.begin() + hi, keys.end(), less);
+  std::sort(keys.begin() + lo, keys.begin() + hi, less);
+  std::vector<int32_t> order(static_cast<size_t>(n));
+  for (int64_t i = 0; i < n; i++) order[i] = keys[i].pos;
+  return order;
 }
 
 }  // namespace
@@ -508,53 +557,72 @@ DomainDivider::DomainDivider(const std::vector<double>& pts, const std::vector<i
     : pts_(pts), point_idcs_(point_idcs), poly_idcs_(poly_point_idcs) {
   constexpr double kOverlapQuota = 0.5;   // domain_divider.hpp:26
   constexpr int64_t kMaxLeafSize = 1024;  // domain_divider.hpp:27
-  std::list<Domain> work;
-  Domain root;
-  root.idx = point_idcs;
-  root.inner.assign(point_idcs.size(), 1);
-  work.push_back(std::move(root));
-  for (auto it = work.begin(); it != work.end();) {
-    Domain& d = *it;
+  // divide_domain (domain_divider.hpp:160-240): sort along the widest axes and
+  // split with an overlap of about half a leaf, until every domain holds at
+  // most kMaxLeafSize points. Domains of one depth are independent: few large
+  // ones use the parallel sort, many small ones are split in parallel.
+  auto divide = [&](const Domain& d, Domain& left, Domain& right, bool parallel_sort) {
     int64_t n = static_cast<int64_t>(d.idx.size());
-    if (n <= kMaxLeafSize) {
-      ++it;
-      continue;
-    }
-    // divide_domain (domain_divider.hpp:160-240): sort along the widest axes,
-    // split with an overlap of about half a leaf.
     Bbox3 bb;
     for (int32_t i : d.idx) bb.add(&pts_[3 * static_cast<size_t>(i)]);
-    std::vector<int32_t> order(n);
-    std::iota(order.begin(), order.end(), 0);
-    auto ax = bb.axes_by_width();
-    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
-      for (int k : ax) {
-        double pa = pts_[3 * static_cast<size_t>(d.idx[a]) + k], pb = pts_[3 * static_cast<size_t>(d.idx[b]) + k];
-        if (pa != pb) return pa < pb;
-      }
-      return false;
-    });
     double q = kOverlapQuota * static_cast<double>(kMaxLeafSize) / static_cast<double>(n);
     int64_t n_sub = static_cast<int64_t>(round_half_to_even((1.0 + q) / 2.0 * static_cast<double>(n)));
     int64_t left_part = n - n_sub, right_part = n_sub;
+    auto order = order_for_split(d.idx.data(), n, pts_, bb.axes_by_width(), left_part, right_part,
+                                 parallel_sort);
     int64_t mid = static_cast<int64_t>(round_half_to_even(static_cast<double>(left_part + right_part) / 2.0));
-    Domain left, right;
+    left.idx.resize(right_part);
+    left.inner.resize(right_part);
     for (int64_t i = 0; i < right_part; i++) {
       int32_t o = order[i];
-      left.idx.push_back(d.idx[o]);
-      left.inner.push_back(d.inner[o] && i < mid);
+      left.idx[i] = d.idx[o];
+      left.inner[i] = d.inner[o] && i < mid;
     }
+    right.idx.resize(n - left_part);
+    right.inner.resize(n - left_part);
     for (int64_t i = left_part; i < n; i++) {
       int32_t o = order[i];
-      right.idx.push_back(d.idx[o]);
-      right.inner.push_back(d.inner[o] && i >= mid);
+      right.idx[i - left_part] = d.idx[o];
+      right.inner[i - left_part] = d.inner[o] && i >= mid;
     }
-    work.push_back(std::move(left));
-    work.push_back(std::move(right));
-    it = work.erase(it);
+  };
+  std::vector<Domain> work, leaves;
+  {
+    Domain root;
+    root.idx = point_idcs;
+    root.inner.assign(point_idcs.size(), 1);
+    work.push_back(std::move(root));
   }
+  while (!work.empty()) {
+    const int64_t nw = static_cast<int64_t>(work.size());
+    std::vector<Domain> next(static_cast<size_t>(2 * nw));
+    std::vector<uint8_t> split(static_cast<size_t>(nw), 0);
+    for (int64_t i = 0; i < nw; i++) split[i] = static_cast<int64_t>(work[i].idx.size()) > kMaxLeafSize;
+    if (nw < 8) {
+      for (int64_t i = 0; i < nw; i++)
+        if (split[i]) divide(work[i], next[2 * i], next[2 * i + 1], true);
+    } else {
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int64_t i = 0; i < nw; i++)
+        if (split[i]) divide(work[i], next[2 * i], next[2 * i + 1], false);
+    }
+    std::vector<Domain> cont;
+    for (int64_t i = 0; i < nw; i++) {
+      if (split[i]) {
+        cont.push_back(std::move(next[2 * i]));
+        cont.push_back(std::move(next[2 * i + 1]));
+      } else {
+        leaves.push_back(std::move(work[i]));
+      }
+    }
+    work.swap(cont);
+  }
+  auto& work_out = leaves;
   // merge_poly_points (domain.hpp:30-51)
-  for (auto& d : work) {
+  domains_.resize(work_out.size());
+#pragma omp parallel for schedule(dynamic, 16)
+  for (size_t di = 0; di < work_out.size(); di++) {
+    Domain& d = work_out[di];
     std::vector<int32_t> order(d.idx.size());
     std::iota(order.begin(), order.end(), 0);
     std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return d.idx[a] < d.idx[b]; });
@@ -578,7 +646,7 @@ DomainDivider::DomainDivider(const std::vector<double>& pts, const std::vector<i
     }
     md.idx.insert(md.idx.end(), sorted_idx.begin(), sorted_idx.end());
     md.inner.insert(md.inner.end(), sorted_inner.begin(), sorted_inner.end());
-    domains_.push_back(std::move(md));
+    domains_[di] = std::move(md);
   }
 }
 
@@ -586,52 +654,45 @@ std::vector<int32_t> DomainDivider::choose_coarse_points(int64_t n_coarse) const
   // domain_divider.hpp:54-125: split the cluster of lowest level (largest bbox
   // volume first) at its median until n_coarse clusters exist; take the point
   // nearest each cluster's bbox centre.
+  //
+  // The reference pops clusters from a priority queue (lowest level first,
+  // then largest volume): every cluster of one level is split before any of
+  // the next, so the split runs level by level; at the last level only the
+  // largest clusters are split, up to n_coarse clusters in total. Clusters of
+  // one level are independent and handled in parallel. Each cluster carries
+  // its points' coordinates, so the passes over it read memory in order.
+  struct Rec {
+    double x[3];
+    int32_t id, pos;
+  };
   struct Cluster {
-    int level;
-    std::vector<int32_t> pts;
-    double volume;
-    int32_t center;
+    int level = 0;
+    std::vector<Rec> pts;
+    double volume = 0.0;
+    int32_t center = -1;
+    std::array<int, 3> axes{0, 1, 2};
   };
   auto init = [&](Cluster& c) {
     Bbox3 bb;
-    for (int32_t i : c.pts) bb.add(&pts_[3 * static_cast<size_t>(i)]);
+    for (const Rec& r : c.pts) bb.add(r.x);
     double cen[3], best = INFINITY;
     c.volume = 1.0;
     for (int k = 0; k < 3; k++) {
       cen[k] = bb.lo[k] + (bb.hi[k] - bb.lo[k]) / 2.0;
       c.volume *= bb.hi[k] - bb.lo[k];
     }
-    c.center = c.pts.front();
-    for (int32_t i : c.pts) {
-      const double* p = &pts_[3 * static_cast<size_t>(i)];
-      double d2 = (p[0] - cen[0]) * (p[0] - cen[0]) + (p[1] - cen[1]) * (p[1] - cen[1]) + (p[2] - cen[2]) * (p[2] - cen[2]);
+    c.center = c.pts.front().id;
+    for (const Rec& r : c.pts) {
+      double d2 = (r.x[0] - cen[0]) * (r.x[0] - cen[0]) + (r.x[1] - cen[1]) * (r.x[1] - cen[1]) +
+                  (r.x[2] - cen[2]) * (r.x[2] - cen[2]);
       if (d2 < best) {
         best = d2;
-        c.center = i;
+        c.center = r.id;
       }
     }
-    sort_by_axes(c.pts, pts_, bb.axes_by_width());
+    c.axes = bb.axes_by_width();
   };
-  auto cmp = [](const Cluster* a, const Cluster* b) {
-    if (a->level != b->level) return a->level > b->level;
-    return a->volume < b->volume;
-  };
-  std::vector<std::unique_ptr<Cluster>> store;
-  std::priority_queue<Cluster*, std::vector<Cluster*>, decltype(cmp)> q(cmp);
-  std::vector<int32_t> out(poly_idcs_);
-  auto root = std::make_unique<Cluster>();
-  root->level = 0;
-  for (int32_t i : point_idcs_)
-    if (std::find(poly_idcs_.begin(), poly_idcs_.end(), i) == poly_idcs_.end()) root->pts.push_back(i);
-  if (root->pts.empty()) return out;
-  init(*root);
-  q.push(root.get());
-  store.push_back(std::move(root));
-  int64_t current = 1;
-  while (current < n_coarse && !q.empty()) {
-    Cluster* c = q.top();
-    int64_t sz = static_cast<int64_t>(c->pts.size());
-    if (sz <= 1) break;
+  auto split_mid = [](int64_t sz) {
     // median split (ties broken towards an even index, domain_divider.hpp:82-88)
     int64_t mid = 0, bestd = INT64_MAX;
     for (int64_t i = 0; i < sz; i++) {
@@ -641,26 +702,75 @@ std::vector<int32_t> DomainDivider::choose_coarse_points(int64_t n_coarse) const
         mid = i;
       }
     }
-    q.pop();
-    current -= 1;
-    auto left = std::make_unique<Cluster>();
-    auto right = std::make_unique<Cluster>();
-    left->level = right->level = c->level + 1;
-    left->pts.assign(c->pts.begin(), c->pts.begin() + mid);
-    right->pts.assign(c->pts.begin() + mid, c->pts.end());
-    for (auto* part : {left.get(), right.get()}) {
-      if (part->pts.empty()) continue;
-      init(*part);
-      q.push(part);
-      current += 1;
+    return mid;
+  };
+  std::vector<int32_t> out(poly_idcs_);
+  Cluster root;
+  for (int32_t i : point_idcs_) {
+    if (std::find(poly_idcs_.begin(), poly_idcs_.end(), i) != poly_idcs_.end()) continue;
+    const double* p = &pts_[3 * static_cast<size_t>(i)];
+    root.pts.push_back(Rec{{p[0], p[1], p[2]}, i, 0});
+  }
+  if (root.pts.empty()) return out;
+  init(root);
+  std::vector<Cluster> cur;
+  cur.push_back(std::move(root));
+  std::vector<int32_t> done;  // centres of clusters that stay unsplit
+  int64_t current = 1;
+  while (current < n_coarse && !cur.empty()) {
+    std::stable_sort(cur.begin(), cur.end(), [](const Cluster& x, const Cluster& y) { return x.volume > y.volume; });
+    int64_t splittable = 0;
+    for (const auto& c : cur) splittable += c.pts.size() > 1;
+    int64_t n_split = std::min<int64_t>(splittable, n_coarse - current);
+    std::vector<int64_t> which;
+    for (int64_t i = 0; i < static_cast<int64_t>(cur.size()) && static_cast<int64_t>(which.size()) < n_split; i++)
+      if (cur[i].pts.size() > 1) which.push_back(i);
+    const int64_t nw = static_cast<int64_t>(which.size());
+    std::vector<Cluster> next(static_cast<size_t>(2 * nw));
+    auto do_split = [&](int64_t w) {
+      Cluster& c = cur[which[w]];
+      const int64_t sz = static_cast<int64_t>(c.pts.size());
+      const int64_t mid = split_mid(sz);
+      // The first mid points in the order of coordinates along the cluster's
+      // axes (ties by position) form the left half; only the membership of the
+      // halves matters, so a selection replaces the full sort.
+      for (int64_t i = 0; i < sz; i++) c.pts[i].pos = static_cast<int32_t>(i);
+      const auto ax = c.axes;
+      auto less = [&ax](const Rec& p, const Rec& q) {
+        for (int k : ax)
+          if (p.x[k] != q.x[k]) return p.x[k] < q.x[k];
+        return p.pos < q.pos;
+      };
+      std::nth_element(c.pts.begin(), c.pts.begin() + mid, c.pts.end(), less);
+      // (the order inside a half only breaks exact ties of the nearest-to-centre search)
+      Cluster& l = next[2 * w];
+      Cluster& r = next[2 * w + 1];
+      l.level = r.level = c.level + 1;
+      l.pts.assign(c.pts.begin(), c.pts.begin() + mid);
+      r.pts.assign(c.pts.begin() + mid, c.pts.end());
+      std::vector<Rec>().swap(c.pts);
+      if (!l.pts.empty()) init(l);
+      if (!r.pts.empty()) init(r);
+    };
+    if (nw < 8) {
+      for (int64_t w = 0; w < nw; w++) do_split(w);
+    } else {
+#pragma omp parallel for schedule(dynamic, 4)
+      for (int64_t w = 0; w < nw; w++) do_split(w);
     }
-    store.push_back(std::move(left));
-    store.push_back(std::move(right));
+    std::vector<uint8_t> was_split(cur.size(), 0);
+    for (int64_t w : which) was_split[w] = 1;
+    for (size_t i = 0; i < cur.size(); i++)
+      if (!was_split[i]) done.push_back(cur[i].center);
+    std::vector<Cluster> lvl;
+    for (auto& c : next)
+      if (!c.pts.empty()) lvl.push_back(std::move(c));
+    current += nw;
+    cur.swap(lvl);
+    if (nw == 0) break;
   }
-  while (!q.empty()) {
-    out.push_back(q.top()->center);
-    q.pop();
-  }
+  out.insert(out.end(), done.begin(), done.end());
+  for (const auto& c : cur) out.push_back(c.center);
   return out;
 }
 
@@ -720,15 +830,26 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
   fine_.clear();
   fine_.resize(n_levels_);
   h_domains_.assign(n_levels_, {});
+  auto tick = [&](const char* what) {
+    if (!opt_.verbose) return;
+    double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::printf("  ras setup %-28s %8.3f s\n", what, t);
+    std::fflush(stdout);
+  };
+  tick("unisolvent set, transforms");
+  // The host divides level l-1 while the device assembles and factors level l.
   for (int level = n_levels_ - 1; level >= 1; level--) {
     DomainDivider div(a_pts, point_idcs_[level], poly_idcs);
+    tick("divide");
     double finest = std::log(static_cast<double>(mu_)) / std::log(kFineToCoarseRatio);
     double coarsest = std::log(static_cast<double>(kNCoarsestPoints)) / std::log(kFineToCoarseRatio);
     auto n_coarse = static_cast<int64_t>(std::pow(
         kFineToCoarseRatio, coarsest + (level - 1) * (finest - coarsest) / (n_levels_ - 1)));
     point_idcs_[level - 1] = div.choose_coarse_points(n_coarse);
+    tick("coarse points");
     h_domains_[level] = std::move(div.domains());
     setup_batch(fine_[level], h_domains_[level], false);
+    tick("batch launched");
   }
   {
     Domain cd;
@@ -740,6 +861,8 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
   int32_t flag = 0;
   PAM_HIP(hipMemcpyAsync(&flag, flag_.get(), sizeof flag, hipMemcpyDeviceToHost, s_));
   PAM_HIP(hipStreamSynchronize(s_));
+  setup_tmp_.clear();
+  tick("factors done");
   if (flag) throw_runtime("a subdomain matrix Q^T A Q is not positive definite");
   d_point_idcs_.resize(n_levels_);
   for (int level = 0; level < n_levels_; level++)
@@ -769,6 +892,7 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
     PAM_HIP(hipStreamSynchronize(s_));
   }
   PAM_HIP(hipStreamSynchronize(s_));
+  tick("A P done");
   setup_s_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -786,30 +910,35 @@ void RasPreconditioner::setup_batch(DomainBatch& b, const std::vector<Domain>& d
     mptr[d + 1] = mptr[d] + n * n;
     b.max_m = std::max<int>(b.max_m, static_cast<int>(m));
   }
-  std::vector<int32_t> idx(b.h_ptr.back());
-  std::vector<uint8_t> inner(b.h_ptr.back());
+  auto tmp = std::make_unique<SetupTmp>();
+  tmp->mptr = std::move(mptr);
+  tmp->idx.resize(b.h_ptr.back());
+  tmp->inner.resize(b.h_ptr.back());
   for (int d = 0; d < b.n_dom; d++) {
-    std::copy(doms[d].idx.begin(), doms[d].idx.end(), idx.begin() + b.h_ptr[d]);
-    std::copy(doms[d].inner.begin(), doms[d].inner.end(), inner.begin() + b.h_ptr[d]);
+    std::copy(doms[d].idx.begin(), doms[d].idx.end(), tmp->idx.begin() + b.h_ptr[d]);
+    std::copy(doms[d].inner.begin(), doms[d].inner.end(), tmp->inner.begin() + b.h_ptr[d]);
   }
-  b.idx.upload(idx.data(), idx.size(), s_);
-  b.inner.upload(inner.data(), inner.size(), s_);
+  b.idx.upload(tmp->idx.data(), tmp->idx.size(), s_);
+  b.inner.upload(tmp->inner.data(), tmp->inner.size(), s_);
   b.ptr.upload(b.h_ptr.data(), b.h_ptr.size(), s_);
-  b.mat_ptr.upload(mptr.data(), mptr.size(), s_);
-  b.mat.resize(static_cast<size_t>(std::max<int64_t>(mptr.back(), 1)));
+  b.mat_ptr.upload(tmp->mptr.data(), tmp->mptr.size(), s_);
+  b.mat.resize(static_cast<size_t>(std::max<int64_t>(tmp->mptr.back(), 1)));
   RbfSet rs = make_rbf_set(model_);
-  DevBuf<double> u(static_cast<size_t>(std::max<int64_t>(l_ * (b.h_ptr.back() - static_cast<int64_t>(b.n_dom) * l_), 1)));
-  DevBuf<double> atl(static_cast<size_t>(std::max(b.n_dom * l_ * l_, 1)));
+  tmp->u.resize(static_cast<size_t>(std::max<int64_t>(l_ * (b.h_ptr.back() - static_cast<int64_t>(b.n_dom) * l_), 1)));
+  tmp->atl.resize(static_cast<size_t>(std::max(b.n_dom * l_ * l_, 1)));
   if (l_ > 0) {
-    assemble_u_kernel<<<dim3(16, b.n_dom), kT, 0, s_>>>(rs, l_, d_pts_.get(), b.ptr.get(), b.idx.get(), u.get(), atl.get());
+    assemble_u_kernel<<<dim3(16, b.n_dom), kT, 0, s_>>>(rs, l_, d_pts_.get(), b.ptr.get(), b.idx.get(),
+                                                        tmp->u.get(), tmp->atl.get());
     PAM_LAUNCH_CHECK();
   }
   assemble_m_kernel<<<dim3(64, b.n_dom), kT, 0, s_>>>(rs, l_, d_pts_.get(), b.ptr.get(), b.idx.get(), lag_.get(),
-                                                       u.get(), atl.get(), b.mat_ptr.get(), b.mat.get());
+                                                       tmp->u.get(), tmp->atl.get(), b.mat_ptr.get(), b.mat.get());
   PAM_LAUNCH_CHECK();
   cholesky_kernel<<<b.n_dom, kT, 0, s_>>>(b.ptr.get(), l_, b.mat_ptr.get(), b.mat.get(), flag_.get());
   PAM_LAUNCH_CHECK();
-  PAM_HIP(hipStreamSynchronize(s_));  // u, atl and host vectors go out of scope
+  // b.h_ptr is a member and outlives the upload; the rest stays until the
+  // constructor synchronizes.
+  setup_tmp_.push_back(std::move(tmp));
 }
 
 int64_t RasPreconditioner::factor_bytes() const {

@@ -49,6 +49,7 @@ struct DomainBatch {
 struct RasOptions {
   FmmConfig level_cfg;      // inter-level residual updates (default accuracy)
   bool use_level_cfg = false;
+  int verbose = 0;          // print the set-up phase times
 };
 
 class RasPreconditioner : public LinearOperator {
@@ -64,7 +65,16 @@ class RasPreconditioner : public LinearOperator {
   int64_t factor_bytes() const;
 
  private:
+  // Assembles and factors a batch asynchronously; its temporaries stay in
+  // setup_tmp_ until the end of the constructor.
   void setup_batch(DomainBatch& b, const std::vector<Domain>& doms, bool coarse);
+  struct SetupTmp {
+    std::vector<int32_t> idx;
+    std::vector<uint8_t> inner;
+    std::vector<int64_t> mptr;
+    DevBuf<double> u, atl;
+  };
+  std::vector<std::unique_ptr<SetupTmp>> setup_tmp_;
   void solve(int level, const double* d_res, double* d_w);
   void update_residuals(int src, int trg, const double* d_w, double* d_res);
   void orthogonalize(double* d_w, double* d_res);

@@ -132,6 +132,7 @@ Solver::Solver(const Model& model, const std::vector<double>& points, const FitO
     RasOptions ro;
     ro.level_cfg = opt_.level_cfg;
     ro.use_level_cfg = opt_.use_level_cfg;
+    ro.verbose = opt_.verbose;
     pc_ = std::make_unique<RasPreconditioner>(model_, points, s_, ro);
   }
   if (l_ > 0) {

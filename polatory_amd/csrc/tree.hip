@@ -275,6 +275,8 @@ void build_lists(const Tree& trg, const Tree& src, Lists& lists, TreeWorkspace& 
     M2LLevel& m = lists.m2l[l];
     int64_t n = tl.n;
     m.src.resize(static_cast<size_t>(n) * kM2LSlots);
+    m.n_cells = n;
+    m.h_blk_oct.clear();
     if (n == 0) {
       m.n_cols = 0;
       continue;
@@ -284,7 +286,7 @@ void build_lists(const Tree& trg, const Tree& src, Lists& lists, TreeWorkspace& 
                                                         int64_t{1} << l, vl, m.src.get());
     PAM_LAUNCH_CHECK();
     // Columns grouped by octant (stable, Morton order inside a group), each group
-    // padded to a multiple of 32 so a column block has a single octant.
+    // padded to a multiple of kM2LCols so a column block has a single octant.
     ws.tmp_codes.resize(n);
     ws.tmp_idx.resize(n);
     octant_key_kernel<<<ceil_div(n, 256), 256, 0, s>>>(tl.codes.get(), n, ws.tmp_codes.get(),
@@ -315,8 +317,9 @@ void build_lists(const Tree& trg, const Tree& src, Lists& lists, TreeWorkspace& 
     fill_cols_kernel<<<ceil_div(n, 256), 256, 0, s>>>(ws.tmp_idx.get(), ws.tmp_codes.get(), n, o,
                                                       m.col_cell.get());
     PAM_LAUNCH_CHECK();
-    m.blk_oct.upload(blk.data(), blk.size(), s);
-    PAM_HIP(hipStreamSynchronize(s));  // blk is a host temporary
+    m.h_blk_oct = blk;
+    m.blk_oct.upload(m.h_blk_oct.data(), m.h_blk_oct.size(), s);
+    PAM_HIP(hipStreamSynchronize(s));  // the upload reads host memory
   }
 }
 

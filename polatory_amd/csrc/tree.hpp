@@ -49,9 +49,11 @@ struct TreeWorkspace {
 };
 
 struct M2LLevel {
-  int64_t n_cols = 0;             // target columns, padded to 32 per octant group
+  int64_t n_cols = 0;             // target columns, padded to kM2LCols per octant group
+  int64_t n_cells = 0;            // target cells of the level
   DevBuf<int32_t> col_cell;       // [n_cols] target cell or -1
-  DevBuf<int32_t> blk_oct;        // [n_cols/32] octant of each 32-column block
+  DevBuf<int32_t> blk_oct;        // [n_cols/kM2LCols] octant of each column block
+  std::vector<int32_t> h_blk_oct; // host copy of blk_oct
   DevBuf<int32_t> src;            // [n_trg_cells][189] source cell or -1 (octant slot order)
 };
 
@@ -61,7 +63,7 @@ struct Lists {
 };
 
 constexpr int kM2LSlots = 189;
-constexpr int kM2LCols = 32;
+constexpr int kM2LCols = 64;
 
 // The 189 transfer vectors valid for each octant, in increasing v order.
 const std::vector<int32_t>& octant_vlist_host();  // [8][189]
