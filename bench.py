@@ -27,6 +27,22 @@ N_TRG = 1_000_000
 N_SUB = 10_000
 PEAK_FP32 = 157.3e12
 PEAK_FP64 = 78.6e12
+PEAK_HBM = 8.0e12
+
+
+def hbm_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the rocprofv3 counter summary in
+    profiles/hbm_traffic.txt (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by
+    the factors measured on a plain device copy); None when not profiled."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 5 and not line.startswith("#") and f[0] == kernel:
+            fetch_kib, write_kib, fs, ws = map(float, f[1:])
+            return (fetch_kib * fs + write_kib * ws) * 1024.0
+    return None
 
 
 def log(*a):
@@ -132,6 +148,66 @@ def fit_section(pa):
     return out
 
 
+def gmm_sites(n, seed_sites=9, seed_field=10, seed_noise=11):
+    """500,000 sites from a 20-component Gaussian mixture (means U[0,10]^3,
+    isotropic std U[0.2,1.0], equal weights); values = 8 random sines plus
+    N(0, 0.05^2) noise."""
+    rng = np.random.default_rng(seed_sites)
+    means = rng.uniform(0, 10, (20, 3))
+    stds = rng.uniform(0.2, 1.0, 20)
+    comp = rng.integers(0, 20, n)
+    pts = means[comp] + stds[comp, None] * rng.normal(size=(n, 3))
+    rf = np.random.default_rng(seed_field)
+    amp = rf.uniform(0.5, 1.0, 8)
+    freq = rf.normal(0.0, 0.5, (8, 3))
+    phase = rf.uniform(0, 2 * np.pi, 8)
+
+    def field(p):
+        return (np.sin(p @ freq.T + phase) * amp).sum(axis=1) / 4.0
+
+    vals = field(pts) + np.random.default_rng(seed_noise).normal(0.0, 0.05, n)
+    return pts, vals
+
+
+def kriging_section(pa):
+    """Anisotropic kriging: exponential covariance (sill 1, range 2), anisotropy
+    A = diag(1, 1, 1/0.3) (ranges 2, 2, 0.6), nugget 0.0025, degree 0, fit to
+    1e-4 absolute; prediction on the 128^3 cell centres of [0,10]^3 with the
+    mixed (FP32 near field + FP32-MFMA M2L) and the FP64 evaluator."""
+    pts, vals = gmm_sites(500_000)
+    rbf = pa.CovExponential([1.0, 2.0])
+    rbf.set_anisotropy(np.diag([1.0, 1.0, 1.0 / 0.3]))
+    ip = pa.Interpolant(pa.Model(rbf, poly_degree=0, nugget=0.0025))
+    t0 = time.perf_counter()
+    ip.fit(pts, vals, 1e-4, max_iter=100)
+    t_fit = time.perf_counter() - t0
+    st = ip.fit_stats()
+    g = (np.arange(128) + 0.5) * (10.0 / 128)
+    grid = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3)
+    out = {"workload": "exp covariance kriging, 500,000 GMM sites, aniso diag(1,1,1/0.3), nugget 0.0025, "
+                       "tol 1e-4; predict 128^3 grid",
+           "fit_s": t_fit, "iterations": st["iterations"], "fp64_residual": st["residual"]}
+    sub = np.random.default_rng(17).permutation(len(grid))[:2000]
+    w = ip.weights()
+    n = ip.num_centers()
+    ref = pa.direct_evaluate(rbf, ip.centers(), w[:n], grid[sub]) + w[n]
+    preds = {}
+    for name, cfg in (("mixed", pa.FmmConfig("mixed", 8, -1, "auto")), ("fp64", pa.FmmConfig("fp64", 10, -1, "auto"))):
+        ip.set_eval_config(cfg)
+        ip.evaluate(grid)  # builds the evaluator and trees
+        t0 = time.perf_counter()
+        y = ip.evaluate(grid)
+        dt = time.perf_counter() - t0
+        preds[name] = y
+        out[name] = {"predict_s": dt, "points_per_s": len(grid) / dt,
+                     "rel_err_vs_fp64_direct": float(np.abs(y[sub] - ref).max() / np.abs(ref).max())}
+        log("kriging", name, json.dumps(out[name]))
+    out["max_rel_diff_mixed_vs_fp64"] = float(np.abs(preds["mixed"] - preds["fp64"]).max()
+                                              / np.abs(preds["fp64"]).max())
+    log("kriging", json.dumps({k: v for k, v in out.items() if not isinstance(v, dict)}))
+    return out
+
+
 def serve_section(pa, n_models, n_requests, rate):
     """32 resident models fitted to f on N ~ log-uniform[1e4, 1e6] U[0,1]^3 sites
     (bh3 or th3, 50/50, tol 1e-4), Poisson arrivals at `rate` req/s, each
@@ -201,16 +277,18 @@ def serve_section(pa, n_models, n_requests, rate):
             "max_rel_err_sampled": worst, "models_ready_s": t_models}
 
 
-def time_steps(de, steps, warmup):
+def time_steps(torch, de, steps, warmup):
     for _ in range(warmup):
         de.step()
     de.sync()
+    torch.cuda.synchronize()
     stats = []
     t0 = time.perf_counter()
     for _ in range(steps):
         de.step()
-        stats.append(de.ev.stats())
+        stats.append(de.ev.stats())  # HIP-event stage times of this step
     de.sync()
+    torch.cuda.synchronize()
     t1 = time.perf_counter()
     return (t1 - t0) / steps, stats
 
@@ -226,6 +304,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fit", action="store_true", help="skip the 1M-point fit comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
+    ap.add_argument("--no-kriging", action="store_true", help="skip the kriging fit and prediction")
     ap.add_argument("--serve-models", type=int, default=32)
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--serve-rate", type=float, default=200.0)
@@ -265,7 +344,7 @@ def main():
         if args.only not in ("both", name):
             continue
         de = DeviceEval(torch, src, w, trg, cfg)
-        sec, stats = time_steps(de, args.steps, args.warmup)
+        sec, stats = time_steps(torch, de, args.steps, args.warmup)
         y = de.output() + poly(c, trg)  # polynomial terms on the host for the error check only
         err = np.abs(y[sub] - ref).max() / ref_inf
         p2p_pairs, m2l_pairs = de.ev.count_work()
@@ -274,7 +353,7 @@ def main():
                ("ms_tree", "ms_upward", "ms_m2l", "ms_l2l", "ms_l2p", "ms_p2p", "ms_total")}
         results[name] = dict(sec=sec, err=err, p2p_pairs=p2p_pairs, m2l_pairs=m2l_pairs,
                              order=st["order"], leaf_level=st["leaf_level"], m2l_rank=st["m2l_rank"],
-                             stage_ms=avg)
+                             n_trg_leaves=st["n_trg_leaves"], stage_ms=avg)
         log(name, json.dumps(results[name]))
         del de
         torch.cuda.synchronize()
@@ -291,17 +370,26 @@ def main():
     m2l_dense_flops = 2.0 * p**6 * m["m2l_pairs"]
     m2l_flops = 2.0 * r * r * m["m2l_pairs"] if r > 0 else m2l_dense_flops
     p2p_flops = 11.0 * m["p2p_pairs"]
-    if stage["ms_m2l"] >= stage["ms_p2p"]:
-        achieved = m2l_flops / (stage["ms_m2l"] * 1e-3)
-        roof = {"kernel": "m2l_kernel<float> (compress + transfer + expand, all levels)",
-                "bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
-                "dense_equivalent_tflops": m2l_dense_flops / (stage["ms_m2l"] * 1e-3) / 1e12}
-    else:
-        achieved = p2p_flops / (stage["ms_p2p"] * 1e-3)
-        roof = {"kernel": "p2p_kernel<float,bh3>", "bound": "valu (fp32)", "achieved": achieved / 1e12,
-                "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
-                "traffic": None, "pairs": m["p2p_pairs"], "flops_per_pair": 11}
+    m2l_ach = m2l_flops / (stage["ms_m2l"] * 1e-3)
+    p2p_ach = p2p_flops / (stage["ms_p2p"] * 1e-3)
+    # L2P: FP64 locals of every target leaf + FP64 tree-order targets read, FP64 far field written
+    l2p_bytes = m["n_trg_leaves"] * p**3 * 8 + N_TRG * (32 + 8)
+    stage_roofs = {
+        "m2l": {"kernel": "m2l_dense_kernel<float> + m2l_reduce_kernel", "bound": "mfma",
+                "achieved": m2l_ach / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+                "frac": m2l_ach / PEAK_FP32, "flops": m2l_flops, "cell_pairs": m["m2l_pairs"]},
+        "p2p": {"kernel": "p2p_kernel<float,bh3>", "bound": "valu (fp32)", "achieved": p2p_ach / 1e12,
+                "peak": PEAK_FP32 / 1e12, "peak_unpacked": PEAK_FP32 / 2e12, "unit": "TFLOP/s",
+                "frac": p2p_ach / PEAK_FP32, "pairs": m["p2p_pairs"], "flops_per_pair": 11},
+        "l2p": {"kernel": "l2p_kernel<double,P>", "bound": "hbm",
+                "achieved": l2p_bytes / (stage["ms_l2p"] * 1e-3) / 1e9, "peak": PEAK_HBM / 1e9,
+                "unit": "GB/s", "frac": l2p_bytes / (stage["ms_l2p"] * 1e-3) / PEAK_HBM, "bytes": l2p_bytes},
+    }
+    dominant = "m2l" if stage["ms_m2l"] >= stage["ms_p2p"] else "p2p"
+    roof = {k: stage_roofs[dominant][k] for k in ("kernel", "bound", "achieved", "peak", "unit", "frac")}
+    roof["traffic"] = hbm_traffic(stage_roofs[dominant]["kernel"].split(" ")[0])
+    if dominant == "m2l":
+        roof["dense_equivalent_tflops"] = m2l_dense_flops / (stage["ms_m2l"] * 1e-3) / 1e12
 
     cpu = None
     if not args.no_cpu:
@@ -316,6 +404,7 @@ def main():
                          f"({dt:.1f} s), OpenMP over {cores} threads"}
 
     fit = None if args.no_fit else fit_section(pa)
+    kriging = None if args.no_kriging else kriging_section(pa)
     serve = None if args.no_serve else serve_section(pa, args.serve_models, args.serve_requests,
                                                      args.serve_rate)
 
@@ -342,8 +431,10 @@ def main():
         "speedup_vs_fp64_gpu": f["sec"] / m["sec"],
         "stage_ms": stage,
         "roofline": roof,
+        "stage_rooflines": stage_roofs,
         "cpu_baseline": cpu,
         "fit": fit,
+        "kriging": kriging,
         "serve": serve,
     }
     print(json.dumps(out), flush=True)

@@ -161,6 +161,7 @@ typedef struct {
   int use_preconditioner;   /* 1: multilevel RAS (ras_preconditioner.hpp), 0: none */
   double refine_rtol;       /* inner relative residual that restarts FGMRES on the FP64 residual */
   int verbose;              /* print the residual per iteration (solver.hpp:111-128) */
+  int use_graphs;           /* replay "preconditioner + matvec" of each FGMRES step from a HIP graph */
 } pam_fit_options;
 int pam_fit_options_default(pam_fit_options* opt);
 /* Interpolant(const Model&) — interpolant.hpp:43 */
@@ -191,6 +192,7 @@ typedef struct {
   int iterations, restarts, n_levels;
   double residual, seconds, setup_seconds, precond_setup_seconds;
   int64_t factor_bytes;
+  int graph_replays;        /* FGMRES steps replayed from the captured graph */
 } pam_fit_stats;
 int pam_interpolant_fit_stats(const pam_interpolant* h, pam_fit_stats* out);
 /* Residual after each FGMRES iteration (out may be NULL to query the length). */

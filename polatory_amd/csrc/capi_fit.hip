@@ -52,6 +52,7 @@ pam::FitOptions to_opts(const pam_fit_options* o) {
   r.use_level_cfg = o->use_level_config != 0;
   r.use_preconditioner = o->use_preconditioner != 0;
   r.refine_rtol = o->refine_rtol;
+  r.use_graphs = o->use_graphs != 0;
   r.verbose = o->verbose;
   return r;
 }
@@ -133,6 +134,7 @@ int pam_fit_options_default(pam_fit_options* opt) {
     opt->use_level_config = 0;
     opt->use_preconditioner = 1;
     opt->refine_rtol = o.refine_rtol;
+    opt->use_graphs = o.use_graphs ? 1 : 0;
     opt->verbose = 0;
   });
 }
@@ -219,6 +221,7 @@ int pam_interpolant_fit_stats(const pam_interpolant* h, pam_fit_stats* out) {
     out->setup_seconds = s.setup_seconds;
     out->precond_setup_seconds = s.precond_setup_seconds;
     out->factor_bytes = s.factor_bytes;
+    out->graph_replays = s.graph_replays;
   });
 }
 

@@ -410,7 +410,16 @@ void FmmEvaluator::attach_source(const FmmEvaluator* owner) {
 }
 
 
-void FmmEvaluator::record(int idx) { PAM_HIP(hipEventRecord(ev_[idx], stream_)); }
+void FmmEvaluator::record(int idx) {
+  // Stage timing is skipped while the stream is being captured into a graph.
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  PAM_HIP(hipStreamIsCapturing(stream_, &cs));
+  if (cs != hipStreamCaptureStatusNone) {
+    timing_valid_ = false;
+    return;
+  }
+  PAM_HIP(hipEventRecord(ev_[idx], stream_));
+}
 
 template <class TP>
 void FmmEvaluator::prepare_source_expansions() {
@@ -671,6 +680,7 @@ void FmmEvaluator::evaluate(double* out, int64_t n, bool on_device) {
     tmp.resize(static_cast<size_t>(nt));
     d_out = tmp.get();
   }
+  timing_valid_ = true;
   record(0);
   stats_ = FmmStats{};
   if (nt > 0) {
@@ -705,6 +715,7 @@ void FmmEvaluator::evaluate(double* out, int64_t n, bool on_device) {
 }
 
 const FmmStats& FmmEvaluator::stats() {
+  if (stats_pending_ && !timing_valid_) stats_pending_ = false;  // last evaluation was captured
   if (stats_pending_) {
     PAM_HIP(hipEventSynchronize(ev_[6]));
     float ms[6];

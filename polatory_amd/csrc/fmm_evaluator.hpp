@@ -147,6 +147,7 @@ class FmmEvaluator {
   hipEvent_t ev_[10] = {};
   FmmStats stats_;
   bool stats_pending_ = false;
+  bool timing_valid_ = true;  // false when the last evaluation was captured into a graph
 };
 
 // Tree height used when no leaf level is configured: target occupancy of

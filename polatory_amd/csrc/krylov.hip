@@ -53,8 +53,12 @@ void Fgmres::iterate_process() {
   double* zj = z_.get() + static_cast<int64_t>(j) * m_;
   double* w = v_.get() + static_cast<int64_t>(j + 1) * m_;
   // Arnoldi with a flexible right preconditioner
-  if (pc_) pc_->apply(vj, zj); else dev_copy(vj, zj, m_, s_);
-  op_.apply(zj, w);
+  if (step_) {
+    step_(vj, zj, w);
+  } else {
+    if (pc_) pc_->apply(vj, zj); else dev_copy(vj, zj, m_, s_);
+    op_.apply(zj, w);
+  }
   // classical Gram-Schmidt: all projections against the unmodified w
   dev_multi_dot(v_.get(), m_, j + 1, w, m_, h_.data(), ws_, s_);
   PAM_HIP(hipMemcpyAsync(coef_.get(), h_.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice, s_));

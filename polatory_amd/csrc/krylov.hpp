@@ -6,6 +6,7 @@
 // state lives on the host.
 #pragma once
 
+#include <functional>
 #include <vector>
 
 #include "linalg.hpp"
@@ -26,6 +27,10 @@ class Fgmres {
   Fgmres(LinearOperator& op, const double* d_rhs, int64_t m, int max_iter, hipStream_t s);
   void set_initial_solution(const double* d_x0);
   void set_right_preconditioner(LinearOperator* pc) { pc_ = pc; }
+  // Replaces "z_j = M v_j; w = A z_j" of an Arnoldi step (e.g. by a replay of
+  // a captured graph); nullptr restores the operator calls.
+  using Step = std::function<void(const double* vj, double* zj, double* w)>;
+  void set_step(Step f) { step_ = std::move(f); }
   void setup();
   void iterate_process();
   // x = x0 + sum_i y_i z_i with R y = g (back substitution).
@@ -39,6 +44,7 @@ class Fgmres {
  private:
   LinearOperator& op_;
   LinearOperator* pc_ = nullptr;
+  Step step_;
   int64_t m_;
   int max_iter_;
   hipStream_t s_;

@@ -201,6 +201,18 @@ void dev_multi_dot(const double* v, int64_t ldv, int nvec, const double* w, int6
   PAM_HIP(hipStreamSynchronize(s));
 }
 
+void dev_multi_dot_device(const double* v, int64_t ldv, int nvec, const double* w, int64_t n,
+                          double* d_h, Workspace1& ws, hipStream_t s) {
+  if (nvec <= 0) return;
+  int nb = red_blocks(n);
+  int64_t chunk = (n + nb - 1) / nb;
+  ws.partial.resize(static_cast<size_t>(nvec) * nb);
+  multi_dot_kernel<<<nb, kT, 0, s>>>(v, ldv, nvec, w, n, chunk, ws.partial.get());
+  PAM_LAUNCH_CHECK();
+  reduce_rows_kernel<<<nvec, kT, 0, s>>>(ws.partial.get(), nb, nvec, d_h);
+  PAM_LAUNCH_CHECK();
+}
+
 double dev_dot(const double* x, const double* y, int64_t n, Workspace1& ws, hipStream_t s) {
   double r = 0.0;
   dev_multi_dot(x, 0, 1, y, n, &r, ws, s);

@@ -18,6 +18,9 @@ double dev_dot(const double* x, const double* y, int64_t n, Workspace1& ws, hipS
 // h[i] = V_i . w for i < nvec, V_i = V + i * ldv (synchronizes, h is host memory).
 void dev_multi_dot(const double* v, int64_t ldv, int nvec, const double* w, int64_t n, double* h,
                    Workspace1& ws, hipStream_t s);
+// d_h[i] = V_i . w for i < nvec into device memory (no synchronization).
+void dev_multi_dot_device(const double* v, int64_t ldv, int nvec, const double* w, int64_t n,
+                          double* d_h, Workspace1& ws, hipStream_t s);
 // w += alpha * sum_i coef[i] V_i (coef is device memory, nvec entries).
 void dev_gemv_update(double* w, const double* v, int64_t ldv, int nvec, const double* d_coef,
                      int64_t n, double alpha, hipStream_t s);

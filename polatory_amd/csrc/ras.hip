@@ -1004,12 +1004,10 @@ void RasPreconditioner::update_residuals(int src, int trg, const double* d_w, do
 void RasPreconditioner::orthogonalize(double* d_w, double* d_res) {
   // ras_preconditioner.hpp:271-278
   if (l_ == 0) return;
-  double dot[10];
-  dev_multi_dot(p_orth_.get(), mu_, l_, d_w, mu_, dot, ws_, s_);
-  PAM_HIP(hipMemcpyAsync(dot_.get(), dot, sizeof(double) * l_, hipMemcpyHostToDevice, s_));
+  // The projections stay on the device: no host round trip inside apply().
+  dev_multi_dot_device(p_orth_.get(), mu_, l_, d_w, mu_, dot_.get(), ws_, s_);
   dev_gemv_update(d_w, p_orth_.get(), mu_, l_, dot_.get(), mu_, -1.0, s_);
   dev_gemv_update(d_res, ap_.get(), mu_, l_, dot_.get(), mu_, 1.0, s_);
-  PAM_HIP(hipStreamSynchronize(s_));  // dot is a host temporary
 }
 
 void RasPreconditioner::apply(const double* d_v, double* d_out) {

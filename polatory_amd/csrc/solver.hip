@@ -144,6 +144,107 @@ Solver::Solver(const Model& model, const std::vector<double>& points, const FitO
   setup_s_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+namespace {
+
+// Wall time of a linear operator's applications (verbose fits only: each
+// application synchronizes the stream).
+class TimedOperator : public LinearOperator {
+ public:
+  TimedOperator(LinearOperator& inner, hipStream_t s) : inner_(inner), s_(s) {}
+  void apply(const double* d_x, double* d_y) override {
+    PAM_HIP(hipStreamSynchronize(s_));
+    auto t0 = std::chrono::steady_clock::now();
+    inner_.apply(d_x, d_y);
+    PAM_HIP(hipStreamSynchronize(s_));
+    seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    calls++;
+  }
+  int64_t size() const override { return inner_.size(); }
+  double seconds = 0.0;
+  int calls = 0;
+
+ private:
+  LinearOperator& inner_;
+  hipStream_t s_;
+};
+
+// "z = M v; w = A z" on fixed device buffers: the first call runs eagerly
+// (it creates every lazily built evaluator and buffer), the second is
+// captured into a HIP graph, and later calls replay the graph. Inputs and
+// outputs are copied through the fixed buffers. If capture fails, the step
+// stays eager (every evaluation re-uploads its weights first, so no state of
+// the failed capture is reused).
+class GraphStep {
+ public:
+  GraphStep(LinearOperator& op, LinearOperator* pc, int64_t m, hipStream_t s)
+      : op_(op), pc_(pc), m_(m), s_(s), in_(m), z_(m), w_(m) {}
+  ~GraphStep() {
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+  }
+  void run(const double* vj, double* zj, double* w) {
+    dev_copy(vj, in_.get(), m_, s_);
+    if (exec_) {
+      PAM_HIP(hipGraphLaunch(exec_, s_));
+      replays++;
+    } else if (calls_ == 0 || failed_) {
+      eager();
+    } else {
+      capture();
+    }
+    calls_++;
+    dev_copy(z_.get(), zj, m_, s_);
+    dev_copy(w_.get(), w, m_, s_);
+  }
+  int replays = 0;
+
+ private:
+  void eager() {
+    if (pc_) pc_->apply(in_.get(), z_.get());
+    else dev_copy(in_.get(), z_.get(), m_, s_);
+    op_.apply(z_.get(), w_.get());
+  }
+  void capture() {
+    hipGraph_t g = nullptr;
+    bool capturing = false;
+    try {
+      PAM_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeRelaxed));
+      capturing = true;
+      eager();
+      capturing = false;
+      PAM_HIP(hipStreamEndCapture(s_, &g));
+      PAM_HIP(hipGraphInstantiate(&exec_, g, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(g);
+      PAM_HIP(hipGraphLaunch(exec_, s_));
+      replays++;
+    } catch (const Error&) {
+      if (capturing) {
+        hipGraph_t bad = nullptr;
+        (void)hipStreamEndCapture(s_, &bad);
+        if (bad) (void)hipGraphDestroy(bad);
+      } else if (g) {
+        (void)hipGraphDestroy(g);
+      }
+      if (exec_) {
+        (void)hipGraphExecDestroy(exec_);
+        exec_ = nullptr;
+      }
+      (void)hipGetLastError();
+      failed_ = true;
+      eager();
+    }
+  }
+  LinearOperator& op_;
+  LinearOperator* pc_;
+  int64_t m_;
+  hipStream_t s_;
+  DevBuf<double> in_, z_, w_;
+  hipGraphExec_t exec_ = nullptr;
+  int calls_ = 0;
+  bool failed_ = false;
+};
+
+}  // namespace
+
 std::vector<double> Solver::solve(const std::vector<double>& values, double tolerance, int max_iter,
                                   const std::vector<double>* initial, FitStats* stats) {
   // solver.hpp:76-142, with the operator in mixed precision and FGMRES
@@ -177,6 +278,22 @@ std::vector<double> Solver::solve(const std::vector<double>& values, double tole
   int total_iter = 0;
   DevBuf<double> r(mu_);
   bool first = true;
+  std::unique_ptr<TimedOperator> t_op, t_pc;
+  LinearOperator* op = op_.get();
+  LinearOperator* pc = pc_.get();
+  if (opt_.verbose) {
+    t_op = std::make_unique<TimedOperator>(*op_, s_);
+    op = t_op.get();
+    if (pc_) {
+      t_pc = std::make_unique<TimedOperator>(*pc_, s_);
+      pc = t_pc.get();
+    }
+  }
+  double t_check = 0.0;
+  // Graph replay of the Arnoldi step (timed runs synchronize inside the step,
+  // which capture does not allow, so verbose fits stay eager).
+  std::unique_ptr<GraphStep> gstep;
+  if (opt_.use_graphs && !opt_.verbose) gstep = std::make_unique<GraphStep>(*op_, pc_.get(), m, s_);
   while (true) {
     // right-hand side of this refinement cycle
     if (first && !initial) {
@@ -192,8 +309,9 @@ std::vector<double> Solver::solve(const std::vector<double>& values, double tole
       }
     }
     first = false;
-    Fgmres solver(*op_, rhs.get(), m, std::max(max_iter - total_iter, 0), s_);
-    solver.set_right_preconditioner(pc_.get());
+    Fgmres solver(*op, rhs.get(), m, std::max(max_iter - total_iter, 0), s_);
+    solver.set_right_preconditioner(pc);
+    if (gstep) solver.set_step([&](const double* vj, double* zj, double* w) { gstep->run(vj, zj, w); });
     solver.setup();
     if (solver.relative_residual() == 0.0) {
       dev_copy(xb.get(), x.get(), m, s_);
@@ -203,7 +321,9 @@ std::vector<double> Solver::solve(const std::vector<double>& values, double tole
     while (true) {
       solver.solution_vector(x.get());
       dev_axpy(1.0, xb.get(), x.get(), m, s_);
+      auto tc = std::chrono::steady_clock::now();
       Convergence c = res_->converged(x.get(), tolerance);
+      if (opt_.verbose) t_check += std::chrono::duration<double>(std::chrono::steady_clock::now() - tc).count();
       st.history.push_back(c.residual);
       st.residual = c.residual;
       if (opt_.verbose)
@@ -226,6 +346,12 @@ std::vector<double> Solver::solve(const std::vector<double>& values, double tole
     if (restart) st.restarts++;
   }
   st.iterations = total_iter;
+  st.graph_replays = gstep ? gstep->replays : 0;
+  if (opt_.verbose) {
+    std::fprintf(stderr, "  operator %d x %.4f s, preconditioner %d x %.4f s, residual checks %.3f s\n",
+                 t_op->calls, t_op->calls ? t_op->seconds / t_op->calls : 0.0, t_pc ? t_pc->calls : 0,
+                 t_pc && t_pc->calls ? t_pc->seconds / t_pc->calls : 0.0, t_check);
+  }
   std::vector<double> out(m);
   PAM_HIP(hipMemcpyAsync(out.data(), x.get(), sizeof(double) * m, hipMemcpyDeviceToHost, s_));
   PAM_HIP(hipStreamSynchronize(s_));

@@ -69,11 +69,15 @@ struct FitOptions {
   bool use_preconditioner = true;
   double refine_rtol = 1e-4;  // inner relative residual that triggers a refinement restart
   int verbose = 0;
+  // Capture "preconditioner + matvec" of an Arnoldi step in a HIP graph after
+  // the first (eager) step and replay it for the following ones.
+  bool use_graphs = true;
 };
 
 struct FitStats {
   int iterations = 0;
   int restarts = 0;
+  int graph_replays = 0;
   double residual = 0.0;
   double seconds = 0.0;
   double setup_seconds = 0.0;

@@ -1,5 +1,6 @@
 // Octree construction on the device: Morton codes, stable radix sort, leaf
 // ranges, coarser levels by shift+unique, and the P2P/M2L interaction lists.
+#include <algorithm>
 #include <mutex>
 
 #include "device_api.hpp"
@@ -66,21 +67,29 @@ __global__ void morton_kernel(const double* __restrict__ pts, int64_t n, Lo3 lo,
   idx[i] = static_cast<int32_t>(i);
 }
 
-// flags[i] = 1 where (codes[i] >> shift) starts a new run
-__global__ void run_flags_kernel(const uint64_t* __restrict__ codes, int64_t n, int shift,
-                                 int32_t* __restrict__ flags) {
+// Run detection over the first n entries of a buffer of n_bound entries; n is
+// read from the device (d_n) so that coarser levels need no host round trip.
+__device__ __forceinline__ int64_t count_of(const int64_t* d_n, int64_t n_bound) {
+  return d_n ? *d_n : n_bound;
+}
+
+// flags[i] = 1 where (codes[i] >> shift) starts a new run (0 beyond n)
+__global__ void run_flags_kernel(const uint64_t* __restrict__ codes, int64_t n_bound,
+                                 const int64_t* __restrict__ d_n, int shift, int32_t* __restrict__ flags) {
   int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  flags[i] = (i == 0 || (codes[i] >> shift) != (codes[i - 1] >> shift)) ? 1 : 0;
+  if (i >= n_bound) return;
+  const int64_t n = count_of(d_n, n_bound);
+  flags[i] = (i < n && (i == 0 || (codes[i] >> shift) != (codes[i - 1] >> shift))) ? 1 : 0;
 }
 
 // For each run start: out_codes[r] = code >> shift, out_start[r] = i; member[i] = run index.
-__global__ void run_emit_kernel(const uint64_t* __restrict__ codes, int64_t n, int shift,
+__global__ void run_emit_kernel(const uint64_t* __restrict__ codes, int64_t n_bound,
+                                const int64_t* __restrict__ d_n, int shift,
                                 const int32_t* __restrict__ flags, const int32_t* __restrict__ excl,
                                 uint64_t* __restrict__ out_codes, int32_t* __restrict__ out_start,
                                 int32_t* __restrict__ member) {
   int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= count_of(d_n, n_bound)) return;
   int32_t r = excl[i] + flags[i] - 1;
   if (flags[i]) {
     out_codes[r] = codes[i] >> shift;
@@ -89,7 +98,15 @@ __global__ void run_emit_kernel(const uint64_t* __restrict__ codes, int64_t n, i
   member[i] = r;
 }
 
-__global__ void set_last_kernel(int32_t* a, int64_t idx, int32_t v) { a[idx] = v; }
+// runs = number of run starts; out_start[runs] = n.
+__global__ void run_count_kernel(const int32_t* __restrict__ flags, const int32_t* __restrict__ excl,
+                                 int64_t n_bound, const int64_t* __restrict__ d_n,
+                                 int32_t* __restrict__ out_start, int64_t* __restrict__ d_runs) {
+  const int64_t n = count_of(d_n, n_bound);
+  int64_t runs = n > 0 ? static_cast<int64_t>(excl[n - 1]) + flags[n - 1] : 0;
+  *d_runs = runs;
+  out_start[runs] = static_cast<int32_t>(n);
+}
 
 __global__ void p2p_list_kernel(const uint64_t* __restrict__ tcodes, int64_t nt,
                                 const uint64_t* __restrict__ scodes, int64_t ns, int64_t side,
@@ -134,18 +151,52 @@ __global__ void octant_key_kernel(const uint64_t* __restrict__ codes, int64_t n,
   idx[i] = static_cast<int32_t>(i);
 }
 
-struct Oct8 {
-  int32_t start[8];   // start of octant group in the sorted list
-  int32_t cstart[8];  // start of octant group in the padded column list
-};
+// Column layout of one level: octant groups in octant order, each padded to a
+// multiple of kM2LCols. lay[0..7] = start of each octant group among the cells
+// sorted by octant, lay[8..15] = its first column, lay[16] = number of columns.
+constexpr int kLayout = 17;
+
+__global__ void octant_layout_kernel(const uint64_t* __restrict__ sorted_keys, int64_t n,
+                                     int64_t* __restrict__ lay) {
+  if (threadIdx.x != 0) return;
+  int64_t start = 0, col = 0;
+  for (int k = 0; k < 8; k++) {
+    // first position whose key is > k (keys are sorted)
+    int64_t lo = start, hi = n;
+    while (lo < hi) {
+      int64_t mid = (lo + hi) >> 1;
+      if (sorted_keys[mid] <= static_cast<uint64_t>(k)) lo = mid + 1; else hi = mid;
+    }
+    int64_t cnt = lo - start;
+    lay[k] = start;
+    lay[8 + k] = col;
+    start = lo;
+    col += (cnt + kM2LCols - 1) / kM2LCols * kM2LCols;
+  }
+  lay[16] = col;
+}
 
 __global__ void fill_cols_kernel(const int32_t* __restrict__ sorted_idx,
-                                 const uint64_t* __restrict__ sorted_keys, int64_t n, Oct8 o,
-                                 int32_t* __restrict__ col_cell) {
+                                 const uint64_t* __restrict__ sorted_keys, int64_t n,
+                                 const int64_t* __restrict__ lay, int32_t* __restrict__ col_cell) {
   int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (j >= n) return;
   int oct = static_cast<int>(sorted_keys[j]);
-  col_cell[o.cstart[oct] + (j - o.start[oct])] = sorted_idx[j];
+  col_cell[lay[8 + oct] + (j - lay[oct])] = sorted_idx[j];
+}
+
+// blk_oct[b] = octant of column block b (blocks beyond the layout: -1)
+__global__ void fill_blk_oct_kernel(const int64_t* __restrict__ lay, int64_t n_blk_bound,
+                                    int32_t* __restrict__ blk_oct) {
+  int64_t b = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= n_blk_bound) return;
+  int64_t c = b * kM2LCols;
+  int oct = -1;
+  if (c < lay[16]) {
+    for (int k = 0; k < 8; k++)
+      if (c >= lay[8 + k]) oct = k;
+  }
+  blk_oct[b] = oct;
 }
 
 __global__ void fill_i32_kernel(int32_t* a, int64_t n, int32_t v) {
@@ -153,29 +204,26 @@ __global__ void fill_i32_kernel(int32_t* a, int64_t n, int32_t v) {
   if (i < n) a[i] = v;
 }
 
-// Unique runs of (codes >> shift). Returns the run count.
-int64_t unique_runs(const uint64_t* codes, int64_t n, int shift, DevBuf<uint64_t>& out_codes,
-                    DevBuf<int32_t>& out_start, DevBuf<int32_t>& member, TreeWorkspace& ws,
-                    hipStream_t s) {
-  ws.flags.resize(n);
-  ws.scan.resize(n);
-  member.resize(n);
-  run_flags_kernel<<<ceil_div(n, 256), 256, 0, s>>>(codes, n, shift, ws.flags.get());
+// Unique runs of (codes >> shift) over the first *d_n of n_bound entries
+// (d_n == nullptr: all n_bound); the run count goes to *d_runs on the device.
+// Output buffers are sized by n_bound.
+void unique_runs(const uint64_t* codes, int64_t n_bound, const int64_t* d_n, int shift,
+                 DevBuf<uint64_t>& out_codes, DevBuf<int32_t>& out_start, DevBuf<int32_t>& member,
+                 int64_t* d_runs, TreeWorkspace& ws, hipStream_t s) {
+  ws.flags.resize(n_bound);
+  ws.scan.resize(n_bound);
+  member.resize(n_bound);
+  out_codes.resize(n_bound);
+  out_start.resize(n_bound + 1);
+  run_flags_kernel<<<ceil_div(n_bound, 256), 256, 0, s>>>(codes, n_bound, d_n, shift, ws.flags.get());
   PAM_LAUNCH_CHECK();
-  exclusive_scan_i32(ws.flags.get(), ws.scan.get(), n, ws.scan_scratch, s);
-  int32_t pair[2];
-  PAM_HIP(hipMemcpyAsync(&pair[0], ws.scan.get() + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  PAM_HIP(hipMemcpyAsync(&pair[1], ws.flags.get() + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  PAM_HIP(hipStreamSynchronize(s));
-  int64_t runs = static_cast<int64_t>(pair[0]) + pair[1];
-  out_codes.resize(runs);
-  out_start.resize(runs + 1);
-  run_emit_kernel<<<ceil_div(n, 256), 256, 0, s>>>(codes, n, shift, ws.flags.get(), ws.scan.get(),
-                                                   out_codes.get(), out_start.get(), member.get());
+  exclusive_scan_i32(ws.flags.get(), ws.scan.get(), n_bound, ws.scan_scratch, s);
+  run_emit_kernel<<<ceil_div(n_bound, 256), 256, 0, s>>>(codes, n_bound, d_n, shift, ws.flags.get(),
+                                                         ws.scan.get(), out_codes.get(), out_start.get(),
+                                                         member.get());
   PAM_LAUNCH_CHECK();
-  set_last_kernel<<<1, 1, 0, s>>>(out_start.get(), runs, static_cast<int32_t>(n));
+  run_count_kernel<<<1, 1, 0, s>>>(ws.flags.get(), ws.scan.get(), n_bound, d_n, out_start.get(), d_runs);
   PAM_LAUNCH_CHECK();
-  return runs;
 }
 
 std::vector<int32_t> make_octant_vlist() {
@@ -244,15 +292,26 @@ void build_tree(Tree& t, const double* d_pts, int64_t n, const Box& box, int lev
   PAM_LAUNCH_CHECK();
   radix_sort_pairs(t.point_codes.get(), t.perm.get(), n, 3 * level, ws.tk, ws.tv, ws.hist,
                    ws.scan_scratch, s);
-  // leaves
+  // Leaves, then coarser levels as unique runs of the child codes >> 3. The
+  // cell counts stay on the device until all levels are built; buffers are
+  // sized by the bounds min(count of the finer level, 8^l).
+  ws.counts.resize(static_cast<size_t>(level) + 1);
+  std::vector<int64_t> bound(static_cast<size_t>(level) + 1);
+  bound[level] = n;
   TreeLevel& leaves = t.levels[level];
-  leaves.n = unique_runs(t.point_codes.get(), n, 0, leaves.codes, t.leaf_start, t.point_leaf, ws, s);
-  // coarser levels: unique runs of the child-level codes >> 3
+  unique_runs(t.point_codes.get(), n, nullptr, 0, leaves.codes, t.leaf_start, t.point_leaf,
+              ws.counts.get() + level, ws, s);
   for (int l = level - 1; l >= 0; l--) {
     TreeLevel& child = t.levels[l + 1];
     TreeLevel& cur = t.levels[l];
-    cur.n = unique_runs(child.codes.get(), child.n, 3, cur.codes, cur.child_start, child.parent, ws, s);
+    bound[l] = std::min<int64_t>(bound[l + 1], int64_t{1} << (3 * l));
+    unique_runs(child.codes.get(), bound[l + 1], ws.counts.get() + l + 1, 3, cur.codes, cur.child_start,
+                child.parent, ws.counts.get() + l, ws, s);
   }
+  std::vector<int64_t> counts(static_cast<size_t>(level) + 1);
+  PAM_HIP(hipMemcpyAsync(counts.data(), ws.counts.get(), counts.size() * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  PAM_HIP(hipStreamSynchronize(s));
+  for (int l = 0; l <= level; l++) t.levels[l].n = counts[l];
 }
 
 void build_lists(const Tree& trg, const Tree& src, Lists& lists, TreeWorkspace& ws, hipStream_t s) {
@@ -269,6 +328,11 @@ void build_lists(const Tree& trg, const Tree& src, Lists& lists, TreeWorkspace& 
   const int32_t* vl = octant_vlist_device(s);
   lists.m2l.clear();
   lists.m2l.resize(L + 1);
+  // Per level: the M2L slots, then the columns grouped by octant (stable, Morton
+  // order inside a group), each group padded to a multiple of kM2LCols so a
+  // column block has a single octant. The layouts of all levels come back to
+  // the host in one copy at the end.
+  ws.layout.resize(static_cast<size_t>(L + 1) * kLayout);
   for (int l = 2; l <= L; l++) {
     const TreeLevel& tl = trg.levels[l];
     const TreeLevel& sl = src.levels[l];
@@ -276,17 +340,13 @@ void build_lists(const Tree& trg, const Tree& src, Lists& lists, TreeWorkspace& 
     int64_t n = tl.n;
     m.src.resize(static_cast<size_t>(n) * kM2LSlots);
     m.n_cells = n;
+    m.n_cols = 0;
     m.h_blk_oct.clear();
-    if (n == 0) {
-      m.n_cols = 0;
-      continue;
-    }
+    if (n == 0) continue;
     int64_t work = n * kM2LSlots;
     m2l_list_kernel<<<ceil_div(work, 256), 256, 0, s>>>(tl.codes.get(), n, sl.codes.get(), sl.n,
                                                         int64_t{1} << l, vl, m.src.get());
     PAM_LAUNCH_CHECK();
-    // Columns grouped by octant (stable, Morton order inside a group), each group
-    // padded to a multiple of kM2LCols so a column block has a single octant.
     ws.tmp_codes.resize(n);
     ws.tmp_idx.resize(n);
     octant_key_kernel<<<ceil_div(n, 256), 256, 0, s>>>(tl.codes.get(), n, ws.tmp_codes.get(),
@@ -294,32 +354,33 @@ void build_lists(const Tree& trg, const Tree& src, Lists& lists, TreeWorkspace& 
     PAM_LAUNCH_CHECK();
     radix_sort_pairs(ws.tmp_codes.get(), ws.tmp_idx.get(), n, 3, ws.tk, ws.tv, ws.hist,
                      ws.scan_scratch, s);
-    std::vector<uint64_t> keys(n);
-    PAM_HIP(hipMemcpyAsync(keys.data(), ws.tmp_codes.get(), n * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-    PAM_HIP(hipStreamSynchronize(s));
-    Oct8 o{};
-    int64_t cnt[8] = {};
-    for (int64_t j = 0; j < n; j++) cnt[keys[j]]++;
-    int64_t a = 0, c = 0;
-    std::vector<int32_t> blk;
-    for (int k = 0; k < 8; k++) {
-      o.start[k] = static_cast<int32_t>(a);
-      o.cstart[k] = static_cast<int32_t>(c);
-      int64_t padded = (cnt[k] + kM2LCols - 1) / kM2LCols * kM2LCols;
-      for (int64_t b = 0; b < padded / kM2LCols; b++) blk.push_back(k);
-      a += cnt[k];
-      c += padded;
-    }
-    m.n_cols = c;
-    m.col_cell.resize(c);
-    fill_i32_kernel<<<ceil_div(c, 256), 256, 0, s>>>(m.col_cell.get(), c, -1);
+    int64_t* lay = ws.layout.get() + static_cast<int64_t>(l) * kLayout;
+    octant_layout_kernel<<<1, 64, 0, s>>>(ws.tmp_codes.get(), n, lay);
     PAM_LAUNCH_CHECK();
-    fill_cols_kernel<<<ceil_div(n, 256), 256, 0, s>>>(ws.tmp_idx.get(), ws.tmp_codes.get(), n, o,
+    const int64_t col_bound = n + 8 * (kM2LCols - 1);
+    const int64_t blk_bound = col_bound / kM2LCols + 1;
+    m.col_cell.resize(col_bound);
+    fill_i32_kernel<<<ceil_div(col_bound, 256), 256, 0, s>>>(m.col_cell.get(), col_bound, -1);
+    PAM_LAUNCH_CHECK();
+    fill_cols_kernel<<<ceil_div(n, 256), 256, 0, s>>>(ws.tmp_idx.get(), ws.tmp_codes.get(), n, lay,
                                                       m.col_cell.get());
     PAM_LAUNCH_CHECK();
-    m.h_blk_oct = blk;
-    m.blk_oct.upload(m.h_blk_oct.data(), m.h_blk_oct.size(), s);
-    PAM_HIP(hipStreamSynchronize(s));  // the upload reads host memory
+    m.blk_oct.resize(blk_bound);
+    fill_blk_oct_kernel<<<ceil_div(blk_bound, 256), 256, 0, s>>>(lay, blk_bound, m.blk_oct.get());
+    PAM_LAUNCH_CHECK();
+  }
+  std::vector<int64_t> lay(static_cast<size_t>(L + 1) * kLayout, 0);
+  PAM_HIP(hipMemcpyAsync(lay.data(), ws.layout.get(), lay.size() * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  PAM_HIP(hipStreamSynchronize(s));
+  for (int l = 2; l <= L; l++) {
+    M2LLevel& m = lists.m2l[l];
+    if (m.n_cells == 0) continue;
+    const int64_t* ly = lay.data() + static_cast<int64_t>(l) * kLayout;
+    m.n_cols = ly[16];
+    for (int k = 0; k < 8; k++) {
+      int64_t end = k < 7 ? ly[8 + k + 1] : ly[16];
+      for (int64_t c = ly[8 + k]; c < end; c += kM2LCols) m.h_blk_oct.push_back(k);
+    }
   }
 }
 

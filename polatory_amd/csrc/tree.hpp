@@ -46,6 +46,8 @@ struct TreeWorkspace {
   DevBuf<int32_t> flags, scan, scan_scratch, hist, tv;
   DevBuf<uint64_t> tk, tmp_codes;
   DevBuf<int32_t> tmp_idx;
+  DevBuf<int64_t> counts;  // per-level cell counts while a tree is built
+  DevBuf<int64_t> layout;  // per-level M2L column layouts while lists are built
 };
 
 struct M2LLevel {

@@ -62,7 +62,7 @@ class FitOptions:
     pam_fit_options_default)."""
 
     def __init__(self, op=None, residual=None, level=None, use_preconditioner=True,
-                 refine_rtol=None, verbose=False):
+                 refine_rtol=None, verbose=False, use_graphs=True):
         d = _capi.PamFitOptions()
         _capi.call("pam_fit_options_default", ctypes.byref(d))
         self._c = d
@@ -77,6 +77,7 @@ class FitOptions:
         if refine_rtol is not None:
             d.refine_rtol = refine_rtol
         d.verbose = 1 if verbose else 0
+        d.use_graphs = 1 if use_graphs else 0
 
 
 class Interpolant:

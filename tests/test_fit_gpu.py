@@ -167,6 +167,28 @@ def test_fit_with_mixed_fmm_operator_refines_to_tolerance(orc):
     assert res <= 1e-5
 
 
+def test_graph_replay_matches_eager(orc):
+    """The Arnoldi steps replayed from the captured HIP graph give the same
+    iterates, bit for bit, as the eager steps (eager residual checks and
+    synchronizations run between the replays)."""
+    log("graph replay vs eager")
+    n = 20000
+    pts = np.random.default_rng(31).uniform(0, 1, (n, 3))
+    vals = sample_values(pts)
+    out = {}
+    for g in (False, True):
+        ip = pa.Interpolant(pa.Model(pa.Biharmonic3D(), poly_degree=1))
+        ip.fit(pts, vals, 1e-6, options=pa.FitOptions(use_graphs=g))
+        out[g] = (ip.weights(), ip.fit_stats())
+    (w0, s0), (w1, s1) = out[False], out[True]
+    log(f"  iterations {s0['iterations']} / {s1['iterations']}, replays {s1['graph_replays']}")
+    assert s0["graph_replays"] == 0
+    assert s1["iterations"] == s0["iterations"]
+    assert s1["graph_replays"] >= max(0, s1["iterations"] - 2 * (s1["restarts"] + 1))
+    np.testing.assert_array_equal(w1, w0)
+    assert oracle_residual(orc, pa.Model(pa.Biharmonic3D(), poly_degree=1), pts, vals, w1) <= 1e-6
+
+
 def test_warm_start_and_max_iter(orc):
     log("warm start")
     rng = np.random.default_rng(27)
