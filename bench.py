@@ -220,11 +220,19 @@ def serve_section(pa, n_models, n_requests, rate):
     srv = pa.Server(n_slots=4, rel_accuracy=1e-6)
     ips, mids = [], []
     t0 = time.perf_counter()
+    n_conv, worst_res = 0, 0.0
     for i, (n, k) in enumerate(zip(sizes, kinds)):
         p = np.random.default_rng(1000 + i).uniform(0, 1, (n, 3))
         rbf = pa.Biharmonic3D() if k == 0 else pa.Triharmonic3D()
         ip = pa.Interpolant(pa.Model(rbf, poly_degree=1))
-        ip.fit(p, sample_f(p), 1e-4)
+        # Large triharmonic systems converge slowly under the multilevel RAS; a
+        # model that has not reached 1e-4 after 30 steps is served with the
+        # weights reached (its residual is reported), since serving latency and
+        # accuracy do not depend on how well the weights fit the data.
+        ip.fit(p, sample_f(p), 1e-4, max_iter=30, options=pa.FitOptions(keep_unconverged=True))
+        st = ip.fit_stats()
+        n_conv += st["converged"]
+        worst_res = max(worst_res, st["residual"])
         mids.append(srv.add_model(ip))
         ips.append((ip, rbf))
     t_models = time.perf_counter() - t0
@@ -258,6 +266,20 @@ def serve_section(pa, n_models, n_requests, rate):
         if i in check:
             outs[i] = o
     st = srv.stats()
+    # service time (admission to completion) by query-count decade
+    by_decade = {}
+    for i, r in enumerate(rids):
+        q_ms, t_ms = srv.request_times(r)
+        d = int(np.log10(max(counts[i], 1)))
+        by_decade.setdefault(d, []).append(t_ms - q_ms)
+    service = {f"1e{d}": {"n": len(v), "mean_ms": float(np.mean(v)), "max_ms": float(np.max(v))}
+               for d, v in sorted(by_decade.items())}
+    cfgs = {}
+    for m in mids:
+        c = srv.model_config(m)[0]
+        key = f"{c.precision}{c.order}"
+        cfgs[key] = cfgs.get(key, 0) + 1
+    log("serve service times", json.dumps(service), "configs", json.dumps(cfgs))
     # accuracy of a sample of answers vs the FP64 direct sum on the GPU
     worst = 0.0
     for i, o in outs.items():
@@ -274,7 +296,9 @@ def serve_section(pa, n_models, n_requests, rate):
             "max_ms": st["max_ms"], "mean_queue_ms": st["mean_queue_ms"],
             "points_per_s": st["points_per_s"], "completed": st["completed"], "failed": st["failed"],
             "max_in_flight": st["max_in_flight"], "resident_gib": resident / 2**30,
-            "max_rel_err_sampled": worst, "models_ready_s": t_models}
+            "max_rel_err_sampled": worst, "models_ready_s": t_models,
+            "models_fit_to_1e-4": n_conv, "worst_fit_residual": worst_res,
+            "service_ms_by_size": service, "model_configs": cfgs}
 
 
 def time_steps(torch, de, steps, warmup):

@@ -162,6 +162,7 @@ typedef struct {
   double refine_rtol;       /* inner relative residual that restarts FGMRES on the FP64 residual */
   int verbose;              /* print the residual per iteration (solver.hpp:111-128) */
   int use_graphs;           /* replay "preconditioner + matvec" of each FGMRES step from a HIP graph */
+  int keep_unconverged;     /* at max_iter keep the last iterate instead of failing (default 0) */
 } pam_fit_options;
 int pam_fit_options_default(pam_fit_options* opt);
 /* Interpolant(const Model&) — interpolant.hpp:43 */
@@ -193,6 +194,8 @@ typedef struct {
   double residual, seconds, setup_seconds, precond_setup_seconds;
   int64_t factor_bytes;
   int graph_replays;        /* FGMRES steps replayed from the captured graph */
+  int escalations;          /* precision raises after FGMRES stagnated (FP64 level updates, then operator) */
+  int converged;            /* 0 when keep_unconverged stopped the fit at max_iter */
 } pam_fit_stats;
 int pam_interpolant_fit_stats(const pam_interpolant* h, pam_fit_stats* out);
 /* Residual after each FGMRES iteration (out may be NULL to query the length). */

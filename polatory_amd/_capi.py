@@ -40,14 +40,15 @@ class PamFitOptions(ctypes.Structure):
     _fields_ = [("op", PamFmmConfig), ("residual", PamFmmConfig), ("level", PamFmmConfig),
                 ("use_level_config", ctypes.c_int), ("use_preconditioner", ctypes.c_int),
                 ("refine_rtol", ctypes.c_double), ("verbose", ctypes.c_int),
-                ("use_graphs", ctypes.c_int)]
+                ("use_graphs", ctypes.c_int), ("keep_unconverged", ctypes.c_int)]
 
 
 class PamFitStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("restarts", ctypes.c_int), ("n_levels", ctypes.c_int),
                 ("residual", ctypes.c_double), ("seconds", ctypes.c_double),
                 ("setup_seconds", ctypes.c_double), ("precond_setup_seconds", ctypes.c_double),
-                ("factor_bytes", ctypes.c_int64), ("graph_replays", ctypes.c_int)]
+                ("factor_bytes", ctypes.c_int64), ("graph_replays", ctypes.c_int),
+                ("escalations", ctypes.c_int), ("converged", ctypes.c_int)]
 
 
 class PamServerOptions(ctypes.Structure):

@@ -53,6 +53,7 @@ pam::FitOptions to_opts(const pam_fit_options* o) {
   r.use_preconditioner = o->use_preconditioner != 0;
   r.refine_rtol = o->refine_rtol;
   r.use_graphs = o->use_graphs != 0;
+  r.keep_unconverged = o->keep_unconverged != 0;
   r.verbose = o->verbose;
   return r;
 }
@@ -135,6 +136,7 @@ int pam_fit_options_default(pam_fit_options* opt) {
     opt->use_preconditioner = 1;
     opt->refine_rtol = o.refine_rtol;
     opt->use_graphs = o.use_graphs ? 1 : 0;
+    opt->keep_unconverged = o.keep_unconverged ? 1 : 0;
     opt->verbose = 0;
   });
 }
@@ -222,6 +224,8 @@ int pam_interpolant_fit_stats(const pam_interpolant* h, pam_fit_stats* out) {
     out->precond_setup_seconds = s.precond_setup_seconds;
     out->factor_bytes = s.factor_bytes;
     out->graph_replays = s.graph_replays;
+    out->escalations = s.escalations;
+    out->converged = s.converged ? 1 : 0;
   });
 }
 

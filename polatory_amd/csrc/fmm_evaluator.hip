@@ -100,8 +100,7 @@ __global__ void identity_cols_kernel(int32_t* cols, int64_t n, int64_t n_cols) {
 }
 
 void build_identity_cols(const Tree& t, std::vector<M2LLevel>& out, hipStream_t s) {
-  out.clear();
-  out.resize(t.level + 1);
+  if (out.size() < static_cast<size_t>(t.level) + 1) out.resize(t.level + 1);
   for (int l = 2; l <= t.level; l++) {
     int64_t n = t.levels[l].n;
     M2LLevel& m = out[l];
@@ -151,10 +150,20 @@ void make_box_from_bbox(const double* bmin, const double* bmax, const double* an
 
 }  // namespace
 
-int auto_leaf_level(int64_t n_src, int64_t n_trg) {
+int auto_leaf_level(int64_t n_src, int64_t n_trg, double points_per_leaf) {
   double n = static_cast<double>(std::max<int64_t>(std::max(n_src, n_trg), 1));
-  int l = static_cast<int>(std::lround(std::log(n / 128.0) / std::log(8.0)));
+  int l = static_cast<int>(std::lround(std::log(n / points_per_leaf) / std::log(8.0)));
   return std::clamp(l, 2, 18);
+}
+
+double leaf_occupancy_target(int order, int precision) {
+  // Per leaf of n points the near field costs ~27 n^2 pair evaluations (rate
+  // R_pair) and the far field ~1.14 x 189 dense transfers of 2 P^6 flops
+  // (rate R_m2l); the cost per point is least at n = P^3 sqrt(16 R_pair/R_m2l):
+  // 0.8 P^3 in the mixed form (1.9e12 pairs/s, 4.8e13 flop/s measured on
+  // MI355X), 0.66 P^3 in FP64 (6.6e11 pairs/s, 2.4e13 flop/s).
+  const double c = precision == kFp64 ? 0.66 : 0.8;
+  return std::max(64.0, c * static_cast<double>(order) * order * order);
 }
 
 FmmEvaluator::FmmEvaluator(const RbfDesc& rbf, const double* bbox_min, const double* bbox_max,
@@ -260,13 +269,35 @@ void FmmEvaluator::set_weights(const double* w, int64_t n, bool on_device) {
   if (!on_device) PAM_HIP(hipStreamSynchronize(stream_));
 }
 
-bool FmmEvaluator::use_direct() const {
+bool FmmEvaluator::use_direct() {
   if (cfg_.mode == kModeDirect) return true;
   if (cfg_.mode == kModeFmm) return false;
-  int64_t ns = S().n_src_;
-  int64_t nt = symmetric_ ? n_src_ : n_trg_;
+  const int64_t ns = S().n_src_;
+  const int64_t nt = symmetric_ ? n_src_ : n_trg_;
+  const double pairs = static_cast<double>(ns) * static_cast<double>(nt);
   // Below ~2^26 pairs the FP64 direct sum is both exact and cheaper on one GPU.
-  return static_cast<double>(ns) * static_cast<double>(nt) < 67108864.0;
+  if (pairs < 67108864.0) return true;
+  if (symmetric_) return false;
+  // Few or clustered targets against many sources: the far field costs one
+  // dense transfer set (189 x 2 P^6 flops) per occupied target cell and level
+  // whatever the number of targets in it, the FP64 direct sum n_src pair
+  // evaluations per target. The trees are built to count the occupied cells
+  // (they are needed for the FMM anyway). Rates measured on MI355X.
+  ensure_trees();
+  const FmmEvaluator& src = S();
+  const Tree& tt = target_tree();
+  const int P = cfg_.order;
+  const int L = tt.level;
+  double cells = 0.0;
+  for (int l = 2; l <= L; l++) cells += static_cast<double>(tt.levels[l].n);
+  const bool fp64 = cfg_.precision == kFp64;
+  const double p6 = std::pow(static_cast<double>(P), 6);
+  const double m2l_s = cells * kM2LSlots * 2.0 * p6 / (fp64 ? 2.4e13 : 4.8e13);
+  const double src_leaves = std::max<double>(1.0, static_cast<double>(src.src_tree_.levels[L].n));
+  const double p2p_s = static_cast<double>(nt) * 27.0 * (static_cast<double>(ns) / src_leaves) /
+                       (fp64 ? 6.6e11 : 1.9e12);
+  const double direct_s = pairs / 8e11;
+  return direct_s < m2l_s + p2p_s;
 }
 
 void FmmEvaluator::ensure_box() {
@@ -338,31 +369,96 @@ void FmmEvaluator::ensure_trees() {
       rebuilt = true;
       st_.t32_valid = st_.t64_valid = false;
     }
-    if (rebuilt || lists_.m2l.size() != static_cast<size_t>(L + 1)) {
+    if (rebuilt || lists_.level != L) {
       build_lists(trg_tree_, stree, lists_, ws_, stream_);
       lists_gen_++;
     }
     return;
   }
-  int L = cfg_.leaf_level >= 0 ? cfg_.leaf_level : auto_leaf_level(n_src_, n_trg());
   bool rebuilt = false;
-  if (src_tree_dirty_ || src_tree_.level != L) {
-    build_tree(src_tree_, src_pts_.get(), n_src_, box_, L, ws_, stream_);
-    build_identity_cols(src_tree_, src_cols_, stream_);
+  auto build_src = [&](int level) {
+    build_tree(src_tree_, src_pts_.get(), n_src_, box_, level, ws_, stream_);
     src_tree_dirty_ = false;
     rebuilt = true;
     st_.s32_valid = st_.s64_valid = false;
     st_.multipoles_valid = false;
-  }
-  if (!symmetric_ && (trg_tree_dirty_ || trg_tree_.level != L)) {
-    build_tree(trg_tree_, trg_pts_.get(), n_trg_, box_, L, ws_, stream_);
-    build_identity_cols(trg_tree_, trg_cols_, stream_);
+  };
+  auto build_trg = [&](int level) {
+    build_tree(trg_tree_, trg_pts_.get(), n_trg_, box_, level, ws_, stream_);
     trg_tree_dirty_ = false;
     rebuilt = true;
     st_.t32_valid = st_.t64_valid = false;
+  };
+  auto counts_of = [](const Tree& t) {
+    std::vector<int64_t> c(static_cast<size_t>(t.level) + 1);
+    for (size_t l = 0; l < c.size(); l++) c[l] = t.levels[l].n;
+    return c;
+  };
+  int L;
+  if (cfg_.leaf_level >= 0) {
+    L = cfg_.leaf_level;
+    if (src_tree_dirty_ || src_tree_.level != L) build_src(L);
+    if (!symmetric_ && (trg_tree_dirty_ || trg_tree_.level != L)) build_trg(L);
+    src_counts_.clear();
+    trg_counts_.clear();
+  } else {
+    // Leaf level from the occupied cells: both trees are built down to
+    // kLevelsBeyond levels below the uniform-density choice, the level whose
+    // occupied leaves hold closest to kPointsPerLeaf points (in ratio) is
+    // kept, and the trees are cut back to it. Clustered or surface-like point
+    // sets get deeper trees than their point count alone suggests.
+    constexpr int kLevelsBeyond = 2;
+    const double kPointsPerLeaf = leaf_occupancy_target(cfg_.order, cfg_.precision);
+    const int L0 = auto_leaf_level(n_src_, n_trg(), kPointsPerLeaf);
+    const int Lmax = std::min(L0 + kLevelsBeyond, 18);
+    if (src_tree_dirty_ || static_cast<int>(src_counts_.size()) != Lmax + 1) {
+      build_src(Lmax);
+      src_counts_ = counts_of(src_tree_);
+    }
+    if (!symmetric_ && (trg_tree_dirty_ || static_cast<int>(trg_counts_.size()) != Lmax + 1)) {
+      build_trg(Lmax);
+      trg_counts_ = counts_of(trg_tree_);
+    }
+    L = L0;
+    double best = INFINITY;
+    for (int l = L0; l <= Lmax; l++) {
+      double occ = 0.0;
+      if (n_src_ > 0 && src_counts_[l] > 0) occ = std::max(occ, static_cast<double>(n_src_) / src_counts_[l]);
+      if (!symmetric_ && n_trg_ > 0 && trg_counts_[l] > 0)
+        occ = std::max(occ, static_cast<double>(n_trg_) / trg_counts_[l]);
+      if (occ <= 0.0) continue;
+      double d = std::abs(std::log(occ / kPointsPerLeaf));
+      if (d < best) {
+        best = d;
+        L = l;
+      }
+    }
+    if (src_tree_.level != L) {
+      if (src_tree_.level < L) {
+        build_src(Lmax);
+        src_counts_ = counts_of(src_tree_);
+      }
+      truncate_tree(src_tree_, L, ws_, stream_);
+      rebuilt = true;
+      st_.s32_valid = st_.s64_valid = false;
+      st_.multipoles_valid = false;
+    }
+    if (!symmetric_ && trg_tree_.level != L) {
+      if (trg_tree_.level < L) {
+        build_trg(Lmax);
+        trg_counts_ = counts_of(trg_tree_);
+      }
+      truncate_tree(trg_tree_, L, ws_, stream_);
+      rebuilt = true;
+      st_.t32_valid = st_.t64_valid = false;
+    }
   }
   if (symmetric_) trg_tree_dirty_ = false;
-  if (rebuilt || lists_.m2l.size() != static_cast<size_t>(L + 1)) {
+  if (rebuilt) {
+    build_identity_cols(src_tree_, src_cols_, stream_);
+    if (!symmetric_) build_identity_cols(trg_tree_, trg_cols_, stream_);
+  }
+  if (rebuilt || lists_.level != L) {
     build_lists(target_tree(), src_tree_, lists_, ws_, stream_);
     lists_gen_++;
   }
@@ -402,7 +498,7 @@ void FmmEvaluator::attach_source(const FmmEvaluator* owner) {
   }
   box_valid_ = false;
   trg_tree_dirty_ = true;
-  lists_ = Lists{};
+  lists_.level = -1;  // rebuilt on the next evaluation (buffers kept)
   lists_gen_++;
   st_.t32_valid = st_.t64_valid = false;
   // Targets are transformed with the rbf's anisotropy: they must be set again.
@@ -427,10 +523,9 @@ void FmmEvaluator::prepare_source_expansions() {
   const int P = cfg_.order;
   const int kp = kpad_of(P);
   const int L = src_tree_.level;
-  if (st.m.size() != static_cast<size_t>(L + 1) || st.multipole_order != P) {
-    st.m.clear();
-    st.m.resize(L + 1);
-    st.mt.clear();
+  if (st.m.size() < static_cast<size_t>(L) + 1) st.m.resize(L + 1);
+  if (st.m_level != L || st.multipole_order != P) {
+    st.m_level = L;
     st.multipoles_valid = false;
     st.multipole_order = P;
   }
@@ -484,11 +579,7 @@ void FmmEvaluator::run_fmm(double* d_out) {
              (sizeof(TP) == 4 && (!sst.s32_valid || !sst.m32_valid))) {
     throw_runtime("the shared source side is not prepared (call prepare_source on the owner)");
   }
-  if (st.l.size() != static_cast<size_t>(L + 1)) {
-    st.l.clear();
-    st.l.resize(L + 1);
-    st.lt.clear();
-  }
+  if (st.l.size() < static_cast<size_t>(L) + 1) st.l.resize(L + 1);
   if (!symmetric_ && !st.t64_valid) {
     st.tpts64.resize(n_trg_);
     launch_prepare_points<double>(trg_tree_, trg_pts_.get(), nullptr, st.tpts64.get(), stream_);
@@ -517,12 +608,8 @@ void FmmEvaluator::run_fmm(double* d_out) {
   for (int l = 2; l <= L; l++) st.l[l].resize(static_cast<size_t>(tt.levels[l].n) * kp);
   if (tol > 0.0) {
     // SVD-compressed M2L: compress multipoles, transfer in rank r, expand locals.
-    if (st.mt.size() != static_cast<size_t>(L + 1)) {
-      st.mt.clear();
-      st.lt.clear();
-      st.mt.resize(L + 1);
-      st.lt.resize(L + 1);
-    }
+    if (st.mt.size() < static_cast<size_t>(L) + 1) st.mt.resize(L + 1);
+    if (st.lt.size() < static_cast<size_t>(L) + 1) st.lt.resize(L + 1);
     std::vector<std::shared_ptr<CompressedM2L<TP>>> ops(L + 1);
     std::shared_ptr<CompressedM2L<TP>> unit;
     if (deg > 0) unit = get_compressed<TP>(P, rbf_, 1.0, tol, stream_);
@@ -646,7 +733,7 @@ void FmmEvaluator::choose_config() {
         FmmConfig t;
         t.precision = cd[0];
         t.order = cd[1];
-        t.leaf_level = auto_leaf_level(n_src_, n_trg());
+        t.leaf_level = -1;
         t.mode = kModeFmm;
         probe.set_config(t);
         probe.evaluate(approx.data(), m, false);

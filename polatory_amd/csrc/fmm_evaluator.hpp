@@ -56,6 +56,7 @@ struct FmmState {
   bool s64_valid = false, t64_valid = false, s32_valid = false, t32_valid = false;
   bool multipoles_valid = false, m32_valid = false;
   int multipole_order = 0;
+  int m_level = -1;  // leaf level the multipoles were computed for
 };
 
 class FmmEvaluator {
@@ -105,7 +106,7 @@ class FmmEvaluator {
   void choose_config();
   void ensure_box();
   void ensure_trees();
-  bool use_direct() const;
+  bool use_direct();
   // The evaluator that holds the source side (this one unless attached).
   const FmmEvaluator& S() const { return so_ ? *so_ : *this; }
   template <class TP>
@@ -134,6 +135,9 @@ class FmmEvaluator {
   bool box_valid_ = false;
 
   Tree src_tree_, trg_tree_;
+  // per-level cell counts of the trees as built before the leaf level was
+  // chosen (automatic leaf level only)
+  std::vector<int64_t> src_counts_, trg_counts_;
   Lists lists_;
   // all cells of each level as M2L columns (compress / decompress transforms)
   std::vector<M2LLevel> src_cols_, trg_cols_;
@@ -150,8 +154,11 @@ class FmmEvaluator {
   bool timing_valid_ = true;  // false when the last evaluation was captured into a graph
 };
 
-// Tree height used when no leaf level is configured: target occupancy of
-// about 128 points per leaf (balances the dense M2L against P2P), clamped to [2, 18].
-int auto_leaf_level(int64_t n_src, int64_t n_trg);
+// Leaf level for uniformly spread points: about points_per_leaf points per
+// leaf, clamped to [2, 18]. The evaluator starts its occupancy search there.
+int auto_leaf_level(int64_t n_src, int64_t n_trg, double points_per_leaf);
+// Points per occupied leaf that balance the near field against the dense M2L
+// at interpolation order `order`.
+double leaf_occupancy_target(int order, int precision);
 
 }  // namespace pam

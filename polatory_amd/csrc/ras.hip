@@ -875,21 +875,7 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
     p_orth_.resize(static_cast<size_t>(mu_) * l_);
     dev_poly_matrix(model_.poly_degree, d_pts_.get(), mu_, p_orth_.get(), s_);
     dev_orthonormalize_cols(p_orth_.get(), mu_, l_, ws_, s_);
-    // ap = A P (ras_preconditioner.hpp:173-188)
-    ap_.resize(static_cast<size_t>(mu_) * l_);
-    Bbox3 bb;
-    for (int64_t i = 0; i < mu_; i++) bb.add(&h_pts_[3 * i]);
-    SymmetricEvaluator ev(model_, bb.lo, bb.hi, INFINITY, s_);
-    if (opt_.use_level_cfg) ev.set_fmm_config(opt_.level_cfg);
-    ev.set_points(d_pts_.get(), mu_);
-    DevBuf<double> wcol(static_cast<size_t>(mu_ + l_));
-    for (int k = 0; k < l_; k++) {
-      dev_fill(0.0, wcol.get(), mu_ + l_, s_);
-      dev_copy(p_orth_.get() + static_cast<int64_t>(k) * mu_, wcol.get(), mu_, s_);
-      ev.set_weights(wcol.get());
-      ev.evaluate(ap_.get() + static_cast<int64_t>(k) * mu_);
-    }
-    PAM_HIP(hipStreamSynchronize(s_));
+    compute_ap();
   }
   PAM_HIP(hipStreamSynchronize(s_));
   tick("A P done");
@@ -959,6 +945,32 @@ void RasPreconditioner::solve(int level, const double* d_res, double* d_w) {
                                                 b.inner.get(), lag_.get(), b.mat_ptr.get(), b.mat.get(), d_res, d_w,
                                                 mu_, level == 0 ? 1 : 0, pinv_.get());
   PAM_LAUNCH_CHECK();
+}
+
+void RasPreconditioner::compute_ap() {
+  // ap = A P (ras_preconditioner.hpp:173-188), with the level-update configuration
+  ap_.resize(static_cast<size_t>(mu_) * l_);
+  Bbox3 bb;
+  for (int64_t i = 0; i < mu_; i++) bb.add(&h_pts_[3 * i]);
+  SymmetricEvaluator ev(model_, bb.lo, bb.hi, INFINITY, s_);
+  if (opt_.use_level_cfg) ev.set_fmm_config(opt_.level_cfg);
+  ev.set_points(d_pts_.get(), mu_);
+  DevBuf<double> wcol(static_cast<size_t>(mu_ + l_));
+  for (int k = 0; k < l_; k++) {
+    dev_fill(0.0, wcol.get(), mu_ + l_, s_);
+    dev_copy(p_orth_.get() + static_cast<int64_t>(k) * mu_, wcol.get(), mu_, s_);
+    ev.set_weights(wcol.get());
+    ev.evaluate(ap_.get() + static_cast<int64_t>(k) * mu_);
+  }
+  PAM_HIP(hipStreamSynchronize(s_));
+}
+
+void RasPreconditioner::set_level_config(const FmmConfig& cfg) {
+  PAM_HIP(hipStreamSynchronize(s_));
+  opt_.level_cfg = cfg;
+  opt_.use_level_cfg = true;
+  evaluators_.clear();
+  if (n_levels_ > 1 && l_ > 0) compute_ap();
 }
 
 Evaluator& RasPreconditioner::evaluator(int src, int trg) {

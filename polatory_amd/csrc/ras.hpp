@@ -62,6 +62,10 @@ class RasPreconditioner : public LinearOperator {
   const std::vector<std::vector<int32_t>>& level_points() const { return point_idcs_; }
   const std::vector<Domain>& level_domains(int level) const { return h_domains_.at(level); }
   double setup_seconds() const { return setup_s_; }
+  // Changes the FMM configuration of the inter-level residual updates (and of
+  // A P); the update evaluators are rebuilt on the next apply.
+  void set_level_config(const FmmConfig& cfg);
+  const RasOptions& options() const { return opt_; }
   int64_t factor_bytes() const;
 
  private:
@@ -78,6 +82,7 @@ class RasPreconditioner : public LinearOperator {
   void solve(int level, const double* d_res, double* d_w);
   void update_residuals(int src, int trg, const double* d_w, double* d_res);
   void orthogonalize(double* d_w, double* d_res);
+  void compute_ap();
   Evaluator& evaluator(int src, int trg);
 
   Model model_;

@@ -1,6 +1,7 @@
 // Fit operator, residual evaluator and the FGMRES solver (see solver.hpp).
 #include "solver.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -128,6 +129,16 @@ Solver::Solver(const Model& model, const std::vector<double>& points, const FitO
   d_pts_.upload(points.data(), points.size(), s_);
   op_ = std::make_unique<Operator>(model_, d_pts_.get(), mu_, opt_.op_cfg, s_);
   res_ = std::make_unique<ResidualEvaluator>(model_, d_pts_.get(), mu_, opt_.res_cfg, s_);
+  if (!opt_.use_level_cfg && model_.cpd_order() >= 2) {
+    // Kernels of conditional positive definiteness order 2 (triharmonic3d)
+    // give ill-conditioned systems whose multilevel updates need FP64: with
+    // the default mixed-precision updates FGMRES stalls above 1e-3 relative
+    // (measured at 2e5 points), with FP64 order-8 updates it converges.
+    opt_.level_cfg = FmmConfig{};
+    opt_.level_cfg.precision = kFp64;
+    opt_.level_cfg.order = 8;
+    opt_.use_level_cfg = true;
+  }
   if (opt_.use_preconditioner) {
     RasOptions ro;
     ro.level_cfg = opt_.level_cfg;
@@ -143,8 +154,6 @@ Solver::Solver(const Model& model, const std::vector<double>& points, const FitO
   PAM_HIP(hipStreamSynchronize(s_));
   setup_s_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
-
-namespace {
 
 // Wall time of a linear operator's applications (verbose fits only: each
 // application synchronizes the stream).
@@ -243,7 +252,48 @@ class GraphStep {
   bool failed_ = false;
 };
 
-}  // namespace
+bool Solver::can_escalate() const {
+  bool levels_fp64 = !pc_ || (pc_->options().use_level_cfg && pc_->options().level_cfg.precision == kFp64);
+  return !(levels_fp64 && opt_.op_cfg.precision == kFp64 && opt_.op_cfg.order >= 12);
+}
+
+void Solver::escalate(std::unique_ptr<GraphStep>& gstep, std::unique_ptr<TimedOperator>& t_op,
+                      std::unique_ptr<TimedOperator>& t_pc, LinearOperator*& op, LinearOperator*& pc,
+                      FitStats& st) {
+  PAM_HIP(hipStreamSynchronize(s_));
+  const char* what = nullptr;
+  if (pc_ && !(pc_->options().use_level_cfg && pc_->options().level_cfg.precision == kFp64)) {
+    FmmConfig c;
+    c.precision = kFp64;
+    c.order = 8;
+    pc_->set_level_config(c);
+    what = "FP64 multilevel updates";
+  } else if (opt_.op_cfg.precision != kFp64 || opt_.op_cfg.order < 10) {
+    opt_.op_cfg.precision = kFp64;
+    opt_.op_cfg.order = std::max(opt_.op_cfg.order, 10);
+    op_ = std::make_unique<Operator>(model_, d_pts_.get(), mu_, opt_.op_cfg, s_);
+    what = "FP64 operator of order 10";
+  } else if (opt_.op_cfg.order < 12) {
+    opt_.op_cfg.order = 12;
+    op_ = std::make_unique<Operator>(model_, d_pts_.get(), mu_, opt_.op_cfg, s_);
+    what = "FP64 operator of order 12";
+  } else {
+    return;  // nothing left to raise
+  }
+  st.escalations++;
+  if (opt_.verbose) std::fprintf(stderr, "  stagnation: switching to %s\n", what);
+  op = op_.get();
+  pc = pc_.get();
+  if (t_op) {
+    t_op = std::make_unique<TimedOperator>(*op_, s_);
+    op = t_op.get();
+  }
+  if (t_pc && pc_) {
+    t_pc = std::make_unique<TimedOperator>(*pc_, s_);
+    pc = t_pc.get();
+  }
+  if (gstep) gstep = std::make_unique<GraphStep>(*op_, pc_.get(), mu_ + l_, s_);
+}
 
 std::vector<double> Solver::solve(const std::vector<double>& values, double tolerance, int max_iter,
                                   const std::vector<double>* initial, FitStats* stats) {
@@ -317,7 +367,8 @@ std::vector<double> Solver::solve(const std::vector<double>& values, double tole
       dev_copy(xb.get(), x.get(), m, s_);
       break;
     }
-    bool done = false, restart = false;
+    bool done = false, restart = false, stagnated = false;
+    std::vector<double> inner_rel;
     while (true) {
       solver.solution_vector(x.get());
       dev_axpy(1.0, xb.get(), x.get(), m, s_);
@@ -333,10 +384,25 @@ std::vector<double> Solver::solve(const std::vector<double>& values, double tole
         done = true;
         break;
       }
-      if (total_iter + solver.iteration_count() >= max_iter)
-        throw_runtime("reached the maximum number of iterations");
-      if (solver.iteration_count() > 0 && solver.relative_residual() < opt_.refine_rtol) {
+      if (total_iter + solver.iteration_count() >= max_iter) {
+        if (!opt_.keep_unconverged) throw_runtime("reached the maximum number of iterations");
+        st.converged = false;
+        done = true;
+        break;
+      }
+      const int it = solver.iteration_count();
+      inner_rel.push_back(solver.relative_residual());
+      if (it > 0 && solver.relative_residual() < opt_.refine_rtol) {
         restart = true;
+        break;
+      }
+      // Stagnation: less than a factor 2 over the last kWindow steps. The
+      // inner operator (or the preconditioner) is too inexact for this
+      // system: restart on the FP64 residual and raise the precision.
+      constexpr int kWindow = 4;
+      if (can_escalate() && it >= kWindow && inner_rel[it] > 0.5 * inner_rel[it - kWindow]) {
+        restart = true;
+        stagnated = true;
         break;
       }
       solver.iterate_process();
@@ -344,6 +410,7 @@ std::vector<double> Solver::solve(const std::vector<double>& values, double tole
     total_iter += solver.iteration_count();
     if (done) break;
     if (restart) st.restarts++;
+    if (stagnated) escalate(gstep, t_op, t_pc, op, pc, st);
   }
   st.iterations = total_iter;
   st.graph_replays = gstep ? gstep->replays : 0;

@@ -72,12 +72,17 @@ struct FitOptions {
   // Capture "preconditioner + matvec" of an Arnoldi step in a HIP graph after
   // the first (eager) step and replay it for the following ones.
   bool use_graphs = true;
+  // At max_iter keep the last iterate (FitStats::converged = false) instead of
+  // throwing, as polatory does (solver.hpp:131-133).
+  bool keep_unconverged = false;
 };
 
 struct FitStats {
   int iterations = 0;
   int restarts = 0;
   int graph_replays = 0;
+  int escalations = 0;  // precision raises after stagnation
+  bool converged = true;
   double residual = 0.0;
   double seconds = 0.0;
   double setup_seconds = 0.0;
@@ -86,6 +91,9 @@ struct FitStats {
   int64_t factor_bytes = 0;
   std::vector<double> history;  // residual per iteration (sampled or exact)
 };
+
+class GraphStep;
+class TimedOperator;
 
 // interpolation/solver.hpp:34-142 + fitter.hpp:21-26.
 class Solver {
@@ -96,6 +104,10 @@ class Solver {
   RasPreconditioner* preconditioner() { return pc_.get(); }
 
  private:
+  bool can_escalate() const;
+  void escalate(std::unique_ptr<GraphStep>& gstep, std::unique_ptr<TimedOperator>& t_op,
+                std::unique_ptr<TimedOperator>& t_pc, LinearOperator*& op, LinearOperator*& pc,
+                FitStats& st);
   Model model_;
   FitOptions opt_;
   hipStream_t s_;

@@ -271,10 +271,12 @@ void build_tree(Tree& t, const double* d_pts, int64_t n, const Box& box, int lev
   t.box = box;
   t.inv_cell = std::ldexp(1.0, level) / box.width;
   t.n_points = n;
-  t.levels.clear();
-  t.levels.resize(level + 1);
+  // Level buffers are kept and only grown: freeing device memory would
+  // synchronize the whole device (and every stream of other evaluators).
+  if (t.levels.size() < static_cast<size_t>(level) + 1) t.levels.resize(level + 1);
   if (n == 0) {
-    for (auto& lv : t.levels) {
+    for (int l = 0; l <= level; l++) {
+      TreeLevel& lv = t.levels[l];
       lv.n = 0;
       lv.child_start.resize(1);
       lv.child_start.zero(s);
@@ -314,6 +316,33 @@ void build_tree(Tree& t, const double* d_pts, int64_t n, const Box& box, int lev
   for (int l = 0; l <= level; l++) t.levels[l].n = counts[l];
 }
 
+namespace {
+
+__global__ void shift_codes_kernel(uint64_t* codes, int64_t n, int shift) {
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) codes[i] >>= shift;
+}
+
+}  // namespace
+
+void truncate_tree(Tree& t, int level, TreeWorkspace& ws, hipStream_t s) {
+  if (level >= t.level) return;
+  if (level < 0) throw_invalid("tree level out of range");
+  const int64_t n = t.n_points;
+  // Morton codes of level l are the level-L codes shifted right by 3(L - l):
+  // the cell index floor((x - lo) * 2^L / w) scales by an exact power of two.
+  if (n > 0) {
+    shift_codes_kernel<<<ceil_div(n, 256), 256, 0, s>>>(t.point_codes.get(), n, 3 * (t.level - level));
+    PAM_LAUNCH_CHECK();
+  }
+  t.level = level;  // deeper level buffers are kept for reuse
+  t.inv_cell = std::ldexp(1.0, level) / t.box.width;
+  if (n == 0) return;
+  ws.counts.resize(static_cast<size_t>(level) + 1);
+  unique_runs(t.point_codes.get(), n, nullptr, 0, t.levels[level].codes, t.leaf_start, t.point_leaf,
+              ws.counts.get() + level, ws, s);
+}
+
 void build_lists(const Tree& trg, const Tree& src, Lists& lists, TreeWorkspace& ws, hipStream_t s) {
   if (trg.level != src.level) throw_runtime("source and target trees must share the leaf level");
   int L = trg.level;
@@ -326,8 +355,8 @@ void build_lists(const Tree& trg, const Tree& src, Lists& lists, TreeWorkspace& 
     PAM_LAUNCH_CHECK();
   }
   const int32_t* vl = octant_vlist_device(s);
-  lists.m2l.clear();
-  lists.m2l.resize(L + 1);
+  if (lists.m2l.size() < static_cast<size_t>(L) + 1) lists.m2l.resize(L + 1);
+  lists.level = L;
   // Per level: the M2L slots, then the columns grouped by octant (stable, Morton
   // order inside a group), each group padded to a multiple of kM2LCols so a
   // column block has a single octant. The layouts of all levels come back to

@@ -37,7 +37,7 @@ struct Tree {
   DevBuf<int32_t> perm;          // [n] sorted position -> input index
   DevBuf<int32_t> point_leaf;    // [n] sorted position -> leaf index
   DevBuf<int32_t> leaf_start;    // [n_leaves+1]
-  std::vector<TreeLevel> levels; // [0..L]; levels[L] are the leaves
+  std::vector<TreeLevel> levels; // [0..L]; levels[L] are the leaves (buffers of deeper entries kept for reuse)
   int64_t n_leaves() const { return level >= 0 ? levels[level].n : 0; }
 };
 
@@ -60,8 +60,9 @@ struct M2LLevel {
 };
 
 struct Lists {
+  int level = -1;                 // leaf level the lists were built for (-1: none)
   DevBuf<int32_t> p2p;            // [n_trg_leaves][27]
-  std::vector<M2LLevel> m2l;      // index = level (entries < 2 unused)
+  std::vector<M2LLevel> m2l;      // index = level (entries < 2 unused; may hold more levels than used)
 };
 
 constexpr int kM2LSlots = 189;
@@ -70,6 +71,11 @@ constexpr int kM2LCols = 64;
 // The 189 transfer vectors valid for each octant, in increasing v order.
 const std::vector<int32_t>& octant_vlist_host();  // [8][189]
 const int32_t* octant_vlist_device(hipStream_t s);
+
+// Cuts a tree built at a deeper leaf level back to leaf level `level`: cells
+// of the levels <= level are unchanged, the points' leaf codes are shifted and
+// the leaf ranges recomputed (points stay sorted by their finer codes).
+void truncate_tree(Tree& t, int level, TreeWorkspace& ws, hipStream_t s);
 
 void build_tree(Tree& t, const double* d_pts, int64_t n, const Box& box, int level,
                 TreeWorkspace& ws, hipStream_t s);

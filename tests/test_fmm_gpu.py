@@ -48,7 +48,10 @@ def test_gpu_direct_matches_oracle(orc, kind):
     assert rel_err(y, ref) < 1e-13
 
 
-def _gpu_tree_vs_oracle(orc, ev, which, pts_raw, aniso):
+def _gpu_tree_vs_oracle(orc, ev, which, pts_raw, aniso, exact_order=True):
+    """Codes, leaves and cells bit-exact against the oracle. exact_order=False:
+    a tree cut back from a deeper level keeps its points sorted by the finer
+    codes, so the permutation is compared as the set of points of each leaf."""
     info = ev.tree_info(which)
     L = info["level"]
     pts = orc.transform_points(aniso.reshape(-1), pts_raw)
@@ -56,7 +59,12 @@ def _gpu_tree_vs_oracle(orc, ev, which, pts_raw, aniso):
     perm = orc.stable_sort_codes(codes)
     sorted_codes = codes[perm]
     gcodes, gperm, glc, gls = ev.export_tree(which)
-    np.testing.assert_array_equal(gperm, perm)
+    if exact_order:
+        np.testing.assert_array_equal(gperm, perm)
+    else:
+        _, ls_ = orc.leaves(sorted_codes)
+        for a, b in zip(ls_[:-1], ls_[1:]):
+            np.testing.assert_array_equal(np.sort(gperm[a:b]), np.sort(perm[a:b]))
     np.testing.assert_array_equal(gcodes, sorted_codes)
     lc, ls = orc.leaves(sorted_codes)
     np.testing.assert_array_equal(glc, lc)
@@ -99,6 +107,34 @@ def test_tree_and_lists_bit_exact(orc, dist):
         tl = ev.export_level(1, l)
         sl = ev.export_level(0, l)
         np.testing.assert_array_equal(ev.export_m2l(l), orc.m2l_lists(tl, sl, l))
+
+
+def test_adaptive_leaf_level_tree_matches_oracle(orc):
+    """Automatic leaf level on a surface-like source set: the trees are built
+    deeper and cut back to the level whose occupied leaves hold closest to
+    0.8 P^3 points (100 at order 5); codes, leaves, cells and lists stay
+    bit-exact."""
+    log("adaptive leaf level")
+    rng = np.random.default_rng(14)
+    u = rng.normal(size=(60000, 3))
+    src = 0.5 + 0.45 * u / np.linalg.norm(u, axis=1)[:, None]  # sphere surface
+    trg = rng.uniform(0.05, 0.95, (20000, 3))
+    r = make("bh3")
+    ev = pa.make_fmm_evaluator(r, ((0, 0, 0), (1, 1, 1)))
+    ev.set_config(pa.FmmConfig("mixed", 5, -1, "fmm"))
+    ev.set_source_points(src)
+    ev.set_target_points(trg)
+    ev.set_weights(np.ones(len(src)))
+    L = _gpu_tree_vs_oracle(orc, ev, 0, src, np.eye(3), exact_order=False)
+    assert _gpu_tree_vs_oracle(orc, ev, 1, trg, np.eye(3), exact_order=False) == L
+    # uniform-density choice: round(log8(60000 / 100)) = 3; the shell's
+    # occupied cells hold 268 points at level 3 and 66 at level 4
+    assert L == 4
+    _, _, slc, _ = ev.export_tree(0)
+    _, _, tlc, _ = ev.export_tree(1)
+    np.testing.assert_array_equal(ev.export_p2p(), orc.p2p_lists(tlc, slc, L))
+    for l in range(2, L + 1):
+        np.testing.assert_array_equal(ev.export_m2l(l), orc.m2l_lists(ev.export_level(1, l), ev.export_level(0, l), l))
 
 
 # (kind, precision, order, tolerance on the relative L-inf error vs FP64 direct)
