@@ -402,7 +402,7 @@ def main():
         "m2l": {"kernel": "m2l_dense_kernel<float> + m2l_reduce_kernel", "bound": "mfma",
                 "achieved": m2l_ach / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
                 "frac": m2l_ach / PEAK_FP32, "flops": m2l_flops, "cell_pairs": m["m2l_pairs"]},
-        "p2p": {"kernel": "p2p_kernel<float,bh3>", "bound": "valu (fp32)", "achieved": p2p_ach / 1e12,
+        "p2p": {"kernel": "p2p_kernel<float,0>", "bound": "valu (fp32)", "achieved": p2p_ach / 1e12,
                 "peak": PEAK_FP32 / 1e12, "peak_unpacked": PEAK_FP32 / 2e12, "unit": "TFLOP/s",
                 "frac": p2p_ach / PEAK_FP32, "pairs": m["p2p_pairs"], "flops_per_pair": 11},
         "l2p": {"kernel": "l2p_kernel<double,P>", "bound": "hbm",
