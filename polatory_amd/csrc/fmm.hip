@@ -343,6 +343,10 @@ l2p_kernel(const int32_t* __restrict__ leaf_start, const V4<T>* __restrict__ pts
   }
 }
 
+// Targets per thread of the near-field kernel: each source read from LDS
+// serves kP2PR pairs.
+constexpr int kP2PR = 4;
+
 template <class T, int KIND>
 __global__ void __launch_bounds__(kThreads)
 p2p_kernel(const int32_t* __restrict__ tls, const int32_t* __restrict__ sls,
@@ -350,13 +354,13 @@ p2p_kernel(const int32_t* __restrict__ tls, const int32_t* __restrict__ sls,
            const V4<T>* __restrict__ spts, const double* __restrict__ y_far,
            const int32_t* __restrict__ tperm, T pa, T pb, double* __restrict__ out) {
   constexpr int CAP = 32768 / static_cast<int>(sizeof(V4<T>));
+  constexpr int R = kP2PR;
   __shared__ V4<T> sp[CAP];
   __shared__ int nb_start[28];
   __shared__ int nb_src[27];
   __shared__ T nb_off[27][3];
   __shared__ int nb_count;
-  __shared__ T red[kThreads];
-  __shared__ T red2[kThreads];
+  __shared__ T red[R][kThreads];
   const int leaf = blockIdx.x;
   const int ts = tls[leaf], te = tls[leaf + 1];
   const int nt = te - ts;
@@ -380,21 +384,24 @@ p2p_kernel(const int32_t* __restrict__ tls, const int32_t* __restrict__ sls,
   __syncthreads();
   const int nnb = nb_count;
   const int total = nb_start[nnb];
-  // Every thread carries two targets (ti and ti + tp), so each source read
-  // from LDS serves two pairs; tp target slots x nsplit source splits fill
-  // the 256 lanes, and a leaf of up to 512 points takes a single pass.
-  const int half = (nt + 1) / 2;
+  // Every thread carries R targets (ti + r tp), so each source read from LDS
+  // serves R pairs; tp target slots x nsplit source splits fill the 256
+  // lanes, and a leaf of up to 256 R points takes a single pass.
+  const int per = (nt + R - 1) / R;
   int tp = 1;
-  while (tp < half && tp < kThreads) tp <<= 1;
+  while (tp < per && tp < kThreads) tp <<= 1;
   const int nsplit = kThreads / tp;
   const int ti = threadIdx.x & (tp - 1);
   const int si = threadIdx.x / tp;
-  for (int c0 = 0; c0 < nt; c0 += 2 * tp) {
-    const bool v0 = c0 + ti < nt, v1 = c0 + tp + ti < nt;
-    V4<T> t0{T(0), T(0), T(0), T(0)}, t1{T(0), T(0), T(0), T(0)};
-    if (v0) t0 = tpts[ts + c0 + ti];
-    if (v1) t1 = tpts[ts + c0 + tp + ti];
-    T acc0 = T(0), acc1 = T(0);
+  for (int c0 = 0; c0 < nt; c0 += R * tp) {
+    V4<T> tq[R];
+    T acc[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int i = c0 + r * tp + ti;
+      tq[r] = i < nt ? tpts[ts + i] : V4<T>{T(0), T(0), T(0), T(0)};
+      acc[r] = T(0);
+    }
     for (int cb = 0; cb < total; cb += CAP) {
       const int cnt = min(CAP, total - cb);
       __syncthreads();
@@ -410,31 +417,27 @@ p2p_kernel(const int32_t* __restrict__ tls, const int32_t* __restrict__ sls,
       }
       __syncthreads();
       for (int j = si; j < cnt; j += nsplit) {
-        V4<T> q = sp[j];
-        T dx = t0.x - q.x, dy = t0.y - q.y, dz = t0.z - q.z;
-        T ex = t1.x - q.x, ey = t1.y - q.y, ez = t1.z - q.z;
-        acc0 += q.w * phi_t<T, KIND>(dx * dx + dy * dy + dz * dz, pa, pb);
-        acc1 += q.w * phi_t<T, KIND>(ex * ex + ey * ey + ez * ez, pa, pb);
+        const V4<T> q = sp[j];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          T dx = tq[r].x - q.x, dy = tq[r].y - q.y, dz = tq[r].z - q.z;
+          acc[r] += q.w * phi_t<T, KIND>(dx * dx + dy * dy + dz * dz, pa, pb);
+        }
       }
     }
-    red[threadIdx.x] = acc0;
-    red2[threadIdx.x] = acc1;
+#pragma unroll
+    for (int r = 0; r < R; r++) red[r][threadIdx.x] = acc[r];
     __syncthreads();
     if (si == 0) {
-      T s0 = T(0), s1 = T(0);
-      for (int k = 0; k < nsplit; k++) {
-        s0 += red[ti + k * tp];
-        s1 += red2[ti + k * tp];
-      }
-      if (v0) {
-        int i = ts + c0 + ti;
-        double far = y_far ? y_far[i] : 0.0;
-        out[tperm[i]] = far + static_cast<double>(s0);
-      }
-      if (v1) {
-        int i = ts + c0 + tp + ti;
-        double far = y_far ? y_far[i] : 0.0;
-        out[tperm[i]] = far + static_cast<double>(s1);
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int i = c0 + r * tp + ti;
+        if (i >= nt) continue;
+        T sum = T(0);
+        for (int k = 0; k < nsplit; k++) sum += red[r][ti + k * tp];
+        const int gi = ts + i;
+        double far = y_far ? y_far[gi] : 0.0;
+        out[tperm[gi]] = far + static_cast<double>(sum);
       }
     }
     __syncthreads();
