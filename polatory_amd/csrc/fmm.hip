@@ -716,6 +716,8 @@ m2l_dense_kernel(M2LDenseArgs<TM> b) {
       int row = rem / kUpr, pc = rem % kUpr;
       int kb = (k0 >> 4) + seg;
       uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      // Always true (kpad is a multiple of the chunk); written unconditionally,
+      // the compiler moved the prefetch registers to scratch memory.
       if (kb * 16 < kpad) {
         const TM* g = av + ((static_cast<int64_t>(kb) * rpad + rb * kM2LR + row) << 4);
         v = reinterpret_cast<const uint4*>(g)[pc];
@@ -725,10 +727,8 @@ m2l_dense_kernel(M2LDenseArgs<TM> b) {
     const int kel = k0 + bq * (16 / static_cast<int>(sizeof(TM)));
     breg[0] = make_uint4(0u, 0u, 0u, 0u);
     breg[1] = make_uint4(0u, 0u, 0u, 0u);
-    if (kel < kpad) {
-      if (src0 >= 0) breg[0] = *reinterpret_cast<const uint4*>(in + static_cast<int64_t>(src0) * kpad + kel);
-      if (src1 >= 0) breg[1] = *reinterpret_cast<const uint4*>(in + static_cast<int64_t>(src1) * kpad + kel);
-    }
+    if (src0 >= 0) breg[0] = *reinterpret_cast<const uint4*>(in + static_cast<int64_t>(src0) * kpad + kel);
+    if (src1 >= 0) breg[1] = *reinterpret_cast<const uint4*>(in + static_cast<int64_t>(src1) * kpad + kel);
   };
   auto store_chunk = [&](int buf, const uint4 (&areg)[4], const uint4 (&breg)[2]) {
 #pragma unroll
@@ -770,25 +770,24 @@ m2l_dense_kernel(M2LDenseArgs<TM> b) {
     const int kc = t % nkc;
 #pragma unroll
     for (int kb = 0; kb < kKB; kb++) {
-      if (kc * kK + kb * 16 < kpad) {
-        using V4T = V4<TM>;
-        V4T af[4], bf[2];
+      if (kc * kK + kb * 16 >= kpad) continue;  // never taken: kpad is a multiple of the chunk
+      using V4T = V4<TM>;
+      V4T af[4], bf[2];
 #pragma unroll
-        for (int i = 0; i < 4; i++)
-          af[i] = *reinterpret_cast<const V4T*>(&as[buf][kb][wr * 64 + i * 16 + (lane & 15)][(lane >> 4) * 4]);
+      for (int i = 0; i < 4; i++)
+        af[i] = *reinterpret_cast<const V4T*>(&as[buf][kb][wr * 64 + i * 16 + (lane & 15)][(lane >> 4) * 4]);
 #pragma unroll
-        for (int j = 0; j < 2; j++)
-          bf[j] = *reinterpret_cast<const V4T*>(&bs[buf][wc * 32 + j * 16 + (lane & 15)][kb * 16 + (lane >> 4) * 4]);
+      for (int j = 0; j < 2; j++)
+        bf[j] = *reinterpret_cast<const V4T*>(&bs[buf][wc * 32 + j * 16 + (lane & 15)][kb * 16 + (lane >> 4) * 4]);
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+      for (int i = 0; i < 4; i++)
 #pragma unroll
-          for (int j = 0; j < 2; j++) {
-            acc[i][j] = Mfma<TM>::op(af[i].x, bf[j].x, acc[i][j]);
-            acc[i][j] = Mfma<TM>::op(af[i].y, bf[j].y, acc[i][j]);
-            acc[i][j] = Mfma<TM>::op(af[i].z, bf[j].z, acc[i][j]);
-            acc[i][j] = Mfma<TM>::op(af[i].w, bf[j].w, acc[i][j]);
-          }
-      }
+        for (int j = 0; j < 2; j++) {
+          acc[i][j] = Mfma<TM>::op(af[i].x, bf[j].x, acc[i][j]);
+          acc[i][j] = Mfma<TM>::op(af[i].y, bf[j].y, acc[i][j]);
+          acc[i][j] = Mfma<TM>::op(af[i].z, bf[j].z, acc[i][j]);
+          acc[i][j] = Mfma<TM>::op(af[i].w, bf[j].w, acc[i][j]);
+        }
     }
     // FP32 partial sums over at most 64 reduction steps are flushed into FP64.
     if constexpr (sizeof(TM) == 4) {
@@ -922,7 +921,7 @@ template <class TM>
 void launch_m2l_dense(const M2LOperator& op, const std::vector<M2LDenseLevel<TM>>& levels,
                       const int32_t* vlist, M2LSchedule& sched, DevBuf<double>& partial, hipStream_t s) {
   if (op.rpad % kM2LR != 0) throw_runtime("M2L operator rows are not padded to 128");
-  if (op.kpad % 16 != 0) throw_runtime("M2L reduction length is not padded to 16");
+  if (op.kpad % M2LTile<TM>::kK != 0) throw_runtime("M2L reduction length is not a multiple of the chunk");
   M2LDenseArgs<TM> b{};
   b.r = op.r;
   b.rpad = op.rpad;

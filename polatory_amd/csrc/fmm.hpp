@@ -25,8 +25,9 @@ constexpr int kMinOrder = 3;
 constexpr int kMaxOrder = 12;
 inline bool order_supported(int p) { return p >= kMinOrder && p <= kMaxOrder; }
 
-// Expansion stride: P^3 padded to 16 (the M2L reduction is blocked by 16).
-inline int kpad_of(int p) { return (p * p * p + 15) / 16 * 16; }
+// Expansion stride: P^3 padded to 32 (the dense M2L reduces in chunks of 32
+// FP32 or 16 FP64 columns, so every chunk is full and needs no bounds test).
+inline int kpad_of(int p) { return (p * p * p + 31) / 32 * 32; }
 // Row padding of the M2L operators: 128 rows per workgroup.
 inline int rpad_of(int r) { return (r + 127) / 128 * 128; }
 
