@@ -148,6 +148,39 @@ def test_fit_reaches_tolerance(orc, n, kind, nugget):
     assert np.abs(P.T @ w[:n]).max() < 1e-6 * np.abs(w[:n]).sum()
 
 
+def test_anisotropic_kriging_fit_and_predict(orc):
+    """Config-4-like kriging at reduced size: clustered sites, exponential
+    covariance with anisotropy diag(1, 1, 1/0.3), nugget, degree 0. The fit
+    meets its tolerance on the oracle's FP64 residual; predictions on a grid
+    match the oracle's direct sum within the stated tolerance of each form."""
+    log("anisotropic kriging")
+    rng = np.random.default_rng(29)
+    means = rng.uniform(0, 10, (6, 3))
+    comp = rng.integers(0, 6, 12000)
+    pts = means[comp] + rng.uniform(0.2, 1.0, 6)[comp, None] * rng.normal(size=(12000, 3))
+    vals = np.sin(pts @ np.array([0.3, -0.2, 0.5])) + 0.05 * rng.normal(size=len(pts))
+    rbf = pa.CovExponential([1.0, 2.0])
+    rbf.set_anisotropy(np.diag([1.0, 1.0, 1.0 / 0.3]))
+    model = pa.Model(rbf, poly_degree=0, nugget=0.0025)
+    ip = pa.Interpolant(model)
+    ip.fit(pts, vals, 1e-4, max_iter=100)
+    w = ip.weights()
+    res = oracle_residual(orc, model, pts, vals, w)
+    log(f"  iterations {ip.fit_stats()['iterations']}, oracle residual {res:.2e}")
+    assert res <= 1e-4
+    g = (np.arange(16) + 0.5) * (10.0 / 16)
+    grid = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3)
+    ref = orc.direct_evaluate(orc_rbfs(orc, model), 0, pts, w, grid)
+    # (form, order, tolerance): the exponential kernel with a short range along
+    # z limits the mixed form at order 8 to about 1e-4 (interpolation error)
+    for prec, order, tol in (("fp64", 10, 1e-6), ("mixed", 8, 3e-4)):
+        ip.set_eval_config(pa.FmmConfig(prec, order, -1, "fmm"))
+        y = ip.evaluate(grid)
+        err = np.abs(y - ref).max() / np.abs(ref).max()
+        log(f"  predict {prec} order {order}: rel err {err:.2e} (tol {tol:g})")
+        assert err <= tol
+
+
 def test_fit_with_mixed_fmm_operator_refines_to_tolerance(orc):
     # The inner operator is the FP32/MFMA FMM; FGMRES restarts on the FP64
     # residual (iterative refinement) until the absolute tolerance holds.
