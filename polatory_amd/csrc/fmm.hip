@@ -881,9 +881,12 @@ void build_m2l_schedule(const std::vector<const M2LLevel*>& lvls, int nrowblk, M
   constexpr int kXcd = 8;
   int64_t items = 0;
   for (const M2LLevel* l : lvls) items += (l->n_cols / kM2LC) * nrowblk;
-  // Enough workgroups to fill 256 CUs about four times over.
-  int nsplit = items > 0 ? static_cast<int>((1024 + items - 1) / items) : 1;
-  sc.nsplit = std::clamp(nsplit, 1, 12);
+  // Enough workgroups to fill 256 CUs about sixteen times over: with the
+  // active-slot mask the work per workgroup varies, and many small splits
+  // balance the tail better than a few large ones (each split still covers
+  // at least three of the 189 slots).
+  int nsplit = items > 0 ? static_cast<int>((4096 + items - 1) / items) : 1;
+  sc.nsplit = std::clamp(nsplit, 1, 48);
   std::map<std::tuple<int, int, int>, std::vector<std::array<int32_t, 4>>> groups;
   for (size_t li = 0; li < lvls.size(); li++) {
     const M2LLevel& l = *lvls[li];
