@@ -604,6 +604,10 @@ m2l_batch_kernel(M2LBatch<TM, TS> b) {
 constexpr int kM2LR = 128;
 constexpr int kM2LC = kM2LCols;
 static_assert(kM2LC == 64, "the dense M2L kernel maps 64 columns onto 2 x 32-column waves");
+// Slots one split may cover; the schedule uses at least
+// ceil(kM2LSlots / kM2LSplitSlots) splits so their source table fits in LDS.
+constexpr int kM2LSplitSlots = 48;
+constexpr int kM2LMinSplits = (kM2LSlots + kM2LSplitSlots - 1) / kM2LSplitSlots;
 
 template <class TM>
 struct M2LTile {
@@ -649,8 +653,9 @@ m2l_dense_kernel(M2LDenseArgs<TM> b) {
   __shared__ __align__(16) TM bs[2][kM2LC][kBP];
   __shared__ int cells[kM2LC];
   __shared__ uint32_t mask[6];
-  __shared__ int act_v[kM2LSlots];
-  __shared__ int act_s[kM2LSlots];
+  __shared__ int act_v[kM2LSplitSlots];
+  __shared__ int act_s[kM2LSplitSlots];
+  __shared__ int src_tab[kM2LSplitSlots][kM2LC];  // source cell per active slot and column, -1 if none
   __shared__ int nact_s;
 
   const int4 w = b.sched[blockIdx.x];
@@ -687,59 +692,67 @@ m2l_dense_kernel(M2LDenseArgs<TM> b) {
   }
   __syncthreads();
   const int nact = nact_s;
-  const int nkc = (kpad + kK - 1) / kK;
+  for (int idx = tid; idx < nact * kM2LC; idx += kThreads) {
+    int ia = idx / kM2LC, c = idx % kM2LC;
+    int cell = cells[c];
+    src_tab[ia][c] = cell >= 0 ? src_slots[static_cast<int64_t>(cell) * kM2LSlots + act_s[ia]] : -1;
+  }
+  __syncthreads();
+  const int nkc = kpad / kK;
   const int total = nact * nkc;
 
   // Loader roles: operator units tid + 256 q (q < 4); multipole columns
   // bc0 = tid / 8 and bc1 = 32 + tid / 8, 16-byte piece bq = tid % 8.
+  // Every global load of the main loop is issued unconditionally (an absent
+  // source reads cell 0 and is zeroed when stored), so the compiler can wait
+  // for exactly the register set being stored while the next one stays in
+  // flight.
   const int bc0 = tid / 8, bc1 = 32 + tid / 8, bq = tid % 8;
-  const int cell0 = cells[bc0], cell1 = cells[bc1];
+  constexpr int kAU = kM2LR * kUpr;  // operator units per 16-wide block
+  static_assert(kAU % kThreads == 0, "operator units of one thread never straddle a block");
   // Two register sets: the chunk two steps ahead is in flight while the next
   // one waits to be stored into the free LDS buffer.
-  uint4 ra0[4], rb0[2], ra1[4], rb1[2];
-  int src0 = -1, src1 = -1;  // sources of the loader's columns for the slot being loaded
-
-  auto load_src = [&](int ia) {
-    int sl = act_s[ia];
-    src0 = cell0 >= 0 ? src_slots[static_cast<int64_t>(cell0) * kM2LSlots + sl] : -1;
-    src1 = cell1 >= 0 ? src_slots[static_cast<int64_t>(cell1) * kM2LSlots + sl] : -1;
+  // Native 4 x 32-bit vectors (not HIP's uint4 wrapper) so the sets stay in
+  // registers.
+  using u32x4 = uint32_t __attribute__((ext_vector_type(4)));
+  struct RegSet {
+    u32x4 a[4], b[2];
+    uint32_t m0, m1;  // all ones where the loader's column has a source in this chunk's slot
   };
-  auto load_chunk = [&](int t, uint4 (&areg)[4], uint4 (&breg)[2]) {
-    if ((t % nkc) == 0) load_src(t / nkc);
-    const int ia = t / nkc, kc = t % nkc;
+  RegSet r0, r1;
+
+  auto load_chunk = [&](int t, RegSet& rs) {
+    const int ia = t / nkc, kc = t - ia * nkc;
     const int k0 = kc * kK;
     const TM* av = b.a[lv] + static_cast<int64_t>(act_v[ia]) * b.per_v;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      int u = tid + q * kThreads;
-      int seg = u / (kM2LR * kUpr), rem = u % (kM2LR * kUpr);
-      int row = rem / kUpr, pc = rem % kUpr;
-      int kb = (k0 >> 4) + seg;
-      uint4 v = make_uint4(0u, 0u, 0u, 0u);
-      // Always true (kpad is a multiple of the chunk); written unconditionally,
-      // the compiler moved the prefetch registers to scratch memory.
-      if (kb * 16 < kpad) {
-        const TM* g = av + ((static_cast<int64_t>(kb) * rpad + rb * kM2LR + row) << 4);
-        v = reinterpret_cast<const uint4*>(g)[pc];
-      }
-      areg[q] = v;
+      const int seg = (q * kThreads) / kAU;
+      const int rem = (q * kThreads) % kAU + tid;
+      const int row = rem / kUpr, pc = rem % kUpr;
+      const int kb = (k0 >> 4) + seg;
+      const TM* g = av + ((static_cast<int64_t>(kb) * rpad + rb * kM2LR + row) << 4);
+      rs.a[q] = reinterpret_cast<const u32x4*>(g)[pc];
     }
     const int kel = k0 + bq * (16 / static_cast<int>(sizeof(TM)));
-    breg[0] = make_uint4(0u, 0u, 0u, 0u);
-    breg[1] = make_uint4(0u, 0u, 0u, 0u);
-    if (src0 >= 0) breg[0] = *reinterpret_cast<const uint4*>(in + static_cast<int64_t>(src0) * kpad + kel);
-    if (src1 >= 0) breg[1] = *reinterpret_cast<const uint4*>(in + static_cast<int64_t>(src1) * kpad + kel);
+    const int s0 = src_tab[ia][bc0], s1 = src_tab[ia][bc1];
+    rs.m0 = s0 >= 0 ? ~0u : 0u;
+    rs.m1 = s1 >= 0 ? ~0u : 0u;
+    rs.b[0] = *reinterpret_cast<const u32x4*>(in + static_cast<int64_t>(max(s0, 0)) * kpad + kel);
+    rs.b[1] = *reinterpret_cast<const u32x4*>(in + static_cast<int64_t>(max(s1, 0)) * kpad + kel);
   };
-  auto store_chunk = [&](int buf, const uint4 (&areg)[4], const uint4 (&breg)[2]) {
+  auto store_chunk = [&](int buf, const RegSet& rs) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      int u = tid + q * kThreads;
-      int seg = u / (kM2LR * kUpr), rem = u % (kM2LR * kUpr);
-      int row = rem / kUpr, pc = rem % kUpr;
-      *reinterpret_cast<uint4*>(&as[buf][seg][row][pc * (16 / static_cast<int>(sizeof(TM)))]) = areg[q];
+      const int seg = (q * kThreads) / kAU;
+      const int rem = (q * kThreads) % kAU + tid;
+      const int row = rem / kUpr, pc = rem % kUpr;
+      *reinterpret_cast<u32x4*>(&as[buf][seg][row][pc * (16 / static_cast<int>(sizeof(TM)))]) = rs.a[q];
     }
-    *reinterpret_cast<uint4*>(&bs[buf][bc0][bq * (16 / static_cast<int>(sizeof(TM)))]) = breg[0];
-    *reinterpret_cast<uint4*>(&bs[buf][bc1][bq * (16 / static_cast<int>(sizeof(TM)))]) = breg[1];
+    // Masked with AND rather than a select: a select of two loaded values
+    // becomes a select of addresses, which pins the sets in scratch.
+    *reinterpret_cast<u32x4*>(&bs[buf][bc0][bq * (16 / static_cast<int>(sizeof(TM)))]) = rs.b[0] & rs.m0;
+    *reinterpret_cast<u32x4*>(&bs[buf][bc1][bq * (16 / static_cast<int>(sizeof(TM)))]) = rs.b[1] & rs.m1;
   };
 
   acc_t acc[4][2];
@@ -770,7 +783,6 @@ m2l_dense_kernel(M2LDenseArgs<TM> b) {
     const int kc = t % nkc;
 #pragma unroll
     for (int kb = 0; kb < kKB; kb++) {
-      if (kc * kK + kb * 16 >= kpad) continue;  // never taken: kpad is a multiple of the chunk
       using V4T = V4<TM>;
       V4T af[4], bf[2];
 #pragma unroll
@@ -796,22 +808,24 @@ m2l_dense_kernel(M2LDenseArgs<TM> b) {
   };
   // Step t: chunk t+1 sits in registers (cur), chunk t+2 is loaded into the
   // other set (nxt) while chunk t is multiplied from LDS buffer t&1; then
-  // chunk t+1 goes to LDS buffer (t+1)&1.
-  auto step = [&](int t, uint4 (&cur_a)[4], uint4 (&cur_b)[2], uint4 (&nxt_a)[4], uint4 (&nxt_b)[2]) {
-    if (t + 2 < total) load_chunk(t + 2, nxt_a, nxt_b);
-    compute(t);
-    if (t + 1 < total) store_chunk((t + 1) & 1, cur_a, cur_b);
+  // chunk t+1 goes to LDS buffer (t+1)&1. Past the end the last chunk is
+  // loaded and stored again into the buffer no later step reads, so no load
+  // or store sits behind a branch.
+  auto step = [&](int t, const RegSet& cur, RegSet& nxt) {
+    load_chunk(min(t + 2, total - 1), nxt);
+    if (t < total) compute(t);
+    store_chunk((t + 1) & 1, cur);
     __syncthreads();
   };
   if (total > 0) {
-    load_chunk(0, ra1, rb1);
-    store_chunk(0, ra1, rb1);
-    if (total > 1) load_chunk(1, ra0, rb0);
-  }
-  __syncthreads();
-  for (int t = 0; t < total; t += 2) {
-    step(t, ra0, rb0, ra1, rb1);
-    if (t + 1 < total) step(t + 1, ra1, rb1, ra0, rb0);
+    load_chunk(0, r1);
+    store_chunk(0, r1);
+    load_chunk(min(1, total - 1), r0);
+    __syncthreads();
+    for (int t = 0; t < total; t += 2) {
+      step(t, r0, r1);
+      step(t + 1, r1, r0);
+    }
   }
   if constexpr (sizeof(TM) == 8) flush();
 
@@ -886,7 +900,7 @@ void build_m2l_schedule(const std::vector<const M2LLevel*>& lvls, int nrowblk, M
   // balance the tail better than a few large ones (each split still covers
   // at least three of the 189 slots).
   int nsplit = items > 0 ? static_cast<int>((4096 + items - 1) / items) : 1;
-  sc.nsplit = std::clamp(nsplit, 1, 48);
+  sc.nsplit = std::clamp(nsplit, kM2LMinSplits, 48);
   std::map<std::tuple<int, int, int>, std::vector<std::array<int32_t, 4>>> groups;
   for (size_t li = 0; li < lvls.size(); li++) {
     const M2LLevel& l = *lvls[li];
