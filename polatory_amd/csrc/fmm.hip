@@ -413,16 +413,35 @@ p2p_kernel(const int32_t* __restrict__ tls, const int32_t* __restrict__ sls,
         q.x += nb_off[k][0];
         q.y += nb_off[k][1];
         q.z += nb_off[k][2];
+        q.w *= pa;  // the kernel's factor a folded into the weight
         sp[e] = q;
       }
       __syncthreads();
-      for (int j = si; j < cnt; j += nsplit) {
-        const V4<T> q = sp[j];
+      // Per pair: three differences, three FMAs seeded with the kernel's
+      // offset, the shape function and one FMA. Four sources per trip keep
+      // their LDS reads ahead of the arithmetic; the sum order over sources
+      // is the plain sequential one.
+      const T seed = phi_seed<T, KIND>(pb);
+      auto pair = [&](const V4<T>& q, int r) {
+        T dx = tq[r].x - q.x, dy = tq[r].y - q.y, dz = tq[r].z - q.z;
+        T r2 = fma(dz, dz, fma(dy, dy, fma(dx, dx, seed)));
+        acc[r] = fma(q.w, phi_shape<T, KIND>(r2, pb), acc[r]);
+      };
+      int j = si;
+      for (; j + 3 * nsplit < cnt; j += 4 * nsplit) {
+        const V4<T> q0 = sp[j], q1 = sp[j + nsplit], q2 = sp[j + 2 * nsplit], q3 = sp[j + 3 * nsplit];
 #pragma unroll
         for (int r = 0; r < R; r++) {
-          T dx = tq[r].x - q.x, dy = tq[r].y - q.y, dz = tq[r].z - q.z;
-          acc[r] += q.w * phi_t<T, KIND>(dx * dx + dy * dy + dz * dz, pa, pb);
+          pair(q0, r);
+          pair(q1, r);
+          pair(q2, r);
+          pair(q3, r);
         }
+      }
+      for (; j < cnt; j += nsplit) {
+        const V4<T> q = sp[j];
+#pragma unroll
+        for (int r = 0; r < R; r++) pair(q, r);
       }
     }
 #pragma unroll

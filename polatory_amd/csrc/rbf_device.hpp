@@ -53,6 +53,42 @@ __device__ __forceinline__ float phi_f32(float r2, float a, float b) {
   }
 }
 
+// phi without its factor a, for loops that fold a into the weights:
+// r2c = r^2 + b for bh3 / th3 (the caller seeds its sum of squares with b),
+// r2c = r^2 for exp.
+template <class T, int KIND>
+__device__ __forceinline__ T phi_shape(T r2c, T b) {
+  if constexpr (sizeof(T) == 8) {
+    if constexpr (KIND == kBh3) {
+      return sqrt(r2c);
+    } else if constexpr (KIND == kTh3) {
+      double rho = sqrt(r2c);
+      return rho * rho * rho;
+    } else {
+      return exp(b * sqrt(r2c));
+    }
+  } else {
+    if constexpr (KIND == kBh3) {
+      return __builtin_amdgcn_sqrtf(r2c);
+    } else if constexpr (KIND == kTh3) {
+      float rho = __builtin_amdgcn_sqrtf(r2c);
+      return rho * rho * rho;
+    } else {
+      return __expf(b * __builtin_amdgcn_sqrtf(r2c));
+    }
+  }
+}
+
+// Seed of the sum of squares that phi_shape expects.
+template <class T, int KIND>
+__device__ __forceinline__ T phi_seed(T b) {
+  if constexpr (KIND == kExp) {
+    return T(0);
+  } else {
+    return b;
+  }
+}
+
 template <class T, int KIND>
 __device__ __forceinline__ T phi_t(T r2, T a, T b) {
   if constexpr (sizeof(T) == 8) {
