@@ -93,6 +93,46 @@ class DevBuf {
   size_t cap_ = 0;
 };
 
+// Pinned, grow-only host staging for uploads that must not synchronize the
+// stream: the buffer is refilled only after the event of its previous upload
+// has completed.
+template <class T>
+class HostStage {
+ public:
+  HostStage() = default;
+  ~HostStage() {
+    if (ev_) (void)hipEventDestroy(ev_);
+    if (p_) (void)hipHostFree(p_);
+  }
+  HostStage(const HostStage&) = delete;
+  HostStage& operator=(const HostStage&) = delete;
+  // Host pointer to n elements, safe to overwrite.
+  T* acquire(size_t n) {
+    if (!ev_) PAM_HIP(hipEventCreateWithFlags(&ev_, hipEventDisableTiming));
+    if (pending_) PAM_HIP(hipEventSynchronize(ev_));
+    pending_ = false;
+    if (n > cap_) {
+      if (p_) PAM_HIP(hipHostFree(p_));
+      p_ = nullptr;
+      PAM_HIP(hipHostMalloc(reinterpret_cast<void**>(&p_), (n == 0 ? 1 : n) * sizeof(T)));
+      cap_ = n;
+    }
+    return p_;
+  }
+  void upload(DevBuf<T>& d, size_t n, hipStream_t s) {
+    d.resize(n);
+    if (n) PAM_HIP(hipMemcpyAsync(d.get(), p_, n * sizeof(T), hipMemcpyHostToDevice, s));
+    PAM_HIP(hipEventRecord(ev_, s));
+    pending_ = true;
+  }
+
+ private:
+  T* p_ = nullptr;
+  size_t cap_ = 0;
+  hipEvent_t ev_ = nullptr;
+  bool pending_ = false;
+};
+
 // RBF families on the path (rbf/polyharmonic_odd.hpp, rbf/cov_exponential.hpp).
 enum RbfKind : int { kBh3 = 0, kTh3 = 1, kExp = 2 };
 

@@ -8,8 +8,6 @@
 //               through LDS, targets x source-splits mapped onto 256 lanes
 #include <array>
 #include <cmath>
-#include <map>
-#include <tuple>
 #include <algorithm>
 #include <type_traits>
 #include <vector>
@@ -920,34 +918,45 @@ void build_m2l_schedule(const std::vector<const M2LLevel*>& lvls, int nrowblk, M
   // at least three of the 189 slots).
   int nsplit = items > 0 ? static_cast<int>((4096 + items - 1) / items) : 1;
   sc.nsplit = std::clamp(nsplit, kM2LMinSplits, 48);
-  std::map<std::tuple<int, int, int>, std::vector<std::array<int32_t, 4>>> groups;
+  // Column blocks per octant in (level, block) order; a group (octant, row
+  // block, split) holds all of its octant's blocks. Groups go round-robin to
+  // the XCDs in (octant, row block, split) order.
+  std::array<std::vector<std::array<int32_t, 2>>, 8> per_oct;
   for (size_t li = 0; li < lvls.size(); li++) {
     const M2LLevel& l = *lvls[li];
-    for (int64_t cb = 0; cb < l.n_cols / kM2LC; cb++) {
-      int oct = l.h_blk_oct[cb];
-      for (int rb = 0; rb < nrowblk; rb++)
-        for (int sp = 0; sp < sc.nsplit; sp++)
-          groups[{oct, rb, sp}].push_back({static_cast<int32_t>(li), static_cast<int32_t>(cb), rb, sp});
-    }
+    for (int64_t cb = 0; cb < l.n_cols / kM2LC; cb++)
+      per_oct[l.h_blk_oct[cb]].push_back({static_cast<int32_t>(li), static_cast<int32_t>(cb)});
   }
-  std::vector<std::vector<std::array<int32_t, 4>>> per_xcd(kXcd);
+  std::array<size_t, kXcd> len_x{};
   int g = 0;
-  for (auto& kv : groups) {
-    auto& dst = per_xcd[g++ % kXcd];
-    dst.insert(dst.end(), kv.second.begin(), kv.second.end());
+  for (int oct = 0; oct < 8; oct++) {
+    if (per_oct[oct].empty()) continue;
+    for (int rb = 0; rb < nrowblk; rb++)
+      for (int sp = 0; sp < sc.nsplit; sp++) len_x[g++ % kXcd] += per_oct[oct].size();
   }
-  size_t len = 0;
-  for (auto& v : per_xcd) len = std::max(len, v.size());
-  sc.h.assign(len * kXcd * 4, -1);
-  for (size_t j = 0; j < len; j++)
-    for (int x = 0; x < kXcd; x++)
-      if (j < per_xcd[x].size())
-        for (int k = 0; k < 4; k++) sc.h[(j * kXcd + x) * 4 + k] = per_xcd[x][j][k];
+  size_t len = *std::max_element(len_x.begin(), len_x.end());
+  int32_t* h = sc.h.acquire(len * kXcd * 4);
+  std::fill(h, h + len * kXcd * 4, -1);
+  std::array<size_t, kXcd> pos{};
+  g = 0;
+  for (int oct = 0; oct < 8; oct++) {
+    if (per_oct[oct].empty()) continue;
+    for (int rb = 0; rb < nrowblk; rb++)
+      for (int sp = 0; sp < sc.nsplit; sp++) {
+        const int x = g++ % kXcd;
+        for (const auto& it : per_oct[oct]) {
+          int32_t* e = h + (pos[x]++ * kXcd + x) * 4;
+          e[0] = it[0];
+          e[1] = it[1];
+          e[2] = rb;
+          e[3] = sp;
+        }
+      }
+  }
   sc.n_wg = static_cast<int>(len * kXcd);
   sc.cells_total = 0;
   for (const M2LLevel* l : lvls) sc.cells_total += l->n_cells;
-  sc.d.upload(sc.h.data(), sc.h.size(), s);
-  PAM_HIP(hipStreamSynchronize(s));  // the upload reads host memory
+  sc.h.upload(sc.d, len * kXcd * 4, s);
   sc.valid = true;
 }
 

@@ -121,7 +121,7 @@ struct M2LDenseLevel {
 // per workgroup, ordered so that the workgroups that read the same operator
 // rows (same octant, row block and split of the slots) land on the same XCD.
 struct M2LSchedule {
-  std::vector<int32_t> h;
+  HostStage<int32_t> h;
   DevBuf<int32_t> d;
   int n_wg = 0;
   int nsplit = 1;

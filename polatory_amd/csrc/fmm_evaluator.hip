@@ -656,8 +656,7 @@ void FmmEvaluator::run_fmm(double* d_out) {
       xfer.push_back({&lists_.m2l[l], in, st.l[l].get(), ops[l]->a.get(), scale});
     }
     if (L >= 2) {
-      if (st.m2l_sched_gen != lists_gen_ || st.m2l_sched.h.empty() ||
-          st.m2l_sched.nsplit < 1) {
+      if (st.m2l_sched_gen != lists_gen_ || st.m2l_sched.n_wg == 0 || st.m2l_sched.nsplit < 1) {
         st.m2l_sched.valid = false;
       }
       if (st.m2l_sched.valid && st.m2l_sched_rpad != ops[2]->op.rpad) st.m2l_sched.valid = false;
