@@ -389,11 +389,6 @@ void FmmEvaluator::ensure_trees() {
     rebuilt = true;
     st_.t32_valid = st_.t64_valid = false;
   };
-  auto counts_of = [](const Tree& t) {
-    std::vector<int64_t> c(static_cast<size_t>(t.level) + 1);
-    for (size_t l = 0; l < c.size(); l++) c[l] = t.levels[l].n;
-    return c;
-  };
   int L;
   if (cfg_.leaf_level >= 0) {
     L = cfg_.leaf_level;
@@ -402,23 +397,41 @@ void FmmEvaluator::ensure_trees() {
     src_counts_.clear();
     trg_counts_.clear();
   } else {
-    // Leaf level from the occupied cells: both trees are built down to
-    // kLevelsBeyond levels below the uniform-density choice, the level whose
-    // occupied leaves hold closest to kPointsPerLeaf points (in ratio) is
-    // kept, and the trees are cut back to it. Clustered or surface-like point
-    // sets get deeper trees than their point count alone suggests.
+    // Leaf level from the occupied cells: the points are sorted by their
+    // codes kLevelsBeyond levels below the uniform-density choice, the cells
+    // of every level are counted on the device (one read-back for both
+    // trees), the level whose occupied leaves hold closest to kPointsPerLeaf
+    // points (in ratio) is kept, and the trees are built at it. Clustered or
+    // surface-like point sets get deeper trees than their point count alone
+    // suggests.
     constexpr int kLevelsBeyond = 2;
     const double kPointsPerLeaf = leaf_occupancy_target(cfg_.order, cfg_.precision);
     const int L0 = auto_leaf_level(n_src_, n_trg(), kPointsPerLeaf);
     const int Lmax = std::min(L0 + kLevelsBeyond, 18);
-    if (src_tree_dirty_ || static_cast<int>(src_counts_.size()) != Lmax + 1) {
-      build_src(Lmax);
-      src_counts_ = counts_of(src_tree_);
-    }
-    if (!symmetric_ && (trg_tree_dirty_ || static_cast<int>(trg_counts_.size()) != Lmax + 1)) {
-      build_trg(Lmax);
-      trg_counts_ = counts_of(trg_tree_);
-    }
+    auto sort_both = [&](bool do_src, bool do_trg) {
+      if (do_src) sort_tree_points(src_tree_, src_pts_.get(), n_src_, box_, Lmax, ws_, stream_);
+      if (do_trg) sort_tree_points(trg_tree_, trg_pts_.get(), n_trg_, box_, Lmax, ws_, stream_);
+      std::vector<int64_t> hs(static_cast<size_t>(Lmax) + 1), ht(static_cast<size_t>(Lmax) + 1);
+      const size_t bytes = hs.size() * sizeof(int64_t);
+      if (do_src) PAM_HIP(hipMemcpyAsync(hs.data(), src_tree_.open_hist.get(), bytes, hipMemcpyDeviceToHost, stream_));
+      if (do_trg) PAM_HIP(hipMemcpyAsync(ht.data(), trg_tree_.open_hist.get(), bytes, hipMemcpyDeviceToHost, stream_));
+      PAM_HIP(hipStreamSynchronize(stream_));
+      if (do_src) {
+        src_counts_ = level_counts(hs);
+        src_tree_dirty_ = false;
+        st_.s32_valid = st_.s64_valid = false;
+        st_.multipoles_valid = false;
+      }
+      if (do_trg) {
+        trg_counts_ = level_counts(ht);
+        trg_tree_dirty_ = false;
+        st_.t32_valid = st_.t64_valid = false;
+      }
+    };
+    const bool sort_src = src_tree_dirty_ || static_cast<int>(src_counts_.size()) != Lmax + 1;
+    const bool sort_trg =
+        !symmetric_ && (trg_tree_dirty_ || static_cast<int>(trg_counts_.size()) != Lmax + 1);
+    if (sort_src || sort_trg) sort_both(sort_src, sort_trg);
     L = L0;
     double best = INFINITY;
     for (int l = L0; l <= Lmax; l++) {
@@ -433,22 +446,19 @@ void FmmEvaluator::ensure_trees() {
         L = l;
       }
     }
-    if (src_tree_.level != L) {
-      if (src_tree_.level < L) {
-        build_src(Lmax);
-        src_counts_ = counts_of(src_tree_);
-      }
-      truncate_tree(src_tree_, L, ws_, stream_);
+    // Points sorted at a level coarser than L (a tree finished earlier at a
+    // coarser leaf level) are sorted again.
+    const bool resort_src = src_tree_.sort_level < L;
+    const bool resort_trg = !symmetric_ && trg_tree_.sort_level < L;
+    if (resort_src || resort_trg) sort_both(resort_src, resort_trg);
+    if (sort_src || resort_src || src_tree_.level != L) {
+      finish_tree(src_tree_, L, src_counts_, ws_, stream_);
       rebuilt = true;
       st_.s32_valid = st_.s64_valid = false;
       st_.multipoles_valid = false;
     }
-    if (!symmetric_ && trg_tree_.level != L) {
-      if (trg_tree_.level < L) {
-        build_trg(Lmax);
-        trg_counts_ = counts_of(trg_tree_);
-      }
-      truncate_tree(trg_tree_, L, ws_, stream_);
+    if (!symmetric_ && (sort_trg || resort_trg || trg_tree_.level != L)) {
+      finish_tree(trg_tree_, L, trg_counts_, ws_, stream_);
       rebuilt = true;
       st_.t32_valid = st_.t64_valid = false;
     }

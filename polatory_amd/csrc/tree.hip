@@ -264,13 +264,145 @@ const int32_t* octant_vlist_device(hipStream_t s) {
   return per_dev[dev];
 }
 
-void build_tree(Tree& t, const double* d_pts, int64_t n, const Box& box, int level,
-                TreeWorkspace& ws, hipStream_t s) {
-  if (level < 0 || level > 20) throw_invalid("tree level out of range");
-  t.level = level;
+namespace {
+
+// Level at which sorted point i first opens a cell: 0 for the first point,
+// else sort_level - floor(msb(c[i] ^ c[i-1]) / 3) (the codes of level l are
+// the top 3l bits); points equal to their predecessor open none.
+__global__ void level_open_kernel(const uint64_t* __restrict__ codes, int64_t n, int sort_level,
+                                  int64_t* __restrict__ hist) {
+  __shared__ unsigned int h[kMaxTreeLevel + 1];
+  if (threadIdx.x <= kMaxTreeLevel) h[threadIdx.x] = 0u;
+  __syncthreads();
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    int lv;
+    if (i == 0) {
+      lv = 0;
+    } else {
+      uint64_t x = codes[i] ^ codes[i - 1];
+      if (x == 0) continue;
+      lv = sort_level - (63 - __clzll(static_cast<long long>(x))) / 3;
+    }
+    atomicAdd(&h[lv], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x <= static_cast<unsigned>(sort_level) && h[threadIdx.x] != 0u)
+    atomicAdd(reinterpret_cast<unsigned long long*>(hist + threadIdx.x),
+              static_cast<unsigned long long>(h[threadIdx.x]));
+}
+
+__global__ void zero_i64_kernel(int64_t* a, int n) {
+  if (static_cast<int>(threadIdx.x) < n) a[threadIdx.x] = 0;
+}
+
+__global__ void shift_codes_kernel(uint64_t* codes, int64_t n, int shift) {
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) codes[i] >>= shift;
+}
+
+// Levels level-1 .. 0 from the leaf codes in one workgroup: for each level,
+// run starts of (child code >> 3) give the cells (codes, child_start) and
+// every child's parent index. Counts are known, so every write is bounded.
+struct CoarseArgs {
+  int level;
+  int64_t n[kMaxTreeLevel + 1];
+  uint64_t* codes[kMaxTreeLevel + 1];
+  int32_t* parent[kMaxTreeLevel + 1];
+  int32_t* child_start[kMaxTreeLevel + 1];
+};
+
+constexpr int kCoarseThreads = 1024;
+constexpr int64_t kCoarseMaxLeaves = 65536;
+
+__global__ void __launch_bounds__(kCoarseThreads) coarse_levels_kernel(CoarseArgs a) {
+  __shared__ int32_t wsum[kCoarseThreads / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int l = a.level - 1; l >= 0; l--) {
+    const uint64_t* cc = a.codes[l + 1];
+    const int64_t nc = a.n[l + 1], np = a.n[l];
+    int32_t carry = 0;
+    for (int64_t base = 0; base < nc; base += kCoarseThreads) {
+      const int64_t i = base + tid;
+      int32_t f = 0;
+      uint64_t c = 0;
+      if (i < nc) {
+        c = cc[i] >> 3;
+        f = (i == 0 || c != (cc[i - 1] >> 3)) ? 1 : 0;
+      }
+      int32_t v = f;  // inclusive scan within the wave
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        int32_t u = __shfl_up(v, d, 64);
+        if (lane >= d) v += u;
+      }
+      if (lane == 63) wsum[wave] = v;
+      __syncthreads();
+      int32_t before = 0, total = 0;
+      for (int w = 0; w < kCoarseThreads / 64; w++) {
+        if (w < wave) before += wsum[w];
+        total += wsum[w];
+      }
+      const int32_t r = carry + before + v - 1;  // cell index of child i
+      if (i < nc && r >= 0 && r < np) {
+        if (f) {
+          a.codes[l][r] = c;
+          a.child_start[l][r] = static_cast<int32_t>(i);
+        }
+        a.parent[l + 1][i] = r;
+      }
+      carry += total;
+      __syncthreads();
+    }
+    if (tid == 0) a.child_start[l][np] = static_cast<int32_t>(nc);
+    // This level's codes are the next level's input.
+    __threadfence();
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+void sort_tree_points(Tree& t, const double* d_pts, int64_t n, const Box& box, int sort_level,
+                      TreeWorkspace& ws, hipStream_t s) {
+  if (sort_level < 0 || sort_level > kMaxTreeLevel) throw_invalid("tree level out of range");
   t.box = box;
-  t.inv_cell = std::ldexp(1.0, level) / box.width;
   t.n_points = n;
+  t.sort_level = sort_level;
+  t.level = -1;
+  t.open_hist.resize(static_cast<size_t>(sort_level) + 1);
+  zero_i64_kernel<<<1, 64, 0, s>>>(t.open_hist.get(), sort_level + 1);
+  PAM_LAUNCH_CHECK();
+  if (n == 0) return;
+  t.point_codes.resize(n);
+  t.perm.resize(n);
+  Lo3 lo{{box.lo[0], box.lo[1], box.lo[2]}};
+  const double inv_cell = std::ldexp(1.0, sort_level) / box.width;
+  morton_kernel<<<ceil_div(n, 256), 256, 0, s>>>(d_pts, n, lo, inv_cell,
+                                                 (int64_t{1} << sort_level) - 1, t.point_codes.get(),
+                                                 t.perm.get());
+  PAM_LAUNCH_CHECK();
+  radix_sort_pairs(t.point_codes.get(), t.perm.get(), n, 3 * sort_level, ws.tk, ws.tv, ws.hist,
+                   ws.scan_scratch, s);
+  const int nb = std::min(ceil_div(n, 256 * 16), 1024);
+  level_open_kernel<<<nb, 256, 0, s>>>(t.point_codes.get(), n, sort_level, t.open_hist.get());
+  PAM_LAUNCH_CHECK();
+}
+
+std::vector<int64_t> level_counts(const std::vector<int64_t>& open_hist) {
+  std::vector<int64_t> c(open_hist.size());
+  int64_t acc = 0;
+  for (size_t l = 0; l < c.size(); l++) c[l] = acc += open_hist[l];
+  return c;
+}
+
+void finish_tree(Tree& t, int level, const std::vector<int64_t>& counts, TreeWorkspace& ws,
+                 hipStream_t s) {
+  if (level < 0 || level > t.sort_level) throw_invalid("finish_tree: level beyond the sort level");
+  if (static_cast<int>(counts.size()) <= level) throw_invalid("finish_tree: missing level counts");
+  const int64_t n = t.n_points;
+  t.level = level;
+  t.inv_cell = std::ldexp(1.0, level) / t.box.width;
   // Level buffers are kept and only grown: freeing device memory would
   // synchronize the whole device (and every stream of other evaluators).
   if (t.levels.size() < static_cast<size_t>(level) + 1) t.levels.resize(level + 1);
@@ -285,62 +417,54 @@ void build_tree(Tree& t, const double* d_pts, int64_t n, const Box& box, int lev
     t.leaf_start.zero(s);
     return;
   }
-  t.point_codes.resize(n);
-  t.perm.resize(n);
-  Lo3 lo{{box.lo[0], box.lo[1], box.lo[2]}};
-  morton_kernel<<<ceil_div(n, 256), 256, 0, s>>>(d_pts, n, lo, t.inv_cell,
-                                                 (int64_t{1} << level) - 1, t.point_codes.get(),
-                                                 t.perm.get());
-  PAM_LAUNCH_CHECK();
-  radix_sort_pairs(t.point_codes.get(), t.perm.get(), n, 3 * level, ws.tk, ws.tv, ws.hist,
-                   ws.scan_scratch, s);
-  // Leaves, then coarser levels as unique runs of the child codes >> 3. The
-  // cell counts stay on the device until all levels are built; buffers are
-  // sized by the bounds min(count of the finer level, 8^l).
-  ws.counts.resize(static_cast<size_t>(level) + 1);
-  std::vector<int64_t> bound(static_cast<size_t>(level) + 1);
-  bound[level] = n;
-  TreeLevel& leaves = t.levels[level];
-  unique_runs(t.point_codes.get(), n, nullptr, 0, leaves.codes, t.leaf_start, t.point_leaf,
-              ws.counts.get() + level, ws, s);
-  for (int l = level - 1; l >= 0; l--) {
-    TreeLevel& child = t.levels[l + 1];
-    TreeLevel& cur = t.levels[l];
-    bound[l] = std::min<int64_t>(bound[l + 1], int64_t{1} << (3 * l));
-    unique_runs(child.codes.get(), bound[l + 1], ws.counts.get() + l + 1, 3, cur.codes, cur.child_start,
-                child.parent, ws.counts.get() + l, ws, s);
-  }
-  std::vector<int64_t> counts(static_cast<size_t>(level) + 1);
-  PAM_HIP(hipMemcpyAsync(counts.data(), ws.counts.get(), counts.size() * sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  PAM_HIP(hipStreamSynchronize(s));
-  for (int l = 0; l <= level; l++) t.levels[l].n = counts[l];
-}
-
-namespace {
-
-__global__ void shift_codes_kernel(uint64_t* codes, int64_t n, int shift) {
-  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) codes[i] >>= shift;
-}
-
-}  // namespace
-
-void truncate_tree(Tree& t, int level, TreeWorkspace& ws, hipStream_t s) {
-  if (level >= t.level) return;
-  if (level < 0) throw_invalid("tree level out of range");
-  const int64_t n = t.n_points;
   // Morton codes of level l are the level-L codes shifted right by 3(L - l):
   // the cell index floor((x - lo) * 2^L / w) scales by an exact power of two.
-  if (n > 0) {
-    shift_codes_kernel<<<ceil_div(n, 256), 256, 0, s>>>(t.point_codes.get(), n, 3 * (t.level - level));
+  if (t.sort_level > level) {
+    shift_codes_kernel<<<ceil_div(n, 256), 256, 0, s>>>(t.point_codes.get(), n, 3 * (t.sort_level - level));
     PAM_LAUNCH_CHECK();
+    t.sort_level = level;
   }
-  t.level = level;  // deeper level buffers are kept for reuse
-  t.inv_cell = std::ldexp(1.0, level) / t.box.width;
-  if (n == 0) return;
   ws.counts.resize(static_cast<size_t>(level) + 1);
+  for (int l = 0; l <= level; l++) t.levels[l].n = counts[l];
   unique_runs(t.point_codes.get(), n, nullptr, 0, t.levels[level].codes, t.leaf_start, t.point_leaf,
               ws.counts.get() + level, ws, s);
+  if (level == 0) return;
+  if (counts[level] <= kCoarseMaxLeaves) {
+    CoarseArgs a{};
+    a.level = level;
+    for (int l = 0; l <= level; l++) {
+      TreeLevel& lv = t.levels[l];
+      a.n[l] = counts[l];
+      if (l < level) {
+        lv.codes.resize(static_cast<size_t>(counts[l]));
+        lv.child_start.resize(static_cast<size_t>(counts[l]) + 1);
+        a.child_start[l] = lv.child_start.get();
+      }
+      if (l >= 1) {
+        lv.parent.resize(static_cast<size_t>(counts[l]));
+        a.parent[l] = lv.parent.get();
+      }
+      a.codes[l] = lv.codes.get();
+    }
+    coarse_levels_kernel<<<1, kCoarseThreads, 0, s>>>(a);
+    PAM_LAUNCH_CHECK();
+  } else {
+    for (int l = level - 1; l >= 0; l--) {
+      TreeLevel& child = t.levels[l + 1];
+      TreeLevel& cur = t.levels[l];
+      unique_runs(child.codes.get(), counts[l + 1], nullptr, 3, cur.codes, cur.child_start, child.parent,
+                  ws.counts.get() + l, ws, s);
+    }
+  }
+}
+
+void build_tree(Tree& t, const double* d_pts, int64_t n, const Box& box, int level,
+                TreeWorkspace& ws, hipStream_t s) {
+  sort_tree_points(t, d_pts, n, box, level, ws, s);
+  std::vector<int64_t> hist(static_cast<size_t>(level) + 1);
+  PAM_HIP(hipMemcpyAsync(hist.data(), t.open_hist.get(), hist.size() * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  PAM_HIP(hipStreamSynchronize(s));
+  finish_tree(t, level, level_counts(hist), ws, s);
 }
 
 void build_lists(const Tree& trg, const Tree& src, Lists& lists, TreeWorkspace& ws, hipStream_t s) {

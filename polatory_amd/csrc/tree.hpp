@@ -28,8 +28,11 @@ struct TreeLevel {
   DevBuf<int32_t> child_start;   // [n+1] child ranges in level+1 (level < leaf level)
 };
 
+constexpr int kMaxTreeLevel = 20;
+
 struct Tree {
   int level = -1;                // leaf level L
+  int sort_level = -1;           // level of the Morton codes the points are sorted by (>= L)
   Box box{};
   double inv_cell = 0.0;         // 2^L / width
   int64_t n_points = 0;
@@ -37,6 +40,7 @@ struct Tree {
   DevBuf<int32_t> perm;          // [n] sorted position -> input index
   DevBuf<int32_t> point_leaf;    // [n] sorted position -> leaf index
   DevBuf<int32_t> leaf_start;    // [n_leaves+1]
+  DevBuf<int64_t> open_hist;     // [sort_level+1] points whose coarsest newly opened cell is at level l
   std::vector<TreeLevel> levels; // [0..L]; levels[L] are the leaves (buffers of deeper entries kept for reuse)
   int64_t n_leaves() const { return level >= 0 ? levels[level].n : 0; }
 };
@@ -72,11 +76,22 @@ constexpr int kM2LCols = 64;
 const std::vector<int32_t>& octant_vlist_host();  // [8][189]
 const int32_t* octant_vlist_device(hipStream_t s);
 
-// Cuts a tree built at a deeper leaf level back to leaf level `level`: cells
-// of the levels <= level are unchanged, the points' leaf codes are shifted and
-// the leaf ranges recomputed (points stay sorted by their finer codes).
-void truncate_tree(Tree& t, int level, TreeWorkspace& ws, hipStream_t s);
+// Sorts the points by their Morton codes at `sort_level` (stable, so ties
+// keep input order) and histograms, on the device into t.open_hist, the
+// coarsest level at which each point opens a new cell. Nothing is read back.
+void sort_tree_points(Tree& t, const double* d_pts, int64_t n, const Box& box, int sort_level,
+                      TreeWorkspace& ws, hipStream_t s);
 
+// Occupied cells per level 0..sort_level from a downloaded open_hist.
+std::vector<int64_t> level_counts(const std::vector<int64_t>& open_hist);
+
+// Builds leaf level `level` (<= t.sort_level) and every coarser level from the
+// sorted points; counts[l] = occupied cells at level l. The points stay
+// sorted by their codes at the sort level.
+void finish_tree(Tree& t, int level, const std::vector<int64_t>& counts, TreeWorkspace& ws,
+                 hipStream_t s);
+
+// sort_tree_points + finish_tree at one level (one host round trip).
 void build_tree(Tree& t, const double* d_pts, int64_t n, const Box& box, int level,
                 TreeWorkspace& ws, hipStream_t s);
 void build_lists(const Tree& trg, const Tree& src, Lists& lists, TreeWorkspace& ws, hipStream_t s);
