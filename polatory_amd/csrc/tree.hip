@@ -108,25 +108,43 @@ __global__ void run_count_kernel(const int32_t* __restrict__ flags, const int32_
   out_start[runs] = static_cast<int32_t>(n);
 }
 
+// One thread per (target leaf, neighbour slot).
 __global__ void p2p_list_kernel(const uint64_t* __restrict__ tcodes, int64_t nt,
                                 const uint64_t* __restrict__ scodes, int64_t ns, int64_t side,
                                 int32_t* __restrict__ out) {
-  int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t t = gid / 27;
+  int s = static_cast<int>(gid % 27);
   if (t >= nt) return;
   int64_t x, y, z;
   decode3(tcodes[t], &x, &y, &z);
-  for (int s = 0; s < 27; s++) {
-    int64_t qx = x + s / 9 - 1, qy = y + (s / 3) % 3 - 1, qz = z + s % 3 - 1;
-    int32_t f = -1;
-    if (qx >= 0 && qx < side && qy >= 0 && qy < side && qz >= 0 && qz < side)
-      f = static_cast<int32_t>(find_code(scodes, ns, encode3(qx, qy, qz)));
-    out[t * 27 + s] = f;
-  }
+  int64_t qx = x + s / 9 - 1, qy = y + (s / 3) % 3 - 1, qz = z + s % 3 - 1;
+  int32_t f = -1;
+  if (qx >= 0 && qx < side && qy >= 0 && qy < side && qz >= 0 && qz < side)
+    f = static_cast<int32_t>(find_code(scodes, ns, encode3(qx, qy, qz)));
+  out[t * 27 + s] = f;
 }
 
-__global__ void m2l_list_kernel(const uint64_t* __restrict__ tcodes, int64_t nt,
-                                const uint64_t* __restrict__ scodes, int64_t ns, int64_t side,
-                                const int32_t* __restrict__ vlist, int32_t* __restrict__ out) {
+// Per-level inputs of the list kernels that cover all M2L levels in one launch
+// (blockIdx.y = level - 2).
+struct LevelLists {
+  int n_levels;
+  int64_t nt[kMaxTreeLevel + 1], ns[kMaxTreeLevel + 1], col_bound[kMaxTreeLevel + 1],
+      blk_bound[kMaxTreeLevel + 1];
+  const uint64_t* tcodes[kMaxTreeLevel + 1];
+  const uint64_t* scodes[kMaxTreeLevel + 1];
+  int32_t* src[kMaxTreeLevel + 1];
+  int32_t* col_cell[kMaxTreeLevel + 1];
+  int32_t* blk_oct[kMaxTreeLevel + 1];
+  int64_t* lay[kMaxTreeLevel + 1];
+};
+
+__global__ void m2l_list_kernel(LevelLists a, const int32_t* __restrict__ vlist) {
+  const int l = static_cast<int>(blockIdx.y) + 2;
+  const uint64_t* __restrict__ tcodes = a.tcodes[l];
+  const uint64_t* __restrict__ scodes = a.scodes[l];
+  const int64_t nt = a.nt[l], ns = a.ns[l], side = int64_t{1} << l;
+  int32_t* __restrict__ out = a.src[l];
   int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   int64_t t = gid / kM2LSlots;
   int k = static_cast<int>(gid % kM2LSlots);
@@ -143,60 +161,92 @@ __global__ void m2l_list_kernel(const uint64_t* __restrict__ tcodes, int64_t nt,
   out[t * kM2LSlots + k] = f;
 }
 
-__global__ void octant_key_kernel(const uint64_t* __restrict__ codes, int64_t n,
-                                  uint64_t* __restrict__ keys, int32_t* __restrict__ idx) {
-  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  keys[i] = codes[i] & 7u;
-  idx[i] = static_cast<int32_t>(i);
-}
-
 // Column layout of one level: octant groups in octant order, each padded to a
-// multiple of kM2LCols. lay[0..7] = start of each octant group among the cells
-// sorted by octant, lay[8..15] = its first column, lay[16] = number of columns.
+// multiple of kM2LCols; inside a group the cells keep their (Morton) order.
+// lay[0..7] = number of cells of the octants before each group, lay[8..15] =
+// its first column, lay[16] = number of columns. col_cell[c] = target cell of
+// column c or -1; blk_oct[b] = octant of column block b (-1 beyond the
+// layout). One workgroup per level.
 constexpr int kLayout = 17;
+constexpr int kColThreads = 1024;
 
-__global__ void octant_layout_kernel(const uint64_t* __restrict__ sorted_keys, int64_t n,
-                                     int64_t* __restrict__ lay) {
-  if (threadIdx.x != 0) return;
-  int64_t start = 0, col = 0;
-  for (int k = 0; k < 8; k++) {
-    // first position whose key is > k (keys are sorted)
-    int64_t lo = start, hi = n;
-    while (lo < hi) {
-      int64_t mid = (lo + hi) >> 1;
-      if (sorted_keys[mid] <= static_cast<uint64_t>(k)) lo = mid + 1; else hi = mid;
+__global__ void __launch_bounds__(kColThreads) octant_columns_kernel(LevelLists a) {
+  __shared__ int32_t cnt[8];
+  __shared__ int64_t first[8], col0[8], total_cols;
+  __shared__ int32_t wcnt[kColThreads / 64][8];
+  const int l = static_cast<int>(blockIdx.x) + 2;
+  const int64_t n = a.nt[l];
+  const uint64_t* __restrict__ codes = a.tcodes[l];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid < 8) cnt[tid] = 0;
+  __syncthreads();
+  for (int64_t i = tid; i < n; i += kColThreads) atomicAdd(&cnt[codes[i] & 7u], 1);
+  __syncthreads();
+  if (tid == 0) {
+    int64_t start = 0, col = 0;
+    for (int k = 0; k < 8; k++) {
+      first[k] = start;
+      col0[k] = col;
+      start += cnt[k];
+      col += (cnt[k] + kM2LCols - 1) / kM2LCols * kM2LCols;
     }
-    int64_t cnt = lo - start;
-    lay[k] = start;
-    lay[8 + k] = col;
-    start = lo;
-    col += (cnt + kM2LCols - 1) / kM2LCols * kM2LCols;
+    total_cols = col;
+    for (int k = 0; k < 8; k++) {
+      a.lay[l][k] = first[k];
+      a.lay[l][8 + k] = col0[k];
+    }
+    a.lay[l][16] = col;
   }
-  lay[16] = col;
-}
-
-__global__ void fill_cols_kernel(const int32_t* __restrict__ sorted_idx,
-                                 const uint64_t* __restrict__ sorted_keys, int64_t n,
-                                 const int64_t* __restrict__ lay, int32_t* __restrict__ col_cell) {
-  int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  int oct = static_cast<int>(sorted_keys[j]);
-  col_cell[lay[8 + oct] + (j - lay[oct])] = sorted_idx[j];
-}
-
-// blk_oct[b] = octant of column block b (blocks beyond the layout: -1)
-__global__ void fill_blk_oct_kernel(const int64_t* __restrict__ lay, int64_t n_blk_bound,
-                                    int32_t* __restrict__ blk_oct) {
-  int64_t b = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (b >= n_blk_bound) return;
-  int64_t c = b * kM2LCols;
-  int oct = -1;
-  if (c < lay[16]) {
+  __syncthreads();
+  // Padding columns (past each group's cells, and past the layout) are -1;
+  // they are disjoint from the columns the scatter below writes.
+  for (int64_t c = tid; c < a.col_bound[l]; c += kColThreads) {
+    bool pad = c >= total_cols;
     for (int k = 0; k < 8; k++)
-      if (c >= lay[8 + k]) oct = k;
+      if (c >= col0[k] + cnt[k] && c < (k < 7 ? col0[k + 1] : total_cols)) pad = true;
+    if (pad) a.col_cell[l][c] = -1;
   }
-  blk_oct[b] = oct;
+  // Stable scatter: a cell's column is its octant's first column plus the
+  // number of earlier cells of the same octant.
+  int64_t run[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) run[k] = col0[k];
+  const uint64_t lt = (uint64_t{1} << lane) - 1;
+  for (int64_t base = 0; base < n; base += kColThreads) {
+    const int64_t i = base + tid;
+    const int oct = i < n ? static_cast<int>(codes[i] & 7u) : -1;
+    int rank = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const uint64_t m = __ballot(oct == k);
+      if (oct == k) rank = __popcll(m & lt);
+      if (lane == 0) wcnt[wave][k] = __popcll(m);
+    }
+    __syncthreads();
+    if (oct >= 0) {
+      int64_t pos = 0;
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        if (k != oct) continue;
+        pos = run[k];
+        for (int w = 0; w < wave; w++) pos += wcnt[w][k];
+      }
+      a.col_cell[l][pos + rank] = static_cast<int32_t>(i);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      for (int w = 0; w < kColThreads / 64; w++) run[k] += wcnt[w][k];
+    __syncthreads();
+  }
+  for (int64_t b = tid; b < a.blk_bound[l]; b += kColThreads) {
+    const int64_t c = b * kM2LCols;
+    int oct = -1;
+    if (c < total_cols) {
+      for (int k = 0; k < 8; k++)
+        if (c >= col0[k]) oct = k;
+    }
+    a.blk_oct[l][b] = oct;
+  }
 }
 
 __global__ void fill_i32_kernel(int32_t* a, int64_t n, int32_t v) {
@@ -473,9 +523,9 @@ void build_lists(const Tree& trg, const Tree& src, Lists& lists, TreeWorkspace& 
   int64_t nt = trg.n_leaves();
   lists.p2p.resize(static_cast<size_t>(nt) * 27);
   if (nt > 0) {
-    p2p_list_kernel<<<ceil_div(nt, 128), 128, 0, s>>>(trg.levels[L].codes.get(), nt,
-                                                      src.levels[L].codes.get(), src.levels[L].n,
-                                                      int64_t{1} << L, lists.p2p.get());
+    p2p_list_kernel<<<ceil_div(nt * 27, 256), 256, 0, s>>>(trg.levels[L].codes.get(), nt,
+                                                          src.levels[L].codes.get(), src.levels[L].n,
+                                                          int64_t{1} << L, lists.p2p.get());
     PAM_LAUNCH_CHECK();
   }
   const int32_t* vl = octant_vlist_device(s);
@@ -483,43 +533,43 @@ void build_lists(const Tree& trg, const Tree& src, Lists& lists, TreeWorkspace& 
   lists.level = L;
   // Per level: the M2L slots, then the columns grouped by octant (stable, Morton
   // order inside a group), each group padded to a multiple of kM2LCols so a
-  // column block has a single octant. The layouts of all levels come back to
-  // the host in one copy at the end.
+  // column block has a single octant. All levels go in one launch of each
+  // kernel; the layouts come back to the host in one copy at the end.
   ws.layout.resize(static_cast<size_t>(L + 1) * kLayout);
+  LevelLists a{};
+  a.n_levels = L >= 2 ? L - 1 : 0;
+  int64_t max_work = 0;
   for (int l = 2; l <= L; l++) {
     const TreeLevel& tl = trg.levels[l];
     const TreeLevel& sl = src.levels[l];
     M2LLevel& m = lists.m2l[l];
-    int64_t n = tl.n;
+    const int64_t n = tl.n;
     m.src.resize(static_cast<size_t>(n) * kM2LSlots);
     m.n_cells = n;
     m.n_cols = 0;
     m.h_blk_oct.clear();
-    if (n == 0) continue;
-    int64_t work = n * kM2LSlots;
-    m2l_list_kernel<<<ceil_div(work, 256), 256, 0, s>>>(tl.codes.get(), n, sl.codes.get(), sl.n,
-                                                        int64_t{1} << l, vl, m.src.get());
-    PAM_LAUNCH_CHECK();
-    ws.tmp_codes.resize(n);
-    ws.tmp_idx.resize(n);
-    octant_key_kernel<<<ceil_div(n, 256), 256, 0, s>>>(tl.codes.get(), n, ws.tmp_codes.get(),
-                                                       ws.tmp_idx.get());
-    PAM_LAUNCH_CHECK();
-    radix_sort_pairs(ws.tmp_codes.get(), ws.tmp_idx.get(), n, 3, ws.tk, ws.tv, ws.hist,
-                     ws.scan_scratch, s);
-    int64_t* lay = ws.layout.get() + static_cast<int64_t>(l) * kLayout;
-    octant_layout_kernel<<<1, 64, 0, s>>>(ws.tmp_codes.get(), n, lay);
-    PAM_LAUNCH_CHECK();
     const int64_t col_bound = n + 8 * (kM2LCols - 1);
     const int64_t blk_bound = col_bound / kM2LCols + 1;
     m.col_cell.resize(col_bound);
-    fill_i32_kernel<<<ceil_div(col_bound, 256), 256, 0, s>>>(m.col_cell.get(), col_bound, -1);
-    PAM_LAUNCH_CHECK();
-    fill_cols_kernel<<<ceil_div(n, 256), 256, 0, s>>>(ws.tmp_idx.get(), ws.tmp_codes.get(), n, lay,
-                                                      m.col_cell.get());
-    PAM_LAUNCH_CHECK();
     m.blk_oct.resize(blk_bound);
-    fill_blk_oct_kernel<<<ceil_div(blk_bound, 256), 256, 0, s>>>(lay, blk_bound, m.blk_oct.get());
+    a.nt[l] = n;
+    a.ns[l] = sl.n;
+    a.col_bound[l] = col_bound;
+    a.blk_bound[l] = blk_bound;
+    a.tcodes[l] = tl.codes.get();
+    a.scodes[l] = sl.codes.get();
+    a.src[l] = m.src.get();
+    a.col_cell[l] = m.col_cell.get();
+    a.blk_oct[l] = m.blk_oct.get();
+    a.lay[l] = ws.layout.get() + static_cast<int64_t>(l) * kLayout;
+    max_work = std::max(max_work, n * kM2LSlots);
+  }
+  if (a.n_levels > 0) {
+    if (max_work > 0) {
+      m2l_list_kernel<<<dim3(ceil_div(max_work, 256), a.n_levels), 256, 0, s>>>(a, vl);
+      PAM_LAUNCH_CHECK();
+    }
+    octant_columns_kernel<<<a.n_levels, kColThreads, 0, s>>>(a);
     PAM_LAUNCH_CHECK();
   }
   std::vector<int64_t> lay(static_cast<size_t>(L + 1) * kLayout, 0);
