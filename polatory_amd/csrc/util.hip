@@ -118,7 +118,56 @@ __global__ void scan_add_kernel(int32_t* out, int64_t n, const int32_t* offs) {
   }
 }
 
+// Exclusive scan of a short array in one workgroup: chunks of 4096 entries
+// (4 consecutive per thread), wave scans by shuffles and a running carry.
+// Every thread reads and writes only its own entries, so in may equal out.
+constexpr int kScan1Threads = 1024;
+constexpr int64_t kScan1Max = 65536;
+
+__global__ void __launch_bounds__(kScan1Threads) scan_single_kernel(const int32_t* in, int32_t* out,
+                                                                    int64_t n) {
+  __shared__ int32_t wsum[kScan1Threads / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int32_t carry = 0;
+  for (int64_t base = 0; base < n; base += 4 * kScan1Threads) {
+    const int64_t i0 = base + 4 * static_cast<int64_t>(tid);
+    int32_t v[4];
+    int32_t tsum = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      v[q] = i0 + q < n ? in[i0 + q] : 0;
+      tsum += v[q];
+    }
+    int32_t incl = tsum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      int32_t u = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += u;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    int32_t before = 0, total = 0;
+    for (int w = 0; w < kScan1Threads / 64; w++) {
+      if (w < wave) before += wsum[w];
+      total += wsum[w];
+    }
+    int32_t run = carry + before + incl - tsum;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      if (i0 + q < n) out[i0 + q] = run;
+      run += v[q];
+    }
+    carry += total;
+    __syncthreads();
+  }
+}
+
 void scan_rec(const int32_t* in, int32_t* out, int64_t n, int32_t* scratch, hipStream_t s) {
+  if (n <= kScan1Max) {
+    scan_single_kernel<<<1, kScan1Threads, 0, s>>>(in, out, n);
+    PAM_LAUNCH_CHECK();
+    return;
+  }
   int64_t nb = (n + kScanTile - 1) / kScanTile;
   if (nb <= 1) {
     scan_block_kernel<<<1, kScanThreads, 0, s>>>(in, out, n, nullptr);
