@@ -660,14 +660,165 @@ struct M2LDenseArgs {
   int64_t cell_off[kMaxM2LLevels];
 };
 
+// Shared-memory tiles of the M2L main loop (double-buffered operator and
+// multipole chunks).
+template <class TM>
+struct M2LSmem {
+  TM as[2][M2LTile<TM>::kKB][kM2LR][M2LTile<TM>::kAP];
+  TM bs[2][kM2LC][M2LTile<TM>::kBP];
+};
+
+// Main loop shared by the M2L kernels: for the workgroup's 128 operator rows
+// (row block rb) and 64 columns, dacc = sum over the nact active transfer
+// vectors act_v[ia] of A_v[rows][0:kpad] x in[src_tab[ia][col]][0:kpad]
+// (columns with src_tab -1 contribute zero). Chunks of kK reduction indices
+// are multiplied on MFMA in the operand type, FP32 partial sums being folded
+// into FP64 every 64 indices. Ends with a barrier.
+template <class TM>
+__device__ __forceinline__ void m2l_mainloop(M2LSmem<TM>& sm, const int (*src_tab)[kM2LC],
+                                             const int* act_v, int nact, const TM* __restrict__ a,
+                                             int64_t per_v, const TM* __restrict__ in, int kpad, int rpad,
+                                             int rb, double (&dacc)[4][2][4]) {
+  using Tl = M2LTile<TM>;
+  using acc_t = typename Mfma<TM>::acc_t;
+  constexpr int kK = Tl::kK, kKB = Tl::kKB, kUpr = Tl::kUpr;
+  const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+  const int nkc = kpad / kK;
+  const int total = nact * nkc;
+
+  // Loader roles: operator units tid + 256 q (q < 4); multipole columns
+  // bc0 = tid / 8 and bc1 = 32 + tid / 8, 16-byte piece bq = tid % 8.
+  // Every global load of the main loop is issued unconditionally (an absent
+  // source reads cell 0 and is zeroed when stored), so the compiler can wait
+  // for exactly the register set being stored while the next one stays in
+  // flight.
+  const int bc0 = tid / 8, bc1 = 32 + tid / 8, bq = tid % 8;
+  constexpr int kAU = kM2LR * kUpr;  // operator units per 16-wide block
+  static_assert(kAU % kThreads == 0, "operator units of one thread never straddle a block");
+  // Two register sets: the chunk two steps ahead is in flight while the next
+  // one waits to be stored into the free LDS buffer.
+  // Native 4 x 32-bit vectors (not HIP's uint4 wrapper) so the sets stay in
+  // registers.
+  using u32x4 = uint32_t __attribute__((ext_vector_type(4)));
+  struct RegSet {
+    u32x4 a[4], b[2];
+    uint32_t m0, m1;  // all ones where the loader's column has a source in this chunk's slot
+  };
+  RegSet r0, r1;
+
+  auto load_chunk = [&](int t, RegSet& rs) {
+    const int ia = t / nkc, kc = t - ia * nkc;
+    const int k0 = kc * kK;
+    const TM* av = a + static_cast<int64_t>(act_v[ia]) * per_v;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int seg = (q * kThreads) / kAU;
+      const int rem = (q * kThreads) % kAU + tid;
+      const int row = rem / kUpr, pc = rem % kUpr;
+      const int kb = (k0 >> 4) + seg;
+      const TM* g = av + ((static_cast<int64_t>(kb) * rpad + rb * kM2LR + row) << 4);
+      rs.a[q] = reinterpret_cast<const u32x4*>(g)[pc];
+    }
+    const int kel = k0 + bq * (16 / static_cast<int>(sizeof(TM)));
+    const int s0 = src_tab[ia][bc0], s1 = src_tab[ia][bc1];
+    rs.m0 = s0 >= 0 ? ~0u : 0u;
+    rs.m1 = s1 >= 0 ? ~0u : 0u;
+    rs.b[0] = *reinterpret_cast<const u32x4*>(in + static_cast<int64_t>(max(s0, 0)) * kpad + kel);
+    rs.b[1] = *reinterpret_cast<const u32x4*>(in + static_cast<int64_t>(max(s1, 0)) * kpad + kel);
+  };
+  auto store_chunk = [&](int buf, const RegSet& rs) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int seg = (q * kThreads) / kAU;
+      const int rem = (q * kThreads) % kAU + tid;
+      const int row = rem / kUpr, pc = rem % kUpr;
+      *reinterpret_cast<u32x4*>(&sm.as[buf][seg][row][pc * (16 / static_cast<int>(sizeof(TM)))]) = rs.a[q];
+    }
+    // Masked with AND rather than a select: a select of two loaded values
+    // becomes a select of addresses, which pins the sets in scratch.
+    *reinterpret_cast<u32x4*>(&sm.bs[buf][bc0][bq * (16 / static_cast<int>(sizeof(TM)))]) = rs.b[0] & rs.m0;
+    *reinterpret_cast<u32x4*>(&sm.bs[buf][bc1][bq * (16 / static_cast<int>(sizeof(TM)))]) = rs.b[1] & rs.m1;
+  };
+
+  acc_t acc[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      acc[i][j] = acc_t{0, 0, 0, 0};
+#pragma unroll
+      for (int q = 0; q < 4; q++) dacc[i][j][q] = 0.0;
+    }
+  auto flush = [&]() {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          dacc[i][j][q] += static_cast<double>(acc[i][j][q]);
+          acc[i][j][q] = TM(0);
+        }
+  };
+
+  const int wr = wave >> 1, wc = wave & 1;
+  auto compute = [&](int t) {
+    const int buf = t & 1;
+    const int kc = t % nkc;
+#pragma unroll
+    for (int kb = 0; kb < kKB; kb++) {
+      using V4T = V4<TM>;
+      V4T af[4], bf[2];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+        af[i] = *reinterpret_cast<const V4T*>(&sm.as[buf][kb][wr * 64 + i * 16 + (lane & 15)][(lane >> 4) * 4]);
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+        bf[j] = *reinterpret_cast<const V4T*>(&sm.bs[buf][wc * 32 + j * 16 + (lane & 15)][kb * 16 + (lane >> 4) * 4]);
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+          acc[i][j] = Mfma<TM>::op(af[i].x, bf[j].x, acc[i][j]);
+          acc[i][j] = Mfma<TM>::op(af[i].y, bf[j].y, acc[i][j]);
+          acc[i][j] = Mfma<TM>::op(af[i].z, bf[j].z, acc[i][j]);
+          acc[i][j] = Mfma<TM>::op(af[i].w, bf[j].w, acc[i][j]);
+        }
+    }
+    // FP32 partial sums over at most 64 reduction steps are flushed into FP64.
+    if constexpr (sizeof(TM) == 4) {
+      if ((kc & 1) == 1 || kc == nkc - 1) flush();
+    }
+  };
+  // Step t: chunk t+1 sits in registers (cur), chunk t+2 is loaded into the
+  // other set (nxt) while chunk t is multiplied from LDS buffer t&1; then
+  // chunk t+1 goes to LDS buffer (t+1)&1. Past the end the last chunk is
+  // loaded and stored again into the buffer no later step reads, so no load
+  // or store sits behind a branch.
+  auto step = [&](int t, const RegSet& cur, RegSet& nxt) {
+    load_chunk(min(t + 2, total - 1), nxt);
+    if (t < total) compute(t);
+    store_chunk((t + 1) & 1, cur);
+    __syncthreads();
+  };
+  if (total > 0) {
+    load_chunk(0, r1);
+    store_chunk(0, r1);
+    load_chunk(min(1, total - 1), r0);
+    __syncthreads();
+    for (int t = 0; t < total; t += 2) {
+      step(t, r0, r1);
+      step(t + 1, r1, r0);
+    }
+  }
+  if constexpr (sizeof(TM) == 8) flush();
+
+}
+
 template <class TM>
 __global__ void __launch_bounds__(kThreads)
 m2l_dense_kernel(M2LDenseArgs<TM> b) {
-  using Tl = M2LTile<TM>;
-  using acc_t = typename Mfma<TM>::acc_t;
-  constexpr int kK = Tl::kK, kKB = Tl::kKB, kAP = Tl::kAP, kBP = Tl::kBP, kUpr = Tl::kUpr;
-  __shared__ __align__(16) TM as[2][kKB][kM2LR][kAP];
-  __shared__ __align__(16) TM bs[2][kM2LC][kBP];
+  __shared__ __align__(16) M2LSmem<TM> sm;
   __shared__ int cells[kM2LC];
   __shared__ uint32_t mask[6];
   __shared__ int act_v[kM2LSplitSlots];
@@ -715,136 +866,9 @@ m2l_dense_kernel(M2LDenseArgs<TM> b) {
     src_tab[ia][c] = cell >= 0 ? src_slots[static_cast<int64_t>(cell) * kM2LSlots + act_s[ia]] : -1;
   }
   __syncthreads();
-  const int nkc = kpad / kK;
-  const int total = nact * nkc;
-
-  // Loader roles: operator units tid + 256 q (q < 4); multipole columns
-  // bc0 = tid / 8 and bc1 = 32 + tid / 8, 16-byte piece bq = tid % 8.
-  // Every global load of the main loop is issued unconditionally (an absent
-  // source reads cell 0 and is zeroed when stored), so the compiler can wait
-  // for exactly the register set being stored while the next one stays in
-  // flight.
-  const int bc0 = tid / 8, bc1 = 32 + tid / 8, bq = tid % 8;
-  constexpr int kAU = kM2LR * kUpr;  // operator units per 16-wide block
-  static_assert(kAU % kThreads == 0, "operator units of one thread never straddle a block");
-  // Two register sets: the chunk two steps ahead is in flight while the next
-  // one waits to be stored into the free LDS buffer.
-  // Native 4 x 32-bit vectors (not HIP's uint4 wrapper) so the sets stay in
-  // registers.
-  using u32x4 = uint32_t __attribute__((ext_vector_type(4)));
-  struct RegSet {
-    u32x4 a[4], b[2];
-    uint32_t m0, m1;  // all ones where the loader's column has a source in this chunk's slot
-  };
-  RegSet r0, r1;
-
-  auto load_chunk = [&](int t, RegSet& rs) {
-    const int ia = t / nkc, kc = t - ia * nkc;
-    const int k0 = kc * kK;
-    const TM* av = b.a[lv] + static_cast<int64_t>(act_v[ia]) * b.per_v;
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int seg = (q * kThreads) / kAU;
-      const int rem = (q * kThreads) % kAU + tid;
-      const int row = rem / kUpr, pc = rem % kUpr;
-      const int kb = (k0 >> 4) + seg;
-      const TM* g = av + ((static_cast<int64_t>(kb) * rpad + rb * kM2LR + row) << 4);
-      rs.a[q] = reinterpret_cast<const u32x4*>(g)[pc];
-    }
-    const int kel = k0 + bq * (16 / static_cast<int>(sizeof(TM)));
-    const int s0 = src_tab[ia][bc0], s1 = src_tab[ia][bc1];
-    rs.m0 = s0 >= 0 ? ~0u : 0u;
-    rs.m1 = s1 >= 0 ? ~0u : 0u;
-    rs.b[0] = *reinterpret_cast<const u32x4*>(in + static_cast<int64_t>(max(s0, 0)) * kpad + kel);
-    rs.b[1] = *reinterpret_cast<const u32x4*>(in + static_cast<int64_t>(max(s1, 0)) * kpad + kel);
-  };
-  auto store_chunk = [&](int buf, const RegSet& rs) {
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int seg = (q * kThreads) / kAU;
-      const int rem = (q * kThreads) % kAU + tid;
-      const int row = rem / kUpr, pc = rem % kUpr;
-      *reinterpret_cast<u32x4*>(&as[buf][seg][row][pc * (16 / static_cast<int>(sizeof(TM)))]) = rs.a[q];
-    }
-    // Masked with AND rather than a select: a select of two loaded values
-    // becomes a select of addresses, which pins the sets in scratch.
-    *reinterpret_cast<u32x4*>(&bs[buf][bc0][bq * (16 / static_cast<int>(sizeof(TM)))]) = rs.b[0] & rs.m0;
-    *reinterpret_cast<u32x4*>(&bs[buf][bc1][bq * (16 / static_cast<int>(sizeof(TM)))]) = rs.b[1] & rs.m1;
-  };
-
-  acc_t acc[4][2];
   double dacc[4][2][4];
-#pragma unroll
-  for (int i = 0; i < 4; i++)
-#pragma unroll
-    for (int j = 0; j < 2; j++) {
-      acc[i][j] = acc_t{0, 0, 0, 0};
-#pragma unroll
-      for (int q = 0; q < 4; q++) dacc[i][j][q] = 0.0;
-    }
-  auto flush = [&]() {
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-      for (int j = 0; j < 2; j++)
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          dacc[i][j][q] += static_cast<double>(acc[i][j][q]);
-          acc[i][j][q] = TM(0);
-        }
-  };
-
+  m2l_mainloop<TM>(sm, src_tab, act_v, nact, b.a[lv], b.per_v, in, kpad, rpad, rb, dacc);
   const int wr = wave >> 1, wc = wave & 1;
-  auto compute = [&](int t) {
-    const int buf = t & 1;
-    const int kc = t % nkc;
-#pragma unroll
-    for (int kb = 0; kb < kKB; kb++) {
-      using V4T = V4<TM>;
-      V4T af[4], bf[2];
-#pragma unroll
-      for (int i = 0; i < 4; i++)
-        af[i] = *reinterpret_cast<const V4T*>(&as[buf][kb][wr * 64 + i * 16 + (lane & 15)][(lane >> 4) * 4]);
-#pragma unroll
-      for (int j = 0; j < 2; j++)
-        bf[j] = *reinterpret_cast<const V4T*>(&bs[buf][wc * 32 + j * 16 + (lane & 15)][kb * 16 + (lane >> 4) * 4]);
-#pragma unroll
-      for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-          acc[i][j] = Mfma<TM>::op(af[i].x, bf[j].x, acc[i][j]);
-          acc[i][j] = Mfma<TM>::op(af[i].y, bf[j].y, acc[i][j]);
-          acc[i][j] = Mfma<TM>::op(af[i].z, bf[j].z, acc[i][j]);
-          acc[i][j] = Mfma<TM>::op(af[i].w, bf[j].w, acc[i][j]);
-        }
-    }
-    // FP32 partial sums over at most 64 reduction steps are flushed into FP64.
-    if constexpr (sizeof(TM) == 4) {
-      if ((kc & 1) == 1 || kc == nkc - 1) flush();
-    }
-  };
-  // Step t: chunk t+1 sits in registers (cur), chunk t+2 is loaded into the
-  // other set (nxt) while chunk t is multiplied from LDS buffer t&1; then
-  // chunk t+1 goes to LDS buffer (t+1)&1. Past the end the last chunk is
-  // loaded and stored again into the buffer no later step reads, so no load
-  // or store sits behind a branch.
-  auto step = [&](int t, const RegSet& cur, RegSet& nxt) {
-    load_chunk(min(t + 2, total - 1), nxt);
-    if (t < total) compute(t);
-    store_chunk((t + 1) & 1, cur);
-    __syncthreads();
-  };
-  if (total > 0) {
-    load_chunk(0, r1);
-    store_chunk(0, r1);
-    load_chunk(min(1, total - 1), r0);
-    __syncthreads();
-    for (int t = 0; t < total; t += 2) {
-      step(t, r0, r1);
-      step(t + 1, r1, r0);
-    }
-  }
-  if constexpr (sizeof(TM) == 8) flush();
 
   // D layout (16 x 16), column lane % 16; register q holds row 4 (lane / 16) + q
   // for the f32 instruction and row lane / 16 + 4 q for the f64 one.
@@ -864,6 +888,110 @@ m2l_dense_kernel(M2LDenseArgs<TM> b) {
           b.partial[(static_cast<int64_t>(sp) * b.cells_total + b.cell_off[lv] + cell) * rpad + row] = dacc[i][j][q];
         }
       }
+  }
+}
+
+// Pair-major M2L: a workgroup takes one block of 64 (target cell, source
+// cell) pairs that share the transfer vector v and one row block, and writes
+// each pair's transfer A_v x in[src] (rows < r) to its own partial row
+// (stride kpad, operand type). The pairs of a block are independent columns,
+// so a pair's result does not depend on which block it sits in.
+template <class TM>
+struct M2LPairArgs {
+  int r, rpad, kpad;
+  int64_t per_v;
+  const int4* sched;  // (level, block, row block, v) per workgroup; level < 0: idle
+  const TM* in[kMaxM2LLevels];
+  const TM* a[kMaxM2LLevels];
+  const int32_t* blk_src[kMaxM2LLevels];
+  TM* partial[kMaxM2LLevels];
+};
+
+template <class TM>
+__global__ void __launch_bounds__(kThreads)
+m2l_pair_kernel(M2LPairArgs<TM> b) {
+  __shared__ __align__(16) M2LSmem<TM> sm;
+  __shared__ int src_tab[1][kM2LC];
+  __shared__ int act_v[1];
+  const int4 w = b.sched[blockIdx.x];
+  if (w.x < 0) return;
+  const int lv = w.x, blk = w.y, rb = w.z;
+  const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+  if (tid < kM2LC) src_tab[0][tid] = b.blk_src[lv][static_cast<int64_t>(blk) * kM2LC + tid];
+  if (tid == 0) act_v[0] = w.w;
+  __syncthreads();
+  double dacc[4][2][4];
+  m2l_mainloop<TM>(sm, src_tab, act_v, 1, b.a[lv], b.per_v, b.in[lv], b.kpad, b.rpad, rb, dacc);
+  const int wr = wave >> 1, wc = wave & 1;
+  TM* __restrict__ out = b.partial[lv];
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    const int col = wc * 32 + j * 16 + (lane & 15);
+    if (src_tab[0][col] < 0) continue;
+    const int64_t pair = static_cast<int64_t>(blk) * kM2LC + col;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        int row = rb * kM2LR + wr * 64 + i * 16 + (sizeof(TM) == 4 ? 4 * (lane >> 4) + q : (lane >> 4) + 4 * q);
+        if (row >= b.r) continue;
+        out[pair * b.kpad + row] = static_cast<TM>(dacc[i][j][q]);
+      }
+  }
+}
+
+// out[cell][row] = scale * sum over the cell's slots, in slot order, of the
+// partial row of its (cell, slot) pair. One workgroup per target cell.
+template <class TM>
+struct M2LPairReduceArgs {
+  int n, r, kpad;
+  int64_t cell_off[kMaxM2LLevels + 1];
+  const int32_t* pair_pos[kMaxM2LLevels];
+  const TM* partial[kMaxM2LLevels];
+  double* out[kMaxM2LLevels];
+  double scale[kMaxM2LLevels];
+};
+
+constexpr int kPairReduceThreads = 128;
+
+template <class TM>
+__global__ void __launch_bounds__(kPairReduceThreads) m2l_pair_reduce_kernel(M2LPairReduceArgs<TM> b) {
+  __shared__ int32_t pos[kM2LSlots + 3];
+  const int64_t g = blockIdx.x;
+  int k = 0;
+  while (k + 1 < b.n && b.cell_off[k + 1] <= g) k++;
+  const int64_t cell = g - b.cell_off[k];
+  for (int sl = threadIdx.x; sl < kM2LSlots + 3; sl += kPairReduceThreads)
+    pos[sl] = sl < kM2LSlots ? b.pair_pos[k][cell * kM2LSlots + sl] : -1;
+  __syncthreads();
+  const TM* __restrict__ partial = b.partial[k];
+  // Three rows per thread and four slots per trip keep twelve loads in
+  // flight; the additions still run in slot order.
+  constexpr int kRows = 3, kSl = 4;
+  for (int row0 = 0; row0 < b.r; row0 += kRows * kPairReduceThreads) {
+    double acc[kRows] = {0.0, 0.0, 0.0};
+    int row[kRows];
+#pragma unroll
+    for (int q = 0; q < kRows; q++) row[q] = min(row0 + q * kPairReduceThreads + static_cast<int>(threadIdx.x), b.r - 1);
+    for (int sl = 0; sl < kM2LSlots; sl += kSl) {
+      TM x[kSl][kRows];
+#pragma unroll
+      for (int u = 0; u < kSl; u++) {
+        const int32_t p = pos[sl + u];
+        const TM* pr = partial + static_cast<int64_t>(max(p, 0)) * b.kpad;
+#pragma unroll
+        for (int q = 0; q < kRows; q++) x[u][q] = p >= 0 ? pr[row[q]] : TM(0);
+      }
+#pragma unroll
+      for (int u = 0; u < kSl; u++)
+#pragma unroll
+        for (int q = 0; q < kRows; q++) acc[q] += static_cast<double>(x[u][q]);
+    }
+#pragma unroll
+    for (int q = 0; q < kRows; q++) {
+      const int rr = row0 + q * kPairReduceThreads + static_cast<int>(threadIdx.x);
+      if (rr < b.r) b.out[k][cell * b.kpad + rr] = acc[q] * b.scale[k];
+    }
   }
 }
 
@@ -1024,6 +1152,104 @@ void launch_m2l_dense(const M2LOperator& op, const std::vector<M2LDenseLevel<TM>
     PAM_LAUNCH_CHECK();
   }
 }
+
+namespace {
+
+// Pair-major work list: workgroups grouped by (level, v, row block), groups
+// dealt round-robin onto the 8 XCDs so that the workgroups streaming the same
+// operator rows share an L2.
+void build_m2l_pair_schedule(const std::vector<const M2LLevel*>& lvls, int nrowblk,
+                             M2LPairSchedule& sc, hipStream_t s) {
+  constexpr int kXcd = 8;
+  std::array<size_t, kXcd> len_x{};
+  int g = 0;
+  for (const M2LLevel* l : lvls)
+    for (int v = 0; v < 343; v++) {
+      if (l->h_v_blocks[v] == 0) continue;
+      for (int rb = 0; rb < nrowblk; rb++) len_x[g++ % kXcd] += l->h_v_blocks[v];
+    }
+  const size_t len = *std::max_element(len_x.begin(), len_x.end());
+  int32_t* h = sc.h.acquire(len * kXcd * 4);
+  std::fill(h, h + len * kXcd * 4, -1);
+  std::array<size_t, kXcd> pos{};
+  g = 0;
+  for (size_t li = 0; li < lvls.size(); li++) {
+    const M2LLevel& l = *lvls[li];
+    for (int v = 0; v < 343; v++) {
+      if (l.h_v_blocks[v] == 0) continue;
+      for (int rb = 0; rb < nrowblk; rb++) {
+        const int x = g++ % kXcd;
+        for (int bk = 0; bk < l.h_v_blocks[v]; bk++) {
+          int32_t* e = h + (pos[x]++ * kXcd + x) * 4;
+          e[0] = static_cast<int32_t>(li);
+          e[1] = l.h_v_first[v] + bk;
+          e[2] = rb;
+          e[3] = v;
+        }
+      }
+    }
+  }
+  sc.n_wg = static_cast<int>(len * kXcd);
+  sc.h.upload(sc.d, len * kXcd * 4, s);
+  sc.valid = true;
+}
+
+}  // namespace
+
+template <class TM>
+void launch_m2l_pairs(const M2LOperator& op, const std::vector<M2LDenseLevel<TM>>& levels,
+                      M2LPairSchedule& sched, DevBuf<TM>& partial, hipStream_t s) {
+  if (op.rpad % kM2LR != 0) throw_runtime("M2L operator rows are not padded to 128");
+  if (op.kpad % M2LTile<TM>::kK != 0) throw_runtime("M2L reduction length is not a multiple of the chunk");
+  M2LPairArgs<TM> b{};
+  M2LPairReduceArgs<TM> rb{};
+  b.r = rb.r = op.r;
+  b.rpad = op.rpad;
+  b.kpad = rb.kpad = op.kpad;
+  b.per_v = static_cast<int64_t>(op.per_v);
+  std::vector<const M2LLevel*> lvls;
+  int64_t pairs = 0, cells = 0;
+  for (const auto& l : levels) {
+    if (!l.lv || l.lv->n_cells == 0) continue;
+    if (static_cast<int>(l.lv->h_v_blocks.size()) != 343) throw_runtime("M2L pair lists missing");
+    pairs += l.lv->n_blk * kM2LC;
+  }
+  partial.resize(static_cast<size_t>(std::max<int64_t>(pairs, 1)) * op.kpad);
+  pairs = 0;
+  for (const auto& l : levels) {
+    if (!l.lv || l.lv->n_cells == 0) continue;
+    if (lvls.size() >= static_cast<size_t>(kMaxM2LLevels)) throw_runtime("too many M2L levels");
+    const int k = static_cast<int>(lvls.size());
+    b.in[k] = l.in;
+    b.a[k] = l.a;
+    b.blk_src[k] = l.lv->blk_src.get();
+    b.partial[k] = partial.get() + pairs * op.kpad;
+    rb.pair_pos[k] = l.lv->pair_pos.get();
+    rb.partial[k] = b.partial[k];
+    rb.out[k] = l.out;
+    rb.scale[k] = l.scale;
+    rb.cell_off[k] = cells;
+    pairs += l.lv->n_blk * kM2LC;
+    cells += l.lv->n_cells;
+    lvls.push_back(l.lv);
+  }
+  if (lvls.empty()) return;
+  rb.n = static_cast<int>(lvls.size());
+  rb.cell_off[rb.n] = cells;
+  if (!sched.valid) build_m2l_pair_schedule(lvls, op.rpad / kM2LR, sched, s);
+  b.sched = reinterpret_cast<const int4*>(sched.d.get());
+  if (sched.n_wg > 0) {
+    m2l_pair_kernel<TM><<<static_cast<unsigned>(sched.n_wg), kThreads, 0, s>>>(b);
+    PAM_LAUNCH_CHECK();
+  }
+  m2l_pair_reduce_kernel<TM><<<static_cast<unsigned>(cells), kPairReduceThreads, 0, s>>>(rb);
+  PAM_LAUNCH_CHECK();
+}
+
+template void launch_m2l_pairs<float>(const M2LOperator&, const std::vector<M2LDenseLevel<float>>&,
+                                      M2LPairSchedule&, DevBuf<float>&, hipStream_t);
+template void launch_m2l_pairs<double>(const M2LOperator&, const std::vector<M2LDenseLevel<double>>&,
+                                       M2LPairSchedule&, DevBuf<double>&, hipStream_t);
 
 namespace {
 

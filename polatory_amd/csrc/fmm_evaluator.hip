@@ -370,7 +370,7 @@ void FmmEvaluator::ensure_trees() {
       st_.t32_valid = st_.t64_valid = false;
     }
     if (rebuilt || lists_.level != L) {
-      build_lists(trg_tree_, stree, lists_, ws_, stream_);
+      build_lists(trg_tree_, stree, lists_, ws_, stream_, effective_svd_tol(cfg_, rbf_) > 0.0);
       lists_gen_++;
     }
     return;
@@ -469,7 +469,7 @@ void FmmEvaluator::ensure_trees() {
     if (!symmetric_) build_identity_cols(trg_tree_, trg_cols_, stream_);
   }
   if (rebuilt || lists_.level != L) {
-    build_lists(target_tree(), src_tree_, lists_, ws_, stream_);
+    build_lists(target_tree(), src_tree_, lists_, ws_, stream_, effective_svd_tol(cfg_, rbf_) > 0.0);
     lists_gen_++;
   }
 }
@@ -618,6 +618,10 @@ void FmmEvaluator::run_fmm(double* d_out) {
   for (int l = 2; l <= L; l++) st.l[l].resize(static_cast<size_t>(tt.levels[l].n) * kp);
   if (tol > 0.0) {
     // SVD-compressed M2L: compress multipoles, transfer in rank r, expand locals.
+    if (!lists_.has_columns) {
+      build_lists(tt, stree, lists_, ws_, stream_, true);
+      lists_gen_++;
+    }
     if (st.mt.size() < static_cast<size_t>(L) + 1) st.mt.resize(L + 1);
     if (st.lt.size() < static_cast<size_t>(L) + 1) st.lt.resize(L + 1);
     std::vector<std::shared_ptr<CompressedM2L<TP>>> ops(L + 1);
@@ -666,15 +670,17 @@ void FmmEvaluator::run_fmm(double* d_out) {
       xfer.push_back({&lists_.m2l[l], in, st.l[l].get(), ops[l]->a.get(), scale});
     }
     if (L >= 2) {
-      if (st.m2l_sched_gen != lists_gen_ || st.m2l_sched.n_wg == 0 || st.m2l_sched.nsplit < 1) {
-        st.m2l_sched.valid = false;
-      }
-      if (st.m2l_sched.valid && st.m2l_sched_rpad != ops[2]->op.rpad) st.m2l_sched.valid = false;
-      if (!st.m2l_sched.valid) {
+      // The work list follows the lists generation and the operator shape.
+      if (st.m2l_sched_gen != lists_gen_ || st.m2l_sched_rpad != ops[2]->op.rpad) {
+        st.m2l_psched.valid = false;
         st.m2l_sched_gen = lists_gen_;
         st.m2l_sched_rpad = ops[2]->op.rpad;
       }
-      launch_m2l_dense<TP>(ops[2]->op, xfer, vl, st.m2l_sched, st.m2l_partial, stream_);
+      if constexpr (sizeof(TP) == 4) {
+        launch_m2l_pairs<float>(ops[2]->op, xfer, st.m2l_psched, st.m2l_pairs32, stream_);
+      } else {
+        launch_m2l_pairs<double>(ops[2]->op, xfer, st.m2l_psched, st.m2l_pairs64, stream_);
+      }
     }
   }
   record(3);

@@ -51,6 +51,9 @@ struct FmmState {
   DevBuf<double> y_far;
   DevBuf<double> m2l_partial;          // split-reduction partial sums of the dense M2L
   M2LSchedule m2l_sched;
+  DevBuf<float> m2l_pairs32;           // per-pair transfers of the pair-major M2L (mixed form)
+  DevBuf<double> m2l_pairs64;          // per-pair transfers of the pair-major M2L (FP64 form)
+  M2LPairSchedule m2l_psched;
   uint64_t m2l_sched_gen = 0;
   int m2l_sched_rpad = 0;
   bool s64_valid = false, t64_valid = false, s32_valid = false, t32_valid = false;

@@ -137,7 +137,85 @@ struct LevelLists {
   int32_t* col_cell[kMaxTreeLevel + 1];
   int32_t* blk_oct[kMaxTreeLevel + 1];
   int64_t* lay[kMaxTreeLevel + 1];
+  int32_t* v_count[kMaxTreeLevel + 1];
+  const int32_t* v_first[kMaxTreeLevel + 1];
+  int32_t* blk_src[kMaxTreeLevel + 1];
+  int32_t* pair_pos[kMaxTreeLevel + 1];
 };
+
+// Pair kernels: a workgroup takes kPairCells target cells (all their slots)
+// of one level (blockIdx.y = level - 2) and counts its pairs per transfer
+// vector in LDS, so that only one global atomic per vector and workgroup is
+// needed.
+constexpr int kPairCells = 32;
+constexpr int kPairIters = (kPairCells * kM2LSlots + 255) / 256;
+
+// Pairs per transfer vector.
+__global__ void __launch_bounds__(256) pair_count_kernel(LevelLists a, const int32_t* __restrict__ vlist) {
+  __shared__ int32_t h[343];
+  const int l = static_cast<int>(blockIdx.y) + 2;
+  const int64_t t0 = static_cast<int64_t>(blockIdx.x) * kPairCells;
+  if (t0 >= a.nt[l]) return;
+  for (int v = threadIdx.x; v < 343; v += 256) h[v] = 0;
+  __syncthreads();
+  for (int it = 0; it < kPairIters; it++) {
+    const int e = it * 256 + static_cast<int>(threadIdx.x);
+    const int64_t t = t0 + e / kM2LSlots;
+    const int k = e % kM2LSlots;
+    if (e >= kPairCells * kM2LSlots || t >= a.nt[l]) continue;
+    if (a.src[l][t * kM2LSlots + k] < 0) continue;
+    const int oct = static_cast<int>(a.tcodes[l][t] & 7u);
+    atomicAdd(&h[vlist[oct * kM2LSlots + k]], 1);
+  }
+  __syncthreads();
+  for (int v = threadIdx.x; v < 343; v += 256)
+    if (h[v] != 0) atomicAdd(&a.v_count[l][v], h[v]);
+}
+
+// Places every pair into its transfer vector's blocks and records its index
+// (-1 for slots without a source): ranks inside the workgroup from LDS
+// atomics, the workgroup's range from one global atomic per vector on the
+// fill counters (v_count, zeroed). The order inside a group is immaterial.
+__global__ void __launch_bounds__(256) pair_scatter_kernel(LevelLists a, const int32_t* __restrict__ vlist) {
+  __shared__ int32_t h[343];
+  __shared__ int32_t base[343];
+  const int l = static_cast<int>(blockIdx.y) + 2;
+  const int64_t t0 = static_cast<int64_t>(blockIdx.x) * kPairCells;
+  if (t0 >= a.nt[l]) return;
+  for (int v = threadIdx.x; v < 343; v += 256) h[v] = 0;
+  __syncthreads();
+  int32_t vv[kPairIters], rank[kPairIters];
+#pragma unroll
+  for (int it = 0; it < kPairIters; it++) {
+    const int e = it * 256 + static_cast<int>(threadIdx.x);
+    const int64_t t = t0 + e / kM2LSlots;
+    const int k = e % kM2LSlots;
+    vv[it] = -2;  // outside the tile
+    rank[it] = 0;
+    if (e >= kPairCells * kM2LSlots || t >= a.nt[l]) continue;
+    vv[it] = -1;  // no source
+    if (a.src[l][t * kM2LSlots + k] < 0) continue;
+    const int oct = static_cast<int>(a.tcodes[l][t] & 7u);
+    vv[it] = vlist[oct * kM2LSlots + k];
+    rank[it] = atomicAdd(&h[vv[it]], 1);
+  }
+  __syncthreads();
+  for (int v = threadIdx.x; v < 343; v += 256)
+    base[v] = h[v] != 0 ? a.v_first[l][v] * kM2LCols + atomicAdd(&a.v_count[l][v], h[v]) : 0;
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < kPairIters; it++) {
+    if (vv[it] == -2) continue;
+    const int e = it * 256 + static_cast<int>(threadIdx.x);
+    const int64_t gid = t0 * kM2LSlots + e;
+    int32_t pos = -1;
+    if (vv[it] >= 0) {
+      pos = base[vv[it]] + rank[it];
+      a.blk_src[l][pos] = a.src[l][gid];
+    }
+    a.pair_pos[l][gid] = pos;
+  }
+}
 
 __global__ void m2l_list_kernel(LevelLists a, const int32_t* __restrict__ vlist) {
   const int l = static_cast<int>(blockIdx.y) + 2;
@@ -517,7 +595,8 @@ void build_tree(Tree& t, const double* d_pts, int64_t n, const Box& box, int lev
   finish_tree(t, level, level_counts(hist), ws, s);
 }
 
-void build_lists(const Tree& trg, const Tree& src, Lists& lists, TreeWorkspace& ws, hipStream_t s) {
+void build_lists(const Tree& trg, const Tree& src, Lists& lists, TreeWorkspace& ws, hipStream_t s,
+                 bool with_columns) {
   if (trg.level != src.level) throw_runtime("source and target trees must share the leaf level");
   int L = trg.level;
   int64_t nt = trg.n_leaves();
@@ -531,6 +610,7 @@ void build_lists(const Tree& trg, const Tree& src, Lists& lists, TreeWorkspace& 
   const int32_t* vl = octant_vlist_device(s);
   if (lists.m2l.size() < static_cast<size_t>(L) + 1) lists.m2l.resize(L + 1);
   lists.level = L;
+  lists.has_columns = with_columns;
   // Per level: the M2L slots, then the columns grouped by octant (stable, Morton
   // order inside a group), each group padded to a multiple of kM2LCols so a
   // column block has a single octant. All levels go in one launch of each
@@ -538,7 +618,7 @@ void build_lists(const Tree& trg, const Tree& src, Lists& lists, TreeWorkspace& 
   ws.layout.resize(static_cast<size_t>(L + 1) * kLayout);
   LevelLists a{};
   a.n_levels = L >= 2 ? L - 1 : 0;
-  int64_t max_work = 0;
+  int64_t max_work = 0, max_cells = 0;
   for (int l = 2; l <= L; l++) {
     const TreeLevel& tl = trg.levels[l];
     const TreeLevel& sl = src.levels[l];
@@ -563,27 +643,81 @@ void build_lists(const Tree& trg, const Tree& src, Lists& lists, TreeWorkspace& 
     a.blk_oct[l] = m.blk_oct.get();
     a.lay[l] = ws.layout.get() + static_cast<int64_t>(l) * kLayout;
     max_work = std::max(max_work, n * kM2LSlots);
+    max_cells = std::max(max_cells, n);
+  }
+  constexpr int kV = 343;
+  lists.v_count.resize(static_cast<size_t>(L + 1) * kV);
+  lists.v_first.resize(static_cast<size_t>(L + 1) * kV);
+  for (int l = 2; l <= L; l++) {
+    a.v_count[l] = lists.v_count.get() + static_cast<int64_t>(l) * kV;
+    a.v_first[l] = lists.v_first.get() + static_cast<int64_t>(l) * kV;
   }
   if (a.n_levels > 0) {
+    fill_i32_kernel<<<ceil_div(static_cast<int64_t>(L + 1) * kV, 256), 256, 0, s>>>(
+        lists.v_count.get(), static_cast<int64_t>(L + 1) * kV, 0);
+    PAM_LAUNCH_CHECK();
     if (max_work > 0) {
       m2l_list_kernel<<<dim3(ceil_div(max_work, 256), a.n_levels), 256, 0, s>>>(a, vl);
       PAM_LAUNCH_CHECK();
+      pair_count_kernel<<<dim3(ceil_div(max_cells, kPairCells), a.n_levels), 256, 0, s>>>(a, vl);
+      PAM_LAUNCH_CHECK();
     }
-    octant_columns_kernel<<<a.n_levels, kColThreads, 0, s>>>(a);
-    PAM_LAUNCH_CHECK();
+    if (with_columns) {
+      octant_columns_kernel<<<a.n_levels, kColThreads, 0, s>>>(a);
+      PAM_LAUNCH_CHECK();
+    }
   }
   std::vector<int64_t> lay(static_cast<size_t>(L + 1) * kLayout, 0);
-  PAM_HIP(hipMemcpyAsync(lay.data(), ws.layout.get(), lay.size() * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  std::vector<int32_t> vc(static_cast<size_t>(L + 1) * kV, 0);
+  if (with_columns)
+    PAM_HIP(hipMemcpyAsync(lay.data(), ws.layout.get(), lay.size() * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  if (a.n_levels > 0)
+    PAM_HIP(hipMemcpyAsync(vc.data(), lists.v_count.get(), vc.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
   PAM_HIP(hipStreamSynchronize(s));
   for (int l = 2; l <= L; l++) {
     M2LLevel& m = lists.m2l[l];
+    m.h_v_blocks.assign(kV, 0);
+    m.h_v_first.assign(kV, 0);
+    m.n_blk = 0;
     if (m.n_cells == 0) continue;
-    const int64_t* ly = lay.data() + static_cast<int64_t>(l) * kLayout;
-    m.n_cols = ly[16];
-    for (int k = 0; k < 8; k++) {
-      int64_t end = k < 7 ? ly[8 + k + 1] : ly[16];
-      for (int64_t c = ly[8 + k]; c < end; c += kM2LCols) m.h_blk_oct.push_back(k);
+    if (with_columns) {
+      const int64_t* ly = lay.data() + static_cast<int64_t>(l) * kLayout;
+      m.n_cols = ly[16];
+      for (int k = 0; k < 8; k++) {
+        int64_t end = k < 7 ? ly[8 + k + 1] : ly[16];
+        for (int64_t c = ly[8 + k]; c < end; c += kM2LCols) m.h_blk_oct.push_back(k);
+      }
     }
+    int64_t nb = 0;
+    for (int v = 0; v < kV; v++) {
+      const int32_t c = vc[static_cast<size_t>(l) * kV + v];
+      m.h_v_first[v] = static_cast<int32_t>(nb);
+      m.h_v_blocks[v] = (c + kM2LCols - 1) / kM2LCols;
+      nb += m.h_v_blocks[v];
+    }
+    m.n_blk = nb;
+    m.blk_src.resize(static_cast<size_t>(std::max<int64_t>(nb, 1)) * kM2LCols);
+    m.pair_pos.resize(static_cast<size_t>(m.n_cells) * kM2LSlots);
+    a.blk_src[l] = m.blk_src.get();
+    a.pair_pos[l] = m.pair_pos.get();
+  }
+  if (a.n_levels > 0 && max_work > 0) {
+    int32_t* h = lists.v_stage.acquire(static_cast<size_t>(L + 1) * kV);
+    for (int l = 0; l <= L; l++)
+      for (int v = 0; v < kV; v++)
+        h[static_cast<size_t>(l) * kV + v] = l >= 2 ? lists.m2l[l].h_v_first[v] : 0;
+    lists.v_stage.upload(lists.v_first, static_cast<size_t>(L + 1) * kV, s);
+    fill_i32_kernel<<<ceil_div(static_cast<int64_t>(L + 1) * kV, 256), 256, 0, s>>>(
+        lists.v_count.get(), static_cast<int64_t>(L + 1) * kV, 0);
+    PAM_LAUNCH_CHECK();
+    for (int l = 2; l <= L; l++) {
+      const M2LLevel& m = lists.m2l[l];
+      if (m.n_blk == 0) continue;
+      fill_i32_kernel<<<ceil_div(m.n_blk * kM2LCols, 256), 256, 0, s>>>(m.blk_src.get(), m.n_blk * kM2LCols, -1);
+      PAM_LAUNCH_CHECK();
+    }
+    pair_scatter_kernel<<<dim3(ceil_div(max_cells, kPairCells), a.n_levels), 256, 0, s>>>(a, vl);
+    PAM_LAUNCH_CHECK();
   }
 }
 

@@ -61,12 +61,25 @@ struct M2LLevel {
   DevBuf<int32_t> blk_oct;        // [n_cols/kM2LCols] octant of each column block
   std::vector<int32_t> h_blk_oct; // host copy of blk_oct
   DevBuf<int32_t> src;            // [n_trg_cells][189] source cell or -1 (octant slot order)
+  // Pair-major form: the (target cell, slot) pairs grouped by transfer vector
+  // v (0..342), each v's group padded to whole blocks of kM2LCols pairs. The
+  // order of the pairs inside a group is immaterial: every pair's transfer is
+  // computed on its own and the per-target sum runs in slot order.
+  int64_t n_blk = 0;                // blocks of all groups
+  std::vector<int32_t> h_v_blocks;  // [343] blocks of each v
+  std::vector<int32_t> h_v_first;   // [343] first block of each v
+  DevBuf<int32_t> blk_src;          // [n_blk * kM2LCols] source cell of each pair, -1 for padding
+  DevBuf<int32_t> pair_pos;         // [n_trg_cells][189] pair index of (cell, slot), -1 if none
 };
 
 struct Lists {
   int level = -1;                 // leaf level the lists were built for (-1: none)
   DevBuf<int32_t> p2p;            // [n_trg_leaves][27]
   std::vector<M2LLevel> m2l;      // index = level (entries < 2 unused; may hold more levels than used)
+  DevBuf<int32_t> v_count;        // [level + 1][343] pairs per transfer vector (then fill counters)
+  DevBuf<int32_t> v_first;        // [level + 1][343] first block of each transfer vector
+  HostStage<int32_t> v_stage;     // pinned staging of v_first
+  bool has_columns = false;       // octant column layouts (col_cell, blk_oct, h_blk_oct) built
 };
 
 constexpr int kM2LSlots = 189;
@@ -94,6 +107,9 @@ void finish_tree(Tree& t, int level, const std::vector<int64_t>& counts, TreeWor
 // sort_tree_points + finish_tree at one level (one host round trip).
 void build_tree(Tree& t, const double* d_pts, int64_t n, const Box& box, int level,
                 TreeWorkspace& ws, hipStream_t s);
-void build_lists(const Tree& trg, const Tree& src, Lists& lists, TreeWorkspace& ws, hipStream_t s);
+// P2P and M2L lists (pair-major M2L groups included); with_columns also
+// builds the octant column layout that the compressed M2L path uses.
+void build_lists(const Tree& trg, const Tree& src, Lists& lists, TreeWorkspace& ws, hipStream_t s,
+                 bool with_columns);
 
 }  // namespace pam
