@@ -673,12 +673,18 @@ struct M2LSmem {
 // vectors act_v[ia] of A_v[rows][0:kpad] x in[src_tab[ia][col]][0:kpad]
 // (columns with src_tab -1 contribute zero). Chunks of kK reduction indices
 // are multiplied on MFMA in the operand type, FP32 partial sums being folded
-// into FP64 every 64 indices. Ends with a barrier.
-template <class TM>
+// into FP64 every 64 indices. With kEmit, the sum of each ia is handed to
+// emit(ia, dacc) as soon as it is complete and dacc restarts from zero.
+// Ends with a barrier.
+struct NoEmit {
+  __device__ void operator()(int, const double (&)[4][2][4]) const {}
+};
+
+template <class TM, bool kEmit = false, class Emit = NoEmit>
 __device__ __forceinline__ void m2l_mainloop(M2LSmem<TM>& sm, const int (*src_tab)[kM2LC],
                                              const int* act_v, int nact, const TM* __restrict__ a,
                                              int64_t per_v, const TM* __restrict__ in, int kpad, int rpad,
-                                             int rb, double (&dacc)[4][2][4]) {
+                                             int rb, double (&dacc)[4][2][4], Emit emit = Emit{}) {
   using Tl = M2LTile<TM>;
   using acc_t = typename Mfma<TM>::acc_t;
   constexpr int kK = Tl::kK, kKB = Tl::kKB, kUpr = Tl::kUpr;
@@ -706,7 +712,11 @@ __device__ __forceinline__ void m2l_mainloop(M2LSmem<TM>& sm, const int (*src_ta
   };
   RegSet r0, r1;
 
-  auto load_chunk = [&](int t, RegSet& rs) {
+  // Chunk t (clamped to the last one past the end, where its multipole
+  // columns are masked to zero so the extra step adds nothing).
+  auto load_chunk = [&](int t_req, RegSet& rs) {
+    const int t = min(t_req, total - 1);
+    const uint32_t live = t_req < total ? ~0u : 0u;
     const int ia = t / nkc, kc = t - ia * nkc;
     const int k0 = kc * kK;
     const TM* av = a + static_cast<int64_t>(act_v[ia]) * per_v;
@@ -721,8 +731,8 @@ __device__ __forceinline__ void m2l_mainloop(M2LSmem<TM>& sm, const int (*src_ta
     }
     const int kel = k0 + bq * (16 / static_cast<int>(sizeof(TM)));
     const int s0 = src_tab[ia][bc0], s1 = src_tab[ia][bc1];
-    rs.m0 = s0 >= 0 ? ~0u : 0u;
-    rs.m1 = s1 >= 0 ? ~0u : 0u;
+    rs.m0 = s0 >= 0 ? live : 0u;
+    rs.m1 = s1 >= 0 ? live : 0u;
     rs.b[0] = *reinterpret_cast<const u32x4*>(in + static_cast<int64_t>(max(s0, 0)) * kpad + kel);
     rs.b[1] = *reinterpret_cast<const u32x4*>(in + static_cast<int64_t>(max(s1, 0)) * kpad + kel);
   };
@@ -788,6 +798,19 @@ __device__ __forceinline__ void m2l_mainloop(M2LSmem<TM>& sm, const int (*src_ta
     // FP32 partial sums over at most 64 reduction steps are flushed into FP64.
     if constexpr (sizeof(TM) == 4) {
       if ((kc & 1) == 1 || kc == nkc - 1) flush();
+    } else if constexpr (kEmit) {
+      if (kc == nkc - 1) flush();
+    }
+    if constexpr (kEmit) {
+      if (kc == nkc - 1) {
+        emit(t / nkc, dacc);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) dacc[i][j][q] = 0.0;
+      }
     }
   };
   // Step t: chunk t+1 sits in registers (cur), chunk t+2 is loaded into the
@@ -796,15 +819,15 @@ __device__ __forceinline__ void m2l_mainloop(M2LSmem<TM>& sm, const int (*src_ta
   // loaded and stored again into the buffer no later step reads, so no load
   // or store sits behind a branch.
   auto step = [&](int t, const RegSet& cur, RegSet& nxt) {
-    load_chunk(min(t + 2, total - 1), nxt);
-    if (t < total) compute(t);
+    load_chunk(t + 2, nxt);
+    compute(t);
     store_chunk((t + 1) & 1, cur);
     __syncthreads();
   };
   if (total > 0) {
     load_chunk(0, r1);
     store_chunk(0, r1);
-    load_chunk(min(1, total - 1), r0);
+    load_chunk(1, r0);
     __syncthreads();
     for (int t = 0; t < total; t += 2) {
       step(t, r0, r1);
@@ -891,16 +914,20 @@ m2l_dense_kernel(M2LDenseArgs<TM> b) {
   }
 }
 
-// Pair-major M2L: a workgroup takes one block of 64 (target cell, source
-// cell) pairs that share the transfer vector v and one row block, and writes
-// each pair's transfer A_v x in[src] (rows < r) to its own partial row
-// (stride kpad, operand type). The pairs of a block are independent columns,
-// so a pair's result does not depend on which block it sits in.
+// Pair-major M2L: a workgroup takes up to kPairBlocks consecutive blocks of
+// 64 (target cell, source cell) pairs that share the transfer vector v, for
+// one row block, and writes each pair's transfer A_v x in[src] to its own
+// partial row (stride rpad, operand type; rows >= r and padding pairs hold
+// zeros). The pairs of a block are independent columns, so a pair's result
+// does not depend on which block it sits in.
+constexpr int kPairBlocks = 4;
+
 template <class TM>
 struct M2LPairArgs {
   int r, rpad, kpad;
   int64_t per_v;
-  const int4* sched;  // (level, block, row block, v) per workgroup; level < 0: idle
+  // (level | (blocks << 5), first block, row block, v) per workgroup; x < 0: idle
+  const int4* sched;
   const TM* in[kMaxM2LLevels];
   const TM* a[kMaxM2LLevels];
   const int32_t* blk_src[kMaxM2LLevels];
@@ -911,40 +938,50 @@ template <class TM>
 __global__ void __launch_bounds__(kThreads)
 m2l_pair_kernel(M2LPairArgs<TM> b) {
   __shared__ __align__(16) M2LSmem<TM> sm;
-  __shared__ int src_tab[1][kM2LC];
-  __shared__ int act_v[1];
+  __shared__ int src_tab[kPairBlocks][kM2LC];
+  __shared__ int act_v[kPairBlocks];
   const int4 w = b.sched[blockIdx.x];
   if (w.x < 0) return;
-  const int lv = w.x, blk = w.y, rb = w.z;
+  const int lv = w.x & 31, nb = w.x >> 5, blk0 = w.y, rb = w.z;
   const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
-  if (tid < kM2LC) src_tab[0][tid] = b.blk_src[lv][static_cast<int64_t>(blk) * kM2LC + tid];
-  if (tid == 0) act_v[0] = w.w;
+  for (int idx = tid; idx < nb * kM2LC; idx += kThreads)
+    src_tab[idx / kM2LC][idx % kM2LC] = b.blk_src[lv][static_cast<int64_t>(blk0) * kM2LC + idx];
+  if (tid < nb) act_v[tid] = w.w;
   __syncthreads();
-  double dacc[4][2][4];
-  m2l_mainloop<TM>(sm, src_tab, act_v, 1, b.a[lv], b.per_v, b.in[lv], b.kpad, b.rpad, rb, dacc);
   const int wr = wave >> 1, wc = wave & 1;
+  const int rpad = b.rpad;
   TM* __restrict__ out = b.partial[lv];
+  // D layout (16 x 16), column lane % 16; register q holds row 4 (lane / 16) + q
+  // for the f32 instruction and row lane / 16 + 4 q for the f64 one.
+  auto emit = [&](int ia, const double (&d)[4][2][4]) {
 #pragma unroll
-  for (int j = 0; j < 2; j++) {
-    const int col = wc * 32 + j * 16 + (lane & 15);
-    if (src_tab[0][col] < 0) continue;
-    const int64_t pair = static_cast<int64_t>(blk) * kM2LC + col;
+    for (int j = 0; j < 2; j++) {
+      const int col = wc * 32 + j * 16 + (lane & 15);
+      TM* __restrict__ po = out + (static_cast<int64_t>(blk0 + ia) * kM2LC + col) * rpad;
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+      for (int i = 0; i < 4; i++) {
+        const int r0 = rb * kM2LR + wr * 64 + i * 16;
+        if constexpr (sizeof(TM) == 4) {
+          using f32x4 = float __attribute__((ext_vector_type(4)));
+          f32x4 v = {static_cast<float>(d[i][j][0]), static_cast<float>(d[i][j][1]),
+                     static_cast<float>(d[i][j][2]), static_cast<float>(d[i][j][3])};
+          *reinterpret_cast<f32x4*>(po + r0 + 4 * (lane >> 4)) = v;
+        } else {
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        int row = rb * kM2LR + wr * 64 + i * 16 + (sizeof(TM) == 4 ? 4 * (lane >> 4) + q : (lane >> 4) + 4 * q);
-        if (row >= b.r) continue;
-        out[pair * b.kpad + row] = static_cast<TM>(dacc[i][j][q]);
+          for (int q = 0; q < 4; q++) po[r0 + (lane >> 4) + 4 * q] = d[i][j][q];
+        }
       }
-  }
+    }
+  };
+  double dacc[4][2][4];
+  m2l_mainloop<TM, true>(sm, src_tab, act_v, nb, b.a[lv], b.per_v, b.in[lv], b.kpad, rpad, rb, dacc, emit);
 }
 
 // out[cell][row] = scale * sum over the cell's slots, in slot order, of the
 // partial row of its (cell, slot) pair. One workgroup per target cell.
 template <class TM>
 struct M2LPairReduceArgs {
-  int n, r, kpad;
+  int n, r, kpad, pstride;
   int64_t cell_off[kMaxM2LLevels + 1];
   const int32_t* pair_pos[kMaxM2LLevels];
   const TM* partial[kMaxM2LLevels];
@@ -978,7 +1015,7 @@ __global__ void __launch_bounds__(kPairReduceThreads) m2l_pair_reduce_kernel(M2L
 #pragma unroll
       for (int u = 0; u < kSl; u++) {
         const int32_t p = pos[sl + u];
-        const TM* pr = partial + static_cast<int64_t>(max(p, 0)) * b.kpad;
+        const TM* pr = partial + static_cast<int64_t>(max(p, 0)) * b.pstride;
 #pragma unroll
         for (int q = 0; q < kRows; q++) x[u][q] = p >= 0 ? pr[row[q]] : TM(0);
       }
@@ -1161,12 +1198,13 @@ namespace {
 void build_m2l_pair_schedule(const std::vector<const M2LLevel*>& lvls, int nrowblk,
                              M2LPairSchedule& sc, hipStream_t s) {
   constexpr int kXcd = 8;
+  auto items_of = [](int blocks) { return (blocks + kPairBlocks - 1) / kPairBlocks; };
   std::array<size_t, kXcd> len_x{};
   int g = 0;
   for (const M2LLevel* l : lvls)
     for (int v = 0; v < 343; v++) {
       if (l->h_v_blocks[v] == 0) continue;
-      for (int rb = 0; rb < nrowblk; rb++) len_x[g++ % kXcd] += l->h_v_blocks[v];
+      for (int rb = 0; rb < nrowblk; rb++) len_x[g++ % kXcd] += items_of(l->h_v_blocks[v]);
     }
   const size_t len = *std::max_element(len_x.begin(), len_x.end());
   int32_t* h = sc.h.acquire(len * kXcd * 4);
@@ -1176,12 +1214,13 @@ void build_m2l_pair_schedule(const std::vector<const M2LLevel*>& lvls, int nrowb
   for (size_t li = 0; li < lvls.size(); li++) {
     const M2LLevel& l = *lvls[li];
     for (int v = 0; v < 343; v++) {
-      if (l.h_v_blocks[v] == 0) continue;
+      const int nblk = l.h_v_blocks[v];
+      if (nblk == 0) continue;
       for (int rb = 0; rb < nrowblk; rb++) {
         const int x = g++ % kXcd;
-        for (int bk = 0; bk < l.h_v_blocks[v]; bk++) {
+        for (int bk = 0; bk < nblk; bk += kPairBlocks) {
           int32_t* e = h + (pos[x]++ * kXcd + x) * 4;
-          e[0] = static_cast<int32_t>(li);
+          e[0] = static_cast<int32_t>(li) | (std::min(kPairBlocks, nblk - bk) << 5);
           e[1] = l.h_v_first[v] + bk;
           e[2] = rb;
           e[3] = v;
@@ -1204,7 +1243,7 @@ void launch_m2l_pairs(const M2LOperator& op, const std::vector<M2LDenseLevel<TM>
   M2LPairArgs<TM> b{};
   M2LPairReduceArgs<TM> rb{};
   b.r = rb.r = op.r;
-  b.rpad = op.rpad;
+  b.rpad = rb.pstride = op.rpad;
   b.kpad = rb.kpad = op.kpad;
   b.per_v = static_cast<int64_t>(op.per_v);
   std::vector<const M2LLevel*> lvls;
@@ -1214,16 +1253,16 @@ void launch_m2l_pairs(const M2LOperator& op, const std::vector<M2LDenseLevel<TM>
     if (static_cast<int>(l.lv->h_v_blocks.size()) != 343) throw_runtime("M2L pair lists missing");
     pairs += l.lv->n_blk * kM2LC;
   }
-  partial.resize(static_cast<size_t>(std::max<int64_t>(pairs, 1)) * op.kpad);
+  partial.resize(static_cast<size_t>(std::max<int64_t>(pairs, 1)) * op.rpad);
   pairs = 0;
   for (const auto& l : levels) {
     if (!l.lv || l.lv->n_cells == 0) continue;
-    if (lvls.size() >= static_cast<size_t>(kMaxM2LLevels)) throw_runtime("too many M2L levels");
+    if (lvls.size() >= static_cast<size_t>(std::min(kMaxM2LLevels, 32))) throw_runtime("too many M2L levels");
     const int k = static_cast<int>(lvls.size());
     b.in[k] = l.in;
     b.a[k] = l.a;
     b.blk_src[k] = l.lv->blk_src.get();
-    b.partial[k] = partial.get() + pairs * op.kpad;
+    b.partial[k] = partial.get() + pairs * op.rpad;
     rb.pair_pos[k] = l.lv->pair_pos.get();
     rb.partial[k] = b.partial[k];
     rb.out[k] = l.out;
