@@ -993,36 +993,41 @@ constexpr int kPairReduceThreads = 128;
 
 template <class TM>
 __global__ void __launch_bounds__(kPairReduceThreads) m2l_pair_reduce_kernel(M2LPairReduceArgs<TM> b) {
-  __shared__ int32_t pos[kM2LSlots + 3];
+  __shared__ int32_t pos[kM2LSlots + 7];
   const int64_t g = blockIdx.x;
   int k = 0;
   while (k + 1 < b.n && b.cell_off[k + 1] <= g) k++;
   const int64_t cell = g - b.cell_off[k];
-  for (int sl = threadIdx.x; sl < kM2LSlots + 3; sl += kPairReduceThreads)
+  for (int sl = threadIdx.x; sl < kM2LSlots + 7; sl += kPairReduceThreads)
     pos[sl] = sl < kM2LSlots ? b.pair_pos[k][cell * kM2LSlots + sl] : -1;
   __syncthreads();
   const TM* __restrict__ partial = b.partial[k];
-  // Three rows per thread and four slots per trip keep twelve loads in
-  // flight; the additions still run in slot order.
-  constexpr int kRows = 3, kSl = 4;
+  // Three rows per thread and eight slots per trip keep 24 loads in flight;
+  // the additions still run in slot order.
+  constexpr int kRows = 3, kSl = 8;
   for (int row0 = 0; row0 < b.r; row0 += kRows * kPairReduceThreads) {
     double acc[kRows] = {0.0, 0.0, 0.0};
     int row[kRows];
 #pragma unroll
     for (int q = 0; q < kRows; q++) row[q] = min(row0 + q * kPairReduceThreads + static_cast<int>(threadIdx.x), b.r - 1);
     for (int sl = 0; sl < kM2LSlots; sl += kSl) {
-      TM x[kSl][kRows];
+      // Loads are unconditional (slots without a pair read row 0) and masked
+      // afterwards: loads behind branches would each be waited for on their own.
+      using UT = std::conditional_t<sizeof(TM) == 4, uint32_t, uint64_t>;
+      UT x[kSl][kRows], m[kSl];
 #pragma unroll
       for (int u = 0; u < kSl; u++) {
         const int32_t p = pos[sl + u];
-        const TM* pr = partial + static_cast<int64_t>(max(p, 0)) * b.pstride;
+        m[u] = p >= 0 ? ~UT(0) : UT(0);
+        const UT* pr = reinterpret_cast<const UT*>(partial) + static_cast<int64_t>(max(p, 0)) * b.pstride;
 #pragma unroll
-        for (int q = 0; q < kRows; q++) x[u][q] = p >= 0 ? pr[row[q]] : TM(0);
+        for (int q = 0; q < kRows; q++) x[u][q] = pr[row[q]];
       }
 #pragma unroll
       for (int u = 0; u < kSl; u++)
 #pragma unroll
-        for (int q = 0; q < kRows; q++) acc[q] += static_cast<double>(x[u][q]);
+        for (int q = 0; q < kRows; q++)
+          acc[q] += static_cast<double>(__builtin_bit_cast(TM, static_cast<UT>(x[u][q] & m[u])));
     }
 #pragma unroll
     for (int q = 0; q < kRows; q++) {
