@@ -265,35 +265,26 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 }
 
 // One workgroup solves one domain (fine_grid.hpp:97-142, coarse_grid.hpp:105-131):
-// gather d, Q^T d, L L^T gamma = Q^T d, lambda = Q gamma, scatter owned entries;
-// the coarse grid also solves P_top c = d_top - A_top lambda.
-__global__ void __launch_bounds__(kT)
-ras_solve_kernel(RbfSet r, int degree, int l, const double* __restrict__ pts,
-                 const int64_t* __restrict__ ptr, const int32_t* __restrict__ idx,
-                 const uint8_t* __restrict__ inner, const double* __restrict__ lag,
-                 const int64_t* __restrict__ mat_ptr, const double* __restrict__ mat,
-                 const double* __restrict__ res, double* __restrict__ out, int64_t mu, int coarse,
-                 const double* __restrict__ pinv) {
-  extern __shared__ double sm[];
-  __shared__ double sh[kT];
+// Q^T d, L L^T gamma = Q^T d, lambda = Q gamma; the coarse grid also solves
+// P_top c = d_top - A_top lambda. vals (LDS, m entries) holds d on entry;
+// emit(u, v) receives entry u < m of the solution (domain order) and
+// emit(m + a, c_a) the polynomial coefficients of the coarse grid.
+template <class Emit>
+__device__ void solve_domain(RbfSet r, int l, const double* __restrict__ pts, const int32_t* __restrict__ didx,
+                             int m, const double* __restrict__ lag, const double* __restrict__ L,
+                             double* vals, double* y, double* sh, int coarse, const double* __restrict__ pinv,
+                             Emit emit) {
   __shared__ double lam_top[10];
-  const int d = blockIdx.x;
-  const int64_t p0 = ptr[d];
-  const int m = static_cast<int>(ptr[d + 1] - p0);
+  __shared__ double rhs_top[10];
   const int n = m - l;
-  double* vals = sm;      // [m]
-  double* y = sm + m;     // [n]
-  for (int u = threadIdx.x; u < m; u += kT) vals[u] = res[idx[p0 + u]];
-  __syncthreads();
   if (n > 0) {
     for (int i = threadIdx.x; i < n; i += kT) {
       double v = vals[l + i];
-      int qi = idx[p0 + l + i];
+      int qi = didx[l + i];
       for (int a = 0; a < l; a++) v -= lag[static_cast<int64_t>(qi) * l + a] * vals[a];
       y[i] = v;
     }
     __syncthreads();
-    const double* L = mat + mat_ptr[d];
     const int lda = n;
     // forward: L y = qtd, 32-row blocks
     for (int k0 = 0; k0 < n; k0 += kNb) {
@@ -343,26 +334,20 @@ ras_solve_kernel(RbfSet r, int degree, int l, const double* __restrict__ pts,
   // lambda_top[a] = sum_i Q[a][i] gamma_i = -sum_i lag[q_i][a] gamma_i
   for (int a = 0; a < l; a++) {
     double acc = 0.0;
-    for (int i = threadIdx.x; i < n; i += kT) acc -= lag[static_cast<int64_t>(idx[p0 + l + i]) * l + a] * y[i];
+    for (int i = threadIdx.x; i < n; i += kT) acc -= lag[static_cast<int64_t>(didx[l + i]) * l + a] * y[i];
     double t = block_sum(acc, sh);
     if (threadIdx.x == 0) lam_top[a] = t;
   }
   __syncthreads();
-  for (int u = threadIdx.x; u < m; u += kT) {
-    if (coarse || inner[p0 + u]) {
-      double v = u < l ? lam_top[u] : y[u - l];
-      out[idx[p0 + u]] = v;
-    }
-  }
+  for (int u = threadIdx.x; u < m; u += kT) emit(u, u < l ? lam_top[u] : y[u - l]);
   if (coarse && l > 0) {
     // polynomial part: P_top c = d_top - A_top lambda
-    __shared__ double rhs_top[10];
     for (int a = 0; a < l; a++) {
       double acc = 0.0;
-      int pa = idx[p0 + a];
+      int pa = didx[a];
       for (int u = threadIdx.x; u < m; u += kT) {
         double lu = u < l ? lam_top[u] : y[u - l];
-        acc += kernel_entry(r, pts, pa, idx[p0 + u]) * lu;
+        acc += kernel_entry(r, pts, pa, didx[u]) * lu;
       }
       double t = block_sum(acc, sh);
       if (threadIdx.x == 0) rhs_top[a] = vals[a] - t;
@@ -371,8 +356,81 @@ ras_solve_kernel(RbfSet r, int degree, int l, const double* __restrict__ pts,
     if (threadIdx.x < static_cast<unsigned>(l)) {
       double c = 0.0;
       for (int b = 0; b < l; b++) c += pinv[threadIdx.x * l + b] * rhs_top[b];
-      out[mu + threadIdx.x] = c;
+      emit(m + static_cast<int>(threadIdx.x), c);
     }
+  }
+}
+
+// Domain solves of one batch: gather d from res, scatter the owned entries
+// of the solution (every entry for the coarse grid) and the coarse grid's
+// polynomial coefficients into out.
+__global__ void __launch_bounds__(kT)
+ras_solve_kernel(RbfSet r, int l, const double* __restrict__ pts,
+                 const int64_t* __restrict__ ptr, const int32_t* __restrict__ idx,
+                 const uint8_t* __restrict__ inner, const double* __restrict__ lag,
+                 const int64_t* __restrict__ mat_ptr, const double* __restrict__ mat,
+                 const double* __restrict__ res, double* __restrict__ out, int64_t mu, int coarse,
+                 const double* __restrict__ pinv) {
+  extern __shared__ double sm[];
+  __shared__ double sh[kT];
+  const int d = blockIdx.x;
+  const int64_t p0 = ptr[d];
+  const int m = static_cast<int>(ptr[d + 1] - p0);
+  double* vals = sm;      // [m]
+  double* y = sm + m;     // [m - l]
+  for (int u = threadIdx.x; u < m; u += kT) vals[u] = res[idx[p0 + u]];
+  __syncthreads();
+  const int32_t* didx = idx + p0;
+  const uint8_t* din = inner + p0;
+  solve_domain(r, l, pts, didx, m, lag, mat + mat_ptr[d], vals, y, sh, coarse, pinv, [&](int u, double v) {
+    if (u < m) {
+      if (coarse || din[u]) out[didx[u]] = v;
+    } else {
+      out[mu + (u - m)] = v;
+    }
+  });
+}
+
+// The coarse grid's solve as a dense operator: workgroup u0 solves for the
+// unit vector e_u0 and writes column u0 of op ((m + l) x m, row-major).
+__global__ void __launch_bounds__(kT)
+coarse_operator_kernel(RbfSet r, int l, const double* __restrict__ pts, const int32_t* __restrict__ idx, int m,
+                       const double* __restrict__ lag, const double* __restrict__ mat,
+                       const double* __restrict__ pinv, double* __restrict__ op) {
+  extern __shared__ double sm[];
+  __shared__ double sh[kT];
+  const int u0 = blockIdx.x;
+  double* vals = sm;
+  double* y = sm + m;
+  for (int u = threadIdx.x; u < m; u += kT) vals[u] = u == u0 ? 1.0 : 0.0;
+  __syncthreads();
+  solve_domain(r, l, pts, idx, m, lag, mat, vals, y, sh, 1, pinv,
+               [&](int u, double v) { op[static_cast<int64_t>(u) * m + u0] = v; });
+}
+
+// Coarse solve by the dense operator: out[idx[u]] = (op d)[u] for u < m and
+// out[mu + a] = (op d)[m + a], d = res[idx[.]]. One wavefront per row, the
+// row's products summed in a fixed order.
+constexpr int kCoarseRowsPerBlock = kT / 64;
+constexpr int kCoarseOpMax = 8192;  // largest coarse grid given a dense operator (512 MiB)
+
+__global__ void __launch_bounds__(kT)
+coarse_apply_kernel(const int32_t* __restrict__ idx, int m, int l, const double* __restrict__ op,
+                    const double* __restrict__ res, double* __restrict__ out, int64_t mu) {
+  extern __shared__ double x[];
+  for (int u = threadIdx.x; u < m; u += kT) x[u] = res[idx[u]];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * kCoarseRowsPerBlock + static_cast<int>(threadIdx.x >> 6);
+  if (row >= m + l) return;
+  const double* __restrict__ orow = op + static_cast<int64_t>(row) * m;
+  double acc = 0.0;
+  for (int u = lane; u < m; u += 64) acc += orow[u] * x[u];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if (lane == 0) {
+    if (row < m) out[idx[row]] = acc;
+    else out[mu + (row - m)] = acc;
   }
 }
 
@@ -857,6 +915,19 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
     cd.inner.assign(cd.idx.size(), 1);
     h_domains_[0] = {cd};
     setup_batch(coarse_, h_domains_[0], true);
+    // The coarse solve is one domain: solved by a single workgroup it would
+    // serialize on one CU, so it is turned into a dense operator once (one
+    // workgroup per unit vector) and applied as a matrix-vector product.
+    const int m = static_cast<int>(coarse_.h_ptr[1] - coarse_.h_ptr[0]);
+    const size_t shmem = sizeof(double) * 2 * static_cast<size_t>(m);
+    if (coarse_.n_dom == 1 && m > 0 && m <= kCoarseOpMax && shmem <= 96 * 1024) {
+      coarse_.op.resize(static_cast<size_t>(m + l_) * m);
+      RbfSet rs = make_rbf_set(model_);
+      coarse_operator_kernel<<<m, kT, shmem, s_>>>(rs, l_, d_pts_.get(), coarse_.idx.get(), m, lag_.get(),
+                                                   coarse_.mat.get(), pinv_.get(), coarse_.op.get());
+      PAM_LAUNCH_CHECK();
+      coarse_.has_op = true;
+    }
   }
   int32_t flag = 0;
   PAM_HIP(hipMemcpyAsync(&flag, flag_.get(), sizeof flag, hipMemcpyDeviceToHost, s_));
@@ -938,12 +1009,19 @@ void RasPreconditioner::solve(int level, const double* d_res, double* d_w) {
   dev_fill(0.0, d_w, mu_ + l_, s_);
   DomainBatch& b = level == 0 ? coarse_ : fine_[level];
   if (b.n_dom == 0) return;
+  if (level == 0 && b.has_op) {
+    const int m = static_cast<int>(b.h_ptr[1] - b.h_ptr[0]);
+    coarse_apply_kernel<<<ceil_div(m + l_, kCoarseRowsPerBlock), kT, sizeof(double) * m, s_>>>(
+        b.idx.get(), m, l_, b.op.get(), d_res, d_w, mu_);
+    PAM_LAUNCH_CHECK();
+    return;
+  }
   size_t shmem = sizeof(double) * 2 * static_cast<size_t>(b.max_m);
   if (shmem > 96 * 1024) throw_runtime("subdomain too large for the solve kernel");
   RbfSet rs = make_rbf_set(model_);
-  ras_solve_kernel<<<b.n_dom, kT, shmem, s_>>>(rs, model_.poly_degree, l_, d_pts_.get(), b.ptr.get(), b.idx.get(),
-                                                b.inner.get(), lag_.get(), b.mat_ptr.get(), b.mat.get(), d_res, d_w,
-                                                mu_, level == 0 ? 1 : 0, pinv_.get());
+  ras_solve_kernel<<<b.n_dom, kT, shmem, s_>>>(rs, l_, d_pts_.get(), b.ptr.get(), b.idx.get(), b.inner.get(),
+                                                lag_.get(), b.mat_ptr.get(), b.mat.get(), d_res, d_w, mu_,
+                                                level == 0 ? 1 : 0, pinv_.get());
   PAM_LAUNCH_CHECK();
 }
 
@@ -1024,6 +1102,40 @@ void RasPreconditioner::orthogonalize(double* d_w, double* d_res) {
 
 void RasPreconditioner::apply(const double* d_v, double* d_out) {
   // ras_preconditioner.hpp:191-250
+  // Verbose: the phases of the first two applies are timed (synchronizing
+  // after each) and printed.
+  const bool timed = opt_.verbose && verbose_applies_ < 2;
+  std::string report;
+  auto t_last = std::chrono::steady_clock::now();
+  auto phase = [&](const std::string& what) {
+    if (!timed) return;
+    PAM_HIP(hipStreamSynchronize(s_));
+    auto now = std::chrono::steady_clock::now();
+    char buf[128];
+    std::snprintf(buf, sizeof buf, " %s %.2f", what.c_str(), std::chrono::duration<double, std::milli>(now - t_last).count());
+    report += buf;
+    t_last = now;
+  };
+  struct Report {
+    const bool& timed;
+    std::string& text;
+    int& count;
+    ~Report() {
+      if (timed) {
+        std::printf("  ras apply (ms):%s\n", text.c_str());
+        std::fflush(stdout);
+        count++;
+      }
+    }
+  } rep{timed, report, verbose_applies_};
+  auto solve_t = [&](int level, const double* r, double* w) {
+    solve(level, r, w);
+    phase("solve" + std::to_string(level));
+  };
+  auto update_t = [&](int src, int trg, const double* w, double* r) {
+    update_residuals(src, trg, w, r);
+    phase("upd" + std::to_string(src) + ">" + std::to_string(trg));
+  };
   dev_copy(d_v, res_.get(), mu_, s_);
   if (n_levels_ == 1) {
     solve(0, res_.get(), d_out);
@@ -1032,25 +1144,25 @@ void RasPreconditioner::apply(const double* d_v, double* d_out) {
   const int n = n_levels_;
   double* tot = total_.get();
   dev_fill(0.0, tot, mu_ + l_, s_);
-  solve(0, res_.get(), w_.get());
-  update_residuals(0, n - 1, w_.get(), res_.get());
+  solve_t(0, res_.get(), w_.get());
+  update_t(0, n - 1, w_.get(), res_.get());
   dev_axpy(1.0, w_.get(), tot, mu_ + l_, s_);
   for (int level = 1; level < n - 1; level++) {
-    solve(level, res_.get(), w_.get());
-    update_residuals(level, n - 1, w_.get(), res_.get());
+    solve_t(level, res_.get(), w_.get());
+    update_t(level, n - 1, w_.get(), res_.get());
     dev_axpy(1.0, w_.get(), tot, mu_ + l_, s_);
     orthogonalize(tot, res_.get());
-    solve(0, res_.get(), w_.get());
-    update_residuals(0, n - 1, w_.get(), res_.get());
+    solve_t(0, res_.get(), w_.get());
+    update_t(0, n - 1, w_.get(), res_.get());
     dev_axpy(1.0, w_.get(), tot, mu_ + l_, s_);
   }
   for (int level = n - 1; level >= 1; level--) {
-    solve(level, res_.get(), w_.get());
-    update_residuals(level, level - 1, w_.get(), res_.get());
+    solve_t(level, res_.get(), w_.get());
+    update_t(level, level - 1, w_.get(), res_.get());
     dev_axpy(1.0, w_.get(), tot, mu_ + l_, s_);
     orthogonalize(tot, res_.get());
-    solve(0, res_.get(), w_.get());
-    if (level > 1) update_residuals(0, level - 1, w_.get(), res_.get());
+    solve_t(0, res_.get(), w_.get());
+    if (level > 1) update_t(0, level - 1, w_.get(), res_.get());
     dev_axpy(1.0, w_.get(), tot, mu_ + l_, s_);
   }
   dev_copy(tot, d_out, mu_ + l_, s_);

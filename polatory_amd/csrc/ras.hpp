@@ -44,6 +44,8 @@ struct DomainBatch {
   DevBuf<uint8_t> inner;
   DevBuf<int64_t> ptr, mat_ptr;          // [n_dom+1]
   DevBuf<double> mat;                    // lower Cholesky factors, column-major
+  DevBuf<double> op;                     // coarse grid: its solve as a dense (m + l) x m operator
+  bool has_op = false;
 };
 
 struct RasOptions {
@@ -104,6 +106,7 @@ class RasPreconditioner : public LinearOperator {
   DevBuf<int32_t> flag_;
   Workspace1 ws_;
   double setup_s_ = 0.0;
+  int verbose_applies_ = 0;  // applies whose phases have been timed and printed
 };
 
 }  // namespace pam
