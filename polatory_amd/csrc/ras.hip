@@ -286,47 +286,84 @@ __device__ void solve_domain(RbfSet r, int l, const double* __restrict__ pts, co
     }
     __syncthreads();
     const int lda = n;
+    // Diagonal blocks are staged in LDS before their sequential solve, and
+    // full panels read their 32 factor entries per row with independent
+    // loads: both keep the sequential parts off global-memory latency.
+    __shared__ double dgl[kNb][kNb + 1];
+    auto stage_diag = [&](int k0, int kb) {
+      for (int e = threadIdx.x; e < kb * kb; e += kT) {
+        const int rr = e % kb, cc = e / kb;
+        dgl[rr][cc] = L[static_cast<int64_t>(k0 + cc) * lda + k0 + rr];
+      }
+      __syncthreads();
+    };
     // forward: L y = qtd, 32-row blocks
     for (int k0 = 0; k0 < n; k0 += kNb) {
       int kb = min(kNb, n - k0);
+      stage_diag(k0, kb);
       if (threadIdx.x < 64) {  // one wavefront solves the diagonal block
         for (int t = 0; t < kb; t++) {
-          if (static_cast<int>(threadIdx.x) == t) y[k0 + t] /= L[static_cast<int64_t>(k0 + t) * lda + k0 + t];
+          if (static_cast<int>(threadIdx.x) == t) y[k0 + t] /= dgl[t][t];
           __builtin_amdgcn_wave_barrier();
           __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
           int c = threadIdx.x;
-          if (c > t && c < kb) y[k0 + c] -= L[static_cast<int64_t>(k0 + t) * lda + k0 + c] * y[k0 + t];
+          if (c > t && c < kb) y[k0 + c] -= dgl[c][t] * y[k0 + t];
           __builtin_amdgcn_wave_barrier();
           __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         }
       }
       __syncthreads();
-      for (int i = k0 + kb + threadIdx.x; i < n; i += kT) {
-        double v = y[i];
-        for (int t = 0; t < kb; t++) v -= L[static_cast<int64_t>(k0 + t) * lda + i] * y[k0 + t];
-        y[i] = v;
+      if (kb == kNb) {
+        for (int i = k0 + kNb + threadIdx.x; i < n; i += kT) {
+          double lv[kNb];
+#pragma unroll
+          for (int t = 0; t < kNb; t++) lv[t] = L[static_cast<int64_t>(k0 + t) * lda + i];
+          double v = y[i];
+#pragma unroll
+          for (int t = 0; t < kNb; t++) v -= lv[t] * y[k0 + t];
+          y[i] = v;
+        }
+      } else {
+        for (int i = k0 + kb + threadIdx.x; i < n; i += kT) {
+          double v = y[i];
+          for (int t = 0; t < kb; t++) v -= L[static_cast<int64_t>(k0 + t) * lda + i] * y[k0 + t];
+          y[i] = v;
+        }
       }
       __syncthreads();
     }
     // backward: L^T x = y
     for (int k0 = ((n - 1) / kNb) * kNb; k0 >= 0; k0 -= kNb) {
       int kb = min(kNb, n - k0);
+      stage_diag(k0, kb);
       if (threadIdx.x < 64) {
         for (int t = kb - 1; t >= 0; t--) {
-          if (static_cast<int>(threadIdx.x) == t) y[k0 + t] /= L[static_cast<int64_t>(k0 + t) * lda + k0 + t];
+          if (static_cast<int>(threadIdx.x) == t) y[k0 + t] /= dgl[t][t];
           __builtin_amdgcn_wave_barrier();
           __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
           int c = threadIdx.x;
-          if (c < t) y[k0 + c] -= L[static_cast<int64_t>(k0 + c) * lda + k0 + t] * y[k0 + t];
+          if (c < t) y[k0 + c] -= dgl[t][c] * y[k0 + t];
           __builtin_amdgcn_wave_barrier();
           __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         }
       }
       __syncthreads();
-      for (int i = threadIdx.x; i < k0; i += kT) {
-        double v = y[i];
-        for (int t = 0; t < kb; t++) v -= L[static_cast<int64_t>(i) * lda + k0 + t] * y[k0 + t];
-        y[i] = v;
+      if (kb == kNb) {
+        for (int i = threadIdx.x; i < k0; i += kT) {
+          double lv[kNb];
+#pragma unroll
+          for (int t = 0; t < kNb; t++) lv[t] = L[static_cast<int64_t>(i) * lda + k0 + t];
+          double v = y[i];
+#pragma unroll
+          for (int t = 0; t < kNb; t++) v -= lv[t] * y[k0 + t];
+          y[i] = v;
+        }
+      } else {
+        for (int i = threadIdx.x; i < k0; i += kT) {
+          double v = y[i];
+          for (int t = 0; t < kb; t++) v -= L[static_cast<int64_t>(i) * lda + k0 + t] * y[k0 + t];
+          y[i] = v;
+        }
       }
       __syncthreads();
     }
