@@ -922,7 +922,7 @@ m2l_dense_kernel(M2LDenseArgs<TM> b) {
 // does not depend on which block it sits in.
 constexpr int kPairBlocks = 4;
 
-template <class TM>
+template <class TM, class PT>
 struct M2LPairArgs {
   int r, rpad, kpad;
   int64_t per_v;
@@ -931,12 +931,12 @@ struct M2LPairArgs {
   const TM* in[kMaxM2LLevels];
   const TM* a[kMaxM2LLevels];
   const int32_t* blk_src[kMaxM2LLevels];
-  TM* partial[kMaxM2LLevels];
+  PT* partial[kMaxM2LLevels];
 };
 
-template <class TM>
+template <class TM, class PT>
 __global__ void __launch_bounds__(kThreads)
-m2l_pair_kernel(M2LPairArgs<TM> b) {
+m2l_pair_kernel(M2LPairArgs<TM, PT> b) {
   __shared__ __align__(16) M2LSmem<TM> sm;
   __shared__ int src_tab[kPairBlocks][kM2LC];
   __shared__ int act_v[kPairBlocks];
@@ -950,22 +950,26 @@ m2l_pair_kernel(M2LPairArgs<TM> b) {
   __syncthreads();
   const int wr = wave >> 1, wc = wave & 1;
   const int rpad = b.rpad;
-  TM* __restrict__ out = b.partial[lv];
+  PT* __restrict__ out = b.partial[lv];
   // D layout (16 x 16), column lane % 16; register q holds row 4 (lane / 16) + q
   // for the f32 instruction and row lane / 16 + 4 q for the f64 one.
   auto emit = [&](int ia, const double (&d)[4][2][4]) {
 #pragma unroll
     for (int j = 0; j < 2; j++) {
       const int col = wc * 32 + j * 16 + (lane & 15);
-      TM* __restrict__ po = out + (static_cast<int64_t>(blk0 + ia) * kM2LC + col) * rpad;
+      PT* __restrict__ po = out + (static_cast<int64_t>(blk0 + ia) * kM2LC + col) * rpad;
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const int r0 = rb * kM2LR + wr * 64 + i * 16;
-        if constexpr (sizeof(TM) == 4) {
+        if constexpr (sizeof(TM) == 4 && sizeof(PT) == 4) {
           using f32x4 = float __attribute__((ext_vector_type(4)));
           f32x4 v = {static_cast<float>(d[i][j][0]), static_cast<float>(d[i][j][1]),
                      static_cast<float>(d[i][j][2]), static_cast<float>(d[i][j][3])};
           *reinterpret_cast<f32x4*>(po + r0 + 4 * (lane >> 4)) = v;
+        } else if constexpr (sizeof(TM) == 4) {
+          using f64x2 = double __attribute__((ext_vector_type(2)));
+          *reinterpret_cast<f64x2*>(po + r0 + 4 * (lane >> 4)) = f64x2{d[i][j][0], d[i][j][1]};
+          *reinterpret_cast<f64x2*>(po + r0 + 4 * (lane >> 4) + 2) = f64x2{d[i][j][2], d[i][j][3]};
         } else {
 #pragma unroll
           for (int q = 0; q < 4; q++) po[r0 + (lane >> 4) + 4 * q] = d[i][j][q];
@@ -980,7 +984,7 @@ m2l_pair_kernel(M2LPairArgs<TM> b) {
 // out[cell][row] = scale * sum over the cell's slots, in slot order, of the
 // partial row of its (cell, slot) pair. One workgroup per target cell.
 template <class TM>
-struct M2LPairReduceArgs {
+struct M2LPairReduceArgs {  // TM: partial type
   int n, r, kpad, pstride;
   int64_t cell_off[kMaxM2LLevels + 1];
   const int32_t* pair_pos[kMaxM2LLevels];
@@ -1013,7 +1017,7 @@ __global__ void __launch_bounds__(kPairReduceThreads) m2l_pair_reduce_kernel(M2L
     for (int sl = 0; sl < kM2LSlots; sl += kSl) {
       // Loads are unconditional (slots without a pair read row 0) and masked
       // afterwards: loads behind branches would each be waited for on their own.
-      using UT = std::conditional_t<sizeof(TM) == 4, uint32_t, uint64_t>;
+      using UT = std::conditional_t<sizeof(TM) == 4, uint32_t, uint64_t>;  // TM: partial type
       UT x[kSl][kRows], m[kSl];
 #pragma unroll
       for (int u = 0; u < kSl; u++) {
@@ -1240,13 +1244,13 @@ void build_m2l_pair_schedule(const std::vector<const M2LLevel*>& lvls, int nrowb
 
 }  // namespace
 
-template <class TM>
+template <class TM, class PT>
 void launch_m2l_pairs(const M2LOperator& op, const std::vector<M2LDenseLevel<TM>>& levels,
-                      M2LPairSchedule& sched, DevBuf<TM>& partial, hipStream_t s) {
+                      M2LPairSchedule& sched, DevBuf<PT>& partial, hipStream_t s) {
   if (op.rpad % kM2LR != 0) throw_runtime("M2L operator rows are not padded to 128");
   if (op.kpad % M2LTile<TM>::kK != 0) throw_runtime("M2L reduction length is not a multiple of the chunk");
-  M2LPairArgs<TM> b{};
-  M2LPairReduceArgs<TM> rb{};
+  M2LPairArgs<TM, PT> b{};
+  M2LPairReduceArgs<PT> rb{};
   b.r = rb.r = op.r;
   b.rpad = rb.pstride = op.rpad;
   b.kpad = rb.kpad = op.kpad;
@@ -1283,17 +1287,17 @@ void launch_m2l_pairs(const M2LOperator& op, const std::vector<M2LDenseLevel<TM>
   if (!sched.valid) build_m2l_pair_schedule(lvls, op.rpad / kM2LR, sched, s);
   b.sched = reinterpret_cast<const int4*>(sched.d.get());
   if (sched.n_wg > 0) {
-    m2l_pair_kernel<TM><<<static_cast<unsigned>(sched.n_wg), kThreads, 0, s>>>(b);
+    m2l_pair_kernel<TM, PT><<<static_cast<unsigned>(sched.n_wg), kThreads, 0, s>>>(b);
     PAM_LAUNCH_CHECK();
   }
-  m2l_pair_reduce_kernel<TM><<<static_cast<unsigned>(cells), kPairReduceThreads, 0, s>>>(rb);
+  m2l_pair_reduce_kernel<PT><<<static_cast<unsigned>(cells), kPairReduceThreads, 0, s>>>(rb);
   PAM_LAUNCH_CHECK();
 }
 
-template void launch_m2l_pairs<float>(const M2LOperator&, const std::vector<M2LDenseLevel<float>>&,
-                                      M2LPairSchedule&, DevBuf<float>&, hipStream_t);
-template void launch_m2l_pairs<double>(const M2LOperator&, const std::vector<M2LDenseLevel<double>>&,
-                                       M2LPairSchedule&, DevBuf<double>&, hipStream_t);
+template void launch_m2l_pairs<float, float>(const M2LOperator&, const std::vector<M2LDenseLevel<float>>&,
+                                             M2LPairSchedule&, DevBuf<float>&, hipStream_t);
+template void launch_m2l_pairs<double, double>(const M2LOperator&, const std::vector<M2LDenseLevel<double>>&,
+                                               M2LPairSchedule&, DevBuf<double>&, hipStream_t);
 
 namespace {
 

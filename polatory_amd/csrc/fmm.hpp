@@ -134,8 +134,8 @@ void launch_m2l_dense(const M2LOperator& op, const std::vector<M2LDenseLevel<TM>
                       const int32_t* vlist, M2LSchedule& sched, DevBuf<double>& partial, hipStream_t s);
 
 // Pair-major dense M2L (same result as launch_m2l_dense, from the pair lists
-// of each level): per-pair transfers into `partial`, then per-target sums in
-// slot order.
+// of each level): per-pair transfers into `partial` (type PT), then
+// per-target sums in slot order in FP64.
 struct M2LPairSchedule {
   HostStage<int32_t> h;
   DevBuf<int32_t> d;
@@ -143,9 +143,9 @@ struct M2LPairSchedule {
   bool valid = false;
 };
 
-template <class TM>
+template <class TM, class PT>
 void launch_m2l_pairs(const M2LOperator& op, const std::vector<M2LDenseLevel<TM>>& levels,
-                      M2LPairSchedule& sched, DevBuf<TM>& partial, hipStream_t s);
+                      M2LPairSchedule& sched, DevBuf<PT>& partial, hipStream_t s);
 
 void launch_convert_f64_f32(const double* in, float* out, int64_t n, hipStream_t s);
 
