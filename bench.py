@@ -399,7 +399,7 @@ def main():
     # L2P: FP64 locals of every target leaf + FP64 tree-order targets read, FP64 far field written
     l2p_bytes = m["n_trg_leaves"] * p**3 * 8 + N_TRG * (32 + 8)
     stage_roofs = {
-        "m2l": {"kernel": "m2l_dense_kernel<float> + m2l_reduce_kernel", "bound": "mfma",
+        "m2l": {"kernel": "m2l_pair_kernel<float,float> + m2l_pair_reduce_kernel<float>", "bound": "mfma",
                 "achieved": m2l_ach / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
                 "frac": m2l_ach / PEAK_FP32, "flops": m2l_flops, "cell_pairs": m["m2l_pairs"]},
         "p2p": {"kernel": "p2p_kernel<float,0>", "bound": "valu (fp32)", "achieved": p2p_ach / 1e12,
@@ -411,7 +411,8 @@ def main():
     }
     dominant = "m2l" if stage["ms_m2l"] >= stage["ms_p2p"] else "p2p"
     roof = {k: stage_roofs[dominant][k] for k in ("kernel", "bound", "achieved", "peak", "unit", "frac")}
-    roof["traffic"] = hbm_traffic(stage_roofs[dominant]["kernel"].split(" ")[0])
+    parts = [hbm_traffic(k) for k in stage_roofs[dominant]["kernel"].split(" + ")]
+    roof["traffic"] = None if any(t is None for t in parts) else sum(parts)
     if dominant == "m2l":
         roof["dense_equivalent_tflops"] = m2l_dense_flops / (stage["ms_m2l"] * 1e-3) / 1e12
 
