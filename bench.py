@@ -373,11 +373,19 @@ def main():
         err = np.abs(y[sub] - ref).max() / ref_inf
         p2p_pairs, m2l_pairs = de.ev.count_work()
         st = stats[-1]
-        avg = {k: float(np.mean([s[k] for s in stats])) for k in
-               ("ms_tree", "ms_upward", "ms_m2l", "ms_l2l", "ms_l2p", "ms_p2p", "ms_total")}
+        keys = ("ms_tree", "ms_upward", "ms_m2l", "ms_l2l", "ms_l2p", "ms_p2p", "ms_total")
+        avg = {k: float(np.mean([s[k] for s in stats])) for k in keys}
+        # The same steps with the stages in sequence: each stage's kernels
+        # alone, for the rooflines (the timed steps above overlap the near
+        # field with the far field).
+        de.ev.set_overlap(False)
+        sec_serial, stats_serial = time_steps(torch, de, args.steps, 1)
+        de.ev.set_overlap(True)
+        avg_serial = {k: float(np.mean([s[k] for s in stats_serial])) for k in keys}
         results[name] = dict(sec=sec, err=err, p2p_pairs=p2p_pairs, m2l_pairs=m2l_pairs,
                              order=st["order"], leaf_level=st["leaf_level"], m2l_rank=st["m2l_rank"],
-                             n_trg_leaves=st["n_trg_leaves"], stage_ms=avg)
+                             n_trg_leaves=st["n_trg_leaves"], stage_ms=avg, sec_serial=sec_serial,
+                             stage_ms_serial=avg_serial)
         log(name, json.dumps(results[name]))
         del de
         torch.cuda.synchronize()
@@ -388,7 +396,7 @@ def main():
     f = results["fp64"]
     p = m["order"]
     r = m["m2l_rank"]
-    stage = m["stage_ms"]
+    stage = m["stage_ms_serial"]
     # M2L algorithmic flops: rank-r transfers (2 r^2 per cell pair) when compressed,
     # dense-equivalent 2 p^6 per cell pair reported beside it.
     m2l_dense_flops = 2.0 * p**6 * m["m2l_pairs"]
@@ -454,7 +462,9 @@ def main():
         "fp64_gpu": {"value": N_TRG / f["sec"], "ms_per_step": f["sec"] * 1e3, "rel_err": f["err"],
                      "order": f["order"]},
         "speedup_vs_fp64_gpu": f["sec"] / m["sec"],
-        "stage_ms": stage,
+        "stage_ms": m["stage_ms"],
+        "stage_ms_serial": stage,
+        "ms_per_step_serial": m["sec_serial"] * 1e3,
         "roofline": roof,
         "stage_rooflines": stage_roofs,
         "cpu_baseline": cpu,

@@ -103,8 +103,16 @@ typedef struct {
 } pam_fmm_config;
 int pam_fmm_evaluator_set_config(pam_fmm_evaluator* h, const pam_fmm_config* cfg);
 int pam_fmm_evaluator_get_config(pam_fmm_evaluator* h, pam_fmm_config* cfg);
+/* No counterpart in the reference (its evaluator runs the stages in
+ * sequence). overlap != 0 (the default): the near field runs on a second
+ * stream beside the far field, the result is the same bit for bit; 0: every
+ * stage runs in sequence on the handle's stream, so that the per-stage times
+ * of pam_fmm_evaluator_stats are those of each stage's kernels alone. */
+int pam_fmm_evaluator_set_overlap(pam_fmm_evaluator* h, int overlap);
 
-/* Per-stage device times of the last evaluate (HIP events on the handle's stream). */
+/* Per-stage device times of the last evaluate (HIP events on the handle's
+ * streams); ms_total is the whole evaluation, less than the sum of the stages
+ * when the near field overlaps the far field. */
 typedef struct {
   double ms_tree, ms_upward, ms_m2l, ms_l2l, ms_l2p, ms_p2p, ms_total;
   int precision, order, leaf_level, direct;

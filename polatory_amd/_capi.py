@@ -103,6 +103,7 @@ _SIGS = {
     "pam_fmm_evaluator_synchronize": [_P],
     "pam_fmm_evaluator_set_config": [_P, ctypes.POINTER(PamFmmConfig)],
     "pam_fmm_evaluator_get_config": [_P, ctypes.POINTER(PamFmmConfig)],
+    "pam_fmm_evaluator_set_overlap": [_P, _INT],
     "pam_fmm_evaluator_stats": [_P, ctypes.POINTER(PamFmmStats)],
     "pam_fmm_evaluator_count_work": [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64)],
     "pam_fmm_evaluator_tree_info": [_P, _INT, ctypes.POINTER(_INT), _P, ctypes.POINTER(_D),

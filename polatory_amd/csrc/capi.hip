@@ -172,6 +172,10 @@ int pam_fmm_evaluator_set_config(pam_fmm_evaluator* h, const pam_fmm_config* cfg
   });
 }
 
+int pam_fmm_evaluator_set_overlap(pam_fmm_evaluator* h, int overlap) {
+  return guard([&] { ev(h).set_overlap(overlap != 0); });
+}
+
 int pam_fmm_evaluator_get_config(pam_fmm_evaluator* h, pam_fmm_config* cfg) {
   return guard([&] {
     const auto& c = ev(h).active_config();

@@ -298,7 +298,7 @@ l2l_kernel(const int32_t* __restrict__ parent_of, const uint64_t* __restrict__ c
 template <class T, int P>
 __global__ void __launch_bounds__(kThreads)
 l2p_kernel(const int32_t* __restrict__ leaf_start, const V4<T>* __restrict__ pts,
-           const T* __restrict__ l, int kpad, T* __restrict__ y_far) {
+           const T* __restrict__ l, int kpad, T* __restrict__ y_far, const int32_t* __restrict__ perm) {
   constexpr int P2 = P * P, P3 = P * P * P;
   __shared__ T tk[P2];
   __shared__ T ls[P3];
@@ -337,7 +337,10 @@ l2p_kernel(const int32_t* __restrict__ leaf_start, const V4<T>* __restrict__ pts
       }
       y += ya * sxs[a][c];
     }
-    y_far[i] = y;
+    // perm: add the far field into the output in input order (the near field
+    // is already there); otherwise store it in tree order.
+    if (perm) y_far[perm[i]] += y;
+    else y_far[i] = y;
   }
 }
 
@@ -1405,12 +1408,13 @@ void launch_l2l(int p, const Tree& t, int l, const T* l_parent, T* l_child, hipS
 }
 
 template <class T>
-void launch_l2p(int p, const Tree& t, const T* l_leaf, const V4<T>* pts, T* y_far, hipStream_t s) {
+void launch_l2p(int p, const Tree& t, const T* l_leaf, const V4<T>* pts, T* y_far, hipStream_t s,
+                bool add_to_output) {
   int64_t nl = t.n_leaves();
   if (nl == 0) return;
   dispatch_order(p, [&](auto P) {
     l2p_kernel<T, decltype(P)::value><<<static_cast<unsigned>(nl), kThreads, 0, s>>>(
-        t.leaf_start.get(), pts, l_leaf, kpad_of(P), y_far);
+        t.leaf_start.get(), pts, l_leaf, kpad_of(P), y_far, add_to_output ? t.perm.get() : nullptr);
   });
   PAM_LAUNCH_CHECK();
 }
@@ -1487,7 +1491,7 @@ void launch_m2l_batch(const M2LOperator& op, const std::vector<M2LLevelSpec<TM, 
   template void launch_p2m<T>(int, const Tree&, const V4<T>*, T*, hipStream_t);                     \
   template void launch_m2m<T>(int, const Tree&, int, const T*, T*, hipStream_t);                    \
   template void launch_l2l<T>(int, const Tree&, int, const T*, T*, hipStream_t);                    \
-  template void launch_l2p<T>(int, const Tree&, const T*, const V4<T>*, T*, hipStream_t);           \
+  template void launch_l2p<T>(int, const Tree&, const T*, const V4<T>*, T*, hipStream_t, bool);           \
   template void launch_p2p<T>(const RbfDesc&, const Tree&, const Tree&, const Lists&, const V4<T>*, \
                               const V4<T>*, const double*, double*, hipStream_t);                       \
   template void build_m2l_dense<T>(int, const RbfDesc&, double, M2LOperator&, DevBuf<T>&,          \

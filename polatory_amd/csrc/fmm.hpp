@@ -53,8 +53,11 @@ void launch_m2m(int p, const Tree& t, int l, const T* m_child, T* m_parent, hipS
 // child level l from parent level l-1 (accumulates into l_child)
 template <class T>
 void launch_l2l(int p, const Tree& t, int l, const T* l_parent, T* l_child, hipStream_t s);
+// add_to_output: y_far is the output in input order, the far field is added
+// to it through the tree permutation.
 template <class T>
-void launch_l2p(int p, const Tree& t, const T* l_leaf, const V4<T>* pts, T* y_far, hipStream_t s);
+void launch_l2p(int p, const Tree& t, const T* l_leaf, const V4<T>* pts, T* y_far, hipStream_t s,
+                bool add_to_output = false);
 // out[perm[i]] = y_far[i] + near field, FP64 output in input order.
 template <class T>
 void launch_p2p(const RbfDesc& rbf, const Tree& trg, const Tree& src, const Lists& lists,

@@ -180,13 +180,20 @@ FmmEvaluator::FmmEvaluator(const RbfDesc& rbf, const double* bbox_min, const dou
     PAM_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   }
   for (auto& e : ev_) PAM_HIP(hipEventCreate(&e));
+  PAM_HIP(hipStreamCreateWithFlags(&p2p_stream_, hipStreamNonBlocking));
+  PAM_HIP(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming));
+  PAM_HIP(hipEventCreateWithFlags(&join_ev_, hipEventDisableTiming));
   bbox_out_.resize(12);
 }
 
 FmmEvaluator::~FmmEvaluator() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  if (p2p_stream_) (void)hipStreamSynchronize(p2p_stream_);
   for (auto& e : ev_)
     if (e) (void)hipEventDestroy(e);
+  if (fork_ev_) (void)hipEventDestroy(fork_ev_);
+  if (join_ev_) (void)hipEventDestroy(join_ev_);
+  if (p2p_stream_) (void)hipStreamDestroy(p2p_stream_);
   if (stream_ && own_stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -516,7 +523,7 @@ void FmmEvaluator::attach_source(const FmmEvaluator* owner) {
 }
 
 
-void FmmEvaluator::record(int idx) {
+void FmmEvaluator::record(int idx, hipStream_t s) {
   // Stage timing is skipped while the stream is being captured into a graph.
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   PAM_HIP(hipStreamIsCapturing(stream_, &cs));
@@ -524,11 +531,17 @@ void FmmEvaluator::record(int idx) {
     timing_valid_ = false;
     return;
   }
-  PAM_HIP(hipEventRecord(ev_[idx], stream_));
+  PAM_HIP(hipEventRecord(ev_[idx], s ? s : stream_));
 }
 
 template <class TP>
 void FmmEvaluator::prepare_source_expansions() {
+  prepare_source_points<TP>();
+  prepare_source_multipoles<TP>();
+}
+
+template <class TP>
+void FmmEvaluator::prepare_source_points() {
   FmmState& st = st_;
   const int P = cfg_.order;
   const int kp = kpad_of(P);
@@ -552,6 +565,14 @@ void FmmEvaluator::prepare_source_expansions() {
       st.s32_valid = true;
     }
   }
+}
+
+template <class TP>
+void FmmEvaluator::prepare_source_multipoles() {
+  FmmState& st = st_;
+  const int P = cfg_.order;
+  const int kp = kpad_of(P);
+  const int L = src_tree_.level;
   if (!st.multipoles_valid) {
     for (int l = 2; l <= L; l++) st.m[l].resize(static_cast<size_t>(src_tree_.levels[l].n) * kp);
     launch_p2m<double>(P, src_tree_, st.spts64.get(), st.m[L].get(), stream_);
@@ -584,7 +605,7 @@ void FmmEvaluator::run_fmm(double* d_out) {
   const int L = stree.level;
   record(1);
   if (!so_) {
-    prepare_source_expansions<TP>();
+    prepare_source_points<TP>();
   } else if (!sst.multipoles_valid || sst.multipole_order != P || !sst.s64_valid ||
              (sizeof(TP) == 4 && (!sst.s32_valid || !sst.m32_valid))) {
     throw_runtime("the shared source side is not prepared (call prepare_source on the owner)");
@@ -610,7 +631,21 @@ void FmmEvaluator::run_fmm(double* d_out) {
     sp2p = sst.spts64.get();
     tp2p = tpts64;
   }
+  if (!so_) prepare_source_multipoles<TP>();
   record(2);
+  // Fork: the near field (VALU work) writes the output on the second stream
+  // while M2L (MFMA work) and L2L run here; L2P adds the far field after the
+  // join (near + far, the same sum in the same order as one stream would do).
+  // Without overlap the near field runs on this stream after L2L.
+  auto near_field = [&](hipStream_t ps) {
+    PAM_HIP(hipEventRecord(fork_ev_, stream_));
+    PAM_HIP(hipStreamWaitEvent(ps, fork_ev_, 0));
+    record(7, ps);
+    launch_p2p<TP>(rbf_, tt, stree, lists_, tp2p, sp2p, nullptr, d_out, ps);
+    record(8, ps);
+    PAM_HIP(hipEventRecord(join_ev_, ps));
+  };
+  if (overlap_) near_field(p2p_stream_);
   const int deg = homogeneity(rbf_);
   const int32_t* vl = octant_vlist_device(stream_);
   const double tol = effective_svd_tol(cfg_, rbf_);
@@ -689,10 +724,11 @@ void FmmEvaluator::run_fmm(double* d_out) {
   record(3);
   for (int l = 3; l <= L; l++) launch_l2l<double>(P, tt, l, st.l[l - 1].get(), st.l[l].get(), stream_);
   record(4);
-  st.y_far.resize(static_cast<size_t>(n_trg()));
-  launch_l2p<double>(P, tt, st.l[L].get(), tpts64, st.y_far.get(), stream_);
+  if (!overlap_) near_field(stream_);
+  // Join the near field, then add the far field to it.
+  PAM_HIP(hipStreamWaitEvent(stream_, join_ev_, 0));
   record(5);
-  launch_p2p<TP>(rbf_, tt, stree, lists_, tp2p, sp2p, st.y_far.get(), d_out, stream_);
+  launch_l2p<double>(P, tt, st.l[L].get(), tpts64, d_out, stream_, true);
   record(6);
 }
 
@@ -791,13 +827,15 @@ void FmmEvaluator::evaluate(double* out, int64_t n, bool on_device) {
   if (nt > 0) {
     if (src.n_src_ == 0) {
       PAM_HIP(hipMemsetAsync(d_out, 0, sizeof(double) * nt, stream_));
-      for (int i = 1; i <= 6; i++) record(i);
+      for (int i = 1; i <= 8; i++) record(i);
       stats_.direct = 1;
     } else if (use_direct()) {
       for (int i = 1; i <= 5; i++) record(i);
       direct_sum_f64(rbf_, src.src_pts_.get(), src.w_.get(), src.n_src_, d_target_points(), nt, d_out, false,
                      direct_scratch_, stream_);
       record(6);
+      record(7);
+      record(8);
       stats_.direct = 1;
     } else if (cfg_.precision == kFp64) {
       run_fmm<double>(d_out);
@@ -805,7 +843,7 @@ void FmmEvaluator::evaluate(double* out, int64_t n, bool on_device) {
       run_fmm<float>(d_out);
     }
   } else {
-    for (int i = 1; i <= 6; i++) record(i);
+    for (int i = 1; i <= 8; i++) record(i);
   }
   stats_.precision = cfg_.precision;
   stats_.order = cfg_.order;
@@ -822,16 +860,22 @@ void FmmEvaluator::evaluate(double* out, int64_t n, bool on_device) {
 const FmmStats& FmmEvaluator::stats() {
   if (stats_pending_ && !timing_valid_) stats_pending_ = false;  // last evaluation was captured
   if (stats_pending_) {
+    // Events: 0 start, 1 trees, 2 upward, 3 M2L, 4 L2L, 5 near field joined,
+    // 6 L2P (end); 7 / 8 around the near field on its own stream.
     PAM_HIP(hipEventSynchronize(ev_[6]));
-    float ms[6];
-    for (int i = 0; i < 6; i++) PAM_HIP(hipEventElapsedTime(&ms[i], ev_[i], ev_[i + 1]));
-    stats_.ms_tree = ms[0];
-    stats_.ms_upward = ms[1];
-    stats_.ms_m2l = ms[2];
-    stats_.ms_l2l = ms[3];
-    stats_.ms_l2p = ms[4];
-    stats_.ms_p2p = ms[5];
-    stats_.ms_total = ms[0] + ms[1] + ms[2] + ms[3] + ms[4] + ms[5];
+    PAM_HIP(hipEventSynchronize(ev_[8]));
+    auto el = [&](int a, int b) {
+      float ms = 0.0f;
+      PAM_HIP(hipEventElapsedTime(&ms, ev_[a], ev_[b]));
+      return static_cast<double>(ms);
+    };
+    stats_.ms_tree = el(0, 1);
+    stats_.ms_upward = el(1, 2);
+    stats_.ms_m2l = el(2, 3);
+    stats_.ms_l2l = el(3, 4);
+    stats_.ms_l2p = el(5, 6);
+    stats_.ms_p2p = el(7, 8);
+    stats_.ms_total = el(0, 6);
     stats_pending_ = false;
   }
   return stats_;

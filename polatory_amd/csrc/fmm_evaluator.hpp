@@ -74,6 +74,9 @@ class FmmEvaluator {
 
   void set_accuracy(double accuracy);
   void set_config(const FmmConfig& c);  // fixes the configuration (no accuracy search)
+  // Near field on a second stream beside the far field (default), or all
+  // stages one after another on the evaluator's stream.
+  void set_overlap(bool on) { overlap_ = on; }
   void set_source_points(const double* xyz, int64_t n, bool on_device);
   void set_target_points(const double* xyz, int64_t n, bool on_device);
   void set_weights(const double* w, int64_t n, bool on_device);
@@ -115,9 +118,13 @@ class FmmEvaluator {
   template <class TP>
   void prepare_source_expansions();
   template <class TP>
+  void prepare_source_points();
+  template <class TP>
+  void prepare_source_multipoles();
+  template <class TP>
   void run_fmm(double* d_out);
   void invalidate_expansions();
-  void record(int idx);
+  void record(int idx, hipStream_t s = nullptr);
 
   RbfDesc rbf_;
   bool symmetric_;
@@ -152,6 +159,11 @@ class FmmEvaluator {
   uint64_t lists_gen_ = 0;            // bumped whenever the interaction lists change
 
   hipEvent_t ev_[10] = {};
+  // The near field runs on a second stream, concurrently with the far field
+  // (VALU work beside MFMA work); fork_ev_ / join_ev_ order it.
+  hipStream_t p2p_stream_ = nullptr;
+  bool overlap_ = true;
+  hipEvent_t fork_ev_ = nullptr, join_ev_ = nullptr;
   FmmStats stats_;
   bool stats_pending_ = false;
   bool timing_valid_ = true;  // false when the last evaluation was captured into a graph

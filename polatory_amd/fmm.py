@@ -92,6 +92,11 @@ class FmmEvaluator:
         c = config.to_c()
         _capi.call("pam_fmm_evaluator_set_config", self._h, ctypes.byref(c))
 
+    def set_overlap(self, on):
+        """Near field on a second stream beside the far field (default), or
+        every stage in sequence (per-stage times of each stage alone)."""
+        _capi.call("pam_fmm_evaluator_set_overlap", self._h, 1 if on else 0)
+
     def set_source_points(self, points):
         p = _points(points)
         self.n_src = p.shape[0]
