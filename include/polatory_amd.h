@@ -141,6 +141,20 @@ int pam_fmm_evaluator_export_m2l(pam_fmm_evaluator* h, int level, int32_t* slots
  * direct_evaluator.hpp:45-79, without polynomial terms). Host pointers. */
 int pam_direct_evaluate(const pam_rbf* rbf, const double* src, int64_t n_src, const double* w,
                         const double* trg, int64_t n_trg, double* out);
+/* Gradient of that sum at the targets, out[3 i + k] = sum_j w_j d/dx_k phi(t_i - s_j):
+ * the full_direct form (src/fmm/full_direct.hpp:7-52) of the gradient kernel
+ * (include/polatory/fmm/gradient_kernel.hpp). Host pointers. */
+int pam_direct_evaluate_gradient(const pam_rbf* rbf, const double* src, int64_t n_src, const double* w,
+                                 const double* trg, int64_t n_trg, double* out);
+
+/* Pointwise kernel values at n differences diff[3 i .. 3 i + 2], evaluated on
+ * the GPU: RbfBase::evaluate (rbf/rbf_base.hpp:40-43) -> out[i];
+ * evaluate_gradient (:45-48) -> out[3 i + k]; evaluate_hessian (:50-53) ->
+ * out[9 i + 3 r + c] (row-major). The anisotropy applies as in the reference:
+ * phi(A d), A^T grad(A d), A^T H(A d) A. Host pointers. */
+int pam_rbf_evaluate(const pam_rbf* rbf, const double* diff, int64_t n, double* out);
+int pam_rbf_evaluate_gradient(const pam_rbf* rbf, const double* diff, int64_t n, double* out);
+int pam_rbf_evaluate_hessian(const pam_rbf* rbf, const double* diff, int64_t n, double* out);
 
 
 /* ---------------------------------------------------------------------------

@@ -41,6 +41,11 @@ def lib():
         L.orc_rbf_evaluate.restype = ctypes.c_double
         L.orc_rbf_evaluate.argtypes = [ctypes.POINTER(OrcRbf), ctypes.c_double, ctypes.c_double,
                                        ctypes.c_double]
+        L.orc_rbf_gradient.argtypes = [ctypes.POINTER(OrcRbf), ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, P]
+        L.orc_rbf_hessian.argtypes = [ctypes.POINTER(OrcRbf), ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_double, P]
+        L.orc_direct_gradient_sum.argtypes = [P, P, i64, P, P, i64, P]
         L.orc_transform_points.argtypes = [P, P, i64, P]
         L.orc_poly_basis_size.restype = i64
         L.orc_poly_basis_size.argtypes = [ctypes.c_int]
@@ -95,6 +100,27 @@ def _rbf_array(rbfs):
 
 def rbf_evaluate(rbf, d):
     return lib().orc_rbf_evaluate(ctypes.byref(rbf), float(d[0]), float(d[1]), float(d[2]))
+
+
+def rbf_gradient(rbf, d):
+    g = np.empty(3)
+    lib().orc_rbf_gradient(ctypes.byref(rbf), float(d[0]), float(d[1]), float(d[2]), _p(g))
+    return g
+
+
+def rbf_hessian(rbf, d):
+    h = np.empty(9)
+    lib().orc_rbf_hessian(ctypes.byref(rbf), float(d[0]), float(d[1]), float(d[2]), _p(h))
+    return h.reshape(3, 3)
+
+
+def direct_gradient_sum(rbf, src, w, trg):
+    s = _f64(src, (-1, 3))
+    t = _f64(trg, (-1, 3))
+    w = _f64(w)
+    out = np.empty((t.shape[0], 3))
+    lib().orc_direct_gradient_sum(ctypes.byref(rbf), _p(s), s.shape[0], _p(w), _p(t), t.shape[0], _p(out))
+    return out
 
 
 def transform_points(aniso, pts):

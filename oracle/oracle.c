@@ -42,6 +42,72 @@ double orc_rbf_evaluate(const orc_rbf* rbf, double dx, double dy, double dz) {
   return phi_iso(rbf, t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
 }
 
+/* Gradient and Hessian (rbf/polyharmonic_odd.hpp:45-72, rbf/cov_exponential.hpp:42-62,
+ * rbf/rbf_base.hpp:45-53): g = A^T g_iso(A d), H = A^T H_iso(A d) A, with
+ * g_iso = f u and H_iso = f (I + t u u^T) for the factor f and coefficient t
+ * of each kernel. */
+static double grad_factor(const orc_rbf* rbf, const double* u, double* t) {
+  double q = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  switch (rbf->kind) {
+    case ORC_BH3:
+    case ORC_TH3: {
+      int k = rbf->kind == ORC_BH3 ? 1 : 3;
+      double sign = rbf->kind == ORC_BH3 ? -1.0 : 1.0;
+      double rho2 = q + rbf->params[1] * rbf->params[1];
+      double rho = sqrt(rho2);
+      if (rho == 0.0) {
+        *t = 0.0;
+        return 0.0;
+      }
+      *t = (k - 2) / rho2;
+      return sign * k * rbf->params[0] * (k == 1 ? 1.0 / rho : rho);
+    }
+    case ORC_EXP: {
+      double r = sqrt(q), range = rbf->params[1];
+      *t = -(1.0 / (r * r) + 3.0 / (range * r));
+      return -3.0 * rbf->params[0] * exp(-3.0 * (r / range)) / (range * r);
+    }
+  }
+  *t = NAN;
+  return NAN;
+}
+
+void orc_rbf_gradient(const orc_rbf* rbf, double dx, double dy, double dz, double* g) {
+  const double* a = rbf->aniso;
+  double u[3], t;
+  apply_aniso(a, dx, dy, dz, u);
+  double f = grad_factor(rbf, u, &t);
+  double gi[3] = {f * u[0], f * u[1], f * u[2]};
+  for (int k = 0; k < 3; k++) g[k] = (a[k] * gi[0] + a[3 + k] * gi[1]) + a[6 + k] * gi[2];
+}
+
+void orc_rbf_hessian(const orc_rbf* rbf, double dx, double dy, double dz, double* h) {
+  const double* a = rbf->aniso;
+  double u[3], t, hi[9], ha[9];
+  apply_aniso(a, dx, dy, dz, u);
+  double f = grad_factor(rbf, u, &t);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) hi[3 * i + j] = f * ((i == j ? 1.0 : 0.0) + t * u[i] * u[j]);
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) ha[3 * r + c] = (hi[3 * r] * a[c] + hi[3 * r + 1] * a[3 + c]) + hi[3 * r + 2] * a[6 + c];
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) h[3 * r + c] = (a[r] * ha[c] + a[3 + r] * ha[3 + c]) + a[6 + r] * ha[6 + c];
+}
+
+/* Gradient of sum_j w_j phi(t - s_j) at each target (full_direct.hpp:7-52 with
+ * the gradient kernel), summed over sources in index order. */
+void orc_direct_gradient_sum(const orc_rbf* rbf, const double* src, int64_t n_src, const double* w,
+                             const double* trg, int64_t n_trg, double* out) {
+  for (int64_t i = 0; i < n_trg; i++) {
+    double acc[3] = {0.0, 0.0, 0.0}, g[3];
+    for (int64_t j = 0; j < n_src; j++) {
+      orc_rbf_gradient(rbf, trg[3 * i] - src[3 * j], trg[3 * i + 1] - src[3 * j + 1], trg[3 * i + 2] - src[3 * j + 2], g);
+      for (int k = 0; k < 3; k++) acc[k] += w[j] * g[k];
+    }
+    for (int k = 0; k < 3; k++) out[3 * i + k] = acc[k];
+  }
+}
+
 void orc_transform_points(const double* aniso, const double* pts, int64_t n, double* out) {
   for (int64_t i = 0; i < n; i++) apply_aniso(aniso, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], out + 3 * i);
 }

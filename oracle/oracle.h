@@ -27,6 +27,10 @@ typedef struct {
 
 /* phi(A d) for one difference vector d (rbf/rbf_base.hpp:40-43). */
 double orc_rbf_evaluate(const orc_rbf* rbf, double dx, double dy, double dz);
+void orc_rbf_gradient(const orc_rbf* rbf, double dx, double dy, double dz, double* g);
+void orc_rbf_hessian(const orc_rbf* rbf, double dx, double dy, double dz, double* h);
+void orc_direct_gradient_sum(const orc_rbf* rbf, const double* src, int64_t n_src, const double* w,
+                             const double* trg, int64_t n_trg, double* out);
 
 /* Points p -> p * A^T, evaluated as ((A0*x + A1*y) + A2*z) without contraction
  * (geometry/point3d.hpp:36-40). */

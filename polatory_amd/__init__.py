@@ -6,7 +6,7 @@ the host-side mirror used by tests and benchmarks.
 """
 from . import _capi
 from .fmm import (FmmConfig, FmmEvaluator, FmmSymmetricEvaluator, direct_evaluate,
-                  make_fmm_evaluator, make_fmm_symmetric_evaluator)
+                  direct_evaluate_gradient, make_fmm_evaluator, make_fmm_symmetric_evaluator)
 from .interpolant import (FitOptions, Interpolant, Model, RasPreconditioner, fgmres_dense,
                           POLY_DEGREE_MIN)
 from .rbf import Biharmonic3D, CovExponential, Rbf, Triharmonic3D, make_rbf
@@ -22,6 +22,7 @@ def device_count():
 
 __all__ = ["Biharmonic3D", "Triharmonic3D", "CovExponential", "Rbf", "make_rbf",
            "FmmConfig", "FmmEvaluator", "FmmSymmetricEvaluator", "make_fmm_evaluator",
-           "make_fmm_symmetric_evaluator", "direct_evaluate", "device_count", "Model",
+           "make_fmm_symmetric_evaluator", "direct_evaluate", "direct_evaluate_gradient",
+           "device_count", "Model",
            "Interpolant", "FitOptions", "RasPreconditioner", "fgmres_dense", "POLY_DEGREE_MIN",
            "Server"]

@@ -113,6 +113,10 @@ _SIGS = {
     "pam_fmm_evaluator_export_p2p": [_P, _P],
     "pam_fmm_evaluator_export_m2l": [_P, _INT, _P],
     "pam_direct_evaluate": [ctypes.POINTER(PamRbf), _P, _I64, _P, _P, _I64, _P],
+    "pam_direct_evaluate_gradient": [ctypes.POINTER(PamRbf), _P, _I64, _P, _P, _I64, _P],
+    "pam_rbf_evaluate": [ctypes.POINTER(PamRbf), _P, _I64, _P],
+    "pam_rbf_evaluate_gradient": [ctypes.POINTER(PamRbf), _P, _I64, _P],
+    "pam_rbf_evaluate_hessian": [ctypes.POINTER(PamRbf), _P, _I64, _P],
     "pam_model_create": [_P, _INT, _INT, _D, ctypes.POINTER(_P)],
     "pam_model_destroy": [_P],
     "pam_model_poly_degree": [_P, ctypes.POINTER(_INT)],

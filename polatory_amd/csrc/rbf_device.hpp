@@ -98,6 +98,67 @@ __device__ __forceinline__ T phi_t(T r2, T a, T b) {
   }
 }
 
+// Value, gradient and Hessian of the isotropic kernel at a difference u that
+// is already transformed by the anisotropy (rbf/polyharmonic_odd.hpp:32-72,
+// rbf/cov_exponential.hpp:33-62). Polyharmonic of degree K (bh3: K = 1, sign
+// -1; th3: K = 3, sign +1), rho = sqrt(|u|^2 + c^2):
+//   g = sign K slope rho^(K-2) u,  H = sign K slope rho^(K-2) (I + (K-2) u u^T / rho^2),
+//   both zero at rho = 0.
+// Exponential, r = |u|:
+//   g = -3 psill e^(-3 r / range) / (range r) u,
+//   H = (same factor) (I - (1/r^2 + 3 / (range r)) u u^T)   (no guard at r = 0).
+template <int KIND>
+__host__ __device__ inline double rbf_value_iso(double p0, double p1, const double u[3]) {
+  const double q = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  if constexpr (KIND == kBh3) {
+    return -p0 * sqrt(q + p1 * p1);
+  } else if constexpr (KIND == kTh3) {
+    const double rho = sqrt(q + p1 * p1);
+    return p0 * (rho * rho * rho);
+  } else {
+    return p0 * exp(-3.0 * (sqrt(q) / p1));
+  }
+}
+
+template <int KIND>
+__host__ __device__ inline double rbf_grad_factor(double p0, double p1, const double u[3], double* t) {
+  // Returns the factor f with g = f u; t receives the coefficient of u u^T in
+  // H / f - I.
+  const double q = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  if constexpr (KIND == kBh3 || KIND == kTh3) {
+    constexpr int K = KIND == kBh3 ? 1 : 3;
+    constexpr double sign = KIND == kBh3 ? -1.0 : 1.0;
+    const double rho2 = q + p1 * p1;
+    const double rho = sqrt(rho2);
+    if (rho == 0.0) {
+      *t = 0.0;
+      return 0.0;
+    }
+    *t = (K - 2) / rho2;
+    const double rk = K == 1 ? 1.0 / rho : rho;  // rho^(K-2)
+    return sign * K * p0 * rk;
+  } else {
+    const double r = sqrt(q);
+    *t = -(1.0 / (r * r) + 3.0 / (p1 * r));
+    return -3.0 * p0 * exp(-3.0 * (r / p1)) / (p1 * r);
+  }
+}
+
+template <int KIND>
+__host__ __device__ inline void rbf_grad_iso(double p0, double p1, const double u[3], double g[3]) {
+  double t;
+  const double f = rbf_grad_factor<KIND>(p0, p1, u, &t);
+  for (int k = 0; k < 3; k++) g[k] = f * u[k];
+}
+
+template <int KIND>
+__host__ __device__ inline void rbf_hess_iso(double p0, double p1, const double u[3], double h[9]) {
+  double t;
+  const double f = rbf_grad_factor<KIND>(p0, p1, u, &t);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) h[3 * i + j] = f * ((i == j ? 1.0 : 0.0) + t * u[i] * u[j]);
+}
+
 // Dispatch a templated functor on the runtime kind.
 template <class F>
 inline void dispatch_kind(int kind, F&& f) {

@@ -201,6 +201,19 @@ def make_fmm_symmetric_evaluator(rbf, bbox=None):
     return FmmSymmetricEvaluator(rbf, bbox)
 
 
+def direct_evaluate_gradient(rbf: Rbf, src, weights, trg):
+    """FP64 direct gradient sum on the GPU: (n_trg, 3) with row i the gradient
+    of sum_j w_j phi(x - s_j) at x = t_i (the gradient kernel's full_direct)."""
+    s = _points(src)
+    t = _points(trg)
+    w = np.ascontiguousarray(weights, dtype=np.float64).reshape(-1)
+    out = np.empty((t.shape[0], 3))
+    rc = rbf.to_c()
+    _capi.call("pam_direct_evaluate_gradient", ctypes.byref(rc), _ptr(s), s.shape[0], _ptr(w), _ptr(t),
+               t.shape[0], _ptr(out))
+    return out
+
+
 def direct_evaluate(rbf: Rbf, src, weights, trg):
     """FP64 direct sum on the GPU (no polynomial terms)."""
     s = _points(src)

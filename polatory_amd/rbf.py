@@ -51,6 +51,28 @@ class Rbf:
         return (type(self) is type(other) and self._params == other._params
                 and np.array_equal(self._aniso, other._aniso))
 
+    # rbf_base.hpp:40-53, evaluated on the GPU for a batch of differences
+    def _point(self, diff, what, per):
+        import ctypes
+        d = np.ascontiguousarray(diff, dtype=np.float64).reshape(-1, 3)
+        out = np.empty((d.shape[0], per) if per > 1 else d.shape[0])
+        rc = self.to_c()
+        _capi.call(what, ctypes.byref(rc), d.ctypes.data_as(ctypes.c_void_p), d.shape[0],
+                   out.ctypes.data_as(ctypes.c_void_p))
+        return out
+
+    def evaluate(self, diff):
+        """phi(A d) for each row d of diff (n, 3) -> (n,)."""
+        return self._point(diff, "pam_rbf_evaluate", 1)
+
+    def evaluate_gradient(self, diff):
+        """A^T grad phi(A d) for each row -> (n, 3)."""
+        return self._point(diff, "pam_rbf_evaluate_gradient", 3)
+
+    def evaluate_hessian(self, diff):
+        """A^T H(A d) A for each row -> (n, 3, 3)."""
+        return self._point(diff, "pam_rbf_evaluate_hessian", 9).reshape(-1, 3, 3)
+
 
 class _PolyharmonicOdd(Rbf):
     def set_parameters(self, params):
