@@ -687,7 +687,7 @@ template <class TM, bool kEmit = false, class Emit = NoEmit>
 __device__ __forceinline__ void m2l_mainloop(M2LSmem<TM>& sm, const int (*src_tab)[kM2LC],
                                              const int* act_v, int nact, const TM* __restrict__ a,
                                              int64_t per_v, const TM* __restrict__ in, int kpad, int rpad,
-                                             int rb, double (&dacc)[4][2][4], Emit emit = Emit{}) {
+                                             int rb, int r, double (&dacc)[4][2][4], Emit emit = Emit{}) {
   using Tl = M2LTile<TM>;
   using acc_t = typename Mfma<TM>::acc_t;
   constexpr int kK = Tl::kK, kKB = Tl::kKB, kUpr = Tl::kUpr;
@@ -775,6 +775,9 @@ __device__ __forceinline__ void m2l_mainloop(M2LSmem<TM>& sm, const int (*src_ta
   };
 
   const int wr = wave >> 1, wc = wave & 1;
+  // 16-row tiles of this wave that hold operator rows (< r); tiles wholly in
+  // the padding rows are not multiplied (their accumulators stay zero).
+  const int nvt = __builtin_amdgcn_readfirstlane(min(4, max(0, (r - rb * kM2LR - wr * 64 + 15) / 16)));
   auto compute = [&](int t) {
     const int buf = t & 1;
     const int kc = t % nkc;
@@ -784,7 +787,8 @@ __device__ __forceinline__ void m2l_mainloop(M2LSmem<TM>& sm, const int (*src_ta
       V4T af[4], bf[2];
 #pragma unroll
       for (int i = 0; i < 4; i++)
-        af[i] = *reinterpret_cast<const V4T*>(&sm.as[buf][kb][wr * 64 + i * 16 + (lane & 15)][(lane >> 4) * 4]);
+        if (i < nvt)
+          af[i] = *reinterpret_cast<const V4T*>(&sm.as[buf][kb][wr * 64 + i * 16 + (lane & 15)][(lane >> 4) * 4]);
 #pragma unroll
       for (int j = 0; j < 2; j++)
         bf[j] = *reinterpret_cast<const V4T*>(&sm.bs[buf][wc * 32 + j * 16 + (lane & 15)][kb * 16 + (lane >> 4) * 4]);
@@ -792,6 +796,7 @@ __device__ __forceinline__ void m2l_mainloop(M2LSmem<TM>& sm, const int (*src_ta
       for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++) {
+          if (i >= nvt) continue;
           acc[i][j] = Mfma<TM>::op(af[i].x, bf[j].x, acc[i][j]);
           acc[i][j] = Mfma<TM>::op(af[i].y, bf[j].y, acc[i][j]);
           acc[i][j] = Mfma<TM>::op(af[i].z, bf[j].z, acc[i][j]);
@@ -893,7 +898,7 @@ m2l_dense_kernel(M2LDenseArgs<TM> b) {
   }
   __syncthreads();
   double dacc[4][2][4];
-  m2l_mainloop<TM>(sm, src_tab, act_v, nact, b.a[lv], b.per_v, in, kpad, rpad, rb, dacc);
+  m2l_mainloop<TM>(sm, src_tab, act_v, nact, b.a[lv], b.per_v, in, kpad, rpad, rb, b.r, dacc);
   const int wr = wave >> 1, wc = wave & 1;
 
   // D layout (16 x 16), column lane % 16; register q holds row 4 (lane / 16) + q
@@ -981,7 +986,7 @@ m2l_pair_kernel(M2LPairArgs<TM, PT> b) {
     }
   };
   double dacc[4][2][4];
-  m2l_mainloop<TM, true>(sm, src_tab, act_v, nb, b.a[lv], b.per_v, b.in[lv], b.kpad, rpad, rb, dacc, emit);
+  m2l_mainloop<TM, true>(sm, src_tab, act_v, nb, b.a[lv], b.per_v, b.in[lv], b.kpad, rpad, rb, b.r, dacc, emit);
 }
 
 // out[cell][row] = scale * sum over the cell's slots, in slot order, of the
