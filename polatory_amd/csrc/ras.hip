@@ -151,105 +151,186 @@ __global__ void assemble_m_kernel(RbfSet r, int l, const double* __restrict__ pt
   }
 }
 
-// One workgroup factors one domain: right-looking Cholesky with 32-wide panels;
-// the trailing update runs over 64x64 tiles of the lower triangle.
-__global__ void __launch_bounds__(kT)
-cholesky_kernel(const int64_t* __restrict__ ptr, int l, const int64_t* __restrict__ mat_ptr,
-                double* __restrict__ mat, int32_t* __restrict__ flag) {
-  __shared__ double dg[kNb][kNb + 1];
-  __shared__ double pi[kTile][kNb + 1];
-  __shared__ double pj[kTile][kNb + 1];
-  const int d = blockIdx.x;
-  const int n = static_cast<int>(ptr[d + 1] - ptr[d]) - l;
-  if (n <= 0) return;
-  double* a = mat + mat_ptr[d];
+// One workgroup factors one domain: right-looking Cholesky with 32-wide
+// panels. Per panel: one wavefront factors the 32 x 32 diagonal block in LDS
+// (wave-level synchronisation only) and inverts it; the rows below are the
+// panel times that inverse (independent dot products, no sequential solve);
+// the trailing update A22 -= L21 L21^T runs over 64 x 64 tiles of the lower
+// triangle on FP64 MFMA (4 waves as 2 x 2, 32 x 32 each), the tile result
+// staged in LDS so that the read-modify-write of A22 is coalesced.
+struct CholSmem {
+  double dg[kNb][kNb + 1];     // diagonal block (L11 after the factorisation)
+  double pi[kTile][kNb + 1];   // panel rows of tile I
+  double pj[kTile][kNb + 1];   // panel rows of tile J
+  double dt[kTile][kTile + 1]; // MFMA result of a tile; the inverse of L11 during the panel phase
+};
+
+// Panel step k0 of the factorisation of the n x n matrix a (column-major,
+// lower triangle): L11 by one wavefront and its inverse, then L21 = A21 L11^-T.
+// Ends with a barrier.
+__device__ void chol_panel(CholSmem& sm, double* __restrict__ a, int n, int k0, int32_t* __restrict__ flag) {
   const int lda = n;
-  for (int k0 = 0; k0 < n; k0 += kNb) {
-    const int kb = min(kNb, n - k0);
-    for (int e = threadIdx.x; e < kb * kb; e += kT) {
-      int rr = e % kb, cc = e / kb;
-      dg[rr][cc] = a[static_cast<int64_t>(k0 + cc) * lda + k0 + rr];
-    }
-    __syncthreads();
+  const int kb = min(kNb, n - k0);
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  double (*inv)[kNb + 1] = reinterpret_cast<double (*)[kNb + 1]>(&sm.dt[0][0]);
+  for (int e = tid; e < kb * kb; e += kT) {
+    int rr = e % kb, cc = e / kb;
+    sm.dg[rr][cc] = a[static_cast<int64_t>(k0 + cc) * lda + k0 + rr];
+  }
+  __syncthreads();
+  if (wave == 0) {
     for (int j = 0; j < kb; j++) {
-      if (threadIdx.x == 0) {
-        double v = dg[j][j];
+      if (lane == 0) {
+        double v = sm.dg[j][j];
         if (!(v > 0.0)) {
           atomicExch(flag, 1);
           v = fabs(v) + 1e-300;
         }
-        dg[j][j] = sqrt(v);
+        sm.dg[j][j] = sqrt(v);
       }
-      __syncthreads();
-      for (int rr = j + 1 + threadIdx.x; rr < kb; rr += kT) dg[rr][j] /= dg[j][j];
-      __syncthreads();
-      int w = kb - j - 1;
-      for (int e = threadIdx.x; e < w * w; e += kT) {
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      if (lane > j && lane < kb) sm.dg[lane][j] /= sm.dg[j][j];
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      const int w = kb - j - 1;
+      for (int e = lane; e < w * w; e += 64) {
         int rr = j + 1 + e / w, cc = j + 1 + e % w;
-        if (cc <= rr) dg[rr][cc] -= dg[rr][j] * dg[cc][j];
+        if (cc <= rr) sm.dg[rr][cc] -= sm.dg[rr][j] * sm.dg[cc][j];
       }
-      __syncthreads();
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     }
-    for (int e = threadIdx.x; e < kb * kb; e += kT) {
-      int rr = e % kb, cc = e / kb;
-      if (rr >= cc) a[static_cast<int64_t>(k0 + cc) * lda + k0 + rr] = dg[rr][cc];
-    }
-    // panel: rows below the diagonal block, X L11^T = A21
-    for (int i = k0 + kb + threadIdx.x; i < n; i += kT) {
-      double x[kNb];
-      for (int c = 0; c < kb; c++) {
-        double v = a[static_cast<int64_t>(k0 + c) * lda + i];
-        for (int t = 0; t < c; t++) v -= x[t] * dg[c][t];
-        x[c] = v / dg[c][c];
-        a[static_cast<int64_t>(k0 + c) * lda + i] = x[c];
-      }
-    }
-    __syncthreads();
-    // trailing update of the lower triangle: A22 -= L21 L21^T
-    const int base = k0 + kb;
-    const int rem = n - base;
-    if (rem <= 0) break;
-    const int nt = (rem + kTile - 1) / kTile;
-    const int ntiles = nt * (nt + 1) / 2;
-    const int tr = threadIdx.x % 16, tc = threadIdx.x / 16;
-    for (int t = 0; t < ntiles; t++) {
-      // t -> (I, J) with I >= J
-      int I = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) / 2.0);
-      while ((I + 1) * (I + 2) / 2 <= t) I++;
-      while (I * (I + 1) / 2 > t) I--;
-      int J = t - I * (I + 1) / 2;
-      int r0 = base + I * kTile, c0 = base + J * kTile;
-      for (int e = threadIdx.x; e < kTile * kb; e += kT) {
-        int rr = e % kTile, cc = e / kTile;
-        pi[rr][cc] = (r0 + rr < n) ? a[static_cast<int64_t>(k0 + cc) * lda + r0 + rr] : 0.0;
-        pj[rr][cc] = (c0 + rr < n) ? a[static_cast<int64_t>(k0 + cc) * lda + c0 + rr] : 0.0;
-      }
-      __syncthreads();
-      double acc[4][4] = {};
-      for (int c = 0; c < kb; c++) {
-        double x[4], y[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          x[q] = pi[tr * 4 + q][c];
-          y[q] = pj[tc * 4 + q][c];
+    // inverse of L11, one column per lane: inv[r][c] for r >= c
+    if (lane < kb) {
+      const int c = lane;
+      for (int r = 0; r < kb; r++) {
+        if (r < c) {
+          inv[r][c] = 0.0;
+          continue;
         }
-#pragma unroll
-        for (int p = 0; p < 4; p++)
-#pragma unroll
-          for (int q = 0; q < 4; q++) acc[p][q] += x[p] * y[q];
+        double v = r == c ? 1.0 : 0.0;
+        for (int t = c; t < r; t++) v -= sm.dg[r][t] * inv[t][c];
+        inv[r][c] = v / sm.dg[r][r];
       }
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        int col = c0 + tc * 4 + q;
-#pragma unroll
-        for (int p = 0; p < 4; p++) {
-          int row = r0 + tr * 4 + p;
-          if (row < n && col < n && row >= col) a[static_cast<int64_t>(col) * lda + row] -= acc[p][q];
-        }
-      }
-      __syncthreads();
     }
   }
+  __syncthreads();
+  for (int e = tid; e < kb * kb; e += kT) {
+    int rr = e % kb, cc = e / kb;
+    if (rr >= cc) a[static_cast<int64_t>(k0 + cc) * lda + k0 + rr] = sm.dg[rr][cc];
+  }
+  // L21 = A21 L11^-T, row by row
+  for (int i = k0 + kb + tid; i < n; i += kT) {
+    double x[kNb];
+#pragma unroll
+    for (int t = 0; t < kNb; t++) x[t] = t < kb ? a[static_cast<int64_t>(k0 + t) * lda + i] : 0.0;
+#pragma unroll 4
+    for (int c = 0; c < kb; c++) {
+      double v = 0.0;
+#pragma unroll
+      for (int t = 0; t < kNb; t++)
+        if (t <= c) v += x[t] * inv[c][t];
+      a[static_cast<int64_t>(k0 + c) * lda + i] = v;
+    }
+  }
+  __syncthreads();
+}
+
+// Trailing tile (I, J), I >= J, of panel step k0: A22 -= L21 L21^T on FP64
+// MFMA (4 waves as 2 x 2, 32 x 32 each). load_i: stage tile I's panel rows
+// (they can stay staged across the J of one I). Ends with a barrier.
+__device__ void chol_tile(CholSmem& sm, double* __restrict__ a, int n, int k0, int I, int J, bool load_i) {
+  using acc_t = double __attribute__((ext_vector_type(4)));
+  const int lda = n;
+  const int kb = min(kNb, n - k0);
+  const int base = k0 + kb;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wi = wave >> 1, wj = wave & 1;
+  const int r0 = base + I * kTile, c0 = base + J * kTile;
+  if (load_i) {
+    for (int e = tid; e < kTile * kNb; e += kT) {
+      int rr = e % kTile, cc = e / kTile;
+      sm.pi[rr][cc] = (r0 + rr < n && cc < kb) ? a[static_cast<int64_t>(k0 + cc) * lda + r0 + rr] : 0.0;
+    }
+  }
+  for (int e = tid; e < kTile * kNb; e += kT) {
+    int rr = e % kTile, cc = e / kTile;
+    sm.pj[rr][cc] = (c0 + rr < n && cc < kb) ? a[static_cast<int64_t>(k0 + cc) * lda + c0 + rr] : 0.0;
+  }
+  __syncthreads();
+  acc_t acc[2][2];
+#pragma unroll
+  for (int ii = 0; ii < 2; ii++)
+#pragma unroll
+    for (int jj = 0; jj < 2; jj++) acc[ii][jj] = acc_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int ks = 0; ks < kNb / 4; ks++) {
+    double af[2], bf[2];
+#pragma unroll
+    for (int ii = 0; ii < 2; ii++) af[ii] = sm.pi[wi * 32 + ii * 16 + (lane & 15)][ks * 4 + (lane >> 4)];
+#pragma unroll
+    for (int jj = 0; jj < 2; jj++) bf[jj] = sm.pj[wj * 32 + jj * 16 + (lane & 15)][ks * 4 + (lane >> 4)];
+#pragma unroll
+    for (int ii = 0; ii < 2; ii++)
+#pragma unroll
+      for (int jj = 0; jj < 2; jj++)
+        acc[ii][jj] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[ii], bf[jj], acc[ii][jj], 0, 0, 0);
+  }
+  // D layout of the f64 instruction: register q holds row lane / 16 + 4 q, column lane % 16
+#pragma unroll
+  for (int ii = 0; ii < 2; ii++)
+#pragma unroll
+    for (int jj = 0; jj < 2; jj++)
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        sm.dt[wi * 32 + ii * 16 + (lane >> 4) + 4 * q][wj * 32 + jj * 16 + (lane & 15)] = acc[ii][jj][q];
+  __syncthreads();
+  for (int e = tid; e < kTile * kTile; e += kT) {
+    int rr = e % kTile, cc = e / kTile;
+    int row = r0 + rr, col = c0 + cc;
+    if (row < n && col < n && row >= col) a[static_cast<int64_t>(col) * lda + row] -= sm.dt[rr][cc];
+  }
+  __syncthreads();
+}
+
+// One workgroup per domain, all panel steps (many domains at once).
+__global__ void __launch_bounds__(kT)
+cholesky_kernel(const int64_t* __restrict__ ptr, int l, const int64_t* __restrict__ mat_ptr,
+                double* __restrict__ mat, int32_t* __restrict__ flag) {
+  __shared__ CholSmem sm;
+  const int d = blockIdx.x;
+  const int n = static_cast<int>(ptr[d + 1] - ptr[d]) - l;
+  if (n <= 0) return;
+  double* a = mat + mat_ptr[d];
+  for (int k0 = 0; k0 < n; k0 += kNb) {
+    chol_panel(sm, a, n, k0, flag);
+    const int rem = n - min(kNb, n - k0) - k0;
+    if (rem <= 0) break;
+    const int nt = (rem + kTile - 1) / kTile;
+    for (int I = 0; I < nt; I++)
+      for (int J = 0; J <= I; J++) chol_tile(sm, a, n, k0, I, J, J == 0);
+  }
+}
+
+// A single large matrix: one launch per panel step for the panel (one
+// workgroup) and one for its trailing tiles (one workgroup per tile).
+__global__ void __launch_bounds__(kT)
+cholesky_panel_kernel(double* __restrict__ a, int n, int k0, int32_t* __restrict__ flag) {
+  __shared__ CholSmem sm;
+  chol_panel(sm, a, n, k0, flag);
+}
+
+__global__ void __launch_bounds__(kT)
+cholesky_trailing_kernel(double* __restrict__ a, int n, int k0) {
+  __shared__ CholSmem sm;
+  // blockIdx.x -> (I, J) with I >= J
+  const int t = blockIdx.x;
+  int I = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) / 2.0);
+  while ((I + 1) * (I + 2) / 2 <= t) I++;
+  while (I * (I + 1) / 2 > t) I--;
+  const int J = t - I * (I + 1) / 2;
+  chol_tile(sm, a, n, k0, I, J, true);
 }
 
 __device__ __forceinline__ double block_sum(double v, double* sh) {
@@ -932,10 +1013,14 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
     std::fflush(stdout);
   };
   tick("unisolvent set, transforms");
-  // The host divides level l-1 while the device assembles and factors level l.
+  // Each level's batch is launched on its own stream as soon as its domains
+  // exist; the host then picks the coarser level's points and divides it
+  // while the device assembles and factors.
   for (int level = n_levels_ - 1; level >= 1; level--) {
     DomainDivider div(a_pts, point_idcs_[level], poly_idcs);
     tick("divide");
+    setup_batch(fine_[level], div.domains(), false, fork_stream());
+    tick("batch launched");
     double finest = std::log(static_cast<double>(mu_)) / std::log(kFineToCoarseRatio);
     double coarsest = std::log(static_cast<double>(kNCoarsestPoints)) / std::log(kFineToCoarseRatio);
     auto n_coarse = static_cast<int64_t>(std::pow(
@@ -943,15 +1028,14 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
     point_idcs_[level - 1] = div.choose_coarse_points(n_coarse);
     tick("coarse points");
     h_domains_[level] = std::move(div.domains());
-    setup_batch(fine_[level], h_domains_[level], false);
-    tick("batch launched");
   }
   {
     Domain cd;
     cd.idx = point_idcs_[0];
     cd.inner.assign(cd.idx.size(), 1);
     h_domains_[0] = {cd};
-    setup_batch(coarse_, h_domains_[0], true);
+    hipStream_t cs = fork_stream();
+    setup_batch(coarse_, h_domains_[0], true, cs);
     // The coarse solve is one domain: solved by a single workgroup it would
     // serialize on one CU, so it is turned into a dense operator once (one
     // workgroup per unit vector) and applied as a matrix-vector product.
@@ -960,16 +1044,19 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
     if (coarse_.n_dom == 1 && m > 0 && m <= kCoarseOpMax && shmem <= 96 * 1024) {
       coarse_.op.resize(static_cast<size_t>(m + l_) * m);
       RbfSet rs = make_rbf_set(model_);
-      coarse_operator_kernel<<<m, kT, shmem, s_>>>(rs, l_, d_pts_.get(), coarse_.idx.get(), m, lag_.get(),
+      coarse_operator_kernel<<<m, kT, shmem, cs>>>(rs, l_, d_pts_.get(), coarse_.idx.get(), m, lag_.get(),
                                                    coarse_.mat.get(), pinv_.get(), coarse_.op.get());
       PAM_LAUNCH_CHECK();
       coarse_.has_op = true;
     }
   }
+  join_streams();
   int32_t flag = 0;
   PAM_HIP(hipMemcpyAsync(&flag, flag_.get(), sizeof flag, hipMemcpyDeviceToHost, s_));
   PAM_HIP(hipStreamSynchronize(s_));
   setup_tmp_.clear();
+  for (hipStream_t ls : setup_streams_) PAM_HIP(hipStreamDestroy(ls));
+  setup_streams_.clear();
   tick("factors done");
   if (flag) throw_runtime("a subdomain matrix Q^T A Q is not positive definite");
   d_point_idcs_.resize(n_levels_);
@@ -990,7 +1077,29 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
   setup_s_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
-void RasPreconditioner::setup_batch(DomainBatch& b, const std::vector<Domain>& doms, bool coarse) {
+hipStream_t RasPreconditioner::fork_stream() {
+  hipStream_t ls = nullptr;
+  PAM_HIP(hipStreamCreateWithFlags(&ls, hipStreamNonBlocking));
+  setup_streams_.push_back(ls);
+  hipEvent_t ev = nullptr;
+  PAM_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  PAM_HIP(hipEventRecord(ev, s_));
+  PAM_HIP(hipStreamWaitEvent(ls, ev, 0));
+  PAM_HIP(hipEventDestroy(ev));
+  return ls;
+}
+
+void RasPreconditioner::join_streams() {
+  for (hipStream_t ls : setup_streams_) {
+    hipEvent_t ev = nullptr;
+    PAM_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    PAM_HIP(hipEventRecord(ev, ls));
+    PAM_HIP(hipStreamWaitEvent(s_, ev, 0));
+    PAM_HIP(hipEventDestroy(ev));
+  }
+}
+
+void RasPreconditioner::setup_batch(DomainBatch& b, const std::vector<Domain>& doms, bool coarse, hipStream_t ls) {
   (void)coarse;
   b.n_dom = static_cast<int>(doms.size());
   b.h_ptr.assign(b.n_dom + 1, 0);
@@ -1012,24 +1121,39 @@ void RasPreconditioner::setup_batch(DomainBatch& b, const std::vector<Domain>& d
     std::copy(doms[d].idx.begin(), doms[d].idx.end(), tmp->idx.begin() + b.h_ptr[d]);
     std::copy(doms[d].inner.begin(), doms[d].inner.end(), tmp->inner.begin() + b.h_ptr[d]);
   }
-  b.idx.upload(tmp->idx.data(), tmp->idx.size(), s_);
-  b.inner.upload(tmp->inner.data(), tmp->inner.size(), s_);
-  b.ptr.upload(b.h_ptr.data(), b.h_ptr.size(), s_);
-  b.mat_ptr.upload(tmp->mptr.data(), tmp->mptr.size(), s_);
+  b.idx.upload(tmp->idx.data(), tmp->idx.size(), ls);
+  b.inner.upload(tmp->inner.data(), tmp->inner.size(), ls);
+  b.ptr.upload(b.h_ptr.data(), b.h_ptr.size(), ls);
+  b.mat_ptr.upload(tmp->mptr.data(), tmp->mptr.size(), ls);
   b.mat.resize(static_cast<size_t>(std::max<int64_t>(tmp->mptr.back(), 1)));
   RbfSet rs = make_rbf_set(model_);
   tmp->u.resize(static_cast<size_t>(std::max<int64_t>(l_ * (b.h_ptr.back() - static_cast<int64_t>(b.n_dom) * l_), 1)));
   tmp->atl.resize(static_cast<size_t>(std::max(b.n_dom * l_ * l_, 1)));
   if (l_ > 0) {
-    assemble_u_kernel<<<dim3(16, b.n_dom), kT, 0, s_>>>(rs, l_, d_pts_.get(), b.ptr.get(), b.idx.get(),
+    assemble_u_kernel<<<dim3(16, b.n_dom), kT, 0, ls>>>(rs, l_, d_pts_.get(), b.ptr.get(), b.idx.get(),
                                                         tmp->u.get(), tmp->atl.get());
     PAM_LAUNCH_CHECK();
   }
-  assemble_m_kernel<<<dim3(64, b.n_dom), kT, 0, s_>>>(rs, l_, d_pts_.get(), b.ptr.get(), b.idx.get(), lag_.get(),
+  assemble_m_kernel<<<dim3(64, b.n_dom), kT, 0, ls>>>(rs, l_, d_pts_.get(), b.ptr.get(), b.idx.get(), lag_.get(),
                                                        tmp->u.get(), tmp->atl.get(), b.mat_ptr.get(), b.mat.get());
   PAM_LAUNCH_CHECK();
-  cholesky_kernel<<<b.n_dom, kT, 0, s_>>>(b.ptr.get(), l_, b.mat_ptr.get(), b.mat.get(), flag_.get());
-  PAM_LAUNCH_CHECK();
+  const int n0 = static_cast<int>(b.h_ptr.back() - l_);
+  if (b.n_dom == 1 && n0 > 4 * kTile) {
+    // One large matrix (the coarse grid): a workgroup per trailing tile
+    // instead of one workgroup for the whole factorisation.
+    for (int k0 = 0; k0 < n0; k0 += kNb) {
+      cholesky_panel_kernel<<<1, kT, 0, ls>>>(b.mat.get(), n0, k0, flag_.get());
+      PAM_LAUNCH_CHECK();
+      const int rem = n0 - std::min(kNb, n0 - k0) - k0;
+      if (rem <= 0) break;
+      const int nt = (rem + kTile - 1) / kTile;
+      cholesky_trailing_kernel<<<nt * (nt + 1) / 2, kT, 0, ls>>>(b.mat.get(), n0, k0);
+      PAM_LAUNCH_CHECK();
+    }
+  } else {
+    cholesky_kernel<<<b.n_dom, kT, 0, ls>>>(b.ptr.get(), l_, b.mat_ptr.get(), b.mat.get(), flag_.get());
+    PAM_LAUNCH_CHECK();
+  }
   // b.h_ptr is a member and outlives the upload; the rest stays until the
   // constructor synchronizes.
   setup_tmp_.push_back(std::move(tmp));

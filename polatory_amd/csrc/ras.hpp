@@ -73,7 +73,12 @@ class RasPreconditioner : public LinearOperator {
  private:
   // Assembles and factors a batch asynchronously; its temporaries stay in
   // setup_tmp_ until the end of the constructor.
-  void setup_batch(DomainBatch& b, const std::vector<Domain>& doms, bool coarse);
+  void setup_batch(DomainBatch& b, const std::vector<Domain>& doms, bool coarse, hipStream_t ls);
+  // Set-up streams: each level's batch (assembly, factorisation) runs on its
+  // own stream, forked from and joined back into s_.
+  hipStream_t fork_stream();
+  void join_streams();
+  std::vector<hipStream_t> setup_streams_;
   struct SetupTmp {
     std::vector<int32_t> idx;
     std::vector<uint8_t> inner;
