@@ -326,6 +326,9 @@ void Server::run_request(Slot& sl, Request& r) {
   if (nr > 1) sl.d_tmp.resize(static_cast<size_t>(n));
   while (sl.ev.size() < nr) {
     sl.ev.push_back(std::make_unique<FmmEvaluator>(m.owners[sl.ev.size()]->rbf(), nullptr, nullptr, false, s));
+    // Requests overlap across slots; a second stream per slot evaluator would
+    // only add hardware-queue sharing between the slots.
+    sl.ev.back()->set_overlap(false);
   }
   for (size_t k = 0; k < nr; k++) {
     FmmEvaluator& e = *sl.ev[k];
