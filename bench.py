@@ -295,7 +295,9 @@ def serve_section(pa, n_models, n_requests, rate):
             "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
             "max_ms": st["max_ms"], "mean_queue_ms": st["mean_queue_ms"],
             "points_per_s": st["points_per_s"], "completed": st["completed"], "failed": st["failed"],
-            "max_in_flight": st["max_in_flight"], "resident_gib": resident / 2**30,
+            "max_in_flight": st["max_in_flight"], "shared_direct_launches": st["batches"],
+            "requests_in_shared_launches": st["batched_requests"], "max_requests_per_launch": st["max_batch"],
+            "resident_gib": resident / 2**30,
             "max_rel_err_sampled": worst, "models_ready_s": t_models,
             "models_fit_to_1e-4": n_conv, "worst_fit_residual": worst_res,
             "service_ms_by_size": service, "model_configs": cfgs}

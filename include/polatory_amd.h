@@ -280,6 +280,11 @@ typedef struct {
   double points_per_s, wall_s;
   int max_in_flight;
   int64_t resident_bytes, budget, peak_admitted_bytes;
+  /* small requests (answered by the FP64 direct sum) of any model are served in
+     shared launches: the launches, the requests answered in them, the most
+     requests in one */
+  int64_t batches, batched_requests;
+  int max_batch;
 } pam_server_stats;
 int pam_server_get_stats(pam_server* h, pam_server_stats* out);
 /* Forgets completed requests and restarts the latency statistics. */

@@ -64,7 +64,8 @@ class PamServerStats(ctypes.Structure):
                 ("mean_queue_ms", ctypes.c_double), ("points_per_s", ctypes.c_double),
                 ("wall_s", ctypes.c_double), ("max_in_flight", ctypes.c_int),
                 ("resident_bytes", ctypes.c_int64), ("budget", ctypes.c_int64),
-                ("peak_admitted_bytes", ctypes.c_int64)]
+                ("peak_admitted_bytes", ctypes.c_int64), ("batches", ctypes.c_int64),
+                ("batched_requests", ctypes.c_int64), ("max_batch", ctypes.c_int)]
 
 
 class PamError(RuntimeError):

@@ -127,6 +127,9 @@ int pam_server_get_stats(pam_server* h, pam_server_stats* out) {
     out->resident_bytes = s.resident_bytes;
     out->budget = s.budget;
     out->peak_admitted_bytes = s.peak_admitted_bytes;
+    out->batches = s.batches;
+    out->batched_requests = s.batched_requests;
+    out->max_batch = s.max_batch;
   });
 }
 

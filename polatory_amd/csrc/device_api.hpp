@@ -1,6 +1,8 @@
 // Host-side launchers of the device kernels (one translation unit per family).
 #pragma once
 
+#include <vector>
+
 #include "common.hpp"
 
 namespace pam {
@@ -29,6 +31,37 @@ void launch_gather_f64(const double* in, const int32_t* idx, int64_t n, double* 
 void direct_sum_f64(const RbfDesc& rbf, const double* src, const double* w, int64_t n_src,
                     const double* trg, int64_t n_trg, double* out, bool accumulate,
                     DevBuf<double>& scratch, hipStream_t s);
+// Batched FP64 direct sums over host query sets, for small evaluations of
+// several resident models at once: one upload, one direct-sum launch, one
+// reduction, one download. A term is one rbf of a model (its transformed
+// sources and weights on the device); a query set sums its terms in order and
+// adds its polynomial. Each term is split into blocks and source chunks as in
+// direct_sum_f64, so a query set's result does not depend on the others in
+// the batch.
+struct DirectBatchTerm {
+  RbfDesc rbf;        // kind, constants and the anisotropy applied to the queries
+  const double* src;  // device, n_src x 3, transformed by rbf.aniso
+  const double* w;    // device, n_src
+  int64_t n_src;
+};
+struct DirectBatchQuery {
+  const double* pts;  // host, n x 3, untransformed
+  double* out;        // host, n
+  int64_t n;
+  int first_term, n_terms;
+  int poly_degree;        // -1: no polynomial
+  const double* coeffs;   // device, the polynomial coefficients
+};
+class DirectBatch {
+ public:
+  // Blocks until the outputs are written.
+  void run(const std::vector<DirectBatchQuery>& queries, const std::vector<DirectBatchTerm>& terms,
+           hipStream_t s);
+
+ private:
+  HostStage<double> up_, down_;
+  DevBuf<double> d_in_, d_part_, d_out_;
+};
 // out[i] += sum_k c_k m_k(p_i) (monomials of degree 0..2, untransformed points).
 void launch_poly_eval(int degree, const double* pts, int64_t n, const double* d_coeffs,
                       double* out, hipStream_t s);

@@ -63,6 +63,7 @@ struct Server::Slot {
   hipStream_t s = nullptr;
   std::vector<std::unique_ptr<FmmEvaluator>> ev;
   DevBuf<double> d_pts, d_out, d_tmp;
+  DirectBatch direct;
   ~Slot() {
     ev.clear();
     if (s) {
@@ -245,10 +246,22 @@ int64_t Server::submit(int model, const double* pts, int64_t n, double* out) {
   return id;
 }
 
+bool Server::batchable(const Request& r) {
+  // Requests that the evaluator would answer with the FP64 direct sum (below
+  // 2^26 source-target pairs, or a model fixed to direct mode).
+  if (r.n <= 0) return false;
+  const ModelEntry& m = *r.model;
+  if (m.cfg.mode == kModeDirect) return true;
+  if (m.cfg.mode != kModeAuto) return false;
+  return static_cast<double>(r.n) * static_cast<double>(m.n) < 67108864.0;
+}
+
 void Server::worker(int slot) {
   Slot& sl = *slots_[slot];
+  std::vector<Request*> batch;
   for (;;) {
     Request* r = nullptr;
+    batch.clear();
     {
       std::unique_lock<std::mutex> lk(mu_);
       for (;;) {
@@ -276,15 +289,36 @@ void Server::worker(int slot) {
         cv_work_.wait(lk);
       }
       admitted_ += r->est;
+      batch.push_back(r);
+      if (batchable(*r)) {
+        // Continuous batching: the queued requests that are also answered by
+        // the direct sum join this launch, whatever their model, in queue
+        // order, as far as the HBM budget and the batch limits allow.
+        const int64_t room = opt_.hbm_budget - resident_;
+        int64_t rows = r->n;
+        for (auto it = queue_.begin(); it != queue_.end() && batch.size() < kMaxBatch;) {
+          Request* q = *it;
+          if (batchable(*q) && rows + q->n <= kMaxBatchRows && admitted_ + q->est <= room) {
+            admitted_ += q->est;
+            rows += q->n;
+            batch.push_back(q);
+            it = queue_.erase(it);
+          } else {
+            ++it;
+          }
+        }
+      }
       peak_admitted_ = std::max(peak_admitted_, admitted_);
-      in_flight_++;
+      in_flight_ += static_cast<int>(batch.size());
       max_in_flight_ = std::max(max_in_flight_, in_flight_);
-      r->t_admit = now_s();
+      const double t = now_s();
+      for (Request* q : batch) q->t_admit = t;
     }
     int code = kOk;
     std::string err;
     try {
-      run_request(sl, *r);
+      if (batchable(*r)) run_batch(sl, batch);
+      else run_request(sl, *r);
     } catch (const Error& e) {
       code = e.code;
       err = e.what();
@@ -294,20 +328,28 @@ void Server::worker(int slot) {
     }
     {
       std::lock_guard<std::mutex> lk(mu_);
-      admitted_ -= r->est;
-      in_flight_--;
-      r->code = code;
-      r->err = err;
-      r->t_done = now_s();
-      r->done = true;
-      if (code == kOk) {
-        completed_++;
-        points_ += r->n;
-        lat_ms_.push_back((r->t_done - r->t_submit) * 1e3);
-        queue_ms_.push_back((r->t_admit - r->t_submit) * 1e3);
-        last_done_ = std::max(last_done_, r->t_done);
-      } else {
-        failed_++;
+      const double t = now_s();
+      if (batchable(*r)) {
+        batches_++;
+        batched_requests_ += static_cast<int64_t>(batch.size());
+        max_batch_ = std::max(max_batch_, static_cast<int>(batch.size()));
+      }
+      for (Request* q : batch) {
+        admitted_ -= q->est;
+        in_flight_--;
+        q->code = code;
+        q->err = err;
+        q->t_done = t;
+        q->done = true;
+        if (code == kOk) {
+          completed_++;
+          points_ += q->n;
+          lat_ms_.push_back((q->t_done - q->t_submit) * 1e3);
+          queue_ms_.push_back((q->t_admit - q->t_submit) * 1e3);
+          last_done_ = std::max(last_done_, q->t_done);
+        } else {
+          failed_++;
+        }
       }
     }
     cv_done_.notify_all();
@@ -340,6 +382,26 @@ void Server::run_request(Slot& sl, Request& r) {
   if (m.model.l() > 0) launch_poly_eval(m.model.poly_degree, sl.d_pts.get(), n, m.d_coeffs.get(), sl.d_out.get(), s);
   sl.d_out.download(r.out, static_cast<size_t>(n), s);
   PAM_HIP(hipStreamSynchronize(s));
+}
+
+void Server::run_batch(Slot& sl, const std::vector<Request*>& batch) {
+  std::vector<DirectBatchQuery> qs;
+  std::vector<DirectBatchTerm> terms;
+  for (Request* r : batch) {
+    const ModelEntry& m = *r->model;
+    DirectBatchQuery q{};
+    q.pts = r->pts;
+    q.out = r->out;
+    q.n = r->n;
+    q.first_term = static_cast<int>(terms.size());
+    q.n_terms = static_cast<int>(m.owners.size());
+    q.poly_degree = m.model.l() > 0 ? m.model.poly_degree : -1;
+    q.coeffs = m.d_coeffs.get();
+    qs.push_back(q);
+    for (const auto& e : m.owners)
+      terms.push_back(DirectBatchTerm{e->rbf(), e->d_source_points(), e->d_weights(), m.n});
+  }
+  sl.direct.run(qs, terms, sl.s);
 }
 
 void Server::wait(int64_t id) {
@@ -396,6 +458,9 @@ ServeStats Server::stats() {
   st.resident_bytes = resident_;
   st.budget = opt_.hbm_budget;
   st.peak_admitted_bytes = peak_admitted_;
+  st.batches = batches_;
+  st.batched_requests = batched_requests_;
+  st.max_batch = max_batch_;
   return st;
 }
 
@@ -408,6 +473,8 @@ void Server::reset_stats() {
   lat_ms_.clear();
   queue_ms_.clear();
   completed_ = failed_ = submitted_ = points_ = 0;
+  batches_ = batched_requests_ = 0;
+  max_batch_ = 0;
   first_submit_ = -1;
   last_done_ = 0;
   max_in_flight_ = in_flight_;

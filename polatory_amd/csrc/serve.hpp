@@ -9,6 +9,9 @@
 // budget holds their working set. Every slot is a worker thread with its own
 // stream whose evaluators attach to the model's source side, so the
 // requests in flight overlap on the device while sharing the resident data.
+// Requests that the FP64 direct sum answers (few query points against the
+// model's centres) are batched: a slot takes the queued ones of any model
+// into one shared direct-sum launch and one reduction.
 #pragma once
 
 #include <atomic>
@@ -37,6 +40,10 @@ struct ServeStats {
   double points_per_s = 0, wall_s = 0;
   int max_in_flight = 0;
   int64_t resident_bytes = 0, budget = 0, peak_admitted_bytes = 0;
+  // shared direct-sum launches, the requests answered in them, and the most
+  // requests in one
+  int64_t batches = 0, batched_requests = 0;
+  int max_batch = 0;
 };
 
 class Server {
@@ -73,6 +80,12 @@ class Server {
   void worker(int slot);
   int64_t estimate_bytes(const ModelEntry& m, int64_t n) const;
   void run_request(Slot& slot, Request& r);
+  void run_batch(Slot& slot, const std::vector<Request*>& batch);
+  static bool batchable(const Request& r);
+
+  // Limits of one shared direct-sum launch.
+  static constexpr size_t kMaxBatch = 64;
+  static constexpr int64_t kMaxBatchRows = int64_t{1} << 19;
 
   ServeOptions opt_;
   std::vector<std::unique_ptr<ModelEntry>> models_;
@@ -89,6 +102,8 @@ class Server {
   int in_flight_ = 0, max_in_flight_ = 0;
   std::vector<double> lat_ms_, queue_ms_;
   int64_t completed_ = 0, failed_ = 0, submitted_ = 0, points_ = 0;
+  int64_t batches_ = 0, batched_requests_ = 0;
+  int max_batch_ = 0;
   double first_submit_ = -1, last_done_ = 0;
 };
 

@@ -1,7 +1,9 @@
 // Generic device building blocks: point transforms, bbox reduction, prefix
 // sums, a stable LSD radix sort (4-bit digits), gathers, polynomial terms and
 // the FP64 direct O(N*M) sum.
+#include <cstring>
 #include <type_traits>
+#include <vector>
 
 #include "device_api.hpp"
 #include "rbf_device.hpp"
@@ -257,6 +259,30 @@ __global__ void gather_f64_kernel(const double* __restrict__ in, const int32_t* 
 // ---- FP64 direct sum: targets x source chunks, deterministic two-stage sum ----
 constexpr int kDirThreads = 256;
 
+// One workgroup's share of an FP64 direct sum: the thread's target (tx, ty,
+// tz) against sources [j0, j1), staged through LDS kDirThreads at a time and
+// summed in source order.
+template <int KIND>
+__device__ __forceinline__ double direct_f64_range(double4* tile, double pa, double pb,
+                                                   const double* __restrict__ src,
+                                                   const double* __restrict__ w, int64_t j0, int64_t j1,
+                                                   double tx, double ty, double tz) {
+  double acc = 0.0;
+  for (int64_t jt = j0; jt < j1; jt += kDirThreads) {
+    __syncthreads();
+    int64_t j = jt + threadIdx.x;
+    if (j < j1) tile[threadIdx.x] = make_double4(src[3 * j], src[3 * j + 1], src[3 * j + 2], w[j]);
+    __syncthreads();
+    int cnt = static_cast<int>((j1 - jt) < kDirThreads ? (j1 - jt) : kDirThreads);
+    for (int k = 0; k < cnt; k++) {
+      double4 q = tile[k];
+      double dx = tx - q.x, dy = ty - q.y, dz = tz - q.z;
+      acc += q.w * phi_f64<KIND>(dx * dx + dy * dy + dz * dz, pa, pb);
+    }
+  }
+  return acc;
+}
+
 template <int KIND>
 __global__ void __launch_bounds__(kDirThreads)
 direct_f64_kernel(double pa, double pb, const double* __restrict__ src, const double* __restrict__ w,
@@ -272,19 +298,7 @@ direct_f64_kernel(double pa, double pb, const double* __restrict__ src, const do
     ty = trg[3 * i + 1];
     tz = trg[3 * i + 2];
   }
-  double acc = 0.0;
-  for (int64_t jt = j0; jt < j1; jt += kDirThreads) {
-    __syncthreads();
-    int64_t j = jt + threadIdx.x;
-    if (j < j1) tile[threadIdx.x] = make_double4(src[3 * j], src[3 * j + 1], src[3 * j + 2], w[j]);
-    __syncthreads();
-    int cnt = static_cast<int>((j1 - jt) < kDirThreads ? (j1 - jt) : kDirThreads);
-    for (int k = 0; k < cnt; k++) {
-      double4 q = tile[k];
-      double dx = tx - q.x, dy = ty - q.y, dz = tz - q.z;
-      acc += q.w * phi_f64<KIND>(dx * dx + dy * dy + dz * dz, pa, pb);
-    }
-  }
+  double acc = direct_f64_range<KIND>(tile, pa, pb, src, w, j0, j1, tx, ty, tz);
   if (i < n_trg) partial[static_cast<int64_t>(blockIdx.y) * n_trg + i] = acc;
 }
 
@@ -297,15 +311,109 @@ __global__ void reduce_partial_kernel(const double* __restrict__ partial, int nc
   out[i] = accumulate ? out[i] + acc : acc;
 }
 
+// ---- batched FP64 direct sums: several query sets, each against its own
+// models' sources, in one launch and one reduction ----
+//
+// A segment is one (query set, rbf) pair. It is split into kDirThreads-target
+// blocks and source chunks by the same rule as a single direct sum, so its
+// partial sums, and hence every query set's result, do not depend on which
+// other query sets share the launch.
+struct DirSeg {
+  const double* src;  // transformed sources, n_src x 3
+  const double* w;    // weights, n_src
+  int64_t n_src, chunk;
+  int64_t part_off;   // its partial sums: nchunks x q_n
+  int64_t q_off, q_n; // rows of the concatenated query points
+  int32_t kind, gx, nchunks, work_first;
+  double pa, pb;
+  double a[9];        // anisotropy applied to the query points on load
+};
+static_assert(sizeof(DirSeg) % sizeof(double) == 0, "DirSeg is staged as doubles");
+
+struct DirQuery {
+  int64_t off, n;
+  int32_t seg_first, n_seg, degree, pad;
+  const double* coeffs;
+};
+static_assert(sizeof(DirQuery) % sizeof(double) == 0, "DirQuery is staged as doubles");
+
+__global__ void __launch_bounds__(kDirThreads)
+direct_batch_kernel(const DirSeg* __restrict__ segs, int nseg, const double* __restrict__ pts,
+                    double* __restrict__ partial) {
+  __shared__ double4 tile[kDirThreads];
+  const int b = static_cast<int>(blockIdx.x);
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (segs[mid].work_first <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const DirSeg& g = segs[lo];
+  const int local = b - g.work_first;
+  const int bx = local % g.gx, by = local / g.gx;
+  const int64_t i = static_cast<int64_t>(bx) * kDirThreads + threadIdx.x;
+  const int64_t j0 = static_cast<int64_t>(by) * g.chunk;
+  const int64_t j1 = j0 + g.chunk < g.n_src ? j0 + g.chunk : g.n_src;
+  double tx = 0, ty = 0, tz = 0;
+  if (i < g.q_n) {
+    const double* p = pts + 3 * (g.q_off + i);
+    const double x = p[0], y = p[1], z = p[2];
+    // the rounding of transform_kernel
+    tx = __dadd_rn(__dadd_rn(__dmul_rn(g.a[0], x), __dmul_rn(g.a[1], y)), __dmul_rn(g.a[2], z));
+    ty = __dadd_rn(__dadd_rn(__dmul_rn(g.a[3], x), __dmul_rn(g.a[4], y)), __dmul_rn(g.a[5], z));
+    tz = __dadd_rn(__dadd_rn(__dmul_rn(g.a[6], x), __dmul_rn(g.a[7], y)), __dmul_rn(g.a[8], z));
+  }
+  double acc;
+  switch (g.kind) {
+    case kBh3: acc = direct_f64_range<kBh3>(tile, g.pa, g.pb, g.src, g.w, j0, j1, tx, ty, tz); break;
+    case kTh3: acc = direct_f64_range<kTh3>(tile, g.pa, g.pb, g.src, g.w, j0, j1, tx, ty, tz); break;
+    default: acc = direct_f64_range<kExp>(tile, g.pa, g.pb, g.src, g.w, j0, j1, tx, ty, tz); break;
+  }
+  if (i < g.q_n) partial[g.part_off + static_cast<int64_t>(by) * g.q_n + i] = acc;
+}
+
+// sum_k c_k m_k(x, y, z) over the monomials of degree 0..2
+// (polynomial/monomial_basis.hpp:198-209 order: 1, x, y, z, x^2, xy, xz, y^2, yz, z^2).
+__device__ __forceinline__ double poly_value(int degree, const double* __restrict__ c, double x, double y,
+                                             double z) {
+  double acc = c[0];
+  if (degree >= 1) acc += c[1] * x + c[2] * y + c[3] * z;
+  if (degree >= 2) acc += c[4] * x * x + c[5] * x * y + c[6] * x * z + c[7] * y * y + c[8] * y * z + c[9] * z * z;
+  return acc;
+}
+
 __global__ void poly_eval_kernel(int degree, const double* __restrict__ pts, int64_t n,
                                  const double* __restrict__ c, double* __restrict__ out) {
   int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
-  double acc = c[0];
-  if (degree >= 1) acc += c[1] * x + c[2] * y + c[3] * z;
-  if (degree >= 2) acc += c[4] * x * x + c[5] * x * y + c[6] * x * z + c[7] * y * y + c[8] * y * z + c[9] * z * z;
-  out[i] += acc;
+  out[i] += poly_value(degree, c, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+}
+
+// One thread per query row: each segment's chunks summed in chunk order, the
+// segments of a query set added in rbf order, then its polynomial.
+__global__ void direct_batch_reduce_kernel(const DirQuery* __restrict__ qs, int nq,
+                                           const DirSeg* __restrict__ segs, const double* __restrict__ pts,
+                                           const double* __restrict__ partial, double* __restrict__ out,
+                                           int64_t rows) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= rows) return;
+  int lo = 0, hi = nq - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (qs[mid].off <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  const DirQuery& q = qs[lo];
+  const int64_t i = t - q.off;
+  double v = 0.0;
+  for (int k = 0; k < q.n_seg; k++) {
+    const DirSeg& g = segs[q.seg_first + k];
+    double acc = 0.0;
+    for (int c = 0; c < g.nchunks; c++) acc += partial[g.part_off + static_cast<int64_t>(c) * g.q_n + i];
+    v = k == 0 ? acc : v + acc;
+  }
+  if (q.degree >= 0) v += poly_value(q.degree, q.coeffs, pts[3 * t], pts[3 * t + 1], pts[3 * t + 2]);
+  out[t] = v;
 }
 
 }  // namespace
@@ -384,17 +492,29 @@ void launch_gather_f64(const double* in, const int32_t* idx, int64_t n, double* 
   PAM_LAUNCH_CHECK();
 }
 
+namespace {
+
+// Targets in blocks of kDirThreads; sources split so that the grid has enough
+// workgroups to fill 256 CUs, at least 1024 sources per chunk.
+void direct_split(int64_t n_trg, int64_t n_src, int* gx, int64_t* chunk, int* nchunks) {
+  *gx = ceil_div(n_trg, kDirThreads);
+  int64_t want_chunks = (2048 + *gx - 1) / *gx;
+  int64_t c = n_src / (want_chunks > 0 ? want_chunks : 1);
+  if (c < 1024) c = 1024;
+  *chunk = c;
+  *nchunks = static_cast<int>(n_src > 0 ? (n_src + c - 1) / c : 1);
+}
+
+}  // namespace
+
 void direct_sum_f64(const RbfDesc& rbf, const double* src, const double* w, int64_t n_src,
                     const double* trg, int64_t n_trg, double* out, bool accumulate,
                     DevBuf<double>& scratch, hipStream_t s) {
   if (n_trg == 0) return;
   PhiConsts pc = phi_consts(rbf, 1.0);
-  int gx = ceil_div(n_trg, kDirThreads);
-  // Split sources so the grid has enough workgroups to fill 256 CUs.
-  int64_t want_chunks = (2048 + gx - 1) / gx;
-  int64_t chunk = n_src / (want_chunks > 0 ? want_chunks : 1);
-  if (chunk < 1024) chunk = 1024;
-  int nchunks = static_cast<int>(n_src > 0 ? (n_src + chunk - 1) / chunk : 1);
+  int gx, nchunks;
+  int64_t chunk;
+  direct_split(n_trg, n_src, &gx, &chunk, &nchunks);
   scratch.resize(static_cast<size_t>(nchunks) * n_trg);
   if (n_src == 0) {
     PAM_HIP(hipMemsetAsync(scratch.get(), 0, sizeof(double) * n_trg, s));
@@ -416,6 +536,87 @@ void launch_poly_eval(int degree, const double* pts, int64_t n, const double* d_
   if (n == 0 || degree < 0) return;
   poly_eval_kernel<<<ceil_div(n, 256), 256, 0, s>>>(degree, pts, n, d_coeffs, out);
   PAM_LAUNCH_CHECK();
+}
+
+void DirectBatch::run(const std::vector<DirectBatchQuery>& queries,
+                      const std::vector<DirectBatchTerm>& terms, hipStream_t s) {
+  std::vector<DirSeg> segs;
+  std::vector<DirQuery> qs;
+  int64_t rows = 0, part = 0, work = 0;
+  for (const DirectBatchQuery& q : queries) {
+    if (q.n < 0) throw_invalid("negative query count");
+    if (q.n == 0) continue;
+    if (!q.pts || !q.out) throw_invalid("null query or output buffer");
+    if (q.n_terms < 1 || q.first_term < 0 || q.first_term + q.n_terms > static_cast<int>(terms.size()))
+      throw_invalid("query terms out of range");
+    if (q.poly_degree > 2) throw_invalid("polynomial degree must be at most 2");
+    if (q.poly_degree >= 0 && !q.coeffs) throw_invalid("null polynomial coefficients");
+    DirQuery d{};
+    d.off = rows;
+    d.n = q.n;
+    d.seg_first = static_cast<int32_t>(segs.size());
+    d.n_seg = q.n_terms;
+    d.degree = q.poly_degree;
+    d.coeffs = q.coeffs;
+    qs.push_back(d);
+    for (int k = q.first_term; k < q.first_term + q.n_terms; k++) {
+      const DirectBatchTerm& t = terms[k];
+      if (t.n_src < 1 || !t.src || !t.w) throw_invalid("a batched term needs sources and weights");
+      DirSeg g{};
+      g.src = t.src;
+      g.w = t.w;
+      g.n_src = t.n_src;
+      direct_split(q.n, t.n_src, &g.gx, &g.chunk, &g.nchunks);
+      g.part_off = part;
+      g.q_off = rows;
+      g.q_n = q.n;
+      g.kind = t.rbf.kind;
+      g.work_first = static_cast<int32_t>(work);
+      PhiConsts pc = phi_consts(t.rbf, 1.0);
+      g.pa = pc.a;
+      g.pb = pc.b;
+      for (int e = 0; e < 9; e++) g.a[e] = t.rbf.aniso[e];
+      segs.push_back(g);
+      part += static_cast<int64_t>(g.nchunks) * q.n;
+      work += static_cast<int64_t>(g.gx) * g.nchunks;
+      if (work > (int64_t{1} << 30)) throw_invalid("batched direct sum too large for one launch");
+    }
+    rows += q.n;
+  }
+  if (rows == 0) return;
+  // One upload: segment table, query table, then the query points.
+  const size_t seg_d = segs.size() * sizeof(DirSeg) / sizeof(double);
+  const size_t q_d = qs.size() * sizeof(DirQuery) / sizeof(double);
+  const size_t total = seg_d + q_d + static_cast<size_t>(rows) * 3;
+  double* h = up_.acquire(total);
+  std::memcpy(h, segs.data(), segs.size() * sizeof(DirSeg));
+  std::memcpy(h + seg_d, qs.data(), qs.size() * sizeof(DirQuery));
+  double* hp = h + seg_d + q_d;
+  for (const DirectBatchQuery& q : queries) {
+    if (q.n == 0) continue;
+    std::memcpy(hp, q.pts, sizeof(double) * 3 * static_cast<size_t>(q.n));
+    hp += 3 * q.n;
+  }
+  up_.upload(d_in_, total, s);
+  const DirSeg* d_segs = reinterpret_cast<const DirSeg*>(d_in_.get());
+  const DirQuery* d_qs = reinterpret_cast<const DirQuery*>(d_in_.get() + seg_d);
+  const double* d_pts = d_in_.get() + seg_d + q_d;
+  d_part_.resize(static_cast<size_t>(part));
+  d_out_.resize(static_cast<size_t>(rows));
+  direct_batch_kernel<<<static_cast<unsigned>(work), kDirThreads, 0, s>>>(d_segs, static_cast<int>(segs.size()),
+                                                                          d_pts, d_part_.get());
+  PAM_LAUNCH_CHECK();
+  direct_batch_reduce_kernel<<<ceil_div(rows, 256), 256, 0, s>>>(d_qs, static_cast<int>(qs.size()), d_segs, d_pts,
+                                                                d_part_.get(), d_out_.get(), rows);
+  PAM_LAUNCH_CHECK();
+  double* ho = down_.acquire(static_cast<size_t>(rows));
+  PAM_HIP(hipMemcpyAsync(ho, d_out_.get(), sizeof(double) * rows, hipMemcpyDeviceToHost, s));
+  PAM_HIP(hipStreamSynchronize(s));
+  for (const DirectBatchQuery& q : queries) {
+    if (q.n == 0) continue;
+    std::memcpy(q.out, ho, sizeof(double) * static_cast<size_t>(q.n));
+    ho += q.n;
+  }
 }
 
 }  // namespace pam

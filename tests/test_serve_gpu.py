@@ -132,3 +132,35 @@ def test_empty_request(served):
     srv, _, ids = served
     out = srv.wait(srv.submit(ids[0], np.zeros((0, 3))))
     assert out.shape == (0,)
+
+
+def test_small_requests_share_launches(orc, served):
+    """Small requests (answered by the FP64 direct sum) queued behind large ones
+    are served several models to a launch; each answer is the same bits as the
+    same request alone and agrees with the oracle's FP64 direct sum."""
+    srv, models, ids = served
+    rng = np.random.default_rng(81)
+    bigq = [rng.uniform(0, 1, (250_000, 3)) for _ in range(4)]
+    small = []
+    for k in range(48):
+        m = k % len(ids)
+        n = int(rng.integers(1, 300 if m == 3 else 3000))
+        small.append((m, rng.uniform(0, 1, (n, 3))))
+    srv.reset_stats()
+    big = [srv.submit(ids[3], q) for q in bigq]
+    rids = [srv.submit(ids[m], q) for m, q in small]
+    mixed = [srv.wait(r) for r in rids]
+    for r in big:
+        srv.wait(r)
+    st = srv.stats()
+    log(f"serve: {len(small)} small requests in {st['batches']} shared launches "
+        f"(largest {st['max_batch']})")
+    assert st["batched_requests"] == len(small)
+    assert st["max_batch"] >= 2, "no two small requests shared a launch"
+    for i, (m, q) in enumerate(small):
+        alone = srv.wait(srv.submit(ids[m], q))
+        assert np.array_equal(alone, mixed[i]), f"request {i} differs alone vs batched"
+        ref = reference(orc, models[m], q)
+        err = np.abs(mixed[i] - ref).max() / max(np.abs(ref).max(), 1e-300)
+        # FP64 sums in another order than the oracle's: rounding only
+        assert err <= 1e-10, f"request {i}: rel err {err:.2e} vs the FP64 direct sum"
