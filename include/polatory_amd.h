@@ -169,6 +169,16 @@ int pam_model_destroy(pam_model* m);
 /* poly_degree() and poly_basis_size() (model.hpp:126-131) */
 int pam_model_poly_degree(const pam_model* m, int* degree);
 int pam_model_poly_basis_size(const pam_model* m, int* l);
+/* rbfs() and nugget() (model.hpp:79, 131-133) */
+int pam_model_num_rbfs(const pam_model* m, int* n);
+int pam_model_rbf(const pam_model* m, int i, pam_rbf* out);
+int pam_model_nugget(const pam_model* m, double* nugget);
+/* Model::save / Model::load (model.hpp:169, POLATORY_IMPLEMENT_LOAD_SAVE in
+ * common/io.hpp:147-187): polatory's binary model file, field order
+ * model.hpp:304-320 and rbf/rbf_io.hpp:9-35. Loading an rbf that polatory knows
+ * but this library does not implement returns PAM_ERR_NOT_IMPLEMENTED. */
+int pam_model_save(const pam_model* m, const char* filename);
+int pam_model_load(const char* filename, pam_model** out);
 
 /* ---------------------------------------------------------------------------
  * Interpolant (include/polatory/interpolant.hpp:34-367): fit and evaluate.
@@ -211,6 +221,14 @@ int pam_interpolant_weights(const pam_interpolant* h, double* out);
 /* Installs centres and weights (e.g. from a saved interpolant, interpolant.hpp:345-367). */
 int pam_interpolant_set_fitted(pam_interpolant* h, const double* centers, int64_t n,
                                const double* weights);
+/* Interpolant::save / Interpolant::load (interpolant.hpp:218; field order
+ * interpolant.hpp:345-367): model, fitted, centres, gradient centres, bbox,
+ * weights. Files with gradient centres return PAM_ERR_NOT_IMPLEMENTED. */
+int pam_interpolant_save(const pam_interpolant* h, const char* filename);
+int pam_interpolant_load(const char* filename, pam_interpolant** out);
+/* model() (interpolant.hpp:199) as a new handle the caller destroys */
+int pam_interpolant_model(const pam_interpolant* h, pam_model** out);
+int pam_interpolant_fitted(const pam_interpolant* h, int* fitted);
 typedef struct {
   int iterations, restarts, n_levels;
   double residual, seconds, setup_seconds, precond_setup_seconds;

@@ -78,6 +78,10 @@ class InvalidArgument(PamError, ValueError):
     """std::invalid_argument in the reference API."""
 
 
+class NotImplementedByLibrary(PamError, NotImplementedError):
+    """A polatory feature outside this library (an rbf or gradient data in a file)."""
+
+
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
 _D = ctypes.c_double
@@ -122,6 +126,11 @@ _SIGS = {
     "pam_model_destroy": [_P],
     "pam_model_poly_degree": [_P, ctypes.POINTER(_INT)],
     "pam_model_poly_basis_size": [_P, ctypes.POINTER(_INT)],
+    "pam_model_num_rbfs": [_P, ctypes.POINTER(_INT)],
+    "pam_model_rbf": [_P, _INT, _P],
+    "pam_model_nugget": [_P, ctypes.POINTER(ctypes.c_double)],
+    "pam_model_save": [_P, ctypes.c_char_p],
+    "pam_model_load": [ctypes.c_char_p, ctypes.POINTER(_P)],
     "pam_fit_options_default": [ctypes.POINTER(PamFitOptions)],
     "pam_interpolant_create": [_P, ctypes.POINTER(_P)],
     "pam_interpolant_destroy": [_P],
@@ -133,6 +142,10 @@ _SIGS = {
     "pam_interpolant_centers": [_P, _P],
     "pam_interpolant_weights": [_P, _P],
     "pam_interpolant_set_fitted": [_P, _P, _I64, _P],
+    "pam_interpolant_save": [_P, ctypes.c_char_p],
+    "pam_interpolant_load": [ctypes.c_char_p, ctypes.POINTER(_P)],
+    "pam_interpolant_model": [_P, ctypes.POINTER(_P)],
+    "pam_interpolant_fitted": [_P, ctypes.POINTER(_INT)],
     "pam_interpolant_fit_stats": [_P, ctypes.POINTER(PamFitStats)],
     "pam_interpolant_residual_history": [_P, _P, ctypes.POINTER(_I64)],
     "pam_fgmres_dense": [_P, _P, _I64, _P, _P, _INT, _P, _P, ctypes.POINTER(_INT)],
@@ -191,6 +204,8 @@ def check(code):
         msg = last_error()
         if code == 1:
             raise InvalidArgument(code, msg)
+        if code == 5:
+            raise NotImplementedByLibrary(code, msg)
         raise PamError(code, msg)
 
 

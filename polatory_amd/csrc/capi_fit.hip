@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "capi_handles.hpp"
+#include "model_io.hpp"
 #include "ras.hpp"
 
 struct pam_ras {
@@ -126,6 +127,48 @@ int pam_model_poly_basis_size(const pam_model* m, int* l) {
   });
 }
 
+int pam_model_num_rbfs(const pam_model* m, int* n) {
+  return guard([&] {
+    if (!m || !n) pam::throw_invalid("null argument");
+    *n = static_cast<int>(m->m.rbfs.size());
+  });
+}
+
+int pam_model_rbf(const pam_model* m, int i, pam_rbf* out) {
+  return guard([&] {
+    if (!m || !out) pam::throw_invalid("null argument");
+    if (i < 0 || i >= static_cast<int>(m->m.rbfs.size())) pam::throw_invalid("rbf index out of range");
+    const pam::RbfDesc& r = m->m.rbfs[i];
+    out->kind = r.kind;
+    out->params[0] = r.p0;
+    out->params[1] = r.p1;
+    for (int k = 0; k < 9; k++) out->aniso[k] = r.aniso[k];
+  });
+}
+
+int pam_model_nugget(const pam_model* m, double* nugget) {
+  return guard([&] {
+    if (!m || !nugget) pam::throw_invalid("null argument");
+    *nugget = m->m.nugget;
+  });
+}
+
+int pam_model_save(const pam_model* m, const char* filename) {
+  return guard([&] {
+    if (!m || !filename) pam::throw_invalid("null argument");
+    pam::save_model(filename, m->m);
+  });
+}
+
+int pam_model_load(const char* filename, pam_model** out) {
+  return guard([&] {
+    if (!filename || !out) pam::throw_invalid("null argument");
+    auto m = std::make_unique<pam_model>();
+    m->m = pam::load_model(filename);
+    *out = m.release();
+  });
+}
+
 int pam_fit_options_default(pam_fit_options* opt) {
   return guard([&] {
     auto o = default_fit_options();
@@ -209,6 +252,38 @@ int pam_interpolant_set_fitted(pam_interpolant* h, const double* centers, int64_
     int l = ip(h).model().l();
     std::vector<double> c(centers, centers + 3 * n), w(weights, weights + n + l);
     ip(h).set_fitted(c, w);
+  });
+}
+
+int pam_interpolant_save(const pam_interpolant* h, const char* filename) {
+  return guard([&] {
+    if (!h || !filename) pam::throw_invalid("null argument");
+    pam::save_interpolant(filename, *h->impl);
+  });
+}
+
+int pam_interpolant_load(const char* filename, pam_interpolant** out) {
+  return guard([&] {
+    if (!filename || !out) pam::throw_invalid("null argument");
+    auto h = std::make_unique<pam_interpolant>();
+    h->impl = pam::load_interpolant(filename);
+    *out = h.release();
+  });
+}
+
+int pam_interpolant_model(const pam_interpolant* h, pam_model** out) {
+  return guard([&] {
+    if (!h || !out) pam::throw_invalid("null argument");
+    auto m = std::make_unique<pam_model>();
+    m->m = h->impl->model();
+    *out = m.release();
+  });
+}
+
+int pam_interpolant_fitted(const pam_interpolant* h, int* fitted) {
+  return guard([&] {
+    if (!h || !fitted) pam::throw_invalid("null argument");
+    *fitted = h->impl->fitted() ? 1 : 0;
   });
 }
 

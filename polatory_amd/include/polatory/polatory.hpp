@@ -136,6 +136,13 @@ class Rbf {
   const std::vector<double>& parameters() const { return params_; }
   int cpd_order() const { return kind_ == PAM_RBF_BH3 ? 1 : (kind_ == PAM_RBF_TH3 ? 2 : 0); }
   std::string short_name() const { return kind_ == PAM_RBF_BH3 ? "bh3" : (kind_ == PAM_RBF_TH3 ? "th3" : "exp"); }
+  static Rbf from_c(const pam_rbf& c) {
+    Rbf r(c.kind, {c.params[0], c.params[1]});
+    Mat<Dim> a;
+    for (int i = 0; i < 9; i++) a[i] = c.aniso[i];
+    r.set_anisotropy(a);
+    return r;
+  }
   pam_rbf to_c() const {
     pam_rbf r{};
     r.kind = kind_;
@@ -224,6 +231,32 @@ class Model {
   const std::vector<rbf::Rbf<Dim>>& rbfs() const { return rbfs_; }
   const pam_model* handle() const { return h_; }
 
+  // Model::save / Model::load (model.hpp:169): polatory's binary model file.
+  void save(const std::string& filename) const { detail::check(pam_model_save(h_, filename.c_str())); }
+  static Model load(const std::string& filename) {
+    pam_model* h = nullptr;
+    detail::check(pam_model_load(filename.c_str(), &h));
+    std::unique_ptr<pam_model, int (*)(pam_model*)> g(h, pam_model_destroy);
+    return from_handle(h);
+  }
+  // A copy of the model behind a C handle.
+  static Model from_handle(const pam_model* h) {
+    int n = 0, degree = 0;
+    double nugget = 0.0;
+    detail::check(pam_model_num_rbfs(h, &n));
+    std::vector<rbf::Rbf<Dim>> rbfs;
+    for (int i = 0; i < n; i++) {
+      pam_rbf c{};
+      detail::check(pam_model_rbf(h, i, &c));
+      rbfs.push_back(rbf::Rbf<Dim>::from_c(c));
+    }
+    detail::check(pam_model_poly_degree(h, &degree));
+    detail::check(pam_model_nugget(h, &nugget));
+    Model m(std::move(rbfs), degree);
+    m.set_nugget(nugget);
+    return m;
+  }
+
  private:
   void rebuild() {
     if (h_) pam_model_destroy(h_);
@@ -286,7 +319,29 @@ class Interpolant {
 
   const Model<Dim>& model() const { return model_; }
 
+  bool fitted() const {
+    int f = 0;
+    detail::check(pam_interpolant_fitted(h_, &f));
+    return f != 0;
+  }
+
+  // Interpolant::save / Interpolant::load (interpolant.hpp:218): polatory's
+  // binary interpolant file.
+  void save(const std::string& filename) const { detail::check(pam_interpolant_save(h_, filename.c_str())); }
+  static Interpolant load(const std::string& filename) {
+    pam_interpolant* h = nullptr;
+    detail::check(pam_interpolant_load(filename.c_str(), &h));
+    std::unique_ptr<pam_interpolant, int (*)(pam_interpolant*)> ih(h, pam_interpolant_destroy);
+    pam_model* mh = nullptr;
+    detail::check(pam_interpolant_model(h, &mh));
+    std::unique_ptr<pam_model, int (*)(pam_model*)> mg(mh, pam_model_destroy);
+    Model<Dim> m = Model<Dim>::from_handle(mh);
+    return Interpolant(ih.release(), m);
+  }
+
  private:
+  Interpolant(pam_interpolant* h, const Model<Dim>& model) : model_(model), h_(h) {}
+
   Model<Dim> model_;
   pam_interpolant* h_ = nullptr;
 };

@@ -7,12 +7,13 @@ fgmres_dense   (test seam) src/krylov/fgmres.cpp, gmres.cpp
 """
 import ctypes
 import math
+import os
 
 import numpy as np
 
 from . import _capi
 from .fmm import FmmConfig, _points, _ptr
-from .rbf import Rbf
+from .rbf import Rbf, from_c
 
 POLY_DEGREE_MIN = -2  # Model::kMinRequiredPolyDegree
 
@@ -56,6 +57,34 @@ class Model:
     def cpd_order(self):
         return max(r.cpd_order for r in self.rbfs)
 
+    # Model::save / Model::load (model.hpp:169; polatory's binary model file)
+    def save(self, filename):
+        _capi.call("pam_model_save", self._h, os.fsencode(filename))
+
+    @staticmethod
+    def load(filename):
+        h = ctypes.c_void_p()
+        _capi.call("pam_model_load", os.fsencode(filename), ctypes.byref(h))
+        try:
+            return Model._from_handle(h)
+        finally:
+            _capi.load().pam_model_destroy(h)
+
+    @staticmethod
+    def _from_handle(h):
+        n = ctypes.c_int()
+        _capi.call("pam_model_num_rbfs", h, ctypes.byref(n))
+        rbfs = []
+        for i in range(n.value):
+            r = _capi.PamRbf()
+            _capi.call("pam_model_rbf", h, i, ctypes.byref(r))
+            rbfs.append(from_c(r))
+        d = ctypes.c_int()
+        _capi.call("pam_model_poly_degree", h, ctypes.byref(d))
+        nug = ctypes.c_double()
+        _capi.call("pam_model_nugget", h, ctypes.byref(nug))
+        return Model(rbfs, poly_degree=d.value, nugget=nug.value)
+
 
 class FitOptions:
     """Configuration of the device fit (no reference counterpart; defaults follow
@@ -96,6 +125,29 @@ class Interpolant:
             except Exception:
                 pass
             self._h = None
+
+    # Interpolant::save / Interpolant::load (interpolant.hpp:218; polatory's binary file)
+    def save(self, filename):
+        _capi.call("pam_interpolant_save", self._h, os.fsencode(filename))
+
+    @staticmethod
+    def load(filename):
+        h = ctypes.c_void_p()
+        _capi.call("pam_interpolant_load", os.fsencode(filename), ctypes.byref(h))
+        ip = Interpolant.__new__(Interpolant)
+        ip._h = h
+        mh = ctypes.c_void_p()
+        _capi.call("pam_interpolant_model", h, ctypes.byref(mh))
+        try:
+            ip.model = Model._from_handle(mh)
+        finally:
+            _capi.load().pam_model_destroy(mh)
+        return ip
+
+    def fitted(self):
+        f = ctypes.c_int()
+        _capi.call("pam_interpolant_fitted", self._h, ctypes.byref(f))
+        return bool(f.value)
 
     def fit(self, points, values, tolerance, max_iter=100, accuracy=math.inf, initial=None,
             options: FitOptions = None):

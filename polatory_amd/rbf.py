@@ -102,6 +102,15 @@ class CovExponential(Rbf):
     cpd_order = 0
 
 
+def from_c(r):
+    """The descriptor of a pam_rbf (kind, parameters, anisotropy)."""
+    table = {_capi.PAM_RBF_BH3: Biharmonic3D, _capi.PAM_RBF_TH3: Triharmonic3D,
+             _capi.PAM_RBF_EXP: CovExponential}
+    rbf = table[r.kind]([r.params[0], r.params[1]])
+    rbf.set_anisotropy(np.array([r.aniso[i] for i in range(9)]).reshape(3, 3))
+    return rbf
+
+
 def make_rbf(name, params):
     """make_rbf (rbf/make_rbf.hpp:23-50) for the kernels on this path."""
     table = {"bh3": Biharmonic3D, "th3": Triharmonic3D, "exp": CovExponential}

@@ -4,9 +4,13 @@
 // Exit code 0 on success; prints one line per check.
 #include <polatory/polatory.hpp>
 
+#include <unistd.h>
+
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <random>
+#include <string>
 
 using polatory::Index;
 using polatory::Interpolant;
@@ -79,6 +83,21 @@ int main() {
   for (Index i = 0; i < n; i++) e2 = std::max(e2, std::abs(back(i) - values(i)));
   std::printf("evaluate at data sites: max |f - s| %.3e\n", e2);
   if (!(e2 <= 10 * tol)) return 4;
+
+  // Interpolant::save / load (interpolant.hpp:218): the loaded interpolant
+  // evaluates to the same bits
+  {
+    const std::string file = std::string(std::getenv("TMPDIR") ? std::getenv("TMPDIR") : "/tmp") +
+                             "/pam_test_api_" + std::to_string(static_cast<long>(getpid())) + ".bin";
+    interpolant.save(file);
+    Interpolant<3> loaded = Interpolant<3>::load(file);
+    std::remove(file.c_str());
+    if (!loaded.fitted() || loaded.model().poly_degree() != 1) return 6;
+    VecX a = interpolant.evaluate(trg), b = loaded.evaluate(trg);
+    for (Index i = 0; i < m; i++)
+      if (a(i) != b(i)) return 7;
+    std::printf("save/load: identical evaluation\n");
+  }
 
   // error contract: invalid arguments throw std::invalid_argument
   try {
