@@ -63,6 +63,11 @@ int pam_fmm_evaluator_create(const pam_rbf* rbf, const double bbox_min[3], const
 /* make_fmm_symmetric_evaluator(rbf, bbox) — fmm_symmetric_evaluator.hpp:80-82 */
 int pam_fmm_symmetric_evaluator_create(const pam_rbf* rbf, const double bbox_min[3],
                                        const double bbox_max[3], pam_fmm_evaluator** out);
+/* make_fmm_gradient_transpose_evaluator(rbf, bbox) — make_fmm_evaluator.cpp:119-148:
+ * a source-target handle whose evaluate() returns 3 n_targets gradients (as
+ * pam_fmm_evaluator_evaluate_gradient) and whose accuracy search measures them. */
+int pam_fmm_gradient_transpose_evaluator_create(const pam_rbf* rbf, const double bbox_min[3],
+                                                const double bbox_max[3], pam_fmm_evaluator** out);
 int pam_fmm_evaluator_destroy(pam_fmm_evaluator* h);
 /* set_accuracy(double) — absolute L-inf accuracy; +inf = default configuration (fmm_evaluator.hpp:36) */
 int pam_fmm_evaluator_set_accuracy(pam_fmm_evaluator* h, double accuracy);
@@ -74,8 +79,14 @@ int pam_fmm_evaluator_set_target_points(pam_fmm_evaluator* h, const double* xyz,
 int pam_fmm_evaluator_set_points(pam_fmm_evaluator* h, const double* xyz, int64_t n);
 /* set_weights(const VecX&) — fmm_evaluator.hpp:42 */
 int pam_fmm_evaluator_set_weights(pam_fmm_evaluator* h, const double* w, int64_t n);
-/* VecX evaluate() const — fmm_evaluator.hpp:34; out has n_targets entries */
+/* VecX evaluate() const — fmm_evaluator.hpp:34; out has n_targets entries
+ * (3 n_targets for a gradient-transpose handle); n is the number of targets */
 int pam_fmm_evaluator_evaluate(pam_fmm_evaluator* h, double* out, int64_t n);
+/* evaluate() of make_fmm_gradient_transpose_evaluator (make_fmm_evaluator.cpp:119-148,
+ * fmm/gradient_transpose_kernel.hpp:47-62) on the same handle: out has 3 n_targets
+ * entries, out[3 i + k] = (A^T sum_j w_j grad phi(A (t_i - s_j)))_k, using the
+ * handle's configuration, trees and expansions. Source-target handles only. */
+int pam_fmm_evaluator_evaluate_gradient(pam_fmm_evaluator* h, double* out, int64_t n);
 
 /* Same calls with pointers into device memory (HBM); asynchronous on the
  * handle's stream, ordered with the handle's other calls. */
@@ -84,6 +95,7 @@ int pam_fmm_evaluator_set_target_points_device(pam_fmm_evaluator* h, const doubl
 int pam_fmm_evaluator_set_points_device(pam_fmm_evaluator* h, const double* d_xyz, int64_t n);
 int pam_fmm_evaluator_set_weights_device(pam_fmm_evaluator* h, const double* d_w, int64_t n);
 int pam_fmm_evaluator_evaluate_device(pam_fmm_evaluator* h, double* d_out, int64_t n);
+int pam_fmm_evaluator_evaluate_gradient_device(pam_fmm_evaluator* h, double* d_out, int64_t n);
 int pam_fmm_evaluator_synchronize(pam_fmm_evaluator* h);
 
 /* Explicit configuration (replaces the accuracy search; no reference counterpart:
@@ -211,6 +223,13 @@ int pam_interpolant_evaluate(pam_interpolant* h, const double* points, int64_t n
                              double* out);
 int pam_interpolant_evaluate_device(pam_interpolant* h, const double* d_points, int64_t n,
                                     double accuracy, double* d_out);
+/* The grad_points half of evaluate(points, grad_points, accuracy, grad_accuracy)
+ * (interpolant.hpp:58-67, interpolation/evaluator.hpp:68-74): gradients of the
+ * fitted function at n points, out has 3 n entries (x y z per point).
+ * `accuracy` is the grad_accuracy: the FMM configuration is searched on the
+ * gradients. */
+int pam_interpolant_evaluate_gradient(pam_interpolant* h, const double* points, int64_t n, double accuracy,
+                                      double* out);
 /* Fixes the evaluation FMM configuration (NULL restores the accuracy-driven choice). */
 int pam_interpolant_set_eval_config(pam_interpolant* h, const pam_fmm_config* cfg);
 int pam_interpolant_num_centers(const pam_interpolant* h, int64_t* n);

@@ -100,6 +100,9 @@ _SIGS = {
     "pam_fmm_evaluator_set_points": [_P, _P, _I64],
     "pam_fmm_evaluator_set_weights": [_P, _P, _I64],
     "pam_fmm_evaluator_evaluate": [_P, _P, _I64],
+    "pam_fmm_evaluator_evaluate_gradient": [_P, _P, _I64],
+    "pam_fmm_gradient_transpose_evaluator_create": [ctypes.POINTER(PamRbf), _P, _P, ctypes.POINTER(_P)],
+    "pam_fmm_evaluator_evaluate_gradient_device": [_P, _P, _I64],
     "pam_fmm_evaluator_set_source_points_device": [_P, _P, _I64],
     "pam_fmm_evaluator_set_target_points_device": [_P, _P, _I64],
     "pam_fmm_evaluator_set_points_device": [_P, _P, _I64],
@@ -142,6 +145,7 @@ _SIGS = {
     "pam_interpolant_centers": [_P, _P],
     "pam_interpolant_weights": [_P, _P],
     "pam_interpolant_set_fitted": [_P, _P, _I64, _P],
+    "pam_interpolant_evaluate_gradient": [_P, _P, _I64, ctypes.c_double, _P],
     "pam_interpolant_save": [_P, ctypes.c_char_p],
     "pam_interpolant_load": [ctypes.c_char_p, ctypes.POINTER(_P)],
     "pam_interpolant_model": [_P, ctypes.POINTER(_P)],

@@ -95,6 +95,13 @@ int pam_fmm_symmetric_evaluator_create(const pam_rbf* rbf, const double bbox_min
   return create(rbf, bbox_min, bbox_max, out, true);
 }
 
+int pam_fmm_gradient_transpose_evaluator_create(const pam_rbf* rbf, const double bbox_min[3],
+                                                const double bbox_max[3], pam_fmm_evaluator** out) {
+  int rc = create(rbf, bbox_min, bbox_max, out, false);
+  if (rc != 0) return rc;
+  return guard([&] { ev(*out).set_gradient_kernel(true); });
+}
+
 int pam_fmm_evaluator_destroy(pam_fmm_evaluator* h) {
   return guard([&] { delete h; });
 }
@@ -153,6 +160,14 @@ int pam_fmm_evaluator_set_weights_device(pam_fmm_evaluator* h, const double* d_w
 
 int pam_fmm_evaluator_evaluate_device(pam_fmm_evaluator* h, double* d_out, int64_t n) {
   return guard([&] { ev(h).evaluate(d_out, n, true); });
+}
+
+int pam_fmm_evaluator_evaluate_gradient(pam_fmm_evaluator* h, double* out, int64_t n) {
+  return guard([&] { ev(h).evaluate_gradient(out, n, false); });
+}
+
+int pam_fmm_evaluator_evaluate_gradient_device(pam_fmm_evaluator* h, double* d_out, int64_t n) {
+  return guard([&] { ev(h).evaluate_gradient(d_out, n, true); });
 }
 
 int pam_fmm_evaluator_synchronize(pam_fmm_evaluator* h) {

@@ -217,6 +217,14 @@ int pam_interpolant_evaluate_device(pam_interpolant* h, const double* d_points, 
   return guard([&] { ip(h).evaluate(d_points, n, accuracy, d_out, true); });
 }
 
+int pam_interpolant_evaluate_gradient(pam_interpolant* h, const double* points, int64_t n, double accuracy,
+                                      double* out) {
+  return guard([&] {
+    if (n > 0 && (!points || !out)) pam::throw_invalid("null argument");
+    ip(h).evaluate_gradient(points, n, accuracy, out, false);
+  });
+}
+
 int pam_interpolant_set_eval_config(pam_interpolant* h, const pam_fmm_config* cfg) {
   return guard([&] {
     if (cfg) {

@@ -31,6 +31,11 @@ void launch_gather_f64(const double* in, const int32_t* idx, int64_t n, double* 
 void direct_sum_f64(const RbfDesc& rbf, const double* src, const double* w, int64_t n_src,
                     const double* trg, int64_t n_trg, double* out, bool accumulate,
                     DevBuf<double>& scratch, hipStream_t s);
+// FP64 direct gradient sum (gradient-transpose kernel): out3[3 i + k] =
+// (A^T sum_j w[j] grad phi(t_i - s_j))_k over transformed points (rbf_eval.hip).
+void direct_gradient_f64(const RbfDesc& rbf, const double* src, const double* w, int64_t n_src,
+                         const double* trg, int64_t n_trg, double* out3, DevBuf<double>& scratch,
+                         hipStream_t s);
 // Batched FP64 direct sums over host query sets, for small evaluations of
 // several resident models at once: one upload, one direct-sum launch, one
 // reduction, one download. A term is one rbf of a model (its transformed
@@ -62,6 +67,9 @@ class DirectBatch {
   HostStage<double> up_, down_;
   DevBuf<double> d_in_, d_part_, d_out_;
 };
+// out3[3 i + k] += d/dx_k sum_m c_m m(p_i) (gradient of the polynomial part).
+void launch_poly_grad(int degree, const double* pts, int64_t n, const double* d_coeffs, double* out3,
+                      hipStream_t s);
 // out[i] += sum_k c_k m_k(p_i) (monomials of degree 0..2, untransformed points).
 void launch_poly_eval(int degree, const double* pts, int64_t n, const double* d_coeffs,
                       double* out, hipStream_t s);

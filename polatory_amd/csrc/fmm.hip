@@ -464,6 +464,221 @@ p2p_kernel(const int32_t* __restrict__ tls, const int32_t* __restrict__ sls,
   }
 }
 
+// ---- gradients at the targets (gradient-transpose kernel,
+// fmm/gradient_transpose_kernel.hpp:47-62): out_i = A^T sum_j w_j grad phi(A (t_i - s_j)) ----
+
+// Lagrange basis and its derivative at the Chebyshev nodes:
+// S_a'(x) = 2/P sum_{k>=1} T_k(x_a) k U_{k-1}(x), written to column `col`.
+template <class T, int P, int NC>
+__device__ __forceinline__ void lagrange_d_lds(T xi, const T* __restrict__ tk, T (*s)[NC], T (*d)[NC],
+                                               int col) {
+  T t[P], du[P];
+  t[0] = T(1);
+  du[0] = T(0);
+  if constexpr (P > 1) {
+    t[1] = xi;
+    du[1] = T(1);  // 1 * U_0
+  }
+  T u0 = T(1), u1 = T(2) * xi;  // U_0, U_1
+#pragma unroll
+  for (int k = 2; k < P; k++) {
+    t[k] = T(2) * xi * t[k - 1] - t[k - 2];
+    du[k] = T(k) * u1;  // k U_{k-1}
+    const T u2 = T(2) * xi * u1 - u0;
+    u0 = u1;
+    u1 = u2;
+  }
+  const T c0 = T(1) / T(P), c1 = T(2) / T(P);
+#pragma unroll 1
+  for (int a = 0; a < P; a++) {
+    T sum = T(0), sd = T(0);
+#pragma unroll
+    for (int k = 1; k < P; k++) {
+      sum += tk[a * P + k] * t[k];
+      sd += tk[a * P + k] * du[k];
+    }
+    s[a][col] = c0 + c1 * sum;
+    d[a][col] = c1 * sd;
+  }
+}
+
+struct Aniso3 {
+  double a[9];  // row-major A
+};
+
+constexpr int kL2PGThreads = 128;
+
+// Far-field gradient: the derivative of each leaf's local interpolant,
+// divided by the leaf half width, turned back by A^T and added to the output
+// in input order (the near field is already there).
+template <int P>
+__global__ void __launch_bounds__(kL2PGThreads)
+l2p_grad_kernel(const int32_t* __restrict__ leaf_start, const V4<double>* __restrict__ pts,
+                const double* __restrict__ l, int kpad, double inv_h, Aniso3 A, const int32_t* __restrict__ perm,
+                double* __restrict__ out) {
+  constexpr int P2 = P * P, P3 = P * P * P;
+  __shared__ double tk[P2];
+  __shared__ double ls[P3];
+  __shared__ double sxs[P][kL2PGThreads], sys[P][kL2PGThreads], szs[P][kL2PGThreads];
+  __shared__ double dxs[P][kL2PGThreads], dys[P][kL2PGThreads], dzs[P][kL2PGThreads];
+  const int leaf = blockIdx.x;
+  init_tk<double, P>(tk);
+  for (int idx = threadIdx.x; idx < P3; idx += kL2PGThreads) ls[idx] = l[static_cast<int64_t>(leaf) * kpad + idx];
+  __syncthreads();
+  const int ps = leaf_start[leaf], pe = leaf_start[leaf + 1];
+  const int c = threadIdx.x;
+  for (int i = ps + c; i < pe; i += kL2PGThreads) {
+    const V4<double> q = pts[i];
+    lagrange_d_lds<double, P, kL2PGThreads>(q.x, tk, sxs, dxs, c);
+    lagrange_d_lds<double, P, kL2PGThreads>(q.y, tk, sys, dys, c);
+    lagrange_d_lds<double, P, kL2PGThreads>(q.z, tk, szs, dzs, c);
+    double sz[P], dz[P];
+#pragma unroll
+    for (int g = 0; g < P; g++) {
+      sz[g] = szs[g][c];
+      dz[g] = dzs[g][c];
+    }
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll 1
+    for (int a = 0; a < P; a++) {
+      const double* la = ls + a * P2;
+      double ya = 0.0, yay = 0.0, yaz = 0.0;
+#pragma unroll 1
+      for (int bb = 0; bb < P; bb++) {
+        double yb = 0.0, ybz = 0.0;
+#pragma unroll
+        for (int g = 0; g < P; g++) {
+          yb += la[bb * P + g] * sz[g];
+          ybz += la[bb * P + g] * dz[g];
+        }
+        ya += yb * sys[bb][c];
+        yay += yb * dys[bb][c];
+        yaz += ybz * sys[bb][c];
+      }
+      gx += ya * dxs[a][c];
+      gy += yay * sxs[a][c];
+      gz += yaz * sxs[a][c];
+    }
+    gx *= inv_h;
+    gy *= inv_h;
+    gz *= inv_h;
+    double* o = out + 3 * static_cast<int64_t>(perm[i]);
+#pragma unroll
+    for (int k = 0; k < 3; k++) o[k] += (A.a[k] * gx + A.a[3 + k] * gy) + A.a[6 + k] * gz;
+  }
+}
+
+// Near-field gradient: the layout of p2p_kernel with three sums per target;
+// writes A^T (sum over the 27 neighbour leaves) to the output in input order.
+template <class T, int KIND>
+__global__ void __launch_bounds__(kThreads)
+p2p_grad_kernel(const int32_t* __restrict__ tls, const int32_t* __restrict__ sls,
+                const int32_t* __restrict__ p2p, const V4<T>* __restrict__ tpts,
+                const V4<T>* __restrict__ spts, const int32_t* __restrict__ tperm, T ag, T pb, Aniso3 A,
+                double* __restrict__ out) {
+  constexpr int CAP = 32768 / static_cast<int>(sizeof(V4<T>));
+  constexpr int R = 2;
+  __shared__ V4<T> sp[CAP];
+  __shared__ int nb_start[28];
+  __shared__ int nb_src[27];
+  __shared__ T nb_off[27][3];
+  __shared__ int nb_count;
+  __shared__ T red[R][3][kThreads];
+  const int leaf = blockIdx.x;
+  const int ts = tls[leaf], te = tls[leaf + 1];
+  const int nt = te - ts;
+  if (threadIdx.x == 0) {
+    int k = 0, total = 0;
+    for (int s = 0; s < 27; s++) {
+      int sl = p2p[static_cast<int64_t>(leaf) * 27 + s];
+      if (sl < 0) continue;
+      int a = sls[sl], b = sls[sl + 1];
+      nb_src[k] = a;
+      nb_start[k] = total;
+      nb_off[k][0] = T(2 * (s / 9 - 1));
+      nb_off[k][1] = T(2 * ((s / 3) % 3 - 1));
+      nb_off[k][2] = T(2 * (s % 3 - 1));
+      total += b - a;
+      k++;
+    }
+    nb_start[k] = total;
+    nb_count = k;
+  }
+  __syncthreads();
+  const int nnb = nb_count;
+  const int total = nb_start[nnb];
+  const int per = (nt + R - 1) / R;
+  int tp = 1;
+  while (tp < per && tp < kThreads) tp <<= 1;
+  const int nsplit = kThreads / tp;
+  const int ti = threadIdx.x & (tp - 1);
+  const int si = threadIdx.x / tp;
+  const T seed = phi_seed<T, KIND>(pb);
+  for (int c0 = 0; c0 < nt; c0 += R * tp) {
+    V4<T> tq[R];
+    T ax[R], ay[R], az[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int i = c0 + r * tp + ti;
+      tq[r] = i < nt ? tpts[ts + i] : V4<T>{T(0), T(0), T(0), T(0)};
+      ax[r] = ay[r] = az[r] = T(0);
+    }
+    for (int cb = 0; cb < total; cb += CAP) {
+      const int cnt = min(CAP, total - cb);
+      __syncthreads();
+      for (int e = threadIdx.x; e < cnt; e += kThreads) {
+        int g = cb + e;
+        int k = 0;
+        while (k + 1 < nnb && nb_start[k + 1] <= g) k++;
+        V4<T> q = spts[nb_src[k] + (g - nb_start[k])];
+        q.x += nb_off[k][0];
+        q.y += nb_off[k][1];
+        q.z += nb_off[k][2];
+        q.w *= ag;  // the gradient's factor folded into the weight
+        sp[e] = q;
+      }
+      __syncthreads();
+      for (int j = si; j < cnt; j += nsplit) {
+        const V4<T> q = sp[j];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const T dx = tq[r].x - q.x, dy = tq[r].y - q.y, dz = tq[r].z - q.z;
+          const T r2 = fma(dz, dz, fma(dy, dy, fma(dx, dx, seed)));
+          const T f = q.w * grad_shape<T, KIND>(r2, pb);
+          ax[r] = fma(f, dx, ax[r]);
+          ay[r] = fma(f, dy, ay[r]);
+          az[r] = fma(f, dz, az[r]);
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      red[r][0][threadIdx.x] = ax[r];
+      red[r][1][threadIdx.x] = ay[r];
+      red[r][2][threadIdx.x] = az[r];
+    }
+    __syncthreads();
+    if (si == 0) {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int i = c0 + r * tp + ti;
+        if (i >= nt) continue;
+        double g[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          T sum = T(0);
+          for (int m = 0; m < nsplit; m++) sum += red[r][k][ti + m * tp];
+          g[k] = static_cast<double>(sum);
+        }
+        double* o = out + 3 * static_cast<int64_t>(tperm[ts + i]);
+#pragma unroll
+        for (int k = 0; k < 3; k++) o[k] = (A.a[k] * g[0] + A.a[3 + k] * g[1]) + A.a[6 + k] * g[2];
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ---- M2L on MFMA ----
 template <class T>
 struct Mfma;
@@ -1439,6 +1654,37 @@ void launch_p2p(const RbfDesc& rbf, const Tree& trg, const Tree& src, const List
   PAM_LAUNCH_CHECK();
 }
 
+void launch_l2p_grad(int p, const Tree& t, const double* l_leaf, const V4<double>* pts, const double* aniso,
+                     double* out3, hipStream_t s) {
+  int64_t nl = t.n_leaves();
+  if (nl == 0) return;
+  const double h = t.box.width / std::ldexp(1.0, t.level) / 2.0;
+  Aniso3 A;
+  for (int k = 0; k < 9; k++) A.a[k] = aniso[k];
+  dispatch_order(p, [&](auto P) {
+    l2p_grad_kernel<decltype(P)::value><<<static_cast<unsigned>(nl), kL2PGThreads, 0, s>>>(
+        t.leaf_start.get(), pts, l_leaf, kpad_of(P), 1.0 / h, A, t.perm.get(), out3);
+  });
+  PAM_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_p2p_grad(const RbfDesc& rbf, const Tree& trg, const Tree& src, const Lists& lists,
+                     const V4<T>* tpts, const V4<T>* spts, double* out3, hipStream_t s) {
+  int64_t nl = trg.n_leaves();
+  if (nl == 0) return;
+  const double h = trg.box.width / std::ldexp(1.0, trg.level) / 2.0;
+  const PhiConsts gc = grad_consts(rbf, h);
+  Aniso3 A;
+  for (int k = 0; k < 9; k++) A.a[k] = rbf.aniso[k];
+  dispatch_kind(rbf.kind, [&](auto K) {
+    p2p_grad_kernel<T, decltype(K)::value><<<static_cast<unsigned>(nl), kThreads, 0, s>>>(
+        trg.leaf_start.get(), src.leaf_start.get(), lists.p2p.get(), tpts, spts, trg.perm.get(),
+        static_cast<T>(gc.a), static_cast<T>(gc.b), A, out3);
+  });
+  PAM_LAUNCH_CHECK();
+}
+
 template <class T>
 void build_m2l_dense(int p, const RbfDesc& rbf, double h, M2LOperator& op, DevBuf<T>& a,
                      hipStream_t s) {
@@ -1499,6 +1745,8 @@ void launch_m2l_batch(const M2LOperator& op, const std::vector<M2LLevelSpec<TM, 
   template void launch_l2p<T>(int, const Tree&, const T*, const V4<T>*, T*, hipStream_t, bool);           \
   template void launch_p2p<T>(const RbfDesc&, const Tree&, const Tree&, const Lists&, const V4<T>*, \
                               const V4<T>*, const double*, double*, hipStream_t);                       \
+  template void launch_p2p_grad<T>(const RbfDesc&, const Tree&, const Tree&, const Lists&, const V4<T>*, \
+                                   const V4<T>*, double*, hipStream_t);                                  \
   template void build_m2l_dense<T>(int, const RbfDesc&, double, M2LOperator&, DevBuf<T>&,          \
                                    hipStream_t);                                                    \
   template void launch_m2l_batch<T, double>(const M2LOperator&,                                        \

@@ -62,6 +62,14 @@ void launch_l2p(int p, const Tree& t, const T* l_leaf, const V4<T>* pts, T* y_fa
 template <class T>
 void launch_p2p(const RbfDesc& rbf, const Tree& trg, const Tree& src, const Lists& lists,
                 const V4<T>* tpts, const V4<T>* spts, const double* y_far, double* out, hipStream_t s);
+// Gradients at the targets (fmm/gradient_transpose_kernel.hpp:47-62): the
+// near field writes A^T sum_j w_j grad phi into out3 (n x 3, input order);
+// the far field then adds A^T grad of the leaf local expansions.
+template <class T>
+void launch_p2p_grad(const RbfDesc& rbf, const Tree& trg, const Tree& src, const Lists& lists,
+                     const V4<T>* tpts, const V4<T>* spts, double* out3, hipStream_t s);
+void launch_l2p_grad(int p, const Tree& t, const double* l_leaf, const V4<double>* pts, const double* aniso,
+                     double* out3, hipStream_t s);
 
 // Builds the dense M2L operator of one level: entries phi(h * |x_a + 2v - x_b|)
 // for cells of half width h (kernel constants folded by phi_consts).

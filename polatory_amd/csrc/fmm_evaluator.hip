@@ -641,7 +641,8 @@ void FmmEvaluator::run_fmm(double* d_out) {
     PAM_HIP(hipEventRecord(fork_ev_, stream_));
     PAM_HIP(hipStreamWaitEvent(ps, fork_ev_, 0));
     record(7, ps);
-    launch_p2p<TP>(rbf_, tt, stree, lists_, tp2p, sp2p, nullptr, d_out, ps);
+    if (grad_) launch_p2p_grad<TP>(rbf_, tt, stree, lists_, tp2p, sp2p, d_out, ps);
+    else launch_p2p<TP>(rbf_, tt, stree, lists_, tp2p, sp2p, nullptr, d_out, ps);
     record(8, ps);
     PAM_HIP(hipEventRecord(join_ev_, ps));
   };
@@ -728,7 +729,8 @@ void FmmEvaluator::run_fmm(double* d_out) {
   // Join the near field, then add the far field to it.
   PAM_HIP(hipStreamWaitEvent(stream_, join_ev_, 0));
   record(5);
-  launch_l2p<double>(P, tt, st.l[L].get(), tpts64, d_out, stream_, true);
+  if (grad_) launch_l2p_grad(P, tt, st.l[L].get(), tpts64, rbf_.aniso, d_out, stream_);
+  else launch_l2p<double>(P, tt, st.l[L].get(), tpts64, d_out, stream_, true);
   record(6);
 }
 
@@ -761,7 +763,14 @@ void FmmEvaluator::choose_config() {
       std::vector<double> smp(static_cast<size_t>(m) * 3);
       for (int64_t i = 0; i < m; i++) {
         int64_t j = (i * n_src_) / m;
-        for (int k = 0; k < 3; k++) smp[3 * i + k] = all[3 * j + k];
+        if (grad_kernel_) {
+          // Gradients are measured between the sources (at a source the
+          // exponential's gradient is undefined): midpoints of two samples.
+          int64_t j2 = (j + n_src_ / 2 + 1) % n_src_;
+          for (int k = 0; k < 3; k++) smp[3 * i + k] = 0.5 * (all[3 * j + k] + all[3 * j2 + k]);
+        } else {
+          for (int k = 0; k < 3; k++) smp[3 * i + k] = all[3 * j + k];
+        }
       }
       RbfDesc iso = rbf_;
       for (int k = 0; k < 9; k++) iso.aniso[k] = (k % 4 == 0) ? 1.0 : 0.0;
@@ -775,7 +784,10 @@ void FmmEvaluator::choose_config() {
       probe.set_source_points(all.data(), n_src_, false);
       probe.set_weights(wts.data(), n_src_, false);
       probe.set_target_points(smp.data(), m, false);
-      std::vector<double> exact(m), approx(m);
+      // A gradient-transpose evaluator measures the error of its gradients.
+      probe.set_gradient_kernel(grad_kernel_);
+      const int64_t per = grad_kernel_ ? 3 : 1;
+      std::vector<double> exact(m * per), approx(m * per);
       FmmConfig dc;
       dc.mode = kModeDirect;
       probe.set_config(dc);
@@ -792,7 +804,10 @@ void FmmEvaluator::choose_config() {
         probe.set_config(t);
         probe.evaluate(approx.data(), m, false);
         double err = 0.0;
-        for (int64_t i = 0; i < m; i++) err = std::max(err, std::abs(approx[i] - exact[i]));
+        for (int64_t i = 0; i < m * per; i++) {
+          const double d = std::abs(approx[i] - exact[i]);
+          if (!(d <= err)) err = d;  // a NaN difference fails the candidate
+        }
         if (err <= accuracy_) {
           c.precision = cd[0];
           c.order = cd[1];
@@ -809,16 +824,46 @@ void FmmEvaluator::choose_config() {
   config_valid_ = true;
 }
 
+void FmmEvaluator::set_gradient_kernel(bool on) {
+  if (on && symmetric_) throw_runtime("gradients are evaluated at target points: use a source-target evaluator");
+  if (on == grad_kernel_) return;
+  grad_kernel_ = on;
+  if (!fixed_config_) config_valid_ = false;
+}
+
 void FmmEvaluator::evaluate(double* out, int64_t n, bool on_device) {
+  if (grad_kernel_) {
+    evaluate_gradient(out, n, on_device);
+    return;
+  }
+  grad_ = false;
+  evaluate_impl(out, n, on_device);
+}
+
+void FmmEvaluator::evaluate_gradient(double* out, int64_t n, bool on_device) {
+  if (symmetric_) throw_runtime("gradients are evaluated at target points: use a source-target evaluator");
+  if (!config_valid_) choose_config();
+  grad_ = true;
+  try {
+    evaluate_impl(out, n, on_device);
+  } catch (...) {
+    grad_ = false;
+    throw;
+  }
+  grad_ = false;
+}
+
+void FmmEvaluator::evaluate_impl(double* out, int64_t n, bool on_device) {
   int64_t nt = n_trg();
   if (n != nt) throw_invalid("output size must equal the number of target points");
   const FmmEvaluator& src = S();
   if (!src.have_weights_ && src.n_src_ > 0) throw_runtime("weights are not set");
   if (!config_valid_) choose_config();
+  const int per = grad_ ? 3 : 1;
   double* d_out = out;
   DevBuf<double> tmp;
   if (!on_device) {
-    tmp.resize(static_cast<size_t>(nt));
+    tmp.resize(static_cast<size_t>(nt) * per);
     d_out = tmp.get();
   }
   timing_valid_ = true;
@@ -826,13 +871,18 @@ void FmmEvaluator::evaluate(double* out, int64_t n, bool on_device) {
   stats_ = FmmStats{};
   if (nt > 0) {
     if (src.n_src_ == 0) {
-      PAM_HIP(hipMemsetAsync(d_out, 0, sizeof(double) * nt, stream_));
+      PAM_HIP(hipMemsetAsync(d_out, 0, sizeof(double) * nt * per, stream_));
       for (int i = 1; i <= 8; i++) record(i);
       stats_.direct = 1;
     } else if (use_direct()) {
       for (int i = 1; i <= 5; i++) record(i);
-      direct_sum_f64(rbf_, src.src_pts_.get(), src.w_.get(), src.n_src_, d_target_points(), nt, d_out, false,
-                     direct_scratch_, stream_);
+      if (grad_) {
+        direct_gradient_f64(rbf_, src.src_pts_.get(), src.w_.get(), src.n_src_, d_target_points(), nt, d_out,
+                            direct_scratch_, stream_);
+      } else {
+        direct_sum_f64(rbf_, src.src_pts_.get(), src.w_.get(), src.n_src_, d_target_points(), nt, d_out, false,
+                       direct_scratch_, stream_);
+      }
       record(6);
       record(7);
       record(8);
@@ -852,7 +902,7 @@ void FmmEvaluator::evaluate(double* out, int64_t n, bool on_device) {
   stats_.n_trg_leaves = stats_.direct ? 0 : target_tree().n_leaves();
   stats_pending_ = true;
   if (!on_device) {
-    tmp.download(out, static_cast<size_t>(nt), stream_);
+    tmp.download(out, static_cast<size_t>(nt) * per, stream_);
     PAM_HIP(hipStreamSynchronize(stream_));
   }
 }

@@ -82,6 +82,15 @@ class FmmEvaluator {
   void set_weights(const double* w, int64_t n, bool on_device);
   // out has n_trg entries (input order of the targets).
   void evaluate(double* out, int64_t n, bool on_device);
+  // Gradients at the targets (make_fmm_gradient_transpose_evaluator,
+  // fmm/gradient_transpose_kernel.hpp): out has 3 n_trg entries, x y z per
+  // target in input order. Same configuration, trees and expansions as the
+  // values; source-target evaluators only.
+  void evaluate_gradient(double* out, int64_t n, bool on_device);
+  // Makes this a gradient-transpose evaluator (make_fmm_evaluator.cpp:119-148):
+  // evaluate() returns the gradients and the accuracy search measures them.
+  void set_gradient_kernel(bool on);
+  bool gradient_kernel() const { return grad_kernel_; }
 
   hipStream_t stream() const { return stream_; }
   const RbfDesc& rbf() const { return rbf_; }
@@ -123,11 +132,14 @@ class FmmEvaluator {
   void prepare_source_multipoles();
   template <class TP>
   void run_fmm(double* d_out);
+  void evaluate_impl(double* out, int64_t n, bool on_device);
   void invalidate_expansions();
   void record(int idx, hipStream_t s = nullptr);
 
   RbfDesc rbf_;
   bool symmetric_;
+  bool grad_ = false;  // the evaluation in progress computes gradients
+  bool grad_kernel_ = false;
   double user_min_[3], user_max_[3];
   hipStream_t stream_ = nullptr;
   bool own_stream_ = true;

@@ -48,6 +48,7 @@ Interpolant::Interpolant(const Model& model) : model_(model) {
 
 Interpolant::~Interpolant() {
   ev_.reset();
+  gev_.reset();
   if (s_) {
     (void)hipStreamSynchronize(s_);
     (void)hipStreamDestroy(s_);
@@ -93,6 +94,7 @@ void Interpolant::fit(const std::vector<double>& points, const std::vector<doubl
   }
   fitted_ = false;
   ev_.reset();
+  gev_.reset();
   Solver solver(model_, points, opt, s_);
   weights_ = solver.solve(values, tolerance, max_iter, initial ? &init : nullptr, &stats_);
   centers_ = points;
@@ -115,6 +117,7 @@ void Interpolant::set_fitted(const std::vector<double>& centers, const std::vect
   weights_ = weights;
   fitted_ = true;
   ev_.reset();
+  gev_.reset();
   for (int k = 0; k < 3; k++) {
     bbox_lo_[k] = INFINITY;
     bbox_hi_[k] = -INFINITY;
@@ -130,6 +133,7 @@ void Interpolant::set_eval_config(const FmmConfig* c) {
   have_eval_cfg_ = c != nullptr;
   if (c) eval_cfg_ = *c;
   ev_.reset();
+  gev_.reset();
 }
 
 Evaluator& Interpolant::evaluator() {
@@ -145,6 +149,21 @@ Evaluator& Interpolant::evaluator() {
     PAM_HIP(hipStreamSynchronize(s_));
   }
   return *ev_;
+}
+
+Evaluator& Interpolant::gradient_evaluator() {
+  throw_if_not_fitted();
+  if (!gev_) {
+    int64_t mu = static_cast<int64_t>(centers_.size() / 3);
+    gev_ = std::make_unique<Evaluator>(model_, bbox_lo_, bbox_hi_, gev_acc_ > 0 ? gev_acc_ : INFINITY, s_, true);
+    if (have_eval_cfg_) gev_->set_fmm_config(eval_cfg_);
+    d_centers_.upload(centers_.data(), centers_.size(), s_);
+    d_weights_.upload(weights_.data(), weights_.size(), s_);
+    gev_->set_source_points(d_centers_.get(), mu);
+    gev_->set_weights(d_weights_.get());
+    PAM_HIP(hipStreamSynchronize(s_));
+  }
+  return *gev_;
 }
 
 void Interpolant::evaluate(const double* points, int64_t n, double accuracy, double* out, bool on_device) {
@@ -169,6 +188,34 @@ void Interpolant::evaluate(const double* points, int64_t n, double accuracy, dou
   ev.evaluate(d_out);
   if (!on_device) {
     PAM_HIP(hipMemcpyAsync(out, d_out, sizeof(double) * n, hipMemcpyDeviceToHost, s_));
+  }
+  PAM_HIP(hipStreamSynchronize(s_));
+}
+
+void Interpolant::evaluate_gradient(const double* points, int64_t n, double accuracy, double* out3,
+                                    bool on_device) {
+  throw_if_not_fitted();
+  if (!(accuracy > 0.0)) throw_invalid("accuracy must be positive");
+  if (n < 0) throw_invalid("negative point count");
+  if (accuracy != gev_acc_) {
+    gev_.reset();
+    gev_acc_ = accuracy;
+  }
+  Evaluator& ev = gradient_evaluator();
+  const double* d_in = points;
+  if (!on_device) {
+    d_pts_.upload(points, static_cast<size_t>(n) * 3, s_);
+    d_in = d_pts_.get();
+  }
+  double* d_out = out3;
+  if (!on_device) {
+    d_out_.resize(static_cast<size_t>(n) * 3);
+    d_out = d_out_.get();
+  }
+  ev.set_target_points(d_in, n);
+  ev.evaluate_gradient(d_out);
+  if (!on_device) {
+    PAM_HIP(hipMemcpyAsync(out3, d_out, sizeof(double) * n * 3, hipMemcpyDeviceToHost, s_));
   }
   PAM_HIP(hipStreamSynchronize(s_));
 }

@@ -21,6 +21,9 @@ class Interpolant {
            int max_iter, double accuracy, const FitOptions& opt, const Interpolant* initial);
   // interpolant.hpp:54-67 (+ polynomial terms); pointers are host or device.
   void evaluate(const double* points, int64_t n, double accuracy, double* out, bool on_device);
+  // Gradients at grad points (the grad_points half of interpolant.hpp:58-67):
+  // out has 3 n entries, x y z per point; accuracy is the grad_accuracy.
+  void evaluate_gradient(const double* points, int64_t n, double accuracy, double* out3, bool on_device);
   // Installs centres and weights directly (a saved or externally fitted model).
   void set_fitted(const std::vector<double>& centers, const std::vector<double>& weights);
   void set_eval_config(const FmmConfig* c);
@@ -32,6 +35,7 @@ class Interpolant {
   const FitStats& fit_stats() const { return stats_; }
   hipStream_t stream() const { return s_; }
   Evaluator& evaluator();
+  Evaluator& gradient_evaluator();
   int64_t resident_bytes() const;
 
  private:
@@ -44,8 +48,8 @@ class Interpolant {
   bool fitted_ = false;
   std::vector<double> centers_, weights_;
   FitStats stats_;
-  std::unique_ptr<Evaluator> ev_;
-  double ev_acc_ = -1.0;
+  std::unique_ptr<Evaluator> ev_, gev_;
+  double ev_acc_ = -1.0, gev_acc_ = -1.0;
   bool have_eval_cfg_ = false;
   FmmConfig eval_cfg_;
   DevBuf<double> d_centers_, d_weights_, d_pts_, d_out_;

@@ -31,12 +31,13 @@ void Model::validate() const {
 }
 
 Evaluator::Evaluator(const Model& model, const double* bbox_min, const double* bbox_max,
-                     double accuracy, hipStream_t s)
+                     double accuracy, hipStream_t s, bool gradient)
     : model_(model), s_(s) {
   model_.validate();
   double acc = accuracy / static_cast<double>(model_.rbfs.size());
   for (const auto& r : model_.rbfs) {
     fmm_.push_back(std::make_unique<FmmEvaluator>(r, bbox_min, bbox_max, false, s));
+    fmm_.back()->set_gradient_kernel(gradient);
     fmm_.back()->set_accuracy(acc);
   }
   coeffs_.resize(10);
@@ -76,6 +77,21 @@ void Evaluator::evaluate(double* d_out) {
     }
   }
   if (model_.l() > 0) launch_poly_eval(model_.poly_degree, trg_pts_.get(), n_trg_, coeffs_.get(), d_out, s_);
+}
+
+void Evaluator::evaluate_gradient(double* d_out3) {
+  const int64_t n3 = n_trg_ * 3;
+  if (fmm_.size() == 1) {
+    fmm_[0]->evaluate_gradient(d_out3, n_trg_, true);
+  } else {
+    tmp_.resize(static_cast<size_t>(n3));
+    dev_fill(0.0, d_out3, n3, s_);
+    for (auto& f : fmm_) {
+      f->evaluate_gradient(tmp_.get(), n_trg_, true);
+      dev_axpy(1.0, tmp_.get(), d_out3, n3, s_);
+    }
+  }
+  if (model_.l() > 0) launch_poly_grad(model_.poly_degree, trg_pts_.get(), n_trg_, coeffs_.get(), d_out3, s_);
 }
 
 SymmetricEvaluator::SymmetricEvaluator(const Model& model, const double* bbox_min,

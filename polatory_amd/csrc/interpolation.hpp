@@ -29,13 +29,18 @@ int rbf_cpd_order(int kind);
 // interpolation/evaluator.hpp:50-144: sum over rbfs of FMM values + polynomial.
 class Evaluator {
  public:
+  // gradient: the FMM evaluators are gradient-transpose evaluators whose
+  // accuracy search measures gradients (evaluate_gradient only).
   Evaluator(const Model& model, const double* bbox_min, const double* bbox_max, double accuracy,
-            hipStream_t s);
+            hipStream_t s, bool gradient = false);
   void set_fmm_config(const FmmConfig& c);
   void set_source_points(const double* d_pts, int64_t n);
   void set_target_points(const double* d_pts, int64_t n);
   void set_weights(const double* d_w);  // n_src + l entries
   void evaluate(double* d_out);         // n_trg entries
+  // Gradients at the targets (the gradient-transpose terms and the polynomial
+  // gradient of interpolation/evaluator.hpp:68-74): 3 n_trg entries.
+  void evaluate_gradient(double* d_out3);
   int64_t n_src() const { return n_src_; }
   int64_t n_trg() const { return n_trg_; }
   FmmEvaluator& fmm(int i) { return *fmm_.at(i); }

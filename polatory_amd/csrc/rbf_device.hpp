@@ -89,6 +89,47 @@ __device__ __forceinline__ T phi_seed(T b) {
   }
 }
 
+// Gradient in physical units from differences d in a local unit h:
+//   grad phi = ag * grad_shape(r2c) * d,  r2c = |d|^2 + phi_seed(b),
+// with (ag, b) from grad_consts(rbf, h) (rbf/polyharmonic_odd.hpp:41-53,
+// rbf/cov_exponential.hpp:42-50):
+//   bh3: ag = -slope,          grad_shape = 1 / sqrt(r2c)  (0 where r2c = 0)
+//   th3: ag = 3 slope h^2,     grad_shape = sqrt(r2c)
+//   exp: ag = -3 psill / range, grad_shape = exp(b r) / r, r = sqrt(r2c)
+// (at r = 0 the exponential's gradient is undefined, as in the reference).
+inline PhiConsts grad_consts(const RbfDesc& rbf, double h) {
+  const PhiConsts pc = phi_consts(rbf, h);
+  switch (rbf.kind) {
+    case kBh3: return {pc.a / h, pc.b};
+    case kTh3: return {3.0 * pc.a / h, pc.b};
+    case kExp: return {pc.a * pc.b / h, pc.b};
+  }
+  throw_invalid("unknown rbf kind");
+}
+
+template <class T, int KIND>
+__device__ __forceinline__ T grad_shape(T r2c, T b) {
+  if constexpr (sizeof(T) == 8) {
+    if constexpr (KIND == kBh3) {
+      return r2c > 0.0 ? 1.0 / sqrt(r2c) : 0.0;
+    } else if constexpr (KIND == kTh3) {
+      return sqrt(r2c);
+    } else {
+      const double r = sqrt(r2c);
+      return exp(b * r) / r;
+    }
+  } else {
+    if constexpr (KIND == kBh3) {
+      return r2c > 0.0f ? __builtin_amdgcn_rsqf(r2c) : 0.0f;
+    } else if constexpr (KIND == kTh3) {
+      return __builtin_amdgcn_sqrtf(r2c);
+    } else {
+      const float rs = __builtin_amdgcn_rsqf(r2c);
+      return __expf(b * (r2c * rs)) * rs;
+    }
+  }
+}
+
 template <class T, int KIND>
 __device__ __forceinline__ T phi_t(T r2, T a, T b) {
   if constexpr (sizeof(T) == 8) {

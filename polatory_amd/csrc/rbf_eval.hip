@@ -92,6 +92,31 @@ __global__ void grad_reduce_kernel(const double* __restrict__ partial, int nchun
   for (int k = 0; k < 3; k++) out[3 * i + k] = (A.a[k] * g[0] + A.a[3 + k] * g[1]) + A.a[6 + k] * g[2];
 }
 
+}  // namespace
+
+void direct_gradient_f64(const RbfDesc& rbf, const double* src, const double* w, int64_t n_src,
+                         const double* trg, int64_t n_trg, double* out3, DevBuf<double>& scratch,
+                         hipStream_t s) {
+  if (n_trg == 0) return;
+  Aniso A;
+  for (int k = 0; k < 9; k++) A.a[k] = rbf.aniso[k];
+  const int64_t chunk = 8192;
+  const int nchunks = n_src > 0 ? ceil_div(n_src, chunk) : 0;
+  scratch.resize(static_cast<size_t>(std::max(nchunks, 1)) * n_trg * 3);
+  if (n_src > 0) {
+    dim3 grid(ceil_div(n_trg, kGradThreads), nchunks);
+    dispatch_kind(rbf.kind, [&](auto K) {
+      direct_grad_kernel<decltype(K)::value><<<grid, kGradThreads, 0, s>>>(rbf.p0, rbf.p1, src, w, n_src, trg,
+                                                                           n_trg, scratch.get(), chunk);
+    });
+    PAM_LAUNCH_CHECK();
+  }
+  grad_reduce_kernel<<<ceil_div(n_trg, 256), 256, 0, s>>>(scratch.get(), nchunks, n_trg, A, out3);
+  PAM_LAUNCH_CHECK();
+}
+
+namespace {
+
 struct Stream {
   hipStream_t s = nullptr;
   Stream() { PAM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
@@ -150,30 +175,18 @@ int pam_direct_evaluate_gradient(const pam_rbf* rbf, const double* src, int64_t 
     const RbfDesc d = to_desc(rbf);
     if (n_trg == 0) return;
     Stream st;
-    Aniso A;
-    for (int k = 0; k < 9; k++) A.a[k] = d.aniso[k];
     DevBuf<double> ds, dw, dt, us, ut, partial, dout;
     dt.upload(trg, static_cast<size_t>(n_trg) * 3, st.s);
     ut.resize(static_cast<size_t>(n_trg) * 3);
     launch_transform_points(d.aniso, dt.get(), n_trg, ut.get(), st.s);
     dout.resize(static_cast<size_t>(n_trg) * 3);
-    const int64_t chunk = 8192;
-    const int nchunks = n_src > 0 ? ceil_div(n_src, chunk) : 0;
-    partial.resize(static_cast<size_t>(std::max(nchunks, 1)) * n_trg * 3);
     if (n_src > 0) {
       ds.upload(src, static_cast<size_t>(n_src) * 3, st.s);
       dw.upload(w, static_cast<size_t>(n_src), st.s);
       us.resize(static_cast<size_t>(n_src) * 3);
       launch_transform_points(d.aniso, ds.get(), n_src, us.get(), st.s);
-      dim3 grid(ceil_div(n_trg, kGradThreads), nchunks);
-      dispatch_kind(d.kind, [&](auto K) {
-        direct_grad_kernel<decltype(K)::value><<<grid, kGradThreads, 0, st.s>>>(
-            d.p0, d.p1, us.get(), dw.get(), n_src, ut.get(), n_trg, partial.get(), chunk);
-      });
-      PAM_LAUNCH_CHECK();
     }
-    grad_reduce_kernel<<<ceil_div(n_trg, 256), 256, 0, st.s>>>(partial.get(), nchunks, n_trg, A, dout.get());
-    PAM_LAUNCH_CHECK();
+    direct_gradient_f64(d, us.get(), dw.get(), n_src, ut.get(), n_trg, dout.get(), partial, st.s);
     dout.download(out, static_cast<size_t>(n_trg) * 3, st.s);
     PAM_HIP(hipStreamSynchronize(st.s));
   });

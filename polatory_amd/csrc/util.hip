@@ -389,6 +389,25 @@ __global__ void poly_eval_kernel(int degree, const double* __restrict__ pts, int
   out[i] += poly_value(degree, c, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
 }
 
+// Gradient of the polynomial part at untransformed points:
+// out3[3 i + k] += d/dx_k sum_m c_m m(p_i) (monomial_basis.hpp:198-209 order).
+__global__ void poly_grad_kernel(int degree, const double* __restrict__ pts, int64_t n,
+                                 const double* __restrict__ c, double* __restrict__ out3) {
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (degree < 1) return;
+  const double x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
+  double gx = c[1], gy = c[2], gz = c[3];
+  if (degree >= 2) {
+    gx += 2.0 * c[4] * x + c[5] * y + c[6] * z;
+    gy += c[5] * x + 2.0 * c[7] * y + c[8] * z;
+    gz += c[6] * x + c[8] * y + 2.0 * c[9] * z;
+  }
+  out3[3 * i] += gx;
+  out3[3 * i + 1] += gy;
+  out3[3 * i + 2] += gz;
+}
+
 // One thread per query row: each segment's chunks summed in chunk order, the
 // segments of a query set added in rbf order, then its polynomial.
 __global__ void direct_batch_reduce_kernel(const DirQuery* __restrict__ qs, int nq,
@@ -528,6 +547,13 @@ void direct_sum_f64(const RbfDesc& rbf, const double* src, const double* w, int6
   }
   reduce_partial_kernel<<<ceil_div(n_trg, 256), 256, 0, s>>>(scratch.get(), nchunks, n_trg, out,
                                                              accumulate ? 1 : 0);
+  PAM_LAUNCH_CHECK();
+}
+
+void launch_poly_grad(int degree, const double* pts, int64_t n, const double* d_coeffs, double* out3,
+                      hipStream_t s) {
+  if (n == 0 || degree < 1) return;
+  poly_grad_kernel<<<ceil_div(n, 256), 256, 0, s>>>(degree, pts, n, d_coeffs, out3);
   PAM_LAUNCH_CHECK();
 }
 

@@ -62,14 +62,14 @@ class FmmEvaluator:
     """Source-target evaluator (FmmGenericEvaluatorBase<3>)."""
 
     _symmetric = False
+    _create = "pam_fmm_evaluator_create"
 
     def __init__(self, rbf: Rbf, bbox=None):
         self._rbf_c = rbf.to_c()
         lo, hi = _bbox(bbox)
         self._bbox = (lo, hi)
         h = ctypes.c_void_p()
-        create = ("pam_fmm_symmetric_evaluator_create" if self._symmetric
-                  else "pam_fmm_evaluator_create")
+        create = self._create
         _capi.call(create, ctypes.byref(self._rbf_c), None if lo is None else _ptr(lo),
                    None if hi is None else _ptr(hi), ctypes.byref(h))
         self._h = h
@@ -115,6 +115,14 @@ class FmmEvaluator:
         n = self.n_src if self._symmetric else self.n_trg
         out = np.empty(n, dtype=np.float64)
         _capi.call("pam_fmm_evaluator_evaluate", self._h, _ptr(out), n)
+        return out
+
+    def evaluate_gradient(self):
+        """Gradients at the targets (gradient-transpose evaluator): (n_trg, 3)."""
+        if self._symmetric:
+            raise RuntimeError("gradients are evaluated at target points: use a source-target evaluator")
+        out = np.empty((self.n_trg, 3), dtype=np.float64)
+        _capi.call("pam_fmm_evaluator_evaluate_gradient", self._h, _ptr(out), self.n_trg)
         return out
 
     def stats(self):
@@ -179,6 +187,7 @@ class FmmSymmetricEvaluator(FmmEvaluator):
     """Evaluator at the source points themselves (FmmGenericSymmetricEvaluatorBase<3>)."""
 
     _symmetric = True
+    _create = "pam_fmm_symmetric_evaluator_create"
 
     def set_points(self, points):
         p = _points(points)
@@ -199,6 +208,10 @@ def make_fmm_evaluator(rbf, bbox=None):
 
 def make_fmm_symmetric_evaluator(rbf, bbox=None):
     return FmmSymmetricEvaluator(rbf, bbox)
+
+
+def make_fmm_gradient_transpose_evaluator(rbf, bbox=None):
+    return FmmGradientTransposeEvaluator(rbf, bbox)
 
 
 def direct_evaluate_gradient(rbf: Rbf, src, weights, trg):
@@ -224,3 +237,16 @@ def direct_evaluate(rbf: Rbf, src, weights, trg):
     _capi.call("pam_direct_evaluate", ctypes.byref(rc), _ptr(s), s.shape[0], _ptr(w), _ptr(t),
                t.shape[0], _ptr(out))
     return out
+
+
+class FmmGradientTransposeEvaluator(FmmEvaluator):
+    """make_fmm_gradient_transpose_evaluator (make_fmm_evaluator.cpp:119-148):
+    evaluate() returns the gradients at the targets, (n_trg, 3), and the
+    accuracy search measures them."""
+
+    _create = "pam_fmm_gradient_transpose_evaluator_create"
+
+    def evaluate(self):
+        out = np.empty((self.n_trg, 3), dtype=np.float64)
+        _capi.call("pam_fmm_evaluator_evaluate", self._h, _ptr(out), self.n_trg)
+        return out

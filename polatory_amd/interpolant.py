@@ -159,10 +159,24 @@ class Interpolant:
         _capi.call("pam_interpolant_fit", self._h, _ptr(p), p.shape[0], _ptr(v), float(tolerance),
                    int(max_iter), float(accuracy), opt, None if initial is None else initial._h)
 
-    def evaluate(self, points, accuracy=math.inf):
+    def evaluate(self, points, grad_points=None, accuracy=math.inf, grad_accuracy=math.inf):
+        """Values at points; with grad_points, the values followed by the
+        gradients at grad_points (x y z per point), as interpolant.hpp:58-67
+        returns them. Each half has its own accuracy search: `accuracy` for
+        the values, `grad_accuracy` for the gradients."""
         p = _points(points)
         out = np.empty(p.shape[0])
         _capi.call("pam_interpolant_evaluate", self._h, _ptr(p), p.shape[0], float(accuracy), _ptr(out))
+        if grad_points is None:
+            return out
+        return np.concatenate([out, self.evaluate_gradient(grad_points, grad_accuracy).reshape(-1)])
+
+    def evaluate_gradient(self, points, accuracy=math.inf):
+        """Gradients of the fitted function at points: (n, 3)."""
+        p = _points(points)
+        out = np.empty((p.shape[0], 3))
+        _capi.call("pam_interpolant_evaluate_gradient", self._h, _ptr(p), p.shape[0], float(accuracy),
+                   _ptr(out))
         return out
 
     def set_eval_config(self, config: FmmConfig = None):
