@@ -61,7 +61,7 @@ def make_inputs():
 class DeviceEval:
     """Evaluator with device-resident inputs (torch owns the HBM buffers)."""
 
-    def __init__(self, torch, src, w, trg, cfg):
+    def __init__(self, torch, src, w, trg, cfg, gradient=False):
         import polatory_amd as pa
         from polatory_amd import _capi
         self.pa, self.capi, self.torch = pa, _capi, torch
@@ -69,7 +69,8 @@ class DeviceEval:
         self.d_src = torch.from_numpy(src).to(dev).contiguous()
         self.d_trg = torch.from_numpy(trg).to(dev).contiguous()
         self.d_w = torch.from_numpy(w).to(dev).contiguous()
-        self.d_out = torch.empty(trg.shape[0], dtype=torch.float64, device=dev)
+        self.gradient = gradient
+        self.d_out = torch.empty(trg.shape[0] * (3 if gradient else 1), dtype=torch.float64, device=dev)
         torch.cuda.synchronize()
         self.ev = pa.make_fmm_evaluator(pa.Biharmonic3D(), ((0, 0, 0), (1, 1, 1)))
         self.ev.set_config(cfg)
@@ -81,7 +82,8 @@ class DeviceEval:
         self.capi.call("pam_fmm_evaluator_set_source_points_device", h, P(self.d_src.data_ptr()), self.n_src)
         self.capi.call("pam_fmm_evaluator_set_weights_device", h, P(self.d_w.data_ptr()), self.n_src)
         self.capi.call("pam_fmm_evaluator_set_target_points_device", h, P(self.d_trg.data_ptr()), self.n_trg)
-        self.capi.call("pam_fmm_evaluator_evaluate_device", h, P(self.d_out.data_ptr()), self.n_trg)
+        what = "pam_fmm_evaluator_evaluate_gradient_device" if self.gradient else "pam_fmm_evaluator_evaluate_device"
+        self.capi.call(what, h, P(self.d_out.data_ptr()), self.n_trg)
 
     def sync(self):
         self.capi.call("pam_fmm_evaluator_synchronize", self.ev._h)
@@ -303,6 +305,28 @@ def serve_section(pa, n_models, n_requests, rate):
             "service_ms_by_size": service, "model_configs": cfgs}
 
 
+def gradient_section(pa, torch, src, w, trg, sub, cfgs, steps, warmup):
+    """Gradients at the 1M targets (gradient-transpose FMM) with the value
+    path's configurations, against the FP64 direct gradient sum on 2,000 of
+    the subsample's targets."""
+    ref = pa.direct_evaluate_gradient(pa.Biharmonic3D(), src, w, trg[sub[:2000]])
+    ref_inf = np.abs(ref).max()
+    out = {"workload": "gradients of the biharmonic3d sum, 1,000,000 sources -> 1,000,000 targets"}
+    for name, cfg in cfgs:
+        de = DeviceEval(torch, src, w, trg, cfg, gradient=True)
+        sec, stats = time_steps(torch, de, steps, warmup)
+        g = de.output().reshape(-1, 3)
+        err = float(np.abs(g[sub[:2000]] - ref).max() / ref_inf)
+        keys = ("ms_tree", "ms_upward", "ms_m2l", "ms_l2l", "ms_l2p", "ms_p2p", "ms_total")
+        out[name] = {"ms_per_step": sec * 1e3, "grads_per_s": N_TRG / sec, "rel_err": err,
+                     "order": stats[-1]["order"],
+                     "stage_ms": {k: float(np.mean([s_[k] for s_ in stats])) for k in keys}}
+        log("gradient", name, json.dumps(out[name]))
+        del de
+        torch.cuda.synchronize()
+    return out
+
+
 def time_steps(torch, de, steps, warmup):
     for _ in range(warmup):
         de.step()
@@ -331,6 +355,7 @@ def main():
     ap.add_argument("--no-fit", action="store_true", help="skip the 1M-point fit comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
     ap.add_argument("--no-kriging", action="store_true", help="skip the kriging fit and prediction")
+    ap.add_argument("--no-gradient", action="store_true", help="skip the gradient evaluation")
     ap.add_argument("--serve-models", type=int, default=32)
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--serve-rate", type=float, default=200.0)
@@ -438,6 +463,8 @@ def main():
                "sample": f"oracle FP64 direct O(N*M) sum, {N_SRC} sources x {n_cpu} targets "
                          f"({dt:.1f} s), OpenMP over {cores} threads"}
 
+    gradient = None if args.no_gradient else gradient_section(
+        pa, torch, src, w, trg, sub, (("mixed", cfg32), ("fp64", cfg64)), args.steps, args.warmup)
     fit = None if args.no_fit else fit_section(pa)
     kriging = None if args.no_kriging else kriging_section(pa)
     serve = None if args.no_serve else serve_section(pa, args.serve_models, args.serve_requests,
@@ -472,6 +499,7 @@ def main():
         "cpu_baseline": cpu,
         "fit": fit,
         "kriging": kriging,
+        "gradient": gradient,
         "serve": serve,
     }
     print(json.dumps(out), flush=True)
