@@ -218,6 +218,19 @@ int pam_interpolant_destroy(pam_interpolant* h);
 int pam_interpolant_fit(pam_interpolant* h, const double* points, int64_t n, const double* values,
                         double tolerance, int max_iter, double accuracy, const pam_fit_options* opt,
                         const pam_interpolant* initial);
+/* fit_incrementally(points, values, tolerance, max_iter = 100, accuracy = inf)
+ * — interpolant.hpp:115-147, interpolation/incremental_fitter.hpp:25-183: fits on a
+ * growing subset of the points; centers() then holds the points chosen, in the
+ * order they were added. opt may be NULL. */
+int pam_interpolant_fit_incrementally(pam_interpolant* h, const double* points, int64_t n,
+                                      const double* values, double tolerance, int max_iter,
+                                      double accuracy, const pam_fit_options* opt);
+/* DistanceFilter(points).filter(distance, indices).filtered_indices()
+ * (point_cloud/distance_filter.hpp:21-56): walks indices in order and drops every
+ * later point within distance of a kept one. out receives the kept indices (at
+ * most n_idx), *n_out their count. Host only. */
+int pam_distance_filter(const double* points, int64_t n, double distance, const int64_t* indices,
+                        int64_t n_idx, int64_t* out, int64_t* n_out);
 /* evaluate(points, accuracy = inf) — interpolant.hpp:54-56 */
 int pam_interpolant_evaluate(pam_interpolant* h, const double* points, int64_t n, double accuracy,
                              double* out);
@@ -255,6 +268,7 @@ typedef struct {
   int graph_replays;        /* FGMRES steps replayed from the captured graph */
   int escalations;          /* precision raises after FGMRES stagnated (FP64 level updates, then operator) */
   int converged;            /* 0 when keep_unconverged stopped the fit at max_iter */
+  int rounds;               /* pam_interpolant_fit_incrementally: rounds of solve + residuals */
 } pam_fit_stats;
 int pam_interpolant_fit_stats(const pam_interpolant* h, pam_fit_stats* out);
 /* Residual after each FGMRES iteration (out may be NULL to query the length). */

@@ -48,7 +48,7 @@ class PamFitStats(ctypes.Structure):
                 ("residual", ctypes.c_double), ("seconds", ctypes.c_double),
                 ("setup_seconds", ctypes.c_double), ("precond_setup_seconds", ctypes.c_double),
                 ("factor_bytes", ctypes.c_int64), ("graph_replays", ctypes.c_int),
-                ("escalations", ctypes.c_int), ("converged", ctypes.c_int)]
+                ("escalations", ctypes.c_int), ("converged", ctypes.c_int), ("rounds", ctypes.c_int)]
 
 
 class PamServerOptions(ctypes.Structure):
@@ -146,6 +146,8 @@ _SIGS = {
     "pam_interpolant_weights": [_P, _P],
     "pam_interpolant_set_fitted": [_P, _P, _I64, _P],
     "pam_interpolant_evaluate_gradient": [_P, _P, _I64, ctypes.c_double, _P],
+    "pam_interpolant_fit_incrementally": [_P, _P, _I64, _P, ctypes.c_double, _INT, ctypes.c_double, _P],
+    "pam_distance_filter": [_P, _I64, ctypes.c_double, _P, _I64, _P, ctypes.POINTER(_I64)],
     "pam_interpolant_save": [_P, ctypes.c_char_p],
     "pam_interpolant_load": [ctypes.c_char_p, ctypes.POINTER(_P)],
     "pam_interpolant_model": [_P, ctypes.POINTER(_P)],

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "capi_handles.hpp"
+#include "incremental_fitter.hpp"
 #include "model_io.hpp"
 #include "ras.hpp"
 
@@ -207,6 +208,30 @@ int pam_interpolant_fit(pam_interpolant* h, const double* points, int64_t n, con
   });
 }
 
+int pam_interpolant_fit_incrementally(pam_interpolant* h, const double* points, int64_t n,
+                                      const double* values, double tolerance, int max_iter,
+                                      double accuracy, const pam_fit_options* opt) {
+  return guard([&] {
+    if (n < 0) pam::throw_invalid("negative point count");
+    if (n > 0 && (!points || !values)) pam::throw_invalid("null argument");
+    std::vector<double> p(points, points + 3 * n), v(values, values + n);
+    ip(h).fit_incrementally(p, v, tolerance, max_iter, accuracy, to_opts(opt));
+  });
+}
+
+int pam_distance_filter(const double* points, int64_t n, double distance, const int64_t* indices,
+                        int64_t n_idx, int64_t* out, int64_t* n_out) {
+  return guard([&] {
+    if (n < 0 || n_idx < 0) pam::throw_invalid("negative count");
+    if ((n > 0 && !points) || (n_idx > 0 && (!indices || !out)) || !n_out) pam::throw_invalid("null argument");
+    std::vector<double> p(points, points + 3 * n);
+    std::vector<int64_t> idx(indices, indices + n_idx);
+    std::vector<int64_t> kept = pam::DistanceFilter(p).filter(distance, idx);
+    std::copy(kept.begin(), kept.end(), out);
+    *n_out = static_cast<int64_t>(kept.size());
+  });
+}
+
 int pam_interpolant_evaluate(pam_interpolant* h, const double* points, int64_t n, double accuracy,
                              double* out) {
   return guard([&] { ip(h).evaluate(points, n, accuracy, out, false); });
@@ -309,6 +334,7 @@ int pam_interpolant_fit_stats(const pam_interpolant* h, pam_fit_stats* out) {
     out->graph_replays = s.graph_replays;
     out->escalations = s.escalations;
     out->converged = s.converged ? 1 : 0;
+    out->rounds = s.rounds;
   });
 }
 

@@ -1,0 +1,41 @@
+// Incremental fitting (interpolation/incremental_fitter.hpp:25-183) and the
+// distance filter it thins candidates with (point_cloud/distance_filter.hpp:16-99).
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "solver.hpp"
+
+namespace pam {
+
+// Greedy thinning: walks `indices` in order and drops every later point
+// within `distance` of a kept one (radius search on a uniform grid of cell
+// size `distance` instead of a k-d tree).
+class DistanceFilter {
+ public:
+  static constexpr double kMinDistance = 1.0842021724855044e-19;  // sqrt(float min normal)
+  explicit DistanceFilter(const std::vector<double>& xyz) : xyz_(xyz) {}
+  std::vector<int64_t> filter(double distance, const std::vector<int64_t>& indices) const;
+
+ private:
+  const std::vector<double>& xyz_;
+};
+
+struct IncrementalResult {
+  std::vector<int64_t> centers;  // indices into the points, in the order they were added
+  std::vector<double> weights;   // centres' weights then the polynomial coefficients
+  int rounds = 0;
+  int iterations = 0;  // FGMRES iterations over all rounds
+};
+
+// Fits on a growing subset of the points: each round solves on the current
+// centres, evaluates the residuals at the other points, and adds the points
+// whose residual is at least `tolerance`, largest first, thinned by a
+// distance that halves every round. Stops when no residual reaches the
+// tolerance or every point is a centre.
+IncrementalResult fit_incrementally(const Model& model, const std::vector<double>& points,
+                                    const std::vector<double>& values, double tolerance, int max_iter,
+                                    double accuracy, const FitOptions& opt, hipStream_t s);
+
+}  // namespace pam

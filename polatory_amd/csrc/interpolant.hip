@@ -1,6 +1,8 @@
 // Interpolant facade (see interpolant.hpp).
 #include "interpolant.hpp"
 
+#include "incremental_fitter.hpp"
+
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -108,6 +110,21 @@ void Interpolant::fit(const std::vector<double>& points, const std::vector<doubl
       bbox_lo_[k] = std::min(bbox_lo_[k], points[3 * i + k]);
       bbox_hi_[k] = std::max(bbox_hi_[k], points[3 * i + k]);
     }
+}
+
+void Interpolant::fit_incrementally(const std::vector<double>& points, const std::vector<double>& values,
+                                    double tolerance, int max_iter, double accuracy, const FitOptions& opt) {
+  fitted_ = false;
+  ev_.reset();
+  gev_.reset();
+  IncrementalResult r = pam::fit_incrementally(model_, points, values, tolerance, max_iter, accuracy, opt, s_);
+  std::vector<double> c(r.centers.size() * 3);
+  for (size_t i = 0; i < r.centers.size(); i++)
+    for (int k = 0; k < 3; k++) c[3 * i + k] = points[3 * r.centers[i] + k];
+  set_fitted(c, r.weights);
+  stats_ = FitStats{};
+  stats_.iterations = r.iterations;
+  stats_.rounds = r.rounds;
 }
 
 void Interpolant::set_fitted(const std::vector<double>& centers, const std::vector<double>& weights) {

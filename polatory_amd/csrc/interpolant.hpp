@@ -24,6 +24,10 @@ class Interpolant {
   // Gradients at grad points (the grad_points half of interpolant.hpp:58-67):
   // out has 3 n entries, x y z per point; accuracy is the grad_accuracy.
   void evaluate_gradient(const double* points, int64_t n, double accuracy, double* out3, bool on_device);
+  // interpolant.hpp:115-147: fits on a growing subset of the points; the
+  // centres are the points chosen, in the order they were added.
+  void fit_incrementally(const std::vector<double>& points, const std::vector<double>& values, double tolerance,
+                         int max_iter, double accuracy, const FitOptions& opt);
   // Installs centres and weights directly (a saved or externally fitted model).
   void set_fitted(const std::vector<double>& centers, const std::vector<double>& weights);
   void set_eval_config(const FmmConfig* c);

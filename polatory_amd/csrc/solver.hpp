@@ -96,6 +96,7 @@ struct FitStats {
   int n_levels = 0;
   int64_t factor_bytes = 0;
   std::vector<double> history;  // residual per iteration (sampled or exact)
+  int rounds = 0;               // incremental fit: rounds of solve + residual evaluation
 };
 
 class GraphStep;

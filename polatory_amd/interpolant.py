@@ -159,6 +159,18 @@ class Interpolant:
         _capi.call("pam_interpolant_fit", self._h, _ptr(p), p.shape[0], _ptr(v), float(tolerance),
                    int(max_iter), float(accuracy), opt, None if initial is None else initial._h)
 
+    def fit_incrementally(self, points, values, tolerance, max_iter=100, accuracy=math.inf,
+                          options: FitOptions = None):
+        """interpolant.hpp:115-147: fits on a growing subset of the points;
+        centers() then holds the chosen points in the order they were added."""
+        p = _points(points)
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        if v.shape[0] != p.shape[0]:
+            raise ValueError(f"values.rows() must be equal to {p.shape[0]}")
+        opt = None if options is None else ctypes.byref(options._c)
+        _capi.call("pam_interpolant_fit_incrementally", self._h, _ptr(p), p.shape[0], _ptr(v),
+                   float(tolerance), int(max_iter), float(accuracy), opt)
+
     def evaluate(self, points, grad_points=None, accuracy=math.inf, grad_accuracy=math.inf):
         """Values at points; with grad_points, the values followed by the
         gradients at grad_points (x y z per point), as interpolant.hpp:58-67
@@ -284,3 +296,16 @@ def fgmres_dense(a, rhs, n_iter, m=None, x0=None):
     _capi.call("pam_fgmres_dense", _ptr(a), None if mm is None else _ptr(mm), n, _ptr(rhs),
                None if x0a is None else _ptr(x0a), int(n_iter), _ptr(x), _ptr(hist), ctypes.byref(it))
     return x, hist, it.value
+
+
+def distance_filter(points, distance, indices=None):
+    """DistanceFilter(points).filter(distance, indices).filtered_indices()
+    (point_cloud/distance_filter.hpp:21-56)."""
+    p = _points(points)
+    idx = np.arange(p.shape[0], dtype=np.int64) if indices is None else \
+        np.ascontiguousarray(indices, dtype=np.int64)
+    out = np.empty(idx.shape[0], dtype=np.int64)
+    n_out = ctypes.c_int64()
+    _capi.call("pam_distance_filter", _ptr(p), p.shape[0], float(distance), _ptr(idx), idx.shape[0],
+               _ptr(out), ctypes.byref(n_out))
+    return out[:n_out.value]
