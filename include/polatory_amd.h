@@ -225,6 +225,15 @@ int pam_interpolant_fit(pam_interpolant* h, const double* points, int64_t n, con
 int pam_interpolant_fit_incrementally(pam_interpolant* h, const double* points, int64_t n,
                                       const double* values, double tolerance, int max_iter,
                                       double accuracy, const pam_fit_options* opt);
+/* fit_inequality(points, values, values_lb, values_ub, tolerance, max_iter = 100,
+ * accuracy = inf, initial = nullptr) — interpolant.hpp:149-190,
+ * interpolation/inequality_fitter.hpp:25-187. NaN in values / values_lb / values_ub
+ * means no equality / lower / upper constraint at that point. centers() then
+ * holds the equality points and the active bounds. opt and initial may be NULL. */
+int pam_interpolant_fit_inequality(pam_interpolant* h, const double* points, int64_t n, const double* values,
+                                   const double* values_lb, const double* values_ub, double tolerance,
+                                   int max_iter, double accuracy, const pam_fit_options* opt,
+                                   const pam_interpolant* initial);
 /* DistanceFilter(points).filter(distance, indices).filtered_indices()
  * (point_cloud/distance_filter.hpp:21-56): walks indices in order and drops every
  * later point within distance of a kept one. out receives the kept indices (at
@@ -268,7 +277,7 @@ typedef struct {
   int graph_replays;        /* FGMRES steps replayed from the captured graph */
   int escalations;          /* precision raises after FGMRES stagnated (FP64 level updates, then operator) */
   int converged;            /* 0 when keep_unconverged stopped the fit at max_iter */
-  int rounds;               /* pam_interpolant_fit_incrementally: rounds of solve + residuals */
+  int rounds;               /* incremental and inequality fits: rounds of solve + residuals */
 } pam_fit_stats;
 int pam_interpolant_fit_stats(const pam_interpolant* h, pam_fit_stats* out);
 /* Residual after each FGMRES iteration (out may be NULL to query the length). */

@@ -219,6 +219,19 @@ int pam_interpolant_fit_incrementally(pam_interpolant* h, const double* points, 
   });
 }
 
+int pam_interpolant_fit_inequality(pam_interpolant* h, const double* points, int64_t n, const double* values,
+                                   const double* values_lb, const double* values_ub, double tolerance,
+                                   int max_iter, double accuracy, const pam_fit_options* opt,
+                                   const pam_interpolant* initial) {
+  return guard([&] {
+    if (n < 0) pam::throw_invalid("negative point count");
+    if (n > 0 && (!points || !values || !values_lb || !values_ub)) pam::throw_invalid("null argument");
+    std::vector<double> p(points, points + 3 * n), v(values, values + n), lo(values_lb, values_lb + n),
+        hi(values_ub, values_ub + n);
+    ip(h).fit_inequality(p, v, lo, hi, tolerance, max_iter, accuracy, to_opts(opt), initial ? &ip(initial) : nullptr);
+  });
+}
+
 int pam_distance_filter(const double* points, int64_t n, double distance, const int64_t* indices,
                         int64_t n_idx, int64_t* out, int64_t* n_out) {
   return guard([&] {

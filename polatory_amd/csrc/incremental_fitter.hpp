@@ -1,5 +1,7 @@
-// Incremental fitting (interpolation/incremental_fitter.hpp:25-183) and the
-// distance filter it thins candidates with (point_cloud/distance_filter.hpp:16-99).
+// Fitting on subsets of the points: the incremental fitter
+// (interpolation/incremental_fitter.hpp:25-183), the inequality fitter
+// (interpolation/inequality_fitter.hpp:25-187) and the distance filter both
+// thin their candidates with (point_cloud/distance_filter.hpp:16-99).
 #pragma once
 
 #include <cstdint>
@@ -37,5 +39,23 @@ struct IncrementalResult {
 IncrementalResult fit_incrementally(const Model& model, const std::vector<double>& points,
                                     const std::vector<double>& values, double tolerance, int max_iter,
                                     double accuracy, const FitOptions& opt, hipStream_t s);
+
+struct InequalityResult {
+  std::vector<int64_t> centers;  // equality points, then the active bounds in index order
+  std::vector<double> weights;   // centres' weights then the polynomial coefficients
+  int rounds = 0;
+  int iterations = 0;
+};
+
+// Fits values where they are given (not NaN) and keeps the fit within
+// [lb, ub] where bounds are given, by an active set: a bound joins the
+// centres (fitted at the bound) when the fit violates it by more than
+// `tolerance`, thinned by a halving distance, and leaves when its weight
+// takes the sign that pulls the fit back inside. initial: n + l weights or
+// nullptr.
+InequalityResult fit_inequality(const Model& model, const std::vector<double>& points,
+                                const std::vector<double>& values, const std::vector<double>& lb,
+                                const std::vector<double>& ub, double tolerance, int max_iter, double accuracy,
+                                const FitOptions& opt, const std::vector<double>* initial, hipStream_t s);
 
 }  // namespace pam

@@ -127,6 +127,26 @@ void Interpolant::fit_incrementally(const std::vector<double>& points, const std
   stats_.rounds = r.rounds;
 }
 
+void Interpolant::fit_inequality(const std::vector<double>& points, const std::vector<double>& values,
+                                 const std::vector<double>& lb, const std::vector<double>& ub, double tolerance,
+                                 int max_iter, double accuracy, const FitOptions& opt, const Interpolant* initial) {
+  // The initial weights are built before clearing: `initial` may be this.
+  std::vector<double> init;
+  if (initial && initial->fitted_) init = build_initial_weights(points, *initial);
+  fitted_ = false;
+  ev_.reset();
+  gev_.reset();
+  InequalityResult r = pam::fit_inequality(model_, points, values, lb, ub, tolerance, max_iter, accuracy, opt,
+                                           init.empty() ? nullptr : &init, s_);
+  std::vector<double> c(r.centers.size() * 3);
+  for (size_t i = 0; i < r.centers.size(); i++)
+    for (int k = 0; k < 3; k++) c[3 * i + k] = points[3 * r.centers[i] + k];
+  set_fitted(c, r.weights);
+  stats_ = FitStats{};
+  stats_.iterations = r.iterations;
+  stats_.rounds = r.rounds;
+}
+
 void Interpolant::set_fitted(const std::vector<double>& centers, const std::vector<double>& weights) {
   int64_t mu = static_cast<int64_t>(centers.size() / 3);
   if (static_cast<int64_t>(weights.size()) != mu + model_.l()) throw_invalid("weights.rows() must be centers.rows() + l");

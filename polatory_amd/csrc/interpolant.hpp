@@ -28,6 +28,11 @@ class Interpolant {
   // centres are the points chosen, in the order they were added.
   void fit_incrementally(const std::vector<double>& points, const std::vector<double>& values, double tolerance,
                          int max_iter, double accuracy, const FitOptions& opt);
+  // interpolant.hpp:149-190: values where not NaN, bounds lb/ub where not NaN;
+  // the centres are the equality points, then the active bounds.
+  void fit_inequality(const std::vector<double>& points, const std::vector<double>& values,
+                      const std::vector<double>& lb, const std::vector<double>& ub, double tolerance, int max_iter,
+                      double accuracy, const FitOptions& opt, const Interpolant* initial);
   // Installs centres and weights directly (a saved or externally fitted model).
   void set_fitted(const std::vector<double>& centers, const std::vector<double>& weights);
   void set_eval_config(const FmmConfig* c);

@@ -171,6 +171,24 @@ class Interpolant:
         _capi.call("pam_interpolant_fit_incrementally", self._h, _ptr(p), p.shape[0], _ptr(v),
                    float(tolerance), int(max_iter), float(accuracy), opt)
 
+    def fit_inequality(self, points, values, values_lb, values_ub, tolerance, max_iter=100,
+                       accuracy=math.inf, initial=None, options: FitOptions = None):
+        """interpolant.hpp:149-190: NaN means no equality / lower / upper
+        constraint at that point; centers() then holds the equality points and
+        the active bounds."""
+        p = _points(points)
+        n = p.shape[0]
+        arrs = []
+        for name, a in (("values", values), ("values_lb", values_lb), ("values_ub", values_ub)):
+            a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+            if a.shape[0] != n:
+                raise ValueError(f"{name}.rows() must be equal to points.rows()")
+            arrs.append(a)
+        opt = None if options is None else ctypes.byref(options._c)
+        _capi.call("pam_interpolant_fit_inequality", self._h, _ptr(p), n, _ptr(arrs[0]), _ptr(arrs[1]),
+                   _ptr(arrs[2]), float(tolerance), int(max_iter), float(accuracy), opt,
+                   None if initial is None else initial._h)
+
     def evaluate(self, points, grad_points=None, accuracy=math.inf, grad_accuracy=math.inf):
         """Values at points; with grad_points, the values followed by the
         gradients at grad_points (x y z per point), as interpolant.hpp:58-67

@@ -1,12 +1,15 @@
 """Incremental fitting (interpolation/incremental_fitter.hpp:25-183,
-Interpolant::fit_incrementally, interpolant.hpp:115-147) and its distance
-filter (point_cloud/distance_filter.hpp:16-99).
+Interpolant::fit_incrementally, interpolant.hpp:115-147), inequality fitting
+(interpolation/inequality_fitter.hpp:25-187, interpolant.hpp:149-190) and the
+distance filter both use (point_cloud/distance_filter.hpp:16-99).
 
 The filter is pinned against a brute-force greedy walk on small inputs. The
 fitter is checked as the reference's own test checks it (every point within
 the tolerance afterwards, test/interpolation/test_incremental_fitter.cpp), and
 its first rounds, which pick centres before any solve (the fit is zero until
-the polynomial is determined), are replayed with the brute-force filter.
+the polynomial is determined), are replayed with the brute-force filter. The
+inequality fitter is checked on the reference test's two cases: the Kostov
+(1986) problem with its data, and bounds only around a smooth function.
 """
 import numpy as np
 import pytest
@@ -102,3 +105,61 @@ def test_incremental_fit_without_polynomial():
     c = ip.centers()
     assert np.array_equal(c[0], pts[np.argmax(np.abs(vals))])
     assert np.abs(ip.evaluate(pts, accuracy=1e-5) - vals).max() < 1e-3
+
+
+def kostov86():
+    """The 1-D inequality problem of the reference's test
+    (test/interpolation/test_inequality_fitter.cpp:60-109, after Kostov 1986),
+    placed on the x axis of 3-D space (the exponential kernel sees distances only)."""
+    nan = np.nan
+    values = [1, nan, 4, nan, 3, nan, nan, nan, nan, nan, 3, nan, nan, 7, nan, 8, nan, nan, nan, 8, 3,
+              nan, nan, 6, nan]
+    lb = [nan, 6, nan, 2, nan, 2, 9, 4, 3, 3, nan, nan, 7, nan, 4, nan, nan, nan, 5, nan, nan, 9, 5, nan, nan]
+    ub = [nan, nan, nan, 4, nan, 4, nan, nan, nan, nan, nan, 1, nan, nan, nan, nan, 4, 7, nan, nan, nan, nan,
+          nan, nan, 3]
+    pts = np.zeros((25, 3))
+    pts[:, 0] = np.arange(25)
+    return pts, np.array(values, float), np.array(lb, float), np.array(ub, float)
+
+
+@pytest.mark.gpu
+def test_inequality_fit_kostov86():
+    pts, v, lb, ub = kostov86()
+    tol = 1e-5
+    ip = pa.Interpolant(pa.Model(pa.CovExponential([1.0, 3.0]), poly_degree=-1))
+    ip.fit_inequality(pts, v, lb, ub, tol, max_iter=32, accuracy=tol / 100)
+    fit = ip.evaluate(pts, accuracy=tol / 100)
+    eq = ~np.isnan(v)
+    assert np.abs(fit[eq] - v[eq]).max() < tol
+    hl, hu = ~np.isnan(lb), ~np.isnan(ub)
+    assert (fit[hl] > lb[hl] - tol).all() and (fit[hu] < ub[hu] + tol).all()
+    c = ip.centers()
+    # the equality points come first, then the active bounds in index order
+    assert np.array_equal(c[:eq.sum()], pts[eq])
+    act = c[eq.sum():, 0].astype(int)
+    assert list(act) == sorted(act) and not eq[act].any()
+    print(f"kostov86: {len(act)} active bounds at x = {[int(i) for i in act]}, "
+          f"rounds {ip.fit_stats()['rounds']}", flush=True)
+    # an active bound is fitted at one of its bounds
+    for i in act:
+        gaps = [abs(fit[i] - b) for b in (lb[i], ub[i]) if not np.isnan(b)]
+        assert min(gaps) < tol
+
+
+@pytest.mark.gpu
+def test_inequality_fit_bounds_only():
+    """The reference's inequality_only case (test_inequality_fitter.cpp:25-58):
+    no values, bounds of +-0.001 around a smooth function."""
+    pts, vals = sample(10_000, 33)
+    tol = 1e-4
+    rbf = pa.Biharmonic3D([1.0])
+    rbf.set_anisotropy(A)
+    ip = pa.Interpolant(pa.Model(rbf, poly_degree=0))
+    lb, ub = vals - 0.001, vals + 0.001
+    ip.fit_inequality(pts, np.full(len(pts), np.nan), lb, ub, tol, max_iter=32, accuracy=tol / 100)
+    st = ip.fit_stats()
+    n = len(ip.centers())
+    print(f"bounds only: {n} of {len(pts)} points active in {st['rounds']} rounds", flush=True)
+    assert ip.weights().shape == (n + 1,)
+    fit = ip.evaluate(pts, accuracy=tol / 100)
+    assert (fit > lb - tol).all() and (fit < ub + tol).all()
