@@ -234,6 +234,13 @@ int pam_interpolant_fit_inequality(pam_interpolant* h, const double* points, int
                                    const double* values_lb, const double* values_ub, double tolerance,
                                    int max_iter, double accuracy, const pam_fit_options* opt,
                                    const pam_interpolant* initial);
+/* kriging::cross_validate(model, points, values, set_ids, tolerance, max_iter, accuracy)
+ * — kriging/cross_validate.hpp:14-47: for each set id, fits on the points of the
+ * other sets and writes the predictions at the set's points into out (n). opt may
+ * be NULL. */
+int pam_cross_validate(const pam_model* model, const double* points, int64_t n, const double* values,
+                       const int32_t* set_ids, double tolerance, int max_iter, double accuracy,
+                       const pam_fit_options* opt, double* out);
 /* DistanceFilter(points).filter(distance, indices).filtered_indices()
  * (point_cloud/distance_filter.hpp:21-56): walks indices in order and drops every
  * later point within distance of a kept one. out receives the kept indices (at

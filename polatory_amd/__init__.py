@@ -8,8 +8,8 @@ from . import _capi
 from .fmm import (FmmConfig, FmmEvaluator, FmmGradientTransposeEvaluator, FmmSymmetricEvaluator,
                   direct_evaluate, direct_evaluate_gradient, make_fmm_evaluator,
                   make_fmm_gradient_transpose_evaluator, make_fmm_symmetric_evaluator)
-from .interpolant import (FitOptions, Interpolant, Model, RasPreconditioner, distance_filter,
-                          fgmres_dense, POLY_DEGREE_MIN)
+from .interpolant import (FitOptions, Interpolant, Model, RasPreconditioner, cross_validate,
+                          distance_filter, fgmres_dense, POLY_DEGREE_MIN)
 from .rbf import Biharmonic3D, CovExponential, Rbf, Triharmonic3D, make_rbf
 from .serve import Server
 
@@ -26,4 +26,5 @@ __all__ = ["Biharmonic3D", "Triharmonic3D", "CovExponential", "Rbf", "make_rbf",
            "make_fmm_evaluator", "make_fmm_symmetric_evaluator", "make_fmm_gradient_transpose_evaluator", "direct_evaluate", "direct_evaluate_gradient",
            "device_count", "Model",
            "Interpolant", "FitOptions", "RasPreconditioner", "fgmres_dense", "POLY_DEGREE_MIN", "distance_filter",
+           "cross_validate",
            "Server"]

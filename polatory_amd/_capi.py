@@ -149,6 +149,7 @@ _SIGS = {
     "pam_interpolant_fit_incrementally": [_P, _P, _I64, _P, ctypes.c_double, _INT, ctypes.c_double, _P],
     "pam_interpolant_fit_inequality": [_P, _P, _I64, _P, _P, _P, ctypes.c_double, _INT, ctypes.c_double, _P,
                                        _P],
+    "pam_cross_validate": [_P, _P, _I64, _P, _P, ctypes.c_double, _INT, ctypes.c_double, _P, _P],
     "pam_distance_filter": [_P, _I64, ctypes.c_double, _P, _I64, _P, ctypes.POINTER(_I64)],
     "pam_interpolant_save": [_P, ctypes.c_char_p],
     "pam_interpolant_load": [ctypes.c_char_p, ctypes.POINTER(_P)],

@@ -232,6 +232,20 @@ int pam_interpolant_fit_inequality(pam_interpolant* h, const double* points, int
   });
 }
 
+int pam_cross_validate(const pam_model* model, const double* points, int64_t n, const double* values,
+                       const int32_t* set_ids, double tolerance, int max_iter, double accuracy,
+                       const pam_fit_options* opt, double* out) {
+  return guard([&] {
+    if (!model) pam::throw_invalid("null model");
+    if (n < 0) pam::throw_invalid("negative point count");
+    if (n > 0 && (!points || !values || !set_ids || !out)) pam::throw_invalid("null argument");
+    std::vector<double> p(points, points + 3 * n), v(values, values + n);
+    std::vector<int32_t> ids(set_ids, set_ids + n);
+    std::vector<double> pred = pam::cross_validate(model->m, p, v, ids, tolerance, max_iter, accuracy, to_opts(opt));
+    std::copy(pred.begin(), pred.end(), out);
+  });
+}
+
 int pam_distance_filter(const double* points, int64_t n, double distance, const int64_t* indices,
                         int64_t n_idx, int64_t* out, int64_t* n_out) {
   return guard([&] {

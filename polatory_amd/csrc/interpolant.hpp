@@ -65,4 +65,10 @@ class Interpolant {
   double bbox_lo_[3] = {0, 0, 0}, bbox_hi_[3] = {0, 0, 0};
 };
 
+// kriging/cross_validate.hpp:14-47: prediction at every point from the fit
+// on the points of the other sets (set_ids labels the folds).
+std::vector<double> cross_validate(const Model& model, const std::vector<double>& points,
+                                   const std::vector<double>& values, const std::vector<int32_t>& set_ids,
+                                   double tolerance, int max_iter, double accuracy, const FitOptions& opt);
+
 }  // namespace pam

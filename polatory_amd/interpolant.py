@@ -327,3 +327,19 @@ def distance_filter(points, distance, indices=None):
     _capi.call("pam_distance_filter", _ptr(p), p.shape[0], float(distance), _ptr(idx), idx.shape[0],
                _ptr(out), ctypes.byref(n_out))
     return out[:n_out.value]
+
+
+def cross_validate(model: Model, points, values, set_ids, tolerance, max_iter=100, accuracy=math.inf,
+                   options: FitOptions = None):
+    """kriging::cross_validate (kriging/cross_validate.hpp:14-47): for each set
+    id, the predictions at its points from the fit on the other points."""
+    p = _points(points)
+    v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+    ids = np.ascontiguousarray(set_ids, dtype=np.int32).reshape(-1)
+    if v.shape[0] != p.shape[0] or ids.shape[0] != p.shape[0]:
+        raise ValueError("values and set_ids must have one entry per point")
+    out = np.empty(p.shape[0])
+    opt = None if options is None else ctypes.byref(options._c)
+    _capi.call("pam_cross_validate", model._h, _ptr(p), p.shape[0], _ptr(v), _ptr(ids), float(tolerance),
+               int(max_iter), float(accuracy), opt, _ptr(out))
+    return out
