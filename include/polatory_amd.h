@@ -68,6 +68,13 @@ int pam_fmm_symmetric_evaluator_create(const pam_rbf* rbf, const double bbox_min
  * pam_fmm_evaluator_evaluate_gradient) and whose accuracy search measures them. */
 int pam_fmm_gradient_transpose_evaluator_create(const pam_rbf* rbf, const double bbox_min[3],
                                                 const double bbox_max[3], pam_fmm_evaluator** out);
+/* make_fmm_gradient_evaluator(rbf, bbox) — make_fmm_evaluator.cpp:78-107,
+ * fmm/gradient_kernel.hpp:46-62: sources carry gradient weights (set_weights takes
+ * 3 n_sources entries, x y z per source) and evaluate() returns n_targets values,
+ * out[i] = sum_j w_j . (-A^T grad phi(A (t_i - s_j))). Evaluated as three
+ * gradient-transpose passes; the accuracy search measures those gradients. */
+int pam_fmm_gradient_evaluator_create(const pam_rbf* rbf, const double bbox_min[3], const double bbox_max[3],
+                                      pam_fmm_evaluator** out);
 int pam_fmm_evaluator_destroy(pam_fmm_evaluator* h);
 /* set_accuracy(double) — absolute L-inf accuracy; +inf = default configuration (fmm_evaluator.hpp:36) */
 int pam_fmm_evaluator_set_accuracy(pam_fmm_evaluator* h, double accuracy);

@@ -7,11 +7,13 @@
 #include "../../include/polatory_amd.h"
 #include "capi_util.hpp"
 #include "device_api.hpp"
+#include "fmm_dipole.hpp"
 #include "fmm_evaluator.hpp"
 
 struct pam_fmm_evaluator {
   std::unique_ptr<pam::FmmEvaluator> impl;
   bool symmetric;
+  std::unique_ptr<pam::DipoleSourceEvaluator> dipole;  // gradient-kernel handles
 };
 
 namespace pam {
@@ -102,6 +104,16 @@ int pam_fmm_gradient_transpose_evaluator_create(const pam_rbf* rbf, const double
   return guard([&] { ev(*out).set_gradient_kernel(true); });
 }
 
+int pam_fmm_gradient_evaluator_create(const pam_rbf* rbf, const double bbox_min[3], const double bbox_max[3],
+                                      pam_fmm_evaluator** out) {
+  int rc = create(rbf, bbox_min, bbox_max, out, false);
+  if (rc != 0) return rc;
+  return guard([&] {
+    ev(*out).set_gradient_kernel(true);
+    (*out)->dipole = std::make_unique<pam::DipoleSourceEvaluator>(ev(*out));
+  });
+}
+
 int pam_fmm_evaluator_destroy(pam_fmm_evaluator* h) {
   return guard([&] { delete h; });
 }
@@ -129,11 +141,17 @@ int pam_fmm_evaluator_set_points(pam_fmm_evaluator* h, const double* xyz, int64_
 }
 
 int pam_fmm_evaluator_set_weights(pam_fmm_evaluator* h, const double* w, int64_t n) {
-  return guard([&] { ev(h).set_weights(w, n, false); });
+  return guard([&] {
+    if (h && h->dipole) h->dipole->set_weights(w, n, false);
+    else ev(h).set_weights(w, n, false);
+  });
 }
 
 int pam_fmm_evaluator_evaluate(pam_fmm_evaluator* h, double* out, int64_t n) {
-  return guard([&] { ev(h).evaluate(out, n, false); });
+  return guard([&] {
+    if (h && h->dipole) h->dipole->evaluate(out, n, false);
+    else ev(h).evaluate(out, n, false);
+  });
 }
 
 int pam_fmm_evaluator_set_source_points_device(pam_fmm_evaluator* h, const double* d_xyz, int64_t n) {
@@ -155,19 +173,31 @@ int pam_fmm_evaluator_set_points_device(pam_fmm_evaluator* h, const double* d_xy
 }
 
 int pam_fmm_evaluator_set_weights_device(pam_fmm_evaluator* h, const double* d_w, int64_t n) {
-  return guard([&] { ev(h).set_weights(d_w, n, true); });
+  return guard([&] {
+    if (h && h->dipole) h->dipole->set_weights(d_w, n, true);
+    else ev(h).set_weights(d_w, n, true);
+  });
 }
 
 int pam_fmm_evaluator_evaluate_device(pam_fmm_evaluator* h, double* d_out, int64_t n) {
-  return guard([&] { ev(h).evaluate(d_out, n, true); });
+  return guard([&] {
+    if (h && h->dipole) h->dipole->evaluate(d_out, n, true);
+    else ev(h).evaluate(d_out, n, true);
+  });
 }
 
 int pam_fmm_evaluator_evaluate_gradient(pam_fmm_evaluator* h, double* out, int64_t n) {
-  return guard([&] { ev(h).evaluate_gradient(out, n, false); });
+  return guard([&] {
+    if (h && h->dipole) pam::throw_runtime("a gradient-kernel handle evaluates values (pam_fmm_evaluator_evaluate)");
+    ev(h).evaluate_gradient(out, n, false);
+  });
 }
 
 int pam_fmm_evaluator_evaluate_gradient_device(pam_fmm_evaluator* h, double* d_out, int64_t n) {
-  return guard([&] { ev(h).evaluate_gradient(d_out, n, true); });
+  return guard([&] {
+    if (h && h->dipole) pam::throw_runtime("a gradient-kernel handle evaluates values (pam_fmm_evaluator_evaluate)");
+    ev(h).evaluate_gradient(d_out, n, true);
+  });
 }
 
 int pam_fmm_evaluator_synchronize(pam_fmm_evaluator* h) {

@@ -250,3 +250,18 @@ class FmmGradientTransposeEvaluator(FmmEvaluator):
         out = np.empty((self.n_trg, 3), dtype=np.float64)
         _capi.call("pam_fmm_evaluator_evaluate", self._h, _ptr(out), self.n_trg)
         return out
+
+
+class FmmGradientEvaluator(FmmEvaluator):
+    """make_fmm_gradient_evaluator (make_fmm_evaluator.cpp:78-107): sources carry
+    gradient weights, (n_src, 3); evaluate() returns values at the targets,
+    sum_j w_j . (-A^T grad phi(A (t_i - s_j)))."""
+
+    _create = "pam_fmm_gradient_evaluator_create"
+
+    def evaluate_gradient(self):
+        raise RuntimeError("a gradient-kernel evaluator evaluates values")
+
+
+def make_fmm_gradient_evaluator(rbf, bbox=None):
+    return FmmGradientEvaluator(rbf, bbox)
