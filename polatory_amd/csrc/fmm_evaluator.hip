@@ -383,6 +383,22 @@ void FmmEvaluator::ensure_trees() {
     return;
   }
   bool rebuilt = false;
+  // The two trees are independent: with the stage overlap on, the target
+  // tree is built on the second stream with its own workspace while the
+  // source tree is built here.
+  const bool par = overlap_ && !symmetric_;
+  hipStream_t ts = par ? p2p_stream_ : stream_;
+  TreeWorkspace& tws = par ? ws_trg_ : ws_;
+  auto fork = [&]() {
+    if (!par) return;
+    PAM_HIP(hipEventRecord(fork_ev_, stream_));
+    PAM_HIP(hipStreamWaitEvent(ts, fork_ev_, 0));
+  };
+  auto join = [&]() {
+    if (!par) return;
+    PAM_HIP(hipEventRecord(join_ev_, ts));
+    PAM_HIP(hipStreamWaitEvent(stream_, join_ev_, 0));
+  };
   auto build_src = [&](int level) {
     build_tree(src_tree_, src_pts_.get(), n_src_, box_, level, ws_, stream_);
     src_tree_dirty_ = false;
@@ -391,7 +407,7 @@ void FmmEvaluator::ensure_trees() {
     st_.multipoles_valid = false;
   };
   auto build_trg = [&](int level) {
-    build_tree(trg_tree_, trg_pts_.get(), n_trg_, box_, level, ws_, stream_);
+    build_tree(trg_tree_, trg_pts_.get(), n_trg_, box_, level, tws, ts);
     trg_tree_dirty_ = false;
     rebuilt = true;
     st_.t32_valid = st_.t64_valid = false;
@@ -399,8 +415,10 @@ void FmmEvaluator::ensure_trees() {
   int L;
   if (cfg_.leaf_level >= 0) {
     L = cfg_.leaf_level;
-    if (src_tree_dirty_ || src_tree_.level != L) build_src(L);
+    fork();
     if (!symmetric_ && (trg_tree_dirty_ || trg_tree_.level != L)) build_trg(L);
+    if (src_tree_dirty_ || src_tree_.level != L) build_src(L);
+    join();
     src_counts_.clear();
     trg_counts_.clear();
   } else {
@@ -416,12 +434,14 @@ void FmmEvaluator::ensure_trees() {
     const int L0 = auto_leaf_level(n_src_, n_trg(), kPointsPerLeaf);
     const int Lmax = std::min(L0 + kLevelsBeyond, 18);
     auto sort_both = [&](bool do_src, bool do_trg) {
+      fork();
+      if (do_trg) sort_tree_points(trg_tree_, trg_pts_.get(), n_trg_, box_, Lmax, tws, ts);
       if (do_src) sort_tree_points(src_tree_, src_pts_.get(), n_src_, box_, Lmax, ws_, stream_);
-      if (do_trg) sort_tree_points(trg_tree_, trg_pts_.get(), n_trg_, box_, Lmax, ws_, stream_);
       std::vector<int64_t> hs(static_cast<size_t>(Lmax) + 1), ht(static_cast<size_t>(Lmax) + 1);
       const size_t bytes = hs.size() * sizeof(int64_t);
+      if (do_trg) PAM_HIP(hipMemcpyAsync(ht.data(), trg_tree_.open_hist.get(), bytes, hipMemcpyDeviceToHost, ts));
       if (do_src) PAM_HIP(hipMemcpyAsync(hs.data(), src_tree_.open_hist.get(), bytes, hipMemcpyDeviceToHost, stream_));
-      if (do_trg) PAM_HIP(hipMemcpyAsync(ht.data(), trg_tree_.open_hist.get(), bytes, hipMemcpyDeviceToHost, stream_));
+      join();
       PAM_HIP(hipStreamSynchronize(stream_));
       if (do_src) {
         src_counts_ = level_counts(hs);
@@ -458,17 +478,19 @@ void FmmEvaluator::ensure_trees() {
     const bool resort_src = src_tree_.sort_level < L;
     const bool resort_trg = !symmetric_ && trg_tree_.sort_level < L;
     if (resort_src || resort_trg) sort_both(resort_src, resort_trg);
+    fork();
+    if (!symmetric_ && (sort_trg || resort_trg || trg_tree_.level != L)) {
+      finish_tree(trg_tree_, L, trg_counts_, tws, ts);
+      rebuilt = true;
+      st_.t32_valid = st_.t64_valid = false;
+    }
     if (sort_src || resort_src || src_tree_.level != L) {
       finish_tree(src_tree_, L, src_counts_, ws_, stream_);
       rebuilt = true;
       st_.s32_valid = st_.s64_valid = false;
       st_.multipoles_valid = false;
     }
-    if (!symmetric_ && (sort_trg || resort_trg || trg_tree_.level != L)) {
-      finish_tree(trg_tree_, L, trg_counts_, ws_, stream_);
-      rebuilt = true;
-      st_.t32_valid = st_.t64_valid = false;
-    }
+    join();
   }
   if (symmetric_) trg_tree_dirty_ = false;
   if (rebuilt) {

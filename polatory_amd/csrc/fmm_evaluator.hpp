@@ -164,6 +164,7 @@ class FmmEvaluator {
   // all cells of each level as M2L columns (compress / decompress transforms)
   std::vector<M2LLevel> src_cols_, trg_cols_;
   TreeWorkspace ws_;
+  TreeWorkspace ws_trg_;  // the target tree's, when it is built on the second stream
   DevBuf<double> direct_scratch_;
 
   FmmState st_;
