@@ -13,7 +13,8 @@
 struct pam_fmm_evaluator {
   std::unique_ptr<pam::FmmEvaluator> impl;
   bool symmetric;
-  std::unique_ptr<pam::DipoleSourceEvaluator> dipole;  // gradient-kernel handles
+  std::unique_ptr<pam::DipoleSourceEvaluator> dipole;   // gradient-kernel handles
+  std::unique_ptr<pam::DipoleHessianEvaluator> hessian;  // Hessian-kernel handles
 };
 
 namespace pam {
@@ -114,6 +115,16 @@ int pam_fmm_gradient_evaluator_create(const pam_rbf* rbf, const double bbox_min[
   });
 }
 
+int pam_fmm_hessian_evaluator_create(const pam_rbf* rbf, const double bbox_min[3], const double bbox_max[3],
+                                     pam_fmm_evaluator** out) {
+  int rc = create(rbf, bbox_min, bbox_max, out, false);
+  if (rc != 0) return rc;
+  return guard([&] {
+    ev(*out).set_gradient_kernel(true);
+    (*out)->hessian = std::make_unique<pam::DipoleHessianEvaluator>(ev(*out));
+  });
+}
+
 int pam_fmm_evaluator_destroy(pam_fmm_evaluator* h) {
   return guard([&] { delete h; });
 }
@@ -143,6 +154,7 @@ int pam_fmm_evaluator_set_points(pam_fmm_evaluator* h, const double* xyz, int64_
 int pam_fmm_evaluator_set_weights(pam_fmm_evaluator* h, const double* w, int64_t n) {
   return guard([&] {
     if (h && h->dipole) h->dipole->set_weights(w, n, false);
+    else if (h && h->hessian) h->hessian->set_weights(w, n, false);
     else ev(h).set_weights(w, n, false);
   });
 }
@@ -150,6 +162,7 @@ int pam_fmm_evaluator_set_weights(pam_fmm_evaluator* h, const double* w, int64_t
 int pam_fmm_evaluator_evaluate(pam_fmm_evaluator* h, double* out, int64_t n) {
   return guard([&] {
     if (h && h->dipole) h->dipole->evaluate(out, n, false);
+    else if (h && h->hessian) h->hessian->evaluate(out, n, false);
     else ev(h).evaluate(out, n, false);
   });
 }
@@ -175,6 +188,7 @@ int pam_fmm_evaluator_set_points_device(pam_fmm_evaluator* h, const double* d_xy
 int pam_fmm_evaluator_set_weights_device(pam_fmm_evaluator* h, const double* d_w, int64_t n) {
   return guard([&] {
     if (h && h->dipole) h->dipole->set_weights(d_w, n, true);
+    else if (h && h->hessian) h->hessian->set_weights(d_w, n, true);
     else ev(h).set_weights(d_w, n, true);
   });
 }
@@ -182,20 +196,21 @@ int pam_fmm_evaluator_set_weights_device(pam_fmm_evaluator* h, const double* d_w
 int pam_fmm_evaluator_evaluate_device(pam_fmm_evaluator* h, double* d_out, int64_t n) {
   return guard([&] {
     if (h && h->dipole) h->dipole->evaluate(d_out, n, true);
+    else if (h && h->hessian) h->hessian->evaluate(d_out, n, true);
     else ev(h).evaluate(d_out, n, true);
   });
 }
 
 int pam_fmm_evaluator_evaluate_gradient(pam_fmm_evaluator* h, double* out, int64_t n) {
   return guard([&] {
-    if (h && h->dipole) pam::throw_runtime("a gradient-kernel handle evaluates values (pam_fmm_evaluator_evaluate)");
+    if (h && (h->dipole || h->hessian)) pam::throw_runtime("use pam_fmm_evaluator_evaluate on this handle");
     ev(h).evaluate_gradient(out, n, false);
   });
 }
 
 int pam_fmm_evaluator_evaluate_gradient_device(pam_fmm_evaluator* h, double* d_out, int64_t n) {
   return guard([&] {
-    if (h && h->dipole) pam::throw_runtime("a gradient-kernel handle evaluates values (pam_fmm_evaluator_evaluate)");
+    if (h && (h->dipole || h->hessian)) pam::throw_runtime("use pam_fmm_evaluator_evaluate on this handle");
     ev(h).evaluate_gradient(d_out, n, true);
   });
 }

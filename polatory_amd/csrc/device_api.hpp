@@ -36,6 +36,11 @@ void direct_sum_f64(const RbfDesc& rbf, const double* src, const double* w, int6
 void direct_gradient_f64(const RbfDesc& rbf, const double* src, const double* w, int64_t n_src,
                          const double* trg, int64_t n_trg, double* out3, DevBuf<double>& scratch,
                          hipStream_t s);
+// FP64 direct sum of column kcol of the Hessian kernel: out3[3 i + c] =
+// (A^T sum_j w[j] H(t_i - s_j) A e_kcol)_c over transformed points (rbf_eval.hip).
+void direct_hessian_column_f64(const RbfDesc& rbf, const double* src, const double* w, int64_t n_src,
+                               const double* trg, int64_t n_trg, int kcol, double* out3, DevBuf<double>& scratch,
+                               hipStream_t s);
 // Batched FP64 direct sums over host query sets, for small evaluations of
 // several resident models at once: one upload, one direct-sum launch, one
 // reduction, one download. A term is one rbf of a model (its transformed

@@ -679,6 +679,221 @@ p2p_grad_kernel(const int32_t* __restrict__ tls, const int32_t* __restrict__ sls
   }
 }
 
+// ---- one column of the Hessian at the targets (the Hessian kernel,
+// fmm/hessian_kernel.hpp:46-64, applied to one weight component):
+// out_i = A^T sum_j w_j H(A (t_i - s_j)) a_k, a_k the k-th column of A ----
+
+// S_a, S_a' and S_a'' at x: T_k'' from T''_{k+1} = 4 T'_k + 2x T''_k - T''_{k-1}.
+template <class T, int P, int NC>
+__device__ __forceinline__ void lagrange_dd_lds(T xi, const T* __restrict__ tk, T (*s)[NC], T (*d)[NC],
+                                                T (*dd)[NC], int col) {
+  T t[P], d1[P], d2[P];
+  t[0] = T(1);
+  d1[0] = T(0);
+  d2[0] = T(0);
+  if constexpr (P > 1) {
+    t[1] = xi;
+    d1[1] = T(1);
+    d2[1] = T(0);
+  }
+#pragma unroll
+  for (int k = 2; k < P; k++) {
+    t[k] = T(2) * xi * t[k - 1] - t[k - 2];
+    d1[k] = T(2) * t[k - 1] + T(2) * xi * d1[k - 1] - d1[k - 2];
+    d2[k] = T(4) * d1[k - 1] + T(2) * xi * d2[k - 1] - d2[k - 2];
+  }
+  const T c0 = T(1) / T(P), c1 = T(2) / T(P);
+#pragma unroll 1
+  for (int a = 0; a < P; a++) {
+    T s0 = T(0), s1 = T(0), s2 = T(0);
+#pragma unroll
+    for (int k = 1; k < P; k++) {
+      s0 += tk[a * P + k] * t[k];
+      s1 += tk[a * P + k] * d1[k];
+      s2 += tk[a * P + k] * d2[k];
+    }
+    s[a][col] = c0 + c1 * s0;
+    d[a][col] = c1 * s1;
+    dd[a][col] = c1 * s2;
+  }
+}
+
+constexpr int kL2PHThreads = 64;
+
+template <int P>
+__global__ void __launch_bounds__(kL2PHThreads)
+l2p_hess_col_kernel(const int32_t* __restrict__ leaf_start, const V4<double>* __restrict__ pts,
+                    const double* __restrict__ l, int kpad, double inv_h2, Aniso3 A, int kcol,
+                    const int32_t* __restrict__ perm, double* __restrict__ out) {
+  constexpr int P2 = P * P, P3 = P * P * P;
+  constexpr int NC = kL2PHThreads;
+  __shared__ double tk[P2];
+  __shared__ double ls[P3];
+  __shared__ double x0[P][NC], x1[P][NC], x2[P][NC];
+  __shared__ double y0[P][NC], y1[P][NC], y2[P][NC];
+  __shared__ double z0s[P][NC], z1s[P][NC], z2s[P][NC];
+  const int leaf = blockIdx.x;
+  init_tk<double, P>(tk);
+  for (int idx = threadIdx.x; idx < P3; idx += NC) ls[idx] = l[static_cast<int64_t>(leaf) * kpad + idx];
+  __syncthreads();
+  const int ps = leaf_start[leaf], pe = leaf_start[leaf + 1];
+  const int c = threadIdx.x;
+  const double ak0 = A.a[kcol], ak1 = A.a[3 + kcol], ak2 = A.a[6 + kcol];
+  for (int i = ps + c; i < pe; i += NC) {
+    const V4<double> q = pts[i];
+    lagrange_dd_lds<double, P, NC>(q.x, tk, x0, x1, x2, c);
+    lagrange_dd_lds<double, P, NC>(q.y, tk, y0, y1, y2, c);
+    lagrange_dd_lds<double, P, NC>(q.z, tk, z0s, z1s, z2s, c);
+    double hxx = 0, hyy = 0, hzz = 0, hxy = 0, hxz = 0, hyz = 0;
+#pragma unroll 1
+    for (int a = 0; a < P; a++) {
+      const double* la = ls + a * P2;
+      double b00 = 0, b20 = 0, b02 = 0, b10 = 0, b01 = 0, b11 = 0;
+#pragma unroll 1
+      for (int bb = 0; bb < P; bb++) {
+        double v0 = 0, v1 = 0, v2 = 0;
+#pragma unroll
+        for (int g = 0; g < P; g++) {
+          const double lv = la[bb * P + g];
+          v0 += lv * z0s[g][c];
+          v1 += lv * z1s[g][c];
+          v2 += lv * z2s[g][c];
+        }
+        const double Y0 = y0[bb][c], Y1 = y1[bb][c], Y2 = y2[bb][c];
+        b00 += Y0 * v0;
+        b20 += Y2 * v0;
+        b10 += Y1 * v0;
+        b02 += Y0 * v2;
+        b01 += Y0 * v1;
+        b11 += Y1 * v1;
+      }
+      const double X0 = x0[a][c], X1 = x1[a][c], X2 = x2[a][c];
+      hxx += X2 * b00;
+      hyy += X0 * b20;
+      hzz += X0 * b02;
+      hxy += X1 * b10;
+      hxz += X1 * b01;
+      hyz += X0 * b11;
+    }
+    const double v[3] = {(hxx * ak0 + hxy * ak1 + hxz * ak2) * inv_h2, (hxy * ak0 + hyy * ak1 + hyz * ak2) * inv_h2,
+                         (hxz * ak0 + hyz * ak1 + hzz * ak2) * inv_h2};
+    double* o = out + 3 * static_cast<int64_t>(perm[i]);
+#pragma unroll
+    for (int k = 0; k < 3; k++) o[k] += (A.a[k] * v[0] + A.a[3 + k] * v[1]) + A.a[6 + k] * v[2];
+  }
+}
+
+template <class T, int KIND>
+__global__ void __launch_bounds__(kThreads)
+p2p_hess_col_kernel(const int32_t* __restrict__ tls, const int32_t* __restrict__ sls,
+                    const int32_t* __restrict__ p2p, const V4<T>* __restrict__ tpts,
+                    const V4<T>* __restrict__ spts, const int32_t* __restrict__ tperm, T hc, T pb, Aniso3 A,
+                    int kcol, double* __restrict__ out) {
+  constexpr int CAP = 32768 / static_cast<int>(sizeof(V4<T>));
+  constexpr int R = 2;
+  __shared__ V4<T> sp[CAP];
+  __shared__ int nb_start[28];
+  __shared__ int nb_src[27];
+  __shared__ T nb_off[27][3];
+  __shared__ int nb_count;
+  __shared__ T red[R][3][kThreads];
+  const int leaf = blockIdx.x;
+  const int ts = tls[leaf], te = tls[leaf + 1];
+  const int nt = te - ts;
+  if (threadIdx.x == 0) {
+    int k = 0, total = 0;
+    for (int s = 0; s < 27; s++) {
+      int sl = p2p[static_cast<int64_t>(leaf) * 27 + s];
+      if (sl < 0) continue;
+      int a = sls[sl], b = sls[sl + 1];
+      nb_src[k] = a;
+      nb_start[k] = total;
+      nb_off[k][0] = T(2 * (s / 9 - 1));
+      nb_off[k][1] = T(2 * ((s / 3) % 3 - 1));
+      nb_off[k][2] = T(2 * (s % 3 - 1));
+      total += b - a;
+      k++;
+    }
+    nb_start[k] = total;
+    nb_count = k;
+  }
+  __syncthreads();
+  const int nnb = nb_count;
+  const int total = nb_start[nnb];
+  const int per = (nt + R - 1) / R;
+  int tp = 1;
+  while (tp < per && tp < kThreads) tp <<= 1;
+  const int nsplit = kThreads / tp;
+  const int ti = threadIdx.x & (tp - 1);
+  const int si = threadIdx.x / tp;
+  const T seed = phi_seed<T, KIND>(pb);
+  const T ak0 = static_cast<T>(A.a[kcol]), ak1 = static_cast<T>(A.a[3 + kcol]), ak2 = static_cast<T>(A.a[6 + kcol]);
+  for (int c0 = 0; c0 < nt; c0 += R * tp) {
+    V4<T> tq[R];
+    T ax[R], ay[R], az[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int i = c0 + r * tp + ti;
+      tq[r] = i < nt ? tpts[ts + i] : V4<T>{T(0), T(0), T(0), T(0)};
+      ax[r] = ay[r] = az[r] = T(0);
+    }
+    for (int cb = 0; cb < total; cb += CAP) {
+      const int cnt = min(CAP, total - cb);
+      __syncthreads();
+      for (int e = threadIdx.x; e < cnt; e += kThreads) {
+        int g = cb + e;
+        int k = 0;
+        while (k + 1 < nnb && nb_start[k + 1] <= g) k++;
+        V4<T> q = spts[nb_src[k] + (g - nb_start[k])];
+        q.x += nb_off[k][0];
+        q.y += nb_off[k][1];
+        q.z += nb_off[k][2];
+        q.w *= hc;
+        sp[e] = q;
+      }
+      __syncthreads();
+      for (int j = si; j < cnt; j += nsplit) {
+        const V4<T> q = sp[j];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const T dx = tq[r].x - q.x, dy = tq[r].y - q.y, dz = tq[r].z - q.z;
+          const T r2 = fma(dz, dz, fma(dy, dy, fma(dx, dx, seed)));
+          const T f = q.w * grad_shape<T, KIND>(r2, pb);
+          const T tu = hess_t<T, KIND>(r2, pb) * fma(dz, ak2, fma(dy, ak1, dx * ak0));
+          ax[r] = fma(f, fma(tu, dx, ak0), ax[r]);
+          ay[r] = fma(f, fma(tu, dy, ak1), ay[r]);
+          az[r] = fma(f, fma(tu, dz, ak2), az[r]);
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      red[r][0][threadIdx.x] = ax[r];
+      red[r][1][threadIdx.x] = ay[r];
+      red[r][2][threadIdx.x] = az[r];
+    }
+    __syncthreads();
+    if (si == 0) {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int i = c0 + r * tp + ti;
+        if (i >= nt) continue;
+        double g[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          T sum = T(0);
+          for (int m = 0; m < nsplit; m++) sum += red[r][k][ti + m * tp];
+          g[k] = static_cast<double>(sum);
+        }
+        double* o = out + 3 * static_cast<int64_t>(tperm[ts + i]);
+#pragma unroll
+        for (int k = 0; k < 3; k++) o[k] = (A.a[k] * g[0] + A.a[3 + k] * g[1]) + A.a[6 + k] * g[2];
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ---- M2L on MFMA ----
 template <class T>
 struct Mfma;
@@ -1668,6 +1883,39 @@ void launch_l2p_grad(int p, const Tree& t, const double* l_leaf, const V4<double
   PAM_LAUNCH_CHECK();
 }
 
+void launch_l2p_hess_col(int p, const Tree& t, const double* l_leaf, const V4<double>* pts, const double* aniso,
+                         int kcol, double* out3, hipStream_t s) {
+  int64_t nl = t.n_leaves();
+  if (nl == 0) return;
+  if (kcol < 0 || kcol > 2) throw_invalid("Hessian column out of range");
+  const double h = t.box.width / std::ldexp(1.0, t.level) / 2.0;
+  Aniso3 A;
+  for (int k = 0; k < 9; k++) A.a[k] = aniso[k];
+  dispatch_order(p, [&](auto P) {
+    l2p_hess_col_kernel<decltype(P)::value><<<static_cast<unsigned>(nl), kL2PHThreads, 0, s>>>(
+        t.leaf_start.get(), pts, l_leaf, kpad_of(P), 1.0 / (h * h), A, kcol, t.perm.get(), out3);
+  });
+  PAM_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_p2p_hess_col(const RbfDesc& rbf, const Tree& trg, const Tree& src, const Lists& lists,
+                         const V4<T>* tpts, const V4<T>* spts, int kcol, double* out3, hipStream_t s) {
+  int64_t nl = trg.n_leaves();
+  if (nl == 0) return;
+  if (kcol < 0 || kcol > 2) throw_invalid("Hessian column out of range");
+  const double h = trg.box.width / std::ldexp(1.0, trg.level) / 2.0;
+  const PhiConsts gc = grad_consts(rbf, h);
+  Aniso3 A;
+  for (int k = 0; k < 9; k++) A.a[k] = rbf.aniso[k];
+  dispatch_kind(rbf.kind, [&](auto K) {
+    p2p_hess_col_kernel<T, decltype(K)::value><<<static_cast<unsigned>(nl), kThreads, 0, s>>>(
+        trg.leaf_start.get(), src.leaf_start.get(), lists.p2p.get(), tpts, spts, trg.perm.get(),
+        static_cast<T>(gc.a / h), static_cast<T>(gc.b), A, kcol, out3);
+  });
+  PAM_LAUNCH_CHECK();
+}
+
 template <class T>
 void launch_p2p_grad(const RbfDesc& rbf, const Tree& trg, const Tree& src, const Lists& lists,
                      const V4<T>* tpts, const V4<T>* spts, double* out3, hipStream_t s) {
@@ -1747,6 +1995,8 @@ void launch_m2l_batch(const M2LOperator& op, const std::vector<M2LLevelSpec<TM, 
                               const V4<T>*, const double*, double*, hipStream_t);                       \
   template void launch_p2p_grad<T>(const RbfDesc&, const Tree&, const Tree&, const Lists&, const V4<T>*, \
                                    const V4<T>*, double*, hipStream_t);                                  \
+  template void launch_p2p_hess_col<T>(const RbfDesc&, const Tree&, const Tree&, const Lists&, const V4<T>*, \
+                                       const V4<T>*, int, double*, hipStream_t);                           \
   template void build_m2l_dense<T>(int, const RbfDesc&, double, M2LOperator&, DevBuf<T>&,          \
                                    hipStream_t);                                                    \
   template void launch_m2l_batch<T, double>(const M2LOperator&,                                        \

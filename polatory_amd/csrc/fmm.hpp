@@ -70,6 +70,14 @@ void launch_p2p_grad(const RbfDesc& rbf, const Tree& trg, const Tree& src, const
                      const V4<T>* tpts, const V4<T>* spts, double* out3, hipStream_t s);
 void launch_l2p_grad(int p, const Tree& t, const double* l_leaf, const V4<double>* pts, const double* aniso,
                      double* out3, hipStream_t s);
+// Column kcol of the Hessian kernel at the targets (fmm/hessian_kernel.hpp:46-64
+// up to its sign): out3 = A^T sum_j w_j H(A (t - s_j)) a_k, a_k = A e_k; the
+// near field writes, the far field adds.
+template <class T>
+void launch_p2p_hess_col(const RbfDesc& rbf, const Tree& trg, const Tree& src, const Lists& lists,
+                         const V4<T>* tpts, const V4<T>* spts, int kcol, double* out3, hipStream_t s);
+void launch_l2p_hess_col(int p, const Tree& t, const double* l_leaf, const V4<double>* pts, const double* aniso,
+                         int kcol, double* out3, hipStream_t s);
 
 // Builds the dense M2L operator of one level: entries phi(h * |x_a + 2v - x_b|)
 // for cells of half width h (kernel constants folded by phi_consts).

@@ -26,4 +26,22 @@ class DipoleSourceEvaluator {
   int64_t n_src_ = -1;
 };
 
+// The Hessian kernel (fmm/hessian_kernel.hpp:46-64, make_fmm_hessian_evaluator,
+// make_fmm_evaluator.cpp:160-188): sources carry 3-vector weights and the
+// targets receive 3-vectors,
+//   out_i = -sum_j A^T H(A (t_i - s_j)) A w_j = -sum_k column k of the Hessian of F_k,
+// as three Hessian-column passes over one source tree.
+class DipoleHessianEvaluator {
+ public:
+  explicit DipoleHessianEvaluator(FmmEvaluator& ev);
+  void set_weights(const double* w3, int64_t n, bool on_device);
+  // out: 3 n_trg entries, x y z per target (input order).
+  void evaluate(double* out, int64_t n, bool on_device);
+
+ private:
+  FmmEvaluator& ev_;
+  DevBuf<double> w3_, wk_, g3_, tmp_;
+  int64_t n_src_ = -1;
+};
+
 }  // namespace pam

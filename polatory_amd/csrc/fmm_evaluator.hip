@@ -663,7 +663,8 @@ void FmmEvaluator::run_fmm(double* d_out) {
     PAM_HIP(hipEventRecord(fork_ev_, stream_));
     PAM_HIP(hipStreamWaitEvent(ps, fork_ev_, 0));
     record(7, ps);
-    if (grad_) launch_p2p_grad<TP>(rbf_, tt, stree, lists_, tp2p, sp2p, d_out, ps);
+    if (hess_col_ >= 0) launch_p2p_hess_col<TP>(rbf_, tt, stree, lists_, tp2p, sp2p, hess_col_, d_out, ps);
+    else if (grad_) launch_p2p_grad<TP>(rbf_, tt, stree, lists_, tp2p, sp2p, d_out, ps);
     else launch_p2p<TP>(rbf_, tt, stree, lists_, tp2p, sp2p, nullptr, d_out, ps);
     record(8, ps);
     PAM_HIP(hipEventRecord(join_ev_, ps));
@@ -751,7 +752,8 @@ void FmmEvaluator::run_fmm(double* d_out) {
   // Join the near field, then add the far field to it.
   PAM_HIP(hipStreamWaitEvent(stream_, join_ev_, 0));
   record(5);
-  if (grad_) launch_l2p_grad(P, tt, st.l[L].get(), tpts64, rbf_.aniso, d_out, stream_);
+  if (hess_col_ >= 0) launch_l2p_hess_col(P, tt, st.l[L].get(), tpts64, rbf_.aniso, hess_col_, d_out, stream_);
+  else if (grad_) launch_l2p_grad(P, tt, st.l[L].get(), tpts64, rbf_.aniso, d_out, stream_);
   else launch_l2p<double>(P, tt, st.l[L].get(), tpts64, d_out, stream_, true);
   record(6);
 }
@@ -875,13 +877,28 @@ void FmmEvaluator::evaluate_gradient(double* out, int64_t n, bool on_device) {
   grad_ = false;
 }
 
+void FmmEvaluator::evaluate_hessian_column(int k, double* out, int64_t n, bool on_device) {
+  if (symmetric_) throw_runtime("Hessian columns are evaluated at target points: use a source-target evaluator");
+  if (k < 0 || k > 2) throw_invalid("Hessian column out of range");
+  if (!config_valid_) choose_config();
+  grad_ = false;
+  hess_col_ = k;
+  try {
+    evaluate_impl(out, n, on_device);
+  } catch (...) {
+    hess_col_ = -1;
+    throw;
+  }
+  hess_col_ = -1;
+}
+
 void FmmEvaluator::evaluate_impl(double* out, int64_t n, bool on_device) {
   int64_t nt = n_trg();
   if (n != nt) throw_invalid("output size must equal the number of target points");
   const FmmEvaluator& src = S();
   if (!src.have_weights_ && src.n_src_ > 0) throw_runtime("weights are not set");
   if (!config_valid_) choose_config();
-  const int per = grad_ ? 3 : 1;
+  const int per = (grad_ || hess_col_ >= 0) ? 3 : 1;
   double* d_out = out;
   DevBuf<double> tmp;
   if (!on_device) {
@@ -898,7 +915,10 @@ void FmmEvaluator::evaluate_impl(double* out, int64_t n, bool on_device) {
       stats_.direct = 1;
     } else if (use_direct()) {
       for (int i = 1; i <= 5; i++) record(i);
-      if (grad_) {
+      if (hess_col_ >= 0) {
+        direct_hessian_column_f64(rbf_, src.src_pts_.get(), src.w_.get(), src.n_src_, d_target_points(), nt,
+                                  hess_col_, d_out, direct_scratch_, stream_);
+      } else if (grad_) {
         direct_gradient_f64(rbf_, src.src_pts_.get(), src.w_.get(), src.n_src_, d_target_points(), nt, d_out,
                             direct_scratch_, stream_);
       } else {

@@ -87,6 +87,10 @@ class FmmEvaluator {
   // target in input order. Same configuration, trees and expansions as the
   // values; source-target evaluators only.
   void evaluate_gradient(double* out, int64_t n, bool on_device);
+  // Column k of the Hessian kernel with the scalar weights (the building
+  // block of make_fmm_hessian_evaluator): out has 3 n_trg entries,
+  // A^T sum_j w_j H(A (t_i - s_j)) A e_k per target. Source-target only.
+  void evaluate_hessian_column(int k, double* out, int64_t n, bool on_device);
   // Makes this a gradient-transpose evaluator (make_fmm_evaluator.cpp:119-148):
   // evaluate() returns the gradients and the accuracy search measures them.
   void set_gradient_kernel(bool on);
@@ -140,6 +144,7 @@ class FmmEvaluator {
   bool symmetric_;
   bool grad_ = false;  // the evaluation in progress computes gradients
   bool grad_kernel_ = false;
+  int hess_col_ = -1;  // >= 0: the evaluation in progress computes this Hessian column
   double user_min_[3], user_max_[3];
   hipStream_t stream_ = nullptr;
   bool own_stream_ = true;

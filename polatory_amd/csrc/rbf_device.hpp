@@ -130,6 +130,23 @@ __device__ __forceinline__ T grad_shape(T r2c, T b) {
   }
 }
 
+// Hessian in physical units from local differences u (unit h):
+//   H a = (ag / h) grad_shape(r2c) (a + hess_t(r2c) u (u . a))
+// (rbf/polyharmonic_odd.hpp:56-72, rbf/cov_exponential.hpp:52-62):
+//   bh3 / th3: hess_t = (K - 2) / r2c   (0 where r2c = 0, where the reference returns 0)
+//   exp:       hess_t = -(1 - b r) / r^2, r = sqrt(r2c)
+template <class T, int KIND>
+__device__ __forceinline__ T hess_t(T r2c, T b) {
+  if constexpr (KIND == kBh3) {
+    return r2c > T(0) ? T(-1) / r2c : T(0);
+  } else if constexpr (KIND == kTh3) {
+    return r2c > T(0) ? T(1) / r2c : T(0);
+  } else {
+    const T r = sqrt(r2c);
+    return -(T(1) - b * r) / r2c;
+  }
+}
+
 template <class T, int KIND>
 __device__ __forceinline__ T phi_t(T r2, T a, T b) {
   if constexpr (sizeof(T) == 8) {

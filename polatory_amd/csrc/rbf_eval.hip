@@ -81,6 +81,44 @@ direct_grad_kernel(double p0, double p1, const double* __restrict__ src, const d
     for (int c = 0; c < 3; c++) partial[(static_cast<int64_t>(blockIdx.y) * n_trg + i) * 3 + c] = acc[c];
 }
 
+// Column kcol of the Hessian kernel by the direct sum:
+// partial = sum_j w_j H(t_i - s_j) a_k (transformed points; A^T applied in the reduction).
+template <int KIND>
+__global__ void __launch_bounds__(kGradThreads)
+direct_hess_col_kernel(double p0, double p1, Aniso A, int kcol, const double* __restrict__ src,
+                       const double* __restrict__ w, int64_t n_src, const double* __restrict__ trg, int64_t n_trg,
+                       double* __restrict__ partial, int64_t chunk) {
+  __shared__ double4 tile[kGradThreads];
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kGradThreads + threadIdx.x;
+  const int64_t j0 = static_cast<int64_t>(blockIdx.y) * chunk;
+  const int64_t j1 = j0 + chunk < n_src ? j0 + chunk : n_src;
+  const double ak[3] = {A.a[kcol], A.a[3 + kcol], A.a[6 + kcol]};
+  double t[3] = {0.0, 0.0, 0.0};
+  if (i < n_trg) {
+    t[0] = trg[3 * i];
+    t[1] = trg[3 * i + 1];
+    t[2] = trg[3 * i + 2];
+  }
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int64_t jt = j0; jt < j1; jt += kGradThreads) {
+    __syncthreads();
+    const int64_t j = jt + threadIdx.x;
+    if (j < j1) tile[threadIdx.x] = make_double4(src[3 * j], src[3 * j + 1], src[3 * j + 2], w[j]);
+    __syncthreads();
+    const int cnt = static_cast<int>((j1 - jt) < kGradThreads ? (j1 - jt) : kGradThreads);
+    for (int k = 0; k < cnt; k++) {
+      const double4 q = tile[k];
+      const double u[3] = {t[0] - q.x, t[1] - q.y, t[2] - q.z};
+      double tt;
+      const double f = rbf_grad_factor<KIND>(p0, p1, u, &tt);
+      const double ua = u[0] * ak[0] + u[1] * ak[1] + u[2] * ak[2];
+      for (int c = 0; c < 3; c++) acc[c] += q.w * (f * (ak[c] + tt * u[c] * ua));
+    }
+  }
+  if (i < n_trg)
+    for (int c = 0; c < 3; c++) partial[(static_cast<int64_t>(blockIdx.y) * n_trg + i) * 3 + c] = acc[c];
+}
+
 // out[i] = A^T (sum of the chunks in order)
 __global__ void grad_reduce_kernel(const double* __restrict__ partial, int nchunks, int64_t n_trg, Aniso A,
                                    double* __restrict__ out) {
@@ -108,6 +146,28 @@ void direct_gradient_f64(const RbfDesc& rbf, const double* src, const double* w,
     dispatch_kind(rbf.kind, [&](auto K) {
       direct_grad_kernel<decltype(K)::value><<<grid, kGradThreads, 0, s>>>(rbf.p0, rbf.p1, src, w, n_src, trg,
                                                                            n_trg, scratch.get(), chunk);
+    });
+    PAM_LAUNCH_CHECK();
+  }
+  grad_reduce_kernel<<<ceil_div(n_trg, 256), 256, 0, s>>>(scratch.get(), nchunks, n_trg, A, out3);
+  PAM_LAUNCH_CHECK();
+}
+
+void direct_hessian_column_f64(const RbfDesc& rbf, const double* src, const double* w, int64_t n_src,
+                               const double* trg, int64_t n_trg, int kcol, double* out3, DevBuf<double>& scratch,
+                               hipStream_t s) {
+  if (n_trg == 0) return;
+  if (kcol < 0 || kcol > 2) throw_invalid("Hessian column out of range");
+  Aniso A;
+  for (int k = 0; k < 9; k++) A.a[k] = rbf.aniso[k];
+  const int64_t chunk = 8192;
+  const int nchunks = n_src > 0 ? ceil_div(n_src, chunk) : 0;
+  scratch.resize(static_cast<size_t>(std::max(nchunks, 1)) * n_trg * 3);
+  if (n_src > 0) {
+    dim3 grid(ceil_div(n_trg, kGradThreads), nchunks);
+    dispatch_kind(rbf.kind, [&](auto K) {
+      direct_hess_col_kernel<decltype(K)::value><<<grid, kGradThreads, 0, s>>>(rbf.p0, rbf.p1, A, kcol, src, w,
+                                                                               n_src, trg, n_trg, scratch.get(), chunk);
     });
     PAM_LAUNCH_CHECK();
   }

@@ -265,3 +265,23 @@ class FmmGradientEvaluator(FmmEvaluator):
 
 def make_fmm_gradient_evaluator(rbf, bbox=None):
     return FmmGradientEvaluator(rbf, bbox)
+
+
+class FmmHessianEvaluator(FmmEvaluator):
+    """make_fmm_hessian_evaluator (make_fmm_evaluator.cpp:160-188): sources carry
+    gradient weights, (n_src, 3); evaluate() returns (n_trg, 3),
+    -sum_j A^T H(A (t_i - s_j)) A w_j."""
+
+    _create = "pam_fmm_hessian_evaluator_create"
+
+    def evaluate(self):
+        out = np.empty((self.n_trg, 3), dtype=np.float64)
+        _capi.call("pam_fmm_evaluator_evaluate", self._h, _ptr(out), self.n_trg)
+        return out
+
+    def evaluate_gradient(self):
+        raise RuntimeError("a Hessian-kernel evaluator evaluates through evaluate()")
+
+
+def make_fmm_hessian_evaluator(rbf, bbox=None):
+    return FmmHessianEvaluator(rbf, bbox)
