@@ -301,6 +301,36 @@ class Interpolant {
     return out;
   }
 
+  // interpolant.hpp:58-67: values at points, then gradients at grad_points (x y z each)
+  VecX evaluate(const Points& points, const Points& grad_points, double accuracy = kInfinity,
+                double grad_accuracy = kInfinity) {
+    VecX out(points.rows() + 3 * grad_points.rows());
+    detail::check(pam_interpolant_evaluate(h_, points.data(), points.rows(), accuracy, out.data()));
+    detail::check(pam_interpolant_evaluate_gradient(h_, grad_points.data(), grad_points.rows(), grad_accuracy,
+                                                    out.data() + points.rows()));
+    return out;
+  }
+
+  // interpolant.hpp:115-147 (value data)
+  void fit_incrementally(const Points& points, const VecX& values, double tolerance, int max_iter = 100,
+                         double accuracy = kInfinity) {
+    if (values.rows() != points.rows())
+      throw std::invalid_argument("values.rows() must be equal to " + std::to_string(points.rows()));
+    detail::check(pam_interpolant_fit_incrementally(h_, points.data(), points.rows(), values.data(), tolerance,
+                                                    max_iter, accuracy, nullptr));
+  }
+
+  // interpolant.hpp:149-190 (NaN: no value / bound at that point)
+  void fit_inequality(const Points& points, const VecX& values, const VecX& values_lb, const VecX& values_ub,
+                      double tolerance, int max_iter = 100, double accuracy = kInfinity,
+                      const Interpolant* initial = nullptr) {
+    if (values.rows() != points.rows() || values_lb.rows() != points.rows() || values_ub.rows() != points.rows())
+      throw std::invalid_argument("values, values_lb and values_ub must have points.rows() rows");
+    detail::check(pam_interpolant_fit_inequality(h_, points.data(), points.rows(), values.data(), values_lb.data(),
+                                                 values_ub.data(), tolerance, max_iter, accuracy, nullptr,
+                                                 initial ? initial->h_ : nullptr));
+  }
+
   Points centers() const {
     Index n = 0;
     detail::check(pam_interpolant_num_centers(h_, &n));
@@ -346,6 +376,22 @@ class Interpolant {
   pam_interpolant* h_ = nullptr;
 };
 
+namespace kriging {
+
+// kriging/cross_validate.hpp:14-47 (set_ids as a std::vector instead of Eigen::VectorXi)
+template <int Dim>
+VecX cross_validate(const Model<Dim>& model, const geometry::Points<Dim>& points, const VecX& values,
+                    const std::vector<int32_t>& set_ids, double tolerance, int max_iter, double accuracy) {
+  if (values.rows() != points.rows() || static_cast<Index>(set_ids.size()) != points.rows())
+    throw std::invalid_argument("values and set_ids must have points.rows() rows");
+  VecX out(points.rows());
+  detail::check(pam_cross_validate(model.handle(), points.data(), points.rows(), values.data(), set_ids.data(),
+                                   tolerance, max_iter, accuracy, nullptr, out.data()));
+  return out;
+}
+
+}  // namespace kriging
+
 namespace fmm {
 
 template <int Dim>
@@ -376,16 +422,22 @@ using FmmGenericSymmetricEvaluatorPtr = std::unique_ptr<FmmGenericSymmetricEvalu
 
 namespace detail {
 
+using CreateFn = int (*)(const pam_rbf*, const double*, const double*, pam_fmm_evaluator**);
+
+// One handle per kernel kind: kn outputs per target (1 for values, 3 for
+// gradients), km weights per source.
 template <int Dim>
 class DeviceEvaluator final : public FmmGenericEvaluatorBase<Dim> {
  public:
-  DeviceEvaluator(const rbf::Rbf<Dim>& r, const geometry::Bbox<Dim>& b) {
+  DeviceEvaluator(const rbf::Rbf<Dim>& r, const geometry::Bbox<Dim>& b, CreateFn create = pam_fmm_evaluator_create,
+                  int kn = 1)
+      : kn_(kn) {
     pam_rbf c = r.to_c();
-    polatory::detail::check(pam_fmm_evaluator_create(&c, b.min().data(), b.max().data(), &h_));
+    polatory::detail::check(create(&c, b.min().data(), b.max().data(), &h_));
   }
   ~DeviceEvaluator() override { pam_fmm_evaluator_destroy(h_); }
   VecX evaluate() const override {
-    VecX out(n_trg_);
+    VecX out(kn_ * n_trg_);
     polatory::detail::check(pam_fmm_evaluator_evaluate(h_, out.data(), n_trg_));
     return out;
   }
@@ -404,6 +456,7 @@ class DeviceEvaluator final : public FmmGenericEvaluatorBase<Dim> {
  private:
   pam_fmm_evaluator* h_ = nullptr;
   Index n_trg_ = 0;
+  int kn_ = 1;
 };
 
 template <int Dim>
@@ -446,6 +499,25 @@ template <int Dim>
 FmmGenericSymmetricEvaluatorPtr<Dim> make_fmm_symmetric_evaluator(const rbf::Rbf<Dim>& r,
                                                                   const geometry::Bbox<Dim>& bbox) {
   return std::make_unique<detail::DeviceSymmetricEvaluator<Dim>>(r, bbox);
+}
+
+// fmm_evaluator.hpp:94-106 (make_fmm_evaluator.cpp:78-188): gradient-weighted
+// sources to values, scalar weights to gradients, gradient-weighted sources to
+// gradients.
+template <int Dim>
+FmmGenericEvaluatorPtr<Dim> make_fmm_gradient_evaluator(const rbf::Rbf<Dim>& r, const geometry::Bbox<Dim>& bbox) {
+  return std::make_unique<detail::DeviceEvaluator<Dim>>(r, bbox, pam_fmm_gradient_evaluator_create, 1);
+}
+
+template <int Dim>
+FmmGenericEvaluatorPtr<Dim> make_fmm_gradient_transpose_evaluator(const rbf::Rbf<Dim>& r,
+                                                                  const geometry::Bbox<Dim>& bbox) {
+  return std::make_unique<detail::DeviceEvaluator<Dim>>(r, bbox, pam_fmm_gradient_transpose_evaluator_create, 3);
+}
+
+template <int Dim>
+FmmGenericEvaluatorPtr<Dim> make_fmm_hessian_evaluator(const rbf::Rbf<Dim>& r, const geometry::Bbox<Dim>& bbox) {
+  return std::make_unique<detail::DeviceEvaluator<Dim>>(r, bbox, pam_fmm_hessian_evaluator_create, 3);
 }
 
 }  // namespace fmm

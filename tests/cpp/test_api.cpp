@@ -99,6 +99,35 @@ int main() {
     std::printf("save/load: identical evaluation\n");
   }
 
+  // values then gradients (interpolant.hpp:58-67); the gradient of the fit
+  // against central differences of its own values
+  {
+    Points3 q(3, 3), gq(2, 3);
+    for (Index i = 0; i < 3; i++)
+      for (int k = 0; k < 3; k++) q(i, k) = 0.3 + 0.1 * i + 0.05 * k;
+    for (Index i = 0; i < 2; i++)
+      for (int k = 0; k < 3; k++) gq(i, k) = 0.4 + 0.2 * i - 0.03 * k;
+    // few points: the evaluators take the FP64 direct sums
+    VecX vg = interpolant.evaluate(q, gq);
+    if (vg.rows() != 3 + 6) return 8;
+    VecX v = interpolant.evaluate(q);
+    for (Index i = 0; i < 3; i++)
+      if (vg(i) != v(i)) return 9;
+    const double h = 1e-5;
+    double gerr = 0.0;
+    for (Index i = 0; i < 2; i++)
+      for (int k = 0; k < 3; k++) {
+        Points3 pp(2, 3);
+        for (int c = 0; c < 3; c++) pp(0, c) = pp(1, c) = gq(i, c);
+        pp(0, k) += h;
+        pp(1, k) -= h;
+        VecX f = interpolant.evaluate(pp);
+        gerr = std::max(gerr, std::abs(vg(3 + 3 * i + k) - (f(0) - f(1)) / (2 * h)));
+      }
+    std::printf("gradients vs central differences: %.3e\n", gerr);
+    if (!(gerr < 1e-4)) return 10;
+  }
+
   // error contract: invalid arguments throw std::invalid_argument
   try {
     interpolant.fit(pts, values, 0.0);
