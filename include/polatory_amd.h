@@ -78,7 +78,9 @@ int pam_fmm_gradient_evaluator_create(const pam_rbf* rbf, const double bbox_min[
 /* make_fmm_hessian_evaluator(rbf, bbox) — make_fmm_evaluator.cpp:160-188,
  * fmm/hessian_kernel.hpp:46-64: sources carry gradient weights (3 n_sources entries)
  * and evaluate() returns 3 n_targets entries, out[3 i + c] =
- * -(sum_j A^T H(A (t_i - s_j)) A w_j)_c. Evaluated as three Hessian-column passes.
+ * -(sum_j A^T H(A (t_i - s_j)) A w_j)_c. Evaluated as three Hessian-column passes; the
+ * accuracy search measures gradient-transpose errors (Hessians at the same
+ * configuration are less accurate).
  * The symmetric form (make_fmm_hessian_symmetric_evaluator) has no counterpart. */
 int pam_fmm_hessian_evaluator_create(const pam_rbf* rbf, const double bbox_min[3], const double bbox_max[3],
                                      pam_fmm_evaluator** out);

@@ -141,17 +141,25 @@ template <class T, int P>
 __global__ void __launch_bounds__(kThreads)
 p2m_kernel(const int32_t* __restrict__ leaf_start, const V4<T>* __restrict__ pts, T* __restrict__ m,
            int kpad) {
-  constexpr int P3 = P * P * P;
-  constexpr int NO = (P3 + kThreads - 1) / kThreads;
+  constexpr int P2 = P * P, P3 = P * P * P;
+  // Thread (g, bc): the P coefficients (a, b, c) of one (b, c) pair, summed
+  // over the points j = g (mod NG) of each chunk; the NG partial sums are
+  // added in group order at the end. Per point a thread reads Y_b, Z_c and
+  // the P weighted X_a from LDS for P multiply-adds.
+  constexpr int NG = kThreads / P2 > 0 ? kThreads / P2 : 1;
   __shared__ T tk[P * P];
   __shared__ T sv[3][kP2mChunk][P];
+  __shared__ T red[NG > 1 ? NG - 1 : 1][P3];
   const int leaf = blockIdx.x;
   const int ps = leaf_start[leaf], pe = leaf_start[leaf + 1];
   init_tk<T, P>(tk);
   __syncthreads();
-  T acc[NO];
+  const int bc = threadIdx.x % P2, g = threadIdx.x / P2;
+  const bool active = g < NG;
+  const int b = bc / P, c = bc % P;
+  T acc[P];
 #pragma unroll
-  for (int q = 0; q < NO; q++) acc[q] = T(0);
+  for (int a = 0; a < P; a++) acc[a] = T(0);
   for (int c0 = ps; c0 < pe; c0 += kP2mChunk) {
     int cnt = min(kP2mChunk, pe - c0);
     if (threadIdx.x < 3 * kP2mChunk) {
@@ -173,23 +181,32 @@ p2m_kernel(const int32_t* __restrict__ leaf_start, const V4<T>* __restrict__ pts
       for (int a = 0; a < P; a++) sv[d][j][a] = S[a];
     }
     __syncthreads();
+    if (active) {
+      for (int j = g; j < cnt; j += NG) {
+        const T yz = sv[1][j][b] * sv[2][j][c];
 #pragma unroll
-    for (int q = 0; q < NO; q++) {
-      int o = threadIdx.x + q * kThreads;
-      if (o < P3) {
-        int a = o / (P * P), bb = (o / P) % P, g = o % P;
-        T sum = T(0);
-        for (int j = 0; j < cnt; j++) sum += sv[0][j][a] * sv[1][j][bb] * sv[2][j][g];
-        acc[q] += sum;
+        for (int a = 0; a < P; a++) acc[a] += sv[0][j][a] * yz;
       }
     }
     __syncthreads();
   }
-  T* dst = m + static_cast<int64_t>(leaf) * kpad;
+  if (NG > 1) {
+    if (active && g > 0) {
 #pragma unroll
-  for (int q = 0; q < NO; q++) {
-    int o = threadIdx.x + q * kThreads;
-    if (o < P3) dst[o] = acc[q];
+      for (int a = 0; a < P; a++) red[g - 1][a * P2 + bc] = acc[a];
+    }
+    __syncthreads();
+    if (g == 0) {
+      for (int k = 1; k < NG; k++) {
+#pragma unroll
+        for (int a = 0; a < P; a++) acc[a] += red[k - 1][a * P2 + bc];
+      }
+    }
+  }
+  T* dst = m + static_cast<int64_t>(leaf) * kpad;
+  if (g == 0) {
+#pragma unroll
+    for (int a = 0; a < P; a++) dst[a * P2 + bc] = acc[a];
   }
   for (int o = P3 + threadIdx.x; o < kpad; o += kThreads) dst[o] = T(0);
 }
