@@ -261,3 +261,28 @@ def test_accuracy_search_meets_accuracy(orc):
     log("  chosen", ev.config(), "err", np.abs(y - ref).max(), "acc", acc)
     # the search checks its sample (source points); targets may land slightly above
     assert np.abs(y - ref).max() < 3 * acc
+
+
+@pytest.mark.parametrize("name", ["bh3", "th3", "exp"])
+def test_fp64_near_field_alone_matches_direct(name):
+    """All points in one leaf of a fixed level-2 tree: the FMM result is the
+    FP64 near field alone, so it must equal the FP64 direct sum up to the
+    summation order (this pins the near field's square root, refined from the
+    hardware reciprocal square root, to a few ulp)."""
+    rbf = {"bh3": pa.Biharmonic3D(), "th3": pa.Triharmonic3D(), "exp": pa.CovExponential([1.0, 0.3])}[name]
+    rng = np.random.default_rng(61)
+    src = rng.uniform(0.01, 0.24, (6000, 3))
+    trg = rng.uniform(0.01, 0.24, (4000, 3))
+    trg[:50] = src[:50]  # coincident pairs (zero distance)
+    w = rng.normal(size=6000)
+    ev = pa.FmmEvaluator(rbf, ((0, 0, 0), (1, 1, 1)))
+    ev.set_config(pa.FmmConfig("fp64", 10, 2, "fmm"))
+    ev.set_source_points(src)
+    ev.set_weights(w)
+    ev.set_target_points(trg)
+    v = ev.evaluate()
+    assert ev.stats()["direct"] == 0
+    ref = pa.direct_evaluate(rbf, src, w, trg)
+    err = np.abs(v - ref).max() / np.abs(ref).max()
+    print(f"{name}: FP64 near field alone vs direct: {err:.2e}", flush=True)
+    assert err <= 1e-13
