@@ -175,7 +175,8 @@ def kriging_section(pa):
     """Anisotropic kriging: exponential covariance (sill 1, range 2), anisotropy
     A = diag(1, 1, 1/0.3) (ranges 2, 2, 0.6), nugget 0.0025, degree 0, fit to
     1e-4 absolute; prediction on the 128^3 cell centres of [0,10]^3 with the
-    mixed (FP32 near field + FP32-MFMA M2L) and the FP64 evaluator."""
+    mixed (FP32 near field + FP32-MFMA M2L), the mixed form with compensated
+    near-field sums, and the FP64 evaluator."""
     pts, vals = gmm_sites(500_000)
     rbf = pa.CovExponential([1.0, 2.0])
     rbf.set_anisotropy(np.diag([1.0, 1.0, 1.0 / 0.3]))
@@ -194,7 +195,9 @@ def kriging_section(pa):
     n = ip.num_centers()
     ref = pa.direct_evaluate(rbf, ip.centers(), w[:n], grid[sub]) + w[n]
     preds = {}
-    for name, cfg in (("mixed", pa.FmmConfig("mixed", 8, -1, "auto")), ("fp64", pa.FmmConfig("fp64", 10, -1, "auto"))):
+    for name, cfg in (("mixed", pa.FmmConfig("mixed", 8, -1, "auto")),
+                      ("mixed_comp", pa.FmmConfig("mixed_comp", 8, -1, "auto")),
+                      ("fp64", pa.FmmConfig("fp64", 10, -1, "auto"))):
         ip.set_eval_config(cfg)
         ip.evaluate(grid)  # builds the evaluator and trees
         t0 = time.perf_counter()

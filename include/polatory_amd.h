@@ -118,6 +118,8 @@ int pam_fmm_evaluator_synchronize(pam_fmm_evaluator* h);
  * polatory picks {tree height, order, d} internally, fmm_accuracy_estimator.hpp:74-121). */
 #define PAM_PRECISION_FP64 0  /* every stage in FP64 */
 #define PAM_PRECISION_MIXED 1 /* FP32 P2P and FP32-MFMA M2L; FP64 expansions, P2M/M2M/L2L/L2P and output */
+#define PAM_PRECISION_MIXED_COMPENSATED 2 /* MIXED with compensated (Kahan) near-field sums, for
+                                             strongly cancelling weights (value evaluations) */
 #define PAM_MODE_AUTO 0
 #define PAM_MODE_DIRECT 1
 #define PAM_MODE_FMM 2

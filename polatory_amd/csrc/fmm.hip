@@ -365,7 +365,9 @@ l2p_kernel(const int32_t* __restrict__ leaf_start, const V4<T>* __restrict__ pts
 // serves kP2PR pairs.
 constexpr int kP2PR = 4;
 
-template <class T, int KIND>
+// kComp: compensated (Kahan) sums per target and FP64 sums over the source
+// splits, for weights that cancel strongly (kriging).
+template <class T, int KIND, bool kComp = false>
 __global__ void __launch_bounds__(kThreads)
 p2p_kernel(const int32_t* __restrict__ tls, const int32_t* __restrict__ sls,
            const int32_t* __restrict__ p2p, const V4<T>* __restrict__ tpts,
@@ -413,12 +415,13 @@ p2p_kernel(const int32_t* __restrict__ tls, const int32_t* __restrict__ sls,
   const int si = threadIdx.x / tp;
   for (int c0 = 0; c0 < nt; c0 += R * tp) {
     V4<T> tq[R];
-    T acc[R];
+    T acc[R], cmp[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
       const int i = c0 + r * tp + ti;
       tq[r] = i < nt ? tpts[ts + i] : V4<T>{T(0), T(0), T(0), T(0)};
       acc[r] = T(0);
+      cmp[r] = T(0);
     }
     for (int cb = 0; cb < total; cb += CAP) {
       const int cnt = min(CAP, total - cb);
@@ -443,7 +446,14 @@ p2p_kernel(const int32_t* __restrict__ tls, const int32_t* __restrict__ sls,
       auto pair = [&](const V4<T>& q, int r) {
         T dx = tq[r].x - q.x, dy = tq[r].y - q.y, dz = tq[r].z - q.z;
         T r2 = fma(dz, dz, fma(dy, dy, fma(dx, dx, seed)));
-        acc[r] = fma(q.w, phi_shape<T, KIND>(r2, pb), acc[r]);
+        if constexpr (kComp) {
+          const T y = q.w * phi_shape<T, KIND>(r2, pb) - cmp[r];
+          const T t = acc[r] + y;
+          cmp[r] = (t - acc[r]) - y;
+          acc[r] = t;
+        } else {
+          acc[r] = fma(q.w, phi_shape<T, KIND>(r2, pb), acc[r]);
+        }
       };
       int j = si;
       for (; j + 3 * nsplit < cnt; j += 4 * nsplit) {
@@ -463,15 +473,16 @@ p2p_kernel(const int32_t* __restrict__ tls, const int32_t* __restrict__ sls,
       }
     }
 #pragma unroll
-    for (int r = 0; r < R; r++) red[r][threadIdx.x] = acc[r];
+    for (int r = 0; r < R; r++) red[r][threadIdx.x] = kComp ? acc[r] - cmp[r] : acc[r];
     __syncthreads();
     if (si == 0) {
 #pragma unroll
       for (int r = 0; r < R; r++) {
         const int i = c0 + r * tp + ti;
         if (i >= nt) continue;
-        T sum = T(0);
-        for (int k = 0; k < nsplit; k++) sum += red[r][ti + k * tp];
+        using S = std::conditional_t<kComp, double, T>;
+        S sum = S(0);
+        for (int k = 0; k < nsplit; k++) sum += static_cast<S>(red[r][ti + k * tp]);
         const int gi = ts + i;
         double far = y_far ? y_far[gi] : 0.0;
         out[tperm[gi]] = far + static_cast<double>(sum);
@@ -1873,12 +1884,21 @@ void launch_l2p(int p, const Tree& t, const T* l_leaf, const V4<T>* pts, T* y_fa
 
 template <class T>
 void launch_p2p(const RbfDesc& rbf, const Tree& trg, const Tree& src, const Lists& lists,
-                const V4<T>* tpts, const V4<T>* spts, const double* y_far, double* out, hipStream_t s) {
+                const V4<T>* tpts, const V4<T>* spts, const double* y_far, double* out, hipStream_t s,
+                bool compensated) {
   int64_t nl = trg.n_leaves();
   if (nl == 0) return;
   double h = trg.box.width / std::ldexp(1.0, trg.level) / 2.0;
   PhiConsts pc = phi_consts(rbf, h);
   dispatch_kind(rbf.kind, [&](auto K) {
+    if constexpr (sizeof(T) == 4) {
+      if (compensated) {
+        p2p_kernel<T, decltype(K)::value, true><<<static_cast<unsigned>(nl), kThreads, 0, s>>>(
+            trg.leaf_start.get(), src.leaf_start.get(), lists.p2p.get(), tpts, spts, y_far, trg.perm.get(),
+            static_cast<T>(pc.a), static_cast<T>(pc.b), out);
+        return;
+      }
+    }
     p2p_kernel<T, decltype(K)::value><<<static_cast<unsigned>(nl), kThreads, 0, s>>>(
         trg.leaf_start.get(), src.leaf_start.get(), lists.p2p.get(), tpts, spts, y_far, trg.perm.get(),
         static_cast<T>(pc.a), static_cast<T>(pc.b), out);
@@ -2009,7 +2029,7 @@ void launch_m2l_batch(const M2LOperator& op, const std::vector<M2LLevelSpec<TM, 
   template void launch_l2l<T>(int, const Tree&, int, const T*, T*, hipStream_t);                    \
   template void launch_l2p<T>(int, const Tree&, const T*, const V4<T>*, T*, hipStream_t, bool);           \
   template void launch_p2p<T>(const RbfDesc&, const Tree&, const Tree&, const Lists&, const V4<T>*, \
-                              const V4<T>*, const double*, double*, hipStream_t);                       \
+                              const V4<T>*, const double*, double*, hipStream_t, bool);                 \
   template void launch_p2p_grad<T>(const RbfDesc&, const Tree&, const Tree&, const Lists&, const V4<T>*, \
                                    const V4<T>*, double*, hipStream_t);                                  \
   template void launch_p2p_hess_col<T>(const RbfDesc&, const Tree&, const Tree&, const Lists&, const V4<T>*, \

@@ -61,7 +61,8 @@ void launch_l2p(int p, const Tree& t, const T* l_leaf, const V4<T>* pts, T* y_fa
 // out[perm[i]] = y_far[i] + near field, FP64 output in input order.
 template <class T>
 void launch_p2p(const RbfDesc& rbf, const Tree& trg, const Tree& src, const Lists& lists,
-                const V4<T>* tpts, const V4<T>* spts, const double* y_far, double* out, hipStream_t s);
+                const V4<T>* tpts, const V4<T>* spts, const double* y_far, double* out, hipStream_t s,
+                bool compensated = false);  // compensated: Kahan sums (FP32 only)
 // Gradients at the targets (fmm/gradient_transpose_kernel.hpp:47-62): the
 // near field writes A^T sum_j w_j grad phi into out3 (n x 3, input order);
 // the far field then adds A^T grad of the leaf local expansions.

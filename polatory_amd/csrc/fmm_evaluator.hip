@@ -204,7 +204,7 @@ void FmmEvaluator::set_accuracy(double accuracy) {
 }
 
 void FmmEvaluator::set_config(const FmmConfig& c) {
-  if (c.precision != kFp64 && c.precision != kMixed) throw_invalid("unknown precision");
+  if (c.precision != kFp64 && c.precision != kMixed && c.precision != kMixedComp) throw_invalid("unknown precision");
   if (!order_supported(c.order)) throw_invalid("unsupported interpolation order");
   if (c.leaf_level != -1 && (c.leaf_level < 2 || c.leaf_level > 18)) throw_invalid("leaf level out of range");
   FmmConfig old = cfg_;
@@ -665,7 +665,7 @@ void FmmEvaluator::run_fmm(double* d_out) {
     record(7, ps);
     if (hess_col_ >= 0) launch_p2p_hess_col<TP>(rbf_, tt, stree, lists_, tp2p, sp2p, hess_col_, d_out, ps);
     else if (grad_) launch_p2p_grad<TP>(rbf_, tt, stree, lists_, tp2p, sp2p, d_out, ps);
-    else launch_p2p<TP>(rbf_, tt, stree, lists_, tp2p, sp2p, nullptr, d_out, ps);
+    else launch_p2p<TP>(rbf_, tt, stree, lists_, tp2p, sp2p, nullptr, d_out, ps, cfg_.precision == kMixedComp);
     record(8, ps);
     PAM_HIP(hipEventRecord(join_ev_, ps));
   };
@@ -816,7 +816,7 @@ void FmmEvaluator::choose_config() {
       dc.mode = kModeDirect;
       probe.set_config(dc);
       probe.evaluate(exact.data(), m, false);
-      const int cand[][2] = {{kMixed, 4}, {kMixed, 5}, {kMixed, 6}, {kMixed, 7}, {kMixed, 8},
+      const int cand[][2] = {{kMixed, 4}, {kMixed, 5}, {kMixed, 6}, {kMixed, 7}, {kMixed, 8}, {kMixedComp, 8},
                              {kFp64, 8}, {kFp64, 10}, {kFp64, 12}};
       bool found = false;
       for (auto& cd : cand) {

@@ -16,7 +16,9 @@
 
 namespace pam {
 
-enum Precision : int { kFp64 = 0, kMixed = 1 };
+// kMixedComp: kMixed with compensated (Kahan) sums in the near field, for
+// strongly cancelling weights.
+enum Precision : int { kFp64 = 0, kMixed = 1, kMixedComp = 2 };
 enum EvalMode : int { kModeAuto = 0, kModeDirect = 1, kModeFmm = 2 };
 
 struct FmmConfig {

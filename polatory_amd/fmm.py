@@ -41,7 +41,7 @@ class FmmConfig:
 
     def to_c(self):
         c = _capi.PamFmmConfig()
-        c.precision = {"fp64": 0, "mixed": 1}[self.precision]
+        c.precision = {"fp64": 0, "mixed": 1, "mixed_comp": 2}[self.precision]
         c.order = self.order
         c.leaf_level = self.leaf_level
         c.mode = {"auto": 0, "direct": 1, "fmm": 2}[self.mode]
@@ -50,7 +50,7 @@ class FmmConfig:
 
     @staticmethod
     def from_c(c):
-        return FmmConfig({0: "fp64", 1: "mixed"}[c.precision], c.order, c.leaf_level,
+        return FmmConfig({0: "fp64", 1: "mixed", 2: "mixed_comp"}[c.precision], c.order, c.leaf_level,
                          {0: "auto", 1: "direct", 2: "fmm"}[c.mode], c.svd_tolerance)
 
     def __repr__(self):
