@@ -130,10 +130,13 @@ InequalityResult fit_inequality(const Model& model, const std::vector<double>& p
     std::vector<char> kept(static_cast<size_t>(n), 0);
     for (int64_t j : filter.filter(dist, sorted)) kept[j] = 1;
 
-    // Update the active set (inequality_fitter.hpp:139-181).
+    // Update the active set (inequality_fitter.hpp:139-181). A point that
+    // also has a value is fitted at that value and never joins as a bound
+    // (the reference would add it to the centres a second time).
     bool changed = false;
     for (int64_t i = 0; i < n_ineq; i++) {
       const int64_t j = ineq[i];
+      if (!std::isnan(values[j])) continue;
       if (!std::isnan(lb[j])) {
         if (act_lb.count(j)) {
           if (weights[j] <= 0.0) {

@@ -163,3 +163,18 @@ def test_inequality_fit_bounds_only():
     assert ip.weights().shape == (n + 1,)
     fit = ip.evaluate(pts, accuracy=tol / 100)
     assert (fit > lb - tol).all() and (fit < ub + tol).all()
+
+
+@pytest.mark.gpu
+def test_inequality_fit_value_and_bound_on_one_point():
+    """A point with a value and a bound that its value violates is fitted at
+    the value and appears once among the centres."""
+    pts, v, lb, ub = kostov86()
+    lb[0] = 5.0  # v[0] = 1
+    tol = 1e-5
+    ip = pa.Interpolant(pa.Model(pa.CovExponential([1.0, 3.0]), poly_degree=-1))
+    ip.fit_inequality(pts, v, lb, ub, tol, max_iter=32, accuracy=tol / 100)
+    c = ip.centers()
+    assert len(np.unique(c, axis=0)) == len(c)
+    fit = ip.evaluate(pts, accuracy=tol / 100)
+    assert abs(fit[0] - 1.0) < tol
