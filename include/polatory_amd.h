@@ -225,6 +225,8 @@ typedef struct {
   int verbose;              /* print the residual per iteration (solver.hpp:111-128) */
   int use_graphs;           /* replay "preconditioner + matvec" of each FGMRES step from a HIP graph */
   int keep_unconverged;     /* at max_iter keep the last iterate instead of failing (default 0) */
+  int tune_solves;          /* time the RAS batched-solve workgroup sizes per level at set-up (default 0:
+                               a fixed rule that reproduces the timed choice on the 1M-point fit) */
 } pam_fit_options;
 int pam_fit_options_default(pam_fit_options* opt);
 /* Interpolant(const Model&) — interpolant.hpp:43 */

@@ -40,7 +40,8 @@ class PamFitOptions(ctypes.Structure):
     _fields_ = [("op", PamFmmConfig), ("residual", PamFmmConfig), ("level", PamFmmConfig),
                 ("use_level_config", ctypes.c_int), ("use_preconditioner", ctypes.c_int),
                 ("refine_rtol", ctypes.c_double), ("verbose", ctypes.c_int),
-                ("use_graphs", ctypes.c_int), ("keep_unconverged", ctypes.c_int)]
+                ("use_graphs", ctypes.c_int), ("keep_unconverged", ctypes.c_int),
+                ("tune_solves", ctypes.c_int)]
 
 
 class PamFitStats(ctypes.Structure):

@@ -57,6 +57,7 @@ pam::FitOptions to_opts(const pam_fit_options* o) {
   r.use_graphs = o->use_graphs != 0;
   r.keep_unconverged = o->keep_unconverged != 0;
   r.verbose = o->verbose;
+  r.tune_solves = o->tune_solves != 0;
   return r;
 }
 
@@ -182,6 +183,7 @@ int pam_fit_options_default(pam_fit_options* opt) {
     opt->use_graphs = o.use_graphs ? 1 : 0;
     opt->keep_unconverged = o.keep_unconverged ? 1 : 0;
     opt->verbose = 0;
+    opt->tune_solves = o.tune_solves ? 1 : 0;
   });
 }
 

@@ -333,14 +333,19 @@ cholesky_trailing_kernel(double* __restrict__ a, int n, int k0) {
   chol_tile(sm, a, n, k0, I, J, true);
 }
 
-__device__ __forceinline__ double block_sum(double v, double* sh) {
-  sh[threadIdx.x] = v;
+
+// Sum of the partials of the first 64 threads, added in lane order by one
+// thread: the result does not depend on the workgroup size.
+__device__ __forceinline__ double lane_ordered_sum(double v, double* sh) {
+  if (threadIdx.x < 64) sh[threadIdx.x] = v;
   __syncthreads();
-  for (int off = kT / 2; off > 0; off >>= 1) {
-    if (static_cast<int>(threadIdx.x) < off) sh[threadIdx.x] += sh[threadIdx.x + off];
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < 64; k++) t += sh[k];
+    sh[64] = t;
   }
-  double r = sh[0];
+  __syncthreads();
+  const double r = sh[64];
   __syncthreads();
   return r;
 }
@@ -350,7 +355,9 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 // P_top c = d_top - A_top lambda. vals (LDS, m entries) holds d on entry;
 // emit(u, v) receives entry u < m of the solution (domain order) and
 // emit(m + a, c_a) the polynomial coefficients of the coarse grid.
-template <class Emit>
+// KT threads (a multiple of 64, at least 128); every sum is taken in an order
+// that does not depend on KT, so the solution is the same bits for any KT.
+template <int KT, class Emit>
 __device__ void solve_domain(RbfSet r, int l, const double* __restrict__ pts, const int32_t* __restrict__ didx,
                              int m, const double* __restrict__ lag, const double* __restrict__ L,
                              double* vals, double* y, double* sh, int coarse, const double* __restrict__ pinv,
@@ -359,7 +366,7 @@ __device__ void solve_domain(RbfSet r, int l, const double* __restrict__ pts, co
   __shared__ double rhs_top[10];
   const int n = m - l;
   if (n > 0) {
-    for (int i = threadIdx.x; i < n; i += kT) {
+    for (int i = threadIdx.x; i < n; i += KT) {
       double v = vals[l + i];
       int qi = didx[l + i];
       for (int a = 0; a < l; a++) v -= lag[static_cast<int64_t>(qi) * l + a] * vals[a];
@@ -372,7 +379,7 @@ __device__ void solve_domain(RbfSet r, int l, const double* __restrict__ pts, co
     // loads: both keep the sequential parts off global-memory latency.
     __shared__ double dgl[kNb][kNb + 1];
     auto stage_diag = [&](int k0, int kb) {
-      for (int e = threadIdx.x; e < kb * kb; e += kT) {
+      for (int e = threadIdx.x; e < kb * kb; e += KT) {
         const int rr = e % kb, cc = e / kb;
         dgl[rr][cc] = L[static_cast<int64_t>(k0 + cc) * lda + k0 + rr];
       }
@@ -395,7 +402,7 @@ __device__ void solve_domain(RbfSet r, int l, const double* __restrict__ pts, co
       }
       __syncthreads();
       if (kb == kNb) {
-        for (int i = k0 + kNb + threadIdx.x; i < n; i += kT) {
+        for (int i = k0 + kNb + threadIdx.x; i < n; i += KT) {
           double lv[kNb];
 #pragma unroll
           for (int t = 0; t < kNb; t++) lv[t] = L[static_cast<int64_t>(k0 + t) * lda + i];
@@ -405,7 +412,7 @@ __device__ void solve_domain(RbfSet r, int l, const double* __restrict__ pts, co
           y[i] = v;
         }
       } else {
-        for (int i = k0 + kb + threadIdx.x; i < n; i += kT) {
+        for (int i = k0 + kb + threadIdx.x; i < n; i += KT) {
           double v = y[i];
           for (int t = 0; t < kb; t++) v -= L[static_cast<int64_t>(k0 + t) * lda + i] * y[k0 + t];
           y[i] = v;
@@ -430,7 +437,7 @@ __device__ void solve_domain(RbfSet r, int l, const double* __restrict__ pts, co
       }
       __syncthreads();
       if (kb == kNb) {
-        for (int i = threadIdx.x; i < k0; i += kT) {
+        for (int i = threadIdx.x; i < k0; i += KT) {
           double lv[kNb];
 #pragma unroll
           for (int t = 0; t < kNb; t++) lv[t] = L[static_cast<int64_t>(i) * lda + k0 + t];
@@ -440,7 +447,7 @@ __device__ void solve_domain(RbfSet r, int l, const double* __restrict__ pts, co
           y[i] = v;
         }
       } else {
-        for (int i = threadIdx.x; i < k0; i += kT) {
+        for (int i = threadIdx.x; i < k0; i += KT) {
           double v = y[i];
           for (int t = 0; t < kb; t++) v -= L[static_cast<int64_t>(i) * lda + k0 + t] * y[k0 + t];
           y[i] = v;
@@ -452,22 +459,24 @@ __device__ void solve_domain(RbfSet r, int l, const double* __restrict__ pts, co
   // lambda_top[a] = sum_i Q[a][i] gamma_i = -sum_i lag[q_i][a] gamma_i
   for (int a = 0; a < l; a++) {
     double acc = 0.0;
-    for (int i = threadIdx.x; i < n; i += kT) acc -= lag[static_cast<int64_t>(didx[l + i]) * l + a] * y[i];
-    double t = block_sum(acc, sh);
+    if (threadIdx.x < 64)
+      for (int i = threadIdx.x; i < n; i += 64) acc -= lag[static_cast<int64_t>(didx[l + i]) * l + a] * y[i];
+    double t = lane_ordered_sum(acc, sh);
     if (threadIdx.x == 0) lam_top[a] = t;
   }
   __syncthreads();
-  for (int u = threadIdx.x; u < m; u += kT) emit(u, u < l ? lam_top[u] : y[u - l]);
+  for (int u = threadIdx.x; u < m; u += KT) emit(u, u < l ? lam_top[u] : y[u - l]);
   if (coarse && l > 0) {
     // polynomial part: P_top c = d_top - A_top lambda
     for (int a = 0; a < l; a++) {
       double acc = 0.0;
       int pa = didx[a];
-      for (int u = threadIdx.x; u < m; u += kT) {
-        double lu = u < l ? lam_top[u] : y[u - l];
-        acc += kernel_entry(r, pts, pa, didx[u]) * lu;
-      }
-      double t = block_sum(acc, sh);
+      if (threadIdx.x < 64)
+        for (int u = threadIdx.x; u < m; u += 64) {
+          double lu = u < l ? lam_top[u] : y[u - l];
+          acc += kernel_entry(r, pts, pa, didx[u]) * lu;
+        }
+      double t = lane_ordered_sum(acc, sh);
       if (threadIdx.x == 0) rhs_top[a] = vals[a] - t;
     }
     __syncthreads();
@@ -482,7 +491,8 @@ __device__ void solve_domain(RbfSet r, int l, const double* __restrict__ pts, co
 // Domain solves of one batch: gather d from res, scatter the owned entries
 // of the solution (every entry for the coarse grid) and the coarse grid's
 // polynomial coefficients into out.
-__global__ void __launch_bounds__(kT)
+template <int KT>
+__global__ void __launch_bounds__(KT)
 ras_solve_kernel(RbfSet r, int l, const double* __restrict__ pts,
                  const int64_t* __restrict__ ptr, const int32_t* __restrict__ idx,
                  const uint8_t* __restrict__ inner, const double* __restrict__ lag,
@@ -490,17 +500,17 @@ ras_solve_kernel(RbfSet r, int l, const double* __restrict__ pts,
                  const double* __restrict__ res, double* __restrict__ out, int64_t mu, int coarse,
                  const double* __restrict__ pinv) {
   extern __shared__ double sm[];
-  __shared__ double sh[kT];
+  __shared__ double sh[KT];
   const int d = blockIdx.x;
   const int64_t p0 = ptr[d];
   const int m = static_cast<int>(ptr[d + 1] - p0);
   double* vals = sm;      // [m]
   double* y = sm + m;     // [m - l]
-  for (int u = threadIdx.x; u < m; u += kT) vals[u] = res[idx[p0 + u]];
+  for (int u = threadIdx.x; u < m; u += KT) vals[u] = res[idx[p0 + u]];
   __syncthreads();
   const int32_t* didx = idx + p0;
   const uint8_t* din = inner + p0;
-  solve_domain(r, l, pts, didx, m, lag, mat + mat_ptr[d], vals, y, sh, coarse, pinv, [&](int u, double v) {
+  solve_domain<KT>(r, l, pts, didx, m, lag, mat + mat_ptr[d], vals, y, sh, coarse, pinv, [&](int u, double v) {
     if (u < m) {
       if (coarse || din[u]) out[didx[u]] = v;
     } else {
@@ -522,7 +532,7 @@ coarse_operator_kernel(RbfSet r, int l, const double* __restrict__ pts, const in
   double* y = sm + m;
   for (int u = threadIdx.x; u < m; u += kT) vals[u] = u == u0 ? 1.0 : 0.0;
   __syncthreads();
-  solve_domain(r, l, pts, idx, m, lag, mat, vals, y, sh, 1, pinv,
+  solve_domain<kT>(r, l, pts, idx, m, lag, mat, vals, y, sh, 1, pinv,
                [&](int u, double v) { op[static_cast<int64_t>(u) * m + u0] = v; });
 }
 
@@ -1074,6 +1084,15 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
   }
   PAM_HIP(hipStreamSynchronize(s_));
   tick("A P done");
+  if (opt_.tune_solves) {
+    tune_solves();
+    tick("solve tuning");
+  } else {
+    // Few large batches: more threads per domain; many domains: 256 (the
+    // timing on the 1M-point fit: 512 for 32 and 256 domains, 256 for 2048).
+    for (auto& b : fine_) b.solve_threads = b.n_dom < 1024 ? 512 : 256;
+    coarse_.solve_threads = 512;
+  }
   setup_s_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -1177,13 +1196,61 @@ void RasPreconditioner::solve(int level, const double* d_res, double* d_w) {
     PAM_LAUNCH_CHECK();
     return;
   }
+  launch_solve(b, level, b.solve_threads > 0 ? b.solve_threads : kT, d_res, d_w);
+}
+
+void RasPreconditioner::launch_solve(const DomainBatch& b, int level, int threads, const double* d_res,
+                                     double* d_w) {
   size_t shmem = sizeof(double) * 2 * static_cast<size_t>(b.max_m);
   if (shmem > 96 * 1024) throw_runtime("subdomain too large for the solve kernel");
   RbfSet rs = make_rbf_set(model_);
-  ras_solve_kernel<<<b.n_dom, kT, shmem, s_>>>(rs, l_, d_pts_.get(), b.ptr.get(), b.idx.get(), b.inner.get(),
-                                                lag_.get(), b.mat_ptr.get(), b.mat.get(), d_res, d_w, mu_,
-                                                level == 0 ? 1 : 0, pinv_.get());
+  auto go = [&](auto kt) {
+    constexpr int KT = decltype(kt)::value;
+    ras_solve_kernel<KT><<<b.n_dom, KT, shmem, s_>>>(rs, l_, d_pts_.get(), b.ptr.get(), b.idx.get(),
+                                                     b.inner.get(), lag_.get(), b.mat_ptr.get(), b.mat.get(),
+                                                     d_res, d_w, mu_, level == 0 ? 1 : 0, pinv_.get());
+  };
+  switch (threads) {
+    case 128: go(std::integral_constant<int, 128>{}); break;
+    case 256: go(std::integral_constant<int, 256>{}); break;
+    case 512: go(std::integral_constant<int, 512>{}); break;
+    default: throw_invalid("unsupported solve workgroup size");
+  }
   PAM_LAUNCH_CHECK();
+}
+
+void RasPreconditioner::tune_solves() {
+  // Workgroup size of each level's batched solve, timed on the batch itself
+  // (a zero right-hand side: the work does not depend on the values). Every
+  // size gives the same bits, so the choice only changes the time.
+  DevBuf<double> res(static_cast<size_t>(mu_ + l_)), w(static_cast<size_t>(mu_ + l_));
+  dev_fill(0.0, res.get(), mu_ + l_, s_);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  PAM_HIP(hipEventCreate(&e0));
+  PAM_HIP(hipEventCreate(&e1));
+  for (int level = 0; level < n_levels_; level++) {
+    DomainBatch& b = level == 0 ? coarse_ : fine_[level];
+    if (b.n_dom == 0 || (level == 0 && b.has_op)) continue;
+    float best = 0.0f;
+    for (int threads : {128, 256, 512}) {
+      launch_solve(b, level, threads, res.get(), w.get());  // warm-up
+      PAM_HIP(hipEventRecord(e0, s_));
+      for (int rep = 0; rep < 2; rep++) launch_solve(b, level, threads, res.get(), w.get());
+      PAM_HIP(hipEventRecord(e1, s_));
+      PAM_HIP(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      PAM_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (b.solve_threads == 0 || ms < best) {
+        best = ms;
+        b.solve_threads = threads;
+      }
+    }
+    if (opt_.verbose)
+      std::printf("RAS level %d: %d domains, solve workgroup %d threads (%.3f ms)\n", level, b.n_dom,
+                  b.solve_threads, best / 2.0f);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
 }
 
 void RasPreconditioner::compute_ap() {

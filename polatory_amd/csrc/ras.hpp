@@ -46,12 +46,17 @@ struct DomainBatch {
   DevBuf<double> mat;                    // lower Cholesky factors, column-major
   DevBuf<double> op;                     // coarse grid: its solve as a dense (m + l) x m operator
   bool has_op = false;
+  int solve_threads = 0;                 // workgroup size of the batched solve (0: default)
 };
 
 struct RasOptions {
   FmmConfig level_cfg;      // inter-level residual updates (default accuracy)
   bool use_level_cfg = false;
   int verbose = 0;          // print the set-up phase times
+  // Time each level's batched solve at 128, 256 and 512 threads per domain and
+  // keep the fastest (off: the rule below, which reproduces what the timing
+  // picked on the 1M-point fit without its cost).
+  bool tune_solves = false;
 };
 
 class RasPreconditioner : public LinearOperator {
@@ -87,6 +92,10 @@ class RasPreconditioner : public LinearOperator {
   };
   std::vector<std::unique_ptr<SetupTmp>> setup_tmp_;
   void solve(int level, const double* d_res, double* d_w);
+  void launch_solve(const DomainBatch& b, int level, int threads, const double* d_res, double* d_w);
+  // Picks each level's solve workgroup size by timing (an autotuner over
+  // launch shapes; the factorisation tiles stay fixed).
+  void tune_solves();
   void update_residuals(int src, int trg, const double* d_w, double* d_res);
   void orthogonalize(double* d_w, double* d_res);
   void compute_ap();

@@ -144,6 +144,7 @@ Solver::Solver(const Model& model, const std::vector<double>& points, const FitO
     ro.level_cfg = opt_.level_cfg;
     ro.use_level_cfg = opt_.use_level_cfg;
     ro.verbose = opt_.verbose;
+    ro.tune_solves = opt_.tune_solves;
     pc_ = std::make_unique<RasPreconditioner>(model_, points, s_, ro);
   }
   if (l_ > 0) {

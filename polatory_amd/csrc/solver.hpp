@@ -81,6 +81,8 @@ struct FitOptions {
   // At max_iter keep the last iterate (FitStats::converged = false) instead of
   // throwing, as polatory does (solver.hpp:131-133).
   bool keep_unconverged = false;
+  // Time the RAS solve workgroup sizes per level at set-up (RasOptions::tune_solves).
+  bool tune_solves = false;
 };
 
 struct FitStats {

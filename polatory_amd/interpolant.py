@@ -91,7 +91,8 @@ class FitOptions:
     pam_fit_options_default)."""
 
     def __init__(self, op=None, residual=None, level=None, use_preconditioner=True,
-                 refine_rtol=None, verbose=False, use_graphs=True, keep_unconverged=False):
+                 refine_rtol=None, verbose=False, use_graphs=True, keep_unconverged=False,
+                 tune_solves=False):
         d = _capi.PamFitOptions()
         _capi.call("pam_fit_options_default", ctypes.byref(d))
         self._c = d
@@ -108,6 +109,7 @@ class FitOptions:
         d.verbose = 1 if verbose else 0
         d.use_graphs = 1 if use_graphs else 0
         d.keep_unconverged = 1 if keep_unconverged else 0
+        d.tune_solves = 1 if tune_solves else 0
 
 
 class Interpolant:

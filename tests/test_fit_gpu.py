@@ -262,3 +262,18 @@ def test_cpp_api_mirror_runs():
     r = subprocess.run([exe], capture_output=True, text=True, timeout=240)
     log(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_solve_tuning_gives_identical_weights():
+    """The RAS solve workgroup size chosen by timing (tune_solves) changes only
+    the time: every size sums in the same order, so the weights are the same
+    bits as with the default rule."""
+    rng = np.random.default_rng(91)
+    pts = rng.uniform(0, 1, (60_000, 3))
+    vals = np.sin(3 * pts[:, 0]) * np.cos(2 * pts[:, 1]) + 0.5 * pts[:, 2] ** 2
+    ws = []
+    for tune in (False, True):
+        ip = pa.Interpolant(pa.Model(pa.Biharmonic3D(), poly_degree=1))
+        ip.fit(pts, vals, 1e-5, options=pa.FitOptions(tune_solves=tune))
+        ws.append(ip.weights())
+    assert np.array_equal(ws[0], ws[1])
