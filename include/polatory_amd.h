@@ -279,6 +279,16 @@ int pam_interpolant_evaluate_device(pam_interpolant* h, const double* d_points, 
  * gradients. */
 int pam_interpolant_evaluate_gradient(pam_interpolant* h, const double* points, int64_t n, double accuracy,
                                       double* out);
+/* set_evaluation_bbox_impl(bbox, accuracy, grad_accuracy) — interpolant.hpp:201-210:
+ * builds the evaluators for the union of [bbox_min, bbox_max] and the centres' box,
+ * at these accuracies; evaluate_impl then reuses them. This pair is the seam
+ * isosurface::RbfFieldFunction calls (rbf_field_function.hpp:21-27). */
+int pam_interpolant_set_evaluation_bbox(pam_interpolant* h, const double* bbox_min, const double* bbox_max,
+                                        double accuracy, double grad_accuracy);
+/* evaluate_impl(points) — interpolant.hpp:69-75: evaluation at the accuracy and box
+ * of the last set_evaluation_bbox (or evaluate) call. */
+int pam_interpolant_evaluate_impl(pam_interpolant* h, const double* points, int64_t n, double* out);
+int pam_interpolant_evaluate_impl_device(pam_interpolant* h, const double* d_points, int64_t n, double* d_out);
 /* Fixes the evaluation FMM configuration (NULL restores the accuracy-driven choice). */
 int pam_interpolant_set_eval_config(pam_interpolant* h, const pam_fmm_config* cfg);
 int pam_interpolant_num_centers(const pam_interpolant* h, int64_t* n);

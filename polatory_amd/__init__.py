@@ -9,7 +9,7 @@ from .fmm import (FmmConfig, FmmEvaluator, FmmGradientEvaluator, FmmGradientTran
                   FmmHessianEvaluator, FmmSymmetricEvaluator, direct_evaluate, direct_evaluate_gradient,
                   make_fmm_evaluator, make_fmm_gradient_evaluator, make_fmm_gradient_transpose_evaluator,
                   make_fmm_hessian_evaluator, make_fmm_symmetric_evaluator)
-from .interpolant import (FitOptions, Interpolant, Model, RasPreconditioner, cross_validate,
+from .interpolant import (FitOptions, Interpolant, Model, RasPreconditioner, RbfFieldFunction, cross_validate,
                           distance_filter, fgmres_dense, POLY_DEGREE_MIN)
 from .rbf import Biharmonic3D, CovExponential, Rbf, Triharmonic3D, make_rbf
 from .serve import Server
@@ -28,5 +28,5 @@ __all__ = ["Biharmonic3D", "Triharmonic3D", "CovExponential", "Rbf", "make_rbf",
            "make_fmm_evaluator", "make_fmm_symmetric_evaluator", "make_fmm_gradient_transpose_evaluator", "direct_evaluate", "direct_evaluate_gradient",
            "device_count", "Model",
            "Interpolant", "FitOptions", "RasPreconditioner", "fgmres_dense", "POLY_DEGREE_MIN", "distance_filter",
-           "cross_validate",
+           "cross_validate", "RbfFieldFunction",
            "Server"]

@@ -144,6 +144,9 @@ _SIGS = {
     "pam_interpolant_evaluate": [_P, _P, _I64, _D, _P],
     "pam_interpolant_evaluate_device": [_P, _P, _I64, _D, _P],
     "pam_interpolant_set_eval_config": [_P, ctypes.POINTER(PamFmmConfig)],
+    "pam_interpolant_set_evaluation_bbox": [_P, _P, _P, _D, _D],
+    "pam_interpolant_evaluate_impl": [_P, _P, _I64, _P],
+    "pam_interpolant_evaluate_impl_device": [_P, _P, _I64, _P],
     "pam_interpolant_num_centers": [_P, ctypes.POINTER(_I64)],
     "pam_interpolant_centers": [_P, _P],
     "pam_interpolant_weights": [_P, _P],

@@ -279,6 +279,30 @@ int pam_interpolant_evaluate_gradient(pam_interpolant* h, const double* points, 
   });
 }
 
+int pam_interpolant_set_evaluation_bbox(pam_interpolant* h, const double* bbox_min, const double* bbox_max,
+                                        double accuracy, double grad_accuracy) {
+  return guard([&] {
+    if (!bbox_min || !bbox_max) pam::throw_invalid("null argument");
+    ip(h).set_evaluation_bbox(bbox_min, bbox_max, accuracy, grad_accuracy);
+  });
+}
+
+int pam_interpolant_evaluate_impl(pam_interpolant* h, const double* points, int64_t n, double* out) {
+  return guard([&] {
+    if (n < 0) pam::throw_invalid("negative point count");
+    if (n > 0 && (!points || !out)) pam::throw_invalid("null argument");
+    ip(h).evaluate_impl(points, n, out, false);
+  });
+}
+
+int pam_interpolant_evaluate_impl_device(pam_interpolant* h, const double* d_points, int64_t n, double* d_out) {
+  return guard([&] {
+    if (n < 0) pam::throw_invalid("negative point count");
+    if (n > 0 && (!d_points || !d_out)) pam::throw_invalid("null argument");
+    ip(h).evaluate_impl(d_points, n, d_out, true);
+  });
+}
+
 int pam_interpolant_set_eval_config(pam_interpolant* h, const pam_fmm_config* cfg) {
   return guard([&] {
     if (cfg) {

@@ -101,6 +101,7 @@ void Interpolant::fit(const std::vector<double>& points, const std::vector<doubl
   weights_ = solver.solve(values, tolerance, max_iter, initial ? &init : nullptr, &stats_);
   centers_ = points;
   fitted_ = true;
+  have_eval_bbox_ = false;
   for (int k = 0; k < 3; k++) {
     bbox_lo_[k] = INFINITY;
     bbox_hi_[k] = -INFINITY;
@@ -153,6 +154,7 @@ void Interpolant::set_fitted(const std::vector<double>& centers, const std::vect
   centers_ = centers;
   weights_ = weights;
   fitted_ = true;
+  have_eval_bbox_ = false;
   ev_.reset();
   gev_.reset();
   for (int k = 0; k < 3; k++) {
@@ -177,7 +179,8 @@ Evaluator& Interpolant::evaluator() {
   throw_if_not_fitted();
   if (!ev_) {
     int64_t mu = static_cast<int64_t>(centers_.size() / 3);
-    ev_ = std::make_unique<Evaluator>(model_, bbox_lo_, bbox_hi_, ev_acc_ > 0 ? ev_acc_ : INFINITY, s_);
+    ev_ = std::make_unique<Evaluator>(model_, have_eval_bbox_ ? eval_lo_ : bbox_lo_,
+                                      have_eval_bbox_ ? eval_hi_ : bbox_hi_, ev_acc_ > 0 ? ev_acc_ : INFINITY, s_);
     if (have_eval_cfg_) ev_->set_fmm_config(eval_cfg_);
     d_centers_.upload(centers_.data(), centers_.size(), s_);
     d_weights_.upload(weights_.data(), weights_.size(), s_);
@@ -192,7 +195,9 @@ Evaluator& Interpolant::gradient_evaluator() {
   throw_if_not_fitted();
   if (!gev_) {
     int64_t mu = static_cast<int64_t>(centers_.size() / 3);
-    gev_ = std::make_unique<Evaluator>(model_, bbox_lo_, bbox_hi_, gev_acc_ > 0 ? gev_acc_ : INFINITY, s_, true);
+    gev_ = std::make_unique<Evaluator>(model_, have_eval_bbox_ ? eval_lo_ : bbox_lo_,
+                                       have_eval_bbox_ ? eval_hi_ : bbox_hi_, gev_acc_ > 0 ? gev_acc_ : INFINITY,
+                                       s_, true);
     if (have_eval_cfg_) gev_->set_fmm_config(eval_cfg_);
     d_centers_.upload(centers_.data(), centers_.size(), s_);
     d_weights_.upload(weights_.data(), weights_.size(), s_);
@@ -201,6 +206,27 @@ Evaluator& Interpolant::gradient_evaluator() {
     PAM_HIP(hipStreamSynchronize(s_));
   }
   return *gev_;
+}
+
+void Interpolant::set_evaluation_bbox(const double lo[3], const double hi[3], double accuracy,
+                                      double grad_accuracy) {
+  throw_if_not_fitted();
+  if (!(accuracy > 0.0) || !(grad_accuracy > 0.0)) throw_invalid("accuracy must be positive");
+  for (int k = 0; k < 3; k++)
+    if (!(lo[k] <= hi[k])) throw_invalid("bbox min must not exceed bbox max");
+  for (int k = 0; k < 3; k++) {
+    eval_lo_[k] = std::min(lo[k], bbox_lo_[k]);
+    eval_hi_[k] = std::max(hi[k], bbox_hi_[k]);
+  }
+  have_eval_bbox_ = true;
+  ev_.reset();
+  gev_.reset();
+  ev_acc_ = accuracy;
+  gev_acc_ = grad_accuracy;
+}
+
+void Interpolant::evaluate_impl(const double* points, int64_t n, double* out, bool on_device) {
+  evaluate(points, n, ev_acc_ > 0.0 ? ev_acc_ : INFINITY, out, on_device);
 }
 
 void Interpolant::evaluate(const double* points, int64_t n, double accuracy, double* out, bool on_device) {

@@ -33,6 +33,13 @@ class Interpolant {
   void fit_inequality(const std::vector<double>& points, const std::vector<double>& values,
                       const std::vector<double>& lb, const std::vector<double>& ub, double tolerance, int max_iter,
                       double accuracy, const FitOptions& opt, const Interpolant* initial);
+  // interpolant.hpp:201-210: the evaluators that evaluate_impl uses are built
+  // for the union of this box and the centres' box, at these accuracies
+  // (the seam isosurface::RbfFieldFunction calls, rbf_field_function.hpp:21-27).
+  void set_evaluation_bbox(const double lo[3], const double hi[3], double accuracy, double grad_accuracy);
+  // interpolant.hpp:69-75: evaluation with the evaluator as it stands (the
+  // accuracy and box of the last set_evaluation_bbox or evaluate call).
+  void evaluate_impl(const double* points, int64_t n, double* out, bool on_device);
   // Installs centres and weights directly (a saved or externally fitted model).
   void set_fitted(const std::vector<double>& centers, const std::vector<double>& weights);
   void set_eval_config(const FmmConfig* c);
@@ -63,6 +70,10 @@ class Interpolant {
   FmmConfig eval_cfg_;
   DevBuf<double> d_centers_, d_weights_, d_pts_, d_out_;
   double bbox_lo_[3] = {0, 0, 0}, bbox_hi_[3] = {0, 0, 0};
+  // The box of the last set_evaluation_bbox (centres' box included); cleared
+  // by every fit.
+  bool have_eval_bbox_ = false;
+  double eval_lo_[3] = {0, 0, 0}, eval_hi_[3] = {0, 0, 0};
 };
 
 // kriging/cross_validate.hpp:14-47: prediction at every point from the fit

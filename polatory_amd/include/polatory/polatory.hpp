@@ -311,6 +311,20 @@ class Interpolant {
     return out;
   }
 
+  // interpolant.hpp:69-75
+  VecX evaluate_impl(const Points& points) const {
+    VecX out(points.rows());
+    detail::check(pam_interpolant_evaluate_impl(h_, points.data(), points.rows(), out.data()));
+    return out;
+  }
+
+  // interpolant.hpp:201-210
+  void set_evaluation_bbox_impl(const geometry::Bbox<Dim>& bbox, double accuracy = kInfinity,
+                                double grad_accuracy = kInfinity) {
+    detail::check(pam_interpolant_set_evaluation_bbox(h_, bbox.min().data(), bbox.max().data(), accuracy,
+                                                      grad_accuracy));
+  }
+
   // interpolant.hpp:115-147 (value data)
   void fit_incrementally(const Points& points, const VecX& values, double tolerance, int max_iter = 100,
                          double accuracy = kInfinity) {
@@ -391,6 +405,32 @@ VecX cross_validate(const Model<Dim>& model, const geometry::Points<Dim>& points
 }
 
 }  // namespace kriging
+
+namespace isosurface {
+
+// isosurface/rbf_field_function.hpp:12-36: the field an isosurface extractor
+// samples (the FieldFunction base is not part of this library).
+class RbfFieldFunction {
+  static constexpr double kInfinity = std::numeric_limits<double>::infinity();
+
+ public:
+  explicit RbfFieldFunction(Interpolant<3>& interpolant, double accuracy = kInfinity,
+                            double grad_accuracy = kInfinity)
+      : interpolant_(interpolant), accuracy_(accuracy), grad_accuracy_(grad_accuracy) {}
+
+  VecX operator()(const geometry::Points<3>& points) const { return interpolant_.evaluate_impl(points); }
+
+  void set_evaluation_bbox(const geometry::Bbox<3>& bbox) {
+    interpolant_.set_evaluation_bbox_impl(bbox, accuracy_, grad_accuracy_);
+  }
+
+ private:
+  Interpolant<3>& interpolant_;
+  double accuracy_;
+  double grad_accuracy_;
+};
+
+}  // namespace isosurface
 
 namespace fmm {
 

@@ -211,6 +211,23 @@ class Interpolant:
                    _ptr(out))
         return out
 
+    def set_evaluation_bbox(self, bbox_min, bbox_max, accuracy=math.inf, grad_accuracy=math.inf):
+        """set_evaluation_bbox_impl (interpolant.hpp:201-210): builds the
+        evaluators for the union of the box and the centres' box, at these
+        accuracies, for evaluate_impl to reuse."""
+        lo = np.ascontiguousarray(bbox_min, dtype=np.float64).reshape(3)
+        hi = np.ascontiguousarray(bbox_max, dtype=np.float64).reshape(3)
+        _capi.call("pam_interpolant_set_evaluation_bbox", self._h, _ptr(lo), _ptr(hi), float(accuracy),
+                   float(grad_accuracy))
+
+    def evaluate_impl(self, points):
+        """evaluate_impl (interpolant.hpp:69-75): values at points with the
+        evaluator of the last set_evaluation_bbox or evaluate call."""
+        p = _points(points)
+        out = np.empty(p.shape[0])
+        _capi.call("pam_interpolant_evaluate_impl", self._h, _ptr(p), p.shape[0], _ptr(out))
+        return out
+
     def set_eval_config(self, config: FmmConfig = None):
         if config is None:
             _capi.call("pam_interpolant_set_eval_config", self._h, None)
@@ -249,6 +266,23 @@ class Interpolant:
         out = np.empty(n.value)
         _capi.call("pam_interpolant_residual_history", self._h, _ptr(out), ctypes.byref(n))
         return out
+
+
+class RbfFieldFunction:
+    """isosurface::RbfFieldFunction (isosurface/rbf_field_function.hpp:12-36):
+    the field an isosurface extractor samples, a fitted interpolant evaluated
+    in batches inside a box fixed beforehand."""
+
+    def __init__(self, interpolant: Interpolant, accuracy=math.inf, grad_accuracy=math.inf):
+        self.interpolant = interpolant
+        self.accuracy = accuracy
+        self.grad_accuracy = grad_accuracy
+
+    def set_evaluation_bbox(self, bbox_min, bbox_max):
+        self.interpolant.set_evaluation_bbox(bbox_min, bbox_max, self.accuracy, self.grad_accuracy)
+
+    def __call__(self, points):
+        return self.interpolant.evaluate_impl(points)
 
 
 class RasPreconditioner:

@@ -128,6 +128,36 @@ int main() {
     if (!(gerr < 1e-4)) return 10;
   }
 
+  // isosurface::RbfFieldFunction (rbf_field_function.hpp:21-27): the field on
+  // a lattice that reaches past the centres' box, against the FP64 direct sum
+  {
+    polatory::isosurface::RbfFieldFunction field(interpolant, 1e-7);
+    field.set_evaluation_bbox(Bbox3({-0.25, -0.25, -0.25}, {1.25, 1.25, 1.25}));
+    const Index g = 16;
+    Points3 lat(g * g * g, 3);
+    for (Index i = 0; i < g * g * g; i++) {
+      lat(i, 0) = -0.25 + 1.5 * (i % g) / (g - 1);
+      lat(i, 1) = -0.25 + 1.5 * ((i / g) % g) / (g - 1);
+      lat(i, 2) = -0.25 + 1.5 * (i / (g * g)) / (g - 1);
+    }
+    VecX f = field(lat);
+    VecX f2 = field(lat);
+    double ferr = 0.0, fscale = 0.0;
+    for (Index i = 0; i < lat.rows(); i++) {
+      if (f(i) != f2(i)) return 11;
+      double d = w(n) + w(n + 1) * lat(i, 0) + w(n + 2) * lat(i, 1) + w(n + 3) * lat(i, 2);
+      for (Index j = 0; j < n; j++) {
+        double dx = lat(i, 0) - pts(j, 0), dy = lat(i, 1) - pts(j, 1), dz = lat(i, 2) - pts(j, 2);
+        d += -std::sqrt(dx * dx + dy * dy + dz * dz) * w(j);
+      }
+      ferr = std::max(ferr, std::abs(f(i) - d));
+      fscale = std::max(fscale, std::abs(d));
+    }
+    std::printf("RbfFieldFunction on a %ld^3 lattice: max error %.3e (scale %.3e)\n", static_cast<long>(g), ferr,
+                fscale);
+    if (!(ferr <= 1e-6 * std::max(1.0, fscale))) return 12;
+  }
+
   // error contract: invalid arguments throw std::invalid_argument
   try {
     interpolant.fit(pts, values, 0.0);
