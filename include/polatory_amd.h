@@ -80,10 +80,15 @@ int pam_fmm_gradient_evaluator_create(const pam_rbf* rbf, const double bbox_min[
  * and evaluate() returns 3 n_targets entries, out[3 i + c] =
  * -(sum_j A^T H(A (t_i - s_j)) A w_j)_c. Evaluated as three Hessian-column passes; the
  * accuracy search measures gradient-transpose errors (Hessians at the same
- * configuration are less accurate).
- * The symmetric form (make_fmm_hessian_symmetric_evaluator) has no counterpart. */
+ * configuration are less accurate). */
 int pam_fmm_hessian_evaluator_create(const pam_rbf* rbf, const double bbox_min[3], const double bbox_max[3],
                                      pam_fmm_evaluator** out);
+/* make_fmm_hessian_symmetric_evaluator (fmm_symmetric_evaluator.hpp:85-86,
+ * make_fmm_evaluator.cpp:241-270): the Hessian kernel with the points as both
+ * sources and targets. Use pam_fmm_evaluator_set_points; weights and results
+ * are 3 per point (x y z). The zero-distance term is zero for bh3 and th3. */
+int pam_fmm_hessian_symmetric_evaluator_create(const pam_rbf* rbf, const double bbox_min[3],
+                                               const double bbox_max[3], pam_fmm_evaluator** out);
 int pam_fmm_evaluator_destroy(pam_fmm_evaluator* h);
 /* set_accuracy(double) — absolute L-inf accuracy; +inf = default configuration (fmm_evaluator.hpp:36) */
 int pam_fmm_evaluator_set_accuracy(pam_fmm_evaluator* h, double accuracy);

@@ -6,9 +6,10 @@ the host-side mirror used by tests and benchmarks.
 """
 from . import _capi
 from .fmm import (FmmConfig, FmmEvaluator, FmmGradientEvaluator, FmmGradientTransposeEvaluator,
-                  FmmHessianEvaluator, FmmSymmetricEvaluator, direct_evaluate, direct_evaluate_gradient,
+                  FmmHessianEvaluator, FmmHessianSymmetricEvaluator, FmmSymmetricEvaluator, direct_evaluate,
+                  direct_evaluate_gradient,
                   make_fmm_evaluator, make_fmm_gradient_evaluator, make_fmm_gradient_transpose_evaluator,
-                  make_fmm_hessian_evaluator, make_fmm_symmetric_evaluator)
+                  make_fmm_hessian_evaluator, make_fmm_hessian_symmetric_evaluator, make_fmm_symmetric_evaluator)
 from .interpolant import (FitOptions, Interpolant, Model, RasPreconditioner, RbfFieldFunction, cross_validate,
                           distance_filter, fgmres_dense, POLY_DEGREE_MIN)
 from .rbf import Biharmonic3D, CovExponential, Rbf, Triharmonic3D, make_rbf
@@ -25,7 +26,7 @@ def device_count():
 __all__ = ["Biharmonic3D", "Triharmonic3D", "CovExponential", "Rbf", "make_rbf",
            "FmmConfig", "FmmEvaluator", "FmmSymmetricEvaluator", "FmmGradientTransposeEvaluator", "FmmGradientEvaluator",
            "make_fmm_gradient_evaluator", "FmmHessianEvaluator", "make_fmm_hessian_evaluator",
-           "make_fmm_evaluator", "make_fmm_symmetric_evaluator", "make_fmm_gradient_transpose_evaluator", "direct_evaluate", "direct_evaluate_gradient",
+           "make_fmm_evaluator", "make_fmm_symmetric_evaluator", "make_fmm_hessian_symmetric_evaluator", "make_fmm_gradient_transpose_evaluator", "direct_evaluate", "direct_evaluate_gradient",
            "device_count", "Model",
            "Interpolant", "FitOptions", "RasPreconditioner", "fgmres_dense", "POLY_DEGREE_MIN", "distance_filter",
            "cross_validate", "RbfFieldFunction",

@@ -103,6 +103,7 @@ _SIGS = {
     "pam_fmm_evaluator_evaluate": [_P, _P, _I64],
     "pam_fmm_evaluator_evaluate_gradient": [_P, _P, _I64],
     "pam_fmm_hessian_evaluator_create": [ctypes.POINTER(PamRbf), _P, _P, ctypes.POINTER(_P)],
+    "pam_fmm_hessian_symmetric_evaluator_create": [ctypes.POINTER(PamRbf), _P, _P, ctypes.POINTER(_P)],
     "pam_fmm_gradient_evaluator_create": [ctypes.POINTER(PamRbf), _P, _P, ctypes.POINTER(_P)],
     "pam_fmm_gradient_transpose_evaluator_create": [ctypes.POINTER(PamRbf), _P, _P, ctypes.POINTER(_P)],
     "pam_fmm_evaluator_evaluate_gradient_device": [_P, _P, _I64],

@@ -15,6 +15,9 @@ struct pam_fmm_evaluator {
   bool symmetric;
   std::unique_ptr<pam::DipoleSourceEvaluator> dipole;   // gradient-kernel handles
   std::unique_ptr<pam::DipoleHessianEvaluator> hessian;  // Hessian-kernel handles
+  // Symmetric at the interface over a source-target evaluator: set_points
+  // installs the points as both sources and targets.
+  bool same_points = false;
 };
 
 namespace pam {
@@ -125,6 +128,15 @@ int pam_fmm_hessian_evaluator_create(const pam_rbf* rbf, const double bbox_min[3
   });
 }
 
+int pam_fmm_hessian_symmetric_evaluator_create(const pam_rbf* rbf, const double bbox_min[3],
+                                               const double bbox_max[3], pam_fmm_evaluator** out) {
+  int rc = pam_fmm_hessian_evaluator_create(rbf, bbox_min, bbox_max, out);
+  if (rc != 0) return rc;
+  (*out)->symmetric = true;
+  (*out)->same_points = true;
+  return 0;
+}
+
 int pam_fmm_evaluator_destroy(pam_fmm_evaluator* h) {
   return guard([&] { delete h; });
 }
@@ -141,13 +153,17 @@ int pam_fmm_evaluator_set_source_points(pam_fmm_evaluator* h, const double* xyz,
 }
 
 int pam_fmm_evaluator_set_target_points(pam_fmm_evaluator* h, const double* xyz, int64_t n) {
-  return guard([&] { ev(h).set_target_points(xyz, n, false); });
+  return guard([&] {
+    if (h && h->same_points) pam::throw_runtime("use set_points on a symmetric evaluator");
+    ev(h).set_target_points(xyz, n, false);
+  });
 }
 
 int pam_fmm_evaluator_set_points(pam_fmm_evaluator* h, const double* xyz, int64_t n) {
   return guard([&] {
     if (h && !h->symmetric) pam::throw_runtime("set_points needs a symmetric evaluator");
     ev(h).set_source_points(xyz, n, false);
+    if (h->same_points) ev(h).set_target_points(xyz, n, false);
   });
 }
 
@@ -175,13 +191,17 @@ int pam_fmm_evaluator_set_source_points_device(pam_fmm_evaluator* h, const doubl
 }
 
 int pam_fmm_evaluator_set_target_points_device(pam_fmm_evaluator* h, const double* d_xyz, int64_t n) {
-  return guard([&] { ev(h).set_target_points(d_xyz, n, true); });
+  return guard([&] {
+    if (h && h->same_points) pam::throw_runtime("use set_points on a symmetric evaluator");
+    ev(h).set_target_points(d_xyz, n, true);
+  });
 }
 
 int pam_fmm_evaluator_set_points_device(pam_fmm_evaluator* h, const double* d_xyz, int64_t n) {
   return guard([&] {
     if (h && !h->symmetric) pam::throw_runtime("set_points needs a symmetric evaluator");
     ev(h).set_source_points(d_xyz, n, true);
+    if (h->same_points) ev(h).set_target_points(d_xyz, n, true);
   });
 }
 

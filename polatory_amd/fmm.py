@@ -285,3 +285,27 @@ class FmmHessianEvaluator(FmmEvaluator):
 
 def make_fmm_hessian_evaluator(rbf, bbox=None):
     return FmmHessianEvaluator(rbf, bbox)
+
+
+class FmmHessianSymmetricEvaluator(FmmHessianEvaluator):
+    """make_fmm_hessian_symmetric_evaluator (make_fmm_evaluator.cpp:241-270):
+    the Hessian kernel with the points as both sources and targets; weights
+    (n, 3) in, (n, 3) out."""
+
+    _create = "pam_fmm_hessian_symmetric_evaluator_create"
+
+    def set_points(self, points):
+        p = _points(points)
+        self.n_src = p.shape[0]
+        self.n_trg = p.shape[0]
+        _capi.call("pam_fmm_evaluator_set_points", self._h, _ptr(p), p.shape[0])
+
+    def set_source_points(self, points):
+        raise TypeError("use set_points on a symmetric evaluator")
+
+    def set_target_points(self, points):
+        raise TypeError("use set_points on a symmetric evaluator")
+
+
+def make_fmm_hessian_symmetric_evaluator(rbf, bbox=None):
+    return FmmHessianSymmetricEvaluator(rbf, bbox)

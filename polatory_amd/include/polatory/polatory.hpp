@@ -502,13 +502,15 @@ class DeviceEvaluator final : public FmmGenericEvaluatorBase<Dim> {
 template <int Dim>
 class DeviceSymmetricEvaluator final : public FmmGenericSymmetricEvaluatorBase<Dim> {
  public:
-  DeviceSymmetricEvaluator(const rbf::Rbf<Dim>& r, const geometry::Bbox<Dim>& b) {
+  DeviceSymmetricEvaluator(const rbf::Rbf<Dim>& r, const geometry::Bbox<Dim>& b,
+                           CreateFn create = pam_fmm_symmetric_evaluator_create, int kn = 1)
+      : kn_(kn) {
     pam_rbf c = r.to_c();
-    polatory::detail::check(pam_fmm_symmetric_evaluator_create(&c, b.min().data(), b.max().data(), &h_));
+    polatory::detail::check(create(&c, b.min().data(), b.max().data(), &h_));
   }
   ~DeviceSymmetricEvaluator() override { pam_fmm_evaluator_destroy(h_); }
   VecX evaluate() const override {
-    VecX out(n_);
+    VecX out(kn_ * n_);
     polatory::detail::check(pam_fmm_evaluator_evaluate(h_, out.data(), n_));
     return out;
   }
@@ -524,6 +526,7 @@ class DeviceSymmetricEvaluator final : public FmmGenericSymmetricEvaluatorBase<D
  private:
   pam_fmm_evaluator* h_ = nullptr;
   Index n_ = 0;
+  int kn_ = 1;
 };
 
 }  // namespace detail
@@ -558,6 +561,14 @@ FmmGenericEvaluatorPtr<Dim> make_fmm_gradient_transpose_evaluator(const rbf::Rbf
 template <int Dim>
 FmmGenericEvaluatorPtr<Dim> make_fmm_hessian_evaluator(const rbf::Rbf<Dim>& r, const geometry::Bbox<Dim>& bbox) {
   return std::make_unique<detail::DeviceEvaluator<Dim>>(r, bbox, pam_fmm_hessian_evaluator_create, 3);
+}
+
+// fmm_symmetric_evaluator.hpp:85-86: 3 weights and 3 results per point
+template <int Dim>
+FmmGenericSymmetricEvaluatorPtr<Dim> make_fmm_hessian_symmetric_evaluator(const rbf::Rbf<Dim>& r,
+                                                                          const geometry::Bbox<Dim>& bbox) {
+  return std::make_unique<detail::DeviceSymmetricEvaluator<Dim>>(r, bbox, pam_fmm_hessian_symmetric_evaluator_create,
+                                                                 3);
 }
 
 }  // namespace fmm

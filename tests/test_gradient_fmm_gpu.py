@@ -225,3 +225,57 @@ def test_hessian_kernel_matches_source_differences(orc, name):
     errd = np.abs(vd[:300] - ref).max() / np.abs(ref).max()
     log(f"hessian kernel {name}, direct: rel err {errd:.2e}")
     assert errd <= 1e-7  # measured 1.1e-9, 1.4e-9
+
+
+@pytest.mark.parametrize("name", ["bh3", "th3"])
+def test_hessian_symmetric_evaluator(orc, name):
+    """make_fmm_hessian_symmetric_evaluator (make_fmm_evaluator.cpp:241-270):
+    the Hessian kernel with the points as sources and targets. It returns the
+    bits of the source-target Hessian evaluator given the same points twice,
+    and on a subsample matches central differences of the oracle's direct
+    gradient sums over the other points (the zero-distance term of bh3 and th3
+    is zero)."""
+    rng = np.random.default_rng(53)
+    pts = rng.uniform(0, 1, (20_000, 3))
+    w3 = rng.normal(size=(20_000, 3))
+    rbf = KINDS[name]()
+    rbf.set_anisotropy(A)
+    cfg = pa.FmmConfig("fp64", 10, mode="fmm")
+    sym = pa.make_fmm_hessian_symmetric_evaluator(rbf)
+    sym.set_config(cfg)
+    sym.set_points(pts)
+    sym.set_weights(w3)
+    v = sym.evaluate()
+    assert v.shape == (20_000, 3) and sym.stats()["direct"] == 0
+    st = pa.make_fmm_hessian_evaluator(rbf)
+    st.set_config(cfg)
+    st.set_source_points(pts)
+    st.set_weights(w3)
+    st.set_target_points(pts)
+    assert np.array_equal(st.evaluate(), v)
+    with pytest.raises(TypeError):
+        sym.set_target_points(pts)
+
+    orbf = orc_rbf(orc, rbf)
+    idx = rng.choice(20_000, 40, replace=False)
+
+    def diff(h):
+        d = np.zeros((len(idx), 3))
+        for r, i in enumerate(idx):
+            others = np.delete(np.arange(20_000), i)
+            for k in range(3):
+                e = np.zeros(3)
+                e[k] = h
+                d[r] += (orc.direct_gradient_sum(orbf, pts[others] + e, w3[others, k], pts[i:i + 1])
+                         - orc.direct_gradient_sum(orbf, pts[others] - e, w3[others, k], pts[i:i + 1]))[0] / (2 * h)
+        return d
+
+    ref = (4.0 * diff(1e-5) - diff(2e-5)) / 3.0
+    err = np.abs(v[idx] - ref).max() / np.abs(ref).max()
+    log(f"hessian symmetric {name}: rel err {err:.2e} vs source differences")
+    assert err <= 5e-6
+    sym.set_config(pa.FmmConfig("fp64", 10, mode="direct"))
+    vd = sym.evaluate()
+    errd = np.abs(vd[idx] - ref).max() / np.abs(ref).max()
+    log(f"hessian symmetric {name}, direct: rel err {errd:.2e}")
+    assert errd <= 1e-7
