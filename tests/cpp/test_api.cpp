@@ -158,6 +158,45 @@ int main() {
     if (!(ferr <= 1e-6 * std::max(1.0, fscale))) return 12;
   }
 
+  // make_fmm_symmetric_evaluator and make_fmm_hessian_symmetric_evaluator
+  // (fmm_symmetric_evaluator.hpp:80-86): the same bits as the source-target
+  // evaluators given the points twice (Hessians), and values within the
+  // mixed form's error (the value evaluator has its own one-tree path)
+  {
+    const Bbox3 box = Bbox3::from_points(pts);
+    VecX w3(3 * n);
+    for (Index i = 0; i < 3 * n; i++) w3(i) = std::sin(0.37 * static_cast<double>(i));
+    auto sym = polatory::fmm::make_fmm_symmetric_evaluator<3>(Biharmonic3D<3>(), box);
+    auto st = polatory::fmm::make_fmm_evaluator<3>(Biharmonic3D<3>(), box);
+    auto hsym = polatory::fmm::make_fmm_hessian_symmetric_evaluator<3>(Biharmonic3D<3>(), box);
+    auto hst = polatory::fmm::make_fmm_hessian_evaluator<3>(Biharmonic3D<3>(), box);
+    VecX wr(n);
+    for (Index i = 0; i < n; i++) wr(i) = w(i);
+    sym->set_points(pts);
+    sym->set_weights(wr);
+    st->set_source_points(pts);
+    st->set_weights(wr);
+    st->set_target_points(pts);
+    hsym->set_points(pts);
+    hsym->set_weights(w3);
+    hst->set_source_points(pts);
+    hst->set_weights(w3);
+    hst->set_target_points(pts);
+    VecX a = sym->evaluate(), b = st->evaluate(), ha = hsym->evaluate(), hb = hst->evaluate();
+    if (a.rows() != n || ha.rows() != 3 * n || hb.rows() != 3 * n) return 13;
+    double dv = 0.0, dh = 0.0, sv = 0.0, sh = 0.0;
+    for (Index i = 0; i < n; i++) {
+      dv = std::max(dv, std::abs(a(i) - b(i)));
+      sv = std::max(sv, std::abs(b(i)));
+    }
+    for (Index i = 0; i < 3 * n; i++) {
+      dh = std::max(dh, std::abs(ha(i) - hb(i)));
+      sh = std::max(sh, std::abs(hb(i)));
+    }
+    std::printf("symmetric vs source-target: values %.3e, Hessians %.3e (relative)\n", dv / sv, dh / sh);
+    if (!(dh == 0.0) || !(dv <= 1e-5 * sv)) return 14;
+  }
+
   // error contract: invalid arguments throw std::invalid_argument
   try {
     interpolant.fit(pts, values, 0.0);
