@@ -92,6 +92,12 @@ int pam_fmm_hessian_symmetric_evaluator_create(const pam_rbf* rbf, const double 
 int pam_fmm_evaluator_destroy(pam_fmm_evaluator* h);
 /* set_accuracy(double) — absolute L-inf accuracy; +inf = default configuration (fmm_evaluator.hpp:36) */
 int pam_fmm_evaluator_set_accuracy(pam_fmm_evaluator* h, double accuracy);
+/* No counterpart in the reference (its search has one candidate list,
+ * fmm_accuracy_estimator.hpp:105-117): restricts the accuracy search to one
+ * precision (PAM_PRECISION_*), trying orders 4..12 of it, cheapest first, and
+ * undoes a configuration fixed by pam_fmm_evaluator_set_config; -1 restores
+ * the default list (mixed orders 4-8, compensated mixed 8, FP64 8, 10, 12). */
+int pam_fmm_evaluator_set_search_precision(pam_fmm_evaluator* h, int precision);
 /* set_source_points(const Points&) — fmm_evaluator.hpp:38 */
 int pam_fmm_evaluator_set_source_points(pam_fmm_evaluator* h, const double* xyz, int64_t n);
 /* set_target_points(const Points&) — fmm_evaluator.hpp:40 */
@@ -109,8 +115,13 @@ int pam_fmm_evaluator_evaluate(pam_fmm_evaluator* h, double* out, int64_t n);
  * handle's configuration, trees and expansions. Source-target handles only. */
 int pam_fmm_evaluator_evaluate_gradient(pam_fmm_evaluator* h, double* out, int64_t n);
 
-/* Same calls with pointers into device memory (HBM); asynchronous on the
- * handle's stream, ordered with the handle's other calls. */
+/* Same calls with pointers into device memory (HBM), enqueued on the
+ * handle's stream and ordered with the handle's other calls. The setters
+ * return without waiting for the device, so the caller keeps its buffers
+ * unchanged until the evaluation that reads them is complete (after
+ * pam_fmm_evaluator_synchronize). The evaluations wait for the device once,
+ * when the trees are rebuilt (their occupied-cell counts choose the leaf level
+ * and size the lists), and return with the output still being written. */
 int pam_fmm_evaluator_set_source_points_device(pam_fmm_evaluator* h, const double* d_xyz, int64_t n);
 int pam_fmm_evaluator_set_target_points_device(pam_fmm_evaluator* h, const double* d_xyz, int64_t n);
 int pam_fmm_evaluator_set_points_device(pam_fmm_evaluator* h, const double* d_xyz, int64_t n);
@@ -239,7 +250,11 @@ int pam_interpolant_create(const pam_model* model, pam_interpolant** out);
 int pam_interpolant_destroy(pam_interpolant* h);
 /* fit(points, values, tolerance, max_iter = 100, accuracy = inf, initial = nullptr)
  * — interpolant.hpp:77-113. Absolute L-inf tolerance, recomputed in FP64.
- * opt and initial may be NULL. */
+ * opt and initial may be NULL. accuracy must be positive, as in the reference,
+ * where it sets the accuracy of the residual evaluation (solver.hpp:34-41);
+ * here that evaluation is always FP64 (opt->residual, by default an order-10
+ * FP64 FMM or the direct sum), at least as accurate as any accuracy asks, so
+ * the value does not change the fit. */
 int pam_interpolant_fit(pam_interpolant* h, const double* points, int64_t n, const double* values,
                         double tolerance, int max_iter, double accuracy, const pam_fit_options* opt,
                         const pam_interpolant* initial);
@@ -344,6 +359,32 @@ int pam_ras_num_domains(const pam_ras* h, int level, int* n);
 int pam_ras_domain(const pam_ras* h, int level, int dom, int32_t* idx, uint8_t* inner, int64_t* n);
 /* operator()(v) — ras_preconditioner.hpp:191-250 (host vectors of length n + l) */
 int pam_ras_apply(pam_ras* h, const double* v, double* out);
+/* The same as a device-resident krylov::LinearOperator (linear_operator.hpp:7-22):
+ * d_v and d_out are device vectors of pam_ras_size entries; the work is
+ * enqueued on the handle's stream and complete after pam_ras_synchronize. */
+int pam_ras_apply_device(pam_ras* h, const double* d_v, double* d_out);
+/* LinearOperator::size() — n + l */
+int pam_ras_size(const pam_ras* h, int64_t* size);
+int pam_ras_synchronize(pam_ras* h);
+
+/* interpolation::Operator(model, points) — operator.hpp:38-111, the fit's
+ * LinearOperator: y = [(A + nugget I) w + P c ; P^T w] for x = [w ; c] at the
+ * points (n x 3, host, copied), with A evaluated by the symmetric FMM of cfg
+ * (NULL: the fit's default inner operator, mixed precision, FP64 direct sum
+ * below 2^26 pairs). operator.hpp builds it at accuracy 0 inside Solver
+ * (solver.hpp:40); pass an FP64 configuration for that form. */
+typedef struct pam_operator pam_operator;
+int pam_operator_create(const pam_model* model, const double* points, int64_t n, const pam_fmm_config* cfg,
+                        pam_operator** out);
+int pam_operator_destroy(pam_operator* h);
+/* LinearOperator::size() — n + l */
+int pam_operator_size(const pam_operator* h, int64_t* size);
+/* operator()(x) with device vectors of pam_operator_size entries, enqueued on
+ * the handle's stream (complete after pam_operator_synchronize) */
+int pam_operator_apply_device(pam_operator* h, const double* d_x, double* d_y);
+/* operator()(x) with host vectors (waits for the result) */
+int pam_operator_apply(pam_operator* h, const double* x, double* y);
+int pam_operator_synchronize(pam_operator* h);
 
 /* ---------------------------------------------------------------------------
  * Serving: fitted models resident in HBM, requests admitted by HBM budget and

@@ -22,6 +22,14 @@ class OrcRbf(ctypes.Structure):
                 ("aniso", ctypes.c_double * 9)]
 
 
+class OrcFmmStats(ctypes.Structure):
+    _fields_ = [("t_tree", ctypes.c_double), ("t_upward", ctypes.c_double), ("t_m2l", ctypes.c_double),
+                ("t_downward", ctypes.c_double), ("t_p2p", ctypes.c_double), ("t_total", ctypes.c_double),
+                ("height", ctypes.c_int), ("order", ctypes.c_int), ("m2l_pairs", ctypes.c_int64),
+                ("p2p_pairs", ctypes.c_int64), ("n_src_leaves", ctypes.c_int64),
+                ("n_trg_leaves", ctypes.c_int64)]
+
+
 def build():
     import subprocess
     subprocess.check_call(["make", "-s", "-C", _HERE])
@@ -63,6 +71,9 @@ def lib():
         L.orc_m2l_lists.argtypes = [P, i64, P, i64, ctypes.c_int, P]
         L.orc_fgmres_dense.restype = ctypes.c_int
         L.orc_fgmres_dense.argtypes = [P, P, i64, P, P, ctypes.c_int, P, P]
+        L.orc_cpu_fmm_evaluate.restype = ctypes.c_int
+        L.orc_cpu_fmm_evaluate.argtypes = [P, P, P, P, i64, P, P, i64, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, P, ctypes.POINTER(OrcFmmStats)]
         _lib = L
     return _lib
 
@@ -233,3 +244,20 @@ def fgmres_dense(a, rhs, n_iter, m=None, x0=None):
     hist = np.zeros(n_iter + 1)
     it = lib().orc_fgmres_dense(_p(a), _p(mm), n, _p(rhs), _p(x0), n_iter, _p(x), _p(hist))
     return x, hist[:it + 1], it
+
+
+def cpu_fmm_evaluate(rbf, src, w, trg, bbox, order, fh_degree=-1, height=0):
+    """The reference-algorithm CPU FMM (cpu_fmm.c): polatory's evaluate path
+    over a uniform-interpolation FMM, FP64, OpenMP. Returns (values, stats)."""
+    src = _f64(src, (-1, 3))
+    trg = _f64(trg, (-1, 3))
+    w = _f64(w)
+    lo = _f64(bbox[0], (3,))
+    hi = _f64(bbox[1], (3,))
+    out = np.empty(trg.shape[0])
+    st = OrcFmmStats()
+    rc = lib().orc_cpu_fmm_evaluate(ctypes.byref(rbf), _p(lo), _p(hi), _p(src), src.shape[0], _p(w), _p(trg),
+                                    trg.shape[0], order, fh_degree, height, _p(out), ctypes.byref(st))
+    if rc != 0:
+        raise ValueError(f"cpu_fmm_evaluate: order or height out of range ({rc})")
+    return out, {k: getattr(st, k) for k, _ in st._fields_}

@@ -98,6 +98,21 @@ void orc_m2l_lists(const uint64_t* trg_codes, int64_t n_trg, const uint64_t* src
 int orc_fgmres_dense(const double* a, const double* m, int64_t n, const double* rhs,
                      const double* x0, int n_iter, double* x, double* resid_hist);
 
+/* Reference-algorithm CPU FMM (cpu_fmm.c): polatory's evaluate path over a
+ * uniform-interpolation FMM with DFT-based M2L, FP64, OpenMP. order = nodes
+ * per axis; fh_degree < 0: classic Lagrange, else Floater-Hormann degree;
+ * height <= 0: the reference's height formula (and the direct sum below
+ * 1024^2 pairs). Returns 0, or < 0 for an order or height out of range. */
+typedef struct {
+  double t_tree, t_upward, t_m2l, t_downward, t_p2p, t_total;
+  int height, order;
+  int64_t m2l_pairs, p2p_pairs, n_src_leaves, n_trg_leaves;
+} orc_fmm_stats;
+int orc_cpu_fmm_evaluate(const orc_rbf* rbf, const double* bbox_min, const double* bbox_max,
+                         const double* src, int64_t n_src, const double* w, const double* trg,
+                         int64_t n_trg, int order, int fh_degree, int height, double* out,
+                         orc_fmm_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

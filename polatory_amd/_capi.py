@@ -96,6 +96,7 @@ _SIGS = {
     "pam_fmm_symmetric_evaluator_create": [ctypes.POINTER(PamRbf), _P, _P, ctypes.POINTER(_P)],
     "pam_fmm_evaluator_destroy": [_P],
     "pam_fmm_evaluator_set_accuracy": [_P, _D],
+    "pam_fmm_evaluator_set_search_precision": [_P, _INT],
     "pam_fmm_evaluator_set_source_points": [_P, _P, _I64],
     "pam_fmm_evaluator_set_target_points": [_P, _P, _I64],
     "pam_fmm_evaluator_set_points": [_P, _P, _I64],
@@ -172,6 +173,15 @@ _SIGS = {
     "pam_ras_num_domains": [_P, _INT, ctypes.POINTER(_INT)],
     "pam_ras_domain": [_P, _INT, _INT, _P, _P, ctypes.POINTER(_I64)],
     "pam_ras_apply": [_P, _P, _P],
+    "pam_ras_apply_device": [_P, _P, _P],
+    "pam_ras_size": [_P, ctypes.POINTER(_I64)],
+    "pam_ras_synchronize": [_P],
+    "pam_operator_create": [_P, _P, _I64, ctypes.POINTER(PamFmmConfig), ctypes.POINTER(_P)],
+    "pam_operator_destroy": [_P],
+    "pam_operator_size": [_P, ctypes.POINTER(_I64)],
+    "pam_operator_apply_device": [_P, _P, _P],
+    "pam_operator_apply": [_P, _P, _P],
+    "pam_operator_synchronize": [_P],
     "pam_server_options_default": [ctypes.POINTER(PamServerOptions)],
     "pam_server_create": [ctypes.POINTER(PamServerOptions), ctypes.POINTER(_P)],
     "pam_server_destroy": [_P],

@@ -145,6 +145,10 @@ int pam_fmm_evaluator_set_accuracy(pam_fmm_evaluator* h, double accuracy) {
   return guard([&] { ev(h).set_accuracy(accuracy); });
 }
 
+int pam_fmm_evaluator_set_search_precision(pam_fmm_evaluator* h, int precision) {
+  return guard([&] { ev(h).set_search_precision(precision); });
+}
+
 int pam_fmm_evaluator_set_source_points(pam_fmm_evaluator* h, const double* xyz, int64_t n) {
   return guard([&] {
     if (h && h->symmetric) pam::throw_runtime("use set_points on a symmetric evaluator");

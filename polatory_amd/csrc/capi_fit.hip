@@ -10,6 +10,21 @@
 #include "model_io.hpp"
 #include "ras.hpp"
 
+struct pam_operator {
+  hipStream_t s = nullptr;
+  pam::Model model;
+  pam::DevBuf<double> pts;
+  std::unique_ptr<pam::Operator> impl;
+  pam::DevBuf<double> x, y;  // host-pointer applies
+  ~pam_operator() {
+    impl.reset();
+    if (s) {
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
+    }
+  }
+};
+
 struct pam_ras {
   hipStream_t s = nullptr;
   pam::Model model;
@@ -205,6 +220,7 @@ int pam_interpolant_fit(pam_interpolant* h, const double* points, int64_t n, con
                         const pam_interpolant* initial) {
   return guard([&] {
     if (n < 0) pam::throw_invalid("negative point count");
+    if (n > 0 && (!points || !values)) pam::throw_invalid("null argument");
     std::vector<double> p(points, points + 3 * n), v(values, values + n);
     ip(h).fit(p, v, tolerance, max_iter, accuracy, to_opts(opt), initial ? &ip(initial) : nullptr);
   });
@@ -474,6 +490,84 @@ int pam_ras_domain(const pam_ras* h, int level, int dom, int32_t* idx, uint8_t* 
     *n = static_cast<int64_t>(d.idx.size());
     if (idx) std::copy(d.idx.begin(), d.idx.end(), idx);
     if (inner) std::copy(d.inner.begin(), d.inner.end(), inner);
+  });
+}
+
+int pam_ras_apply_device(pam_ras* h, const double* d_v, double* d_out) {
+  return guard([&] {
+    if (!h || !h->impl) pam::throw_invalid("null preconditioner handle");
+    if (h->impl->size() > 0 && (!d_v || !d_out)) pam::throw_invalid("null argument");
+    h->impl->apply(d_v, d_out);
+  });
+}
+
+int pam_ras_size(const pam_ras* h, int64_t* size) {
+  return guard([&] {
+    if (!h || !h->impl || !size) pam::throw_invalid("null argument");
+    *size = h->impl->size();
+  });
+}
+
+int pam_ras_synchronize(pam_ras* h) {
+  return guard([&] {
+    if (!h) pam::throw_invalid("null preconditioner handle");
+    PAM_HIP(hipStreamSynchronize(h->s));
+  });
+}
+
+int pam_operator_create(const pam_model* model, const double* points, int64_t n, const pam_fmm_config* cfg,
+                        pam_operator** out) {
+  return guard([&] {
+    if (!model || !out) pam::throw_invalid("null argument");
+    if (n < 0) pam::throw_invalid("negative point count");
+    if (n > 0 && !points) pam::throw_invalid("null argument");
+    auto h = std::make_unique<pam_operator>();
+    PAM_HIP(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
+    h->model = model->m;
+    h->pts.upload(points, static_cast<size_t>(3 * n), h->s);
+    pam::FmmConfig c = cfg ? to_cfg(*cfg) : default_fit_options().op_cfg;
+    h->impl = std::make_unique<pam::Operator>(h->model, h->pts.get(), n, c, h->s);
+    PAM_HIP(hipStreamSynchronize(h->s));
+    *out = h.release();
+  });
+}
+
+int pam_operator_destroy(pam_operator* h) {
+  return guard([&] { delete h; });
+}
+
+int pam_operator_size(const pam_operator* h, int64_t* size) {
+  return guard([&] {
+    if (!h || !h->impl || !size) pam::throw_invalid("null argument");
+    *size = h->impl->size();
+  });
+}
+
+int pam_operator_apply_device(pam_operator* h, const double* d_x, double* d_y) {
+  return guard([&] {
+    if (!h || !h->impl) pam::throw_invalid("null operator handle");
+    if (h->impl->size() > 0 && (!d_x || !d_y)) pam::throw_invalid("null argument");
+    h->impl->apply(d_x, d_y);
+  });
+}
+
+int pam_operator_apply(pam_operator* h, const double* x, double* y) {
+  return guard([&] {
+    if (!h || !h->impl) pam::throw_invalid("null operator handle");
+    const int64_t m = h->impl->size();
+    if (m > 0 && (!x || !y)) pam::throw_invalid("null argument");
+    h->x.upload(x, static_cast<size_t>(m), h->s);
+    h->y.resize(static_cast<size_t>(m));
+    h->impl->apply(h->x.get(), h->y.get());
+    h->y.download(y, static_cast<size_t>(m), h->s);
+    PAM_HIP(hipStreamSynchronize(h->s));
+  });
+}
+
+int pam_operator_synchronize(pam_operator* h) {
+  return guard([&] {
+    if (!h) pam::throw_invalid("null operator handle");
+    PAM_HIP(hipStreamSynchronize(h->s));
   });
 }
 

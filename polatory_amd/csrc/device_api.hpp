@@ -31,6 +31,15 @@ void launch_gather_f64(const double* in, const int32_t* idx, int64_t n, double* 
 void direct_sum_f64(const RbfDesc& rbf, const double* src, const double* w, int64_t n_src,
                     const double* trg, int64_t n_trg, double* out, bool accumulate,
                     DevBuf<double>& scratch, hipStream_t s);
+// Doubles of partial sums the FP64 direct sum uses for n_trg targets and n_src
+// sources (its split of the sources into chunks).
+size_t direct_partial_size(int64_t n_trg, int64_t n_src);
+// direct_sum_f64 with caller-provided partial sums (direct_partial_size
+// doubles, device memory): allocates nothing, same kernels and split, so the
+// same bits.
+void direct_sum_f64_into(const RbfDesc& rbf, const double* src, const double* w, int64_t n_src,
+                         const double* trg, int64_t n_trg, double* out, bool accumulate, double* partial,
+                         hipStream_t s);
 // FP64 direct gradient sum (gradient-transpose kernel): out3[3 i + k] =
 // (A^T sum_j w[j] grad phi(t_i - s_j))_k over transformed points (rbf_eval.hip).
 void direct_gradient_f64(const RbfDesc& rbf, const double* src, const double* w, int64_t n_src,

@@ -2,6 +2,8 @@
 // upward/downward passes and the accuracy-driven configuration search.
 #include "fmm_evaluator.hpp"
 
+#include <array>
+
 #include <algorithm>
 #include <cmath>
 #include <limits>
@@ -203,6 +205,14 @@ void FmmEvaluator::set_accuracy(double accuracy) {
   if (!fixed_config_) config_valid_ = false;
 }
 
+void FmmEvaluator::set_search_precision(int precision) {
+  if (precision != -1 && precision != kFp64 && precision != kMixed && precision != kMixedComp)
+    throw_invalid("unknown precision");
+  search_precision_ = precision;
+  fixed_config_ = false;
+  config_valid_ = false;
+}
+
 void FmmEvaluator::set_config(const FmmConfig& c) {
   if (c.precision != kFp64 && c.precision != kMixed && c.precision != kMixedComp) throw_invalid("unknown precision");
   if (!order_supported(c.order)) throw_invalid("unsupported interpolation order");
@@ -239,7 +249,8 @@ void FmmEvaluator::set_source_points(const double* xyz, int64_t n, bool on_devic
   have_weights_ = false;
   invalidate_expansions();
   if (!fixed_config_) config_valid_ = false;
-  PAM_HIP(hipStreamSynchronize(stream_));
+  // A host input is staged through pinned memory that the next call reuses.
+  if (!on_device) PAM_HIP(hipStreamSynchronize(stream_));
 }
 
 void FmmEvaluator::set_target_points(const double* xyz, int64_t n, bool on_device) {
@@ -257,7 +268,7 @@ void FmmEvaluator::set_target_points(const double* xyz, int64_t n, bool on_devic
   trg_tree_dirty_ = true;
   box_valid_ = false;
   st_.t64_valid = st_.t32_valid = false;
-  PAM_HIP(hipStreamSynchronize(stream_));
+  if (!on_device) PAM_HIP(hipStreamSynchronize(stream_));
 }
 
 void FmmEvaluator::set_weights(const double* w, int64_t n, bool on_device) {
@@ -816,8 +827,13 @@ void FmmEvaluator::choose_config() {
       dc.mode = kModeDirect;
       probe.set_config(dc);
       probe.evaluate(exact.data(), m, false);
-      const int cand[][2] = {{kMixed, 4}, {kMixed, 5}, {kMixed, 6}, {kMixed, 7}, {kMixed, 8}, {kMixedComp, 8},
-                             {kFp64, 8}, {kFp64, 10}, {kFp64, 12}};
+      std::vector<std::array<int, 2>> cand;
+      if (search_precision_ < 0) {
+        cand = {{kMixed, 4}, {kMixed, 5}, {kMixed, 6}, {kMixed, 7}, {kMixed, 8}, {kMixedComp, 8},
+                {kFp64, 8}, {kFp64, 10}, {kFp64, 12}};
+      } else {
+        for (int o = 4; o <= 12; o++) cand.push_back({search_precision_, o});
+      }
       bool found = false;
       for (auto& cd : cand) {
         FmmConfig t;

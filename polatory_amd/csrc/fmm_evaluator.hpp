@@ -75,6 +75,12 @@ class FmmEvaluator {
   FmmEvaluator& operator=(const FmmEvaluator&) = delete;
 
   void set_accuracy(double accuracy);
+
+  // Restricts the accuracy search to one precision (orders 4..12 of it); -1
+
+  // restores the default list (mixed orders first, then FP64).
+
+  void set_search_precision(int precision);
   void set_config(const FmmConfig& c);  // fixes the configuration (no accuracy search)
   // Near field on a second stream beside the far field (default), or all
   // stages one after another on the evaluator's stream.
@@ -152,6 +158,7 @@ class FmmEvaluator {
   bool own_stream_ = true;
 
   double accuracy_;
+  int search_precision_ = -1;  // accuracy search over one precision's orders; -1: the mixed list
   bool fixed_config_ = false;
   bool config_valid_ = false;
   FmmConfig cfg_;

@@ -233,8 +233,12 @@ void Interpolant::evaluate(const double* points, int64_t n, double accuracy, dou
   throw_if_not_fitted();
   if (!(accuracy > 0.0)) throw_invalid("accuracy must be positive");
   if (accuracy != ev_acc_) {
+    // A box installed by set_evaluation_bbox belongs to its accuracy; at
+    // another accuracy the evaluator is rebuilt for the centres' box, as the
+    // reference rebuilds it per call (interpolant.hpp:58-67).
     ev_.reset();
     ev_acc_ = accuracy;
+    have_eval_bbox_ = false;
   }
   Evaluator& ev = evaluator();
   const double* d_in = points;
@@ -263,6 +267,7 @@ void Interpolant::evaluate_gradient(const double* points, int64_t n, double accu
   if (accuracy != gev_acc_) {
     gev_.reset();
     gev_acc_ = accuracy;
+    have_eval_bbox_ = false;
   }
   Evaluator& ev = gradient_evaluator();
   const double* d_in = points;

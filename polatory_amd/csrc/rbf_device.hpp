@@ -53,6 +53,23 @@ __device__ __forceinline__ float phi_f32(float r2, float a, float b) {
   }
 }
 
+// Square root for the FP64 near field: the hardware reciprocal square root
+// (v_rsq_f64, accurate to about 2^-23 relative) refined by one Goldschmidt
+// step and a final Newton correction of the remainder (Markstein's method),
+// within a few ulp of the correctly rounded sqrt in 8 instructions instead of
+// the compiler's IEEE-rounded expansion. 0 for x = 0 (coincident points); x is
+// a sum of squares, never negative.
+__device__ __forceinline__ double sqrt_nr(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double s = x * y, h = 0.5 * y;
+  const double r = fma(-s, h, 0.5);
+  s = fma(s, r, s);
+  h = fma(h, r, h);
+  const double d = fma(-s, s, x);
+  s = fma(d, h, s);
+  return x > 0.0 ? s : 0.0;
+}
+
 // phi without its factor a, for loops that fold a into the weights:
 // r2c = r^2 + b for bh3 / th3 (the caller seeds its sum of squares with b),
 // r2c = r^2 for exp.
@@ -60,12 +77,12 @@ template <class T, int KIND>
 __device__ __forceinline__ T phi_shape(T r2c, T b) {
   if constexpr (sizeof(T) == 8) {
     if constexpr (KIND == kBh3) {
-      return sqrt(r2c);
+      return sqrt_nr(r2c);
     } else if constexpr (KIND == kTh3) {
-      double rho = sqrt(r2c);
+      double rho = sqrt_nr(r2c);
       return rho * rho * rho;
     } else {
-      return exp(b * sqrt(r2c));
+      return exp(b * sqrt_nr(r2c));
     }
   } else {
     if constexpr (KIND == kBh3) {

@@ -311,6 +311,11 @@ __global__ void reduce_partial_kernel(const double* __restrict__ partial, int nc
   out[i] = accumulate ? out[i] + acc : acc;
 }
 
+__global__ void fill_f64_kernel(double* __restrict__ out, int64_t n, double v) {
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = v;
+}
+
 // ---- batched FP64 direct sums: several query sets, each against its own
 // models' sources, in one launch and one reduction ----
 //
@@ -526,28 +531,44 @@ void direct_split(int64_t n_trg, int64_t n_src, int* gx, int64_t* chunk, int* nc
 
 }  // namespace
 
-void direct_sum_f64(const RbfDesc& rbf, const double* src, const double* w, int64_t n_src,
-                    const double* trg, int64_t n_trg, double* out, bool accumulate,
-                    DevBuf<double>& scratch, hipStream_t s) {
+size_t direct_partial_size(int64_t n_trg, int64_t n_src) {
+  if (n_trg == 0) return 0;
+  int gx, nchunks;
+  int64_t chunk;
+  direct_split(n_trg, n_src, &gx, &chunk, &nchunks);
+  return static_cast<size_t>(nchunks) * n_trg;
+}
+
+void direct_sum_f64_into(const RbfDesc& rbf, const double* src, const double* w, int64_t n_src,
+                         const double* trg, int64_t n_trg, double* out, bool accumulate, double* partial,
+                         hipStream_t s) {
   if (n_trg == 0) return;
   PhiConsts pc = phi_consts(rbf, 1.0);
   int gx, nchunks;
   int64_t chunk;
   direct_split(n_trg, n_src, &gx, &chunk, &nchunks);
-  scratch.resize(static_cast<size_t>(nchunks) * n_trg);
   if (n_src == 0) {
-    PAM_HIP(hipMemsetAsync(scratch.get(), 0, sizeof(double) * n_trg, s));
+    fill_f64_kernel<<<ceil_div(n_trg, 256), 256, 0, s>>>(partial, n_trg, 0.0);
+    PAM_LAUNCH_CHECK();
   } else {
     dim3 grid(gx, nchunks);
     dispatch_kind(rbf.kind, [&](auto k) {
       direct_f64_kernel<decltype(k)::value><<<grid, kDirThreads, 0, s>>>(pc.a, pc.b, src, w, n_src, trg,
-                                                                        n_trg, scratch.get(), chunk);
+                                                                        n_trg, partial, chunk);
     });
     PAM_LAUNCH_CHECK();
   }
-  reduce_partial_kernel<<<ceil_div(n_trg, 256), 256, 0, s>>>(scratch.get(), nchunks, n_trg, out,
+  reduce_partial_kernel<<<ceil_div(n_trg, 256), 256, 0, s>>>(partial, nchunks, n_trg, out,
                                                              accumulate ? 1 : 0);
   PAM_LAUNCH_CHECK();
+}
+
+void direct_sum_f64(const RbfDesc& rbf, const double* src, const double* w, int64_t n_src,
+                    const double* trg, int64_t n_trg, double* out, bool accumulate,
+                    DevBuf<double>& scratch, hipStream_t s) {
+  if (n_trg == 0) return;
+  scratch.resize(direct_partial_size(n_trg, n_src));
+  direct_sum_f64_into(rbf, src, w, n_src, trg, n_trg, out, accumulate, scratch.get(), s);
 }
 
 void launch_poly_grad(int degree, const double* pts, int64_t n, const double* d_coeffs, double* out3,

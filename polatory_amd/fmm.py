@@ -88,6 +88,12 @@ class FmmEvaluator:
     def set_accuracy(self, accuracy):
         _capi.call("pam_fmm_evaluator_set_accuracy", self._h, float(accuracy))
 
+    def set_search_precision(self, precision):
+        """Restrict the accuracy search to one precision ("fp64", "mixed",
+        "mixed_comp"; None for the default list)."""
+        c = -1 if precision is None else {"fp64": 0, "mixed": 1, "mixed_comp": 2}[precision]
+        _capi.call("pam_fmm_evaluator_set_search_precision", self._h, c)
+
     def set_config(self, config: FmmConfig):
         c = config.to_c()
         _capi.call("pam_fmm_evaluator_set_config", self._h, ctypes.byref(c))

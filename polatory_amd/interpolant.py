@@ -336,6 +336,62 @@ class RasPreconditioner:
         _capi.call("pam_ras_apply", self._h, _ptr(v), _ptr(out))
         return out
 
+    def size(self):
+        n = ctypes.c_int64()
+        _capi.call("pam_ras_size", self._h, ctypes.byref(n))
+        return n.value
+
+    def apply_device(self, d_v, d_out):
+        """operator()(v) on device vectors (addresses as ints), enqueued on the
+        handle's stream; complete after synchronize()."""
+        _capi.call("pam_ras_apply_device", self._h, ctypes.c_void_p(d_v), ctypes.c_void_p(d_out))
+
+    def synchronize(self):
+        _capi.call("pam_ras_synchronize", self._h)
+
+
+class Operator:
+    """interpolation::Operator (operator.hpp:38-111) as a LinearOperator:
+    y = [(A + nugget I) w + P c ; P^T w] for x = [w ; c] at the points."""
+
+    def __init__(self, model: Model, points, config: FmmConfig = None):
+        self.model = model
+        p = _points(points)
+        h = ctypes.c_void_p()
+        c = None if config is None else ctypes.byref(config.to_c())
+        _capi.call("pam_operator_create", model._h, _ptr(p), p.shape[0], c, ctypes.byref(h))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _capi.load().pam_operator_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def size(self):
+        n = ctypes.c_int64()
+        _capi.call("pam_operator_size", self._h, ctypes.byref(n))
+        return n.value
+
+    def __call__(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        if x.shape[0] != self.size():
+            raise ValueError("x must have size() entries")
+        y = np.empty_like(x)
+        _capi.call("pam_operator_apply", self._h, _ptr(x), _ptr(y))
+        return y
+
+    def apply_device(self, d_x, d_y):
+        """operator()(x) on device vectors (addresses as ints), enqueued on the
+        handle's stream; complete after synchronize()."""
+        _capi.call("pam_operator_apply_device", self._h, ctypes.c_void_p(d_x), ctypes.c_void_p(d_y))
+
+    def synchronize(self):
+        _capi.call("pam_operator_synchronize", self._h)
+
 
 def fgmres_dense(a, rhs, n_iter, m=None, x0=None):
     """Device FGMRES on a dense operator; returns (x, |g_j| history, iterations)."""

@@ -58,3 +58,31 @@ def test_evaluate_impl_reuses_the_last_evaluator():
     ip.fit(pts, -vals, 1e-6, accuracy=1e-8)
     b = ip.evaluate(q, accuracy=1e-6)
     assert np.abs(b + a).max() < 1e-4
+
+
+def test_rbf_field_function_through_the_fmm(orc):
+    """The same field with the evaluator forced onto the FMM (FP64, order 10)
+    over the box enlarged past the centres: within 1e-8 of the oracle, and not
+    the direct sum's bits (the FMM path ran)."""
+    rng = np.random.default_rng(52)
+    pts = rng.uniform(0, 1, (3000, 3))
+    vals = np.sin(3 * pts[:, 0]) * np.cos(2 * pts[:, 1]) + 0.5 * pts[:, 2] ** 2
+    ip = pa.Interpolant(pa.Model(pa.Biharmonic3D(), poly_degree=1))
+    ip.fit(pts, vals, 1e-6)
+    lo, hi = np.full(3, -0.3), np.full(3, 1.3)
+    q = lattice(lo, hi, 30)
+    orbf = [orc.make_rbf(orc.BH3, [1.0], np.eye(3))]
+    ref = orc.direct_evaluate(orbf, 1, ip.centers(), ip.weights(), q)
+    ip.set_eval_config(pa.FmmConfig("fp64", 10, -1, "direct"))
+    field = pa.RbfFieldFunction(ip)
+    field.set_evaluation_bbox(lo, hi)
+    direct = field(q)
+    ip.set_eval_config(pa.FmmConfig("fp64", 10, -1, "fmm"))
+    field = pa.RbfFieldFunction(ip)
+    field.set_evaluation_bbox(lo, hi)
+    f = field(q)
+    err = np.abs(f - ref).max() / np.abs(ref).max()
+    print(f"field through the FMM on a 30^3 lattice: rel error {err:.3e}", flush=True)
+    assert err <= 1e-8
+    assert np.abs(direct - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert not np.array_equal(f, direct)

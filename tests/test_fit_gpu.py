@@ -277,3 +277,109 @@ def test_solve_tuning_gives_identical_weights():
         ip.fit(pts, vals, 1e-5, options=pa.FitOptions(tune_solves=tune))
         ws.append(ip.weights())
     assert np.array_equal(ws[0], ws[1])
+
+
+def ellipsoid_points(n_surf, seed):
+    """Area-uniform points on the ellipsoid with semi-axes (1, 0.7, 0.5) with
+    value 0, plus the same points offset by +-0.01 along the analytic normal
+    with values +-0.01 (the surface-reconstruction workload, scaled down)."""
+    a = np.array([1.0, 0.7, 0.5])
+    rng = np.random.default_rng(seed)
+    gmax = max(a[1] * a[2], a[0] * a[2], a[0] * a[1])
+    parts, got = [], 0
+    while got < n_surf:
+        u = rng.normal(size=(2 * n_surf, 3))
+        u /= np.linalg.norm(u, axis=1)[:, None]
+        g = np.sqrt((a[1] * a[2] * u[:, 0]) ** 2 + (a[0] * a[2] * u[:, 1]) ** 2 + (a[0] * a[1] * u[:, 2]) ** 2)
+        keep = u[rng.uniform(size=len(u)) * gmax < g] * a
+        parts.append(keep)
+        got += len(keep)
+    p = np.concatenate(parts)[:n_surf]
+    nrm = p / a ** 2
+    nrm /= np.linalg.norm(nrm, axis=1)[:, None]
+    pts = np.concatenate([p, p + 0.01 * nrm, p - 0.01 * nrm])
+    vals = np.concatenate([np.zeros(n_surf), np.full(n_surf, 0.01), np.full(n_surf, -0.01)])
+    return pts, vals
+
+
+def test_ellipsoid_surface_fit_reaches_tolerance(orc):
+    """Surface reconstruction at 30,000 points (10,000 on the ellipsoid and
+    their +-0.01 offsets): bh3, degree 1, tolerance 1e-5, the default
+    mixed-precision operator and a multilevel RAS; the final residual is
+    recomputed by the oracle in FP64 (test/interpolation/test_fitter.cpp:51-63)."""
+    log("ellipsoid fit")
+    pts, vals = ellipsoid_points(10_000, 8)
+    model = pa.Model(pa.Biharmonic3D(), 1)
+    ip = pa.Interpolant(model)
+    ip.fit(pts, vals, 1e-5, max_iter=100)
+    st = ip.fit_stats()
+    res = oracle_residual(orc, model, pts, vals, ip.weights())
+    log(f"  iterations {st['iterations']} restarts {st['restarts']} levels {st['n_levels']} "
+        f"residual(device) {st['residual']:.2e} residual(oracle FP64) {res:.2e}")
+    assert st["n_levels"] >= 2
+    assert res <= 1e-5
+
+
+def test_operator_and_preconditioner_as_device_linear_operators(orc):
+    """interpolation::Operator and RasPreconditioner through their
+    LinearOperator entry points (krylov/linear_operator.hpp:7-22): the operator
+    matches the oracle's FP64 direct operator (direct_operator.hpp:42-76) at the
+    accuracy of its configuration, the device-pointer applies return the bits
+    of the host-pointer applies, and a right-preconditioned FGMRES built from
+    the two device applies (numpy driver) solves the saddle-point system."""
+    import torch
+    rng = np.random.default_rng(27)
+    n = 12000
+    pts = rng.uniform(0, 1, (n, 3))
+    model = pa.Model(pa.Biharmonic3D(), 1, nugget=0.001)
+    x = np.concatenate([rng.normal(size=n), rng.normal(size=4)])
+    ref = orc.direct_operator(orc_rbfs(orc, model), 1, 0.001, pts, x)
+    for cfg, tol in ((pa.FmmConfig("fp64", 10, -1, "direct"), 1e-13), (pa.FmmConfig("fp64", 10, -1, "fmm"), 1e-8),
+                     (None, 1e-5)):
+        op = pa.Operator(model, pts, cfg)
+        assert op.size() == n + 4
+        y = op(x)
+        err = np.abs(y - ref).max() / np.abs(ref).max()
+        log(f"  operator {cfg}: rel err {err:.2e}")
+        assert err <= tol
+    dev = torch.device("cuda:0")
+    dx = torch.from_numpy(x).to(dev)
+    dy = torch.empty_like(dx)
+    torch.cuda.synchronize()
+    op.apply_device(dx.data_ptr(), dy.data_ptr())
+    op.synchronize()
+    np.testing.assert_array_equal(dy.cpu().numpy(), op(x))
+    pc = pa.RasPreconditioner(model, pts)
+    assert pc.size() == n + 4
+    dz = torch.empty_like(dx)
+    pc.apply_device(dx.data_ptr(), dz.data_ptr())
+    pc.synchronize()
+    np.testing.assert_array_equal(dz.cpu().numpy(), pc.apply(x))
+    # FGMRES(30) with right preconditioning, both applies on device vectors
+    b = np.concatenate([sample_values(pts), np.zeros(4)])
+    beta = np.linalg.norm(b)
+    V = [b / beta]
+    Z, H = [], np.zeros((31, 30))
+    for j in range(30):
+        dv = torch.from_numpy(V[j]).to(dev)
+        pc.apply_device(dv.data_ptr(), dz.data_ptr())
+        pc.synchronize()
+        op.apply_device(dz.data_ptr(), dy.data_ptr())
+        op.synchronize()
+        Z.append(dz.cpu().numpy())
+        w = dy.cpu().numpy()
+        for i in range(j + 1):
+            H[i, j] = V[i] @ w
+            w = w - H[i, j] * V[i]
+        H[j + 1, j] = np.linalg.norm(w)
+        V.append(w / H[j + 1, j])
+        e1 = np.zeros(j + 2)
+        e1[0] = beta
+        yk = np.linalg.lstsq(H[:j + 2, :j + 1], e1, rcond=None)[0]
+        res = np.linalg.norm(H[:j + 2, :j + 1] @ yk - e1)
+        if res < 1e-9 * beta:
+            break
+    sol = np.array(Z).T @ yk
+    true = orc.direct_operator(orc_rbfs(orc, model), 1, 0.001, pts, sol)
+    log(f"  FGMRES: {j + 1} steps, true relative residual {np.linalg.norm(true - b) / beta:.2e}")
+    assert np.linalg.norm(true - b) <= 1e-6 * beta

@@ -310,3 +310,55 @@ def test_compensated_near_field_on_cancelling_weights():
         errs[prec] = np.abs(v[:2000] - ref).max() / np.abs(ref).max()
     print(f"cancelling weights: mixed {errs['mixed']:.2e}, compensated {errs['mixed_comp']:.2e}", flush=True)
     assert errs["mixed_comp"] <= errs["mixed"]
+
+
+@pytest.fixture(scope="module")
+def headline_problem(orc):
+    """The evaluation workload of the benchmark: 1,000,000 bh3 sources U[0,1]^3
+    (seed 3) with N(0,1) weights (seed 4) at 1,000,000 targets U[0,1]^3
+    (seed 6); the oracle's FP64 direct sum on 2,000 of the targets (seed 7)."""
+    src = np.random.default_rng(3).uniform(0, 1, (1_000_000, 3))
+    w = np.random.default_rng(4).normal(size=1_000_000)
+    trg = np.random.default_rng(6).uniform(0, 1, (1_000_000, 3))
+    sub = np.random.default_rng(7).permutation(1_000_000)[:2000]
+    ref = orc.direct_rbf_sum(orc.make_rbf(orc.BH3), src, w, trg[sub])
+    return src, w, trg, sub, ref
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+def test_headline_config_within_1e6_relative(headline_problem, prec):
+    """bh3 at order 7 with the leaf level chosen by occupancy (the benchmark's
+    configuration): relative L-inf error against the oracle's FP64 direct sum
+    within 1e-6, with at least 150 sources per occupied leaf."""
+    src, w, trg, sub, ref = headline_problem
+    log("headline", prec)
+    ev = pa.make_fmm_evaluator(make("bh3"), ((0, 0, 0), (1, 1, 1)))
+    ev.set_config(pa.FmmConfig(prec, 7, -1, "fmm"))
+    ev.set_source_points(src)
+    ev.set_weights(w)
+    ev.set_target_points(trg)
+    y = ev.evaluate()
+    st = ev.stats()
+    per_leaf = len(src) / st["n_src_leaves"]
+    e = rel_err(y[sub], ref)
+    log(f"  leaf level {st['leaf_level']}, {per_leaf:.0f} sources per leaf, rel err {e:.3e}")
+    assert st["direct"] == 0
+    assert per_leaf >= 150
+    assert e <= 1e-6
+
+
+def test_headline_accuracy_search_within_1e6_relative(headline_problem):
+    """The accuracy search at 1e-6 x max|y| (absolute, as polatory's accuracy
+    is) picks a configuration whose error at the benchmark's targets is within
+    that accuracy."""
+    src, w, trg, sub, ref = headline_problem
+    acc = 1e-6 * np.abs(ref).max()
+    ev = pa.make_fmm_evaluator(make("bh3"), ((0, 0, 0), (1, 1, 1)))
+    ev.set_source_points(src)
+    ev.set_weights(w)
+    ev.set_target_points(trg)
+    ev.set_accuracy(acc)
+    y = ev.evaluate()
+    e = np.abs(y[sub] - ref).max()
+    log(f"  search chose {ev.config()}, abs err {e:.3e} (accuracy {acc:.3e})")
+    assert e <= acc

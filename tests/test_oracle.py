@@ -211,3 +211,65 @@ def test_fgmres_planted_sign_error_is_caught(orc):
     wrong = x.copy()
     wrong[0] += 1e-6
     assert abs(hist[-1] - np.linalg.norm(rhs - a @ wrong)) > 1e-12 * np.linalg.norm(rhs)
+
+
+# ---- the reference-algorithm CPU FMM (oracle/cpu_fmm.c), the CPU baseline ----
+
+@pytest.fixture(scope="module")
+def fmm_case(orc):
+    rng = np.random.default_rng(91)
+    src = rng.uniform(0, 1, (12000, 3))
+    w = rng.normal(size=12000)
+    trg = rng.uniform(0, 1, (3000, 3))
+    return src, w, trg
+
+
+@pytest.mark.parametrize("kind", ["bh3", "th3", "exp"])
+def test_cpu_fmm_converges_to_direct_sum(orc, fmm_case, kind):
+    """Against the brute-force FP64 sum: the error falls by at least 10x per two
+    orders and is within 1e-6 relative at order 8, the first order polatory's
+    accuracy search tries (fmm_accuracy_estimator.hpp:105-108). A dropped
+    transfer, a wrong DFT index or sign, or a misplaced node fails this."""
+    src, w, trg = fmm_case
+    aniso = np.array([[1.0, 0.1, 0.0], [0.0, 0.8, 0.2], [0.0, 0.0, 1.3]])
+    r = orc.make_rbf({"bh3": orc.BH3, "th3": orc.TH3, "exp": orc.EXP}[kind], None, aniso)
+    ref = orc.direct_rbf_sum(r, src, w, trg)
+    errs = []
+    for order in (4, 6, 8):
+        y, st = orc.cpu_fmm_evaluate(r, src, w, trg, ((0, 0, 0), (1, 1, 1)), order)
+        assert st["height"] == 5 and st["m2l_pairs"] > 0 and st["p2p_pairs"] > 0
+        errs.append(np.abs(y - ref).max() / np.abs(ref).max())
+    assert errs[1] < errs[0] / 10 and errs[2] < errs[1] / 10, errs
+    assert errs[2] < 1e-6, errs
+
+
+def test_cpu_fmm_height_and_direct_threshold(orc, fmm_case):
+    """Height max(2, round(log8 max(N, M))) (src/fmm/utility.hpp:12-17); below
+    1024^2 pairs the direct sum (fmm_evaluator.hpp:227-234), bit for bit; at
+    height 2 every leaf neighbours every other, so the FMM is the near field
+    alone and equals the direct sum up to the summation order."""
+    src, w, trg = fmm_case
+    r = orc.make_rbf(orc.BH3)
+    y, st = orc.cpu_fmm_evaluate(r, src[:1000], w[:1000], trg[:1000], ((0, 0, 0), (1, 1, 1)), 8)
+    assert st["height"] == 0
+    np.testing.assert_array_equal(y, orc.direct_rbf_sum(r, src[:1000], w[:1000], trg[:1000]))
+    y, st = orc.cpu_fmm_evaluate(r, src, w, trg, ((0, 0, 0), (1, 1, 1)), 8, height=2)
+    ref = orc.direct_rbf_sum(r, src, w, trg)
+    assert st["m2l_pairs"] == 0
+    assert np.abs(y - ref).max() <= 1e-12 * np.abs(ref).max()
+    for n, h in ((4097, 4), (300_000, 6), (1_000_000, 7)):
+        assert max(2, round(np.log(n) / np.log(8))) == h
+
+
+def test_cpu_fmm_floater_hormann(orc, fmm_case):
+    """The fit operator's interpolant (order 12, Floater-Hormann degree 8,
+    solver.hpp:40 with fmm_accuracy_estimator.hpp:80-82) is accurate, and degree
+    order - 1 is the classic Lagrange interpolant (same bits)."""
+    src, w, trg = fmm_case
+    r = orc.make_rbf(orc.BH3)
+    ref = orc.direct_rbf_sum(r, src, w, trg)
+    y, _ = orc.cpu_fmm_evaluate(r, src, w, trg, ((0, 0, 0), (1, 1, 1)), 12, fh_degree=8)
+    assert np.abs(y - ref).max() / np.abs(ref).max() < 1e-8
+    a, _ = orc.cpu_fmm_evaluate(r, src, w, trg, ((0, 0, 0), (1, 1, 1)), 6, fh_degree=5)
+    b, _ = orc.cpu_fmm_evaluate(r, src, w, trg, ((0, 0, 0), (1, 1, 1)), 6, fh_degree=-1)
+    np.testing.assert_array_equal(a, b)
