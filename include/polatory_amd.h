@@ -193,6 +193,23 @@ int pam_direct_evaluate(const pam_rbf* rbf, const double* src, int64_t n_src, co
 int pam_direct_evaluate_gradient(const pam_rbf* rbf, const double* src, int64_t n_src, const double* w,
                                  const double* trg, int64_t n_trg, double* out);
 
+/* pam_direct_evaluate with device pointers that allocates no memory. No
+ * counterpart in the reference: it is for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). The transformed points and the partial sums live in a fixed
+ * device work area of PAM_NOALLOC_SCRATCH_BYTES per GPU, held from the first
+ * call on that GPU until the process ends. Sizes whose work area
+ * (3 n_trg + 3 n_src + the partial sums, in doubles) exceeds it are refused
+ * with PAM_ERR_INVALID_ARGUMENT. Work is enqueued on `stream` (a hipStream_t; NULL for
+ * the default stream) and the call returns without waiting; d_out is complete
+ * when the stream reaches that point. Same kernels and summation order as
+ * pam_direct_evaluate, so the same bits. All calls on one GPU share the work
+ * area: calls that overlap on one GPU (other streams or threads) race. */
+#define PAM_NOALLOC_SCRATCH_BYTES (64LL << 20)
+int pam_direct_evaluate_device_noalloc(const pam_rbf* rbf, const double* d_src, int64_t n_src,
+                                       const double* d_w, const double* d_trg, int64_t n_trg,
+                                       double* d_out, void* stream);
+
 /* Pointwise kernel values at n differences diff[3 i .. 3 i + 2], evaluated on
  * the GPU: RbfBase::evaluate (rbf/rbf_base.hpp:40-43) -> out[i];
  * evaluate_gradient (:45-48) -> out[3 i + k]; evaluate_hessian (:50-53) ->

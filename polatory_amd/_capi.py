@@ -127,6 +127,7 @@ _SIGS = {
     "pam_fmm_evaluator_export_m2l": [_P, _INT, _P],
     "pam_direct_evaluate": [ctypes.POINTER(PamRbf), _P, _I64, _P, _P, _I64, _P],
     "pam_direct_evaluate_gradient": [ctypes.POINTER(PamRbf), _P, _I64, _P, _P, _I64, _P],
+    "pam_direct_evaluate_device_noalloc": [ctypes.POINTER(PamRbf), _P, _I64, _P, _P, _I64, _P, _P],
     "pam_rbf_evaluate": [ctypes.POINTER(PamRbf), _P, _I64, _P],
     "pam_rbf_evaluate_gradient": [ctypes.POINTER(PamRbf), _P, _I64, _P],
     "pam_rbf_evaluate_hessian": [ctypes.POINTER(PamRbf), _P, _I64, _P],
