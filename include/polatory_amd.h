@@ -260,6 +260,9 @@ typedef struct {
   int keep_unconverged;     /* at max_iter keep the last iterate instead of failing (default 0) */
   int tune_solves;          /* time the RAS batched-solve workgroup sizes per level at set-up (default 0:
                                a fixed rule that reproduces the timed choice on the 1M-point fit) */
+  int ras_fp32_factors;     /* RAS fine-level factors in FP32 (FP32 MFMA factorisation, FP64 solve
+                               arithmetic; the coarse grid stays FP64): 1 on, 0 off, -1 (default) on
+                               when the operator is not FP64 and no rbf has CPD order 2 (bh3, exp) */
 } pam_fit_options;
 int pam_fit_options_default(pam_fit_options* opt);
 /* Interpolant(const Model&) — interpolant.hpp:43 */
@@ -445,6 +448,9 @@ typedef struct {
      requests in one */
   int64_t batches, batched_requests;
   int max_batch;
+  int64_t fmm_batches;          /* FMM batches: near fields and far-field L2Ps in shared launches */
+  int64_t fmm_batched_requests; /* requests served in FMM batches */
+  int max_fmm_batch;            /* most requests in one FMM batch */
 } pam_server_stats;
 int pam_server_get_stats(pam_server* h, pam_server_stats* out);
 /* Forgets completed requests and restarts the latency statistics. */

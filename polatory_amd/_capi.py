@@ -41,7 +41,7 @@ class PamFitOptions(ctypes.Structure):
                 ("use_level_config", ctypes.c_int), ("use_preconditioner", ctypes.c_int),
                 ("refine_rtol", ctypes.c_double), ("verbose", ctypes.c_int),
                 ("use_graphs", ctypes.c_int), ("keep_unconverged", ctypes.c_int),
-                ("tune_solves", ctypes.c_int)]
+                ("tune_solves", ctypes.c_int), ("ras_fp32_factors", ctypes.c_int)]
 
 
 class PamFitStats(ctypes.Structure):
@@ -66,7 +66,9 @@ class PamServerStats(ctypes.Structure):
                 ("wall_s", ctypes.c_double), ("max_in_flight", ctypes.c_int),
                 ("resident_bytes", ctypes.c_int64), ("budget", ctypes.c_int64),
                 ("peak_admitted_bytes", ctypes.c_int64), ("batches", ctypes.c_int64),
-                ("batched_requests", ctypes.c_int64), ("max_batch", ctypes.c_int)]
+                ("batched_requests", ctypes.c_int64), ("max_batch", ctypes.c_int),
+                ("fmm_batches", ctypes.c_int64), ("fmm_batched_requests", ctypes.c_int64),
+                ("max_fmm_batch", ctypes.c_int)]
 
 
 class PamError(RuntimeError):

@@ -73,6 +73,8 @@ pam::FitOptions to_opts(const pam_fit_options* o) {
   r.keep_unconverged = o->keep_unconverged != 0;
   r.verbose = o->verbose;
   r.tune_solves = o->tune_solves != 0;
+  if (o->ras_fp32_factors < -1 || o->ras_fp32_factors > 1) pam::throw_invalid("ras_fp32_factors must be -1, 0 or 1");
+  r.ras_fp32_factors = o->ras_fp32_factors;
   return r;
 }
 
@@ -199,6 +201,7 @@ int pam_fit_options_default(pam_fit_options* opt) {
     opt->keep_unconverged = o.keep_unconverged ? 1 : 0;
     opt->verbose = 0;
     opt->tune_solves = o.tune_solves ? 1 : 0;
+    opt->ras_fp32_factors = o.ras_fp32_factors;
   });
 }
 

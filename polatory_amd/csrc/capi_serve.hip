@@ -130,6 +130,9 @@ int pam_server_get_stats(pam_server* h, pam_server_stats* out) {
     out->batches = s.batches;
     out->batched_requests = s.batched_requests;
     out->max_batch = s.max_batch;
+    out->fmm_batches = s.fmm_batches;
+    out->fmm_batched_requests = s.fmm_batched_requests;
+    out->max_fmm_batch = s.max_fmm_batch;
   });
 }
 

@@ -313,9 +313,9 @@ l2l_kernel(const int32_t* __restrict__ parent_of, const uint64_t* __restrict__ c
 }
 
 template <class T, int P>
-__global__ void __launch_bounds__(kThreads)
-l2p_kernel(const int32_t* __restrict__ leaf_start, const V4<T>* __restrict__ pts,
-           const T* __restrict__ l, int kpad, T* __restrict__ y_far, const int32_t* __restrict__ perm) {
+__device__ __forceinline__ void l2p_leaf(int leaf, const int32_t* __restrict__ leaf_start,
+                                         const V4<T>* __restrict__ pts, const T* __restrict__ l, int kpad,
+                                         T* __restrict__ y_far, const int32_t* __restrict__ perm) {
   constexpr int P2 = P * P, P3 = P * P * P;
   __shared__ T tk[P2];
   __shared__ T ls[P3];
@@ -326,7 +326,6 @@ l2p_kernel(const int32_t* __restrict__ leaf_start, const V4<T>* __restrict__ pts
   __shared__ T sys[P][kThreads];
   __shared__ T szs[P][kThreads];
   constexpr int kUnrollB = P <= 8 ? P : 2;
-  const int leaf = blockIdx.x;
   init_tk<T, P>(tk);
   for (int idx = threadIdx.x; idx < P3; idx += kThreads) ls[idx] = l[static_cast<int64_t>(leaf) * kpad + idx];
   __syncthreads();
@@ -361,18 +360,29 @@ l2p_kernel(const int32_t* __restrict__ leaf_start, const V4<T>* __restrict__ pts
   }
 }
 
+// The far fields (L2P) of up to kMaxFmmJobs evaluations of one order in one
+// launch, job lookup as in p2p_jobs_kernel.
+template <class T, int P>
+__global__ void __launch_bounds__(kThreads) l2p_jobs_kernel(L2PJobs<T> jobs) {
+  const int g = blockIdx.x;
+  int k = 0;
+  while (k + 1 < jobs.n && jobs.j[k + 1].first_leaf <= g) k++;
+  const L2PJob<T>& j = jobs.j[k];
+  l2p_leaf<T, P>(g - j.first_leaf, j.leaf_start, j.pts, j.l, j.kpad, j.y_far, j.perm);
+}
+
 // Targets per thread of the near-field kernel: each source read from LDS
 // serves kP2PR pairs.
 constexpr int kP2PR = 4;
 
 // kComp: compensated (Kahan) sums per target and FP64 sums over the source
 // splits, for weights that cancel strongly (kriging).
-template <class T, int KIND, bool kComp = false>
-__global__ void __launch_bounds__(kThreads)
-p2p_kernel(const int32_t* __restrict__ tls, const int32_t* __restrict__ sls,
-           const int32_t* __restrict__ p2p, const V4<T>* __restrict__ tpts,
-           const V4<T>* __restrict__ spts, const double* __restrict__ y_far,
-           const int32_t* __restrict__ tperm, T pa, T pb, double* __restrict__ out) {
+// One target leaf of one evaluation's near field.
+template <class T, int KIND, bool kComp>
+__device__ __forceinline__ void p2p_leaf(int leaf, const int32_t* __restrict__ tls, const int32_t* __restrict__ sls,
+                                         const int32_t* __restrict__ p2p, const V4<T>* __restrict__ tpts,
+                                         const V4<T>* __restrict__ spts, const double* __restrict__ y_far,
+                                         const int32_t* __restrict__ tperm, T pa, T pb, double* __restrict__ out) {
   constexpr int CAP = 32768 / static_cast<int>(sizeof(V4<T>));
   constexpr int R = kP2PR;
   __shared__ V4<T> sp[CAP];
@@ -381,7 +391,6 @@ p2p_kernel(const int32_t* __restrict__ tls, const int32_t* __restrict__ sls,
   __shared__ T nb_off[27][3];
   __shared__ int nb_count;
   __shared__ T red[R][kThreads];
-  const int leaf = blockIdx.x;
   const int ts = tls[leaf], te = tls[leaf + 1];
   const int nt = te - ts;
   if (threadIdx.x == 0) {
@@ -490,6 +499,21 @@ p2p_kernel(const int32_t* __restrict__ tls, const int32_t* __restrict__ sls,
     }
     __syncthreads();
   }
+}
+
+// The near fields of up to kMaxFmmJobs evaluations (same precision, kernel
+// kind and summation form) in one launch: workgroup g takes target leaf
+// g - first_leaf of the last job whose first_leaf <= g. A single evaluation
+// is the one-job case of the same kernel, so its result does not depend on
+// what shares the launch.
+template <class T, int KIND, bool kComp>
+__global__ void __launch_bounds__(kThreads) p2p_jobs_kernel(P2PJobs<T> jobs) {
+  const int g = blockIdx.x;
+  int k = 0;
+  while (k + 1 < jobs.n && jobs.j[k + 1].first_leaf <= g) k++;
+  const P2PJob<T>& j = jobs.j[k];
+  p2p_leaf<T, KIND, kComp>(g - j.first_leaf, j.tls, j.sls, j.p2p, j.tpts, j.spts, j.y_far, j.tperm, j.pa, j.pb,
+                           j.out);
 }
 
 // ---- gradients at the targets (gradient-transpose kernel,
@@ -1093,8 +1117,11 @@ struct M2LTile {
   static constexpr int kKB = kK / 16;                    // 16-wide fragment blocks per chunk
   // Row / column pitches padded by 4 elements: keeps V4 fragment reads aligned
   // and spreads the 16 rows of a fragment over distinct LDS banks.
-  static constexpr int kAP = 16 + 4;
-  static constexpr int kBP = kK + 4;
+  // FP32: no padding, 16-byte units XOR-swizzled by row (A) / column (B)
+  // instead, so that a workgroup's LDS stays under 53 KB (three per CU).
+  static constexpr bool kSwz = sizeof(TM) == 4;
+  static constexpr int kAP = kSwz ? 16 : 16 + 4;
+  static constexpr int kBP = kSwz ? kK : kK + 4;
   static constexpr int kUpr = static_cast<int>(sizeof(TM));  // 16-byte units per 16-element row
   static constexpr int kAUnits = kKB * kM2LR * kUpr;         // 1024
   static constexpr int kBUnitsPerCol = kK * static_cast<int>(sizeof(TM)) / 16;  // 8
@@ -1138,14 +1165,15 @@ struct M2LSmem {
 // emit(ia, dacc) as soon as it is complete and dacc restarts from zero.
 // Ends with a barrier.
 struct NoEmit {
-  __device__ void operator()(int, const double (&)[4][2][4]) const {}
+  template <class DT>
+  __device__ void operator()(int, const DT (&)[4][2][4]) const {}
 };
 
-template <class TM, bool kEmit = false, class Emit = NoEmit>
+template <class TM, bool kEmit = false, class Emit = NoEmit, class DT = double>
 __device__ __forceinline__ void m2l_mainloop(M2LSmem<TM>& sm, const int (*src_tab)[kM2LC],
                                              const int* act_v, int nact, const TM* __restrict__ a,
                                              int64_t per_v, const TM* __restrict__ in, int kpad, int rpad,
-                                             int rb, int r, double (&dacc)[4][2][4], Emit emit = Emit{}) {
+                                             int rb, int r, DT (&dacc)[4][2][4], Emit emit = Emit{}) {
   using Tl = M2LTile<TM>;
   using acc_t = typename Mfma<TM>::acc_t;
   constexpr int kK = Tl::kK, kKB = Tl::kKB, kUpr = Tl::kUpr;
@@ -1203,12 +1231,14 @@ __device__ __forceinline__ void m2l_mainloop(M2LSmem<TM>& sm, const int (*src_ta
       const int seg = (q * kThreads) / kAU;
       const int rem = (q * kThreads) % kAU + tid;
       const int row = rem / kUpr, pc = rem % kUpr;
-      *reinterpret_cast<u32x4*>(&sm.as[buf][seg][row][pc * (16 / static_cast<int>(sizeof(TM)))]) = rs.a[q];
+      const int pcs = Tl::kSwz ? (pc ^ ((row >> 2) & 3)) : pc;
+      *reinterpret_cast<u32x4*>(&sm.as[buf][seg][row][pcs * (16 / static_cast<int>(sizeof(TM)))]) = rs.a[q];
     }
     // Masked with AND rather than a select: a select of two loaded values
     // becomes a select of addresses, which pins the sets in scratch.
-    *reinterpret_cast<u32x4*>(&sm.bs[buf][bc0][bq * (16 / static_cast<int>(sizeof(TM)))]) = rs.b[0] & rs.m0;
-    *reinterpret_cast<u32x4*>(&sm.bs[buf][bc1][bq * (16 / static_cast<int>(sizeof(TM)))]) = rs.b[1] & rs.m1;
+    const int bq0 = Tl::kSwz ? (bq ^ (bc0 & 7)) : bq, bq1 = Tl::kSwz ? (bq ^ (bc1 & 7)) : bq;
+    *reinterpret_cast<u32x4*>(&sm.bs[buf][bc0][bq0 * (16 / static_cast<int>(sizeof(TM)))]) = rs.b[0] & rs.m0;
+    *reinterpret_cast<u32x4*>(&sm.bs[buf][bc1][bq1 * (16 / static_cast<int>(sizeof(TM)))]) = rs.b[1] & rs.m1;
   };
 
   acc_t acc[4][2];
@@ -1227,7 +1257,7 @@ __device__ __forceinline__ void m2l_mainloop(M2LSmem<TM>& sm, const int (*src_ta
       for (int j = 0; j < 2; j++)
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-          dacc[i][j][q] += static_cast<double>(acc[i][j][q]);
+          dacc[i][j][q] += static_cast<DT>(acc[i][j][q]);
           acc[i][j][q] = TM(0);
         }
   };
@@ -1246,10 +1276,14 @@ __device__ __forceinline__ void m2l_mainloop(M2LSmem<TM>& sm, const int (*src_ta
 #pragma unroll
       for (int i = 0; i < 4; i++)
         if (i < nvt)
-          af[i] = *reinterpret_cast<const V4T*>(&sm.as[buf][kb][wr * 64 + i * 16 + (lane & 15)][(lane >> 4) * 4]);
+          af[i] = *reinterpret_cast<const V4T*>(
+              &sm.as[buf][kb][wr * 64 + i * 16 + (lane & 15)]
+                    [(Tl::kSwz ? ((lane >> 4) ^ (((lane & 15) >> 2) & 3)) : (lane >> 4)) * 4]);
 #pragma unroll
       for (int j = 0; j < 2; j++)
-        bf[j] = *reinterpret_cast<const V4T*>(&sm.bs[buf][wc * 32 + j * 16 + (lane & 15)][kb * 16 + (lane >> 4) * 4]);
+        bf[j] = *reinterpret_cast<const V4T*>(
+            &sm.bs[buf][wc * 32 + j * 16 + (lane & 15)]
+                  [(Tl::kSwz ? ((kb * 4 + (lane >> 4)) ^ (lane & 7)) : (kb * 4 + (lane >> 4))) * 4]);
 #pragma unroll
       for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -1419,7 +1453,11 @@ m2l_pair_kernel(M2LPairArgs<TM, PT> b) {
   PT* __restrict__ out = b.partial[lv];
   // D layout (16 x 16), column lane % 16; register q holds row 4 (lane / 16) + q
   // for the f32 instruction and row lane / 16 + 4 q for the f64 one.
-  auto emit = [&](int ia, const double (&d)[4][2][4]) {
+  // FP32 operands: the partial sums of 64 reduction steps are added in FP32
+  // (the per-pair transfer is stored in FP32 anyway), which keeps the
+  // workgroup within three per CU.
+  using DT = std::conditional_t<sizeof(TM) == 4 && sizeof(PT) == 4, float, double>;
+  auto emit = [&](int ia, const DT (&d)[4][2][4]) {
 #pragma unroll
     for (int j = 0; j < 2; j++) {
       const int col = wc * 32 + j * 16 + (lane & 15);
@@ -1443,8 +1481,9 @@ m2l_pair_kernel(M2LPairArgs<TM, PT> b) {
       }
     }
   };
-  double dacc[4][2][4];
-  m2l_mainloop<TM, true>(sm, src_tab, act_v, nb, b.a[lv], b.per_v, b.in[lv], b.kpad, rpad, rb, b.r, dacc, emit);
+  DT dacc[4][2][4];
+  m2l_mainloop<TM, true, decltype(emit), DT>(sm, src_tab, act_v, nb, b.a[lv], b.per_v, b.in[lv], b.kpad, rpad, rb,
+                                             b.r, dacc, emit);
 }
 
 // out[cell][row] = scale * sum over the cell's slots, in slot order, of the
@@ -1871,39 +1910,134 @@ void launch_l2l(int p, const Tree& t, int l, const T* l_parent, T* l_child, hipS
 }
 
 template <class T>
-void launch_l2p(int p, const Tree& t, const T* l_leaf, const V4<T>* pts, T* y_far, hipStream_t s,
-                bool add_to_output) {
-  int64_t nl = t.n_leaves();
-  if (nl == 0) return;
+L2PJob<T> make_l2p_job(const Tree& t, const T* l_leaf, const V4<T>* pts, T* y_far, bool add_to_output, int kpad) {
+  L2PJob<T> j{};
+  j.leaf_start = t.leaf_start.get();
+  j.pts = pts;
+  j.l = l_leaf;
+  j.kpad = kpad;
+  j.y_far = y_far;
+  j.perm = add_to_output ? t.perm.get() : nullptr;
+  j.n_leaves = static_cast<int>(t.n_leaves());
+  return j;
+}
+
+template <class T>
+P2PJob<T> make_p2p_job(const RbfDesc& rbf, const Tree& trg, const Tree& src, const Lists& lists, const V4<T>* tpts,
+                       const V4<T>* spts, const double* y_far, double* out) {
+  P2PJob<T> j{};
+  const double h = trg.box.width / std::ldexp(1.0, trg.level) / 2.0;
+  const PhiConsts pc = phi_consts(rbf, h);
+  j.tls = trg.leaf_start.get();
+  j.sls = src.leaf_start.get();
+  j.p2p = lists.p2p.get();
+  j.tpts = tpts;
+  j.spts = spts;
+  j.y_far = y_far;
+  j.tperm = trg.perm.get();
+  j.pa = static_cast<T>(pc.a);
+  j.pb = static_cast<T>(pc.b);
+  j.out = out;
+  j.n_leaves = static_cast<int>(trg.n_leaves());
+  return j;
+}
+
+template <class T>
+void launch_l2p_jobs(int p, const L2PJob<T>* jobs, int n, hipStream_t s) {
+  if (n < 1 || n > kMaxFmmJobs) throw_runtime("L2P job count out of range");
+  L2PJobs<T> t{};
+  int64_t total = 0;
+  for (int k = 0; k < n; k++) {
+    t.j[t.n] = jobs[k];
+    t.j[t.n].first_leaf = static_cast<int>(total);
+    total += jobs[k].n_leaves;
+    if (jobs[k].n_leaves > 0) t.n++;
+  }
+  if (total == 0) return;
+  if (total > INT32_MAX) throw_runtime("too many leaves in one launch");
   dispatch_order(p, [&](auto P) {
-    l2p_kernel<T, decltype(P)::value><<<static_cast<unsigned>(nl), kThreads, 0, s>>>(
-        t.leaf_start.get(), pts, l_leaf, kpad_of(P), y_far, add_to_output ? t.perm.get() : nullptr);
+    l2p_jobs_kernel<T, decltype(P)::value><<<static_cast<unsigned>(total), kThreads, 0, s>>>(t);
   });
   PAM_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_p2p_jobs(int kind, bool compensated, const P2PJob<T>* jobs, int n, hipStream_t s) {
+  if (n < 1 || n > kMaxFmmJobs) throw_runtime("P2P job count out of range");
+  P2PJobs<T> t{};
+  int64_t total = 0;
+  for (int k = 0; k < n; k++) {
+    t.j[t.n] = jobs[k];
+    t.j[t.n].first_leaf = static_cast<int>(total);
+    total += jobs[k].n_leaves;
+    if (jobs[k].n_leaves > 0) t.n++;
+  }
+  if (total == 0) return;
+  if (total > INT32_MAX) throw_runtime("too many leaves in one launch");
+  dispatch_kind(kind, [&](auto K) {
+    if constexpr (sizeof(T) == 4) {
+      if (compensated) {
+        p2p_jobs_kernel<T, decltype(K)::value, true><<<static_cast<unsigned>(total), kThreads, 0, s>>>(t);
+        return;
+      }
+    }
+    p2p_jobs_kernel<T, decltype(K)::value, false><<<static_cast<unsigned>(total), kThreads, 0, s>>>(t);
+  });
+  PAM_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_l2p(int p, const Tree& t, const T* l_leaf, const V4<T>* pts, T* y_far, hipStream_t s,
+                bool add_to_output) {
+  if (t.n_leaves() == 0) return;
+  const L2PJob<T> j = make_l2p_job<T>(t, l_leaf, pts, y_far, add_to_output, kpad_of(p));
+  launch_l2p_jobs<T>(p, &j, 1, s);
 }
 
 template <class T>
 void launch_p2p(const RbfDesc& rbf, const Tree& trg, const Tree& src, const Lists& lists,
                 const V4<T>* tpts, const V4<T>* spts, const double* y_far, double* out, hipStream_t s,
                 bool compensated) {
-  int64_t nl = trg.n_leaves();
-  if (nl == 0) return;
-  double h = trg.box.width / std::ldexp(1.0, trg.level) / 2.0;
-  PhiConsts pc = phi_consts(rbf, h);
-  dispatch_kind(rbf.kind, [&](auto K) {
-    if constexpr (sizeof(T) == 4) {
-      if (compensated) {
-        p2p_kernel<T, decltype(K)::value, true><<<static_cast<unsigned>(nl), kThreads, 0, s>>>(
-            trg.leaf_start.get(), src.leaf_start.get(), lists.p2p.get(), tpts, spts, y_far, trg.perm.get(),
-            static_cast<T>(pc.a), static_cast<T>(pc.b), out);
-        return;
+  if (trg.n_leaves() == 0) return;
+  const P2PJob<T> j = make_p2p_job<T>(rbf, trg, src, lists, tpts, spts, y_far, out);
+  launch_p2p_jobs<T>(rbf.kind, compensated, &j, 1, s);
+}
+
+void launch_deferred(const std::vector<const DeferredFmm*>& jobs, hipStream_t s) {
+  // Near fields first (each writes its output), grouped by precision, kernel
+  // kind and summation form; then the far fields (added to the outputs),
+  // grouped by order. Each group goes out in launches of kMaxFmmJobs.
+  auto flush_p2p = [&](auto tag, int kind, bool comp) {
+    using T = decltype(tag);
+    std::vector<P2PJob<T>> g;
+    for (const DeferredFmm* d : jobs) {
+      if (!d || !d->valid || d->kind != kind || d->compensated != comp || (sizeof(T) == 4) != d->fp32) continue;
+      if constexpr (sizeof(T) == 4) g.push_back(d->p2p32);
+      else g.push_back(d->p2p64);
+      if (static_cast<int>(g.size()) == kMaxFmmJobs) {
+        launch_p2p_jobs<T>(kind, comp, g.data(), kMaxFmmJobs, s);
+        g.clear();
       }
     }
-    p2p_kernel<T, decltype(K)::value><<<static_cast<unsigned>(nl), kThreads, 0, s>>>(
-        trg.leaf_start.get(), src.leaf_start.get(), lists.p2p.get(), tpts, spts, y_far, trg.perm.get(),
-        static_cast<T>(pc.a), static_cast<T>(pc.b), out);
-  });
-  PAM_LAUNCH_CHECK();
+    if (!g.empty()) launch_p2p_jobs<T>(kind, comp, g.data(), static_cast<int>(g.size()), s);
+  };
+  for (int kind : {kBh3, kTh3, kExp}) {
+    flush_p2p(0.0, kind, false);
+    flush_p2p(0.0f, kind, false);
+    flush_p2p(0.0f, kind, true);
+  }
+  for (int p = kMinOrder; p <= kMaxOrder; p++) {
+    std::vector<L2PJob<double>> g;
+    for (const DeferredFmm* d : jobs) {
+      if (!d || !d->valid || d->order != p) continue;
+      g.push_back(d->l2p);
+      if (static_cast<int>(g.size()) == kMaxFmmJobs) {
+        launch_l2p_jobs<double>(p, g.data(), kMaxFmmJobs, s);
+        g.clear();
+      }
+    }
+    if (!g.empty()) launch_l2p_jobs<double>(p, g.data(), static_cast<int>(g.size()), s);
+  }
 }
 
 void launch_l2p_grad(int p, const Tree& t, const double* l_leaf, const V4<double>* pts, const double* aniso,
@@ -2028,6 +2162,9 @@ void launch_m2l_batch(const M2LOperator& op, const std::vector<M2LLevelSpec<TM, 
   template void launch_m2m<T>(int, const Tree&, int, const T*, T*, hipStream_t);                    \
   template void launch_l2l<T>(int, const Tree&, int, const T*, T*, hipStream_t);                    \
   template void launch_l2p<T>(int, const Tree&, const T*, const V4<T>*, T*, hipStream_t, bool);           \
+  template L2PJob<T> make_l2p_job<T>(const Tree&, const T*, const V4<T>*, T*, bool, int);                   \
+  template P2PJob<T> make_p2p_job<T>(const RbfDesc&, const Tree&, const Tree&, const Lists&, const V4<T>*,  \
+                                     const V4<T>*, const double*, double*);                                 \
   template void launch_p2p<T>(const RbfDesc&, const Tree&, const Tree&, const Lists&, const V4<T>*, \
                               const V4<T>*, const double*, double*, hipStream_t, bool);                 \
   template void launch_p2p_grad<T>(const RbfDesc&, const Tree&, const Tree&, const Lists&, const V4<T>*, \

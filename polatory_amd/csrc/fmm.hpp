@@ -53,6 +53,63 @@ void launch_m2m(int p, const Tree& t, int l, const T* m_child, T* m_parent, hipS
 // child level l from parent level l-1 (accumulates into l_child)
 template <class T>
 void launch_l2l(int p, const Tree& t, int l, const T* l_parent, T* l_child, hipStream_t s);
+// Near-field (P2P) and far-field (L2P) work of one evaluation as kernel
+// arguments, so that several evaluations, of any model, can share one launch
+// of each (serving). A lone evaluation runs the same kernels with one job.
+constexpr int kMaxFmmJobs = 8;
+template <class T>
+struct P2PJob {
+  const int32_t* tls;    // target leaf starts
+  const int32_t* sls;    // source leaf starts
+  const int32_t* p2p;    // [target leaves][27] source leaves
+  const V4<T>* tpts;     // targets in tree order, leaf-local
+  const V4<T>* spts;     // sources in tree order, leaf-local, with weights
+  const double* y_far;   // far field in tree order, or null
+  const int32_t* tperm;  // tree order -> input order of the targets
+  double* out;           // output, input order
+  T pa, pb;              // kernel constants in the leaf unit
+  int n_leaves, first_leaf;
+};
+template <class T>
+struct P2PJobs {
+  int n;
+  P2PJob<T> j[kMaxFmmJobs];
+};
+template <class T>
+struct L2PJob {
+  const int32_t* leaf_start;
+  const V4<T>* pts;
+  const T* l;            // leaf local expansions
+  T* y_far;              // output (input order when perm is set)
+  const int32_t* perm;
+  int kpad, n_leaves, first_leaf;
+};
+template <class T>
+struct L2PJobs {
+  int n;
+  L2PJob<T> j[kMaxFmmJobs];
+};
+template <class T>
+P2PJob<T> make_p2p_job(const RbfDesc& rbf, const Tree& trg, const Tree& src, const Lists& lists, const V4<T>* tpts,
+                       const V4<T>* spts, const double* y_far, double* out);
+template <class T>
+L2PJob<T> make_l2p_job(const Tree& t, const T* l_leaf, const V4<T>* pts, T* y_far, bool add_to_output, int kpad);
+// The near field and far field of one evaluation, recorded instead of
+// launched (FmmEvaluator::evaluate_deferred).
+struct DeferredFmm {
+  bool valid = false;
+  bool fp32 = false, compensated = false;
+  int kind = 0, order = 0;
+  P2PJob<float> p2p32{};
+  P2PJob<double> p2p64{};
+  L2PJob<double> l2p{};
+};
+// Launches the recorded work of several evaluations: all near fields (shared
+// launches per precision, kind and summation form), then all far fields
+// (shared launches per order). Each output gets the bits a lone evaluation
+// would give it.
+void launch_deferred(const std::vector<const DeferredFmm*>& jobs, hipStream_t s);
+
 // add_to_output: y_far is the output in input order, the far field is added
 // to it through the tree permutation.
 template <class T>

@@ -671,6 +671,15 @@ void FmmEvaluator::run_fmm(double* d_out) {
   // join (near + far, the same sum in the same order as one stream would do).
   // Without overlap the near field runs on this stream after L2L.
   auto near_field = [&](hipStream_t ps) {
+    if (defer_) {
+      // Recorded for a launch shared with other evaluations (value kernel).
+      defer_->fp32 = sizeof(TP) == 4;
+      defer_->compensated = cfg_.precision == kMixedComp;
+      defer_->kind = rbf_.kind;
+      if constexpr (sizeof(TP) == 4) defer_->p2p32 = make_p2p_job<float>(rbf_, tt, stree, lists_, tp2p, sp2p, nullptr, d_out);
+      else defer_->p2p64 = make_p2p_job<double>(rbf_, tt, stree, lists_, tp2p, sp2p, nullptr, d_out);
+      return;
+    }
     PAM_HIP(hipEventRecord(fork_ev_, stream_));
     PAM_HIP(hipStreamWaitEvent(ps, fork_ev_, 0));
     record(7, ps);
@@ -680,7 +689,8 @@ void FmmEvaluator::run_fmm(double* d_out) {
     record(8, ps);
     PAM_HIP(hipEventRecord(join_ev_, ps));
   };
-  if (overlap_) near_field(p2p_stream_);
+  const bool overlap = overlap_ && !defer_;
+  if (overlap) near_field(p2p_stream_);
   const int deg = homogeneity(rbf_);
   const int32_t* vl = octant_vlist_device(stream_);
   const double tol = effective_svd_tol(cfg_, rbf_);
@@ -759,7 +769,14 @@ void FmmEvaluator::run_fmm(double* d_out) {
   record(3);
   for (int l = 3; l <= L; l++) launch_l2l<double>(P, tt, l, st.l[l - 1].get(), st.l[l].get(), stream_);
   record(4);
-  if (!overlap_) near_field(stream_);
+  if (!overlap) near_field(stream_);
+  if (defer_) {
+    defer_->l2p = make_l2p_job<double>(tt, st.l[L].get(), tpts64, d_out, true, kp);
+    defer_->order = P;
+    defer_->valid = true;
+    for (int i = 5; i <= 8; i++) record(i);
+    return;
+  }
   // Join the near field, then add the far field to it.
   PAM_HIP(hipStreamWaitEvent(stream_, join_ev_, 0));
   record(5);
@@ -878,6 +895,21 @@ void FmmEvaluator::evaluate(double* out, int64_t n, bool on_device) {
   }
   grad_ = false;
   evaluate_impl(out, n, on_device);
+}
+
+void FmmEvaluator::evaluate_deferred(double* d_out, int64_t n, DeferredFmm* job) {
+  if (!job) throw_invalid("null job");
+  if (grad_kernel_ || symmetric_) throw_runtime("deferred evaluation is for source-target value evaluators");
+  *job = DeferredFmm{};
+  grad_ = false;
+  defer_ = job;
+  try {
+    evaluate_impl(d_out, n, true);
+  } catch (...) {
+    defer_ = nullptr;
+    throw;
+  }
+  defer_ = nullptr;
 }
 
 void FmmEvaluator::evaluate_gradient(double* out, int64_t n, bool on_device) {

@@ -75,11 +75,8 @@ class FmmEvaluator {
   FmmEvaluator& operator=(const FmmEvaluator&) = delete;
 
   void set_accuracy(double accuracy);
-
   // Restricts the accuracy search to one precision (orders 4..12 of it); -1
-
   // restores the default list (mixed orders first, then FP64).
-
   void set_search_precision(int precision);
   void set_config(const FmmConfig& c);  // fixes the configuration (no accuracy search)
   // Near field on a second stream beside the far field (default), or all
@@ -90,6 +87,14 @@ class FmmEvaluator {
   void set_weights(const double* w, int64_t n, bool on_device);
   // out has n_trg entries (input order of the targets).
   void evaluate(double* out, int64_t n, bool on_device);
+  // evaluate() on device memory that stops before the near field and the
+  // far-field L2P and records them in *job instead (job->valid), for launches
+  // shared with other evaluations (launch_deferred); the output is complete
+  // once they have run. When the evaluation goes to the direct sum, the
+  // output is written and job->valid is false. Source-target value
+  // evaluators only; the recorded buffers stay valid until this evaluator's
+  // next call.
+  void evaluate_deferred(double* d_out, int64_t n, DeferredFmm* job);
   // Gradients at the targets (make_fmm_gradient_transpose_evaluator,
   // fmm/gradient_transpose_kernel.hpp): out has 3 n_trg entries, x y z per
   // target in input order. Same configuration, trees and expansions as the
@@ -190,6 +195,7 @@ class FmmEvaluator {
   // (VALU work beside MFMA work); fork_ev_ / join_ev_ order it.
   hipStream_t p2p_stream_ = nullptr;
   bool overlap_ = true;
+  DeferredFmm* defer_ = nullptr;  // evaluation in progress records its near and far field here
   hipEvent_t fork_ev_ = nullptr, join_ev_ = nullptr;
   FmmStats stats_;
   bool stats_pending_ = false;

@@ -2,6 +2,8 @@
 // batched FP64 Cholesky of Q^T A Q and batched subdomain solves.
 #include "ras.hpp"
 
+#include <future>
+
 #include <algorithm>
 #include <parallel/algorithm>
 #include <array>
@@ -110,11 +112,13 @@ __global__ void assemble_u_kernel(RbfSet r, int l, const double* __restrict__ pt
 }
 
 // Lower triangle of M = Q^T A Q (fine_grid.hpp:79-86), column-major, Q = [-Lag_inner^T ; I].
+// Entries are computed in FP64 and stored in T (FP32 for single-precision factors).
+template <class T>
 __global__ void assemble_m_kernel(RbfSet r, int l, const double* __restrict__ pts,
                                   const int64_t* __restrict__ ptr, const int32_t* __restrict__ idx,
                                   const double* __restrict__ lag, const double* __restrict__ u,
                                   const double* __restrict__ atl, const int64_t* __restrict__ mat_ptr,
-                                  double* __restrict__ mat) {
+                                  T* __restrict__ mat) {
   int d = blockIdx.y;
   int64_t p0 = ptr[d];
   int m = static_cast<int>(ptr[d + 1] - p0);
@@ -123,13 +127,13 @@ __global__ void assemble_m_kernel(RbfSet r, int l, const double* __restrict__ pt
   int64_t uoff = static_cast<int64_t>(l) * (p0 - static_cast<int64_t>(d) * l);
   const double* ud = u + uoff;
   const double* ad = atl + static_cast<int64_t>(d) * l * l;
-  double* md = mat + mat_ptr[d];
+  T* md = mat + mat_ptr[d];
   int64_t total = static_cast<int64_t>(n) * n;
   for (int64_t e = static_cast<int64_t>(blockIdx.x) * kT + threadIdx.x; e < total;
        e += static_cast<int64_t>(gridDim.x) * kT) {
     int i = static_cast<int>(e % n), j = static_cast<int>(e / n);
     if (i < j) {
-      md[e] = 0.0;
+      md[e] = T(0);
       continue;
     }
     int qi = idx[p0 + l + i], qj = idx[p0 + l + j];
@@ -147,7 +151,7 @@ __global__ void assemble_m_kernel(RbfSet r, int l, const double* __restrict__ pt
         v += qa[a] * t;
       }
     }
-    md[e] = v;
+    md[e] = static_cast<T>(v);
   }
 }
 
@@ -158,21 +162,48 @@ __global__ void assemble_m_kernel(RbfSet r, int l, const double* __restrict__ pt
 // the trailing update A22 -= L21 L21^T runs over 64 x 64 tiles of the lower
 // triangle on FP64 MFMA (4 waves as 2 x 2, 32 x 32 each), the tile result
 // staged in LDS so that the read-modify-write of A22 is coalesced.
+// T = double (FP64 MFMA) or float (FP32 MFMA, single-precision fine-level
+// factors).
+template <class T>
 struct CholSmem {
-  double dg[kNb][kNb + 1];     // diagonal block (L11 after the factorisation)
-  double pi[kTile][kNb + 1];   // panel rows of tile I
-  double pj[kTile][kNb + 1];   // panel rows of tile J
-  double dt[kTile][kTile + 1]; // MFMA result of a tile; the inverse of L11 during the panel phase
+  T dg[kNb][kNb + 1];     // diagonal block (L11 after the factorisation)
+  T pi[kTile][kNb + 1];   // panel rows of tile I
+  T pj[kTile][kNb + 1];   // panel rows of tile J
+  T dt[kTile][kTile + 1]; // MFMA result of a tile; the inverse of L11 during the panel phase
+};
+
+template <class T>
+struct CholMfma;
+template <>
+struct CholMfma<double> {
+  using acc_t = double __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // D layout: register q holds row lane / 16 + 4 q, column lane % 16
+  static __device__ __forceinline__ int row(int lane, int q) { return (lane >> 4) + 4 * q; }
+  static constexpr double kTiny = 1e-300;
+};
+template <>
+struct CholMfma<float> {
+  using acc_t = float __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  // D layout: register q holds row 4 (lane / 16) + q, column lane % 16
+  static __device__ __forceinline__ int row(int lane, int q) { return 4 * (lane >> 4) + q; }
+  static constexpr float kTiny = 1e-30f;
 };
 
 // Panel step k0 of the factorisation of the n x n matrix a (column-major,
 // lower triangle): L11 by one wavefront and its inverse, then L21 = A21 L11^-T.
 // Ends with a barrier.
-__device__ void chol_panel(CholSmem& sm, double* __restrict__ a, int n, int k0, int32_t* __restrict__ flag) {
+template <class T>
+__device__ void chol_panel(CholSmem<T>& sm, T* __restrict__ a, int n, int k0, int32_t* __restrict__ flag) {
   const int lda = n;
   const int kb = min(kNb, n - k0);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  double (*inv)[kNb + 1] = reinterpret_cast<double (*)[kNb + 1]>(&sm.dt[0][0]);
+  T (*inv)[kNb + 1] = reinterpret_cast<T (*)[kNb + 1]>(&sm.dt[0][0]);
   for (int e = tid; e < kb * kb; e += kT) {
     int rr = e % kb, cc = e / kb;
     sm.dg[rr][cc] = a[static_cast<int64_t>(k0 + cc) * lda + k0 + rr];
@@ -181,10 +212,10 @@ __device__ void chol_panel(CholSmem& sm, double* __restrict__ a, int n, int k0, 
   if (wave == 0) {
     for (int j = 0; j < kb; j++) {
       if (lane == 0) {
-        double v = sm.dg[j][j];
-        if (!(v > 0.0)) {
+        T v = sm.dg[j][j];
+        if (!(v > T(0))) {
           atomicExch(flag, 1);
-          v = fabs(v) + 1e-300;
+          v = fabs(v) + CholMfma<T>::kTiny;
         }
         sm.dg[j][j] = sqrt(v);
       }
@@ -206,10 +237,10 @@ __device__ void chol_panel(CholSmem& sm, double* __restrict__ a, int n, int k0, 
       const int c = lane;
       for (int r = 0; r < kb; r++) {
         if (r < c) {
-          inv[r][c] = 0.0;
+          inv[r][c] = T(0);
           continue;
         }
-        double v = r == c ? 1.0 : 0.0;
+        T v = r == c ? T(1) : T(0);
         for (int t = c; t < r; t++) v -= sm.dg[r][t] * inv[t][c];
         inv[r][c] = v / sm.dg[r][r];
       }
@@ -222,12 +253,12 @@ __device__ void chol_panel(CholSmem& sm, double* __restrict__ a, int n, int k0, 
   }
   // L21 = A21 L11^-T, row by row
   for (int i = k0 + kb + tid; i < n; i += kT) {
-    double x[kNb];
+    T x[kNb];
 #pragma unroll
-    for (int t = 0; t < kNb; t++) x[t] = t < kb ? a[static_cast<int64_t>(k0 + t) * lda + i] : 0.0;
+    for (int t = 0; t < kNb; t++) x[t] = t < kb ? a[static_cast<int64_t>(k0 + t) * lda + i] : T(0);
 #pragma unroll 4
     for (int c = 0; c < kb; c++) {
-      double v = 0.0;
+      T v = T(0);
 #pragma unroll
       for (int t = 0; t < kNb; t++)
         if (t <= c) v += x[t] * inv[c][t];
@@ -240,8 +271,9 @@ __device__ void chol_panel(CholSmem& sm, double* __restrict__ a, int n, int k0, 
 // Trailing tile (I, J), I >= J, of panel step k0: A22 -= L21 L21^T on FP64
 // MFMA (4 waves as 2 x 2, 32 x 32 each). load_i: stage tile I's panel rows
 // (they can stay staged across the J of one I). Ends with a barrier.
-__device__ void chol_tile(CholSmem& sm, double* __restrict__ a, int n, int k0, int I, int J, bool load_i) {
-  using acc_t = double __attribute__((ext_vector_type(4)));
+template <class T>
+__device__ void chol_tile(CholSmem<T>& sm, T* __restrict__ a, int n, int k0, int I, int J, bool load_i) {
+  using acc_t = typename CholMfma<T>::acc_t;
   const int lda = n;
   const int kb = min(kNb, n - k0);
   const int base = k0 + kb;
@@ -251,22 +283,22 @@ __device__ void chol_tile(CholSmem& sm, double* __restrict__ a, int n, int k0, i
   if (load_i) {
     for (int e = tid; e < kTile * kNb; e += kT) {
       int rr = e % kTile, cc = e / kTile;
-      sm.pi[rr][cc] = (r0 + rr < n && cc < kb) ? a[static_cast<int64_t>(k0 + cc) * lda + r0 + rr] : 0.0;
+      sm.pi[rr][cc] = (r0 + rr < n && cc < kb) ? a[static_cast<int64_t>(k0 + cc) * lda + r0 + rr] : T(0);
     }
   }
   for (int e = tid; e < kTile * kNb; e += kT) {
     int rr = e % kTile, cc = e / kTile;
-    sm.pj[rr][cc] = (c0 + rr < n && cc < kb) ? a[static_cast<int64_t>(k0 + cc) * lda + c0 + rr] : 0.0;
+    sm.pj[rr][cc] = (c0 + rr < n && cc < kb) ? a[static_cast<int64_t>(k0 + cc) * lda + c0 + rr] : T(0);
   }
   __syncthreads();
   acc_t acc[2][2];
 #pragma unroll
   for (int ii = 0; ii < 2; ii++)
 #pragma unroll
-    for (int jj = 0; jj < 2; jj++) acc[ii][jj] = acc_t{0.0, 0.0, 0.0, 0.0};
+    for (int jj = 0; jj < 2; jj++) acc[ii][jj] = acc_t{T(0), T(0), T(0), T(0)};
 #pragma unroll
   for (int ks = 0; ks < kNb / 4; ks++) {
-    double af[2], bf[2];
+    T af[2], bf[2];
 #pragma unroll
     for (int ii = 0; ii < 2; ii++) af[ii] = sm.pi[wi * 32 + ii * 16 + (lane & 15)][ks * 4 + (lane >> 4)];
 #pragma unroll
@@ -275,16 +307,15 @@ __device__ void chol_tile(CholSmem& sm, double* __restrict__ a, int n, int k0, i
     for (int ii = 0; ii < 2; ii++)
 #pragma unroll
       for (int jj = 0; jj < 2; jj++)
-        acc[ii][jj] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[ii], bf[jj], acc[ii][jj], 0, 0, 0);
+        acc[ii][jj] = CholMfma<T>::op(af[ii], bf[jj], acc[ii][jj]);
   }
-  // D layout of the f64 instruction: register q holds row lane / 16 + 4 q, column lane % 16
 #pragma unroll
   for (int ii = 0; ii < 2; ii++)
 #pragma unroll
     for (int jj = 0; jj < 2; jj++)
 #pragma unroll
       for (int q = 0; q < 4; q++)
-        sm.dt[wi * 32 + ii * 16 + (lane >> 4) + 4 * q][wj * 32 + jj * 16 + (lane & 15)] = acc[ii][jj][q];
+        sm.dt[wi * 32 + ii * 16 + CholMfma<T>::row(lane, q)][wj * 32 + jj * 16 + (lane & 15)] = acc[ii][jj][q];
   __syncthreads();
   for (int e = tid; e < kTile * kTile; e += kT) {
     int rr = e % kTile, cc = e / kTile;
@@ -295,14 +326,15 @@ __device__ void chol_tile(CholSmem& sm, double* __restrict__ a, int n, int k0, i
 }
 
 // One workgroup per domain, all panel steps (many domains at once).
+template <class T>
 __global__ void __launch_bounds__(kT)
 cholesky_kernel(const int64_t* __restrict__ ptr, int l, const int64_t* __restrict__ mat_ptr,
-                double* __restrict__ mat, int32_t* __restrict__ flag) {
-  __shared__ CholSmem sm;
+                T* __restrict__ mat, int32_t* __restrict__ flag) {
+  __shared__ CholSmem<T> sm;
   const int d = blockIdx.x;
   const int n = static_cast<int>(ptr[d + 1] - ptr[d]) - l;
   if (n <= 0) return;
-  double* a = mat + mat_ptr[d];
+  T* a = mat + mat_ptr[d];
   for (int k0 = 0; k0 < n; k0 += kNb) {
     chol_panel(sm, a, n, k0, flag);
     const int rem = n - min(kNb, n - k0) - k0;
@@ -317,13 +349,13 @@ cholesky_kernel(const int64_t* __restrict__ ptr, int l, const int64_t* __restric
 // workgroup) and one for its trailing tiles (one workgroup per tile).
 __global__ void __launch_bounds__(kT)
 cholesky_panel_kernel(double* __restrict__ a, int n, int k0, int32_t* __restrict__ flag) {
-  __shared__ CholSmem sm;
+  __shared__ CholSmem<double> sm;
   chol_panel(sm, a, n, k0, flag);
 }
 
 __global__ void __launch_bounds__(kT)
 cholesky_trailing_kernel(double* __restrict__ a, int n, int k0) {
-  __shared__ CholSmem sm;
+  __shared__ CholSmem<double> sm;
   // blockIdx.x -> (I, J) with I >= J
   const int t = blockIdx.x;
   int I = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) / 2.0);
@@ -357,9 +389,9 @@ __device__ __forceinline__ double lane_ordered_sum(double v, double* sh) {
 // emit(m + a, c_a) the polynomial coefficients of the coarse grid.
 // KT threads (a multiple of 64, at least 128); every sum is taken in an order
 // that does not depend on KT, so the solution is the same bits for any KT.
-template <int KT, class Emit>
+template <int KT, class Emit, class TF = double>
 __device__ void solve_domain(RbfSet r, int l, const double* __restrict__ pts, const int32_t* __restrict__ didx,
-                             int m, const double* __restrict__ lag, const double* __restrict__ L,
+                             int m, const double* __restrict__ lag, const TF* __restrict__ L,
                              double* vals, double* y, double* sh, int coarse, const double* __restrict__ pinv,
                              Emit emit) {
   __shared__ double lam_top[10];
@@ -381,7 +413,7 @@ __device__ void solve_domain(RbfSet r, int l, const double* __restrict__ pts, co
     auto stage_diag = [&](int k0, int kb) {
       for (int e = threadIdx.x; e < kb * kb; e += KT) {
         const int rr = e % kb, cc = e / kb;
-        dgl[rr][cc] = L[static_cast<int64_t>(k0 + cc) * lda + k0 + rr];
+        dgl[rr][cc] = static_cast<double>(L[static_cast<int64_t>(k0 + cc) * lda + k0 + rr]);
       }
       __syncthreads();
     };
@@ -405,7 +437,7 @@ __device__ void solve_domain(RbfSet r, int l, const double* __restrict__ pts, co
         for (int i = k0 + kNb + threadIdx.x; i < n; i += KT) {
           double lv[kNb];
 #pragma unroll
-          for (int t = 0; t < kNb; t++) lv[t] = L[static_cast<int64_t>(k0 + t) * lda + i];
+          for (int t = 0; t < kNb; t++) lv[t] = static_cast<double>(L[static_cast<int64_t>(k0 + t) * lda + i]);
           double v = y[i];
 #pragma unroll
           for (int t = 0; t < kNb; t++) v -= lv[t] * y[k0 + t];
@@ -414,7 +446,7 @@ __device__ void solve_domain(RbfSet r, int l, const double* __restrict__ pts, co
       } else {
         for (int i = k0 + kb + threadIdx.x; i < n; i += KT) {
           double v = y[i];
-          for (int t = 0; t < kb; t++) v -= L[static_cast<int64_t>(k0 + t) * lda + i] * y[k0 + t];
+          for (int t = 0; t < kb; t++) v -= static_cast<double>(L[static_cast<int64_t>(k0 + t) * lda + i]) * y[k0 + t];
           y[i] = v;
         }
       }
@@ -440,7 +472,7 @@ __device__ void solve_domain(RbfSet r, int l, const double* __restrict__ pts, co
         for (int i = threadIdx.x; i < k0; i += KT) {
           double lv[kNb];
 #pragma unroll
-          for (int t = 0; t < kNb; t++) lv[t] = L[static_cast<int64_t>(i) * lda + k0 + t];
+          for (int t = 0; t < kNb; t++) lv[t] = static_cast<double>(L[static_cast<int64_t>(i) * lda + k0 + t]);
           double v = y[i];
 #pragma unroll
           for (int t = 0; t < kNb; t++) v -= lv[t] * y[k0 + t];
@@ -449,7 +481,7 @@ __device__ void solve_domain(RbfSet r, int l, const double* __restrict__ pts, co
       } else {
         for (int i = threadIdx.x; i < k0; i += KT) {
           double v = y[i];
-          for (int t = 0; t < kb; t++) v -= L[static_cast<int64_t>(i) * lda + k0 + t] * y[k0 + t];
+          for (int t = 0; t < kb; t++) v -= static_cast<double>(L[static_cast<int64_t>(i) * lda + k0 + t]) * y[k0 + t];
           y[i] = v;
         }
       }
@@ -491,12 +523,12 @@ __device__ void solve_domain(RbfSet r, int l, const double* __restrict__ pts, co
 // Domain solves of one batch: gather d from res, scatter the owned entries
 // of the solution (every entry for the coarse grid) and the coarse grid's
 // polynomial coefficients into out.
-template <int KT>
+template <int KT, class TF>
 __global__ void __launch_bounds__(KT)
 ras_solve_kernel(RbfSet r, int l, const double* __restrict__ pts,
                  const int64_t* __restrict__ ptr, const int32_t* __restrict__ idx,
                  const uint8_t* __restrict__ inner, const double* __restrict__ lag,
-                 const int64_t* __restrict__ mat_ptr, const double* __restrict__ mat,
+                 const int64_t* __restrict__ mat_ptr, const TF* __restrict__ mat,
                  const double* __restrict__ res, double* __restrict__ out, int64_t mu, int coarse,
                  const double* __restrict__ pinv) {
   extern __shared__ double sm[];
@@ -688,6 +720,7 @@ std::vector<int32_t> order_by_axes(const int32_t* ids, int64_t n, const std::vec
     int32_t pos;
   };
   std::vector<Key> keys(static_cast<size_t>(n));
+#pragma omp parallel for schedule(static) if (parallel && n >= (int64_t{1} << 16))
   for (int64_t i = 0; i < n; i++) {
     const double* p = &pts[3 * static_cast<size_t>(ids[i])];
     keys[i] = Key{{p[ax[0]], p[ax[1]], p[ax[2]]}, static_cast<int32_t>(i)};
@@ -716,6 +749,7 @@ std::vector<int32_t> order_for_split(const int32_t* ids, int64_t n, const std::v
     int32_t pos;
   };
   std::vector<Key> keys(static_cast<size_t>(n));
+#pragma omp parallel for schedule(static) if (parallel && n >= (int64_t{1} << 16))
   for (int64_t i = 0; i < n; i++) {
     const double* p = &pts[3 * static_cast<size_t>(ids[i])];
     keys[i] = Key{{p[ax[0]], p[ax[1]], p[ax[2]]}, static_cast<int32_t>(i)};
@@ -784,9 +818,13 @@ DomainDivider::DomainDivider(const std::vector<double>& pts, const std::vector<i
     std::vector<Domain> next(static_cast<size_t>(2 * nw));
     std::vector<uint8_t> split(static_cast<size_t>(nw), 0);
     for (int64_t i = 0; i < nw; i++) split[i] = static_cast<int64_t>(work[i].idx.size()) > kMaxLeafSize;
-    if (nw < 8) {
+    if (nw == 1) {
+      if (split[0]) divide(work[0], next[0], next[1], true);
+    } else if (nw < 8) {
+      // the few large domains of the first levels, one thread each
+#pragma omp parallel for schedule(static, 1) num_threads(static_cast<int>(nw))
       for (int64_t i = 0; i < nw; i++)
-        if (split[i]) divide(work[i], next[2 * i], next[2 * i + 1], true);
+        if (split[i]) divide(work[i], next[2 * i], next[2 * i + 1], false);
     } else {
 #pragma omp parallel for schedule(dynamic, 1)
       for (int64_t i = 0; i < nw; i++)
@@ -837,6 +875,11 @@ DomainDivider::DomainDivider(const std::vector<double>& pts, const std::vector<i
 }
 
 std::vector<int32_t> DomainDivider::choose_coarse_points(int64_t n_coarse) const {
+  return pam::choose_coarse_points(pts_, point_idcs_, poly_idcs_, n_coarse);
+}
+
+std::vector<int32_t> choose_coarse_points(const std::vector<double>& pts, const std::vector<int32_t>& point_idcs,
+                                          const std::vector<int32_t>& poly_idcs, int64_t n_coarse) {
   // domain_divider.hpp:54-125: split the cluster of lowest level (largest bbox
   // volume first) at its median until n_coarse clusters exist; take the point
   // nearest each cluster's bbox centre.
@@ -845,37 +888,86 @@ std::vector<int32_t> DomainDivider::choose_coarse_points(int64_t n_coarse) const
   // then largest volume): every cluster of one level is split before any of
   // the next, so the split runs level by level; at the last level only the
   // largest clusters are split, up to n_coarse clusters in total. Clusters of
-  // one level are independent and handled in parallel. Each cluster carries
-  // its points' coordinates, so the passes over it read memory in order.
+  // one level are independent and handled in parallel. All points live in
+  // one array that the splits partition in place: a cluster is a range of it,
+  // its points' coordinates stored with them so the passes read memory in
+  // order.
   struct Rec {
     double x[3];
     int32_t id, pos;
   };
   struct Cluster {
-    int level = 0;
-    std::vector<Rec> pts;
+    int64_t b = 0, e = 0;
     double volume = 0.0;
     int32_t center = -1;
     std::array<int, 3> axes{0, 1, 2};
   };
+  std::vector<Rec> all;
+  all.reserve(point_idcs.size());
+  {
+    std::vector<int32_t> poly_sorted(poly_idcs);
+    std::sort(poly_sorted.begin(), poly_sorted.end());
+    for (int32_t i : point_idcs) {
+      if (std::binary_search(poly_sorted.begin(), poly_sorted.end(), i)) continue;
+      const double* p = &pts[3 * static_cast<size_t>(i)];
+      all.push_back(Rec{{p[0], p[1], p[2]}, i, 0});
+    }
+  }
+  // Bbox, volume, axes and the point nearest the bbox centre (the first in
+  // the range's order among equally near ones); large ranges in parallel with
+  // the same result.
   auto init = [&](Cluster& c) {
+    const int64_t n = c.e - c.b;
+    const Rec* r = all.data() + c.b;
     Bbox3 bb;
-    for (const Rec& r : c.pts) bb.add(r.x);
-    double cen[3], best = INFINITY;
+    if (n >= (int64_t{1} << 16)) {
+      double lo0 = INFINITY, lo1 = INFINITY, lo2 = INFINITY, hi0 = -INFINITY, hi1 = -INFINITY, hi2 = -INFINITY;
+#pragma omp parallel for schedule(static) reduction(min : lo0, lo1, lo2) reduction(max : hi0, hi1, hi2)
+      for (int64_t i = 0; i < n; i++) {
+        lo0 = std::min(lo0, r[i].x[0]);
+        lo1 = std::min(lo1, r[i].x[1]);
+        lo2 = std::min(lo2, r[i].x[2]);
+        hi0 = std::max(hi0, r[i].x[0]);
+        hi1 = std::max(hi1, r[i].x[1]);
+        hi2 = std::max(hi2, r[i].x[2]);
+      }
+      bb.lo[0] = lo0, bb.lo[1] = lo1, bb.lo[2] = lo2, bb.hi[0] = hi0, bb.hi[1] = hi1, bb.hi[2] = hi2;
+    } else {
+      for (int64_t i = 0; i < n; i++) bb.add(r[i].x);
+    }
+    double cen[3];
     c.volume = 1.0;
     for (int k = 0; k < 3; k++) {
       cen[k] = bb.lo[k] + (bb.hi[k] - bb.lo[k]) / 2.0;
       c.volume *= bb.hi[k] - bb.lo[k];
     }
-    c.center = c.pts.front().id;
-    for (const Rec& r : c.pts) {
-      double d2 = (r.x[0] - cen[0]) * (r.x[0] - cen[0]) + (r.x[1] - cen[1]) * (r.x[1] - cen[1]) +
-                  (r.x[2] - cen[2]) * (r.x[2] - cen[2]);
-      if (d2 < best) {
-        best = d2;
-        c.center = r.id;
+    auto d2 = [&](const Rec& q) {
+      return (q.x[0] - cen[0]) * (q.x[0] - cen[0]) + (q.x[1] - cen[1]) * (q.x[1] - cen[1]) +
+             (q.x[2] - cen[2]) * (q.x[2] - cen[2]);
+    };
+    int64_t best_i = 0;
+    if (n >= (int64_t{1} << 16)) {
+      double best = INFINITY;
+#pragma omp parallel
+      {
+        double tb = INFINITY;
+        int64_t ti = n;
+#pragma omp for schedule(static) nowait
+        for (int64_t i = 0; i < n; i++) {
+          const double d = d2(r[i]);
+          if (d < tb) tb = d, ti = i;
+        }
+#pragma omp critical
+        if (tb < best || (tb == best && ti < best_i)) best = tb, best_i = ti;
+      }
+    } else {
+      double best = INFINITY;
+      for (int64_t i = 0; i < n; i++) {
+        const double d = d2(r[i]);
+        if (d < best) best = d, best_i = i;
       }
     }
+    c.center = r[best_i].id;
     c.axes = bb.axes_by_width();
   };
   auto split_mid = [](int64_t sz) {
@@ -890,58 +982,57 @@ std::vector<int32_t> DomainDivider::choose_coarse_points(int64_t n_coarse) const
     }
     return mid;
   };
-  std::vector<int32_t> out(poly_idcs_);
-  Cluster root;
-  for (int32_t i : point_idcs_) {
-    if (std::find(poly_idcs_.begin(), poly_idcs_.end(), i) != poly_idcs_.end()) continue;
-    const double* p = &pts_[3 * static_cast<size_t>(i)];
-    root.pts.push_back(Rec{{p[0], p[1], p[2]}, i, 0});
-  }
-  if (root.pts.empty()) return out;
-  init(root);
-  std::vector<Cluster> cur;
-  cur.push_back(std::move(root));
+  std::vector<int32_t> out(poly_idcs);
+  if (all.empty()) return out;
+  std::vector<Cluster> cur(1);
+  cur[0].b = 0;
+  cur[0].e = static_cast<int64_t>(all.size());
+  init(cur[0]);
   std::vector<int32_t> done;  // centres of clusters that stay unsplit
   int64_t current = 1;
   while (current < n_coarse && !cur.empty()) {
     std::stable_sort(cur.begin(), cur.end(), [](const Cluster& x, const Cluster& y) { return x.volume > y.volume; });
     int64_t splittable = 0;
-    for (const auto& c : cur) splittable += c.pts.size() > 1;
+    for (const auto& c : cur) splittable += c.e - c.b > 1;
     int64_t n_split = std::min<int64_t>(splittable, n_coarse - current);
     std::vector<int64_t> which;
     for (int64_t i = 0; i < static_cast<int64_t>(cur.size()) && static_cast<int64_t>(which.size()) < n_split; i++)
-      if (cur[i].pts.size() > 1) which.push_back(i);
+      if (cur[i].e - cur[i].b > 1) which.push_back(i);
     const int64_t nw = static_cast<int64_t>(which.size());
     std::vector<Cluster> next(static_cast<size_t>(2 * nw));
     auto do_split = [&](int64_t w) {
-      Cluster& c = cur[which[w]];
-      const int64_t sz = static_cast<int64_t>(c.pts.size());
+      const Cluster& c = cur[which[w]];
+      const int64_t sz = c.e - c.b;
       const int64_t mid = split_mid(sz);
       // The first mid points in the order of coordinates along the cluster's
       // axes (ties by position) form the left half; only the membership of the
       // halves matters, so a selection replaces the full sort.
-      for (int64_t i = 0; i < sz; i++) c.pts[i].pos = static_cast<int32_t>(i);
+      Rec* r = all.data() + c.b;
+      for (int64_t i = 0; i < sz; i++) r[i].pos = static_cast<int32_t>(i);
       const auto ax = c.axes;
       auto less = [&ax](const Rec& p, const Rec& q) {
         for (int k : ax)
           if (p.x[k] != q.x[k]) return p.x[k] < q.x[k];
         return p.pos < q.pos;
       };
-      std::nth_element(c.pts.begin(), c.pts.begin() + mid, c.pts.end(), less);
+      std::nth_element(r, r + mid, r + sz, less);
       // (the order inside a half only breaks exact ties of the nearest-to-centre search)
       Cluster& l = next[2 * w];
-      Cluster& r = next[2 * w + 1];
-      l.level = r.level = c.level + 1;
-      l.pts.assign(c.pts.begin(), c.pts.begin() + mid);
-      r.pts.assign(c.pts.begin() + mid, c.pts.end());
-      std::vector<Rec>().swap(c.pts);
-      if (!l.pts.empty()) init(l);
-      if (!r.pts.empty()) init(r);
+      Cluster& rt = next[2 * w + 1];
+      l.b = c.b;
+      l.e = rt.b = c.b + mid;
+      rt.e = c.e;
+      if (l.e > l.b) init(l);
+      if (rt.e > rt.b) init(rt);
     };
-    if (nw < 8) {
+    if (nw == 1) {
+      do_split(0);  // its passes over the points run in parallel
+    } else if (nw < 8) {
+      // the few large clusters of the first levels, one thread each
+#pragma omp parallel for schedule(static, 1) num_threads(static_cast<int>(nw))
       for (int64_t w = 0; w < nw; w++) do_split(w);
     } else {
-#pragma omp parallel for schedule(dynamic, 4)
+#pragma omp parallel for schedule(guided)
       for (int64_t w = 0; w < nw; w++) do_split(w);
     }
     std::vector<uint8_t> was_split(cur.size(), 0);
@@ -949,8 +1040,9 @@ std::vector<int32_t> DomainDivider::choose_coarse_points(int64_t n_coarse) const
     for (size_t i = 0; i < cur.size(); i++)
       if (!was_split[i]) done.push_back(cur[i].center);
     std::vector<Cluster> lvl;
+    lvl.reserve(next.size());
     for (auto& c : next)
-      if (!c.pts.empty()) lvl.push_back(std::move(c));
+      if (c.e > c.b) lvl.push_back(c);
     current += nw;
     cur.swap(lvl);
     if (nw == 0) break;
@@ -1011,7 +1103,8 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
       for (int k = 0; k < 3; k++) a_pts[3 * i + k] = a[3 * k] * p[0] + a[3 * k + 1] * p[1] + a[3 * k + 2] * p[2];
     }
   }
-  flag_.resize(1);
+  // one positive-definiteness flag per level (0 = the coarse grid)
+  flag_.resize(static_cast<size_t>(n_levels_));
   flag_.zero(s_);
   fine_.clear();
   fine_.resize(n_levels_);
@@ -1027,17 +1120,30 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
   // exist; the host then picks the coarser level's points and divides it
   // while the device assembles and factors.
   for (int level = n_levels_ - 1; level >= 1; level--) {
-    DomainDivider div(a_pts, point_idcs_[level], poly_idcs);
-    tick("divide");
-    setup_batch(fine_[level], div.domains(), false, fork_stream());
-    tick("batch launched");
     double finest = std::log(static_cast<double>(mu_)) / std::log(kFineToCoarseRatio);
     double coarsest = std::log(static_cast<double>(kNCoarsestPoints)) / std::log(kFineToCoarseRatio);
     auto n_coarse = static_cast<int64_t>(std::pow(
         kFineToCoarseRatio, coarsest + (level - 1) * (finest - coarsest) / (n_levels_ - 1)));
-    point_idcs_[level - 1] = div.choose_coarse_points(n_coarse);
+    // The coarser level's points depend on this level's points only, not on
+    // its domains: they are chosen on a second host thread while this one
+    // divides the level and launches its batch.
+    const std::vector<int32_t>& lp = point_idcs_[level];
+    auto coarse = std::async(std::launch::async,
+                             [&] { return choose_coarse_points(a_pts, lp, poly_idcs, n_coarse); });
+    std::vector<Domain> doms;
+    try {
+      DomainDivider div(a_pts, lp, poly_idcs);
+      doms = std::move(div.domains());
+    } catch (...) {
+      coarse.wait();
+      throw;
+    }
+    tick("divide");
+    setup_batch(fine_[level], doms, false, fork_stream(), opt_.fp32_factors, flag_.get() + level);
+    tick("batch launched");
+    point_idcs_[level - 1] = coarse.get();
     tick("coarse points");
-    h_domains_[level] = std::move(div.domains());
+    h_domains_[level] = std::move(doms);
   }
   {
     Domain cd;
@@ -1045,7 +1151,7 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
     cd.inner.assign(cd.idx.size(), 1);
     h_domains_[0] = {cd};
     hipStream_t cs = fork_stream();
-    setup_batch(coarse_, h_domains_[0], true, cs);
+    setup_batch(coarse_, h_domains_[0], true, cs, false, flag_.get());
     // The coarse solve is one domain: solved by a single workgroup it would
     // serialize on one CU, so it is turned into a dense operator once (one
     // workgroup per unit vector) and applied as a matrix-vector product.
@@ -1061,14 +1167,29 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
     }
   }
   join_streams();
-  int32_t flag = 0;
-  PAM_HIP(hipMemcpyAsync(&flag, flag_.get(), sizeof flag, hipMemcpyDeviceToHost, s_));
+  std::vector<int32_t> flags(static_cast<size_t>(n_levels_), 0);
+  PAM_HIP(hipMemcpyAsync(flags.data(), flag_.get(), sizeof(int32_t) * flags.size(), hipMemcpyDeviceToHost, s_));
   PAM_HIP(hipStreamSynchronize(s_));
   setup_tmp_.clear();
+  // A single-precision level that met a non-positive pivot is factored again in FP64.
+  bool refactored = false;
+  for (int level = 1; level < n_levels_; level++) {
+    if (!flags[level] || !fine_[level].fp32) continue;
+    if (opt_.verbose) std::printf("  ras setup level %d: FP32 factorisation failed, refactoring in FP64\n", level);
+    flag_.zero(s_);
+    setup_batch(fine_[level], h_domains_[level], false, s_, false, flag_.get() + level);
+    int32_t f = 0;
+    PAM_HIP(hipMemcpyAsync(&f, flag_.get() + level, sizeof f, hipMemcpyDeviceToHost, s_));
+    PAM_HIP(hipStreamSynchronize(s_));
+    flags[level] = f;
+    refactored = true;
+  }
+  if (refactored) setup_tmp_.clear();
   for (hipStream_t ls : setup_streams_) PAM_HIP(hipStreamDestroy(ls));
   setup_streams_.clear();
   tick("factors done");
-  if (flag) throw_runtime("a subdomain matrix Q^T A Q is not positive definite");
+  for (int32_t f : flags)
+    if (f) throw_runtime("a subdomain matrix Q^T A Q is not positive definite");
   d_point_idcs_.resize(n_levels_);
   for (int level = 0; level < n_levels_; level++)
     d_point_idcs_[level].upload(point_idcs_[level].data(), point_idcs_[level].size(), s_);
@@ -1118,7 +1239,8 @@ void RasPreconditioner::join_streams() {
   }
 }
 
-void RasPreconditioner::setup_batch(DomainBatch& b, const std::vector<Domain>& doms, bool coarse, hipStream_t ls) {
+void RasPreconditioner::setup_batch(DomainBatch& b, const std::vector<Domain>& doms, bool coarse, hipStream_t ls,
+                                    bool fp32, int32_t* flag) {
   (void)coarse;
   b.n_dom = static_cast<int>(doms.size());
   b.h_ptr.assign(b.n_dom + 1, 0);
@@ -1144,7 +1266,14 @@ void RasPreconditioner::setup_batch(DomainBatch& b, const std::vector<Domain>& d
   b.inner.upload(tmp->inner.data(), tmp->inner.size(), ls);
   b.ptr.upload(b.h_ptr.data(), b.h_ptr.size(), ls);
   b.mat_ptr.upload(tmp->mptr.data(), tmp->mptr.size(), ls);
-  b.mat.resize(static_cast<size_t>(std::max<int64_t>(tmp->mptr.back(), 1)));
+  b.fp32 = fp32;
+  if (fp32) {
+    b.mat32.resize(static_cast<size_t>(std::max<int64_t>(tmp->mptr.back(), 1)));
+    b.mat.release();
+  } else {
+    b.mat.resize(static_cast<size_t>(std::max<int64_t>(tmp->mptr.back(), 1)));
+    b.mat32.release();
+  }
   RbfSet rs = make_rbf_set(model_);
   tmp->u.resize(static_cast<size_t>(std::max<int64_t>(l_ * (b.h_ptr.back() - static_cast<int64_t>(b.n_dom) * l_), 1)));
   tmp->atl.resize(static_cast<size_t>(std::max(b.n_dom * l_ * l_, 1)));
@@ -1153,15 +1282,26 @@ void RasPreconditioner::setup_batch(DomainBatch& b, const std::vector<Domain>& d
                                                         tmp->u.get(), tmp->atl.get());
     PAM_LAUNCH_CHECK();
   }
-  assemble_m_kernel<<<dim3(64, b.n_dom), kT, 0, ls>>>(rs, l_, d_pts_.get(), b.ptr.get(), b.idx.get(), lag_.get(),
-                                                       tmp->u.get(), tmp->atl.get(), b.mat_ptr.get(), b.mat.get());
+  if (fp32) {
+    assemble_m_kernel<float><<<dim3(64, b.n_dom), kT, 0, ls>>>(rs, l_, d_pts_.get(), b.ptr.get(), b.idx.get(),
+                                                              lag_.get(), tmp->u.get(), tmp->atl.get(),
+                                                              b.mat_ptr.get(), b.mat32.get());
+    PAM_LAUNCH_CHECK();
+    cholesky_kernel<float><<<b.n_dom, kT, 0, ls>>>(b.ptr.get(), l_, b.mat_ptr.get(), b.mat32.get(), flag);
+    PAM_LAUNCH_CHECK();
+    setup_tmp_.push_back(std::move(tmp));
+    return;
+  }
+  assemble_m_kernel<double><<<dim3(64, b.n_dom), kT, 0, ls>>>(rs, l_, d_pts_.get(), b.ptr.get(), b.idx.get(),
+                                                               lag_.get(), tmp->u.get(), tmp->atl.get(),
+                                                               b.mat_ptr.get(), b.mat.get());
   PAM_LAUNCH_CHECK();
   const int n0 = static_cast<int>(b.h_ptr.back() - l_);
   if (b.n_dom == 1 && n0 > 4 * kTile) {
     // One large matrix (the coarse grid): a workgroup per trailing tile
     // instead of one workgroup for the whole factorisation.
     for (int k0 = 0; k0 < n0; k0 += kNb) {
-      cholesky_panel_kernel<<<1, kT, 0, ls>>>(b.mat.get(), n0, k0, flag_.get());
+      cholesky_panel_kernel<<<1, kT, 0, ls>>>(b.mat.get(), n0, k0, flag);
       PAM_LAUNCH_CHECK();
       const int rem = n0 - std::min(kNb, n0 - k0) - k0;
       if (rem <= 0) break;
@@ -1170,7 +1310,7 @@ void RasPreconditioner::setup_batch(DomainBatch& b, const std::vector<Domain>& d
       PAM_LAUNCH_CHECK();
     }
   } else {
-    cholesky_kernel<<<b.n_dom, kT, 0, ls>>>(b.ptr.get(), l_, b.mat_ptr.get(), b.mat.get(), flag_.get());
+    cholesky_kernel<double><<<b.n_dom, kT, 0, ls>>>(b.ptr.get(), l_, b.mat_ptr.get(), b.mat.get(), flag);
     PAM_LAUNCH_CHECK();
   }
   // b.h_ptr is a member and outlives the upload; the rest stays until the
@@ -1180,7 +1320,7 @@ void RasPreconditioner::setup_batch(DomainBatch& b, const std::vector<Domain>& d
 
 int64_t RasPreconditioner::factor_bytes() const {
   int64_t b = static_cast<int64_t>(coarse_.mat.bytes());
-  for (const auto& f : fine_) b += static_cast<int64_t>(f.mat.bytes());
+  for (const auto& f : fine_) b += static_cast<int64_t>(f.mat.bytes() + f.mat32.bytes());
   return b;
 }
 
@@ -1206,9 +1346,17 @@ void RasPreconditioner::launch_solve(const DomainBatch& b, int level, int thread
   RbfSet rs = make_rbf_set(model_);
   auto go = [&](auto kt) {
     constexpr int KT = decltype(kt)::value;
-    ras_solve_kernel<KT><<<b.n_dom, KT, shmem, s_>>>(rs, l_, d_pts_.get(), b.ptr.get(), b.idx.get(),
-                                                     b.inner.get(), lag_.get(), b.mat_ptr.get(), b.mat.get(),
-                                                     d_res, d_w, mu_, level == 0 ? 1 : 0, pinv_.get());
+    if (b.fp32) {
+      ras_solve_kernel<KT, float><<<b.n_dom, KT, shmem, s_>>>(rs, l_, d_pts_.get(), b.ptr.get(), b.idx.get(),
+                                                              b.inner.get(), lag_.get(), b.mat_ptr.get(),
+                                                              b.mat32.get(), d_res, d_w, mu_, level == 0 ? 1 : 0,
+                                                              pinv_.get());
+    } else {
+      ras_solve_kernel<KT, double><<<b.n_dom, KT, shmem, s_>>>(rs, l_, d_pts_.get(), b.ptr.get(), b.idx.get(),
+                                                               b.inner.get(), lag_.get(), b.mat_ptr.get(),
+                                                               b.mat.get(), d_res, d_w, mu_, level == 0 ? 1 : 0,
+                                                               pinv_.get());
+    }
   };
   switch (threads) {
     case 128: go(std::integral_constant<int, 128>{}); break;

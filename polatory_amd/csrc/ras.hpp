@@ -35,6 +35,11 @@ class DomainDivider {
   std::vector<Domain> domains_;
 };
 
+// DomainDivider::choose_coarse_points (domain_divider.hpp:54-125) as a free
+// function of the level's points, so that it can run beside the division.
+std::vector<int32_t> choose_coarse_points(const std::vector<double>& pts, const std::vector<int32_t>& point_idcs,
+                                          const std::vector<int32_t>& poly_idcs, int64_t n_coarse);
+
 // Batched subdomain factors of one level (or the coarse grid as one domain).
 struct DomainBatch {
   int n_dom = 0;
@@ -43,7 +48,9 @@ struct DomainBatch {
   DevBuf<int32_t> idx;
   DevBuf<uint8_t> inner;
   DevBuf<int64_t> ptr, mat_ptr;          // [n_dom+1]
-  DevBuf<double> mat;                    // lower Cholesky factors, column-major
+  DevBuf<double> mat;                    // lower Cholesky factors, column-major (FP64)
+  DevBuf<float> mat32;                   // the same in FP32 (single-precision fine levels)
+  bool fp32 = false;                     // the factors are mat32
   DevBuf<double> op;                     // coarse grid: its solve as a dense (m + l) x m operator
   bool has_op = false;
   int solve_threads = 0;                 // workgroup size of the batched solve (0: default)
@@ -57,6 +64,11 @@ struct RasOptions {
   // keep the fastest (off: the rule below, which reproduces what the timing
   // picked on the 1M-point fit without its cost).
   bool tune_solves = false;
+  // Fine-level factors in FP32 (assembled in FP64, factored on FP32 MFMA,
+  // solved with FP64 arithmetic on the FP32 factors); the coarse grid stays
+  // FP64. A level whose FP32 factorisation meets a non-positive pivot is
+  // factored again in FP64.
+  bool fp32_factors = false;
 };
 
 class RasPreconditioner : public LinearOperator {
@@ -78,7 +90,8 @@ class RasPreconditioner : public LinearOperator {
  private:
   // Assembles and factors a batch asynchronously; its temporaries stay in
   // setup_tmp_ until the end of the constructor.
-  void setup_batch(DomainBatch& b, const std::vector<Domain>& doms, bool coarse, hipStream_t ls);
+  void setup_batch(DomainBatch& b, const std::vector<Domain>& doms, bool coarse, hipStream_t ls, bool fp32,
+                   int32_t* flag);
   // Set-up streams: each level's batch (assembly, factorisation) runs on its
   // own stream, forked from and joined back into s_.
   hipStream_t fork_stream();

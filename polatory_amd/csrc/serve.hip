@@ -61,11 +61,17 @@ struct Server::Request {
 
 struct Server::Slot {
   hipStream_t s = nullptr;
-  std::vector<std::unique_ptr<FmmEvaluator>> ev;
-  DevBuf<double> d_pts, d_out, d_tmp;
+  // Resources of one request of a shared FMM batch (position in the batch).
+  struct Pos {
+    std::vector<std::unique_ptr<FmmEvaluator>> ev;  // one per rbf, attached to the model's source side
+    std::vector<DeferredFmm> jobs;                   // their recorded near and far fields
+    DevBuf<double> d_pts, d_out;
+    std::vector<DevBuf<double>> d_tmp;               // per rbf after the first
+  };
+  std::vector<std::unique_ptr<Pos>> pos;
   DirectBatch direct;
   ~Slot() {
-    ev.clear();
+    pos.clear();
     if (s) {
       (void)hipStreamSynchronize(s);
       (void)hipStreamDestroy(s);
@@ -290,6 +296,25 @@ void Server::worker(int slot) {
       }
       admitted_ += r->est;
       batch.push_back(r);
+      if (!batchable(*r) && r->n > 0) {
+        // Requests for the FMM: queued ones of any model join this batch, in
+        // queue order, up to kMaxFmmBatch requests and kMaxFmmBatchPoints
+        // query points, as far as the HBM budget allows; their near fields and
+        // far-field L2Ps then run in shared launches.
+        const int64_t room = opt_.hbm_budget - resident_;
+        int64_t pts = r->n;
+        for (auto it = queue_.begin(); it != queue_.end() && batch.size() < kMaxFmmBatch;) {
+          Request* q = *it;
+          if (!batchable(*q) && q->n > 0 && pts + q->n <= kMaxFmmBatchPoints && admitted_ + q->est <= room) {
+            admitted_ += q->est;
+            pts += q->n;
+            batch.push_back(q);
+            it = queue_.erase(it);
+          } else {
+            ++it;
+          }
+        }
+      }
       if (batchable(*r)) {
         // Continuous batching: the queued requests that are also answered by
         // the direct sum join this launch, whatever their model, in queue
@@ -318,7 +343,7 @@ void Server::worker(int slot) {
     std::string err;
     try {
       if (batchable(*r)) run_batch(sl, batch);
-      else run_request(sl, *r);
+      else run_fmm_batch(sl, batch);
     } catch (const Error& e) {
       code = e.code;
       err = e.what();
@@ -333,6 +358,10 @@ void Server::worker(int slot) {
         batches_++;
         batched_requests_ += static_cast<int64_t>(batch.size());
         max_batch_ = std::max(max_batch_, static_cast<int>(batch.size()));
+      } else if (r->n > 0) {
+        fmm_batches_++;
+        fmm_batched_requests_ += static_cast<int64_t>(batch.size());
+        max_fmm_batch_ = std::max(max_fmm_batch_, static_cast<int>(batch.size()));
       }
       for (Request* q : batch) {
         admitted_ -= q->est;
@@ -357,30 +386,56 @@ void Server::worker(int slot) {
   }
 }
 
-void Server::run_request(Slot& sl, Request& r) {
-  const int64_t n = r.n;
-  if (n == 0) return;
-  ModelEntry& m = *r.model;
+void Server::run_fmm_batch(Slot& sl, const std::vector<Request*>& batch) {
+  // Per request and rbf: the target tree, lists and far field up to L2L on
+  // the slot's stream; then the near fields of the whole batch in shared
+  // launches and its far-field L2Ps in shared launches (launch_deferred),
+  // whatever the models; then each request's rbf terms, polynomial and
+  // download. Every output gets the bits of the same request served alone.
   hipStream_t s = sl.s;
-  sl.d_pts.upload(r.pts, static_cast<size_t>(n) * 3, s);
-  sl.d_out.resize(static_cast<size_t>(n));
-  const size_t nr = m.owners.size();
-  if (nr > 1) sl.d_tmp.resize(static_cast<size_t>(n));
-  while (sl.ev.size() < nr) {
-    sl.ev.push_back(std::make_unique<FmmEvaluator>(m.owners[sl.ev.size()]->rbf(), nullptr, nullptr, false, s));
-    // Requests overlap across slots; a second stream per slot evaluator would
-    // only add hardware-queue sharing between the slots.
-    sl.ev.back()->set_overlap(false);
+  while (sl.pos.size() < batch.size()) sl.pos.push_back(std::make_unique<Slot::Pos>());
+  std::vector<const DeferredFmm*> jobs;
+  for (size_t b = 0; b < batch.size(); b++) {
+    Request& r = *batch[b];
+    const int64_t n = r.n;
+    if (n == 0) continue;
+    ModelEntry& m = *r.model;
+    Slot::Pos& p = *sl.pos[b];
+    const size_t nr = m.owners.size();
+    p.d_pts.upload(r.pts, static_cast<size_t>(n) * 3, s);
+    p.d_out.resize(static_cast<size_t>(n));
+    if (p.d_tmp.size() < nr) p.d_tmp.resize(nr);
+    if (p.jobs.size() < nr) p.jobs.resize(nr);
+    while (p.ev.size() < nr) {
+      p.ev.push_back(std::make_unique<FmmEvaluator>(m.owners[p.ev.size()]->rbf(), nullptr, nullptr, false, s));
+      // Requests overlap across slots; a second stream per slot evaluator would
+      // only add hardware-queue sharing between the slots.
+      p.ev.back()->set_overlap(false);
+    }
+    for (size_t k = 0; k < nr; k++) {
+      FmmEvaluator& e = *p.ev[k];
+      e.attach_source(m.owners[k].get());
+      e.set_target_points(p.d_pts.get(), n, true);
+      double* o = p.d_out.get();
+      if (k > 0) {
+        p.d_tmp[k].resize(static_cast<size_t>(n));
+        o = p.d_tmp[k].get();
+      }
+      e.evaluate_deferred(o, n, &p.jobs[k]);
+      if (p.jobs[k].valid) jobs.push_back(&p.jobs[k]);
+    }
   }
-  for (size_t k = 0; k < nr; k++) {
-    FmmEvaluator& e = *sl.ev[k];
-    e.attach_source(m.owners[k].get());
-    e.set_target_points(sl.d_pts.get(), n, true);
-    e.evaluate(k == 0 ? sl.d_out.get() : sl.d_tmp.get(), n, true);
-    if (k > 0) dev_axpy(1.0, sl.d_tmp.get(), sl.d_out.get(), n, s);
+  if (!jobs.empty()) launch_deferred(jobs, s);
+  for (size_t b = 0; b < batch.size(); b++) {
+    Request& r = *batch[b];
+    const int64_t n = r.n;
+    if (n == 0) continue;
+    ModelEntry& m = *r.model;
+    Slot::Pos& p = *sl.pos[b];
+    for (size_t k = 1; k < m.owners.size(); k++) dev_axpy(1.0, p.d_tmp[k].get(), p.d_out.get(), n, s);
+    if (m.model.l() > 0) launch_poly_eval(m.model.poly_degree, p.d_pts.get(), n, m.d_coeffs.get(), p.d_out.get(), s);
+    p.d_out.download(r.out, static_cast<size_t>(n), s);
   }
-  if (m.model.l() > 0) launch_poly_eval(m.model.poly_degree, sl.d_pts.get(), n, m.d_coeffs.get(), sl.d_out.get(), s);
-  sl.d_out.download(r.out, static_cast<size_t>(n), s);
   PAM_HIP(hipStreamSynchronize(s));
 }
 
@@ -461,6 +516,9 @@ ServeStats Server::stats() {
   st.batches = batches_;
   st.batched_requests = batched_requests_;
   st.max_batch = max_batch_;
+  st.fmm_batches = fmm_batches_;
+  st.fmm_batched_requests = fmm_batched_requests_;
+  st.max_fmm_batch = max_fmm_batch_;
   return st;
 }
 
@@ -475,6 +533,8 @@ void Server::reset_stats() {
   completed_ = failed_ = submitted_ = points_ = 0;
   batches_ = batched_requests_ = 0;
   max_batch_ = 0;
+  fmm_batches_ = fmm_batched_requests_ = 0;
+  max_fmm_batch_ = 0;
   first_submit_ = -1;
   last_done_ = 0;
   max_in_flight_ = in_flight_;

@@ -9,9 +9,13 @@
 // budget holds their working set. Every slot is a worker thread with its own
 // stream whose evaluators attach to the model's source side, so the
 // requests in flight overlap on the device while sharing the resident data.
-// Requests that the FP64 direct sum answers (few query points against the
-// model's centres) are batched: a slot takes the queued ones of any model
-// into one shared direct-sum launch and one reduction.
+// Requests are batched across models. Those that the FP64 direct sum answers
+// (few query points against the model's centres): a slot takes the queued
+// ones of any model into one shared direct-sum launch and one reduction.
+// Those that need the FMM: a slot takes up to four queued ones of any model,
+// builds each one's target tree, lists and far field up to L2L, then runs the
+// near fields of the whole batch in shared launches and the far-field L2Ps in
+// shared launches (one per precision and kernel kind, one per order).
 #pragma once
 
 #include <atomic>
@@ -44,6 +48,10 @@ struct ServeStats {
   // requests in one
   int64_t batches = 0, batched_requests = 0;
   int max_batch = 0;
+  // FMM batches (shared near-field and far-field launches), the requests in
+  // them, and the most requests in one
+  int64_t fmm_batches = 0, fmm_batched_requests = 0;
+  int max_fmm_batch = 0;
 };
 
 class Server {
@@ -79,13 +87,16 @@ class Server {
   struct Slot;
   void worker(int slot);
   int64_t estimate_bytes(const ModelEntry& m, int64_t n) const;
-  void run_request(Slot& slot, Request& r);
+  void run_fmm_batch(Slot& slot, const std::vector<Request*>& batch);
   void run_batch(Slot& slot, const std::vector<Request*>& batch);
   static bool batchable(const Request& r);
 
   // Limits of one shared direct-sum launch.
   static constexpr size_t kMaxBatch = 64;
   static constexpr int64_t kMaxBatchRows = int64_t{1} << 19;
+  // Limits of one FMM batch (shared near-field and L2P launches).
+  static constexpr size_t kMaxFmmBatch = 4;
+  static constexpr int64_t kMaxFmmBatchPoints = int64_t{1} << 21;
 
   ServeOptions opt_;
   std::vector<std::unique_ptr<ModelEntry>> models_;
@@ -104,6 +115,8 @@ class Server {
   int64_t completed_ = 0, failed_ = 0, submitted_ = 0, points_ = 0;
   int64_t batches_ = 0, batched_requests_ = 0;
   int max_batch_ = 0;
+  int64_t fmm_batches_ = 0, fmm_batched_requests_ = 0;
+  int max_fmm_batch_ = 0;
   double first_submit_ = -1, last_done_ = 0;
 };
 

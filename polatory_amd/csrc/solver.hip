@@ -145,6 +145,10 @@ Solver::Solver(const Model& model, const std::vector<double>& points, const FitO
     ro.use_level_cfg = opt_.use_level_cfg;
     ro.verbose = opt_.verbose;
     ro.tune_solves = opt_.tune_solves;
+    // Triharmonic (CPD order 2) subdomain matrices are too ill-conditioned for
+    // FP32 factors; an FP64 operator asks for the FP64 fit throughout.
+    ro.fp32_factors = opt_.ras_fp32_factors == 1 ||
+                      (opt_.ras_fp32_factors == -1 && model_.cpd_order() <= 1 && opt_.op_cfg.precision != kFp64);
     pc_ = std::make_unique<RasPreconditioner>(model_, points, s_, ro);
   }
   if (l_ > 0) {

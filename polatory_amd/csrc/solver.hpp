@@ -83,6 +83,9 @@ struct FitOptions {
   bool keep_unconverged = false;
   // Time the RAS solve workgroup sizes per level at set-up (RasOptions::tune_solves).
   bool tune_solves = false;
+  // RAS fine-level factors in FP32 (RasOptions::fp32_factors): 1 on, 0 off,
+  // -1 on when the operator is not FP64 and every rbf has CPD order <= 1.
+  int ras_fp32_factors = -1;
 };
 
 struct FitStats {

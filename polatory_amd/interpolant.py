@@ -92,7 +92,7 @@ class FitOptions:
 
     def __init__(self, op=None, residual=None, level=None, use_preconditioner=True,
                  refine_rtol=None, verbose=False, use_graphs=True, keep_unconverged=False,
-                 tune_solves=False):
+                 tune_solves=False, ras_fp32_factors=None):
         d = _capi.PamFitOptions()
         _capi.call("pam_fit_options_default", ctypes.byref(d))
         self._c = d
@@ -110,6 +110,8 @@ class FitOptions:
         d.use_graphs = 1 if use_graphs else 0
         d.keep_unconverged = 1 if keep_unconverged else 0
         d.tune_solves = 1 if tune_solves else 0
+        if ras_fp32_factors is not None:  # None: the library's rule (-1)
+            d.ras_fp32_factors = 1 if ras_fp32_factors else 0
 
 
 class Interpolant:

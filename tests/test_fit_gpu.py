@@ -334,14 +334,16 @@ def test_operator_and_preconditioner_as_device_linear_operators(orc):
     model = pa.Model(pa.Biharmonic3D(), 1, nugget=0.001)
     x = np.concatenate([rng.normal(size=n), rng.normal(size=4)])
     ref = orc.direct_operator(orc_rbfs(orc, model), 1, 0.001, pts, x)
-    for cfg, tol in ((pa.FmmConfig("fp64", 10, -1, "direct"), 1e-13), (pa.FmmConfig("fp64", 10, -1, "fmm"), 1e-8),
-                     (None, 1e-5)):
+    ops = {}
+    for name, cfg, tol in (("direct", pa.FmmConfig("fp64", 10, -1, "direct"), 1e-13),
+                           ("fp64", pa.FmmConfig("fp64", 10, -1, "fmm"), 1e-8), ("default", None, 1e-5)):
         op = pa.Operator(model, pts, cfg)
         assert op.size() == n + 4
         y = op(x)
         err = np.abs(y - ref).max() / np.abs(ref).max()
         log(f"  operator {cfg}: rel err {err:.2e}")
         assert err <= tol
+        ops[name] = op
     dev = torch.device("cuda:0")
     dx = torch.from_numpy(x).to(dev)
     dy = torch.empty_like(dx)
@@ -355,7 +357,9 @@ def test_operator_and_preconditioner_as_device_linear_operators(orc):
     pc.apply_device(dx.data_ptr(), dz.data_ptr())
     pc.synchronize()
     np.testing.assert_array_equal(dz.cpu().numpy(), pc.apply(x))
-    # FGMRES(30) with right preconditioning, both applies on device vectors
+    # FGMRES(30) with right preconditioning, both applies on device vectors,
+    # with the FP64 FMM operator (the true residual then follows the Krylov one)
+    op = ops["fp64"]
     b = np.concatenate([sample_values(pts), np.zeros(4)])
     beta = np.linalg.norm(b)
     V = [b / beta]
@@ -383,3 +387,26 @@ def test_operator_and_preconditioner_as_device_linear_operators(orc):
     true = orc.direct_operator(orc_rbfs(orc, model), 1, 0.001, pts, sol)
     log(f"  FGMRES: {j + 1} steps, true relative residual {np.linalg.norm(true - b) / beta:.2e}")
     assert np.linalg.norm(true - b) <= 1e-6 * beta
+
+
+@pytest.mark.parametrize("kind,fp32", [("bh3", True), ("bh3", False), ("th3", True)])
+def test_fit_with_single_precision_ras_factors(orc, kind, fp32):
+    """RAS fine-level factors in FP32 (FP32 MFMA factorisation, FP64 solve
+    arithmetic) or FP64: the fit reaches its tolerance on the oracle's FP64
+    residual either way. Triharmonic subdomain matrices may meet a
+    non-positive FP32 pivot; that level is then factored in FP64."""
+    log("fit ras factors", kind, "fp32" if fp32 else "fp64")
+    rng = np.random.default_rng(28)
+    n = 20000 if kind == "bh3" else 8000
+    pts = rng.uniform(0, 1, (n, 3))
+    vals = sample_values(pts)
+    rbf = pa.Biharmonic3D() if kind == "bh3" else pa.Triharmonic3D()
+    model = pa.Model(rbf, 1)
+    ip = pa.Interpolant(model)
+    ip.fit(pts, vals, 1e-5, max_iter=100, options=pa.FitOptions(ras_fp32_factors=fp32))
+    st = ip.fit_stats()
+    res = oracle_residual(orc, model, pts, vals, ip.weights())
+    log(f"  iterations {st['iterations']} levels {st['n_levels']} factor GB {st['factor_bytes'] / 1e9:.3f} "
+        f"residual(oracle FP64) {res:.2e}")
+    assert st["n_levels"] >= 2
+    assert res <= 1e-5

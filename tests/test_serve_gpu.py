@@ -164,3 +164,30 @@ def test_small_requests_share_launches(orc, served):
         err = np.abs(mixed[i] - ref).max() / max(np.abs(ref).max(), 1e-300)
         # FP64 sums in another order than the oracle's: rounding only
         assert err <= 1e-10, f"request {i}: rel err {err:.2e} vs the FP64 direct sum"
+
+
+def test_fmm_requests_share_near_and_far_field_launches(orc, served):
+    """FMM-sized requests of several models queued together are served in
+    batches whose near fields and far-field L2Ps run in shared launches; every
+    answer is the same bits as the same request served alone, and within the
+    accuracy of the oracle's FP64 direct sum."""
+    srv, models, ids = served
+    rng = np.random.default_rng(91)
+    reqs = []
+    for k in range(12):
+        m = k % len(ids)
+        reqs.append((m, rng.uniform(0, 1, (int(rng.integers(30_000, 120_000)), 3))))
+    srv.reset_stats()
+    rids = [srv.submit(ids[m], q) for m, q in reqs]
+    mixed = [srv.wait(r) for r in rids]
+    st = srv.stats()
+    log(f"serve: {len(reqs)} FMM requests in {st['fmm_batches']} batches (largest {st['max_fmm_batch']})")
+    assert st["fmm_batched_requests"] == len(reqs)
+    assert st["max_fmm_batch"] >= 2, "no two FMM requests shared a launch"
+    for i, (m, q) in enumerate(reqs):
+        alone = srv.wait(srv.submit(ids[m], q))
+        assert np.array_equal(alone, mixed[i]), f"request {i} differs alone vs in an FMM batch"
+        sub = np.arange(300)
+        ref = reference(orc, models[m], q[sub])
+        err = np.abs(mixed[i][sub] - ref).max() / np.abs(ref).max()
+        assert err <= REL_TOL, f"request {i}: rel err {err:.2e}"
