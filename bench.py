@@ -291,11 +291,9 @@ def serve_section(pa, n_models, n_requests, rate):
         p = np.random.default_rng(1000 + i).uniform(0, 1, (n, 3))
         rbf = pa.Biharmonic3D() if k == 0 else pa.Triharmonic3D()
         ip = pa.Interpolant(pa.Model(rbf, poly_degree=1))
-        # Large triharmonic systems converge slowly under the multilevel RAS; a
-        # model that has not reached 1e-4 after 30 steps is served with the
-        # weights reached (its residual is reported), since serving latency and
-        # accuracy do not depend on how well the weights fit the data.
-        ip.fit(p, sample_f(p), 1e-4, max_iter=30, options=pa.FitOptions(keep_unconverged=True))
+        # keep_unconverged only so that a model short of 1e-4 after 100 steps
+        # shows in models_fit_to_1e-4 instead of ending the run
+        ip.fit(p, sample_f(p), 1e-4, max_iter=100, options=pa.FitOptions(keep_unconverged=True))
         st = ip.fit_stats()
         n_conv += st["converged"]
         worst_res = max(worst_res, st["residual"])
@@ -515,10 +513,10 @@ def main():
         "m2l": {"kernel": "m2l_pair_kernel<float,float> + m2l_pair_reduce_kernel<float>", "bound": "mfma",
                 "achieved": m2l_ach / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
                 "frac": m2l_ach / PEAK_FP32, "flops": m2l_flops, "cell_pairs": m["m2l_pairs"]},
-        "p2p": {"kernel": "p2p_kernel<float,0>", "bound": "valu (fp32)", "achieved": p2p_ach / 1e12,
+        "p2p": {"kernel": "p2p_jobs_kernel<float,0,false>", "bound": "valu (fp32)", "achieved": p2p_ach / 1e12,
                 "peak": PEAK_FP32 / 1e12, "peak_unpacked": PEAK_FP32 / 2e12, "unit": "TFLOP/s",
                 "frac": p2p_ach / PEAK_FP32, "pairs": m["p2p_pairs"], "flops_per_pair": 11},
-        "l2p": {"kernel": "l2p_kernel<double,P>", "bound": "hbm",
+        "l2p": {"kernel": f"l2p_jobs_kernel<double,{p}>", "bound": "hbm",
                 "achieved": l2p_bytes / (stage["ms_l2p"] * 1e-3) / 1e9, "peak": PEAK_HBM / 1e9,
                 "unit": "GB/s", "frac": l2p_bytes / (stage["ms_l2p"] * 1e-3) / PEAK_HBM, "bytes": l2p_bytes},
     }

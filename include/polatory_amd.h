@@ -263,6 +263,15 @@ typedef struct {
   int ras_fp32_factors;     /* RAS fine-level factors in FP32 (FP32 MFMA factorisation, FP64 solve
                                arithmetic; the coarse grid stays FP64): 1 on, 0 off, -1 (default) on
                                when the operator is not FP64 and no rbf has CPD order 2 (bh3, exp) */
+  int tune_factor;          /* time the RAS batched factorisation's forms (left-looking, right-looking
+                               with 32- or 64-wide panels) on a sample of each fine level's domains
+                               (default 0: a fixed rule from that timing) */
+  int factor_panel;         /* RAS batched factorisation form: 0 left-looking, 32 or 64 right-looking
+                               with that panel width, -1 (default) the rule or tune_factor */
+  int cpd2_auto;            /* 1 (default): for kernels of CPD order 2 (triharmonic3d) a mixed-precision
+                               operator is raised to FP64 of order >= 10, and the RAS coarse grid has
+                               4096 points from 2e5 points and 6000 from 8e5 (the reference's 2048
+                               stalls there); 0: the configurations as given, 2048 coarse points */
 } pam_fit_options;
 int pam_fit_options_default(pam_fit_options* opt);
 /* Interpolant(const Model&) — interpolant.hpp:43 */
@@ -371,6 +380,11 @@ int pam_fgmres_dense(const double* a, const double* m, int64_t n, const double* 
 /* RasPreconditioner(model, points) — ras_preconditioner.hpp:56-189 */
 typedef struct pam_ras pam_ras;
 int pam_ras_create(const pam_model* model, const double* points, int64_t n, pam_ras** out);
+/* The preconditioner that a fit with these options builds for these points
+ * (RasPreconditioner(model, points, ...) with the fit's precision, factor form
+ * and coarse-grid choices); opt must not be NULL. */
+int pam_ras_create_with_options(const pam_model* model, const double* points, int64_t n,
+                                const pam_fit_options* opt, pam_ras** out);
 int pam_ras_destroy(pam_ras* h);
 int pam_ras_num_levels(const pam_ras* h, int* n_levels);
 /* point indices of a level (level n_levels-1 = all points; 0 = coarse grid) */

@@ -41,7 +41,10 @@ class PamFitOptions(ctypes.Structure):
                 ("use_level_config", ctypes.c_int), ("use_preconditioner", ctypes.c_int),
                 ("refine_rtol", ctypes.c_double), ("verbose", ctypes.c_int),
                 ("use_graphs", ctypes.c_int), ("keep_unconverged", ctypes.c_int),
-                ("tune_solves", ctypes.c_int), ("ras_fp32_factors", ctypes.c_int)]
+                ("tune_solves", ctypes.c_int), ("ras_fp32_factors", ctypes.c_int),
+                ("tune_factor", ctypes.c_int),
+                ("factor_panel", ctypes.c_int),
+                ("cpd2_auto", ctypes.c_int)]
 
 
 class PamFitStats(ctypes.Structure):
@@ -170,6 +173,7 @@ _SIGS = {
     "pam_interpolant_residual_history": [_P, _P, ctypes.POINTER(_I64)],
     "pam_fgmres_dense": [_P, _P, _I64, _P, _P, _INT, _P, _P, ctypes.POINTER(_INT)],
     "pam_ras_create": [_P, _P, _I64, ctypes.POINTER(_P)],
+    "pam_ras_create_with_options": [_P, _P, _I64, ctypes.POINTER(PamFitOptions), ctypes.POINTER(_P)],
     "pam_ras_destroy": [_P],
     "pam_ras_num_levels": [_P, ctypes.POINTER(_INT)],
     "pam_ras_level_points": [_P, _INT, _P, ctypes.POINTER(_I64)],

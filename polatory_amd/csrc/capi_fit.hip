@@ -75,6 +75,11 @@ pam::FitOptions to_opts(const pam_fit_options* o) {
   r.tune_solves = o->tune_solves != 0;
   if (o->ras_fp32_factors < -1 || o->ras_fp32_factors > 1) pam::throw_invalid("ras_fp32_factors must be -1, 0 or 1");
   r.ras_fp32_factors = o->ras_fp32_factors;
+  r.tune_factor = o->tune_factor != 0;
+  if (o->factor_panel != -1 && o->factor_panel != 0 && o->factor_panel != 32 && o->factor_panel != 64)
+    pam::throw_invalid("factor_panel must be -1, 0, 32 or 64");
+  r.factor_panel = o->factor_panel;
+  r.cpd2_auto = o->cpd2_auto != 0;
   return r;
 }
 
@@ -202,6 +207,9 @@ int pam_fit_options_default(pam_fit_options* opt) {
     opt->verbose = 0;
     opt->tune_solves = o.tune_solves ? 1 : 0;
     opt->ras_fp32_factors = o.ras_fp32_factors;
+    opt->tune_factor = o.tune_factor ? 1 : 0;
+    opt->factor_panel = o.factor_panel;
+    opt->cpd2_auto = o.cpd2_auto ? 1 : 0;
   });
 }
 
@@ -462,6 +470,21 @@ int pam_ras_create(const pam_model* model, const double* points, int64_t n, pam_
     h->model = model->m;
     std::vector<double> p(points, points + 3 * n);
     h->impl = std::make_unique<pam::RasPreconditioner>(h->model, p, h->s, pam::RasOptions{});
+    *out = h.release();
+  });
+}
+
+int pam_ras_create_with_options(const pam_model* model, const double* points, int64_t n, const pam_fit_options* opt,
+                                pam_ras** out) {
+  return guard([&] {
+    if (!model || !out || !opt) pam::throw_invalid("null argument");
+    auto h = std::make_unique<pam_ras>();
+    PAM_HIP(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
+    h->model = model->m;
+    pam::FitOptions fo = to_opts(opt);
+    pam::resolve_fit_options(h->model, fo);
+    std::vector<double> p(points, points + 3 * n);
+    h->impl = std::make_unique<pam::RasPreconditioner>(h->model, p, h->s, pam::fit_ras_options(h->model, fo, n));
     *out = h.release();
   });
 }

@@ -520,6 +520,16 @@ void FmmEvaluator::prepare_trees() {
   PAM_HIP(hipStreamSynchronize(stream_));
 }
 
+void FmmEvaluator::prebuild_operator(hipStream_t s) {
+  if (!fixed_config_ || cfg_.mode == kModeDirect || effective_svd_tol(cfg_, rbf_) > 0.0 || homogeneity(rbf_) == 0)
+    return;
+  // Below 2^16 points the tree may have no level with well-separated cells.
+  const int64_t nt = symmetric_ ? n_src_ : n_trg_;
+  if (std::min(n_src_, nt) < (int64_t{1} << 16)) return;
+  if (cfg_.precision == kFp64) get_op<double>(cfg_.order, rbf_, 1.0, s);
+  else get_op<float>(cfg_.order, rbf_, 1.0, s);
+}
+
 void FmmEvaluator::prepare_source() {
   if (so_) throw_runtime("prepare_source on an evaluator that shares another's sources");
   if (!have_weights_ && n_src_ > 0) throw_runtime("weights are not set");

@@ -117,6 +117,12 @@ class FmmEvaluator {
   const Lists& lists() const { return lists_; }
   // Builds trees/lists for the current configuration without evaluating.
   void prepare_trees();
+  // Builds, on stream s, the dense M2L operator that evaluations with the
+  // fixed configuration will take from the process-wide operator cache, so
+  // that the build can overlap other work. Only for kernels homogeneous in r
+  // (their operator does not depend on the tree) and point sets large enough
+  // for the FMM; otherwise does nothing. Synchronizes s.
+  void prebuild_operator(hipStream_t s);
   // Builds the source tree and the multipole expansions now, so that other
   // evaluators can share them (attach_source). Synchronizes the stream.
   void prepare_source();

@@ -3,6 +3,7 @@
 #include "ras.hpp"
 
 #include <future>
+#include <mutex>
 
 #include <algorithm>
 #include <parallel/algorithm>
@@ -46,9 +47,8 @@ RbfSet make_rbf_set(const Model& m) {
   return r;
 }
 
-// mat_a entry (preconditioner/mat_a.hpp:11-61): sum of rbfs + nugget on the diagonal.
-__device__ __forceinline__ double kernel_entry(const RbfSet& r, const double* __restrict__ pts, int u, int v) {
-  double dx = pts[3 * u] - pts[3 * v], dy = pts[3 * u + 1] - pts[3 * v + 1], dz = pts[3 * u + 2] - pts[3 * v + 2];
+// Sum of the rbfs at the separation (dx, dy, dz) (preconditioner/mat_a.hpp:11-61).
+__device__ __forceinline__ double kernel_value(const RbfSet& r, double dx, double dy, double dz) {
   double s = 0.0;
   for (int k = 0; k < r.n; k++) {
     const double* a = r.a[k];
@@ -65,6 +65,12 @@ __device__ __forceinline__ double kernel_entry(const RbfSet& r, const double* __
       s += r.p0[k] * exp(-3.0 * sqrt(r2) / r.p1[k]);
     }
   }
+  return s;
+}
+
+// mat_a entry (preconditioner/mat_a.hpp:11-61): sum of rbfs + nugget on the diagonal.
+__device__ __forceinline__ double kernel_entry(const RbfSet& r, const double* __restrict__ pts, int u, int v) {
+  double s = kernel_value(r, pts[3 * u] - pts[3 * v], pts[3 * u + 1] - pts[3 * v + 1], pts[3 * u + 2] - pts[3 * v + 2]);
   if (u == v) s += r.nugget;
   return s;
 }
@@ -111,48 +117,110 @@ __global__ void assemble_u_kernel(RbfSet r, int l, const double* __restrict__ pt
   }
 }
 
-// Lower triangle of M = Q^T A Q (fine_grid.hpp:79-86), column-major, Q = [-Lag_inner^T ; I].
-// Entries are computed in FP64 and stored in T (FP32 for single-precision factors).
-template <class T>
-__global__ void assemble_m_kernel(RbfSet r, int l, const double* __restrict__ pts,
-                                  const int64_t* __restrict__ ptr, const int32_t* __restrict__ idx,
-                                  const double* __restrict__ lag, const double* __restrict__ u,
-                                  const double* __restrict__ atl, const int64_t* __restrict__ mat_ptr,
-                                  T* __restrict__ mat) {
-  int d = blockIdx.y;
-  int64_t p0 = ptr[d];
-  int m = static_cast<int>(ptr[d + 1] - p0);
-  int n = m - l;
-  if (n <= 0) return;
-  int64_t uoff = static_cast<int64_t>(l) * (p0 - static_cast<int64_t>(d) * l);
-  const double* ud = u + uoff;
+// Rows of the inner points of every domain for the assembly of
+// M = Q^T A Q (fine_grid.hpp:79-86), Q = [-Lag_inner^T ; I]. Writing
+// q_i = -lag(x_i) and g_i = U[:, i] + Atl q_i / 2 (U, Atl from
+// assemble_u_kernel, Atl symmetric),
+//   M_ij = A(x_i, x_j) + q_i . U[:, j] + U[:, i] . q_j + q_i^T Atl q_j
+//        = A(x_i, x_j) + q_i . g_j + g_i . q_j.
+// Per domain d (inner offset o_d = ptr[d] - d l, n_d inner points) the
+// 3 + 2l fields are stored field-major: rows[(3 + 2l) o_d + f n_d + i] with
+// f = x, y, z, q_0..q_{l-1}, g_0..g_{l-1}.
+__global__ void assemble_rows_kernel(int l, const double* __restrict__ pts, const int64_t* __restrict__ ptr,
+                                     const int32_t* __restrict__ idx, const double* __restrict__ lag,
+                                     const double* __restrict__ u, const double* __restrict__ atl,
+                                     double* __restrict__ rows) {
+  const int d = blockIdx.y;
+  const int64_t p0 = ptr[d];
+  const int n = static_cast<int>(ptr[d + 1] - p0) - l;
+  const int64_t o = p0 - static_cast<int64_t>(d) * l;
+  double* rd = rows + static_cast<int64_t>(3 + 2 * l) * o;
+  const double* ud = u + static_cast<int64_t>(l) * o;
   const double* ad = atl + static_cast<int64_t>(d) * l * l;
-  T* md = mat + mat_ptr[d];
-  int64_t total = static_cast<int64_t>(n) * n;
-  for (int64_t e = static_cast<int64_t>(blockIdx.x) * kT + threadIdx.x; e < total;
-       e += static_cast<int64_t>(gridDim.x) * kT) {
-    int i = static_cast<int>(e % n), j = static_cast<int>(e / n);
-    if (i < j) {
-      md[e] = T(0);
-      continue;
+  for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) {
+    const int qi = idx[p0 + l + i];
+    rd[i] = pts[3 * qi];
+    rd[n + i] = pts[3 * qi + 1];
+    rd[2 * n + i] = pts[3 * qi + 2];
+    double q[10];
+    for (int a = 0; a < l; a++) {
+      q[a] = -lag[static_cast<int64_t>(qi) * l + a];
+      rd[static_cast<int64_t>(3 + a) * n + i] = q[a];
     }
-    int qi = idx[p0 + l + i], qj = idx[p0 + l + j];
-    double v = kernel_entry(r, pts, qi, qj);
-    if (l > 0) {
-      double qa[10], qb[10];
-      for (int a = 0; a < l; a++) {
-        qa[a] = -lag[static_cast<int64_t>(qi) * l + a];
-        qb[a] = -lag[static_cast<int64_t>(qj) * l + a];
-      }
-      for (int a = 0; a < l; a++) v += qa[a] * ud[a * n + j] + ud[a * n + i] * qb[a];
-      for (int a = 0; a < l; a++) {
-        double t = 0.0;
-        for (int b = 0; b < l; b++) t += ad[a * l + b] * qb[b];
-        v += qa[a] * t;
-      }
+    for (int a = 0; a < l; a++) {
+      double t = 0.0;
+      for (int b = 0; b < l; b++) t += ad[a * l + b] * q[b];
+      rd[static_cast<int64_t>(3 + l + a) * n + i] = ud[static_cast<int64_t>(a) * n + i] + 0.5 * t;
     }
-    md[e] = static_cast<T>(v);
   }
+}
+
+// Lower triangle of M (column-major, n_d x n_d per domain) in 64 x 64 tiles:
+// workgroup tile t of domain d covers rows 64 I.., columns 64 J.., I >= J.
+// The tile's column fields are staged in LDS; each thread keeps one row's
+// fields in registers and computes 16 columns, so the stores of a wavefront
+// go to 64 consecutive rows of one column. Entries are computed in FP64 and
+// stored in T (FP32 for single-precision factors). The strict upper triangle
+// is never written: the factorisation and the solves read only the lower one.
+template <class T, int L>
+__global__ void __launch_bounds__(kT)
+assemble_m_kernel(RbfSet r, const int64_t* __restrict__ ptr, const double* __restrict__ rows,
+                  const int64_t* __restrict__ mat_ptr, T* __restrict__ mat) {
+  constexpr int F = 3 + 2 * L;
+  __shared__ double cj[F][kTile];
+  const int d = blockIdx.y;
+  const int64_t p0 = ptr[d];
+  const int n = static_cast<int>(ptr[d + 1] - p0) - L;
+  if (n <= 0) return;
+  const int nt = (n + kTile - 1) / kTile;
+  const int ntiles = nt * (nt + 1) / 2;
+  const double* rd = rows + static_cast<int64_t>(F) * (p0 - static_cast<int64_t>(d) * L);
+  T* md = mat + mat_ptr[d];
+  const int tid = threadIdx.x, li = tid & 63, cg = tid >> 6;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    int I = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((I + 1) * (I + 2) / 2 <= t) I++;
+    while (I * (I + 1) / 2 > t) I--;
+    const int J = t - I * (I + 1) / 2;
+    __syncthreads();
+    for (int e = tid; e < F * kTile; e += kT) {
+      const int f = e / kTile, c = e % kTile, j = J * kTile + c;
+      cj[f][c] = j < n ? rd[static_cast<int64_t>(f) * n + j] : 0.0;
+    }
+    const int i = I * kTile + li;
+    double ri[F];
+#pragma unroll
+    for (int f = 0; f < F; f++) ri[f] = i < n ? rd[static_cast<int64_t>(f) * n + i] : 0.0;
+    __syncthreads();
+    if (i >= n) continue;
+#pragma unroll 4
+    for (int c = 0; c < 16; c++) {
+      const int jc = cg * 16 + c, j = J * kTile + jc;
+      if (j > i || j >= n) break;
+      double v = kernel_value(r, ri[0] - cj[0][jc], ri[1] - cj[1][jc], ri[2] - cj[2][jc]);
+      if (j == i) v += r.nugget;
+#pragma unroll
+      for (int a = 0; a < L; a++) v += ri[3 + a] * cj[3 + L + a][jc] + ri[3 + L + a] * cj[3 + a][jc];
+      md[static_cast<int64_t>(j) * n + i] = static_cast<T>(v);
+    }
+  }
+}
+
+// Launch of assemble_m_kernel for the polynomial basis size l.
+template <class T>
+void launch_assemble_m(int l, int max_n, int n_dom, const RbfSet& rs, const int64_t* ptr, const double* rows,
+                       const int64_t* mat_ptr, T* mat, hipStream_t s) {
+  if (n_dom <= 0 || max_n <= 0) return;
+  const int nt = (max_n + kTile - 1) / kTile;
+  const dim3 grid(static_cast<unsigned>(std::min(nt * (nt + 1) / 2, 2048)), static_cast<unsigned>(n_dom));
+  switch (l) {
+    case 0: assemble_m_kernel<T, 0><<<grid, kT, 0, s>>>(rs, ptr, rows, mat_ptr, mat); break;
+    case 1: assemble_m_kernel<T, 1><<<grid, kT, 0, s>>>(rs, ptr, rows, mat_ptr, mat); break;
+    case 4: assemble_m_kernel<T, 4><<<grid, kT, 0, s>>>(rs, ptr, rows, mat_ptr, mat); break;
+    case 10: assemble_m_kernel<T, 10><<<grid, kT, 0, s>>>(rs, ptr, rows, mat_ptr, mat); break;
+    default: throw_invalid("unsupported polynomial basis size");
+  }
+  PAM_LAUNCH_CHECK();
 }
 
 // One workgroup factors one domain: right-looking Cholesky with 32-wide
@@ -164,11 +232,11 @@ __global__ void assemble_m_kernel(RbfSet r, int l, const double* __restrict__ pt
 // staged in LDS so that the read-modify-write of A22 is coalesced.
 // T = double (FP64 MFMA) or float (FP32 MFMA, single-precision fine-level
 // factors).
-template <class T>
+template <class T, int NB>
 struct CholSmem {
-  T dg[kNb][kNb + 1];     // diagonal block (L11 after the factorisation)
-  T pi[kTile][kNb + 1];   // panel rows of tile I
-  T pj[kTile][kNb + 1];   // panel rows of tile J
+  T dg[NB][NB + 1];     // diagonal block (L11 after the factorisation)
+  T pi[kTile][NB + 1];   // panel rows of tile I
+  T pj[kTile][NB + 1];   // panel rows of tile J
   T dt[kTile][kTile + 1]; // MFMA result of a tile; the inverse of L11 during the panel phase
 };
 
@@ -198,12 +266,12 @@ struct CholMfma<float> {
 // Panel step k0 of the factorisation of the n x n matrix a (column-major,
 // lower triangle): L11 by one wavefront and its inverse, then L21 = A21 L11^-T.
 // Ends with a barrier.
-template <class T>
-__device__ void chol_panel(CholSmem<T>& sm, T* __restrict__ a, int n, int k0, int32_t* __restrict__ flag) {
+template <class T, int NB>
+__device__ void chol_panel(CholSmem<T, NB>& sm, T* __restrict__ a, int n, int k0, int32_t* __restrict__ flag) {
   const int lda = n;
-  const int kb = min(kNb, n - k0);
+  const int kb = min(NB, n - k0);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  T (*inv)[kNb + 1] = reinterpret_cast<T (*)[kNb + 1]>(&sm.dt[0][0]);
+  T (*inv)[NB + 1] = reinterpret_cast<T (*)[NB + 1]>(&sm.dt[0][0]);
   for (int e = tid; e < kb * kb; e += kT) {
     int rr = e % kb, cc = e / kb;
     sm.dg[rr][cc] = a[static_cast<int64_t>(k0 + cc) * lda + k0 + rr];
@@ -253,14 +321,14 @@ __device__ void chol_panel(CholSmem<T>& sm, T* __restrict__ a, int n, int k0, in
   }
   // L21 = A21 L11^-T, row by row
   for (int i = k0 + kb + tid; i < n; i += kT) {
-    T x[kNb];
+    T x[NB];
 #pragma unroll
-    for (int t = 0; t < kNb; t++) x[t] = t < kb ? a[static_cast<int64_t>(k0 + t) * lda + i] : T(0);
+    for (int t = 0; t < NB; t++) x[t] = t < kb ? a[static_cast<int64_t>(k0 + t) * lda + i] : T(0);
 #pragma unroll 4
     for (int c = 0; c < kb; c++) {
       T v = T(0);
 #pragma unroll
-      for (int t = 0; t < kNb; t++)
+      for (int t = 0; t < NB; t++)
         if (t <= c) v += x[t] * inv[c][t];
       a[static_cast<int64_t>(k0 + c) * lda + i] = v;
     }
@@ -271,22 +339,22 @@ __device__ void chol_panel(CholSmem<T>& sm, T* __restrict__ a, int n, int k0, in
 // Trailing tile (I, J), I >= J, of panel step k0: A22 -= L21 L21^T on FP64
 // MFMA (4 waves as 2 x 2, 32 x 32 each). load_i: stage tile I's panel rows
 // (they can stay staged across the J of one I). Ends with a barrier.
-template <class T>
-__device__ void chol_tile(CholSmem<T>& sm, T* __restrict__ a, int n, int k0, int I, int J, bool load_i) {
+template <class T, int NB>
+__device__ void chol_tile(CholSmem<T, NB>& sm, T* __restrict__ a, int n, int k0, int I, int J, bool load_i) {
   using acc_t = typename CholMfma<T>::acc_t;
   const int lda = n;
-  const int kb = min(kNb, n - k0);
+  const int kb = min(NB, n - k0);
   const int base = k0 + kb;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wi = wave >> 1, wj = wave & 1;
   const int r0 = base + I * kTile, c0 = base + J * kTile;
   if (load_i) {
-    for (int e = tid; e < kTile * kNb; e += kT) {
+    for (int e = tid; e < kTile * NB; e += kT) {
       int rr = e % kTile, cc = e / kTile;
       sm.pi[rr][cc] = (r0 + rr < n && cc < kb) ? a[static_cast<int64_t>(k0 + cc) * lda + r0 + rr] : T(0);
     }
   }
-  for (int e = tid; e < kTile * kNb; e += kT) {
+  for (int e = tid; e < kTile * NB; e += kT) {
     int rr = e % kTile, cc = e / kTile;
     sm.pj[rr][cc] = (c0 + rr < n && cc < kb) ? a[static_cast<int64_t>(k0 + cc) * lda + c0 + rr] : T(0);
   }
@@ -297,7 +365,7 @@ __device__ void chol_tile(CholSmem<T>& sm, T* __restrict__ a, int n, int k0, int
 #pragma unroll
     for (int jj = 0; jj < 2; jj++) acc[ii][jj] = acc_t{T(0), T(0), T(0), T(0)};
 #pragma unroll
-  for (int ks = 0; ks < kNb / 4; ks++) {
+  for (int ks = 0; ks < NB / 4; ks++) {
     T af[2], bf[2];
 #pragma unroll
     for (int ii = 0; ii < 2; ii++) af[ii] = sm.pi[wi * 32 + ii * 16 + (lane & 15)][ks * 4 + (lane >> 4)];
@@ -326,18 +394,18 @@ __device__ void chol_tile(CholSmem<T>& sm, T* __restrict__ a, int n, int k0, int
 }
 
 // One workgroup per domain, all panel steps (many domains at once).
-template <class T>
+template <class T, int NB>
 __global__ void __launch_bounds__(kT)
 cholesky_kernel(const int64_t* __restrict__ ptr, int l, const int64_t* __restrict__ mat_ptr,
                 T* __restrict__ mat, int32_t* __restrict__ flag) {
-  __shared__ CholSmem<T> sm;
+  __shared__ CholSmem<T, NB> sm;
   const int d = blockIdx.x;
   const int n = static_cast<int>(ptr[d + 1] - ptr[d]) - l;
   if (n <= 0) return;
   T* a = mat + mat_ptr[d];
-  for (int k0 = 0; k0 < n; k0 += kNb) {
+  for (int k0 = 0; k0 < n; k0 += NB) {
     chol_panel(sm, a, n, k0, flag);
-    const int rem = n - min(kNb, n - k0) - k0;
+    const int rem = n - min(NB, n - k0) - k0;
     if (rem <= 0) break;
     const int nt = (rem + kTile - 1) / kTile;
     for (int I = 0; I < nt; I++)
@@ -345,17 +413,252 @@ cholesky_kernel(const int64_t* __restrict__ ptr, int l, const int64_t* __restric
   }
 }
 
+// Left-looking batched factorisation: one workgroup per domain, block column
+// by block column (64 wide). Tile (I, J), I >= J, is
+//   A_IJ - sum_{K<J} L_IK L_JK^T
+// accumulated on MFMA in registers (4 waves as 2 x 2, 32 x 32 each) from
+// 32-column chunks of the two row panels, staged through LDS with the next
+// chunk's loads in flight; the diagonal tile is then factored in LDS by all
+// threads (one barrier per column) and inverted (Gauss-Jordan, one barrier per
+// column), and each tile below it is multiplied by that inverse's transpose on
+// MFMA. Every entry of L is written once and A is never read back, so the
+// trailing matrix is not re-read and re-written per panel as in the
+// right-looking form.
+constexpr int kKc = 32;  // reduction chunk of the tile updates
+
+template <class T>
+struct LlSmem {
+  union {
+    struct {
+      T pi[kTile][kKc + 1];  // chunk of row panel I (negated)
+      T pj[kTile][kKc + 1];  // chunk of row panel J
+    } g;
+    T dt[kTile][kTile + 1];  // a tile in transit
+  } u;
+  T dg[kTile][kTile + 1];   // diagonal tile; L_JJ after the factorisation
+  T inv[kTile][kTile + 1];  // L_JJ^-1 (lower)
+};
+
+// acc = A_IJ - sum_{K<J} L_IK L_JK^T for the calling wave's 32 x 32 part.
+template <class T>
+__device__ void ll_tile_update(LlSmem<T>& sm, const T* __restrict__ a, int n, int I, int J,
+                               typename CholMfma<T>::acc_t (&acc)[2][2]) {
+  using M = CholMfma<T>;
+  const int64_t lda = n;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wi = wave >> 1, wj = wave & 1;
+  const int r0 = I * kTile, c0 = J * kTile;
+#pragma unroll
+  for (int ii = 0; ii < 2; ii++)
+#pragma unroll
+    for (int jj = 0; jj < 2; jj++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int r = r0 + wi * 32 + ii * 16 + M::row(lane, q), c = c0 + wj * 32 + jj * 16 + (lane & 15);
+        acc[ii][jj][q] = (r < n && c < n) ? a[static_cast<int64_t>(c) * lda + r] : T(0);
+      }
+  const int nk = c0 / kKc;
+  if (nk == 0) return;
+  constexpr int kPer = kTile * kKc / kT;  // 8 entries of each panel chunk per thread
+  const int rr = tid & 63, kk0 = tid >> 6;
+  const bool in_i = r0 + rr < n, in_j = c0 + rr < n;
+  T ri[kPer], rj[kPer];
+  auto load = [&](int k) {
+#pragma unroll
+    for (int q = 0; q < kPer; q++) {
+      const int64_t col = static_cast<int64_t>(k * kKc + kk0 + 4 * q) * lda;
+      ri[q] = in_i ? a[col + r0 + rr] : T(0);
+      rj[q] = in_j ? a[col + c0 + rr] : T(0);
+    }
+  };
+  load(0);
+  for (int k = 0; k < nk; k++) {
+    __syncthreads();  // the previous chunk's reads are done
+#pragma unroll
+    for (int q = 0; q < kPer; q++) {
+      sm.u.g.pi[rr][kk0 + 4 * q] = -ri[q];
+      sm.u.g.pj[rr][kk0 + 4 * q] = rj[q];
+    }
+    __syncthreads();
+    if (k + 1 < nk) load(k + 1);
+#pragma unroll
+    for (int ks = 0; ks < kKc / 4; ks++) {
+      T af[2], bf[2];
+#pragma unroll
+      for (int ii = 0; ii < 2; ii++) af[ii] = sm.u.g.pi[wi * 32 + ii * 16 + (lane & 15)][ks * 4 + (lane >> 4)];
+#pragma unroll
+      for (int jj = 0; jj < 2; jj++) bf[jj] = sm.u.g.pj[wj * 32 + jj * 16 + (lane & 15)][ks * 4 + (lane >> 4)];
+#pragma unroll
+      for (int ii = 0; ii < 2; ii++)
+#pragma unroll
+        for (int jj = 0; jj < 2; jj++) acc[ii][jj] = M::op(af[ii], bf[jj], acc[ii][jj]);
+    }
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void ll_store_acc(T (*dst)[kTile + 1], const typename CholMfma<T>::acc_t (&acc)[2][2]) {
+  using M = CholMfma<T>;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, wi = wave >> 1, wj = wave & 1;
+#pragma unroll
+  for (int ii = 0; ii < 2; ii++)
+#pragma unroll
+    for (int jj = 0; jj < 2; jj++)
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        dst[wi * 32 + ii * 16 + M::row(lane, q)][wj * 32 + jj * 16 + (lane & 15)] = acc[ii][jj][q];
+}
+
+template <class T>
+__global__ void __launch_bounds__(kT)
+cholesky_ll_kernel(const int64_t* __restrict__ ptr, int l, const int64_t* __restrict__ mat_ptr,
+                   T* __restrict__ mat, int32_t* __restrict__ flag) {
+  using M = CholMfma<T>;
+  using acc_t = typename M::acc_t;
+  __shared__ LlSmem<T> sm;
+  const int d = blockIdx.x;
+  const int n = static_cast<int>(ptr[d + 1] - ptr[d]) - l;
+  if (n <= 0) return;
+  T* a = mat + mat_ptr[d];
+  const int64_t lda = n;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wi = wave >> 1, wj = wave & 1;
+  const int nt = (n + kTile - 1) / kTile;
+  const int rr = tid & 63, cg = tid >> 6;  // row, and column group (of four) of the LDS passes
+  for (int J = 0; J < nt; J++) {
+    const int c0 = J * kTile, kb = min(kTile, n - c0);
+    acc_t acc[2][2];
+    ll_tile_update(sm, a, n, J, J, acc);
+    // The diagonal tile, padded with the identity past the matrix.
+#pragma unroll
+    for (int ii = 0; ii < 2; ii++)
+#pragma unroll
+      for (int jj = 0; jj < 2; jj++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int r = wi * 32 + ii * 16 + M::row(lane, q), c = wj * 32 + jj * 16 + (lane & 15);
+          sm.dg[r][c] = (r < kb && c < kb) ? acc[ii][jj][q] : (r == c ? T(1) : T(0));
+        }
+    __syncthreads();
+    // Cholesky of dg. Step j updates columns c > j with column j as it was
+    // before its scaling (a_rj a_cj / a_jj); column j is scaled in step j + 1,
+    // which reads columns > j + 1 only, so one barrier per step suffices.
+    for (int j = 0; j < kTile; j++) {
+      T piv = sm.dg[j][j];
+      if (!(piv > T(0))) {
+        if (tid == 0) atomicExch(flag, 1);
+        piv = fabs(piv) + M::kTiny;
+      }
+      const T rp = T(1) / piv;
+      if (j > 0 && tid < kTile) {  // scale column j - 1
+        const T pj1 = sm.dg[j - 1][j - 1];
+        const T p1 = pj1 > T(0) ? pj1 : fabs(pj1) + M::kTiny;
+        const T s = sqrt(p1);
+        if (rr > j - 1) sm.dg[rr][j - 1] /= s;
+      }
+      const T lrj = sm.dg[rr][j];
+      for (int c = j + 1 + cg; c < kTile; c += 4)
+        if (rr >= c) sm.dg[rr][c] -= lrj * (sm.dg[c][j] * rp);
+      __syncthreads();
+      if (j > 0 && tid == 0) {  // the diagonal entry of column j - 1, after every reader of it
+        const T pj1 = sm.dg[j - 1][j - 1];
+        sm.dg[j - 1][j - 1] = sqrt(pj1 > T(0) ? pj1 : fabs(pj1) + M::kTiny);
+      }
+    }
+    if (tid < kTile) {
+      const T p = sm.dg[kTile - 1][kTile - 1];
+      const T s = sqrt(p > T(0) ? p : fabs(p) + M::kTiny);
+      if (rr == kTile - 1) sm.dg[rr][rr] = s;
+    }
+    __syncthreads();
+    for (int e = tid; e < kb * kb; e += kT) {
+      const int r = e % kb, c = e / kb;
+      if (r >= c) a[static_cast<int64_t>(c0 + c) * lda + c0 + r] = sm.dg[r][c];
+    }
+    if (J + 1 == nt) break;
+    // inv = L_JJ^-1 by forward elimination on the identity. Step j
+    // subtracts multiples of row j (as it was before its division) from the
+    // rows below; row j is divided in step j + 1, which does not read it.
+    for (int e = tid; e < kTile * kTile; e += kT) {
+      const int r = e % kTile, c = e / kTile;
+      sm.inv[r][c] = r == c ? T(1) : T(0);
+    }
+    __syncthreads();
+    for (int j = 0; j < kTile; j++) {
+      const T rd = T(1) / sm.dg[j][j];
+      if (j > 0 && tid < kTile && rr <= j - 1) sm.inv[j - 1][rr] /= sm.dg[j - 1][j - 1];
+      if (rr > j) {
+        const T f = sm.dg[rr][j] * rd;
+        for (int c = cg; c <= j; c += 4) sm.inv[rr][c] -= f * sm.inv[j][c];
+      }
+      __syncthreads();
+    }
+    if (tid < kTile) sm.inv[kTile - 1][rr] /= sm.dg[kTile - 1][kTile - 1];
+    __syncthreads();
+    for (int I = J + 1; I < nt; I++) {
+      ll_tile_update(sm, a, n, I, J, acc);
+      __syncthreads();  // dt overlays the panel chunks
+      ll_store_acc(sm.u.dt, acc);
+      __syncthreads();
+      acc_t o[2][2];
+#pragma unroll
+      for (int ii = 0; ii < 2; ii++)
+#pragma unroll
+        for (int jj = 0; jj < 2; jj++) o[ii][jj] = acc_t{T(0), T(0), T(0), T(0)};
+#pragma unroll
+      for (int ks = 0; ks < kTile / 4; ks++) {
+        T af[2], bf[2];
+#pragma unroll
+        for (int ii = 0; ii < 2; ii++) af[ii] = sm.u.dt[wi * 32 + ii * 16 + (lane & 15)][ks * 4 + (lane >> 4)];
+#pragma unroll
+        for (int jj = 0; jj < 2; jj++) bf[jj] = sm.inv[wj * 32 + jj * 16 + (lane & 15)][ks * 4 + (lane >> 4)];
+#pragma unroll
+        for (int ii = 0; ii < 2; ii++)
+#pragma unroll
+          for (int jj = 0; jj < 2; jj++) o[ii][jj] = M::op(af[ii], bf[jj], o[ii][jj]);
+      }
+      __syncthreads();
+      ll_store_acc(sm.u.dt, o);
+      __syncthreads();
+      const int r0 = I * kTile;
+      for (int e = tid; e < kTile * kTile; e += kT) {
+        const int r = e % kTile, c = e / kTile;
+        if (r0 + r < n) a[static_cast<int64_t>(c0 + c) * lda + r0 + r] = sm.u.dt[r][c];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// The batched factorisation: right-looking with panel width nb (32 or 64), or
+// left-looking (nb = 0).
+template <class T>
+void launch_cholesky_batch(int nb, int n_dom, const int64_t* ptr, int l, const int64_t* mat_ptr, T* mat,
+                           int32_t* flag, hipStream_t s) {
+  if (n_dom <= 0) return;
+  if (nb == 0) {
+    cholesky_ll_kernel<T><<<n_dom, kT, 0, s>>>(ptr, l, mat_ptr, mat, flag);
+  } else if (nb == 64) {
+    cholesky_kernel<T, 64><<<n_dom, kT, 0, s>>>(ptr, l, mat_ptr, mat, flag);
+  } else if (nb == 32) {
+    cholesky_kernel<T, 32><<<n_dom, kT, 0, s>>>(ptr, l, mat_ptr, mat, flag);
+  } else {
+    throw_invalid("unsupported Cholesky panel width");
+  }
+  PAM_LAUNCH_CHECK();
+}
+
 // A single large matrix: one launch per panel step for the panel (one
 // workgroup) and one for its trailing tiles (one workgroup per tile).
 __global__ void __launch_bounds__(kT)
 cholesky_panel_kernel(double* __restrict__ a, int n, int k0, int32_t* __restrict__ flag) {
-  __shared__ CholSmem<double> sm;
+  __shared__ CholSmem<double, kNb> sm;
   chol_panel(sm, a, n, k0, flag);
 }
 
 __global__ void __launch_bounds__(kT)
 cholesky_trailing_kernel(double* __restrict__ a, int n, int k0) {
-  __shared__ CholSmem<double> sm;
+  __shared__ CholSmem<double, kNb> sm;
   // blockIdx.x -> (I, J) with I >= J
   const int t = blockIdx.x;
   int I = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) / 2.0);
@@ -1060,7 +1363,9 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
   auto t0 = std::chrono::steady_clock::now();
   model_.validate();
   constexpr double kFineToCoarseRatio = 10.0;  // ras_preconditioner.hpp:52
-  constexpr int64_t kNCoarsestPoints = 2048;    // ras_preconditioner.hpp:53
+  const int64_t kNCoarsestPoints = opt_.n_coarsest;  // 2048 in ras_preconditioner.hpp:53
+  if (kNCoarsestPoints < 1 || kNCoarsestPoints > RasOptions::kMaxCoarsest)
+    throw_invalid("the coarse grid size must be in [1, " + std::to_string(RasOptions::kMaxCoarsest) + "]");
   h_pts_ = points;
   d_pts_.upload(h_pts_.data(), h_pts_.size(), s_);
   n_levels_ = std::max(static_cast<int>(std::ceil(std::log(static_cast<double>(mu_) / kNCoarsestPoints) /
@@ -1282,20 +1587,21 @@ void RasPreconditioner::setup_batch(DomainBatch& b, const std::vector<Domain>& d
                                                         tmp->u.get(), tmp->atl.get());
     PAM_LAUNCH_CHECK();
   }
+  tmp->rows.resize(static_cast<size_t>(std::max<int64_t>((3 + 2 * l_) * (b.h_ptr.back() - static_cast<int64_t>(b.n_dom) * l_), 1)));
+  const int max_n = b.max_m - l_;
+  if (max_n > 0) {
+    assemble_rows_kernel<<<dim3(static_cast<unsigned>(ceil_div(max_n, kT)), b.n_dom), kT, 0, ls>>>(
+        l_, d_pts_.get(), b.ptr.get(), b.idx.get(), lag_.get(), tmp->u.get(), tmp->atl.get(), tmp->rows.get());
+    PAM_LAUNCH_CHECK();
+  }
   if (fp32) {
-    assemble_m_kernel<float><<<dim3(64, b.n_dom), kT, 0, ls>>>(rs, l_, d_pts_.get(), b.ptr.get(), b.idx.get(),
-                                                              lag_.get(), tmp->u.get(), tmp->atl.get(),
-                                                              b.mat_ptr.get(), b.mat32.get());
-    PAM_LAUNCH_CHECK();
-    cholesky_kernel<float><<<b.n_dom, kT, 0, ls>>>(b.ptr.get(), l_, b.mat_ptr.get(), b.mat32.get(), flag);
-    PAM_LAUNCH_CHECK();
+    launch_assemble_m<float>(l_, max_n, b.n_dom, rs, b.ptr.get(), tmp->rows.get(), b.mat_ptr.get(), b.mat32.get(), ls);
+    launch_cholesky_batch<float>(factor_panel(b, true, ls, tmp->mptr), b.n_dom, b.ptr.get(), l_, b.mat_ptr.get(), b.mat32.get(),
+                                 flag, ls);
     setup_tmp_.push_back(std::move(tmp));
     return;
   }
-  assemble_m_kernel<double><<<dim3(64, b.n_dom), kT, 0, ls>>>(rs, l_, d_pts_.get(), b.ptr.get(), b.idx.get(),
-                                                               lag_.get(), tmp->u.get(), tmp->atl.get(),
-                                                               b.mat_ptr.get(), b.mat.get());
-  PAM_LAUNCH_CHECK();
+  launch_assemble_m<double>(l_, max_n, b.n_dom, rs, b.ptr.get(), tmp->rows.get(), b.mat_ptr.get(), b.mat.get(), ls);
   const int n0 = static_cast<int>(b.h_ptr.back() - l_);
   if (b.n_dom == 1 && n0 > 4 * kTile) {
     // One large matrix (the coarse grid): a workgroup per trailing tile
@@ -1310,12 +1616,67 @@ void RasPreconditioner::setup_batch(DomainBatch& b, const std::vector<Domain>& d
       PAM_LAUNCH_CHECK();
     }
   } else {
-    cholesky_kernel<double><<<b.n_dom, kT, 0, ls>>>(b.ptr.get(), l_, b.mat_ptr.get(), b.mat.get(), flag);
-    PAM_LAUNCH_CHECK();
+    launch_cholesky_batch<double>(coarse ? kNb : factor_panel(b, false, ls, tmp->mptr), b.n_dom, b.ptr.get(), l_,
+                                  b.mat_ptr.get(), b.mat.get(), flag, ls);
   }
   // b.h_ptr is a member and outlives the upload; the rest stays until the
   // constructor synchronizes.
   setup_tmp_.push_back(std::move(tmp));
+}
+
+int RasPreconditioner::factor_panel(const DomainBatch& b, bool fp32, hipStream_t ls,
+                                    const std::vector<int64_t>& mptr) {
+  if (opt_.factor_panel >= 0) return opt_.factor_panel;
+  // Without tuning, the rule that the timing below found on MI355X for the
+  // reference's subdomain size (about 1,000 points, 1,500 with the overlap):
+  // the left-looking form, 3.7x faster than the right-looking one in FP32
+  // (3.6 against 13.5 ms for 32 domains of the 1M-point ellipsoid fit) and
+  // 3.3x in FP64 (4.7 against 15.3 ms).
+  if (!opt_.tune_factor) return 0;
+  static std::mutex mu;
+  static std::map<std::pair<bool, int>, int> chosen;  // (fp32, size class) -> panel width
+  const std::pair<bool, int> key{fp32, b.max_m / 256};
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = chosen.find(key);
+    if (it != chosen.end()) return it->second;
+  }
+  // Time each width on copies of the first domains' assembled matrices.
+  const int k = std::min(b.n_dom, 32);
+  const int64_t elems = mptr[static_cast<size_t>(k)];
+  const size_t es = fp32 ? sizeof(float) : sizeof(double);
+  DevBuf<char> orig(static_cast<size_t>(elems) * es), work(static_cast<size_t>(elems) * es);
+  DevBuf<int32_t> dummy(1);
+  dummy.zero(ls);
+  const void* src = fp32 ? static_cast<const void*>(b.mat32.get()) : static_cast<const void*>(b.mat.get());
+  PAM_HIP(hipMemcpyAsync(orig.get(), src, static_cast<size_t>(elems) * es, hipMemcpyDeviceToDevice, ls));
+  hipEvent_t e0, e1;
+  PAM_HIP(hipEventCreate(&e0));
+  PAM_HIP(hipEventCreate(&e1));
+  int best = 0;
+  float best_ms = INFINITY;
+  for (int nb : {0, 32, 64}) {
+    float ms = 0.0f;
+    for (int rep = 0; rep < 2; rep++) {  // the first run warms the code object and caches
+      PAM_HIP(hipMemcpyAsync(work.get(), orig.get(), static_cast<size_t>(elems) * es, hipMemcpyDeviceToDevice, ls));
+      PAM_HIP(hipEventRecord(e0, ls));
+      if (fp32) launch_cholesky_batch<float>(nb, k, b.ptr.get(), l_, b.mat_ptr.get(),
+                                            reinterpret_cast<float*>(work.get()), dummy.get(), ls);
+      else launch_cholesky_batch<double>(nb, k, b.ptr.get(), l_, b.mat_ptr.get(),
+                                        reinterpret_cast<double*>(work.get()), dummy.get(), ls);
+      PAM_HIP(hipEventRecord(e1, ls));
+      PAM_HIP(hipEventSynchronize(e1));
+      PAM_HIP(hipEventElapsedTime(&ms, e0, e1));
+    }
+    if (opt_.verbose) std::printf("  ras factor tuning (%s): %s %d, %d domains, %.3f ms\n", fp32 ? "FP32" : "FP64",
+                                  nb ? "right-looking, panel" : "left-looking, tile", nb ? nb : kTile, k, ms);
+    if (ms < best_ms) best_ms = ms, best = nb;
+  }
+  PAM_HIP(hipEventDestroy(e0));
+  PAM_HIP(hipEventDestroy(e1));
+  std::lock_guard<std::mutex> lk(mu);
+  chosen[key] = best;
+  return best;
 }
 
 int64_t RasPreconditioner::factor_bytes() const {

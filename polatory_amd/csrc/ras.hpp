@@ -69,6 +69,17 @@ struct RasOptions {
   // FP64. A level whose FP32 factorisation meets a non-positive pivot is
   // factored again in FP64.
   bool fp32_factors = false;
+  // Time the batched factorisation's forms (left-looking; right-looking with
+  // 32- or 64-wide panels) on a sample of each fine level's domains and keep
+  // the fastest (cached per precision and domain size for the process); off:
+  // a fixed rule from such a timing.
+  bool tune_factor = false;
+  // Fixes the form instead: 0 left-looking, 32 or 64 right-looking; -1 none.
+  int factor_panel = -1;
+  // Points of the coarse grid (ras_preconditioner.hpp:53 uses 2048); at most
+  // kMaxCoarsest.
+  int64_t n_coarsest = 2048;
+  static constexpr int64_t kMaxCoarsest = 6000;
 };
 
 class RasPreconditioner : public LinearOperator {
@@ -101,11 +112,14 @@ class RasPreconditioner : public LinearOperator {
     std::vector<int32_t> idx;
     std::vector<uint8_t> inner;
     std::vector<int64_t> mptr;
-    DevBuf<double> u, atl;
+    DevBuf<double> u, atl, rows;
   };
   std::vector<std::unique_ptr<SetupTmp>> setup_tmp_;
   void solve(int level, const double* d_res, double* d_w);
   void launch_solve(const DomainBatch& b, int level, int threads, const double* d_res, double* d_w);
+  // Form of a fine level's batched factorisation (0 left-looking, else the
+  // right-looking panel width; RasOptions::factor_panel, tune_factor).
+  int factor_panel(const DomainBatch& b, bool fp32, hipStream_t ls, const std::vector<int64_t>& mptr);
   // Picks each level's solve workgroup size by timing (an autotuner over
   // launch shapes; the factorisation tiles stay fixed).
   void tune_solves();

@@ -21,6 +21,10 @@ Operator::Operator(const Model& model, const double* d_pts, int64_t n, const Fmm
   }
 }
 
+void Operator::prebuild(hipStream_t s) {
+  for (auto& f : fmm_) f->prebuild_operator(s);
+}
+
 void Operator::apply(const double* d_x, double* d_y) {
   // y[0:mu] = nugget w + sum_rbf A w + P c; y[mu:] = P^T w   (operator.hpp:56-82)
   tmp_.resize(static_cast<size_t>(mu_));
@@ -121,36 +125,72 @@ Convergence ResidualEvaluator::converged(const double* d_weights, double toleran
   return c;
 }
 
+void resolve_fit_options(const Model& model, FitOptions& opt) {
+  if (model.cpd_order() < 2) return;
+  if (opt.cpd2_auto && opt.op_cfg.precision != kFp64) {
+    // Triharmonic systems are ill-conditioned: FGMRES on the mixed operator
+    // stagnates and escalates to FP64 after wasted steps (measured from 3e4
+    // to 1e6 points), so the fit starts with the FP64 operator.
+    opt.op_cfg.precision = kFp64;
+    opt.op_cfg.order = std::max(opt.op_cfg.order, 10);
+  }
+  if (!opt.use_level_cfg) {
+    // Their multilevel updates need FP64 too: with the default mixed-precision
+    // updates FGMRES stalls above 1e-3 relative (measured at 2e5 points), with
+    // FP64 order-8 updates it converges.
+    opt.level_cfg = FmmConfig{};
+    opt.level_cfg.precision = kFp64;
+    opt.level_cfg.order = 8;
+    opt.use_level_cfg = true;
+  }
+}
+
+RasOptions fit_ras_options(const Model& model, const FitOptions& opt, int64_t n) {
+  RasOptions ro;
+  ro.level_cfg = opt.level_cfg;
+  ro.use_level_cfg = opt.use_level_cfg;
+  ro.verbose = opt.verbose;
+  ro.tune_solves = opt.tune_solves;
+  ro.tune_factor = opt.tune_factor;
+  ro.factor_panel = opt.factor_panel;
+  // With the reference's 2048-point coarse grid, triharmonic fits stall above
+  // about 5e5 points (at 1e-3 absolute at 1e6 after 100 steps); a coarse grid
+  // of 4096 points from 2e5 and 6000 from 8e5 converges (1e6 points: 23 steps
+  // to 1e-4).
+  if (model.cpd_order() >= 2 && opt.cpd2_auto) ro.n_coarsest = n >= 800000 ? 6000 : n >= 200000 ? 4096 : 2048;
+  // Triharmonic (CPD order 2) subdomain matrices are too ill-conditioned for
+  // FP32 factors; an FP64 operator asks for the FP64 fit throughout.
+  ro.fp32_factors = opt.ras_fp32_factors == 1 ||
+                    (opt.ras_fp32_factors == -1 && model.cpd_order() <= 1 && opt.op_cfg.precision != kFp64);
+  return ro;
+}
+
 Solver::Solver(const Model& model, const std::vector<double>& points, const FitOptions& opt, hipStream_t s)
     : model_(model), opt_(opt), s_(s), mu_(static_cast<int64_t>(points.size() / 3)), l_(model.l()) {
   auto t0 = std::chrono::steady_clock::now();
   model_.validate();
   if (mu_ < l_) throw_invalid("points.rows() must be greater than or equal to " + std::to_string(l_));
   d_pts_.upload(points.data(), points.size(), s_);
+  resolve_fit_options(model_, opt_);
   op_ = std::make_unique<Operator>(model_, d_pts_.get(), mu_, opt_.op_cfg, s_);
   res_ = std::make_unique<ResidualEvaluator>(model_, d_pts_.get(), mu_, opt_.res_cfg, s_);
-  if (!opt_.use_level_cfg && model_.cpd_order() >= 2) {
-    // Kernels of conditional positive definiteness order 2 (triharmonic3d)
-    // give ill-conditioned systems whose multilevel updates need FP64: with
-    // the default mixed-precision updates FGMRES stalls above 1e-3 relative
-    // (measured at 2e5 points), with FP64 order-8 updates it converges.
-    opt_.level_cfg = FmmConfig{};
-    opt_.level_cfg.precision = kFp64;
-    opt_.level_cfg.order = 8;
-    opt_.use_level_cfg = true;
+  {
+    int dev = 0;
+    PAM_HIP(hipGetDevice(&dev));
+    prebuild_ = std::async(std::launch::async, [this, dev] {
+      PAM_HIP(hipSetDevice(dev));
+      hipStream_t ps = nullptr;
+      PAM_HIP(hipStreamCreateWithFlags(&ps, hipStreamNonBlocking));
+      try {
+        res_->prebuild(ps);
+      } catch (...) {
+        hipStreamDestroy(ps);
+        throw;
+      }
+      PAM_HIP(hipStreamDestroy(ps));
+    });
   }
-  if (opt_.use_preconditioner) {
-    RasOptions ro;
-    ro.level_cfg = opt_.level_cfg;
-    ro.use_level_cfg = opt_.use_level_cfg;
-    ro.verbose = opt_.verbose;
-    ro.tune_solves = opt_.tune_solves;
-    // Triharmonic (CPD order 2) subdomain matrices are too ill-conditioned for
-    // FP32 factors; an FP64 operator asks for the FP64 fit throughout.
-    ro.fp32_factors = opt_.ras_fp32_factors == 1 ||
-                      (opt_.ras_fp32_factors == -1 && model_.cpd_order() <= 1 && opt_.op_cfg.precision != kFp64);
-    pc_ = std::make_unique<RasPreconditioner>(model_, points, s_, ro);
-  }
+  if (opt_.use_preconditioner) pc_ = std::make_unique<RasPreconditioner>(model_, points, s_, fit_ras_options(model_, opt_, mu_));
   if (l_ > 0) {
     p_orth_.resize(static_cast<size_t>(mu_) * l_);
     dev_poly_matrix(model_.poly_degree, d_pts_.get(), mu_, p_orth_.get(), s_);
@@ -322,6 +362,7 @@ std::vector<double> Solver::solve(const std::vector<double>& values, double tole
     }
     PAM_HIP(hipStreamSynchronize(s_));
   }
+  if (prebuild_.valid()) prebuild_.get();
   res_->set_values(values);
   FitStats st;
   st.setup_seconds = setup_s_;

@@ -2,6 +2,7 @@
 // FGMRES+RAS solver with mixed-precision iterative refinement.
 #pragma once
 
+#include <future>
 #include <memory>
 #include <vector>
 
@@ -17,6 +18,9 @@ class Operator : public LinearOperator {
   Operator(const Model& model, const double* d_pts, int64_t n, const FmmConfig& cfg, hipStream_t s);
   void apply(const double* d_x, double* d_y) override;
   int64_t size() const override { return mu_ + l_; }
+  // Builds the M2L operators of the FMM evaluators on stream s (see
+  // FmmEvaluator::prebuild_operator).
+  void prebuild(hipStream_t s);
 
  private:
   Model model_;
@@ -46,6 +50,8 @@ class ResidualEvaluator {
   Convergence converged(const double* d_weights, double tolerance);
   // r = values - (A + nugget) w - P c at all sites (FP64), device output.
   void full_residual(const double* d_weights, double* d_r);
+  // Builds the FP64 operator's M2L matrices on stream s.
+  void prebuild(hipStream_t s) { full_.prebuild(s); }
 
  private:
   Model model_;
@@ -86,6 +92,13 @@ struct FitOptions {
   // RAS fine-level factors in FP32 (RasOptions::fp32_factors): 1 on, 0 off,
   // -1 on when the operator is not FP64 and every rbf has CPD order <= 1.
   int ras_fp32_factors = -1;
+  // Time the RAS factorisation's forms (RasOptions::tune_factor), or fix one
+  // (RasOptions::factor_panel).
+  bool tune_factor = false;
+  int factor_panel = -1;
+  // Kernels of CPD order 2 (triharmonic3d): raise a mixed-precision operator
+  // to FP64 of order >= 10 and grow the RAS coarse grid with the point count.
+  bool cpd2_auto = true;
 };
 
 struct FitStats {
@@ -108,6 +121,12 @@ class GraphStep;
 class TimedOperator;
 
 // interpolation/solver.hpp:34-142 + fitter.hpp:21-26.
+// Applies the rules for kernels of CPD order 2 (FP64 operator with
+// cpd2_auto, FP64 level updates unless configured) to a fit's options.
+void resolve_fit_options(const Model& model, FitOptions& opt);
+// The preconditioner options of a fit of n points with resolved options.
+RasOptions fit_ras_options(const Model& model, const FitOptions& opt, int64_t n);
+
 class Solver {
  public:
   Solver(const Model& model, const std::vector<double>& points, const FitOptions& opt, hipStream_t s);
@@ -131,6 +150,11 @@ class Solver {
   std::unique_ptr<RasPreconditioner> pc_;
   Workspace1 ws_;
   double setup_s_ = 0.0;
+  // Build of the residual evaluator's M2L operator on a host thread and
+  // stream of its own, overlapping the preconditioner set-up; joined before
+  // the first residual check (and, being the last member, before the
+  // evaluator is destroyed).
+  std::future<void> prebuild_;
 };
 
 }  // namespace pam

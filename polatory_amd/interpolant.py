@@ -92,7 +92,8 @@ class FitOptions:
 
     def __init__(self, op=None, residual=None, level=None, use_preconditioner=True,
                  refine_rtol=None, verbose=False, use_graphs=True, keep_unconverged=False,
-                 tune_solves=False, ras_fp32_factors=None):
+                 tune_solves=False, ras_fp32_factors=None, tune_factor=False, factor_panel=None,
+                 cpd2_auto=True):
         d = _capi.PamFitOptions()
         _capi.call("pam_fit_options_default", ctypes.byref(d))
         self._c = d
@@ -110,6 +111,10 @@ class FitOptions:
         d.use_graphs = 1 if use_graphs else 0
         d.keep_unconverged = 1 if keep_unconverged else 0
         d.tune_solves = 1 if tune_solves else 0
+        d.tune_factor = 1 if tune_factor else 0
+        if factor_panel is not None:  # None: the rule or tune_factor (-1)
+            d.factor_panel = int(factor_panel)
+        d.cpd2_auto = 1 if cpd2_auto else 0
         if ras_fp32_factors is not None:  # None: the library's rule (-1)
             d.ras_fp32_factors = 1 if ras_fp32_factors else 0
 
@@ -290,12 +295,18 @@ class RbfFieldFunction:
 class RasPreconditioner:
     """Device multilevel RAS preconditioner (exposed for tests)."""
 
-    def __init__(self, model: Model, points):
+    def __init__(self, model: Model, points, options=None):
+        """options: a FitOptions, for the preconditioner that a fit with those
+        options builds (precision and form of the factors, coarse grid)."""
         self.model = model
         p = _points(points)
         self.n = p.shape[0]
         h = ctypes.c_void_p()
-        _capi.call("pam_ras_create", model._h, _ptr(p), p.shape[0], ctypes.byref(h))
+        if options is None:
+            _capi.call("pam_ras_create", model._h, _ptr(p), p.shape[0], ctypes.byref(h))
+        else:
+            _capi.call("pam_ras_create_with_options", model._h, _ptr(p), p.shape[0], ctypes.byref(options._c),
+                       ctypes.byref(h))
         self._h = h
 
     def __del__(self):

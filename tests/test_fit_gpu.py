@@ -410,3 +410,70 @@ def test_fit_with_single_precision_ras_factors(orc, kind, fp32):
         f"residual(oracle FP64) {res:.2e}")
     assert st["n_levels"] >= 2
     assert res <= 1e-5
+
+
+def test_fit_with_tuned_factor_panels(orc):
+    """The RAS factorisation's panel width picked by timing (tune_factor): the
+    ellipsoid fit still reaches its tolerance on the oracle's FP64 residual."""
+    pts, vals = ellipsoid_points(10_000, 9)
+    model = pa.Model(pa.Biharmonic3D(), 1)
+    ip = pa.Interpolant(model)
+    ip.fit(pts, vals, 1e-5, max_iter=100, options=pa.FitOptions(tune_factor=True))
+    res = oracle_residual(orc, model, pts, vals, ip.weights())
+    log(f"  tuned panels: iterations {ip.fit_stats()['iterations']} residual(oracle FP64) {res:.2e}")
+    assert res <= 1e-5
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_factor_forms_agree(orc, fp32):
+    """The batched RAS factorisation in its three forms (left-looking; right-
+    looking with 32- and 64-wide panels): each ellipsoid fit reaches its
+    tolerance on the oracle's FP64 residual, and with FP64 factors the three
+    preconditioners apply to a vector with the same result up to rounding."""
+    log("factor forms", "fp32" if fp32 else "fp64")
+    pts, vals = ellipsoid_points(10_000, 11)
+    model = pa.Model(pa.Biharmonic3D(), 1)
+    v = np.random.default_rng(12).normal(size=len(pts) + model.poly_basis_size())
+    outs = []
+    for form in (0, 32, 64):
+        opt = pa.FitOptions(ras_fp32_factors=fp32, factor_panel=form)
+        ip = pa.Interpolant(model)
+        ip.fit(pts, vals, 1e-5, max_iter=100, options=opt)
+        st = ip.fit_stats()
+        res = oracle_residual(orc, model, pts, vals, ip.weights())
+        log(f"  form {form}: iterations {st['iterations']} levels {st['n_levels']} residual(oracle FP64) {res:.2e}")
+        assert st["n_levels"] >= 2
+        assert res <= 1e-5
+        ras = pa.RasPreconditioner(model, pts, opt)
+        assert ras.n_levels() == st["n_levels"]
+        outs.append(ras.apply(v))
+    scale = np.abs(outs[1]).max()
+    for form, o in ((0, outs[0]), (64, outs[2])):
+        diff = np.abs(o - outs[1]).max() / scale
+        log(f"  preconditioner application, form {form} vs 32-wide panels: {diff:.2e}")
+        if not fp32:
+            assert diff <= 1e-6
+
+
+def test_triharmonic_fit_at_300k(orc):
+    """A triharmonic fit of 300,000 points in the unit cube reaches 1e-4 (the
+    solver's own FP64 residual) with FP64 operator and the 4096-point coarse
+    grid that the CPD-order-2 rule picks; the oracle's FP64 residual on 2,000
+    sampled sites is within the tolerance too."""
+    log("triharmonic 300k")
+    rng = np.random.default_rng(1300)
+    pts = rng.uniform(0, 1, (300_000, 3))
+    vals = sample_values(pts)
+    model = pa.Model(pa.Triharmonic3D(), 1)
+    ip = pa.Interpolant(model)
+    ip.fit(pts, vals, 1e-4, max_iter=100)
+    st = ip.fit_stats()
+    sub = rng.permutation(len(pts))[:2000]
+    w = ip.weights()
+    y = orc.direct_evaluate(orc_rbfs(orc, model), model.poly_degree(), pts, w, pts[sub])
+    res = np.abs(vals[sub] - y).max()
+    log(f"  iterations {st['iterations']} levels {st['n_levels']} escalations {st['escalations']} "
+        f"residual(device FP64) {st['residual']:.2e} residual(oracle FP64, 2000 sites) {res:.2e}")
+    assert st["converged"]
+    assert st["residual"] <= 1e-4
+    assert res <= 1e-4
