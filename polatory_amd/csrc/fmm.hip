@@ -371,54 +371,42 @@ __global__ void __launch_bounds__(kThreads) l2p_jobs_kernel(L2PJobs<T> jobs) {
   l2p_leaf<T, P>(g - j.first_leaf, j.leaf_start, j.pts, j.l, j.kpad, j.y_far, j.perm);
 }
 
-// Targets per thread of the near-field kernel: each source read from LDS
-// serves kP2PR pairs.
-constexpr int kP2PR = 4;
+// Targets per thread of the near-field kernel (each source read from LDS
+// serves R pairs): chosen per leaf in [4, kP2PMaxR].
+constexpr int kP2PMinR = 4, kP2PMaxR = 8;
+
+// The R and tp (target slots, a power of two) for a leaf of nt targets: the
+// pair with the fewest idle target slots R tp - nt (ties: the smaller R). A
+// fixed R = 4 rounds 257 targets up to 512 slots, and every idle slot costs
+// a full pass over the leaf's sources; on 1M uniform points (about 244 per
+// leaf) that was a quarter of the pairs computed.
+__device__ __forceinline__ int p2p_shape(int nt, int* tp_out) {
+  int best_r = kP2PMinR, best_tp = kThreads, best_cap = 1 << 30;
+  for (int r = kP2PMinR; r <= kP2PMaxR; r++) {
+    const int per = (nt + r - 1) / r;
+    int tp = 1;
+    while (tp < per && tp < kThreads) tp <<= 1;
+    const int passes = (per + tp - 1) / tp;
+    const int cap = r * tp * passes;
+    if (cap < best_cap) best_cap = cap, best_r = r, best_tp = tp;
+  }
+  *tp_out = best_tp;
+  return best_r;
+}
 
 // kComp: compensated (Kahan) sums per target and FP64 sums over the source
 // splits, for weights that cancel strongly (kriging).
-// One target leaf of one evaluation's near field.
-template <class T, int KIND, bool kComp>
-__device__ __forceinline__ void p2p_leaf(int leaf, const int32_t* __restrict__ tls, const int32_t* __restrict__ sls,
-                                         const int32_t* __restrict__ p2p, const V4<T>* __restrict__ tpts,
-                                         const V4<T>* __restrict__ spts, const double* __restrict__ y_far,
-                                         const int32_t* __restrict__ tperm, T pa, T pb, double* __restrict__ out) {
-  constexpr int CAP = 32768 / static_cast<int>(sizeof(V4<T>));
-  constexpr int R = kP2PR;
-  __shared__ V4<T> sp[CAP];
-  __shared__ int nb_start[28];
-  __shared__ int nb_src[27];
-  __shared__ T nb_off[27][3];
-  __shared__ int nb_count;
-  __shared__ T red[R][kThreads];
-  const int ts = tls[leaf], te = tls[leaf + 1];
-  const int nt = te - ts;
-  if (threadIdx.x == 0) {
-    int k = 0, total = 0;
-    for (int s = 0; s < 27; s++) {
-      int sl = p2p[static_cast<int64_t>(leaf) * 27 + s];
-      if (sl < 0) continue;
-      int a = sls[sl], b = sls[sl + 1];
-      nb_src[k] = a;
-      nb_start[k] = total;
-      nb_off[k][0] = T(2 * (s / 9 - 1));
-      nb_off[k][1] = T(2 * ((s / 3) % 3 - 1));
-      nb_off[k][2] = T(2 * (s % 3 - 1));
-      total += b - a;
-      k++;
-    }
-    nb_start[k] = total;
-    nb_count = k;
-  }
-  __syncthreads();
-  const int nnb = nb_count;
-  const int total = nb_start[nnb];
+// The passes of one target leaf with R targets per thread and tp target
+// slots (tp x nsplit threads; the sources staged through sp in chunks).
+template <class T, int KIND, bool kComp, int R>
+__device__ __forceinline__ void p2p_passes(int nt, int ts, int tp, int total, int nnb, const int* nb_start,
+                                           const int* nb_src, const T (*nb_off)[3], V4<T>* sp, int cap,
+                                           T (*red)[kThreads], const V4<T>* __restrict__ tpts,
+                                           const V4<T>* __restrict__ spts, const double* __restrict__ y_far,
+                                           const int32_t* __restrict__ tperm, T pa, T pb,
+                                           double* __restrict__ out) {
   // Every thread carries R targets (ti + r tp), so each source read from LDS
-  // serves R pairs; tp target slots x nsplit source splits fill the 256
-  // lanes, and a leaf of up to 256 R points takes a single pass.
-  const int per = (nt + R - 1) / R;
-  int tp = 1;
-  while (tp < per && tp < kThreads) tp <<= 1;
+  // serves R pairs; tp target slots x nsplit source splits fill the 256 lanes.
   const int nsplit = kThreads / tp;
   const int ti = threadIdx.x & (tp - 1);
   const int si = threadIdx.x / tp;
@@ -432,8 +420,8 @@ __device__ __forceinline__ void p2p_leaf(int leaf, const int32_t* __restrict__ t
       acc[r] = T(0);
       cmp[r] = T(0);
     }
-    for (int cb = 0; cb < total; cb += CAP) {
-      const int cnt = min(CAP, total - cb);
+    for (int cb = 0; cb < total; cb += cap) {
+      const int cnt = min(cap, total - cb);
       __syncthreads();
       for (int e = threadIdx.x; e < cnt; e += kThreads) {
         int g = cb + e;
@@ -481,6 +469,7 @@ __device__ __forceinline__ void p2p_leaf(int leaf, const int32_t* __restrict__ t
         for (int r = 0; r < R; r++) pair(q, r);
       }
     }
+    __syncthreads();  // red overlays the staged sources
 #pragma unroll
     for (int r = 0; r < R; r++) red[r][threadIdx.x] = kComp ? acc[r] - cmp[r] : acc[r];
     __syncthreads();
@@ -499,6 +488,58 @@ __device__ __forceinline__ void p2p_leaf(int leaf, const int32_t* __restrict__ t
     }
     __syncthreads();
   }
+}
+
+// One target leaf of one evaluation's near field.
+template <class T, int KIND, bool kComp>
+__device__ __forceinline__ void p2p_leaf(int leaf, const int32_t* __restrict__ tls, const int32_t* __restrict__ sls,
+                                         const int32_t* __restrict__ p2p, const V4<T>* __restrict__ tpts,
+                                         const V4<T>* __restrict__ spts, const double* __restrict__ y_far,
+                                         const int32_t* __restrict__ tperm, T pa, T pb, double* __restrict__ out) {
+  constexpr int CAP = 32768 / static_cast<int>(sizeof(V4<T>));
+  __shared__ V4<T> sp[CAP];
+  __shared__ int nb_start[28];
+  __shared__ int nb_src[27];
+  __shared__ T nb_off[27][3];
+  __shared__ int nb_count;
+  // The split partials of a pass, over the staged sources (read after them).
+  static_assert(kP2PMaxR * kThreads * sizeof(T) <= sizeof(V4<T>) * CAP, "partials exceed the staging buffer");
+  T (*red)[kThreads] = reinterpret_cast<T (*)[kThreads]>(sp);
+  const int ts = tls[leaf], te = tls[leaf + 1];
+  const int nt = te - ts;
+  if (threadIdx.x == 0) {
+    int k = 0, total = 0;
+    for (int s = 0; s < 27; s++) {
+      int sl = p2p[static_cast<int64_t>(leaf) * 27 + s];
+      if (sl < 0) continue;
+      int a = sls[sl], b = sls[sl + 1];
+      nb_src[k] = a;
+      nb_start[k] = total;
+      nb_off[k][0] = T(2 * (s / 9 - 1));
+      nb_off[k][1] = T(2 * ((s / 3) % 3 - 1));
+      nb_off[k][2] = T(2 * (s % 3 - 1));
+      total += b - a;
+      k++;
+    }
+    nb_start[k] = total;
+    nb_count = k;
+  }
+  __syncthreads();
+  const int nnb = nb_count;
+  const int total = nb_start[nnb];
+  int tp = kThreads;
+  const int r = p2p_shape(nt, &tp);
+#define PAM_P2P_PASSES(RR)                                                                                     \
+  p2p_passes<T, KIND, kComp, RR>(nt, ts, tp, total, nnb, nb_start, nb_src, nb_off, sp, CAP, red, tpts, spts, \
+                                 y_far, tperm, pa, pb, out)
+  switch (r) {
+    case 4: PAM_P2P_PASSES(4); break;
+    case 5: PAM_P2P_PASSES(5); break;
+    case 6: PAM_P2P_PASSES(6); break;
+    case 7: PAM_P2P_PASSES(7); break;
+    default: PAM_P2P_PASSES(8); break;
+  }
+#undef PAM_P2P_PASSES
 }
 
 // The near fields of up to kMaxFmmJobs evaluations (same precision, kernel

@@ -89,7 +89,7 @@ void ResidualEvaluator::full_residual(const double* d_weights, double* d_r) {
   dev_axpy(-1.0, y_.get(), d_r, mu_, s_);
 }
 
-Convergence ResidualEvaluator::converged(const double* d_weights, double tolerance) {
+Convergence ResidualEvaluator::converged(const double* d_weights, double tolerance, bool allow_full) {
   // residual_evaluator.hpp:56-108
   Convergence c;
   int64_t ns = static_cast<int64_t>(sample_.size());
@@ -115,6 +115,7 @@ Convergence ResidualEvaluator::converged(const double* d_weights, double toleran
       c.converged = true;
       return c;
     }
+    if (!allow_full) return c;
   }
   tmp_.resize(static_cast<size_t>(mu_));
   full_residual(d_weights, tmp_.get());
@@ -415,11 +416,31 @@ std::vector<double> Solver::solve(const std::vector<double>& values, double tole
     }
     bool done = false, restart = false, stagnated = false;
     std::vector<double> inner_rel;
+    // A check at every site costs an FP64 evaluation of the whole operator.
+    // After one has failed in this cycle, the next is made when the inner
+    // residual has fallen enough for the full residual, scaled by the same
+    // factor, to be within twice the tolerance; every third step at the
+    // latest; and at the last step allowed.
+    double full_res = 0.0, full_inner = 0.0;
+    int since_full = 0;
     while (true) {
       solver.solution_vector(x.get());
       dev_axpy(1.0, xb.get(), x.get(), m, s_);
       auto tc = std::chrono::steady_clock::now();
-      Convergence c = res_->converged(x.get(), tolerance);
+      bool allow_full = true;
+      if (full_res > 0.0 && full_inner > 0.0) {
+        const double predicted = full_res * solver.relative_residual() / full_inner;
+        allow_full = predicted <= 2.0 * tolerance || since_full >= 2 ||
+                     total_iter + solver.iteration_count() >= max_iter;
+      }
+      Convergence c = res_->converged(x.get(), tolerance, allow_full);
+      if (c.exact && !c.converged) {
+        full_res = c.residual;
+        full_inner = solver.relative_residual();
+        since_full = 0;
+      } else {
+        since_full++;
+      }
       if (opt_.verbose) t_check += std::chrono::duration<double>(std::chrono::steady_clock::now() - tc).count();
       st.history.push_back(c.residual);
       st.residual = c.residual;

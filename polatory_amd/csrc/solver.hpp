@@ -47,7 +47,9 @@ class ResidualEvaluator {
   ResidualEvaluator(const Model& model, const double* d_pts, int64_t n, const FmmConfig& full_cfg,
                     hipStream_t s);
   void set_values(const std::vector<double>& values);
-  Convergence converged(const double* d_weights, double tolerance);
+  // allow_full = false: a sample within the tolerance is reported as not
+  // converged (exact = false) instead of being confirmed at every site.
+  Convergence converged(const double* d_weights, double tolerance, bool allow_full = true);
   // r = values - (A + nugget) w - P c at all sites (FP64), device output.
   void full_residual(const double* d_weights, double* d_r);
   // Builds the FP64 operator's M2L matrices on stream s.
