@@ -155,6 +155,60 @@ def sample_f(p):
     return np.sin(3 * p[:, 0]) * np.cos(2 * p[:, 1]) + 0.5 * p[:, 2] ** 2
 
 
+def config1_section(pa, no_cpu):
+    """The plain fit + evaluate that polatory runs on the CPU today: 10,000
+    biharmonic3d sites in the unit cube with a degree-1 polynomial, f as in
+    the other fits, tolerance 1e-4; then 100,000 evaluation points. The GPU
+    fit and evaluation (default options) beside the reference algorithm on the
+    host cores (oracle/cpu_fit.py: FGMRES + multilevel RAS with FP64 direct
+    sums; oracle/cpu_fmm.c at interpolation order 6, polatory's default
+    evaluation accuracy), both checked by the FP64 direct sum."""
+    import oracle
+    from oracle import cpu_fit
+    pts = np.random.default_rng(19).uniform(0, 1, (10_000, 3))
+    vals = sample_f(pts)
+    q = np.random.default_rng(20).uniform(0, 1, (100_000, 3))
+    sub = np.random.default_rng(21).permutation(len(q))[:2_000]
+    r = oracle.make_rbf(oracle.BH3)
+    out = {"workload": "10,000 bh3 sites in [0,1]^3, degree 1, tol 1e-4; evaluate at 100,000 points"}
+    ip = pa.Interpolant(pa.Model(pa.Biharmonic3D(), poly_degree=1))
+    ip.fit(pts, vals, 1e-4)  # warm-up (code objects, allocations)
+    ip = pa.Interpolant(pa.Model(pa.Biharmonic3D(), poly_degree=1))
+    t0 = time.perf_counter()
+    ip.fit(pts, vals, 1e-4)
+    t_fit = time.perf_counter() - t0
+    w = ip.weights()
+    res = float(np.abs(oracle.direct_evaluate([r], 1, pts, w, pts) - vals).max())
+    ip.evaluate(q[:10])
+    t0 = time.perf_counter()
+    y = ip.evaluate(q)
+    t_eval = time.perf_counter() - t0
+    ref = oracle.direct_evaluate([r], 1, pts, w, q[sub])
+    out["gpu"] = {"fit_s": t_fit, "iterations": ip.fit_stats()["iterations"], "fp64_residual_oracle": res,
+                  "eval_s": t_eval, "eval_rel_err_vs_fp64_direct": float(np.abs(y[sub] - ref).max() / np.abs(ref).max())}
+    log("config1 gpu", json.dumps(out["gpu"]))
+    if no_cpu:
+        return out
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    wc, st = cpu_fit.cpu_fit([r], 1, 0.0, pts, vals, 1e-4, threads=cores)
+    t_cfit = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    yc, _ = oracle.cpu_fmm_evaluate(r, pts, wc[:len(pts)], q, ((0, 0, 0), (1, 1, 1)), 6)
+    yc += wc[len(pts)] + q @ wc[len(pts) + 1:]
+    t_ceval = time.perf_counter() - t0
+    refc = oracle.direct_evaluate([r], 1, pts, wc, q[sub])
+    out["cpu_baseline"] = {"kind": "port", "cores": cores, "cpu": cpu_model(), "fit_s": t_cfit,
+                           "iterations": st["iterations"], "fp64_residual": st["residual"], "levels": st["n_levels"],
+                           "eval_s": t_ceval,
+                           "eval_rel_err_vs_fp64_direct": float(np.abs(yc[sub] - refc).max() / np.abs(refc).max()),
+                           "sample": "the whole workload: oracle/cpu_fit.py (FGMRES + multilevel RAS, FP64 direct "
+                                     "sums, LAPACK factors) and oracle/cpu_fmm.c (order 6), one run each"}
+    out["fit_speedup_vs_cpu"] = t_cfit / t_fit
+    log("config1 cpu", json.dumps(out["cpu_baseline"]))
+    return out
+
+
 def ellipsoid_points(n_surf, seed=8):
     """333,334 area-uniform points on the ellipsoid (1, 0.7, 0.5) (rejection on the
     sphere's image) with value 0, plus the same points offset by +-0.01 along the
@@ -422,6 +476,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
     ap.add_argument("--no-kriging", action="store_true", help="skip the kriging fit and prediction")
     ap.add_argument("--no-gradient", action="store_true", help="skip the gradient evaluation")
+    ap.add_argument("--no-config1", action="store_true", help="skip the 10k-site fit and evaluate")
     ap.add_argument("--serve-models", type=int, default=32)
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--serve-rate", type=float, default=200.0)
@@ -532,6 +587,7 @@ def main():
     gradient = None if args.no_gradient else gradient_section(
         pa, torch, src, w, trg, sub, (("mixed", cfg32), ("fp64", cfg64)), args.steps, args.warmup)
     fit = None if args.no_fit else fit_section(pa)
+    config1 = None if args.no_config1 else config1_section(pa, args.no_cpu)
     kriging = None if args.no_kriging else kriging_section(pa)
     serve = None if args.no_serve else serve_section(pa, args.serve_models, args.serve_requests,
                                                      args.serve_rate)
@@ -564,6 +620,7 @@ def main():
         "stage_rooflines": stage_roofs,
         "cpu_baseline": cpu,
         "fit": fit,
+        "config1": config1,
         "kriging": kriging,
         "gradient": gradient,
         "serve": serve,

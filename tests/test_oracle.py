@@ -273,3 +273,59 @@ def test_cpu_fmm_floater_hormann(orc, fmm_case):
     a, _ = orc.cpu_fmm_evaluate(r, src, w, trg, ((0, 0, 0), (1, 1, 1)), 6, fh_degree=5)
     b, _ = orc.cpu_fmm_evaluate(r, src, w, trg, ((0, 0, 0), (1, 1, 1)), 6, fh_degree=-1)
     np.testing.assert_array_equal(a, b)
+
+
+# ---- the reference-algorithm CPU fit (oracle/cpu_fit.py), the config-1 CPU baseline ----
+
+def test_cpu_fit_kernel_matrix_matches_closed_forms(orc):
+    """The baseline's kernel matrices equal the oracle's pinned closed forms,
+    anisotropy included (a transposed anisotropy or a sign slip fails)."""
+    from oracle import cpu_fit
+    rng = np.random.default_rng(93)
+    a, b = rng.uniform(0, 1, (7, 3)), rng.uniform(0, 1, (5, 3))
+    aniso = np.array([[1.0, 0.2, 0.0], [0.0, 0.7, 0.3], [0.1, 0.0, 1.4]])
+    for kind, params in ((orc.BH3, [1.3, 0.0]), (orc.TH3, [0.7, 0.1]), (orc.EXP, [2.0, 0.8])):
+        r = orc.make_rbf(kind, params, aniso)
+        k = cpu_fit.kernel_matrix(r, a, b)
+        ref = np.array([[orc.rbf_evaluate(r, a[i] - b[j]) for j in range(5)] for i in range(7)])
+        np.testing.assert_allclose(k, ref, rtol=1e-13, atol=1e-15)
+
+
+def test_cpu_fit_single_level_is_exact(orc):
+    """Below 2048 points the preconditioner is the coarse grid alone, the exact
+    solve (coarse_grid.hpp:103-131): its weights interpolate the data."""
+    from oracle import cpu_fit
+    rng = np.random.default_rng(94)
+    pts = rng.uniform(0, 1, (1500, 3))
+    v = np.sin(3 * pts[:, 0]) * np.cos(2 * pts[:, 1]) + 0.5 * pts[:, 2] ** 2
+    r = orc.make_rbf(orc.BH3)
+    pc = cpu_fit.RasPreconditioner([r], 1, 0.0, pts, np.random.default_rng(0))
+    assert pc.n_levels == 1
+    w = pc.apply(np.concatenate([v, np.zeros(4)]))
+    pc.pool.shutdown()
+    y = orc.direct_evaluate([r], 1, pts, w, pts)
+    assert np.abs(y - v).max() <= 1e-9 * np.abs(v).max()
+    # the polynomial part annihilates the weights: P^T w = 0
+    assert np.abs(orc.monomials(1, pts).T @ w[:1500]).max() <= 1e-8 * np.abs(w).max()
+
+
+def test_cpu_fit_two_levels_reaches_tolerance(orc):
+    """3,000 points (two levels: fine domains of at most 1,024 points and the
+    2,048-point coarse grid): the fit reaches 1e-4 on the oracle's FP64
+    residual within 10 steps, and every point is the inner point of exactly
+    one fine domain."""
+    from oracle import cpu_fit
+    rng = np.random.default_rng(95)
+    pts = rng.uniform(0, 1, (3000, 3))
+    v = np.sin(3 * pts[:, 0]) * np.cos(2 * pts[:, 1]) + 0.5 * pts[:, 2] ** 2
+    r = orc.make_rbf(orc.BH3)
+    w, st = cpu_fit.cpu_fit([r], 1, 0.0, pts, v, 1e-4, max_iter=30, threads=4)
+    assert st["n_levels"] == 2 and st["converged"] and st["iterations"] <= 10
+    y = orc.direct_evaluate([r], 1, pts, w, pts)
+    assert np.abs(y - v).max() <= 1e-4
+    doms = cpu_fit.divide_domains(pts, np.arange(3000), [0, 1, 2, 3])
+    inner = np.zeros(3000, dtype=int)
+    for idx, inn in doms:
+        assert len(idx) <= 1024 + 4
+        np.add.at(inner, idx[inn], 1)
+    assert np.all(inner == 1)  # every point is the inner point of exactly one domain
