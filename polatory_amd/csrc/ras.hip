@@ -263,11 +263,15 @@ struct CholMfma<float> {
   static constexpr float kTiny = 1e-30f;
 };
 
-// Panel step k0 of the factorisation of the n x n matrix a (column-major,
-// lower triangle): L11 by one wavefront and its inverse, then L21 = A21 L11^-T.
-// Ends with a barrier.
 template <class T, int NB>
-__device__ void chol_panel(CholSmem<T, NB>& sm, T* __restrict__ a, int n, int k0, int32_t* __restrict__ flag) {
+__device__ void chol_rows(T* __restrict__ a, int n, int k0, const T (*inv)[NB + 1], int first, int stride);
+
+// Diagonal block of panel step k0 of the factorisation of the n x n matrix a
+// (column-major, lower triangle): L11 by one wavefront, stored, and its
+// inverse left in sm.dt. chol_panel adds L21 = A21 L11^-T and ends with a
+// barrier.
+template <class T, int NB>
+__device__ void chol_diag(CholSmem<T, NB>& sm, T* __restrict__ a, int n, int k0, int32_t* __restrict__ flag) {
   const int lda = n;
   const int kb = min(NB, n - k0);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -319,8 +323,24 @@ __device__ void chol_panel(CholSmem<T, NB>& sm, T* __restrict__ a, int n, int k0
     int rr = e % kb, cc = e / kb;
     if (rr >= cc) a[static_cast<int64_t>(k0 + cc) * lda + k0 + rr] = sm.dg[rr][cc];
   }
-  // L21 = A21 L11^-T, row by row
-  for (int i = k0 + kb + tid; i < n; i += kT) {
+}
+
+template <class T, int NB>
+__device__ void chol_panel(CholSmem<T, NB>& sm, T* __restrict__ a, int n, int k0, int32_t* __restrict__ flag) {
+  chol_diag(sm, a, n, k0, flag);
+  const int kb = min(NB, n - k0);
+  chol_rows<T, NB>(a, n, k0, reinterpret_cast<const T (*)[NB + 1]>(&sm.dt[0][0]), k0 + kb + static_cast<int>(threadIdx.x),
+                   kT);
+  __syncthreads();
+}
+
+// Rows i = first, first + stride, ... < n of L21 = A21 L11^-T, given the
+// inverse of the panel's diagonal block.
+template <class T, int NB>
+__device__ void chol_rows(T* __restrict__ a, int n, int k0, const T (*inv)[NB + 1], int first, int stride) {
+  const int lda = n;
+  const int kb = min(NB, n - k0);
+  for (int i = first; i < n; i += stride) {
     T x[NB];
 #pragma unroll
     for (int t = 0; t < NB; t++) x[t] = t < kb ? a[static_cast<int64_t>(k0 + t) * lda + i] : T(0);
@@ -333,7 +353,6 @@ __device__ void chol_panel(CholSmem<T, NB>& sm, T* __restrict__ a, int n, int k0
       a[static_cast<int64_t>(k0 + c) * lda + i] = v;
     }
   }
-  __syncthreads();
 }
 
 // Trailing tile (I, J), I >= J, of panel step k0: A22 -= L21 L21^T on FP64
@@ -648,12 +667,26 @@ void launch_cholesky_batch(int nb, int n_dom, const int64_t* ptr, int l, const i
   PAM_LAUNCH_CHECK();
 }
 
-// A single large matrix: one launch per panel step for the panel (one
-// workgroup) and one for its trailing tiles (one workgroup per tile).
+// A single large matrix, per panel step: the diagonal block by one
+// workgroup (its inverse written out), the rows below it by one thread per
+// row, the trailing tiles by one workgroup per tile.
 __global__ void __launch_bounds__(kT)
-cholesky_panel_kernel(double* __restrict__ a, int n, int k0, int32_t* __restrict__ flag) {
+cholesky_panel_kernel(double* __restrict__ a, int n, int k0, int32_t* __restrict__ flag, double* __restrict__ inv_out) {
   __shared__ CholSmem<double, kNb> sm;
-  chol_panel(sm, a, n, k0, flag);
+  chol_diag(sm, a, n, k0, flag);
+  const double (*inv)[kNb + 1] = reinterpret_cast<const double (*)[kNb + 1]>(&sm.dt[0][0]);
+  for (int e = threadIdx.x; e < kNb * kNb; e += kT) inv_out[e] = inv[e / kNb][e % kNb];
+}
+
+// The rows below the panel's diagonal block, spread over workgroups.
+__global__ void __launch_bounds__(kT)
+cholesky_rows_kernel(double* __restrict__ a, int n, int k0, const double* __restrict__ inv_in) {
+  __shared__ double inv[kNb][kNb + 1];
+  for (int e = threadIdx.x; e < kNb * kNb; e += kT) inv[e / kNb][e % kNb] = inv_in[e];
+  __syncthreads();
+  const int kb = min(kNb, n - k0);
+  chol_rows<double, kNb>(a, n, k0, inv, k0 + kb + static_cast<int>(blockIdx.x * kT + threadIdx.x),
+                         static_cast<int>(gridDim.x * kT));
 }
 
 __global__ void __launch_bounds__(kT)
@@ -1408,6 +1441,30 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
       for (int k = 0; k < 3; k++) a_pts[3 * i + k] = a[3 * k] * p[0] + a[3 * k + 1] * p[1] + a[3 * k + 2] * p[2];
     }
   }
+  // A P depends on the points only: it is computed on a host thread and a
+  // stream of its own (ras_preconditioner.hpp:173-188) while this thread
+  // divides the levels and the device factors them.
+  std::future<void> ap_done;
+  if (n_levels_ > 1 && l_ > 0) {
+    p_orth_.resize(static_cast<size_t>(mu_) * l_);
+    dev_poly_matrix(model_.poly_degree, d_pts_.get(), mu_, p_orth_.get(), s_);
+    dev_orthonormalize_cols(p_orth_.get(), mu_, l_, ws_, s_);
+    PAM_HIP(hipStreamSynchronize(s_));
+    int dev = 0;
+    PAM_HIP(hipGetDevice(&dev));
+    ap_done = std::async(std::launch::async, [this, dev] {
+      PAM_HIP(hipSetDevice(dev));
+      hipStream_t as = nullptr;
+      PAM_HIP(hipStreamCreateWithFlags(&as, hipStreamNonBlocking));
+      try {
+        compute_ap(as);
+      } catch (...) {
+        hipStreamDestroy(as);
+        throw;
+      }
+      PAM_HIP(hipStreamDestroy(as));
+    });
+  }
   // one positive-definiteness flag per level (0 = the coarse grid)
   flag_.resize(static_cast<size_t>(n_levels_));
   flag_.zero(s_);
@@ -1502,12 +1559,7 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
   w_.resize(mu_ + l_);
   total_.resize(mu_ + l_);
   dot_.resize(10);
-  if (n_levels_ > 1 && l_ > 0) {
-    p_orth_.resize(static_cast<size_t>(mu_) * l_);
-    dev_poly_matrix(model_.poly_degree, d_pts_.get(), mu_, p_orth_.get(), s_);
-    dev_orthonormalize_cols(p_orth_.get(), mu_, l_, ws_, s_);
-    compute_ap();
-  }
+  if (ap_done.valid()) ap_done.get();
   PAM_HIP(hipStreamSynchronize(s_));
   tick("A P done");
   if (opt_.tune_solves) {
@@ -1604,13 +1656,16 @@ void RasPreconditioner::setup_batch(DomainBatch& b, const std::vector<Domain>& d
   launch_assemble_m<double>(l_, max_n, b.n_dom, rs, b.ptr.get(), tmp->rows.get(), b.mat_ptr.get(), b.mat.get(), ls);
   const int n0 = static_cast<int>(b.h_ptr.back() - l_);
   if (b.n_dom == 1 && n0 > 4 * kTile) {
-    // One large matrix (the coarse grid): a workgroup per trailing tile
+    // One large matrix (the coarse grid): many workgroups per panel step
     // instead of one workgroup for the whole factorisation.
+    tmp->inv.resize(static_cast<size_t>(kNb) * kNb);
     for (int k0 = 0; k0 < n0; k0 += kNb) {
-      cholesky_panel_kernel<<<1, kT, 0, ls>>>(b.mat.get(), n0, k0, flag);
+      cholesky_panel_kernel<<<1, kT, 0, ls>>>(b.mat.get(), n0, k0, flag, tmp->inv.get());
       PAM_LAUNCH_CHECK();
       const int rem = n0 - std::min(kNb, n0 - k0) - k0;
       if (rem <= 0) break;
+      cholesky_rows_kernel<<<ceil_div(rem, kT), kT, 0, ls>>>(b.mat.get(), n0, k0, tmp->inv.get());
+      PAM_LAUNCH_CHECK();
       const int nt = (rem + kTile - 1) / kTile;
       cholesky_trailing_kernel<<<nt * (nt + 1) / 2, kT, 0, ls>>>(b.mat.get(), n0, k0);
       PAM_LAUNCH_CHECK();
@@ -1762,22 +1817,22 @@ void RasPreconditioner::tune_solves() {
   (void)hipEventDestroy(e1);
 }
 
-void RasPreconditioner::compute_ap() {
+void RasPreconditioner::compute_ap(hipStream_t s) {
   // ap = A P (ras_preconditioner.hpp:173-188), with the level-update configuration
   ap_.resize(static_cast<size_t>(mu_) * l_);
   Bbox3 bb;
   for (int64_t i = 0; i < mu_; i++) bb.add(&h_pts_[3 * i]);
-  SymmetricEvaluator ev(model_, bb.lo, bb.hi, INFINITY, s_);
+  SymmetricEvaluator ev(model_, bb.lo, bb.hi, INFINITY, s);
   if (opt_.use_level_cfg) ev.set_fmm_config(opt_.level_cfg);
   ev.set_points(d_pts_.get(), mu_);
   DevBuf<double> wcol(static_cast<size_t>(mu_ + l_));
   for (int k = 0; k < l_; k++) {
-    dev_fill(0.0, wcol.get(), mu_ + l_, s_);
-    dev_copy(p_orth_.get() + static_cast<int64_t>(k) * mu_, wcol.get(), mu_, s_);
+    dev_fill(0.0, wcol.get(), mu_ + l_, s);
+    dev_copy(p_orth_.get() + static_cast<int64_t>(k) * mu_, wcol.get(), mu_, s);
     ev.set_weights(wcol.get());
     ev.evaluate(ap_.get() + static_cast<int64_t>(k) * mu_);
   }
-  PAM_HIP(hipStreamSynchronize(s_));
+  PAM_HIP(hipStreamSynchronize(s));
 }
 
 void RasPreconditioner::set_level_config(const FmmConfig& cfg) {
@@ -1785,7 +1840,7 @@ void RasPreconditioner::set_level_config(const FmmConfig& cfg) {
   opt_.level_cfg = cfg;
   opt_.use_level_cfg = true;
   evaluators_.clear();
-  if (n_levels_ > 1 && l_ > 0) compute_ap();
+  if (n_levels_ > 1 && l_ > 0) compute_ap(s_);
 }
 
 Evaluator& RasPreconditioner::evaluator(int src, int trg) {

@@ -112,7 +112,7 @@ class RasPreconditioner : public LinearOperator {
     std::vector<int32_t> idx;
     std::vector<uint8_t> inner;
     std::vector<int64_t> mptr;
-    DevBuf<double> u, atl, rows;
+    DevBuf<double> u, atl, rows, inv;
   };
   std::vector<std::unique_ptr<SetupTmp>> setup_tmp_;
   void solve(int level, const double* d_res, double* d_w);
@@ -125,7 +125,8 @@ class RasPreconditioner : public LinearOperator {
   void tune_solves();
   void update_residuals(int src, int trg, const double* d_w, double* d_res);
   void orthogonalize(double* d_w, double* d_res);
-  void compute_ap();
+  // ap = A P with the level-update configuration, on stream s (synchronized).
+  void compute_ap(hipStream_t s);
   Evaluator& evaluator(int src, int trg);
 
   Model model_;
