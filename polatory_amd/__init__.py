@@ -23,11 +23,24 @@ def device_count():
     return n.value
 
 
+def device_cache_release():
+    """Frees the device buffers the library keeps for reuse on the current GPU."""
+    _capi.call("pam_device_cache_release")
+
+
+def device_cache_bytes():
+    """Bytes of released device buffers held for reuse on the current GPU."""
+    import ctypes
+    n = ctypes.c_size_t()
+    _capi.call("pam_device_cache_bytes", ctypes.byref(n))
+    return n.value
+
+
 __all__ = ["Biharmonic3D", "Triharmonic3D", "CovExponential", "Rbf", "make_rbf",
            "FmmConfig", "FmmEvaluator", "FmmSymmetricEvaluator", "FmmGradientTransposeEvaluator", "FmmGradientEvaluator",
            "make_fmm_gradient_evaluator", "FmmHessianEvaluator", "make_fmm_hessian_evaluator",
            "make_fmm_evaluator", "make_fmm_symmetric_evaluator", "make_fmm_hessian_symmetric_evaluator", "make_fmm_gradient_transpose_evaluator", "direct_evaluate", "direct_evaluate_gradient",
-           "device_count", "Model",
+           "device_count", "device_cache_release", "device_cache_bytes", "Model",
            "Interpolant", "FitOptions", "Operator", "RasPreconditioner", "fgmres_dense", "POLY_DEGREE_MIN", "distance_filter",
            "cross_validate", "RbfFieldFunction",
            "Server"]

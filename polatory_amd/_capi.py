@@ -97,6 +97,8 @@ _INT = ctypes.c_int
 _SIGS = {
     "pam_last_error": [ctypes.c_char_p, ctypes.c_size_t],
     "pam_device_count": [ctypes.POINTER(ctypes.c_int)],
+    "pam_device_cache_release": [],
+    "pam_device_cache_bytes": [ctypes.POINTER(ctypes.c_size_t)],
     "pam_fmm_evaluator_create": [ctypes.POINTER(PamRbf), _P, _P, ctypes.POINTER(_P)],
     "pam_fmm_symmetric_evaluator_create": [ctypes.POINTER(PamRbf), _P, _P, ctypes.POINTER(_P)],
     "pam_fmm_evaluator_destroy": [_P],

@@ -91,6 +91,17 @@ int pam_device_count(int* count) {
   });
 }
 
+int pam_device_cache_release(void) {
+  return guard([&] { pam::device_pool_trim(); });
+}
+
+int pam_device_cache_bytes(size_t* bytes) {
+  return guard([&] {
+    if (!bytes) pam::throw_invalid("null argument");
+    *bytes = pam::device_pool_cached_bytes();
+  });
+}
+
 int pam_fmm_evaluator_create(const pam_rbf* rbf, const double bbox_min[3], const double bbox_max[3],
                              pam_fmm_evaluator** out) {
   return create(rbf, bbox_min, bbox_max, out, false);

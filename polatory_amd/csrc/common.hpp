@@ -40,7 +40,23 @@ inline void hip_check(hipError_t e, const char* what, const char* file, int line
 #define PAM_HIP(x) ::pam::hip_check((x), #x, __FILE__, __LINE__)
 #define PAM_LAUNCH_CHECK() ::pam::hip_check(hipGetLastError(), "kernel launch", __FILE__, __LINE__)
 
-// Owning device allocation (hipMalloc'd, freed on destruction).
+// Device memory through a per-device cache of released blocks
+// (device_pool.hip): device_alloc never returns a block that work enqueued
+// before its release may still use.
+void* device_alloc(size_t bytes);
+void device_release(void* p);
+void device_pool_trim();             // frees every cached block of the current device
+size_t device_pool_cached_bytes();   // bytes held for reuse on the current device
+// Marks the calling thread as capturing a graph: allocations then never
+// synchronize the device.
+struct PoolCaptureScope {
+  PoolCaptureScope();
+  ~PoolCaptureScope();
+  PoolCaptureScope(const PoolCaptureScope&) = delete;
+  PoolCaptureScope& operator=(const PoolCaptureScope&) = delete;
+};
+
+// Owning device allocation (from device_alloc, released on destruction).
 template <class T>
 class DevBuf {
  public:
@@ -63,13 +79,13 @@ class DevBuf {
     if (n > cap_) {
       release();
       size_t bytes = (n == 0 ? 1 : n) * sizeof(T);
-      PAM_HIP(hipMalloc(reinterpret_cast<void**>(&p_), bytes));
+      p_ = static_cast<T*>(device_alloc(bytes));
       cap_ = n;
     }
     n_ = n;
   }
   void release() {
-    if (p_) (void)hipFree(p_);
+    if (p_) device_release(p_);
     p_ = nullptr;
     n_ = cap_ = 0;
   }

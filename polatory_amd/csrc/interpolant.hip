@@ -3,7 +3,9 @@
 
 #include "incremental_fitter.hpp"
 
+#include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <limits>
 #include <unordered_map>
@@ -97,8 +99,19 @@ void Interpolant::fit(const std::vector<double>& points, const std::vector<doubl
   fitted_ = false;
   ev_.reset();
   gev_.reset();
-  Solver solver(model_, points, opt, s_);
-  weights_ = solver.solve(values, tolerance, max_iter, initial ? &init : nullptr, &stats_);
+  auto t0 = std::chrono::steady_clock::now();
+  auto since = [](std::chrono::steady_clock::time_point t) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+  };
+  {
+    Solver solver(model_, points, opt, s_);
+    const double t_setup = since(t0);
+    weights_ = solver.solve(values, tolerance, max_iter, initial ? &init : nullptr, &stats_);
+    const double t_solve = since(t0);
+    if (opt.verbose) std::fprintf(stderr, "  fit: solver set-up %.3f s, solve to %.3f s\n", t_setup, t_solve);
+    t0 = std::chrono::steady_clock::now();
+  }
+  if (opt.verbose) std::fprintf(stderr, "  fit: solver teardown %.3f s\n", since(t0));
   centers_ = points;
   fitted_ = true;
   have_eval_bbox_ = false;

@@ -264,7 +264,10 @@ class GraphStep {
     try {
       PAM_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeRelaxed));
       capturing = true;
-      eager();
+      {
+        PoolCaptureScope no_sync;
+        eager();
+      }
       capturing = false;
       PAM_HIP(hipStreamEndCapture(s_, &g));
       PAM_HIP(hipGraphInstantiate(&exec_, g, nullptr, nullptr, 0));
