@@ -20,7 +20,7 @@ struct pam_operator {
     impl.reset();
     if (s) {
       (void)hipStreamSynchronize(s);
-      (void)hipStreamDestroy(s);
+      pam::stream_release(s);
     }
   }
 };
@@ -33,7 +33,7 @@ struct pam_ras {
     impl.reset();
     if (s) {
       (void)hipStreamSynchronize(s);
-      (void)hipStreamDestroy(s);
+      pam::stream_release(s);
     }
   }
 };
@@ -429,13 +429,12 @@ int pam_interpolant_residual_history(const pam_interpolant* h, double* out, int6
 int pam_fgmres_dense(const double* a, const double* m, int64_t n, const double* rhs, const double* x0,
                      int n_iter, double* x, double* resid_hist, int* iters) {
   return guard([&] {
-    hipStream_t s;
-    PAM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipStream_t s = pam::stream_acquire();
     struct StreamGuard {
       hipStream_t s;
       ~StreamGuard() {
         (void)hipStreamSynchronize(s);
-        (void)hipStreamDestroy(s);
+        pam::stream_release(s);
       }
     } sg{s};
     DenseOperator op(a, n, s);
@@ -466,7 +465,7 @@ int pam_ras_create(const pam_model* model, const double* points, int64_t n, pam_
   return guard([&] {
     if (!model || !out) pam::throw_invalid("null argument");
     auto h = std::make_unique<pam_ras>();
-    PAM_HIP(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
+    h->s = pam::stream_acquire();
     h->model = model->m;
     std::vector<double> p(points, points + 3 * n);
     h->impl = std::make_unique<pam::RasPreconditioner>(h->model, p, h->s, pam::RasOptions{});
@@ -479,7 +478,7 @@ int pam_ras_create_with_options(const pam_model* model, const double* points, in
   return guard([&] {
     if (!model || !out || !opt) pam::throw_invalid("null argument");
     auto h = std::make_unique<pam_ras>();
-    PAM_HIP(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
+    h->s = pam::stream_acquire();
     h->model = model->m;
     pam::FitOptions fo = to_opts(opt);
     pam::resolve_fit_options(h->model, fo);
@@ -548,7 +547,7 @@ int pam_operator_create(const pam_model* model, const double* points, int64_t n,
     if (n < 0) pam::throw_invalid("negative point count");
     if (n > 0 && !points) pam::throw_invalid("null argument");
     auto h = std::make_unique<pam_operator>();
-    PAM_HIP(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
+    h->s = pam::stream_acquire();
     h->model = model->m;
     h->pts.upload(points, static_cast<size_t>(3 * n), h->s);
     pam::FmmConfig c = cfg ? to_cfg(*cfg) : default_fit_options().op_cfg;

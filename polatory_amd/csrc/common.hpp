@@ -47,6 +47,9 @@ void* device_alloc(size_t bytes);
 void device_release(void* p);
 void device_pool_trim();             // frees every cached block of the current device
 size_t device_pool_cached_bytes();   // bytes held for reuse on the current device
+// Non-blocking streams of the current device, reused across owners.
+hipStream_t stream_acquire();
+void stream_release(hipStream_t s);
 // Marks the calling thread as capturing a graph: allocations then never
 // synchronize the device.
 struct PoolCaptureScope {

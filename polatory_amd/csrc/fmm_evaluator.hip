@@ -179,10 +179,10 @@ FmmEvaluator::FmmEvaluator(const RbfDesc& rbf, const double* bbox_min, const dou
     stream_ = stream;
     own_stream_ = false;
   } else {
-    PAM_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    stream_ = stream_acquire();
   }
   for (auto& e : ev_) PAM_HIP(hipEventCreate(&e));
-  PAM_HIP(hipStreamCreateWithFlags(&p2p_stream_, hipStreamNonBlocking));
+  p2p_stream_ = stream_acquire();
   PAM_HIP(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming));
   PAM_HIP(hipEventCreateWithFlags(&join_ev_, hipEventDisableTiming));
   bbox_out_.resize(12);
@@ -195,8 +195,8 @@ FmmEvaluator::~FmmEvaluator() {
     if (e) (void)hipEventDestroy(e);
   if (fork_ev_) (void)hipEventDestroy(fork_ev_);
   if (join_ev_) (void)hipEventDestroy(join_ev_);
-  if (p2p_stream_) (void)hipStreamDestroy(p2p_stream_);
-  if (stream_ && own_stream_) (void)hipStreamDestroy(stream_);
+  if (p2p_stream_) stream_release(p2p_stream_);
+  if (stream_ && own_stream_) stream_release(stream_);
 }
 
 void FmmEvaluator::set_accuracy(double accuracy) {

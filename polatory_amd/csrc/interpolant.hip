@@ -47,7 +47,7 @@ bool same_model(const Model& a, const Model& b) {
 
 Interpolant::Interpolant(const Model& model) : model_(model) {
   model_.validate();
-  PAM_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+  s_ = stream_acquire();
 }
 
 Interpolant::~Interpolant() {
@@ -55,7 +55,7 @@ Interpolant::~Interpolant() {
   gev_.reset();
   if (s_) {
     (void)hipStreamSynchronize(s_);
-    (void)hipStreamDestroy(s_);
+    stream_release(s_);
   }
 }
 

@@ -1454,15 +1454,14 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
     PAM_HIP(hipGetDevice(&dev));
     ap_done = std::async(std::launch::async, [this, dev] {
       PAM_HIP(hipSetDevice(dev));
-      hipStream_t as = nullptr;
-      PAM_HIP(hipStreamCreateWithFlags(&as, hipStreamNonBlocking));
+      hipStream_t as = stream_acquire();
       try {
         compute_ap(as);
       } catch (...) {
-        hipStreamDestroy(as);
+        stream_release(as);
         throw;
       }
-      PAM_HIP(hipStreamDestroy(as));
+      stream_release(as);
     });
   }
   // one positive-definiteness flag per level (0 = the coarse grid)
@@ -1547,7 +1546,7 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
     refactored = true;
   }
   if (refactored) setup_tmp_.clear();
-  for (hipStream_t ls : setup_streams_) PAM_HIP(hipStreamDestroy(ls));
+  for (hipStream_t ls : setup_streams_) stream_release(ls);
   setup_streams_.clear();
   tick("factors done");
   for (int32_t f : flags)
@@ -1575,8 +1574,7 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
 }
 
 hipStream_t RasPreconditioner::fork_stream() {
-  hipStream_t ls = nullptr;
-  PAM_HIP(hipStreamCreateWithFlags(&ls, hipStreamNonBlocking));
+  hipStream_t ls = stream_acquire();
   setup_streams_.push_back(ls);
   hipEvent_t ev = nullptr;
   PAM_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));

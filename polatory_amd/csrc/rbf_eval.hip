@@ -179,11 +179,11 @@ namespace {
 
 struct Stream {
   hipStream_t s = nullptr;
-  Stream() { PAM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+  Stream() { s = stream_acquire(); }
   ~Stream() {
     if (s) {
       (void)hipStreamSynchronize(s);
-      (void)hipStreamDestroy(s);
+      stream_release(s);
     }
   }
 };

@@ -180,15 +180,15 @@ Solver::Solver(const Model& model, const std::vector<double>& points, const FitO
     PAM_HIP(hipGetDevice(&dev));
     prebuild_ = std::async(std::launch::async, [this, dev] {
       PAM_HIP(hipSetDevice(dev));
-      hipStream_t ps = nullptr;
-      PAM_HIP(hipStreamCreateWithFlags(&ps, hipStreamNonBlocking));
+      hipStream_t ps = stream_acquire();
       try {
         res_->prebuild(ps);
+        PAM_HIP(hipStreamSynchronize(ps));  // the operator is complete before solve() uses it
       } catch (...) {
-        hipStreamDestroy(ps);
+        stream_release(ps);
         throw;
       }
-      PAM_HIP(hipStreamDestroy(ps));
+      stream_release(ps);
     });
   }
   if (opt_.use_preconditioner) pc_ = std::make_unique<RasPreconditioner>(model_, points, s_, fit_ras_options(model_, opt_, mu_));
