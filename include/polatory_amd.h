@@ -421,6 +421,16 @@ int pam_ras_synchronize(pam_ras* h);
 typedef struct pam_operator pam_operator;
 int pam_operator_create(const pam_model* model, const double* points, int64_t n, const pam_fmm_config* cfg,
                         pam_operator** out);
+/* Operator(model, points, grad_points) — operator.hpp:30-36, 56-82: the
+ * operator of a fit with gradient data, on vectors of mu + 3 sigma + l entries
+ * [w; g (x y z per gradient point); c]:
+ *   [A_vv w + nugget w + A_vg g + P_v c; A_gv w + A_gg g + P_g c; P_v^T w + P_g^T g],
+ * A_vv = phi(p_i - p_j), A_vg = -grad phi(p_i - q_k)^T, A_gv = grad phi(q_k - p_j),
+ * A_gg = -Hessian phi(q_k - q_m) (mat_a.hpp:11-61); P_g holds the monomial
+ * gradients. cfg == NULL: FP64 FMM of order 10. */
+int pam_operator_create_hermite(const pam_model* model, const double* points, int64_t mu,
+                                const double* grad_points, int64_t sigma, const pam_fmm_config* cfg,
+                                pam_operator** out);
 int pam_operator_destroy(pam_operator* h);
 /* LinearOperator::size() — n + l */
 int pam_operator_size(const pam_operator* h, int64_t* size);

@@ -62,6 +62,9 @@ def lib():
         L.orc_direct_rbf_sum.argtypes = [P, P, i64, P, P, i64, P]
         L.orc_direct_operator.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, P, i64,
                                           P, P]
+        L.orc_monomial_gradients.argtypes = [ctypes.c_int, P, i64, P]
+        L.orc_hermite_operator.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, P, i64, P, i64,
+                                           P, P]
         L.orc_make_box.argtypes = [P, P, P, P, P]
         L.orc_morton_codes.argtypes = [P, i64, P, ctypes.c_double, ctypes.c_int, P]
         L.orc_stable_sort_codes.argtypes = [P, i64, P]
@@ -182,6 +185,27 @@ def direct_operator(rbfs, poly_degree, nugget, pts, x):
     y = np.empty_like(x)
     lib().orc_direct_operator(ctypes.cast(arr, ctypes.c_void_p), n, poly_degree, float(nugget),
                               _p(pts), pts.shape[0], _p(x), _p(y))
+    return y
+
+
+def monomial_gradients(degree, pts):
+    """(3 n) x l: row 3 i + c holds the derivatives of the monomials along axis c at point i."""
+    pts = _f64(pts, (-1, 3))
+    l = lib().orc_poly_basis_size(degree)
+    out = np.empty((3 * pts.shape[0], max(l, 1)))
+    lib().orc_monomial_gradients(degree, _p(pts), pts.shape[0], _p(out))
+    return out[:, :l]
+
+
+def hermite_operator(rbfs, poly_degree, nugget, pts, gpts, x):
+    """Operator of a fit with gradient data: x = [w (mu); g (3 sigma); c (l)]."""
+    arr, n = _rbf_array(rbfs)
+    pts = _f64(pts, (-1, 3))
+    gpts = _f64(gpts, (-1, 3))
+    x = _f64(x)
+    y = np.empty_like(x)
+    lib().orc_hermite_operator(ctypes.cast(arr, ctypes.c_void_p), n, poly_degree, float(nugget),
+                               _p(pts), pts.shape[0], _p(gpts), gpts.shape[0], _p(x), _p(y))
     return y
 
 

@@ -196,6 +196,96 @@ void orc_direct_operator(const orc_rbf* rbfs, int n_rbfs, int poly_degree, doubl
   free(tmp);
 }
 
+/* Gradients of the monomials: out[(3 i + c) * l + k] = d m_k / d x_c at point i
+ * (MonomialBasis::evaluate with gradient points, monomial_basis.hpp:198-247). */
+void orc_monomial_gradients(int degree, const double* pts, int64_t n, double* out) {
+  int64_t l = orc_poly_basis_size(degree);
+  for (int64_t i = 0; i < n; i++) {
+    double x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
+    double gx[10] = {0.0, 1.0, 0.0, 0.0, 2.0 * x, y, z, 0.0, 0.0, 0.0};
+    double gy[10] = {0.0, 0.0, 1.0, 0.0, 0.0, x, 0.0, 2.0 * y, z, 0.0};
+    double gz[10] = {0.0, 0.0, 0.0, 1.0, 0.0, 0.0, x, 0.0, y, 2.0 * z};
+    for (int64_t k = 0; k < l; k++) {
+      out[(3 * i) * l + k] = gx[k];
+      out[(3 * i + 1) * l + k] = gy[k];
+      out[(3 * i + 2) * l + k] = gz[k];
+    }
+  }
+}
+
+/* The operator of a fit with gradient data (interpolation/operator.hpp:56-82,
+ * with the blocks of preconditioner/mat_a.hpp:11-61), by direct summation:
+ * x = [w (mu); g (3 sigma, x y z per gradient point); c (l)],
+ *   y_v[i] = nugget w_i + sum_j w_j phi(p_i - p_j) - sum_k g_k . grad phi(p_i - q_k) + P_v c,
+ *   y_g[k] = sum_j w_j grad phi(q_k - p_j) - sum_m H(q_k - q_m) g_m + P_g c,
+ *   y_p    = P_v^T w + P_g^T g,
+ * summed over the rbfs; P_v holds the monomials at the points p, P_g their
+ * gradients at the gradient points q. */
+void orc_hermite_operator(const orc_rbf* rbfs, int n_rbfs, int poly_degree, double nugget,
+                          const double* pts, int64_t mu, const double* gpts, int64_t sigma,
+                          const double* x, double* y) {
+  int64_t l = orc_poly_basis_size(poly_degree);
+  int64_t n = mu + 3 * sigma;
+  const double* g = x + mu;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < mu; i++) {
+    double acc = nugget * x[i], gr[3];
+    for (int r = 0; r < n_rbfs; r++) {
+      for (int64_t j = 0; j < mu; j++)
+        acc += x[j] * orc_rbf_evaluate(&rbfs[r], pts[3 * i] - pts[3 * j], pts[3 * i + 1] - pts[3 * j + 1],
+                                       pts[3 * i + 2] - pts[3 * j + 2]);
+      for (int64_t k = 0; k < sigma; k++) {
+        orc_rbf_gradient(&rbfs[r], pts[3 * i] - gpts[3 * k], pts[3 * i + 1] - gpts[3 * k + 1],
+                         pts[3 * i + 2] - gpts[3 * k + 2], gr);
+        acc -= (g[3 * k] * gr[0] + g[3 * k + 1] * gr[1]) + g[3 * k + 2] * gr[2];
+      }
+    }
+    y[i] = acc;
+  }
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < sigma; k++) {
+    double acc[3] = {0.0, 0.0, 0.0}, gr[3], h[9];
+    for (int r = 0; r < n_rbfs; r++) {
+      for (int64_t j = 0; j < mu; j++) {
+        orc_rbf_gradient(&rbfs[r], gpts[3 * k] - pts[3 * j], gpts[3 * k + 1] - pts[3 * j + 1],
+                         gpts[3 * k + 2] - pts[3 * j + 2], gr);
+        for (int c = 0; c < 3; c++) acc[c] += x[j] * gr[c];
+      }
+      for (int64_t m = 0; m < sigma; m++) {
+        orc_rbf_hessian(&rbfs[r], gpts[3 * k] - gpts[3 * m], gpts[3 * k + 1] - gpts[3 * m + 1],
+                        gpts[3 * k + 2] - gpts[3 * m + 2], h);
+        for (int c = 0; c < 3; c++)
+          acc[c] -= (h[3 * c] * g[3 * m] + h[3 * c + 1] * g[3 * m + 1]) + h[3 * c + 2] * g[3 * m + 2];
+      }
+    }
+    for (int c = 0; c < 3; c++) y[mu + 3 * k + c] = acc[c];
+  }
+  if (l > 0) {
+    double m[10], mg[30];
+    for (int64_t a = 0; a < l; a++) y[n + a] = 0.0;
+    for (int64_t i = 0; i < mu; i++) {
+      orc_monomials(poly_degree, pts + 3 * i, 1, m);
+      double acc = 0.0;
+      for (int64_t a = 0; a < l; a++) {
+        acc += m[a] * x[n + a];
+        y[n + a] += m[a] * x[i];
+      }
+      y[i] += acc;
+    }
+    for (int64_t k = 0; k < sigma; k++) {
+      orc_monomial_gradients(poly_degree, gpts + 3 * k, 1, mg);
+      for (int c = 0; c < 3; c++) {
+        double acc = 0.0;
+        for (int64_t a = 0; a < l; a++) {
+          acc += mg[c * l + a] * x[n + a];
+          y[n + a] += mg[c * l + a] * g[3 * k + c];
+        }
+        y[mu + 3 * k + c] += acc;
+      }
+    }
+  }
+}
+
 /* ---- Box and octree (src/fmm/utility.hpp:19-33 for the box) */
 
 void orc_make_box(const double* bmin, const double* bmax, const double* aniso, double* center,

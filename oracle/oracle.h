@@ -55,6 +55,13 @@ void orc_direct_rbf_sum(const orc_rbf* rbf, const double* src, int64_t n_src, co
 
 /* DirectOperator::operator() (interpolation/direct_operator.hpp:42-76):
  * y = [(A + nugget I) w + P c ; P^T w] for x = [w ; c]. */
+/* Gradients of the monomials: out[(3 i + c) * l + k] = d m_k / d x_c at point i. */
+void orc_monomial_gradients(int degree, const double* pts, int64_t n, double* out);
+/* Operator of a fit with gradient data: x, y have mu + 3 sigma + l entries
+ * ([values; gradients x y z per gradient point; polynomial]). */
+void orc_hermite_operator(const orc_rbf* rbfs, int n_rbfs, int poly_degree, double nugget,
+                          const double* pts, int64_t mu, const double* gpts, int64_t sigma,
+                          const double* x, double* y);
 void orc_direct_operator(const orc_rbf* rbfs, int n_rbfs, int poly_degree, double nugget,
                          const double* pts, int64_t n, const double* x, double* y);
 

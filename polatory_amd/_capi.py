@@ -186,6 +186,7 @@ _SIGS = {
     "pam_ras_size": [_P, ctypes.POINTER(_I64)],
     "pam_ras_synchronize": [_P],
     "pam_operator_create": [_P, _P, _I64, ctypes.POINTER(PamFmmConfig), ctypes.POINTER(_P)],
+    "pam_operator_create_hermite": [_P, _P, _I64, _P, _I64, _P, ctypes.POINTER(_P)],
     "pam_operator_destroy": [_P],
     "pam_operator_size": [_P, ctypes.POINTER(_I64)],
     "pam_operator_apply_device": [_P, _P, _P],

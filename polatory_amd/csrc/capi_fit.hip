@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "capi_handles.hpp"
+#include "hermite.hpp"
 #include "incremental_fitter.hpp"
 #include "model_io.hpp"
 #include "ras.hpp"
@@ -13,8 +14,8 @@
 struct pam_operator {
   hipStream_t s = nullptr;
   pam::Model model;
-  pam::DevBuf<double> pts;
-  std::unique_ptr<pam::Operator> impl;
+  pam::DevBuf<double> pts, gpts;
+  std::unique_ptr<pam::LinearOperator> impl;
   pam::DevBuf<double> x, y;  // host-pointer applies
   ~pam_operator() {
     impl.reset();
@@ -552,6 +553,31 @@ int pam_operator_create(const pam_model* model, const double* points, int64_t n,
     h->pts.upload(points, static_cast<size_t>(3 * n), h->s);
     pam::FmmConfig c = cfg ? to_cfg(*cfg) : default_fit_options().op_cfg;
     h->impl = std::make_unique<pam::Operator>(h->model, h->pts.get(), n, c, h->s);
+    PAM_HIP(hipStreamSynchronize(h->s));
+    *out = h.release();
+  });
+}
+
+int pam_operator_create_hermite(const pam_model* model, const double* points, int64_t mu,
+                                const double* grad_points, int64_t sigma, const pam_fmm_config* cfg,
+                                pam_operator** out) {
+  return guard([&] {
+    if (!model || !out) pam::throw_invalid("null argument");
+    if (mu < 0 || sigma < 0) pam::throw_invalid("negative point count");
+    if ((mu > 0 && !points) || (sigma > 0 && !grad_points)) pam::throw_invalid("null argument");
+    auto h = std::make_unique<pam_operator>();
+    h->s = pam::stream_acquire();
+    h->model = model->m;
+    h->pts.upload(points, static_cast<size_t>(3 * mu), h->s);
+    h->gpts.upload(grad_points, static_cast<size_t>(3 * sigma), h->s);
+    pam::FmmConfig c;
+    if (cfg) {
+      c = to_cfg(*cfg);
+    } else {
+      c.precision = pam::kFp64;
+      c.order = 10;
+    }
+    h->impl = std::make_unique<pam::HermiteOperator>(h->model, h->pts.get(), mu, h->gpts.get(), sigma, c, h->s);
     PAM_HIP(hipStreamSynchronize(h->s));
     *out = h.release();
   });

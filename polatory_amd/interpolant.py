@@ -367,12 +367,21 @@ class Operator:
     """interpolation::Operator (operator.hpp:38-111) as a LinearOperator:
     y = [(A + nugget I) w + P c ; P^T w] for x = [w ; c] at the points."""
 
-    def __init__(self, model: Model, points, config: FmmConfig = None):
+    def __init__(self, model: Model, points, config: FmmConfig = None, grad_points=None):
+        """With grad_points (operator.hpp:30-36), x = [w ; g ; c] with g the 3-vector
+        weights of the gradient points (x y z per point) and the output rows
+        [values ; gradients ; P^T [w ; g]]; the default configuration is then FP64
+        FMM of order 10."""
         self.model = model
         p = _points(points)
         h = ctypes.c_void_p()
         c = None if config is None else ctypes.byref(config.to_c())
-        _capi.call("pam_operator_create", model._h, _ptr(p), p.shape[0], c, ctypes.byref(h))
+        if grad_points is None:
+            _capi.call("pam_operator_create", model._h, _ptr(p), p.shape[0], c, ctypes.byref(h))
+        else:
+            g = _points(grad_points)
+            _capi.call("pam_operator_create_hermite", model._h, _ptr(p), p.shape[0], _ptr(g), g.shape[0], c,
+                       ctypes.byref(h))
         self._h = h
 
     def __del__(self):

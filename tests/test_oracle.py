@@ -329,3 +329,84 @@ def test_cpu_fit_two_levels_reaches_tolerance(orc):
         assert len(idx) <= 1024 + 4
         np.add.at(inner, idx[inn], 1)
     assert np.all(inner == 1)  # every point is the inner point of exactly one domain
+
+
+# ---- The operator of fits with gradient data (interpolation/operator.hpp:56-82) ----
+
+def _hermite_case(orc, seed=41, mu=12, sigma=9, degree=1):
+    rng = np.random.default_rng(seed)
+    aniso = np.array([[1.1, 0.2, 0.0], [0.0, 0.9, 0.1], [0.1, 0.0, 1.2]])
+    rbfs = [_rbf(orc, "th3", aniso), _rbf(orc, "bh3c")]
+    pts = rng.uniform(-1, 1, (mu, 3))
+    gpts = rng.uniform(-1, 1, (sigma, 3))
+    return rng, rbfs, pts, gpts, degree
+
+
+def _hermite_matrix(orc, rbfs, degree, nugget, pts, gpts):
+    n = len(pts) + 3 * len(gpts) + orc.poly_basis_size(degree)
+    eye = np.eye(n)
+    return np.stack([orc.hermite_operator(rbfs, degree, nugget, pts, gpts, eye[:, j]) for j in range(n)], axis=1)
+
+
+def test_hermite_operator_without_gradient_points_is_the_value_operator(orc):
+    rng, rbfs, pts, _, deg = _hermite_case(orc)
+    x = rng.normal(size=len(pts) + orc.poly_basis_size(deg))
+    y = orc.hermite_operator(rbfs, deg, 0.01, pts, np.zeros((0, 3)), x)
+    ref = orc.direct_operator(rbfs, deg, 0.01, pts, x)
+    np.testing.assert_allclose(y, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+def test_hermite_operator_is_symmetric(orc):
+    # The Gram matrix of the functionals {delta_p, d/dq_c delta_q} is symmetric;
+    # a sign error in either mixed block, or in P_g against P_g^T, breaks it.
+    _, rbfs, pts, gpts, deg = _hermite_case(orc)
+    a = _hermite_matrix(orc, rbfs, deg, 0.01, pts, gpts)
+    assert np.abs(a - a.T).max() <= 1e-12 * np.abs(a).max()
+
+
+def test_hermite_operator_blocks_are_derivatives_of_the_value_field(orc):
+    # Each block against central differences of the value kernel:
+    #   gradient rows from value weights = gradient of sum_j w_j phi(q - p_j) at q_k;
+    #   value rows from gradient weights = d/de sum_k phi(p_i - (q_k + e g_k)) at e = 0;
+    #   gradient rows from gradient weights = gradient at q_k of the latter field;
+    #   polynomial gradient rows = differences of the monomials.
+    rng, rbfs, pts, gpts, deg = _hermite_case(orc, mu=10, sigma=7)
+    mu, sigma, l = len(pts), len(gpts), orc.poly_basis_size(deg)
+    n = mu + 3 * sigma
+    h = 1e-5
+    w = rng.normal(size=mu)
+    g = rng.normal(size=(sigma, 3))
+    c = rng.normal(size=l)
+
+    def field(src, wts, trg):  # sum over rbfs of sum_j wts_j phi(trg - src_j)
+        return sum(orc.direct_rbf_sum(r, src, wts, trg) for r in rbfs)
+
+    y = orc.hermite_operator(rbfs, deg, 0.0, pts, gpts, np.concatenate([w, np.zeros(3 * sigma + l)]))
+    fd = np.stack([(field(pts, w, gpts + h * e) - field(pts, w, gpts - h * e)) / (2 * h) for e in np.eye(3)], axis=1)
+    np.testing.assert_allclose(y[mu:n].reshape(-1, 3), fd, rtol=1e-6, atol=1e-6 * np.abs(fd).max())
+
+    y = orc.hermite_operator(rbfs, deg, 0.0, pts, gpts, np.concatenate([np.zeros(mu), g.ravel(), np.zeros(l)]))
+    fd = (field(gpts + h * g, np.ones(sigma), pts) - field(gpts - h * g, np.ones(sigma), pts)) / (2 * h)
+    np.testing.assert_allclose(y[:mu], fd, rtol=1e-6, atol=1e-6 * np.abs(fd).max())
+
+    def vfield(trg):  # value rows from the gradient weights, at arbitrary points
+        x = np.concatenate([np.zeros(len(trg)), g.ravel(), np.zeros(l)])
+        return orc.hermite_operator(rbfs, deg, 0.0, trg, gpts, x)[:len(trg)]
+
+    fd = np.stack([(vfield(gpts + h * e) - vfield(gpts - h * e)) / (2 * h) for e in np.eye(3)], axis=1)
+    np.testing.assert_allclose(y[mu:n].reshape(-1, 3), fd, rtol=1e-4, atol=1e-4 * np.abs(fd).max())
+
+    y = orc.hermite_operator(rbfs, deg, 0.0, pts, gpts, np.concatenate([np.zeros(n), c]))
+    mono = lambda q: orc.monomials(deg, q) @ c
+    fd = np.stack([(mono(gpts + h * e) - mono(gpts - h * e)) / (2 * h) for e in np.eye(3)], axis=1)
+    np.testing.assert_allclose(y[mu:n].reshape(-1, 3), fd, rtol=1e-8, atol=1e-8)
+    np.testing.assert_allclose(y[:mu], mono(pts), rtol=1e-12, atol=1e-12)
+
+
+def test_hermite_operator_nugget_only_on_value_rows(orc):
+    rng, rbfs, pts, gpts, deg = _hermite_case(orc)
+    n = len(pts) + 3 * len(gpts) + orc.poly_basis_size(deg)
+    x = rng.normal(size=n)
+    d = orc.hermite_operator(rbfs, deg, 0.25, pts, gpts, x) - orc.hermite_operator(rbfs, deg, 0.0, pts, gpts, x)
+    np.testing.assert_allclose(d[:len(pts)], 0.25 * x[:len(pts)], rtol=1e-12, atol=1e-12)
+    assert np.all(d[len(pts):] == 0.0)
