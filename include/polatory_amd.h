@@ -298,6 +298,30 @@ int pam_interpolant_destroy(pam_interpolant* h);
 int pam_interpolant_fit(pam_interpolant* h, const double* points, int64_t n, const double* values,
                         double tolerance, int max_iter, double accuracy, const pam_fit_options* opt,
                         const pam_interpolant* initial);
+/* fit(points, grad_points, values, tolerance, grad_tolerance, max_iter, accuracy,
+ * grad_accuracy, initial) — interpolant.hpp:83-113 with gradient data
+ * (Hermite-Birkhoff): values has mu + 3 sigma entries, the values at the points
+ * then the gradients (x y z) at the grad points; tolerance bounds the value rows
+ * and grad_tolerance the gradient rows (absolute L-inf, recomputed in FP64).
+ * The weights are then [w (mu); g (3 sigma); c (l)]. The operator, the RAS
+ * level updates and the residual are FP64 FMM (order 10, or opt's FP64
+ * configurations) or direct sums; the RAS factors are FP64. accuracy and
+ * grad_accuracy must be positive (see pam_interpolant_fit). opt and initial
+ * may be NULL. mu may be 1 with degree 1 and sigma >= 1 (the reference's
+ * special case, ras_preconditioner.hpp:81-87). */
+int pam_interpolant_fit_hermite(pam_interpolant* h, const double* points, int64_t mu, const double* grad_points,
+                                int64_t sigma, const double* values, double tolerance, double grad_tolerance,
+                                int max_iter, double accuracy, double grad_accuracy, const pam_fit_options* opt,
+                                const pam_interpolant* initial);
+/* evaluate(points, grad_points) — interpolant.hpp:58-67 for any interpolant:
+ * out receives n values at points, then 3 ng gradient entries (x y z) at
+ * grad_points. With gradient centres the evaluation is FP64 FMM of order 10
+ * (or direct sums). */
+int pam_interpolant_evaluate_hermite(pam_interpolant* h, const double* points, int64_t n,
+                                     const double* grad_points, int64_t ng, double* out);
+/* grad_centers() — interpolant.hpp:48 */
+int pam_interpolant_num_grad_centers(const pam_interpolant* h, int64_t* n);
+int pam_interpolant_grad_centers(const pam_interpolant* h, double* out);
 /* fit_incrementally(points, values, tolerance, max_iter = 100, accuracy = inf)
  * — interpolant.hpp:115-147, interpolation/incremental_fitter.hpp:25-183: fits on a
  * growing subset of the points; centers() then holds the points chosen, in the
@@ -375,6 +399,7 @@ typedef struct {
   int escalations;          /* precision raises after FGMRES stagnated (FP64 level updates, then operator) */
   int converged;            /* 0 when keep_unconverged stopped the fit at max_iter */
   int rounds;               /* incremental and inequality fits: rounds of solve + residuals */
+  double grad_residual;     /* fits with gradient data: L-inf residual of the gradient rows */
 } pam_fit_stats;
 int pam_interpolant_fit_stats(const pam_interpolant* h, pam_fit_stats* out);
 /* Residual after each FGMRES iteration (out may be NULL to query the length). */
@@ -396,6 +421,12 @@ int pam_ras_create(const pam_model* model, const double* points, int64_t n, pam_
  * and coarse-grid choices); opt must not be NULL. */
 int pam_ras_create_with_options(const pam_model* model, const double* points, int64_t n,
                                 const pam_fit_options* opt, pam_ras** out);
+/* RasPreconditioner(model, points, grad_points) — ras_preconditioner.hpp:56-189
+ * with gradient points: vectors of mu + 3 sigma + l entries. The preconditioner
+ * of pam_interpolant_fit_hermite (FP64 factors and level updates); opt may be
+ * NULL. */
+int pam_ras_create_hermite(const pam_model* model, const double* points, int64_t mu, const double* grad_points,
+                           int64_t sigma, const pam_fit_options* opt, pam_ras** out);
 int pam_ras_destroy(pam_ras* h);
 int pam_ras_num_levels(const pam_ras* h, int* n_levels);
 /* point indices of a level (level n_levels-1 = all points; 0 = coarse grid) */

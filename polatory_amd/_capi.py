@@ -52,7 +52,8 @@ class PamFitStats(ctypes.Structure):
                 ("residual", ctypes.c_double), ("seconds", ctypes.c_double),
                 ("setup_seconds", ctypes.c_double), ("precond_setup_seconds", ctypes.c_double),
                 ("factor_bytes", ctypes.c_int64), ("graph_replays", ctypes.c_int),
-                ("escalations", ctypes.c_int), ("converged", ctypes.c_int), ("rounds", ctypes.c_int)]
+                ("escalations", ctypes.c_int), ("converged", ctypes.c_int), ("rounds", ctypes.c_int),
+                ("grad_residual", ctypes.c_double)]
 
 
 class PamServerOptions(ctypes.Structure):
@@ -186,6 +187,11 @@ _SIGS = {
     "pam_ras_size": [_P, ctypes.POINTER(_I64)],
     "pam_ras_synchronize": [_P],
     "pam_operator_create": [_P, _P, _I64, ctypes.POINTER(PamFmmConfig), ctypes.POINTER(_P)],
+    "pam_interpolant_fit_hermite": [_P, _P, _I64, _P, _I64, _P, _D, _D, _INT, _D, _D, _P, _P],
+    "pam_interpolant_evaluate_hermite": [_P, _P, _I64, _P, _I64, _P],
+    "pam_interpolant_num_grad_centers": [_P, ctypes.POINTER(_I64)],
+    "pam_interpolant_grad_centers": [_P, _P],
+    "pam_ras_create_hermite": [_P, _P, _I64, _P, _I64, _P, ctypes.POINTER(_P)],
     "pam_operator_create_hermite": [_P, _P, _I64, _P, _I64, _P, ctypes.POINTER(_P)],
     "pam_operator_destroy": [_P],
     "pam_operator_size": [_P, ctypes.POINTER(_I64)],

@@ -238,6 +238,41 @@ int pam_interpolant_fit(pam_interpolant* h, const double* points, int64_t n, con
   });
 }
 
+int pam_interpolant_fit_hermite(pam_interpolant* h, const double* points, int64_t mu, const double* grad_points,
+                                int64_t sigma, const double* values, double tolerance, double grad_tolerance,
+                                int max_iter, double accuracy, double grad_accuracy, const pam_fit_options* opt,
+                                const pam_interpolant* initial) {
+  return guard([&] {
+    if (mu < 0 || sigma < 0) pam::throw_invalid("negative point count");
+    if ((mu > 0 && !points) || (sigma > 0 && !grad_points) || (mu + sigma > 0 && !values))
+      pam::throw_invalid("null argument");
+    if (!(accuracy > 0.0) || !(grad_accuracy > 0.0)) pam::throw_invalid("accuracy must be positive");
+    std::vector<double> p(points, points + 3 * mu), g(grad_points, grad_points + 3 * sigma),
+        v(values, values + mu + 3 * sigma);
+    ip(h).fit_hermite(p, g, v, tolerance, grad_tolerance, max_iter, to_opts(opt), initial ? &ip(initial) : nullptr);
+  });
+}
+
+int pam_interpolant_evaluate_hermite(pam_interpolant* h, const double* points, int64_t n,
+                                     const double* grad_points, int64_t ng, double* out) {
+  return guard([&] {
+    if (n < 0 || ng < 0) pam::throw_invalid("negative point count");
+    if ((n > 0 && !points) || (ng > 0 && !grad_points) || (n + ng > 0 && !out)) pam::throw_invalid("null argument");
+    ip(h).evaluate_hermite(points, n, grad_points, ng, out, false);
+  });
+}
+
+int pam_interpolant_num_grad_centers(const pam_interpolant* h, int64_t* n) {
+  return guard([&] { *n = static_cast<int64_t>(ip(h).grad_centers().size() / 3); });
+}
+
+int pam_interpolant_grad_centers(const pam_interpolant* h, double* out) {
+  return guard([&] {
+    const auto& c = ip(h).grad_centers();
+    std::copy(c.begin(), c.end(), out);
+  });
+}
+
 int pam_interpolant_fit_incrementally(pam_interpolant* h, const double* points, int64_t n,
                                       const double* values, double tolerance, int max_iter,
                                       double accuracy, const pam_fit_options* opt) {
@@ -416,6 +451,7 @@ int pam_interpolant_fit_stats(const pam_interpolant* h, pam_fit_stats* out) {
     out->escalations = s.escalations;
     out->converged = s.converged ? 1 : 0;
     out->rounds = s.rounds;
+    out->grad_residual = s.grad_residual;
   });
 }
 
@@ -485,6 +521,31 @@ int pam_ras_create_with_options(const pam_model* model, const double* points, in
     pam::resolve_fit_options(h->model, fo);
     std::vector<double> p(points, points + 3 * n);
     h->impl = std::make_unique<pam::RasPreconditioner>(h->model, p, h->s, pam::fit_ras_options(h->model, fo, n));
+    *out = h.release();
+  });
+}
+
+int pam_ras_create_hermite(const pam_model* model, const double* points, int64_t mu, const double* grad_points,
+                           int64_t sigma, const pam_fit_options* opt, pam_ras** out) {
+  return guard([&] {
+    if (!model || !out) pam::throw_invalid("null argument");
+    if (mu < 0 || sigma < 0) pam::throw_invalid("negative point count");
+    if ((mu > 0 && !points) || (sigma > 0 && !grad_points)) pam::throw_invalid("null argument");
+    auto h = std::make_unique<pam_ras>();
+    h->s = pam::stream_acquire();
+    h->model = model->m;
+    pam::FitOptions fo = to_opts(opt);
+    pam::resolve_fit_options(h->model, fo);
+    pam::RasOptions ro = pam::fit_ras_options(h->model, fo, mu + 3 * sigma);
+    ro.fp32_factors = false;
+    if (!ro.use_level_cfg || ro.level_cfg.precision != pam::kFp64) {
+      ro.level_cfg = pam::FmmConfig{};
+      ro.level_cfg.precision = pam::kFp64;
+      ro.level_cfg.order = 10;
+      ro.use_level_cfg = true;
+    }
+    std::vector<double> p(points, points + 3 * mu), g(grad_points, grad_points + 3 * sigma);
+    h->impl = std::make_unique<pam::RasPreconditioner>(h->model, p, g, h->s, ro);
     *out = h.release();
   });
 }

@@ -62,6 +62,8 @@ int pam_server_add_model(pam_server* h, const pam_interpolant* fitted, const dou
     if (!model_id) pam::throw_invalid("null model id");
     const pam::Interpolant& ip = *fitted->impl;
     if (!ip.fitted()) pam::throw_runtime("the interpolant is not fitted");
+    if (!ip.grad_centers().empty())
+      throw pam::Error(pam::kNotImplemented, "serving interpolants with gradient centres is not implemented");
     pam::FmmConfig c;
     if (cfg) c = pam::to_cfg(*cfg);
     int64_t n = static_cast<int64_t>(ip.centers().size() / 3);

@@ -77,9 +77,115 @@ std::vector<double> Interpolant::build_initial_weights(const std::vector<double>
     auto it = ini.find(key_of(&points[3 * i]));
     if (it != ini.end()) w[i] = initial.weights_[it->second];
   }
-  for (int k = 0; k < l; k++) w[mu + k] = initial.weights_[ini_mu + k];
+  const int64_t ini_n = ini_mu + static_cast<int64_t>(initial.grad_centers_.size());  // + 3 sigma
+  for (int k = 0; k < l; k++) w[mu + k] = initial.weights_[ini_n + k];
   return w;
 }
+
+std::vector<double> Interpolant::build_initial_weights(const std::vector<double>& points,
+                                                       const std::vector<double>& grad_points,
+                                                       const Interpolant& initial) const {
+  // interpolant.hpp:232-273: weights of coincident centres and gradient
+  // centres, polynomial coefficients from the initial interpolant.
+  const int l = model_.l();
+  const int64_t mu = static_cast<int64_t>(points.size() / 3), sigma = static_cast<int64_t>(grad_points.size() / 3);
+  std::vector<double> w(static_cast<size_t>(mu + 3 * sigma + l), 0.0);
+  if (!same_model(model_, initial.model_)) return w;
+  const int64_t ini_mu = static_cast<int64_t>(initial.centers_.size() / 3);
+  const int64_t ini_sigma = static_cast<int64_t>(initial.grad_centers_.size() / 3);
+  std::unordered_map<PointKey, int64_t, PointHash> ini, gini;
+  for (int64_t i = 0; i < ini_mu; i++) ini.emplace(key_of(&initial.centers_[3 * i]), i);
+  for (int64_t i = 0; i < ini_sigma; i++) gini.emplace(key_of(&initial.grad_centers_[3 * i]), i);
+  for (int64_t i = 0; i < mu; i++) {
+    auto it = ini.find(key_of(&points[3 * i]));
+    if (it != ini.end()) w[i] = initial.weights_[it->second];
+  }
+  for (int64_t i = 0; i < sigma; i++) {
+    auto it = gini.find(key_of(&grad_points[3 * i]));
+    if (it != gini.end())
+      for (int c = 0; c < 3; c++) w[mu + 3 * i + c] = initial.weights_[ini_mu + 3 * it->second + c];
+  }
+  for (int k = 0; k < l; k++) w[mu + 3 * sigma + k] = initial.weights_[ini_mu + 3 * ini_sigma + k];
+  return w;
+}
+
+void Interpolant::set_bbox() {
+  for (int k = 0; k < 3; k++) {
+    bbox_lo_[k] = INFINITY;
+    bbox_hi_[k] = -INFINITY;
+  }
+  for (const std::vector<double>* v : {&centers_, &grad_centers_})
+    for (size_t i = 0; i + 2 < v->size(); i += 3)
+      for (int k = 0; k < 3; k++) {
+        bbox_lo_[k] = std::min(bbox_lo_[k], (*v)[i + k]);
+        bbox_hi_[k] = std::max(bbox_hi_[k], (*v)[i + k]);
+      }
+}
+
+void Interpolant::fit_hermite(const std::vector<double>& points, const std::vector<double>& grad_points,
+                              const std::vector<double>& values, double tolerance, double grad_tolerance,
+                              int max_iter, const FitOptions& opt, const Interpolant* initial) {
+  const int64_t mu = static_cast<int64_t>(points.size() / 3), sigma = static_cast<int64_t>(grad_points.size() / 3);
+  if (static_cast<int64_t>(values.size()) != mu + 3 * sigma)
+    throw_invalid("values.rows() must be equal to points.rows() + 3 * grad_points.rows()");
+  if (!(tolerance > 0.0) || !(grad_tolerance > 0.0)) throw_invalid("tolerance must be positive");
+  if (max_iter < 0) throw_invalid("max_iter must be nonnegative");
+  // The initial weights are built before clearing: `initial` may be this.
+  std::vector<double> init;
+  if (initial && initial->fitted_) init = build_initial_weights(points, grad_points, *initial);
+  fitted_ = false;
+  ev_.reset();
+  gev_.reset();
+  hev_.reset();
+  {
+    HermiteSolver solver(model_, points, grad_points, opt, s_);
+    weights_ = solver.solve(values, tolerance, grad_tolerance, max_iter, init.empty() ? nullptr : &init, &stats_);
+  }
+  centers_ = points;
+  grad_centers_ = grad_points;
+  fitted_ = true;
+  have_eval_bbox_ = false;
+  set_bbox();
+}
+
+void Interpolant::evaluate_hermite(const double* points, int64_t n, const double* grad_points, int64_t ng,
+                                   double* out, bool on_device) {
+  throw_if_not_fitted();
+  if (n < 0 || ng < 0) throw_invalid("negative point count");
+  if (!hev_) {
+    FmmConfig c;
+    c.precision = kFp64;
+    c.order = 10;
+    if (have_eval_cfg_) {
+      c = eval_cfg_;
+      if (c.precision != kFp64) c.precision = kFp64, c.order = std::max(c.order, 10);
+    }
+    hev_ = std::make_unique<HermiteEvaluator>(model_, c, s_);
+    d_centers_.upload(centers_.data(), centers_.size(), s_);
+    d_gcenters_.upload(grad_centers_.data(), grad_centers_.size(), s_);
+    d_weights_.upload(weights_.data(), weights_.size(), s_);
+    hev_->set_sources(d_centers_.get(), static_cast<int64_t>(centers_.size() / 3), d_gcenters_.get(),
+                      static_cast<int64_t>(grad_centers_.size() / 3));
+  }
+  const double *d_in = points, *d_gin = grad_points;
+  if (!on_device) {
+    d_pts_.upload(points, static_cast<size_t>(n) * 3, s_);
+    d_gpts_.upload(grad_points, static_cast<size_t>(ng) * 3, s_);
+    d_in = d_pts_.get();
+    d_gin = d_gpts_.get();
+  }
+  double* d_out = out;
+  if (!on_device) {
+    d_out_.resize(static_cast<size_t>(n + 3 * ng));
+    d_out = d_out_.get();
+  }
+  hev_->set_targets(d_in, n, d_gin, ng);
+  hev_->set_weights(d_weights_.get());
+  hev_->evaluate(d_out);
+  if (!on_device) PAM_HIP(hipMemcpyAsync(out, d_out, sizeof(double) * (n + 3 * ng), hipMemcpyDeviceToHost, s_));
+  PAM_HIP(hipStreamSynchronize(s_));
+}
+
 
 void Interpolant::fit(const std::vector<double>& points, const std::vector<double>& values,
                       double tolerance, int max_iter, double accuracy, const FitOptions& opt,
@@ -99,6 +205,8 @@ void Interpolant::fit(const std::vector<double>& points, const std::vector<doubl
   fitted_ = false;
   ev_.reset();
   gev_.reset();
+  hev_.reset();
+  grad_centers_.clear();
   auto t0 = std::chrono::steady_clock::now();
   auto since = [](std::chrono::steady_clock::time_point t) {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
@@ -170,6 +278,8 @@ void Interpolant::set_fitted(const std::vector<double>& centers, const std::vect
   have_eval_bbox_ = false;
   ev_.reset();
   gev_.reset();
+  hev_.reset();
+  grad_centers_.clear();
   for (int k = 0; k < 3; k++) {
     bbox_lo_[k] = INFINITY;
     bbox_hi_[k] = -INFINITY;
@@ -245,6 +355,10 @@ void Interpolant::evaluate_impl(const double* points, int64_t n, double* out, bo
 void Interpolant::evaluate(const double* points, int64_t n, double accuracy, double* out, bool on_device) {
   throw_if_not_fitted();
   if (!(accuracy > 0.0)) throw_invalid("accuracy must be positive");
+  if (!grad_centers_.empty()) {
+    evaluate_hermite(points, n, nullptr, 0, out, on_device);
+    return;
+  }
   if (accuracy != ev_acc_) {
     // A box installed by set_evaluation_bbox belongs to its accuracy; at
     // another accuracy the evaluator is rebuilt for the centres' box, as the
@@ -277,6 +391,10 @@ void Interpolant::evaluate_gradient(const double* points, int64_t n, double accu
   throw_if_not_fitted();
   if (!(accuracy > 0.0)) throw_invalid("accuracy must be positive");
   if (n < 0) throw_invalid("negative point count");
+  if (!grad_centers_.empty()) {
+    evaluate_hermite(nullptr, 0, points, n, out3, on_device);
+    return;
+  }
   if (accuracy != gev_acc_) {
     gev_.reset();
     gev_acc_ = accuracy;

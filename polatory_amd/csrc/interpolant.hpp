@@ -19,6 +19,18 @@ class Interpolant {
   // interpolant.hpp:77-113
   void fit(const std::vector<double>& points, const std::vector<double>& values, double tolerance,
            int max_iter, double accuracy, const FitOptions& opt, const Interpolant* initial);
+  // interpolant.hpp:77-113 with gradient data: values has mu + 3 sigma entries
+  // (values, then x y z per gradient point); the weights are [w; g; c].
+  void fit_hermite(const std::vector<double>& points, const std::vector<double>& grad_points,
+                   const std::vector<double>& values, double tolerance, double grad_tolerance, int max_iter,
+                   const FitOptions& opt, const Interpolant* initial);
+  // interpolant.hpp:58-67: values at points, then gradients (x y z) at
+  // grad_points, of an interpolant with value and gradient centres (FP64 FMM
+  // of order 10, or the evaluation configuration raised to FP64). Host or
+  // device pointers; out has n + 3 ng entries.
+  void evaluate_hermite(const double* points, int64_t n, const double* grad_points, int64_t ng, double* out,
+                        bool on_device);
+  const std::vector<double>& grad_centers() const { return grad_centers_; }
   // interpolant.hpp:54-67 (+ polynomial terms); pointers are host or device.
   void evaluate(const double* points, int64_t n, double accuracy, double* out, bool on_device);
   // Gradients at grad points (the grad_points half of interpolant.hpp:58-67):
@@ -57,12 +69,18 @@ class Interpolant {
  private:
   std::vector<double> build_initial_weights(const std::vector<double>& points,
                                             const Interpolant& initial) const;
+  std::vector<double> build_initial_weights(const std::vector<double>& points,
+                                            const std::vector<double>& grad_points,
+                                            const Interpolant& initial) const;
+  void set_bbox();
   void throw_if_not_fitted() const;
 
   Model model_;
   hipStream_t s_ = nullptr;
   bool fitted_ = false;
-  std::vector<double> centers_, weights_;
+  std::vector<double> centers_, weights_, grad_centers_;
+  std::unique_ptr<HermiteEvaluator> hev_;
+  DevBuf<double> d_gcenters_, d_gpts_;
   FitStats stats_;
   std::unique_ptr<Evaluator> ev_, gev_;
   double ev_acc_ = -1.0, gev_acc_ = -1.0;

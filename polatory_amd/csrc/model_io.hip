@@ -157,6 +157,8 @@ Model load_model(const std::string& filename) {
 }
 
 void save_interpolant(const std::string& filename, const Interpolant& ip) {
+  if (!ip.grad_centers().empty())
+    throw Error(kNotImplemented, "saving interpolants with gradient centres is not implemented");
   with_file(filename, std::ios::out | std::ios::trunc, [&](std::ostream& os) {
     write_model(os, ip.model());
     const bool fitted = ip.fitted();

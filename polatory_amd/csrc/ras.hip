@@ -16,6 +16,7 @@
 #include <queue>
 
 #include "device_api.hpp"
+#include "rbf_device.hpp"
 
 namespace pam {
 
@@ -75,6 +76,62 @@ __device__ __forceinline__ double kernel_entry(const RbfSet& r, const double* __
   return s;
 }
 
+// Entry of mat_a between two rows of a fit with gradient data
+// (preconditioner/mat_a.hpp:25-56), summed over the rbfs; ti, tj = -1 for a
+// value row, else the gradient component; d = position_i - position_j:
+//   value, value:       phi(d)
+//   value, gradient e:  -d phi / d x_e (d)
+//   gradient c, value:  +d phi / d x_c (d)
+//   gradient c, gradient e: -d^2 phi / d x_c d x_e (d)
+// With u = A d and the isotropic g = f u, H = f (I + t u u^T) (rbf_grad_factor),
+// grad phi = f A^T u and Hess phi = f (A^T A + t (A^T u)(A^T u)^T).
+__device__ double kernel_hb(const RbfSet& r, int ti, int tj, double dx, double dy, double dz) {
+  if (ti < 0 && tj < 0) return kernel_value(r, dx, dy, dz);
+  double s = 0.0;
+  for (int k = 0; k < r.n; k++) {
+    const double* a = r.a[k];
+    const double u[3] = {a[0] * dx + a[1] * dy + a[2] * dz, a[3] * dx + a[4] * dy + a[5] * dz,
+                         a[6] * dx + a[7] * dy + a[8] * dz};
+    double t = 0.0, f = 0.0;
+    if (r.kind[k] == kBh3) f = rbf_grad_factor<kBh3>(r.p0[k], r.p1[k], u, &t);
+    else if (r.kind[k] == kTh3) f = rbf_grad_factor<kTh3>(r.p0[k], r.p1[k], u, &t);
+    else f = rbf_grad_factor<kExp>(r.p0[k], r.p1[k], u, &t);
+    double v[3];
+    for (int c = 0; c < 3; c++) v[c] = a[c] * u[0] + a[3 + c] * u[1] + a[6 + c] * u[2];
+    if (ti < 0) {
+      s -= f * v[tj];
+    } else if (tj < 0) {
+      s += f * v[ti];
+    } else {
+      const double g = a[ti] * a[tj] + a[3 + ti] * a[3 + tj] + a[6 + ti] * a[6 + tj];
+      s -= f * (g + t * v[ti] * v[tj]);
+    }
+  }
+  return s;
+}
+
+// Position and type (-1 value, else gradient component) of flat row f.
+__device__ __forceinline__ const double* flat_row(const double* __restrict__ pts, const double* __restrict__ gpts,
+                                                  int64_t mu, int64_t f, int* type) {
+  if (f < mu) {
+    *type = -1;
+    return pts + 3 * f;
+  }
+  const int64_t k = (f - mu) / 3;
+  *type = static_cast<int>(f - mu - 3 * k);
+  return gpts + 3 * k;
+}
+
+// mat_a entry between flat rows u and v (nugget on the diagonal of value rows).
+__device__ __forceinline__ double flat_entry(const RbfSet& r, const double* __restrict__ pts,
+                                             const double* __restrict__ gpts, int64_t mu, int64_t u, int64_t v) {
+  if (u < mu && v < mu) return kernel_entry(r, pts, static_cast<int>(u), static_cast<int>(v));
+  int tu, tv;
+  const double* xu = flat_row(pts, gpts, mu, u, &tu);
+  const double* xv = flat_row(pts, gpts, mu, v, &tv);
+  return kernel_hb(r, tu, tv, xu[0] - xv[0], xu[1] - xv[1], xu[2] - xv[2]);
+}
+
 __device__ __forceinline__ void monomials(int degree, double x, double y, double z, double* m) {
   m[0] = 1.0;
   if (degree >= 1) { m[1] = x; m[2] = y; m[3] = z; }
@@ -92,6 +149,66 @@ __global__ void lagrange_kernel(int degree, int l, const double* __restrict__ pt
     double s = 0.0;
     for (int j = 0; j < l; j++) s += m[j] * pinv[j * l + k];
     lag[i * l + k] = s;
+  }
+}
+
+// Gradient rows of the Lagrange basis: lag[(3 k + c) * l + a] =
+// sum_j d m_j / d x_c (q_k) pinv[j * l + a] (LagrangeBasis::evaluate with
+// gradient points, lagrange_basis.hpp:50-58); lag points at the first
+// gradient row.
+__global__ void lagrange_grad_kernel(int degree, int l, const double* __restrict__ gpts, int64_t n,
+                                     const double* __restrict__ pinv, double* __restrict__ lag) {
+  int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double x = gpts[3 * k], y = gpts[3 * k + 1], z = gpts[3 * k + 2];
+  const double gm[3][10] = {{0.0, 1.0, 0.0, 0.0, 2.0 * x, y, z, 0.0, 0.0, 0.0},
+                            {0.0, 0.0, 1.0, 0.0, 0.0, x, 0.0, 2.0 * y, z, 0.0},
+                            {0.0, 0.0, 0.0, 1.0, 0.0, 0.0, x, 0.0, y, 2.0 * z}};
+  (void)degree;
+  for (int c = 0; c < 3; c++)
+    for (int a = 0; a < l; a++) {
+      double s = 0.0;
+      for (int j = 0; j < l; j++) s += gm[c][j] * pinv[j * l + a];
+      lag[(3 * k + c) * l + a] = s;
+    }
+}
+
+// Fields of the inner rows of every domain of a fit with gradient data, for
+// M = Q^T A Q as in assemble_rows_kernel: x, y, z, type, q_0.., g_0.. with
+// g_i = U[:, i] + Atl q_i / 2, U[a][i] = A(top_a, row_i), Atl = A(top, top)
+// (the first l rows of a domain are the polynomial rows). Field-major:
+// rows[(4 + 2l) o_d + f n_d + i].
+__global__ void assemble_hb_rows_kernel(RbfSet r, int l, const double* __restrict__ pts,
+                                        const double* __restrict__ gpts, int64_t mu,
+                                        const int64_t* __restrict__ ptr, const int32_t* __restrict__ idx,
+                                        const double* __restrict__ lag, double* __restrict__ rows) {
+  __shared__ double atl[10][10];
+  const int d = blockIdx.y;
+  const int64_t p0 = ptr[d];
+  const int n = static_cast<int>(ptr[d + 1] - p0) - l;
+  const int64_t o = p0 - static_cast<int64_t>(d) * l;
+  for (int e = threadIdx.x; e < l * l; e += kT)
+    atl[e / l][e % l] = flat_entry(r, pts, gpts, mu, idx[p0 + e / l], idx[p0 + e % l]);
+  __syncthreads();
+  double* rd = rows + static_cast<int64_t>(4 + 2 * l) * o;
+  for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) {
+    const int64_t f = idx[p0 + l + i];
+    int type;
+    const double* x = flat_row(pts, gpts, mu, f, &type);
+    rd[i] = x[0];
+    rd[n + i] = x[1];
+    rd[2 * n + i] = x[2];
+    rd[3 * n + i] = static_cast<double>(type);
+    double q[10];
+    for (int a = 0; a < l; a++) {
+      q[a] = -lag[f * l + a];
+      rd[static_cast<int64_t>(4 + a) * n + i] = q[a];
+    }
+    for (int a = 0; a < l; a++) {
+      double t = 0.0;
+      for (int b = 0; b < l; b++) t += atl[a][b] * q[b];
+      rd[static_cast<int64_t>(4 + l + a) * n + i] = flat_entry(r, pts, gpts, mu, idx[p0 + a], f) + 0.5 * t;
+    }
   }
 }
 
@@ -204,6 +321,67 @@ assemble_m_kernel(RbfSet r, const int64_t* __restrict__ ptr, const double* __res
       md[static_cast<int64_t>(j) * n + i] = static_cast<T>(v);
     }
   }
+}
+
+// assemble_m_kernel for fits with gradient data: the rows carry their type
+// (field 3) and entries come from kernel_hb.
+template <class T, int L>
+__global__ void __launch_bounds__(kT)
+assemble_m_hb_kernel(RbfSet r, const int64_t* __restrict__ ptr, const double* __restrict__ rows,
+                     const int64_t* __restrict__ mat_ptr, T* __restrict__ mat) {
+  constexpr int F = 4 + 2 * L;
+  __shared__ double cj[F][kTile];
+  const int d = blockIdx.y;
+  const int64_t p0 = ptr[d];
+  const int n = static_cast<int>(ptr[d + 1] - p0) - L;
+  if (n <= 0) return;
+  const int nt = (n + kTile - 1) / kTile;
+  const int ntiles = nt * (nt + 1) / 2;
+  const double* rd = rows + static_cast<int64_t>(F) * (p0 - static_cast<int64_t>(d) * L);
+  T* md = mat + mat_ptr[d];
+  const int tid = threadIdx.x, li = tid & 63, cg = tid >> 6;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    int I = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((I + 1) * (I + 2) / 2 <= t) I++;
+    while (I * (I + 1) / 2 > t) I--;
+    const int J = t - I * (I + 1) / 2;
+    __syncthreads();
+    for (int e = tid; e < F * kTile; e += kT) {
+      const int f = e / kTile, c = e % kTile, j = J * kTile + c;
+      cj[f][c] = j < n ? rd[static_cast<int64_t>(f) * n + j] : 0.0;
+    }
+    const int i = I * kTile + li;
+    double ri[F];
+    for (int f = 0; f < F; f++) ri[f] = i < n ? rd[static_cast<int64_t>(f) * n + i] : 0.0;
+    __syncthreads();
+    if (i >= n) continue;
+    const int ti = static_cast<int>(ri[3]);
+    for (int c = 0; c < 16; c++) {
+      const int jc = cg * 16 + c, j = J * kTile + jc;
+      if (j > i || j >= n) break;
+      const int tj = static_cast<int>(cj[3][jc]);
+      double v = kernel_hb(r, ti, tj, ri[0] - cj[0][jc], ri[1] - cj[1][jc], ri[2] - cj[2][jc]);
+      if (j == i && ti < 0) v += r.nugget;
+      for (int a = 0; a < L; a++) v += ri[4 + a] * cj[4 + L + a][jc] + ri[4 + L + a] * cj[4 + a][jc];
+      md[static_cast<int64_t>(j) * n + i] = static_cast<T>(v);
+    }
+  }
+}
+
+template <class T>
+void launch_assemble_m_hb(int l, int max_n, int n_dom, const RbfSet& rs, const int64_t* ptr, const double* rows,
+                          const int64_t* mat_ptr, T* mat, hipStream_t s) {
+  if (n_dom <= 0 || max_n <= 0) return;
+  const int nt = (max_n + kTile - 1) / kTile;
+  const dim3 grid(static_cast<unsigned>(std::min(nt * (nt + 1) / 2, 2048)), static_cast<unsigned>(n_dom));
+  switch (l) {
+    case 0: assemble_m_hb_kernel<T, 0><<<grid, kT, 0, s>>>(rs, ptr, rows, mat_ptr, mat); break;
+    case 1: assemble_m_hb_kernel<T, 1><<<grid, kT, 0, s>>>(rs, ptr, rows, mat_ptr, mat); break;
+    case 4: assemble_m_hb_kernel<T, 4><<<grid, kT, 0, s>>>(rs, ptr, rows, mat_ptr, mat); break;
+    case 10: assemble_m_hb_kernel<T, 10><<<grid, kT, 0, s>>>(rs, ptr, rows, mat_ptr, mat); break;
+    default: throw_invalid("unsupported polynomial basis size");
+  }
+  PAM_LAUNCH_CHECK();
 }
 
 // Launch of assemble_m_kernel for the polynomial basis size l.
@@ -726,7 +904,8 @@ __device__ __forceinline__ double lane_ordered_sum(double v, double* sh) {
 // KT threads (a multiple of 64, at least 128); every sum is taken in an order
 // that does not depend on KT, so the solution is the same bits for any KT.
 template <int KT, class Emit, class TF = double>
-__device__ void solve_domain(RbfSet r, int l, const double* __restrict__ pts, const int32_t* __restrict__ didx,
+__device__ void solve_domain(RbfSet r, int l, const double* __restrict__ pts, const double* __restrict__ gpts,
+                             int64_t mu_v, const int32_t* __restrict__ didx,
                              int m, const double* __restrict__ lag, const TF* __restrict__ L,
                              double* vals, double* y, double* sh, int coarse, const double* __restrict__ pinv,
                              Emit emit) {
@@ -842,7 +1021,7 @@ __device__ void solve_domain(RbfSet r, int l, const double* __restrict__ pts, co
       if (threadIdx.x < 64)
         for (int u = threadIdx.x; u < m; u += 64) {
           double lu = u < l ? lam_top[u] : y[u - l];
-          acc += kernel_entry(r, pts, pa, didx[u]) * lu;
+          acc += flat_entry(r, pts, gpts, mu_v, pa, didx[u]) * lu;
         }
       double t = lane_ordered_sum(acc, sh);
       if (threadIdx.x == 0) rhs_top[a] = vals[a] - t;
@@ -861,7 +1040,7 @@ __device__ void solve_domain(RbfSet r, int l, const double* __restrict__ pts, co
 // polynomial coefficients into out.
 template <int KT, class TF>
 __global__ void __launch_bounds__(KT)
-ras_solve_kernel(RbfSet r, int l, const double* __restrict__ pts,
+ras_solve_kernel(RbfSet r, int l, const double* __restrict__ pts, const double* __restrict__ gpts, int64_t mu_v,
                  const int64_t* __restrict__ ptr, const int32_t* __restrict__ idx,
                  const uint8_t* __restrict__ inner, const double* __restrict__ lag,
                  const int64_t* __restrict__ mat_ptr, const TF* __restrict__ mat,
@@ -878,7 +1057,7 @@ ras_solve_kernel(RbfSet r, int l, const double* __restrict__ pts,
   __syncthreads();
   const int32_t* didx = idx + p0;
   const uint8_t* din = inner + p0;
-  solve_domain<KT>(r, l, pts, didx, m, lag, mat + mat_ptr[d], vals, y, sh, coarse, pinv, [&](int u, double v) {
+  solve_domain<KT>(r, l, pts, gpts, mu_v, didx, m, lag, mat + mat_ptr[d], vals, y, sh, coarse, pinv, [&](int u, double v) {
     if (u < m) {
       if (coarse || din[u]) out[didx[u]] = v;
     } else {
@@ -890,7 +1069,8 @@ ras_solve_kernel(RbfSet r, int l, const double* __restrict__ pts,
 // The coarse grid's solve as a dense operator: workgroup u0 solves for the
 // unit vector e_u0 and writes column u0 of op ((m + l) x m, row-major).
 __global__ void __launch_bounds__(kT)
-coarse_operator_kernel(RbfSet r, int l, const double* __restrict__ pts, const int32_t* __restrict__ idx, int m,
+coarse_operator_kernel(RbfSet r, int l, const double* __restrict__ pts, const double* __restrict__ gpts, int64_t mu_v,
+                       const int32_t* __restrict__ idx, int m,
                        const double* __restrict__ lag, const double* __restrict__ mat,
                        const double* __restrict__ pinv, double* __restrict__ op) {
   extern __shared__ double sm[];
@@ -900,7 +1080,7 @@ coarse_operator_kernel(RbfSet r, int l, const double* __restrict__ pts, const in
   double* y = sm + m;
   for (int u = threadIdx.x; u < m; u += kT) vals[u] = u == u0 ? 1.0 : 0.0;
   __syncthreads();
-  solve_domain<kT>(r, l, pts, idx, m, lag, mat, vals, y, sh, 1, pinv,
+  solve_domain<kT>(r, l, pts, gpts, mu_v, idx, m, lag, mat, vals, y, sh, 1, pinv,
                [&](int u, double v) { op[static_cast<int64_t>(u) * m + u0] = v; });
 }
 
@@ -1388,11 +1568,235 @@ std::vector<int32_t> choose_coarse_points(const std::vector<double>& pts, const 
   return out;
 }
 
+// ---- Division with gradient points ----
+
+namespace {
+
+// A point or a gradient point of a domain or cluster (domain_divider.hpp:127-139).
+struct MixedPoint {
+  int32_t index;
+  uint8_t inner;
+  uint8_t grad;
+  int mult() const { return grad ? 3 : 1; }
+};
+
+struct MixedSet {
+  const std::vector<double>& pts;
+  const std::vector<double>& gpts;
+  const double* at(const MixedPoint& p) const { return (p.grad ? gpts : pts).data() + 3 * static_cast<size_t>(p.index); }
+  Bbox3 bbox(const std::vector<MixedPoint>& v) const {
+    Bbox3 bb;
+    for (const auto& p : v) bb.add(at(p));
+    return bb;
+  }
+  // Sort along the axes (widest first); ties keep their order.
+  void sort(std::vector<MixedPoint>& v, const std::array<int, 3>& ax) const {
+    std::stable_sort(v.begin(), v.end(), [&](const MixedPoint& a, const MixedPoint& b) {
+      const double *p = at(a), *q = at(b);
+      for (int k : ax)
+        if (p[k] != q[k]) return p[k] < q[k];
+      return false;
+    });
+  }
+};
+
+// Largest i with prefix[i] <= x (std::upper_bound(...) - 1 in domain_divider.hpp:199-212).
+int64_t prefix_position(const std::vector<int64_t>& prefix, int64_t x) {
+  return static_cast<int64_t>(std::upper_bound(prefix.begin(), prefix.end(), x) - prefix.begin()) - 1;
+}
+
+}  // namespace
+
+std::vector<Domain> divide_mixed(const std::vector<double>& pts, const std::vector<double>& gpts,
+                                 const std::vector<int32_t>& point_idcs, const std::vector<int32_t>& grad_idcs,
+                                 const std::vector<int32_t>& poly_idcs) {
+  // divide_domain (domain_divider.hpp:160-249) with the multiplicities of
+  // gradient points, until every domain holds at most 1024 rows.
+  constexpr double kOverlapQuota = 0.5;
+  constexpr int64_t kMaxLeafSize = 1024;
+  const MixedSet ms{pts, gpts};
+  std::vector<std::vector<MixedPoint>> work(1), leaves;
+  for (int32_t i : point_idcs) work[0].push_back({i, 1, 0});
+  for (int32_t i : grad_idcs) work[0].push_back({i, 1, 1});
+  auto rows = [](const std::vector<MixedPoint>& v) {
+    int64_t n = 0;
+    for (const auto& p : v) n += p.mult();
+    return n;
+  };
+  while (!work.empty()) {
+    const int64_t nw = static_cast<int64_t>(work.size());
+    std::vector<std::vector<MixedPoint>> next(static_cast<size_t>(2 * nw));
+    std::vector<uint8_t> split(static_cast<size_t>(nw));
+    for (int64_t w = 0; w < nw; w++) split[w] = rows(work[w]) > kMaxLeafSize;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t w = 0; w < nw; w++) {
+      if (!split[w]) continue;
+      std::vector<MixedPoint> v = work[w];
+      ms.sort(v, ms.bbox(v).axes_by_width());
+      std::vector<int64_t> prefix{0};
+      for (const auto& p : v) prefix.push_back(prefix.back() + p.mult());
+      const int64_t n_mult = prefix.back();
+      const double q = kOverlapQuota * static_cast<double>(kMaxLeafSize) / static_cast<double>(n_mult);
+      const int64_t n_sub = static_cast<int64_t>(round_half_to_even((1.0 + q) / 2.0 * static_cast<double>(n_mult)));
+      const int64_t left_mult = n_mult - n_sub, right_mult = n_sub;
+      const int64_t mid_mult =
+          static_cast<int64_t>(round_half_to_even(static_cast<double>(left_mult + right_mult) / 2.0));
+      const int64_t left = prefix_position(prefix, left_mult);
+      const int64_t right = prefix_position(prefix, right_mult);
+      const int64_t mid = prefix_position(prefix, mid_mult);
+      const int64_t n = static_cast<int64_t>(v.size());
+      auto& lo = next[2 * w];
+      auto& hi = next[2 * w + 1];
+      for (int64_t i = 0; i < right; i++) lo.push_back({v[i].index, static_cast<uint8_t>(v[i].inner && i < mid), v[i].grad});
+      for (int64_t i = left; i < n; i++) hi.push_back({v[i].index, static_cast<uint8_t>(v[i].inner && i >= mid), v[i].grad});
+    }
+    std::vector<std::vector<MixedPoint>> cont;
+    for (int64_t w = 0; w < nw; w++) {
+      if (split[w]) {
+        cont.push_back(std::move(next[2 * w]));
+        cont.push_back(std::move(next[2 * w + 1]));
+      } else {
+        leaves.push_back(std::move(work[w]));
+      }
+    }
+    work.swap(cont);
+  }
+  // merge_poly_points (domain.hpp:30-51) on the points; the gradient points
+  // are kept in index order as well.
+  std::vector<Domain> out(leaves.size());
+#pragma omp parallel for schedule(dynamic, 16)
+  for (size_t di = 0; di < leaves.size(); di++) {
+    std::vector<std::pair<int32_t, uint8_t>> vp, gp;
+    for (const auto& p : leaves[di]) (p.grad ? gp : vp).push_back({p.index, p.inner});
+    std::sort(vp.begin(), vp.end());
+    std::sort(gp.begin(), gp.end());
+    Domain& d = out[di];
+    d.idx = poly_idcs;
+    d.inner.assign(poly_idcs.size(), 0);
+    for (size_t a = 0; a < poly_idcs.size(); a++) {
+      auto it = std::lower_bound(vp.begin(), vp.end(), std::make_pair(poly_idcs[a], uint8_t{0}));
+      if (it != vp.end() && it->first == poly_idcs[a]) {
+        d.inner[a] = it->second;
+        vp.erase(it);
+      }
+    }
+    for (const auto& p : vp) {
+      d.idx.push_back(p.first);
+      d.inner.push_back(p.second);
+    }
+    for (const auto& p : gp) {
+      d.gidx.push_back(p.first);
+      d.ginner.push_back(p.second);
+    }
+  }
+  return out;
+}
+
+std::pair<std::vector<int32_t>, std::vector<int32_t>> choose_coarse_points_mixed(
+    const std::vector<double>& pts, const std::vector<double>& gpts, const std::vector<int32_t>& point_idcs,
+    const std::vector<int32_t>& grad_idcs, const std::vector<int32_t>& poly_idcs, int64_t n_coarse) {
+  // domain_divider.hpp:54-125: split the cluster of lowest level (largest
+  // bbox volume first) at the middle of its rows until the clusters' centres
+  // count n_coarse rows; the centre of a cluster is its point (or gradient
+  // point) nearest its bbox centre.
+  const MixedSet ms{pts, gpts};
+  struct Cluster {
+    std::vector<MixedPoint> p;
+    int level = 0;
+    double volume = 0.0;
+    int64_t seq = 0;
+    MixedPoint center{};
+  };
+  int64_t seq = 0;
+  auto init = [&](Cluster& c) {
+    const Bbox3 bb = ms.bbox(c.p);
+    double cen[3];
+    c.volume = 1.0;
+    for (int k = 0; k < 3; k++) {
+      cen[k] = bb.lo[k] + (bb.hi[k] - bb.lo[k]) / 2.0;
+      c.volume *= bb.hi[k] - bb.lo[k];
+    }
+    double best = INFINITY;
+    for (const auto& p : c.p) {
+      const double* x = ms.at(p);
+      const double d = (x[0] - cen[0]) * (x[0] - cen[0]) + (x[1] - cen[1]) * (x[1] - cen[1]) +
+                       (x[2] - cen[2]) * (x[2] - cen[2]);
+      if (d < best) best = d, c.center = p;
+    }
+    ms.sort(c.p, bb.axes_by_width());
+    c.seq = seq++;
+  };
+  auto lower = [](const Cluster& a, const Cluster& b) {  // a pops after b
+    if (a.level != b.level) return a.level > b.level;
+    if (a.volume != b.volume) return a.volume < b.volume;
+    return a.seq > b.seq;
+  };
+  std::priority_queue<Cluster, std::vector<Cluster>, decltype(lower)> q(lower);
+  std::vector<int32_t> sorted_poly(poly_idcs);
+  std::sort(sorted_poly.begin(), sorted_poly.end());
+  Cluster root;
+  for (int32_t i : point_idcs)
+    if (!std::binary_search(sorted_poly.begin(), sorted_poly.end(), i)) root.p.push_back({i, 1, 0});
+  for (int32_t i : grad_idcs) root.p.push_back({i, 1, 1});
+  std::vector<int32_t> idcs(poly_idcs), gidcs;
+  if (root.p.empty()) return {idcs, gidcs};
+  int64_t total = 0;  // all clusters single points: their centres count every row
+  for (const auto& p : root.p) total += p.mult();
+  n_coarse = std::min(n_coarse, total);
+  init(root);
+  int64_t current = root.center.mult();
+  q.push(std::move(root));
+  while (current < n_coarse) {
+    Cluster c = q.top();
+    q.pop();
+    std::vector<int64_t> prefix{0};
+    for (const auto& p : c.p) prefix.push_back(prefix.back() + p.mult());
+    const int64_t size = prefix.back();
+    int64_t mid = 0, bestd = INT64_MAX;
+    for (int64_t i = 0; i < static_cast<int64_t>(c.p.size()); i++) {
+      const int64_t dd = std::llabs(2 * prefix[i] - size);
+      if (dd < bestd || (dd == bestd && i % 2 == 0)) bestd = dd, mid = i;
+    }
+    current -= c.center.mult();
+    Cluster l, r;
+    l.p.assign(c.p.begin(), c.p.begin() + mid);
+    r.p.assign(c.p.begin() + mid, c.p.end());
+    l.level = r.level = c.level + 1;
+    if (!l.p.empty()) {
+      init(l);
+      current += l.center.mult();
+      q.push(std::move(l));
+    }
+    if (!r.p.empty()) {
+      init(r);
+      current += r.center.mult();
+      q.push(std::move(r));
+    }
+    if (q.empty()) break;
+  }
+  while (!q.empty()) {
+    const MixedPoint& p = q.top().center;
+    (p.grad ? gidcs : idcs).push_back(p.index);
+    q.pop();
+  }
+  return {idcs, gidcs};
+}
+
 // ---- RasPreconditioner ----
 
 RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<double>& points,
                                      hipStream_t s, const RasOptions& opt)
-    : model_(model), s_(s), opt_(opt), mu_(static_cast<int64_t>(points.size() / 3)), l_(model.l()) {
+    : RasPreconditioner(model, points, std::vector<double>{}, s, opt) {}
+
+RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<double>& points,
+                                     const std::vector<double>& grad_points, hipStream_t s, const RasOptions& opt)
+    : model_(model),
+      s_(s),
+      opt_(opt),
+      mu_(static_cast<int64_t>(points.size() / 3)),
+      sigma_(static_cast<int64_t>(grad_points.size() / 3)),
+      n_(mu_ + 3 * sigma_),
+      l_(model.l()) {
   auto t0 = std::chrono::steady_clock::now();
   model_.validate();
   constexpr double kFineToCoarseRatio = 10.0;  // ras_preconditioner.hpp:52
@@ -1400,24 +1804,52 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
   if (kNCoarsestPoints < 1 || kNCoarsestPoints > RasOptions::kMaxCoarsest)
     throw_invalid("the coarse grid size must be in [1, " + std::to_string(RasOptions::kMaxCoarsest) + "]");
   h_pts_ = points;
+  h_gpts_ = grad_points;
   d_pts_.upload(h_pts_.data(), h_pts_.size(), s_);
-  n_levels_ = std::max(static_cast<int>(std::ceil(std::log(static_cast<double>(mu_) / kNCoarsestPoints) /
+  d_gpts_.upload(h_gpts_.data(), h_gpts_.size(), s_);
+  n_levels_ = std::max(static_cast<int>(std::ceil(std::log(static_cast<double>(n_) / kNCoarsestPoints) /
                                                   std::log(kFineToCoarseRatio))), 0) + 1;
   point_idcs_.resize(n_levels_);
+  gpoint_idcs_.resize(n_levels_);
   std::vector<int32_t> poly_idcs;
   if (l_ > 0) {
-    poly_idcs = unisolvent_point_set(h_pts_, model_.poly_degree);
+    // The polynomial rows: a unisolvent set of points; in the special case of
+    // one point with gradient points and degree 1, that point and the first
+    // gradient point (ras_preconditioner.hpp:81-92). Degree-1 Lagrange
+    // polynomials have constant gradients, so any gradient point of a domain
+    // serves as its top rows.
+    const bool special = model_.poly_degree == 1 && mu_ == 1 && sigma_ >= 1;
     std::vector<double> p(static_cast<size_t>(l_) * l_), inv;
-    for (int i = 0; i < l_; i++) {
+    if (special) {
+      poly_idcs = {0};
       double m[10];
-      host_monomials(model_.poly_degree, &h_pts_[3 * static_cast<size_t>(poly_idcs[i])], m);
-      for (int k = 0; k < l_; k++) p[i * l_ + k] = m[k];
+      host_monomials(1, &h_pts_[0], m);
+      for (int k = 0; k < l_; k++) p[k] = m[k];
+      const double gm[3][4] = {{0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+      for (int c = 0; c < 3; c++)
+        for (int k = 0; k < l_; k++) p[(1 + c) * l_ + k] = gm[c][k];
+    } else {
+      if (mu_ < l_) throw_invalid("points.rows() must be greater than or equal to " + std::to_string(l_));
+      poly_idcs = unisolvent_point_set(h_pts_, model_.poly_degree);
+      for (int i = 0; i < l_; i++) {
+        double m[10];
+        host_monomials(model_.poly_degree, &h_pts_[3 * static_cast<size_t>(poly_idcs[i])], m);
+        for (int k = 0; k < l_; k++) p[i * l_ + k] = m[k];
+      }
     }
     if (invert(p, l_, inv) == 0.0) throw_invalid("the set of points is not unisolvent");
     pinv_.upload(inv.data(), inv.size(), s_);
-    lag_.resize(static_cast<size_t>(mu_) * l_);
-    lagrange_kernel<<<ceil_div(mu_, kT), kT, 0, s_>>>(model_.poly_degree, l_, d_pts_.get(), mu_, pinv_.get(), lag_.get());
-    PAM_LAUNCH_CHECK();
+    lag_.resize(static_cast<size_t>(n_) * l_);
+    if (mu_ > 0) {
+      lagrange_kernel<<<ceil_div(mu_, kT), kT, 0, s_>>>(model_.poly_degree, l_, d_pts_.get(), mu_, pinv_.get(),
+                                                        lag_.get());
+      PAM_LAUNCH_CHECK();
+    }
+    if (sigma_ > 0) {
+      lagrange_grad_kernel<<<ceil_div(sigma_, kT), kT, 0, s_>>>(model_.poly_degree, l_, d_gpts_.get(), sigma_,
+                                                                pinv_.get(), lag_.get() + mu_ * l_);
+      PAM_LAUNCH_CHECK();
+    }
     auto& top = point_idcs_[n_levels_ - 1];
     top = poly_idcs;
     std::vector<uint8_t> is_poly(mu_, 0);
@@ -1431,14 +1863,20 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
     lag_.resize(1);
     pinv_.resize(1);
   }
+  gpoint_idcs_[n_levels_ - 1].resize(sigma_);
+  std::iota(gpoint_idcs_[n_levels_ - 1].begin(), gpoint_idcs_[n_levels_ - 1].end(), 0);
   // Domain geometry in the anisotropic frame when there is a single rbf
   // (ras_preconditioner.hpp:105-113).
-  std::vector<double> a_pts = h_pts_;
+  std::vector<double> a_pts = h_pts_, a_gpts = h_gpts_;
   if (model_.rbfs.size() == 1 && !is_identity(model_.rbfs[0].aniso)) {
     const double* a = model_.rbfs[0].aniso;
     for (int64_t i = 0; i < mu_; i++) {
       const double* p = &h_pts_[3 * i];
       for (int k = 0; k < 3; k++) a_pts[3 * i + k] = a[3 * k] * p[0] + a[3 * k + 1] * p[1] + a[3 * k + 2] * p[2];
+    }
+    for (int64_t i = 0; i < sigma_; i++) {
+      const double* p = &h_gpts_[3 * i];
+      for (int k = 0; k < 3; k++) a_gpts[3 * i + k] = a[3 * k] * p[0] + a[3 * k + 1] * p[1] + a[3 * k + 2] * p[2];
     }
   }
   // A P depends on the points only: it is computed on a host thread and a
@@ -1446,9 +1884,10 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
   // divides the levels and the device factors them.
   std::future<void> ap_done;
   if (n_levels_ > 1 && l_ > 0) {
-    p_orth_.resize(static_cast<size_t>(mu_) * l_);
-    dev_poly_matrix(model_.poly_degree, d_pts_.get(), mu_, p_orth_.get(), s_);
-    dev_orthonormalize_cols(p_orth_.get(), mu_, l_, ws_, s_);
+    p_orth_.resize(static_cast<size_t>(n_) * l_);
+    if (sigma_ == 0) dev_poly_matrix(model_.poly_degree, d_pts_.get(), mu_, p_orth_.get(), s_);
+    else dev_poly_matrix_hermite(model_.poly_degree, d_pts_.get(), mu_, d_gpts_.get(), sigma_, p_orth_.get(), s_);
+    dev_orthonormalize_cols(p_orth_.get(), n_, l_, ws_, s_);
     PAM_HIP(hipStreamSynchronize(s_));
     int dev = 0;
     PAM_HIP(hipGetDevice(&dev));
@@ -1481,7 +1920,7 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
   // exist; the host then picks the coarser level's points and divides it
   // while the device assembles and factors.
   for (int level = n_levels_ - 1; level >= 1; level--) {
-    double finest = std::log(static_cast<double>(mu_)) / std::log(kFineToCoarseRatio);
+    double finest = std::log(static_cast<double>(n_)) / std::log(kFineToCoarseRatio);
     double coarsest = std::log(static_cast<double>(kNCoarsestPoints)) / std::log(kFineToCoarseRatio);
     auto n_coarse = static_cast<int64_t>(std::pow(
         kFineToCoarseRatio, coarsest + (level - 1) * (finest - coarsest) / (n_levels_ - 1)));
@@ -1489,6 +1928,19 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
     // its domains: they are chosen on a second host thread while this one
     // divides the level and launches its batch.
     const std::vector<int32_t>& lp = point_idcs_[level];
+    const std::vector<int32_t>& lg = gpoint_idcs_[level];
+    if (sigma_ > 0) {
+      // with gradient points (the sizes of fits with gradient data are small:
+      // one host thread)
+      auto cp = choose_coarse_points_mixed(a_pts, a_gpts, lp, lg, poly_idcs, n_coarse);
+      std::vector<Domain> doms = divide_mixed(a_pts, a_gpts, lp, lg, poly_idcs);
+      tick("divide");
+      setup_batch(fine_[level], doms, false, fork_stream(), opt_.fp32_factors, flag_.get() + level);
+      point_idcs_[level - 1] = std::move(cp.first);
+      gpoint_idcs_[level - 1] = std::move(cp.second);
+      h_domains_[level] = std::move(doms);
+      continue;
+    }
     auto coarse = std::async(std::launch::async,
                              [&] { return choose_coarse_points(a_pts, lp, poly_idcs, n_coarse); });
     std::vector<Domain> doms;
@@ -1510,6 +1962,8 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
     Domain cd;
     cd.idx = point_idcs_[0];
     cd.inner.assign(cd.idx.size(), 1);
+    cd.gidx = gpoint_idcs_[0];
+    cd.ginner.assign(cd.gidx.size(), 1);
     h_domains_[0] = {cd};
     hipStream_t cs = fork_stream();
     setup_batch(coarse_, h_domains_[0], true, cs, false, flag_.get());
@@ -1521,8 +1975,8 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
     if (coarse_.n_dom == 1 && m > 0 && m <= kCoarseOpMax && shmem <= 96 * 1024) {
       coarse_.op.resize(static_cast<size_t>(m + l_) * m);
       RbfSet rs = make_rbf_set(model_);
-      coarse_operator_kernel<<<m, kT, shmem, cs>>>(rs, l_, d_pts_.get(), coarse_.idx.get(), m, lag_.get(),
-                                                   coarse_.mat.get(), pinv_.get(), coarse_.op.get());
+      coarse_operator_kernel<<<m, kT, shmem, cs>>>(rs, l_, d_pts_.get(), d_gpts_.get(), mu_, coarse_.idx.get(), m,
+                                                   lag_.get(), coarse_.mat.get(), pinv_.get(), coarse_.op.get());
       PAM_LAUNCH_CHECK();
       coarse_.has_op = true;
     }
@@ -1552,11 +2006,22 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
   for (int32_t f : flags)
     if (f) throw_runtime("a subdomain matrix Q^T A Q is not positive definite");
   d_point_idcs_.resize(n_levels_);
-  for (int level = 0; level < n_levels_; level++)
+  d_gpoint_idcs_.resize(n_levels_);
+  d_level_rows_.resize(n_levels_);
+  level_rows_.assign(n_levels_, 0);
+  for (int level = 0; level < n_levels_; level++) {
     d_point_idcs_[level].upload(point_idcs_[level].data(), point_idcs_[level].size(), s_);
-  res_.resize(mu_);
-  w_.resize(mu_ + l_);
-  total_.resize(mu_ + l_);
+    d_gpoint_idcs_[level].upload(gpoint_idcs_[level].data(), gpoint_idcs_[level].size(), s_);
+    std::vector<int32_t> rows(point_idcs_[level]);
+    for (int32_t k : gpoint_idcs_[level])
+      for (int c = 0; c < 3; c++) rows.push_back(static_cast<int32_t>(mu_ + 3 * static_cast<int64_t>(k) + c));
+    level_rows_[level] = static_cast<int64_t>(rows.size());
+    d_level_rows_[level].upload(rows.data(), rows.size(), s_);
+    PAM_HIP(hipStreamSynchronize(s_));  // rows is a host temporary
+  }
+  res_.resize(n_);
+  w_.resize(n_ + l_);
+  total_.resize(n_ + l_);
   dot_.resize(10);
   if (ap_done.valid()) ap_done.get();
   PAM_HIP(hipStreamSynchronize(s_));
@@ -1602,7 +2067,7 @@ void RasPreconditioner::setup_batch(DomainBatch& b, const std::vector<Domain>& d
   std::vector<int64_t> mptr(b.n_dom + 1, 0);
   b.max_m = 0;
   for (int d = 0; d < b.n_dom; d++) {
-    int64_t m = static_cast<int64_t>(doms[d].idx.size());
+    int64_t m = static_cast<int64_t>(doms[d].idx.size() + 3 * doms[d].gidx.size());
     if (l_ > 0 && m < l_) throw_runtime("a domain holds fewer points than the polynomial basis");
     b.h_ptr[d + 1] = b.h_ptr[d] + m;
     int64_t n = m - l_;
@@ -1616,6 +2081,12 @@ void RasPreconditioner::setup_batch(DomainBatch& b, const std::vector<Domain>& d
   for (int d = 0; d < b.n_dom; d++) {
     std::copy(doms[d].idx.begin(), doms[d].idx.end(), tmp->idx.begin() + b.h_ptr[d]);
     std::copy(doms[d].inner.begin(), doms[d].inner.end(), tmp->inner.begin() + b.h_ptr[d]);
+    int64_t r = b.h_ptr[d] + static_cast<int64_t>(doms[d].idx.size());
+    for (size_t g = 0; g < doms[d].gidx.size(); g++)
+      for (int c = 0; c < 3; c++, r++) {
+        tmp->idx[r] = static_cast<int32_t>(mu_ + 3 * static_cast<int64_t>(doms[d].gidx[g]) + c);
+        tmp->inner[r] = doms[d].ginner[g];
+      }
   }
   b.idx.upload(tmp->idx.data(), tmp->idx.size(), ls);
   b.inner.upload(tmp->inner.data(), tmp->inner.size(), ls);
@@ -1630,28 +2101,48 @@ void RasPreconditioner::setup_batch(DomainBatch& b, const std::vector<Domain>& d
     b.mat32.release();
   }
   RbfSet rs = make_rbf_set(model_);
-  tmp->u.resize(static_cast<size_t>(std::max<int64_t>(l_ * (b.h_ptr.back() - static_cast<int64_t>(b.n_dom) * l_), 1)));
-  tmp->atl.resize(static_cast<size_t>(std::max(b.n_dom * l_ * l_, 1)));
-  if (l_ > 0) {
-    assemble_u_kernel<<<dim3(16, b.n_dom), kT, 0, ls>>>(rs, l_, d_pts_.get(), b.ptr.get(), b.idx.get(),
-                                                        tmp->u.get(), tmp->atl.get());
-    PAM_LAUNCH_CHECK();
-  }
-  tmp->rows.resize(static_cast<size_t>(std::max<int64_t>((3 + 2 * l_) * (b.h_ptr.back() - static_cast<int64_t>(b.n_dom) * l_), 1)));
-  const int max_n = b.max_m - l_;
-  if (max_n > 0) {
-    assemble_rows_kernel<<<dim3(static_cast<unsigned>(ceil_div(max_n, kT)), b.n_dom), kT, 0, ls>>>(
-        l_, d_pts_.get(), b.ptr.get(), b.idx.get(), lag_.get(), tmp->u.get(), tmp->atl.get(), tmp->rows.get());
-    PAM_LAUNCH_CHECK();
+  if (sigma_ > 0) {
+    // rows of mixed type: fields with the row type, entries by kernel_hb
+    const int max_n = b.max_m - l_;
+    tmp->rows.resize(static_cast<size_t>(
+        std::max<int64_t>((4 + 2 * l_) * (b.h_ptr.back() - static_cast<int64_t>(b.n_dom) * l_), 1)));
+    if (max_n > 0) {
+      assemble_hb_rows_kernel<<<dim3(static_cast<unsigned>(ceil_div(max_n, kT)), b.n_dom), kT, 0, ls>>>(
+          rs, l_, d_pts_.get(), d_gpts_.get(), mu_, b.ptr.get(), b.idx.get(), lag_.get(), tmp->rows.get());
+      PAM_LAUNCH_CHECK();
+    }
+    if (fp32) launch_assemble_m_hb<float>(l_, max_n, b.n_dom, rs, b.ptr.get(), tmp->rows.get(), b.mat_ptr.get(),
+                                          b.mat32.get(), ls);
+    else launch_assemble_m_hb<double>(l_, max_n, b.n_dom, rs, b.ptr.get(), tmp->rows.get(), b.mat_ptr.get(),
+                                      b.mat.get(), ls);
+  } else {
+    tmp->u.resize(static_cast<size_t>(
+        std::max<int64_t>(l_ * (b.h_ptr.back() - static_cast<int64_t>(b.n_dom) * l_), 1)));
+    tmp->atl.resize(static_cast<size_t>(std::max(b.n_dom * l_ * l_, 1)));
+    if (l_ > 0) {
+      assemble_u_kernel<<<dim3(16, b.n_dom), kT, 0, ls>>>(rs, l_, d_pts_.get(), b.ptr.get(), b.idx.get(),
+                                                          tmp->u.get(), tmp->atl.get());
+      PAM_LAUNCH_CHECK();
+    }
+    tmp->rows.resize(static_cast<size_t>(
+        std::max<int64_t>((3 + 2 * l_) * (b.h_ptr.back() - static_cast<int64_t>(b.n_dom) * l_), 1)));
+    const int max_n = b.max_m - l_;
+    if (max_n > 0) {
+      assemble_rows_kernel<<<dim3(static_cast<unsigned>(ceil_div(max_n, kT)), b.n_dom), kT, 0, ls>>>(
+          l_, d_pts_.get(), b.ptr.get(), b.idx.get(), lag_.get(), tmp->u.get(), tmp->atl.get(), tmp->rows.get());
+      PAM_LAUNCH_CHECK();
+    }
+    if (fp32) launch_assemble_m<float>(l_, max_n, b.n_dom, rs, b.ptr.get(), tmp->rows.get(), b.mat_ptr.get(),
+                                       b.mat32.get(), ls);
+    else launch_assemble_m<double>(l_, max_n, b.n_dom, rs, b.ptr.get(), tmp->rows.get(), b.mat_ptr.get(),
+                                   b.mat.get(), ls);
   }
   if (fp32) {
-    launch_assemble_m<float>(l_, max_n, b.n_dom, rs, b.ptr.get(), tmp->rows.get(), b.mat_ptr.get(), b.mat32.get(), ls);
-    launch_cholesky_batch<float>(factor_panel(b, true, ls, tmp->mptr), b.n_dom, b.ptr.get(), l_, b.mat_ptr.get(), b.mat32.get(),
-                                 flag, ls);
+    launch_cholesky_batch<float>(factor_panel(b, true, ls, tmp->mptr), b.n_dom, b.ptr.get(), l_, b.mat_ptr.get(),
+                                 b.mat32.get(), flag, ls);
     setup_tmp_.push_back(std::move(tmp));
     return;
   }
-  launch_assemble_m<double>(l_, max_n, b.n_dom, rs, b.ptr.get(), tmp->rows.get(), b.mat_ptr.get(), b.mat.get(), ls);
   const int n0 = static_cast<int>(b.h_ptr.back() - l_);
   if (b.n_dom == 1 && n0 > 4 * kTile) {
     // One large matrix (the coarse grid): many workgroups per panel step
@@ -1740,13 +2231,13 @@ int64_t RasPreconditioner::factor_bytes() const {
 
 void RasPreconditioner::solve(int level, const double* d_res, double* d_w) {
   // ras_preconditioner.hpp:280-297
-  dev_fill(0.0, d_w, mu_ + l_, s_);
+  dev_fill(0.0, d_w, n_ + l_, s_);
   DomainBatch& b = level == 0 ? coarse_ : fine_[level];
   if (b.n_dom == 0) return;
   if (level == 0 && b.has_op) {
     const int m = static_cast<int>(b.h_ptr[1] - b.h_ptr[0]);
     coarse_apply_kernel<<<ceil_div(m + l_, kCoarseRowsPerBlock), kT, sizeof(double) * m, s_>>>(
-        b.idx.get(), m, l_, b.op.get(), d_res, d_w, mu_);
+        b.idx.get(), m, l_, b.op.get(), d_res, d_w, n_);
     PAM_LAUNCH_CHECK();
     return;
   }
@@ -1761,14 +2252,14 @@ void RasPreconditioner::launch_solve(const DomainBatch& b, int level, int thread
   auto go = [&](auto kt) {
     constexpr int KT = decltype(kt)::value;
     if (b.fp32) {
-      ras_solve_kernel<KT, float><<<b.n_dom, KT, shmem, s_>>>(rs, l_, d_pts_.get(), b.ptr.get(), b.idx.get(),
-                                                              b.inner.get(), lag_.get(), b.mat_ptr.get(),
-                                                              b.mat32.get(), d_res, d_w, mu_, level == 0 ? 1 : 0,
+      ras_solve_kernel<KT, float><<<b.n_dom, KT, shmem, s_>>>(rs, l_, d_pts_.get(), d_gpts_.get(), mu_, b.ptr.get(),
+                                                              b.idx.get(), b.inner.get(), lag_.get(), b.mat_ptr.get(),
+                                                              b.mat32.get(), d_res, d_w, n_, level == 0 ? 1 : 0,
                                                               pinv_.get());
     } else {
-      ras_solve_kernel<KT, double><<<b.n_dom, KT, shmem, s_>>>(rs, l_, d_pts_.get(), b.ptr.get(), b.idx.get(),
-                                                               b.inner.get(), lag_.get(), b.mat_ptr.get(),
-                                                               b.mat.get(), d_res, d_w, mu_, level == 0 ? 1 : 0,
+      ras_solve_kernel<KT, double><<<b.n_dom, KT, shmem, s_>>>(rs, l_, d_pts_.get(), d_gpts_.get(), mu_, b.ptr.get(),
+                                                               b.idx.get(), b.inner.get(), lag_.get(), b.mat_ptr.get(),
+                                                               b.mat.get(), d_res, d_w, n_, level == 0 ? 1 : 0,
                                                                pinv_.get());
     }
   };
@@ -1785,8 +2276,8 @@ void RasPreconditioner::tune_solves() {
   // Workgroup size of each level's batched solve, timed on the batch itself
   // (a zero right-hand side: the work does not depend on the values). Every
   // size gives the same bits, so the choice only changes the time.
-  DevBuf<double> res(static_cast<size_t>(mu_ + l_)), w(static_cast<size_t>(mu_ + l_));
-  dev_fill(0.0, res.get(), mu_ + l_, s_);
+  DevBuf<double> res(static_cast<size_t>(n_ + l_)), w(static_cast<size_t>(n_ + l_));
+  dev_fill(0.0, res.get(), n_ + l_, s_);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   PAM_HIP(hipEventCreate(&e0));
   PAM_HIP(hipEventCreate(&e1));
@@ -1817,7 +2308,21 @@ void RasPreconditioner::tune_solves() {
 
 void RasPreconditioner::compute_ap(hipStream_t s) {
   // ap = A P (ras_preconditioner.hpp:173-188), with the level-update configuration
-  ap_.resize(static_cast<size_t>(mu_) * l_);
+  ap_.resize(static_cast<size_t>(n_) * l_);
+  if (sigma_ > 0) {
+    HermiteEvaluator ev(model_, hermite_level_cfg(), s);
+    ev.set_sources(d_pts_.get(), mu_, d_gpts_.get(), sigma_);
+    ev.set_targets(d_pts_.get(), mu_, d_gpts_.get(), sigma_);
+    DevBuf<double> wcol(static_cast<size_t>(n_ + l_));
+    for (int k = 0; k < l_; k++) {
+      dev_fill(0.0, wcol.get(), n_ + l_, s);
+      dev_copy(p_orth_.get() + static_cast<int64_t>(k) * n_, wcol.get(), n_, s);
+      ev.set_weights(wcol.get());
+      ev.evaluate(ap_.get() + static_cast<int64_t>(k) * n_);
+    }
+    PAM_HIP(hipStreamSynchronize(s));
+    return;
+  }
   Bbox3 bb;
   for (int64_t i = 0; i < mu_; i++) bb.add(&h_pts_[3 * i]);
   SymmetricEvaluator ev(model_, bb.lo, bb.hi, INFINITY, s);
@@ -1838,7 +2343,50 @@ void RasPreconditioner::set_level_config(const FmmConfig& cfg) {
   opt_.level_cfg = cfg;
   opt_.use_level_cfg = true;
   evaluators_.clear();
+  hevaluators_.clear();
   if (n_levels_ > 1 && l_ > 0) compute_ap(s_);
+}
+
+FmmConfig RasPreconditioner::hermite_level_cfg() const {
+  if (opt_.use_level_cfg) return opt_.level_cfg;
+  FmmConfig c;
+  c.precision = kFp64;
+  c.order = 10;
+  return c;
+}
+
+const double* RasPreconditioner::level_points(int level) {
+  auto jt = level_pts_.find(level);
+  if (jt != level_pts_.end()) return jt->second.get();
+  DevBuf<double>& buf = level_pts_[level];
+  buf.resize(point_idcs_[level].size() * 3);
+  dev_gather_points(d_pts_.get(), d_point_idcs_[level].get(), static_cast<int64_t>(point_idcs_[level].size()),
+                    buf.get(), s_);
+  return buf.get();
+}
+
+const double* RasPreconditioner::level_grad_points(int level) {
+  auto jt = level_gpts_.find(level);
+  if (jt != level_gpts_.end()) return jt->second.get();
+  DevBuf<double>& buf = level_gpts_[level];
+  buf.resize(gpoint_idcs_[level].size() * 3);
+  dev_gather_points(d_gpts_.get(), d_gpoint_idcs_[level].get(), static_cast<int64_t>(gpoint_idcs_[level].size()),
+                    buf.get(), s_);
+  return buf.get();
+}
+
+HermiteEvaluator& RasPreconditioner::hermite_evaluator(int src, int trg) {
+  auto key = std::make_pair(src, trg);
+  auto it = hevaluators_.find(key);
+  if (it != hevaluators_.end()) return *it->second;
+  auto ev = std::make_unique<HermiteEvaluator>(model_, hermite_level_cfg(), s_);
+  ev->set_sources(level_points(src), static_cast<int64_t>(point_idcs_[src].size()), level_grad_points(src),
+                  static_cast<int64_t>(gpoint_idcs_[src].size()));
+  ev->set_targets(level_points(trg), static_cast<int64_t>(point_idcs_[trg].size()), level_grad_points(trg),
+                  static_cast<int64_t>(gpoint_idcs_[trg].size()));
+  HermiteEvaluator& ref = *ev;
+  hevaluators_[key] = std::move(ev);
+  return ref;
 }
 
 Evaluator& RasPreconditioner::evaluator(int src, int trg) {
@@ -1869,6 +2417,18 @@ Evaluator& RasPreconditioner::evaluator(int src, int trg) {
 
 void RasPreconditioner::update_residuals(int src, int trg, const double* d_w, double* d_res) {
   // ras_preconditioner.hpp:299-328
+  if (sigma_ > 0) {
+    const int64_t ns = level_rows_[src], nt = level_rows_[trg];
+    sw_.resize(static_cast<size_t>(ns + l_));
+    dev_gather(d_w, d_level_rows_[src].get(), ns, sw_.get(), s_);
+    if (l_ > 0) dev_copy(d_w + n_, sw_.get() + ns, l_, s_);
+    HermiteEvaluator& ev = hermite_evaluator(src, trg);
+    ev.set_weights(sw_.get());
+    tmp_.resize(static_cast<size_t>(nt));
+    ev.evaluate(tmp_.get());
+    dev_scatter_add(tmp_.get(), d_level_rows_[trg].get(), nt, -1.0, d_res, s_);
+    return;
+  }
   int64_t ns = static_cast<int64_t>(point_idcs_[src].size());
   int64_t nt = static_cast<int64_t>(point_idcs_[trg].size());
   sw_.resize(static_cast<size_t>(ns + l_));
@@ -1885,9 +2445,9 @@ void RasPreconditioner::orthogonalize(double* d_w, double* d_res) {
   // ras_preconditioner.hpp:271-278
   if (l_ == 0) return;
   // The projections stay on the device: no host round trip inside apply().
-  dev_multi_dot_device(p_orth_.get(), mu_, l_, d_w, mu_, dot_.get(), ws_, s_);
-  dev_gemv_update(d_w, p_orth_.get(), mu_, l_, dot_.get(), mu_, -1.0, s_);
-  dev_gemv_update(d_res, ap_.get(), mu_, l_, dot_.get(), mu_, 1.0, s_);
+  dev_multi_dot_device(p_orth_.get(), n_, l_, d_w, n_, dot_.get(), ws_, s_);
+  dev_gemv_update(d_w, p_orth_.get(), n_, l_, dot_.get(), n_, -1.0, s_);
+  dev_gemv_update(d_res, ap_.get(), n_, l_, dot_.get(), n_, 1.0, s_);
 }
 
 void RasPreconditioner::apply(const double* d_v, double* d_out) {
@@ -1926,36 +2486,36 @@ void RasPreconditioner::apply(const double* d_v, double* d_out) {
     update_residuals(src, trg, w, r);
     phase("upd" + std::to_string(src) + ">" + std::to_string(trg));
   };
-  dev_copy(d_v, res_.get(), mu_, s_);
+  dev_copy(d_v, res_.get(), n_, s_);
   if (n_levels_ == 1) {
     solve(0, res_.get(), d_out);
     return;
   }
   const int n = n_levels_;
   double* tot = total_.get();
-  dev_fill(0.0, tot, mu_ + l_, s_);
+  dev_fill(0.0, tot, n_ + l_, s_);
   solve_t(0, res_.get(), w_.get());
   update_t(0, n - 1, w_.get(), res_.get());
-  dev_axpy(1.0, w_.get(), tot, mu_ + l_, s_);
+  dev_axpy(1.0, w_.get(), tot, n_ + l_, s_);
   for (int level = 1; level < n - 1; level++) {
     solve_t(level, res_.get(), w_.get());
     update_t(level, n - 1, w_.get(), res_.get());
-    dev_axpy(1.0, w_.get(), tot, mu_ + l_, s_);
+    dev_axpy(1.0, w_.get(), tot, n_ + l_, s_);
     orthogonalize(tot, res_.get());
     solve_t(0, res_.get(), w_.get());
     update_t(0, n - 1, w_.get(), res_.get());
-    dev_axpy(1.0, w_.get(), tot, mu_ + l_, s_);
+    dev_axpy(1.0, w_.get(), tot, n_ + l_, s_);
   }
   for (int level = n - 1; level >= 1; level--) {
     solve_t(level, res_.get(), w_.get());
     update_t(level, level - 1, w_.get(), res_.get());
-    dev_axpy(1.0, w_.get(), tot, mu_ + l_, s_);
+    dev_axpy(1.0, w_.get(), tot, n_ + l_, s_);
     orthogonalize(tot, res_.get());
     solve_t(0, res_.get(), w_.get());
     if (level > 1) update_t(0, level - 1, w_.get(), res_.get());
-    dev_axpy(1.0, w_.get(), tot, mu_ + l_, s_);
+    dev_axpy(1.0, w_.get(), tot, n_ + l_, s_);
   }
-  dev_copy(tot, d_out, mu_ + l_, s_);
+  dev_copy(tot, d_out, n_ + l_, s_);
 }
 
 }  // namespace pam

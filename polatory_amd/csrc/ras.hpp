@@ -11,6 +11,7 @@
 #include <memory>
 #include <vector>
 
+#include "hermite.hpp"
 #include "interpolation.hpp"
 #include "krylov.hpp"
 
@@ -19,6 +20,8 @@ namespace pam {
 struct Domain {
   std::vector<int32_t> idx;     // point indices (polynomial points first after merging)
   std::vector<uint8_t> inner;   // 1 where this domain owns the point's solution
+  std::vector<int32_t> gidx;    // gradient point indices (fits with gradient data)
+  std::vector<uint8_t> ginner;  // 1 where this domain owns the gradient point's solution
 };
 
 // DomainDivider (domain_divider.hpp:30-269), point multiplicity 1 (no gradient data).
@@ -40,7 +43,21 @@ class DomainDivider {
 std::vector<int32_t> choose_coarse_points(const std::vector<double>& pts, const std::vector<int32_t>& point_idcs,
                                           const std::vector<int32_t>& poly_idcs, int64_t n_coarse);
 
+// With gradient points (domain_divider.hpp:30-48, 160-269): every gradient
+// point counts 3 (its three rows) in the sizes that drive the splits, and the
+// points of a domain are sorted along its widest axes together.
+std::vector<Domain> divide_mixed(const std::vector<double>& pts, const std::vector<double>& gpts,
+                                 const std::vector<int32_t>& point_idcs, const std::vector<int32_t>& grad_idcs,
+                                 const std::vector<int32_t>& poly_idcs);
+// choose_coarse_points with gradient points (domain_divider.hpp:54-125):
+// returns the coarse level's point and gradient point indices.
+std::pair<std::vector<int32_t>, std::vector<int32_t>> choose_coarse_points_mixed(
+    const std::vector<double>& pts, const std::vector<double>& gpts, const std::vector<int32_t>& point_idcs,
+    const std::vector<int32_t>& grad_idcs, const std::vector<int32_t>& poly_idcs, int64_t n_coarse);
+
 // Batched subdomain factors of one level (or the coarse grid as one domain).
+// Rows are flat indices into the fit's vectors: a point's row is its index,
+// row mu + 3 k + c is component c of gradient point k.
 struct DomainBatch {
   int n_dom = 0;
   int max_m = 0;
@@ -86,8 +103,14 @@ class RasPreconditioner : public LinearOperator {
  public:
   RasPreconditioner(const Model& model, const std::vector<double>& points, hipStream_t s,
                     const RasOptions& opt);
+  // With gradient points (ras_preconditioner.hpp:56-189): vectors have
+  // mu + 3 sigma + l entries ([values; gradients x y z per point; polynomial]).
+  RasPreconditioner(const Model& model, const std::vector<double>& points, const std::vector<double>& grad_points,
+                    hipStream_t s, const RasOptions& opt);
   void apply(const double* d_v, double* d_out) override;
-  int64_t size() const override { return mu_ + l_; }
+  int64_t size() const override { return n_ + l_; }
+  int64_t n_grad_points() const { return sigma_; }
+  const std::vector<std::vector<int32_t>>& level_grad_points() const { return gpoint_idcs_; }
   int n_levels() const { return n_levels_; }
   const std::vector<std::vector<int32_t>>& level_points() const { return point_idcs_; }
   const std::vector<Domain>& level_domains(int level) const { return h_domains_.at(level); }
@@ -128,22 +151,35 @@ class RasPreconditioner : public LinearOperator {
   // ap = A P with the level-update configuration, on stream s (synchronized).
   void compute_ap(hipStream_t s);
   Evaluator& evaluator(int src, int trg);
+  HermiteEvaluator& hermite_evaluator(int src, int trg);
+  // Level updates and A P of fits with gradient data: the level configuration
+  // when set, else FP64 of order 10.
+  FmmConfig hermite_level_cfg() const;
+  void setup(const std::vector<double>& points, const std::vector<double>& grad_points);
+  // Device copies of a level's points and gradient points.
+  const double* level_points(int level);
+  const double* level_grad_points(int level);
 
   Model model_;
   hipStream_t s_;
   RasOptions opt_;
   int64_t mu_;
+  int64_t sigma_ = 0;  // gradient points
+  int64_t n_ = 0;      // rows: mu + 3 sigma
   int l_;
   int n_levels_ = 1;
-  std::vector<double> h_pts_;
-  DevBuf<double> d_pts_, lag_, p_orth_, ap_, pinv_;
-  std::vector<std::vector<int32_t>> point_idcs_;
-  std::vector<DevBuf<int32_t>> d_point_idcs_;
+  std::vector<double> h_pts_, h_gpts_;
+  DevBuf<double> d_pts_, d_gpts_, lag_, p_orth_, ap_, pinv_;
+  std::vector<std::vector<int32_t>> point_idcs_, gpoint_idcs_;
+  std::vector<DevBuf<int32_t>> d_point_idcs_, d_gpoint_idcs_;
+  std::vector<DevBuf<int32_t>> d_level_rows_;  // per level: its rows (flat indices)
+  std::vector<int64_t> level_rows_;
   std::vector<std::vector<Domain>> h_domains_;
   std::vector<DomainBatch> fine_;  // index = level (1..n-1)
   DomainBatch coarse_;
   std::map<std::pair<int, int>, std::unique_ptr<Evaluator>> evaluators_;
-  std::map<int, DevBuf<double>> level_pts_;
+  std::map<std::pair<int, int>, std::unique_ptr<HermiteEvaluator>> hevaluators_;
+  std::map<int, DevBuf<double>> level_pts_, level_gpts_;
   DevBuf<double> res_, w_, total_, tmp_, sw_, dot_;
   DevBuf<int32_t> flag_;
   Workspace1 ws_;

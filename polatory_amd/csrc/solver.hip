@@ -498,3 +498,259 @@ std::vector<double> Solver::solve(const std::vector<double>& values, double tole
 }
 
 }  // namespace pam
+
+// ---- Fits with gradient data ----
+
+namespace pam {
+
+namespace {
+
+// Up to k sites, evenly strided among those with nonzero data, then the
+// others; sorted (the sample of ResidualEvaluator::set_values).
+std::vector<int32_t> strided_sample(const std::vector<uint8_t>& nonzero, int64_t k) {
+  std::vector<int32_t> nz, zero;
+  for (size_t i = 0; i < nonzero.size(); i++) (nonzero[i] ? nz : zero).push_back(static_cast<int32_t>(i));
+  std::vector<int32_t> out;
+  const int64_t nnz = static_cast<int64_t>(nz.size());
+  if (nnz >= k) {
+    for (int64_t i = 0; i < k; i++) out.push_back(nz[(i * nnz) / k]);
+  } else {
+    out = nz;
+    for (int64_t i = 0; static_cast<int64_t>(out.size()) < k; i++) out.push_back(zero[i]);
+  }
+  std::sort(out.begin(), out.end());
+  return out;
+}
+
+FmmConfig fp64_at_least(const FmmConfig& c, int order) {
+  FmmConfig r = c;
+  if (r.precision != kFp64) {
+    r.precision = kFp64;
+    r.order = order;
+  }
+  r.order = std::max(r.order, order);
+  return r;
+}
+
+}  // namespace
+
+HermiteResidualEvaluator::HermiteResidualEvaluator(const Model& model, const double* d_pts, int64_t mu,
+                                                   const double* d_gpts, int64_t sigma, const FmmConfig& full_cfg,
+                                                   hipStream_t s)
+    : model_(model),
+      s_(s),
+      mu_(mu),
+      sigma_(sigma),
+      d_pts_(d_pts),
+      d_gpts_(d_gpts),
+      full_(model, d_pts, mu, d_gpts, sigma, full_cfg, s) {}
+
+void HermiteResidualEvaluator::set_values(const std::vector<double>& values) {
+  // residual_evaluator.hpp:121-151 with gradient points
+  constexpr int64_t kDirect = 1024;
+  const int64_t n = mu_ + 3 * sigma_;
+  if (static_cast<int64_t>(values.size()) != n) throw_invalid("values.rows() must be mu + 3 sigma");
+  values_.upload(values.data(), values.size(), s_);
+  std::vector<uint8_t> nzv(static_cast<size_t>(mu_)), nzg(static_cast<size_t>(sigma_));
+  for (int64_t i = 0; i < mu_; i++) nzv[i] = values[i] != 0.0;
+  for (int64_t k = 0; k < sigma_; k++)
+    nzg[k] = values[mu_ + 3 * k] != 0.0 || values[mu_ + 3 * k + 1] != 0.0 || values[mu_ + 3 * k + 2] != 0.0;
+  sample_ = strided_sample(nzv, std::min(mu_, kDirect));
+  gsample_ = strided_sample(nzg, std::min(sigma_, kDirect));
+  const int64_t ns = static_cast<int64_t>(sample_.size()), ng = static_cast<int64_t>(gsample_.size());
+  d_sample_.upload(sample_.data(), sample_.size(), s_);
+  d_gsample_.upload(gsample_.data(), gsample_.size(), s_);
+  sample_pts_.resize(static_cast<size_t>(ns) * 3);
+  sample_gpts_.resize(static_cast<size_t>(ng) * 3);
+  if (ns > 0) dev_gather_points(d_pts_, d_sample_.get(), ns, sample_pts_.get(), s_);
+  if (ng > 0) dev_gather_points(d_gpts_, d_gsample_.get(), ng, sample_gpts_.get(), s_);
+  std::vector<double> sv;
+  for (int32_t i : sample_) sv.push_back(values[i]);
+  for (int32_t k : gsample_)
+    for (int c = 0; c < 3; c++) sv.push_back(values[mu_ + 3 * static_cast<int64_t>(k) + c]);
+  sample_vals_.upload(sv.data(), sv.size(), s_);
+  FmmConfig direct;
+  direct.precision = kFp64;
+  direct.mode = kModeDirect;
+  direct_ = std::make_unique<HermiteEvaluator>(model_, direct, s_);
+  direct_->set_sources(d_pts_, mu_, d_gpts_, sigma_);
+  direct_->set_targets(sample_pts_.get(), ns, sample_gpts_.get(), ng);
+  PAM_HIP(hipStreamSynchronize(s_));  // sv is a host temporary
+}
+
+void HermiteResidualEvaluator::full_residual(const double* d_weights, double* d_r) {
+  const int64_t n = mu_ + 3 * sigma_;
+  y_.resize(static_cast<size_t>(full_.size()));
+  full_.apply(d_weights, y_.get());
+  dev_copy(values_.get(), d_r, n, s_);
+  dev_axpy(-1.0, y_.get(), d_r, n, s_);
+}
+
+HermiteResidualEvaluator::Result HermiteResidualEvaluator::converged(const double* d_weights, double tolerance,
+                                                                     double grad_tolerance) {
+  // residual_evaluator.hpp:56-108
+  Result c;
+  const int64_t ns = static_cast<int64_t>(sample_.size()), ng = static_cast<int64_t>(gsample_.size());
+  const int64_t n = mu_ + 3 * sigma_;
+  const bool exact = ns == mu_ && ng == sigma_;
+  auto max_diff = [&](const double* a, const double* b, int64_t k) {
+    return k > 0 ? dev_max_abs_diff(a, b, k, ws_, s_) : 0.0;
+  };
+  {
+    y_.resize(static_cast<size_t>(std::max(ns + 3 * ng, full_.size())));
+    direct_->set_weights(d_weights);
+    direct_->evaluate(y_.get());
+    if (model_.nugget != 0.0 && ns > 0) {
+      tmp_.resize(static_cast<size_t>(std::max(ns, n)));
+      dev_gather(d_weights, d_sample_.get(), ns, tmp_.get(), s_);
+      dev_axpy(model_.nugget, tmp_.get(), y_.get(), ns, s_);
+    }
+    c.residual = max_diff(y_.get(), sample_vals_.get(), ns);
+    c.grad_residual = max_diff(y_.get() + ns, sample_vals_.get() + ns, 3 * ng);
+    c.exact = exact;
+    if (c.residual > tolerance || c.grad_residual > grad_tolerance) return c;
+    if (exact) {
+      c.converged = true;
+      return c;
+    }
+  }
+  tmp_.resize(static_cast<size_t>(std::max(ns, n)));
+  full_residual(d_weights, tmp_.get());
+  c.residual = max_diff(tmp_.get(), nullptr, mu_);
+  c.grad_residual = max_diff(tmp_.get() + mu_, nullptr, 3 * sigma_);
+  c.exact = true;
+  c.converged = c.residual <= tolerance && c.grad_residual <= grad_tolerance;
+  return c;
+}
+
+HermiteSolver::HermiteSolver(const Model& model, const std::vector<double>& points,
+                             const std::vector<double>& grad_points, const FitOptions& opt, hipStream_t s)
+    : model_(model),
+      opt_(opt),
+      s_(s),
+      mu_(static_cast<int64_t>(points.size() / 3)),
+      sigma_(static_cast<int64_t>(grad_points.size() / 3)),
+      n_(mu_ + 3 * sigma_),
+      l_(model.l()) {
+  auto t0 = std::chrono::steady_clock::now();
+  model_.validate();
+  if (!(model_.poly_degree == 1 && mu_ == 1 && sigma_ >= 1) && mu_ < l_)
+    throw_invalid("points.rows() must be greater than or equal to " + std::to_string(l_));
+  resolve_fit_options(model_, opt_);
+  d_pts_.upload(points.data(), points.size(), s_);
+  d_gpts_.upload(grad_points.data(), grad_points.size(), s_);
+  // The gradient rows are less accurate than the values in a mixed-precision
+  // FMM (5.8e-5 relative at order 7): the operator of these fits is FP64.
+  const FmmConfig op_cfg = fp64_at_least(opt_.op_cfg, 10);
+  op_ = std::make_unique<HermiteOperator>(model_, d_pts_.get(), mu_, d_gpts_.get(), sigma_, op_cfg, s_);
+  res_ = std::make_unique<HermiteResidualEvaluator>(model_, d_pts_.get(), mu_, d_gpts_.get(), sigma_,
+                                                    fp64_at_least(opt_.res_cfg, 10), s_);
+  if (opt_.use_preconditioner) {
+    RasOptions ro = fit_ras_options(model_, opt_, n_);
+    ro.fp32_factors = false;
+    if (!ro.use_level_cfg) {
+      ro.level_cfg = op_cfg;
+      ro.use_level_cfg = true;
+    }
+    ro.level_cfg = fp64_at_least(ro.level_cfg, 8);
+    pc_ = std::make_unique<RasPreconditioner>(model_, points, grad_points, s_, ro);
+  }
+  if (l_ > 0) {
+    p_orth_.resize(static_cast<size_t>(n_) * l_);
+    dev_poly_matrix_hermite(model_.poly_degree, d_pts_.get(), mu_, d_gpts_.get(), sigma_, p_orth_.get(), s_);
+    dev_orthonormalize_cols(p_orth_.get(), n_, l_, ws_, s_);
+  }
+  PAM_HIP(hipStreamSynchronize(s_));
+  setup_s_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+std::vector<double> HermiteSolver::solve(const std::vector<double>& values, double tolerance, double grad_tolerance,
+                                         int max_iter, const std::vector<double>* initial, FitStats* stats) {
+  // solver.hpp:76-142 with gradient data
+  auto t0 = std::chrono::steady_clock::now();
+  if (static_cast<int64_t>(values.size()) != n_)
+    throw_invalid("values.rows() must be equal to points.rows() + 3 * grad_points.rows()");
+  if (!(tolerance > 0.0) || !(grad_tolerance > 0.0)) throw_invalid("tolerance must be positive");
+  if (max_iter < 0) throw_invalid("max_iter must be nonnegative");
+  const int64_t m = n_ + l_;
+  DevBuf<double> x(m), xb(m), rhs(m), dot(10);
+  dev_fill(0.0, x.get(), m, s_);
+  if (initial) {
+    if (static_cast<int64_t>(initial->size()) != m) throw_invalid("initial weights have the wrong size");
+    PAM_HIP(hipMemcpyAsync(x.get(), initial->data(), sizeof(double) * m, hipMemcpyHostToDevice, s_));
+    if (l_ > 0) {  // orthogonalize the warm start against P (solver.hpp:85-93)
+      double h[10];
+      dev_multi_dot(p_orth_.get(), n_, l_, x.get(), n_, h, ws_, s_);
+      PAM_HIP(hipMemcpyAsync(dot.get(), h, sizeof(double) * l_, hipMemcpyHostToDevice, s_));
+      dev_gemv_update(x.get(), p_orth_.get(), n_, l_, dot.get(), n_, -1.0, s_);
+    }
+    PAM_HIP(hipStreamSynchronize(s_));
+  }
+  res_->set_values(values);
+  FitStats st;
+  st.setup_seconds = setup_s_;
+  if (pc_) {
+    st.precond_setup_seconds = pc_->setup_seconds();
+    st.n_levels = pc_->n_levels();
+    st.factor_bytes = pc_->factor_bytes();
+  }
+  int total_iter = 0;
+  bool first = true;
+  while (true) {
+    if (first && !initial) {
+      PAM_HIP(hipMemcpyAsync(rhs.get(), values.data(), sizeof(double) * n_, hipMemcpyHostToDevice, s_));
+      dev_fill(0.0, rhs.get() + n_, l_, s_);
+      dev_fill(0.0, xb.get(), m, s_);
+    } else {
+      dev_copy(x.get(), xb.get(), m, s_);
+      res_->full_residual(xb.get(), rhs.get());
+      if (l_ > 0) {  // second block: 0 - P^T [w; g]
+        dev_multi_dot_device(op_->poly_matrix(), n_, l_, xb.get(), n_, rhs.get() + n_, ws_, s_);
+        dev_scal(-1.0, rhs.get() + n_, l_, s_);
+      }
+    }
+    first = false;
+    Fgmres solver(*op_, rhs.get(), m, std::max(max_iter - total_iter, 0), s_);
+    solver.set_right_preconditioner(pc_.get());
+    solver.setup();
+    if (solver.relative_residual() == 0.0) {
+      dev_copy(xb.get(), x.get(), m, s_);
+      break;
+    }
+    bool done = false;
+    while (true) {
+      solver.solution_vector(x.get());
+      dev_axpy(1.0, xb.get(), x.get(), m, s_);
+      HermiteResidualEvaluator::Result c = res_->converged(x.get(), tolerance, grad_tolerance);
+      st.history.push_back(std::max(c.residual, c.grad_residual));
+      st.residual = c.residual;
+      st.grad_residual = c.grad_residual;
+      if (opt_.verbose)
+        std::fprintf(stderr, "%8d %s%12.6e %12.6e  (inner rel %.3e)\n", total_iter + solver.iteration_count(),
+                     c.exact ? " " : "~", c.residual, c.grad_residual, solver.relative_residual());
+      if (c.converged) {
+        done = true;
+        break;
+      }
+      if (total_iter + solver.iteration_count() >= max_iter) {
+        if (!opt_.keep_unconverged) throw_runtime("reached the maximum number of iterations");
+        st.converged = false;
+        done = true;
+        break;
+      }
+      solver.iterate_process();
+    }
+    total_iter += solver.iteration_count();
+    if (done) break;
+    st.restarts++;
+  }
+  st.iterations = total_iter;
+  std::vector<double> out(m);
+  PAM_HIP(hipMemcpyAsync(out.data(), x.get(), sizeof(double) * m, hipMemcpyDeviceToHost, s_));
+  PAM_HIP(hipStreamSynchronize(s_));
+  st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (stats) *stats = st;
+  return out;
+}
+
+}  // namespace pam

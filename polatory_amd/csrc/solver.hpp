@@ -110,6 +110,7 @@ struct FitStats {
   int escalations = 0;  // precision raises after stagnation
   bool converged = true;
   double residual = 0.0;
+  double grad_residual = 0.0;   // fits with gradient data: L-inf residual of the gradient rows
   double seconds = 0.0;
   double setup_seconds = 0.0;
   double precond_setup_seconds = 0.0;
@@ -157,6 +158,60 @@ class Solver {
   // the first residual check (and, being the last member, before the
   // evaluator is destroyed).
   std::future<void> prebuild_;
+};
+
+// Fits with gradient data (interpolant.hpp:77-113, residual_evaluator.hpp:35-151
+// with gradient points): vectors [w (mu); g (3 sigma); c (l)] and data
+// [values (mu); gradients (3 sigma)].
+class HermiteResidualEvaluator {
+ public:
+  HermiteResidualEvaluator(const Model& model, const double* d_pts, int64_t mu, const double* d_gpts,
+                           int64_t sigma, const FmmConfig& full_cfg, hipStream_t s);
+  void set_values(const std::vector<double>& values);
+  struct Result {
+    bool converged = false;
+    double residual = 0.0, grad_residual = 0.0;
+    bool exact = false;
+  };
+  // First <= 1024 sampled sites with nonzero values and <= 1024 gradient
+  // sites with nonzero gradients by the FP64 direct sum, then every row with
+  // the FP64 operator (residual_evaluator.hpp:56-108).
+  Result converged(const double* d_weights, double tolerance, double grad_tolerance);
+  // r = data - (operator w)[0 : mu + 3 sigma] in FP64.
+  void full_residual(const double* d_weights, double* d_r);
+
+ private:
+  Model model_;
+  hipStream_t s_;
+  int64_t mu_, sigma_;
+  const double *d_pts_, *d_gpts_;
+  HermiteOperator full_;
+  std::unique_ptr<HermiteEvaluator> direct_;
+  std::vector<int32_t> sample_, gsample_;
+  DevBuf<int32_t> d_sample_, d_gsample_;
+  DevBuf<double> sample_pts_, sample_gpts_, sample_vals_, values_, tmp_, y_;
+  Workspace1 ws_;
+};
+
+class HermiteSolver {
+ public:
+  HermiteSolver(const Model& model, const std::vector<double>& points, const std::vector<double>& grad_points,
+                const FitOptions& opt, hipStream_t s);
+  std::vector<double> solve(const std::vector<double>& values, double tolerance, double grad_tolerance,
+                            int max_iter, const std::vector<double>* initial, FitStats* stats);
+
+ private:
+  Model model_;
+  FitOptions opt_;
+  hipStream_t s_;
+  int64_t mu_, sigma_, n_;
+  int l_;
+  DevBuf<double> d_pts_, d_gpts_, p_orth_;
+  std::unique_ptr<HermiteOperator> op_;
+  std::unique_ptr<HermiteResidualEvaluator> res_;
+  std::unique_ptr<RasPreconditioner> pc_;
+  Workspace1 ws_;
+  double setup_s_ = 0.0;
 };
 
 }  // namespace pam

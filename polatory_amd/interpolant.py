@@ -168,6 +168,42 @@ class Interpolant:
         _capi.call("pam_interpolant_fit", self._h, _ptr(p), p.shape[0], _ptr(v), float(tolerance),
                    int(max_iter), float(accuracy), opt, None if initial is None else initial._h)
 
+    def fit_hermite(self, points, grad_points, values, tolerance, grad_tolerance, max_iter=100,
+                    accuracy=math.inf, grad_accuracy=math.inf, initial=None, options: FitOptions = None):
+        """fit(points, grad_points, values, tolerance, grad_tolerance, ...)
+        (interpolant.hpp:83-113): values holds the values at points, then the
+        gradients at grad_points (x y z per point, or an (n, 3) array appended
+        after them). The weights are then [w ; g ; c]."""
+        p = _points(points)
+        g = _points(grad_points)
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        if v.shape[0] != p.shape[0] + 3 * g.shape[0]:
+            raise ValueError("values.rows() must be equal to points.rows() + 3 * grad_points.rows()")
+        opt = None if options is None else ctypes.byref(options._c)
+        _capi.call("pam_interpolant_fit_hermite", self._h, _ptr(p), p.shape[0], _ptr(g), g.shape[0], _ptr(v),
+                   float(tolerance), float(grad_tolerance), int(max_iter), float(accuracy), float(grad_accuracy),
+                   opt, None if initial is None else initial._h)
+
+    def evaluate_hermite(self, points, grad_points):
+        """Values at points, then gradients (x y z) at grad_points, of any
+        interpolant (interpolant.hpp:58-67)."""
+        p = _points(points)
+        g = _points(grad_points)
+        out = np.empty(p.shape[0] + 3 * g.shape[0])
+        _capi.call("pam_interpolant_evaluate_hermite", self._h, _ptr(p), p.shape[0], _ptr(g), g.shape[0],
+                   _ptr(out))
+        return out
+
+    def num_grad_centers(self):
+        n = ctypes.c_int64()
+        _capi.call("pam_interpolant_num_grad_centers", self._h, ctypes.byref(n))
+        return n.value
+
+    def grad_centers(self):
+        out = np.empty((self.num_grad_centers(), 3))
+        _capi.call("pam_interpolant_grad_centers", self._h, _ptr(out))
+        return out
+
     def fit_incrementally(self, points, values, tolerance, max_iter=100, accuracy=math.inf,
                           options: FitOptions = None):
         """interpolant.hpp:115-147: fits on a growing subset of the points;
@@ -253,7 +289,7 @@ class Interpolant:
         return out
 
     def weights(self):
-        out = np.empty(self.num_centers() + self.model.poly_basis_size())
+        out = np.empty(self.num_centers() + 3 * self.num_grad_centers() + self.model.poly_basis_size())
         _capi.call("pam_interpolant_weights", self._h, _ptr(out))
         return out
 
@@ -295,14 +331,21 @@ class RbfFieldFunction:
 class RasPreconditioner:
     """Device multilevel RAS preconditioner (exposed for tests)."""
 
-    def __init__(self, model: Model, points, options=None):
+    def __init__(self, model: Model, points, options=None, grad_points=None):
         """options: a FitOptions, for the preconditioner that a fit with those
-        options builds (precision and form of the factors, coarse grid)."""
+        options builds (precision and form of the factors, coarse grid).
+        grad_points: the preconditioner of a fit with gradient data (vectors
+        [w ; g ; c])."""
         self.model = model
         p = _points(points)
         self.n = p.shape[0]
         h = ctypes.c_void_p()
-        if options is None:
+        if grad_points is not None:
+            g = _points(grad_points)
+            opt = None if options is None else ctypes.byref(options._c)
+            _capi.call("pam_ras_create_hermite", model._h, _ptr(p), p.shape[0], _ptr(g), g.shape[0], opt,
+                       ctypes.byref(h))
+        elif options is None:
             _capi.call("pam_ras_create", model._h, _ptr(p), p.shape[0], ctypes.byref(h))
         else:
             _capi.call("pam_ras_create_with_options", model._h, _ptr(p), p.shape[0], ctypes.byref(options._c),
