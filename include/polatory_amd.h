@@ -428,6 +428,26 @@ int pam_ras_create_with_options(const pam_model* model, const double* points, in
 int pam_ras_create_hermite(const pam_model* model, const double* points, int64_t mu, const double* grad_points,
                            int64_t sigma, const pam_fit_options* opt, pam_ras** out);
 int pam_ras_destroy(pam_ras* h);
+/* DomainDivider(points, grad_points, point_indices, grad_point_indices,
+ * poly_point_indices) — domain_divider.hpp:30-48, 160-269 (host only): the
+ * overlapping domains of one RAS level, with every gradient point counting 3
+ * rows. Indices are int32 into points (mu) and grad_points (sigma). */
+typedef struct pam_domain_divider pam_domain_divider;
+int pam_domain_divider_create(const double* points, int64_t mu, const double* grad_points, int64_t sigma,
+                              const int32_t* point_idcs, int64_t n_idx, const int32_t* grad_idcs, int64_t n_gidx,
+                              const int32_t* poly_idcs, int64_t n_poly, pam_domain_divider** out);
+int pam_domain_divider_destroy(pam_domain_divider* h);
+int pam_domain_divider_num_domains(const pam_domain_divider* h, int64_t* n);
+/* Domain d: its point indices (polynomial points first) and inner flags (n),
+ * its gradient point indices and inner flags (ng); any output may be NULL. */
+int pam_domain_divider_domain(const pam_domain_divider* h, int64_t d, int32_t* idx, uint8_t* inner, int64_t* n,
+                              int32_t* gidx, uint8_t* ginner, int64_t* ng);
+/* choose_coarse_points(n_coarse) — domain_divider.hpp:54-125: the coarser
+ * level's points (polynomial points first) and gradient points. With idx and
+ * gidx NULL, *n and *ng receive the counts; otherwise they give the
+ * capacities of idx and gidx and receive the counts. */
+int pam_domain_divider_choose_coarse_points(const pam_domain_divider* h, int64_t n_coarse, int32_t* idx, int64_t* n,
+                                            int32_t* gidx, int64_t* ng);
 int pam_ras_num_levels(const pam_ras* h, int* n_levels);
 /* point indices of a level (level n_levels-1 = all points; 0 = coarse grid) */
 int pam_ras_level_points(const pam_ras* h, int level, int32_t* out, int64_t* n);

@@ -184,6 +184,112 @@ def divide_domains(pts, point_idcs, poly_idcs):
     return domains
 
 
+def _prefix_position(prefix, x):
+    """Largest i with prefix[i] <= x (std::upper_bound - 1)."""
+    return int(np.searchsorted(prefix, x, side="right")) - 1
+
+
+def divide_domains_mixed(pts, gpts, point_idcs, grad_idcs, poly_idcs):
+    """domain_divider.hpp:160-249 with gradient points: a gradient point counts
+    3 rows in the sizes and split positions (prefix sums of the multiplicities);
+    leaves hold at most 1024 rows. Returns [(idx, inner, gidx, ginner)], the
+    points with the polynomial points first, the rest of each list in index
+    order."""
+    mu = len(pts)
+    allp = np.vstack([np.asarray(pts, dtype=float).reshape(-1, 3), np.asarray(gpts, dtype=float).reshape(-1, 3)])
+    rec = np.concatenate([np.asarray(point_idcs, dtype=np.int64), mu + np.asarray(grad_idcs, dtype=np.int64)])
+    work = [(rec, np.ones(len(rec), dtype=bool))]
+    out = []
+    while work:
+        r, inner = work.pop(0)
+        mult = np.where(r >= mu, 3, 1)
+        n_mult = int(mult.sum())
+        if n_mult <= K_MAX_LEAF:
+            out.append((r, inner))
+            continue
+        order = _sort_by_axes(allp, r)
+        r, inner, mult = r[order], inner[order], mult[order]
+        prefix = np.concatenate([[0], np.cumsum(mult)])
+        q = K_OVERLAP_QUOTA * K_MAX_LEAF / n_mult
+        n_sub = int(_round_half_even((1.0 + q) / 2.0 * n_mult))
+        left_mult, right_mult = n_mult - n_sub, n_sub
+        mid_mult = int(_round_half_even((left_mult + right_mult) / 2.0))
+        left, right, mid = (_prefix_position(prefix, x) for x in (left_mult, right_mult, mid_mult))
+        pos = np.arange(len(r))
+        work.append((r[:right], inner[:right] & (pos[:right] < mid)))
+        work.append((r[left:], inner[left:] & (pos[left:] >= mid)))
+    domains = []
+    poly = list(poly_idcs)
+    for r, inner in out:
+        is_g = r >= mu
+        v_idx, v_in = r[~is_g], inner[~is_g]
+        g_idx, g_in = r[is_g] - mu, inner[is_g]
+        o = np.argsort(v_idx, kind="stable")
+        v_idx, v_in = v_idx[o], v_in[o]
+        o = np.argsort(g_idx, kind="stable")
+        g_idx, g_in = g_idx[o], g_in[o]
+        is_poly = np.isin(v_idx, poly)
+        p_inner = np.zeros(len(poly), dtype=bool)
+        for k, pi in enumerate(poly):
+            hit = np.nonzero(v_idx == pi)[0]
+            if len(hit):
+                p_inner[k] = v_in[hit[0]]
+        domains.append((np.concatenate([np.array(poly, dtype=np.int64), v_idx[~is_poly]]),
+                        np.concatenate([p_inner, v_in[~is_poly]]), g_idx, g_in))
+    return domains
+
+
+def choose_coarse_points_mixed(pts, gpts, point_idcs, grad_idcs, poly_idcs, n_coarse):
+    """domain_divider.hpp:54-125 with gradient points: bisect the cluster of
+    the lowest level (ties: the largest bbox volume) at the middle of its rows
+    until the clusters' centres count n_coarse rows (a gradient point counts
+    3); a cluster's centre is its point nearest its bbox centre. Returns the
+    point indices (polynomial points first) and gradient point indices."""
+    mu = len(pts)
+    allp = np.vstack([np.asarray(pts, dtype=float).reshape(-1, 3), np.asarray(gpts, dtype=float).reshape(-1, 3)])
+    poly = set(int(i) for i in poly_idcs)
+    root = np.array([i for i in point_idcs if int(i) not in poly] + [mu + int(k) for k in grad_idcs], dtype=np.int64)
+    mult_of = lambda r: 3 if r >= mu else 1
+
+    def init(rec, level, seq):
+        p = allp[rec]
+        lo, hi = p.min(axis=0), p.max(axis=0)
+        c = 0.5 * (lo + hi)
+        center = int(rec[int(np.argmin(((p - c) ** 2).sum(axis=1)))])
+        vol = float(np.prod(hi - lo))
+        return (level, -vol, seq, rec[_sort_by_axes(allp, rec)], center)
+
+    total = sum(mult_of(int(r)) for r in root)
+    n_coarse = min(n_coarse, total)
+    seq = 0
+    heap = [init(root, 0, seq)]
+    size = mult_of(heap[0][4])
+    while size < n_coarse:
+        level, _, _, rec, center = heapq.heappop(heap)
+        prefix = np.concatenate([[0], np.cumsum([mult_of(int(r)) for r in rec])])
+        cs = prefix[-1]
+        mid, best = 0, None
+        for i in range(len(rec)):
+            d = abs(2 * int(prefix[i]) - int(cs))
+            if best is None or d < best or (d == best and i % 2 == 0):
+                best, mid = d, i
+        size -= mult_of(center)
+        for part in (rec[:mid], rec[mid:]):
+            if len(part):
+                seq += 1
+                cl = init(part, level + 1, seq)
+                size += mult_of(cl[4])
+                heapq.heappush(heap, cl)
+    idx, gidx = list(poly_idcs), []
+    for cl in heap:
+        c = cl[4]
+        if c >= mu:
+            gidx.append(c - mu)
+        else:
+            idx.append(c)
+    return np.array(idx, dtype=np.int64), np.array(gidx, dtype=np.int64)
+
+
 class _Grid:
     """FineGrid / CoarseGrid on one domain (poly points first)."""
 

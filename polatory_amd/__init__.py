@@ -10,7 +10,7 @@ from .fmm import (FmmConfig, FmmEvaluator, FmmGradientEvaluator, FmmGradientTran
                   direct_evaluate_gradient,
                   make_fmm_evaluator, make_fmm_gradient_evaluator, make_fmm_gradient_transpose_evaluator,
                   make_fmm_hessian_evaluator, make_fmm_hessian_symmetric_evaluator, make_fmm_symmetric_evaluator)
-from .interpolant import (FitOptions, Interpolant, Model, Operator, RasPreconditioner, RbfFieldFunction, cross_validate,
+from .interpolant import (DomainDivider, FitOptions, Interpolant, Model, Operator, RasPreconditioner, RbfFieldFunction, cross_validate,
                           distance_filter, fgmres_dense, POLY_DEGREE_MIN)
 from .rbf import Biharmonic3D, CovExponential, Rbf, Triharmonic3D, make_rbf
 from .serve import Server
@@ -41,6 +41,6 @@ __all__ = ["Biharmonic3D", "Triharmonic3D", "CovExponential", "Rbf", "make_rbf",
            "make_fmm_gradient_evaluator", "FmmHessianEvaluator", "make_fmm_hessian_evaluator",
            "make_fmm_evaluator", "make_fmm_symmetric_evaluator", "make_fmm_hessian_symmetric_evaluator", "make_fmm_gradient_transpose_evaluator", "direct_evaluate", "direct_evaluate_gradient",
            "device_count", "device_cache_release", "device_cache_bytes", "Model",
-           "Interpolant", "FitOptions", "Operator", "RasPreconditioner", "fgmres_dense", "POLY_DEGREE_MIN", "distance_filter",
+           "Interpolant", "FitOptions", "DomainDivider", "Operator", "RasPreconditioner", "fgmres_dense", "POLY_DEGREE_MIN", "distance_filter",
            "cross_validate", "RbfFieldFunction",
            "Server"]

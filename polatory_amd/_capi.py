@@ -191,6 +191,11 @@ _SIGS = {
     "pam_interpolant_evaluate_hermite": [_P, _P, _I64, _P, _I64, _P],
     "pam_interpolant_num_grad_centers": [_P, ctypes.POINTER(_I64)],
     "pam_interpolant_grad_centers": [_P, _P],
+    "pam_domain_divider_create": [_P, _I64, _P, _I64, _P, _I64, _P, _I64, _P, _I64, ctypes.POINTER(_P)],
+    "pam_domain_divider_destroy": [_P],
+    "pam_domain_divider_num_domains": [_P, ctypes.POINTER(_I64)],
+    "pam_domain_divider_domain": [_P, _I64, _P, _P, ctypes.POINTER(_I64), _P, _P, ctypes.POINTER(_I64)],
+    "pam_domain_divider_choose_coarse_points": [_P, _I64, _P, ctypes.POINTER(_I64), _P, ctypes.POINTER(_I64)],
     "pam_ras_create_hermite": [_P, _P, _I64, _P, _I64, _P, ctypes.POINTER(_P)],
     "pam_operator_create_hermite": [_P, _P, _I64, _P, _I64, _P, ctypes.POINTER(_P)],
     "pam_operator_destroy": [_P],

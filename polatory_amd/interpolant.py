@@ -500,3 +500,57 @@ def cross_validate(model: Model, points, values, set_ids, tolerance, max_iter=10
     _capi.call("pam_cross_validate", model._h, _ptr(p), p.shape[0], _ptr(v), _ptr(ids), float(tolerance),
                int(max_iter), float(accuracy), opt, _ptr(out))
     return out
+
+
+class DomainDivider:
+    """preconditioner::DomainDivider (domain_divider.hpp:30-125), host only:
+    the overlapping domains of one RAS level and the choice of the coarser
+    level's points; gradient points count 3 rows each."""
+
+    def __init__(self, points, grad_points, point_indices, grad_point_indices, poly_point_indices):
+        p = _points(points)
+        g = _points(grad_points)
+        self._idx = [np.ascontiguousarray(a, dtype=np.int32).reshape(-1)
+                     for a in (point_indices, grad_point_indices, poly_point_indices)]
+        h = ctypes.c_void_p()
+        i, gi, pi = self._idx
+        _capi.call("pam_domain_divider_create", _ptr(p), p.shape[0], _ptr(g), g.shape[0], _ptr(i), i.shape[0],
+                   _ptr(gi), gi.shape[0], _ptr(pi), pi.shape[0], ctypes.byref(h))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _capi.load().pam_domain_divider_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def domains(self):
+        """[(point_indices, inner_point, grad_point_indices, inner_grad_point)] per domain."""
+        nd = ctypes.c_int64()
+        _capi.call("pam_domain_divider_num_domains", self._h, ctypes.byref(nd))
+        out = []
+        for d in range(nd.value):
+            n, ng = ctypes.c_int64(), ctypes.c_int64()
+            _capi.call("pam_domain_divider_domain", self._h, d, None, None, ctypes.byref(n), None, None,
+                       ctypes.byref(ng))
+            idx = np.empty(n.value, dtype=np.int32)
+            inner = np.empty(n.value, dtype=np.uint8)
+            gidx = np.empty(ng.value, dtype=np.int32)
+            ginner = np.empty(ng.value, dtype=np.uint8)
+            _capi.call("pam_domain_divider_domain", self._h, d, _ptr(idx), _ptr(inner), ctypes.byref(n),
+                       _ptr(gidx), _ptr(ginner), ctypes.byref(ng))
+            out.append((idx, inner.astype(bool), gidx, ginner.astype(bool)))
+        return out
+
+    def choose_coarse_points(self, n_coarse):
+        n, ng = ctypes.c_int64(), ctypes.c_int64()
+        _capi.call("pam_domain_divider_choose_coarse_points", self._h, int(n_coarse), None, ctypes.byref(n), None,
+                   ctypes.byref(ng))
+        idx = np.empty(n.value, dtype=np.int32)
+        gidx = np.empty(ng.value, dtype=np.int32)
+        _capi.call("pam_domain_divider_choose_coarse_points", self._h, int(n_coarse), _ptr(idx), ctypes.byref(n),
+                   _ptr(gidx), ctypes.byref(ng))
+        return idx, gidx
