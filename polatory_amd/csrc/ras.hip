@@ -1276,10 +1276,58 @@ std::vector<int32_t> order_for_split(const int32_t* ids, int64_t n, const std::v
     if (a.k[2] != b.k[2]) return a.k[2] < b.k[2];
     return a.pos < b.pos;
   };
+  std::vector<int32_t> order(static_cast<size_t>(n));
+  if (parallel && n >= (int64_t{1} << 17)) {
+    // Parallel selection: pivots from an evenly strided sample bracket the
+    // ranks [lo, hi) with a margin; one pass sorts the keys into the three
+    // ranges below, between and above the pivots (fixed chunks, so the
+    // result does not depend on the thread count); the middle range then
+    // holds ranks lo..hi and is finished sequentially. If the pivots miss
+    // (the sample is off by more than the margin), the sequential selection
+    // below runs instead.
+    constexpr int64_t kSample = 8192, kMargin = 192, kChunk = int64_t{1} << 15;
+    std::vector<Key> sample(static_cast<size_t>(kSample));
+    for (int64_t i = 0; i < kSample; i++) sample[i] = keys[(i * n) / kSample];
+    std::sort(sample.begin(), sample.end(), less);
+    const int64_t s_lo = (lo * kSample) / n - kMargin, s_hi = (hi * kSample + n - 1) / n + kMargin;
+    if (s_lo >= 0 && s_hi < kSample) {
+      const Key p_lo = sample[s_lo], p_hi = sample[s_hi];
+      const int64_t n_chunks = (n + kChunk - 1) / kChunk;
+      std::vector<int64_t> cnt(static_cast<size_t>(3 * n_chunks), 0);
+      auto cls = [&](const Key& k) { return less(k, p_lo) ? 0 : (less(p_hi, k) ? 2 : 1); };
+#pragma omp parallel for schedule(static)
+      for (int64_t c = 0; c < n_chunks; c++)
+        for (int64_t i = c * kChunk; i < std::min(n, (c + 1) * kChunk); i++) cnt[3 * c + cls(keys[i])]++;
+      int64_t tot[3] = {0, 0, 0};
+      for (int64_t c = 0; c < n_chunks; c++)
+        for (int b = 0; b < 3; b++) tot[b] += cnt[3 * c + b];
+      if (tot[0] <= lo && tot[0] + tot[1] >= hi) {
+        std::vector<int64_t> off(static_cast<size_t>(3 * n_chunks));
+        int64_t base[3] = {0, tot[0], tot[0] + tot[1]};
+        for (int64_t c = 0; c < n_chunks; c++)
+          for (int b = 0; b < 3; b++) {
+            off[3 * c + b] = base[b];
+            base[b] += cnt[3 * c + b];
+          }
+        std::vector<Key> out(static_cast<size_t>(n));
+#pragma omp parallel for schedule(static)
+        for (int64_t c = 0; c < n_chunks; c++) {
+          int64_t o[3] = {off[3 * c], off[3 * c + 1], off[3 * c + 2]};
+          for (int64_t i = c * kChunk; i < std::min(n, (c + 1) * kChunk); i++) out[o[cls(keys[i])]++] = keys[i];
+        }
+        auto mid_b = out.begin() + tot[0], mid_e = mid_b + tot[1];
+        std::nth_element(mid_b, out.begin() + lo, mid_e, less);
+        std::nth_element(out.begin() + lo, out.begin() + hi, mid_e, less);
+        std::sort(out.begin() + lo, out.begin() + hi, less);
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; i++) order[i] = out[i].pos;
+        return order;
+      }
+    }
+  }
   std::nth_element(keys.begin(), keys.begin() + lo, keys.end(), less);
   std::nth_element(keys.begin() + lo, keys.begin() + hi, keys.end(), less);
   std::sort(keys.begin() + lo, keys.begin() + hi, less);
-  std::vector<int32_t> order(static_cast<size_t>(n));
   for (int64_t i = 0; i < n; i++) order[i] = keys[i].pos;
   return order;
 }
@@ -1299,8 +1347,24 @@ DomainDivider::DomainDivider(const std::vector<double>& pts, const std::vector<i
   // ones use the parallel sort, many small ones are split in parallel.
   auto divide = [&](const Domain& d, Domain& left, Domain& right, bool parallel_sort) {
     int64_t n = static_cast<int64_t>(d.idx.size());
+    const bool par = parallel_sort && n >= (int64_t{1} << 17);
     Bbox3 bb;
-    for (int32_t i : d.idx) bb.add(&pts_[3 * static_cast<size_t>(i)]);
+    if (par) {
+      double lo0 = INFINITY, lo1 = INFINITY, lo2 = INFINITY, hi0 = -INFINITY, hi1 = -INFINITY, hi2 = -INFINITY;
+#pragma omp parallel for schedule(static) reduction(min : lo0, lo1, lo2) reduction(max : hi0, hi1, hi2)
+      for (int64_t k = 0; k < n; k++) {
+        const double* p = &pts_[3 * static_cast<size_t>(d.idx[k])];
+        lo0 = std::min(lo0, p[0]);
+        lo1 = std::min(lo1, p[1]);
+        lo2 = std::min(lo2, p[2]);
+        hi0 = std::max(hi0, p[0]);
+        hi1 = std::max(hi1, p[1]);
+        hi2 = std::max(hi2, p[2]);
+      }
+      bb.lo[0] = lo0, bb.lo[1] = lo1, bb.lo[2] = lo2, bb.hi[0] = hi0, bb.hi[1] = hi1, bb.hi[2] = hi2;
+    } else {
+      for (int32_t i : d.idx) bb.add(&pts_[3 * static_cast<size_t>(i)]);
+    }
     double q = kOverlapQuota * static_cast<double>(kMaxLeafSize) / static_cast<double>(n);
     int64_t n_sub = static_cast<int64_t>(round_half_to_even((1.0 + q) / 2.0 * static_cast<double>(n)));
     int64_t left_part = n - n_sub, right_part = n_sub;
@@ -1309,6 +1373,7 @@ DomainDivider::DomainDivider(const std::vector<double>& pts, const std::vector<i
     int64_t mid = static_cast<int64_t>(round_half_to_even(static_cast<double>(left_part + right_part) / 2.0));
     left.idx.resize(right_part);
     left.inner.resize(right_part);
+#pragma omp parallel for schedule(static) if (par)
     for (int64_t i = 0; i < right_part; i++) {
       int32_t o = order[i];
       left.idx[i] = d.idx[o];
@@ -1316,6 +1381,7 @@ DomainDivider::DomainDivider(const std::vector<double>& pts, const std::vector<i
     }
     right.idx.resize(n - left_part);
     right.inner.resize(n - left_part);
+#pragma omp parallel for schedule(static) if (par)
     for (int64_t i = left_part; i < n; i++) {
       int32_t o = order[i];
       right.idx[i - left_part] = d.idx[o];
@@ -1336,8 +1402,13 @@ DomainDivider::DomainDivider(const std::vector<double>& pts, const std::vector<i
     for (int64_t i = 0; i < nw; i++) split[i] = static_cast<int64_t>(work[i].idx.size()) > kMaxLeafSize;
     if (nw == 1) {
       if (split[0]) divide(work[0], next[0], next[1], true);
+    } else if (nw < 8 && static_cast<int64_t>(work[0].idx.size()) >= (int64_t{1} << 18)) {
+      // the few large domains of the first levels, one after another, each
+      // with every thread
+      for (int64_t i = 0; i < nw; i++)
+        if (split[i]) divide(work[i], next[2 * i], next[2 * i + 1], true);
     } else if (nw < 8) {
-      // the few large domains of the first levels, one thread each
+      // a few domains, one thread each
 #pragma omp parallel for schedule(static, 1) num_threads(static_cast<int>(nw))
       for (int64_t i = 0; i < nw; i++)
         if (split[i]) divide(work[i], next[2 * i], next[2 * i + 1], false);

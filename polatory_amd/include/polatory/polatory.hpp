@@ -294,6 +294,29 @@ class Interpolant {
                                       max_iter, accuracy, nullptr, initial ? initial->h_ : nullptr));
   }
 
+  // interpolant.hpp:83-113: with gradient data, values = [values at points;
+  // gradients at grad_points, x y z each]; the weights become [w; g; c].
+  void fit(const Points& points, const Points& grad_points, const VecX& values, double tolerance,
+           double grad_tolerance, int max_iter = 100, double accuracy = kInfinity,
+           double grad_accuracy = kInfinity, const Interpolant* initial = nullptr) {
+    if (values.rows() != points.rows() + 3 * grad_points.rows())
+      throw std::invalid_argument("values.rows() must be equal to " +
+                                  std::to_string(points.rows() + 3 * grad_points.rows()));
+    detail::check(pam_interpolant_fit_hermite(h_, points.data(), points.rows(), grad_points.data(),
+                                              grad_points.rows(), values.data(), tolerance, grad_tolerance,
+                                              max_iter, accuracy, grad_accuracy, nullptr,
+                                              initial ? initial->h_ : nullptr));
+  }
+
+  // interpolant.hpp:48
+  Points grad_centers() const {
+    int64_t n = 0;
+    detail::check(pam_interpolant_num_grad_centers(h_, &n));
+    Points out(n, Dim);
+    detail::check(pam_interpolant_grad_centers(h_, out.data()));
+    return out;
+  }
+
   // interpolant.hpp:54-56
   VecX evaluate(const Points& points, double accuracy = kInfinity) {
     VecX out(points.rows());
@@ -305,6 +328,13 @@ class Interpolant {
   VecX evaluate(const Points& points, const Points& grad_points, double accuracy = kInfinity,
                 double grad_accuracy = kInfinity) {
     VecX out(points.rows() + 3 * grad_points.rows());
+    int64_t n_grad_centers = 0;
+    detail::check(pam_interpolant_num_grad_centers(h_, &n_grad_centers));
+    if (n_grad_centers > 0) {
+      detail::check(pam_interpolant_evaluate_hermite(h_, points.data(), points.rows(), grad_points.data(),
+                                                     grad_points.rows(), out.data()));
+      return out;
+    }
     detail::check(pam_interpolant_evaluate(h_, points.data(), points.rows(), accuracy, out.data()));
     detail::check(pam_interpolant_evaluate_gradient(h_, grad_points.data(), grad_points.rows(), grad_accuracy,
                                                     out.data() + points.rows()));
@@ -356,7 +386,9 @@ class Interpolant {
   VecX weights() const {
     Index n = 0;
     detail::check(pam_interpolant_num_centers(h_, &n));
-    VecX w(n + model_.poly_basis_size());
+    int64_t n_grad = 0;
+    detail::check(pam_interpolant_num_grad_centers(h_, &n_grad));
+    VecX w(n + 3 * n_grad + model_.poly_basis_size());
     detail::check(pam_interpolant_weights(h_, w.data()));
     return w;
   }

@@ -6,6 +6,7 @@
 
 #include <unistd.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -19,6 +20,7 @@ using polatory::VecX;
 using polatory::geometry::Bbox3;
 using polatory::geometry::Points3;
 using polatory::rbf::Biharmonic3D;
+using polatory::rbf::Triharmonic3D;
 
 static double value_at(double x, double y, double z) { return std::sin(3 * x) * std::cos(2 * y) + 0.5 * z * z; }
 
@@ -195,6 +197,33 @@ int main() {
     }
     std::printf("symmetric vs source-target: values %.3e, Hessians %.3e (relative)\n", dv / sv, dh / sh);
     if (!(dh == 0.0) || !(dv <= 1e-5 * sv)) return 14;
+  }
+
+  // fit with gradient data (interpolant.hpp:83-113): f = x + 2 y - z + x y
+  {
+    const Index mu = 2000, sigma = 1000;
+    Points3 p(mu, 3), g(sigma, 3);
+    VecX rhs(mu + 3 * sigma);
+    for (Index i = 0; i < mu; i++) {
+      for (int k = 0; k < 3; k++) p(i, k) = u(gen);
+      rhs(i) = p(i, 0) + 2.0 * p(i, 1) - p(i, 2) + p(i, 0) * p(i, 1);
+    }
+    for (Index i = 0; i < sigma; i++) {
+      for (int k = 0; k < 3; k++) g(i, k) = u(gen);
+      rhs(mu + 3 * i) = 1.0 + g(i, 1);
+      rhs(mu + 3 * i + 1) = 2.0 + g(i, 0);
+      rhs(mu + 3 * i + 2) = -1.0;
+    }
+    Model<3> hmodel(Triharmonic3D<3>(), 1);
+    Interpolant<3> hip(hmodel);
+    hip.fit(p, g, rhs, 1e-4, 1e-4);
+    if (hip.weights().rows() != mu + 3 * sigma + 4) return 15;
+    if (hip.grad_centers().rows() != sigma) return 16;
+    VecX f = hip.evaluate(p, g);
+    double e = 0.0;
+    for (Index i = 0; i < mu + 3 * sigma; i++) e = std::max(e, std::abs(f(i) - rhs(i)));
+    std::printf("fit with gradient data: max |f - data| %.3e\n", e);
+    if (!(e <= 1e-4 + 1e-6)) return 17;
   }
 
   // error contract: invalid arguments throw std::invalid_argument

@@ -43,7 +43,7 @@ def test_reference_invariants(n_grad):
     assert np.array_equal(cidx[:n_poly], poly)
 
 
-@pytest.mark.parametrize("n_pts,n_grad", [(3000, 2000), (1, 2500), (6000, 0)])
+@pytest.mark.parametrize("n_pts,n_grad", [(3000, 2000), (1, 2500), (6000, 0), (300000, 0)])
 def test_matches_oracle_restatement(n_pts, n_grad):
     rng = np.random.default_rng(5 + n_pts)
     pts = rng.uniform(-1, 1, (n_pts, 3)) * np.array([1.0, 0.7, 0.4])
