@@ -286,6 +286,36 @@ def gmm_sites(n, seed_sites=9, seed_field=10, seed_noise=11):
     return pts, vals
 
 
+HB_ANISO = np.array([[1.1, 0.2, 0.0], [0.0, 0.9, 0.1], [0.1, 0.0, 1.2]])
+
+
+def hermite_section(pa):
+    """Fits with gradient data: the reference's fitter case (test_fitter.cpp:24-73:
+    triharmonic3d with anisotropy, degree 1, nugget 0.01, tolerance 1e-3 on the
+    values and on the gradients, f = sum_j sin(pi (A p)_j) on [-1, 1]^3) at
+    10,000 + 10,000 points and at 100,000 + 50,000."""
+    out = {"workload": "th3 + anisotropy, degree 1, nugget 0.01, values and gradients of "
+                       "sum_j sin(pi (A p)_j), tolerance 1e-3 on both"}
+    for n, ng in ((10_000, 10_000), (100_000, 50_000)):
+        rng = np.random.default_rng(n + ng)
+        p = rng.uniform(-1, 1, (n, 3))
+        g = rng.uniform(-1, 1, (ng, 3))
+        data = np.concatenate([np.sin(np.pi * p @ HB_ANISO.T).sum(axis=1),
+                               ((np.pi * np.cos(np.pi * g @ HB_ANISO.T)) @ HB_ANISO).ravel()])
+        r = pa.Triharmonic3D([1.0])
+        r.set_anisotropy(HB_ANISO)
+        ip = pa.Interpolant(pa.Model(r, poly_degree=1, nugget=0.01))
+        t0 = time.perf_counter()
+        ip.fit_hermite(p, g, data, 1e-3, 1e-3, max_iter=100)
+        wall = time.perf_counter() - t0
+        st = ip.fit_stats()
+        out[f"{n}+{ng}"] = {"wall_s": wall, "iterations": st["iterations"], "ras_levels": st["n_levels"],
+                            "residual": st["residual"], "grad_residual": st["grad_residual"]}
+        log("hermite", n, ng, json.dumps(out[f"{n}+{ng}"]))
+        del ip
+    return out
+
+
 def kriging_section(pa):
     """Anisotropic kriging: exponential covariance (sill 1, range 2), anisotropy
     A = diag(1, 1, 1/0.3) (ranges 2, 2, 0.6), nugget 0.0025, degree 0, fit to
@@ -477,6 +507,7 @@ def main():
     ap.add_argument("--no-kriging", action="store_true", help="skip the kriging fit and prediction")
     ap.add_argument("--no-gradient", action="store_true", help="skip the gradient evaluation")
     ap.add_argument("--no-config1", action="store_true", help="skip the 10k-site fit and evaluate")
+    ap.add_argument("--no-hermite", action="store_true", help="skip the fits with gradient data")
     ap.add_argument("--serve-models", type=int, default=32)
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--serve-rate", type=float, default=200.0)
@@ -589,6 +620,7 @@ def main():
     fit = None if args.no_fit else fit_section(pa)
     config1 = None if args.no_config1 else config1_section(pa, args.no_cpu)
     kriging = None if args.no_kriging else kriging_section(pa)
+    hermite = None if args.no_hermite else hermite_section(pa)
     serve = None if args.no_serve else serve_section(pa, args.serve_models, args.serve_requests,
                                                      args.serve_rate)
 
@@ -622,6 +654,7 @@ def main():
         "fit": fit,
         "config1": config1,
         "kriging": kriging,
+        "hermite": hermite,
         "gradient": gradient,
         "serve": serve,
     }

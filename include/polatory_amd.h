@@ -385,7 +385,7 @@ int pam_interpolant_set_fitted(pam_interpolant* h, const double* centers, int64_
                                const double* weights);
 /* Interpolant::save / Interpolant::load (interpolant.hpp:218; field order
  * interpolant.hpp:345-367): model, fitted, centres, gradient centres, bbox,
- * weights. Files with gradient centres return PAM_ERR_NOT_IMPLEMENTED. */
+ * weights (the weights [w; g; c] when there are gradient centres). */
 int pam_interpolant_save(const pam_interpolant* h, const char* filename);
 int pam_interpolant_load(const char* filename, pam_interpolant** out);
 /* model() (interpolant.hpp:199) as a new handle the caller destroys */

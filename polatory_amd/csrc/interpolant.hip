@@ -291,6 +291,26 @@ void Interpolant::set_fitted(const std::vector<double>& centers, const std::vect
     }
 }
 
+void Interpolant::set_fitted(const std::vector<double>& centers, const std::vector<double>& weights,
+                             const std::vector<double>& grad_centers) {
+  if (grad_centers.empty()) {
+    set_fitted(centers, weights);
+    return;
+  }
+  const int64_t mu = static_cast<int64_t>(centers.size() / 3), sigma = static_cast<int64_t>(grad_centers.size() / 3);
+  if (static_cast<int64_t>(weights.size()) != mu + 3 * sigma + model_.l())
+    throw_invalid("weights.rows() must be centers.rows() + 3 grad_centers.rows() + l");
+  centers_ = centers;
+  grad_centers_ = grad_centers;
+  weights_ = weights;
+  fitted_ = true;
+  have_eval_bbox_ = false;
+  ev_.reset();
+  gev_.reset();
+  hev_.reset();
+  set_bbox();
+}
+
 void Interpolant::set_eval_config(const FmmConfig* c) {
   have_eval_cfg_ = c != nullptr;
   if (c) eval_cfg_ = *c;

@@ -54,6 +54,9 @@ class Interpolant {
   void evaluate_impl(const double* points, int64_t n, double* out, bool on_device);
   // Installs centres and weights directly (a saved or externally fitted model).
   void set_fitted(const std::vector<double>& centers, const std::vector<double>& weights);
+  // The same with gradient centres: weights = [w; g; c].
+  void set_fitted(const std::vector<double>& centers, const std::vector<double>& weights,
+                  const std::vector<double>& grad_centers);
   void set_eval_config(const FmmConfig* c);
 
   bool fitted() const { return fitted_; }

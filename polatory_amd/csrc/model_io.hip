@@ -157,24 +157,25 @@ Model load_model(const std::string& filename) {
 }
 
 void save_interpolant(const std::string& filename, const Interpolant& ip) {
-  if (!ip.grad_centers().empty())
-    throw Error(kNotImplemented, "saving interpolants with gradient centres is not implemented");
   with_file(filename, std::ios::out | std::ios::trunc, [&](std::ostream& os) {
     write_model(os, ip.model());
     const bool fitted = ip.fitted();
     put<bool>(os, fitted);
     const std::vector<double>& c = ip.centers();
+    const std::vector<double>& g = ip.grad_centers();
     put_points(os, fitted ? c : std::vector<double>{});
-    put_points(os, {});  // grad_centers
-    // bbox_ = Bbox::from_points(centers) (interpolant.hpp:112, 190); the
-    // default Bbox (+inf, -inf) when not fitted (geometry/bbox3d.hpp:18-20).
+    put_points(os, fitted ? g : std::vector<double>{});
+    // bbox_ = the box of the centres and the gradient centres
+    // (interpolant.hpp:112, 190); the default Bbox (+inf, -inf) when not
+    // fitted (geometry/bbox3d.hpp:18-20).
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     if (fitted) {
-      for (size_t i = 0; i < c.size() / 3; i++)
-        for (int k = 0; k < 3; k++) {
-          lo[k] = std::min(lo[k], c[3 * i + k]);
-          hi[k] = std::max(hi[k], c[3 * i + k]);
-        }
+      for (const std::vector<double>* v : {&c, &g})
+        for (size_t i = 0; i < v->size() / 3; i++)
+          for (int k = 0; k < 3; k++) {
+            lo[k] = std::min(lo[k], (*v)[3 * i + k]);
+            hi[k] = std::max(hi[k], (*v)[3 * i + k]);
+          }
     }
     for (double v : lo) put<double>(os, v);
     for (double v : hi) put<double>(os, v);
@@ -198,10 +199,8 @@ std::unique_ptr<Interpolant> load_interpolant(const std::string& filename) {
     if (nw < 0 || static_cast<uint64_t>(nw) > remaining(is) / 8) throw_runtime("corrupt weight count");
     std::vector<double> w(static_cast<size_t>(nw));
     if (nw > 0) get_bytes(is, w.data(), sizeof(double) * w.size());
-    if (!grad.empty())
-      throw Error(kNotImplemented, "interpolants with gradient data are not supported by this library");
     ip = std::make_unique<Interpolant>(m);
-    if (fitted) ip->set_fitted(centers, w);
+    if (fitted) ip->set_fitted(centers, w, grad);
   });
   return ip;
 }

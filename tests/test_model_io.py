@@ -157,8 +157,20 @@ def test_interpolant_file_round_trip(tmp_path):
                                              np.zeros((0, 3)), np.zeros((0, 3)), [np.inf] * 3,
                                              [-np.inf] * 3, [])
     assert not pa.Interpolant.load(f).fitted()
-    # gradient data is outside this library
-    f.write_bytes(enc_interpolant(enc_model([("bh3", [1.0, 0.0], np.eye(3))], 1, 0.0), True, pts[:4],
-                                  pts[4:6], [0] * 3, [1] * 3, np.zeros(4 + 6 + 4)))
-    with pytest.raises(NotImplementedError):
-        pa.Interpolant.load(f)
+    # with gradient centres (interpolant.hpp:340-367): grad_centers_ after the
+    # centres, the box of both, weights [w; g; c]
+    gp = rng.uniform(0, 1, (300, 3))
+    # f = 4 x + y z: values at 500 points, gradients (4, z, y) at 300 more
+    gvals = np.concatenate([4 * pts[:500, 0] + pts[:500, 1] * pts[:500, 2],
+                            np.stack([np.full(300, 4.0), gp[:, 2], gp[:, 1]], axis=1).ravel()])
+    hip_ = pa.Interpolant(pa.Model(pa.Triharmonic3D([1.0]), poly_degree=1))
+    hip_.fit_hermite(pts[:500], gp, gvals, 1e-5, 1e-5)
+    hip_.save(f)
+    hw = hip_.weights()
+    assert hw.shape[0] == 500 + 900 + 4
+    allc = np.vstack([pts[:500], gp])
+    assert f.read_bytes() == enc_interpolant(enc_model([("th3", [1.0, 0.0], np.eye(3))], 1, 0.0), True, pts[:500],
+                                             gp, allc.min(axis=0), allc.max(axis=0), hw)
+    h2 = pa.Interpolant.load(f)
+    assert np.array_equal(h2.grad_centers(), gp) and np.array_equal(h2.weights(), hw)
+    assert np.array_equal(h2.evaluate_hermite(q, q[:50]), hip_.evaluate_hermite(q, q[:50]))
