@@ -329,6 +329,15 @@ int pam_interpolant_grad_centers(const pam_interpolant* h, double* out);
 int pam_interpolant_fit_incrementally(pam_interpolant* h, const double* points, int64_t n,
                                       const double* values, double tolerance, int max_iter,
                                       double accuracy, const pam_fit_options* opt);
+/* fit_incrementally(points, grad_points, values, tolerance, grad_tolerance,
+ * max_iter, accuracy, grad_accuracy) — interpolant.hpp:121-147,
+ * incremental_fitter.hpp:36-183 with gradient data: values as in
+ * pam_interpolant_fit_hermite; centers() and grad_centers() then hold the
+ * points and gradient points chosen, in the order they were added. */
+int pam_interpolant_fit_incrementally_hermite(pam_interpolant* h, const double* points, int64_t mu,
+                                              const double* grad_points, int64_t sigma, const double* values,
+                                              double tolerance, double grad_tolerance, int max_iter,
+                                              double accuracy, double grad_accuracy, const pam_fit_options* opt);
 /* fit_inequality(points, values, values_lb, values_ub, tolerance, max_iter = 100,
  * accuracy = inf, initial = nullptr) — interpolant.hpp:149-190,
  * interpolation/inequality_fitter.hpp:25-187. NaN in values / values_lb / values_ub

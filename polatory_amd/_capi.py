@@ -188,6 +188,7 @@ _SIGS = {
     "pam_ras_synchronize": [_P],
     "pam_operator_create": [_P, _P, _I64, ctypes.POINTER(PamFmmConfig), ctypes.POINTER(_P)],
     "pam_interpolant_fit_hermite": [_P, _P, _I64, _P, _I64, _P, _D, _D, _INT, _D, _D, _P, _P],
+    "pam_interpolant_fit_incrementally_hermite": [_P, _P, _I64, _P, _I64, _P, _D, _D, _INT, _D, _D, _P],
     "pam_interpolant_evaluate_hermite": [_P, _P, _I64, _P, _I64, _P],
     "pam_interpolant_num_grad_centers": [_P, ctypes.POINTER(_I64)],
     "pam_interpolant_grad_centers": [_P, _P],

@@ -253,6 +253,21 @@ int pam_interpolant_fit_hermite(pam_interpolant* h, const double* points, int64_
   });
 }
 
+int pam_interpolant_fit_incrementally_hermite(pam_interpolant* h, const double* points, int64_t mu,
+                                              const double* grad_points, int64_t sigma, const double* values,
+                                              double tolerance, double grad_tolerance, int max_iter,
+                                              double accuracy, double grad_accuracy, const pam_fit_options* opt) {
+  return guard([&] {
+    if (mu < 0 || sigma < 0) pam::throw_invalid("negative point count");
+    if ((mu > 0 && !points) || (sigma > 0 && !grad_points) || (mu + sigma > 0 && !values))
+      pam::throw_invalid("null argument");
+    if (!(accuracy > 0.0) || !(grad_accuracy > 0.0)) pam::throw_invalid("accuracy must be positive");
+    std::vector<double> p(points, points + 3 * mu), g(grad_points, grad_points + 3 * sigma),
+        v(values, values + mu + 3 * sigma);
+    ip(h).fit_incrementally_hermite(p, g, v, tolerance, grad_tolerance, max_iter, to_opts(opt));
+  });
+}
+
 int pam_interpolant_evaluate_hermite(pam_interpolant* h, const double* points, int64_t n,
                                      const double* grad_points, int64_t ng, double* out) {
   return guard([&] {

@@ -40,6 +40,21 @@ IncrementalResult fit_incrementally(const Model& model, const std::vector<double
                                     const std::vector<double>& values, double tolerance, int max_iter,
                                     double accuracy, const FitOptions& opt, hipStream_t s);
 
+struct IncrementalHermiteResult {
+  std::vector<int64_t> centers, grad_centers;  // in the order they were added
+  std::vector<double> weights;                 // [w; g; c]
+  int rounds = 0;
+  int iterations = 0;
+};
+
+// fit_incrementally with gradient points (incremental_fitter.hpp:36-183):
+// values has n + 3 ng entries (values, then gradients x y z).
+IncrementalHermiteResult fit_incrementally_hermite(const Model& model, const std::vector<double>& points,
+                                                   const std::vector<double>& grad_points,
+                                                   const std::vector<double>& values, double tolerance,
+                                                   double grad_tolerance, int max_iter, const FitOptions& opt,
+                                                   hipStream_t s);
+
 struct InequalityResult {
   std::vector<int64_t> centers;  // equality points, then the active bounds in index order
   std::vector<double> weights;   // centres' weights then the polynomial coefficients

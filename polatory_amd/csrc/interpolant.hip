@@ -249,6 +249,26 @@ void Interpolant::fit_incrementally(const std::vector<double>& points, const std
   stats_.rounds = r.rounds;
 }
 
+void Interpolant::fit_incrementally_hermite(const std::vector<double>& points, const std::vector<double>& grad_points,
+                                            const std::vector<double>& values, double tolerance,
+                                            double grad_tolerance, int max_iter, const FitOptions& opt) {
+  fitted_ = false;
+  ev_.reset();
+  gev_.reset();
+  hev_.reset();
+  IncrementalHermiteResult r =
+      pam::fit_incrementally_hermite(model_, points, grad_points, values, tolerance, grad_tolerance, max_iter, opt, s_);
+  std::vector<double> c(r.centers.size() * 3), g(r.grad_centers.size() * 3);
+  for (size_t i = 0; i < r.centers.size(); i++)
+    for (int k = 0; k < 3; k++) c[3 * i + k] = points[3 * r.centers[i] + k];
+  for (size_t i = 0; i < r.grad_centers.size(); i++)
+    for (int k = 0; k < 3; k++) g[3 * i + k] = grad_points[3 * r.grad_centers[i] + k];
+  set_fitted(c, r.weights, g);
+  stats_ = FitStats{};
+  stats_.iterations = r.iterations;
+  stats_.rounds = r.rounds;
+}
+
 void Interpolant::fit_inequality(const std::vector<double>& points, const std::vector<double>& values,
                                  const std::vector<double>& lb, const std::vector<double>& ub, double tolerance,
                                  int max_iter, double accuracy, const FitOptions& opt, const Interpolant* initial) {

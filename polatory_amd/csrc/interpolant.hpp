@@ -40,6 +40,10 @@ class Interpolant {
   // centres are the points chosen, in the order they were added.
   void fit_incrementally(const std::vector<double>& points, const std::vector<double>& values, double tolerance,
                          int max_iter, double accuracy, const FitOptions& opt);
+  // interpolant.hpp:121-147 with gradient data.
+  void fit_incrementally_hermite(const std::vector<double>& points, const std::vector<double>& grad_points,
+                                 const std::vector<double>& values, double tolerance, double grad_tolerance,
+                                 int max_iter, const FitOptions& opt);
   // interpolant.hpp:149-190: values where not NaN, bounds lb/ub where not NaN;
   // the centres are the equality points, then the active bounds.
   void fit_inequality(const std::vector<double>& points, const std::vector<double>& values,

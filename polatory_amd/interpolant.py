@@ -184,6 +184,19 @@ class Interpolant:
                    float(tolerance), float(grad_tolerance), int(max_iter), float(accuracy), float(grad_accuracy),
                    opt, None if initial is None else initial._h)
 
+    def fit_incrementally_hermite(self, points, grad_points, values, tolerance, grad_tolerance, max_iter=100,
+                                  accuracy=math.inf, grad_accuracy=math.inf, options: FitOptions = None):
+        """fit_incrementally with gradient data (interpolant.hpp:121-147)."""
+        p = _points(points)
+        g = _points(grad_points)
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        if v.shape[0] != p.shape[0] + 3 * g.shape[0]:
+            raise ValueError("values.rows() must be equal to points.rows() + 3 * grad_points.rows()")
+        opt = None if options is None else ctypes.byref(options._c)
+        _capi.call("pam_interpolant_fit_incrementally_hermite", self._h, _ptr(p), p.shape[0], _ptr(g), g.shape[0],
+                   _ptr(v), float(tolerance), float(grad_tolerance), int(max_iter), float(accuracy),
+                   float(grad_accuracy), opt)
+
     def evaluate_hermite(self, points, grad_points):
         """Values at points, then gradients (x y z) at grad_points, of any
         interpolant (interpolant.hpp:58-67)."""

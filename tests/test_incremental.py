@@ -96,6 +96,44 @@ def test_incremental_fit_meets_tolerance_everywhere():
 
 
 @pytest.mark.gpu
+def test_incremental_fit_with_gradient_data(orc):
+    # test/interpolation/test_incremental_fitter.cpp: 10,000 points and 10,000
+    # gradient points, triharmonic3d with anisotropy, degree 1, tolerance 1e-2
+    # on the values and 1e-1 on the gradients; the fit from the chosen centres
+    # is checked at every point and gradient point by the oracle's direct
+    # operator (the targets enter as centres of weight zero).
+    rng = np.random.default_rng(41)
+    pts = rng.uniform(-1, 1, (10_000, 3))
+    gpts = rng.uniform(-1, 1, (10_000, 3))
+    vals = np.sin(np.pi * pts @ A.T).sum(axis=1)
+    grads = (np.pi * np.cos(np.pi * gpts @ A.T)) @ A
+    data = np.concatenate([vals, grads.ravel()])
+    tol, gtol = 1e-2, 1e-1
+    rbf = pa.Triharmonic3D([1.0])
+    rbf.set_anisotropy(A)
+    ip = pa.Interpolant(pa.Model(rbf, poly_degree=1))
+    ip.fit_incrementally_hermite(pts, gpts, data, tol, gtol, max_iter=100)
+    c, g, w = ip.centers(), ip.grad_centers(), ip.weights()
+    st = ip.fit_stats()
+    print(f"incremental with gradients: {len(c)} points and {len(g)} gradient points as centres in "
+          f"{st['rounds']} rounds, {st['iterations']} FGMRES iterations", flush=True)
+    assert w.shape == (len(c) + 3 * len(g) + 4,)
+    assert len(c) < len(pts) and len(g) < len(gpts)
+    mu, sg = len(c), len(g)
+    x = np.concatenate([w[:mu], np.zeros(len(pts)), w[mu:mu + 3 * sg], np.zeros(3 * len(gpts)), w[-4:]])
+    r = orc.make_rbf(orc.TH3, [1.0, 0.0], A)
+    y = orc.hermite_operator([r], 1, 0.0, np.vstack([c, pts]), np.vstack([g, gpts]), x)
+    fv = y[mu:mu + len(pts)]
+    fg = y[mu + len(pts) + 3 * sg:mu + len(pts) + 3 * sg + 3 * len(gpts)]
+    assert np.abs(fv - vals).max() < tol
+    assert np.abs(fg - grads.ravel()).max() < gtol
+    # the interpolant's own evaluation agrees
+    f = ip.evaluate_hermite(pts[:2000], gpts[:2000])
+    assert np.abs(f[:2000] - fv[:2000]).max() <= 1e-6
+    assert np.abs(f[2000:] - fg[:6000]).max() <= 1e-5
+
+
+@pytest.mark.gpu
 def test_incremental_fit_without_polynomial():
     # An exponential model: kernel without polynomial (degree -1 allowed,
     # cpd order 0); the first centre is the point of largest |value|.
