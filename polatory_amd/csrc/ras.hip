@@ -1856,11 +1856,12 @@ std::pair<std::vector<int32_t>, std::vector<int32_t>> choose_coarse_points_mixed
 // ---- RasPreconditioner ----
 
 RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<double>& points,
-                                     hipStream_t s, const RasOptions& opt)
-    : RasPreconditioner(model, points, std::vector<double>{}, s, opt) {}
+                                     hipStream_t s, const RasOptions& opt, const double* d_points)
+    : RasPreconditioner(model, points, std::vector<double>{}, s, opt, d_points) {}
 
 RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<double>& points,
-                                     const std::vector<double>& grad_points, hipStream_t s, const RasOptions& opt)
+                                     const std::vector<double>& grad_points, hipStream_t s, const RasOptions& opt,
+                                     const double* d_points)
     : model_(model),
       s_(s),
       opt_(opt),
@@ -1876,7 +1877,12 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
     throw_invalid("the coarse grid size must be in [1, " + std::to_string(RasOptions::kMaxCoarsest) + "]");
   h_pts_ = points;
   h_gpts_ = grad_points;
-  d_pts_.upload(h_pts_.data(), h_pts_.size(), s_);
+  if (d_points) {
+    d_pts_.resize(h_pts_.size());
+    dev_copy(d_points, d_pts_.get(), static_cast<int64_t>(h_pts_.size()), s_);
+  } else {
+    d_pts_.upload(h_pts_.data(), h_pts_.size(), s_);
+  }
   d_gpts_.upload(h_gpts_.data(), h_gpts_.size(), s_);
   n_levels_ = std::max(static_cast<int>(std::ceil(std::log(static_cast<double>(n_) / kNCoarsestPoints) /
                                                   std::log(kFineToCoarseRatio))), 0) + 1;
@@ -1938,8 +1944,15 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
   std::iota(gpoint_idcs_[n_levels_ - 1].begin(), gpoint_idcs_[n_levels_ - 1].end(), 0);
   // Domain geometry in the anisotropic frame when there is a single rbf
   // (ras_preconditioner.hpp:105-113).
-  std::vector<double> a_pts = h_pts_, a_gpts = h_gpts_;
-  if (model_.rbfs.size() == 1 && !is_identity(model_.rbfs[0].aniso)) {
+  const bool transform = model_.rbfs.size() == 1 && !is_identity(model_.rbfs[0].aniso);
+  std::vector<double> a_pts_t, a_gpts_t;
+  if (transform) {
+    a_pts_t = h_pts_;
+    a_gpts_t = h_gpts_;
+  }
+  std::vector<double>& a_pts = transform ? a_pts_t : h_pts_;
+  std::vector<double>& a_gpts = transform ? a_gpts_t : h_gpts_;
+  if (transform) {
     const double* a = model_.rbfs[0].aniso;
     for (int64_t i = 0; i < mu_; i++) {
       const double* p = &h_pts_[3 * i];
@@ -1956,16 +1969,18 @@ RasPreconditioner::RasPreconditioner(const Model& model, const std::vector<doubl
   std::future<void> ap_done;
   if (n_levels_ > 1 && l_ > 0) {
     p_orth_.resize(static_cast<size_t>(n_) * l_);
-    if (sigma_ == 0) dev_poly_matrix(model_.poly_degree, d_pts_.get(), mu_, p_orth_.get(), s_);
-    else dev_poly_matrix_hermite(model_.poly_degree, d_pts_.get(), mu_, d_gpts_.get(), sigma_, p_orth_.get(), s_);
-    dev_orthonormalize_cols(p_orth_.get(), n_, l_, ws_, s_);
-    PAM_HIP(hipStreamSynchronize(s_));
+    PAM_HIP(hipStreamSynchronize(s_));  // the points are on the device
     int dev = 0;
     PAM_HIP(hipGetDevice(&dev));
     ap_done = std::async(std::launch::async, [this, dev] {
       PAM_HIP(hipSetDevice(dev));
       hipStream_t as = stream_acquire();
       try {
+        // P, orthonormalized (ras_preconditioner.hpp:168-171), then A P
+        Workspace1 ws;
+        if (sigma_ == 0) dev_poly_matrix(model_.poly_degree, d_pts_.get(), mu_, p_orth_.get(), as);
+        else dev_poly_matrix_hermite(model_.poly_degree, d_pts_.get(), mu_, d_gpts_.get(), sigma_, p_orth_.get(), as);
+        dev_orthonormalize_cols(p_orth_.get(), n_, l_, ws, as);
         compute_ap(as);
       } catch (...) {
         stream_release(as);

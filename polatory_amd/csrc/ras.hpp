@@ -101,12 +101,14 @@ struct RasOptions {
 
 class RasPreconditioner : public LinearOperator {
  public:
+  // d_points: the same points already on the device (optional; copied on the
+  // device instead of uploaded again).
   RasPreconditioner(const Model& model, const std::vector<double>& points, hipStream_t s,
-                    const RasOptions& opt);
+                    const RasOptions& opt, const double* d_points = nullptr);
   // With gradient points (ras_preconditioner.hpp:56-189): vectors have
   // mu + 3 sigma + l entries ([values; gradients x y z per point; polynomial]).
   RasPreconditioner(const Model& model, const std::vector<double>& points, const std::vector<double>& grad_points,
-                    hipStream_t s, const RasOptions& opt);
+                    hipStream_t s, const RasOptions& opt, const double* d_points = nullptr);
   void apply(const double* d_v, double* d_out) override;
   int64_t size() const override { return n_ + l_; }
   int64_t n_grad_points() const { return sigma_; }

@@ -191,7 +191,8 @@ Solver::Solver(const Model& model, const std::vector<double>& points, const FitO
       stream_release(ps);
     });
   }
-  if (opt_.use_preconditioner) pc_ = std::make_unique<RasPreconditioner>(model_, points, s_, fit_ras_options(model_, opt_, mu_));
+  if (opt_.use_preconditioner)
+    pc_ = std::make_unique<RasPreconditioner>(model_, points, s_, fit_ras_options(model_, opt_, mu_), d_pts_.get());
   if (l_ > 0) {
     p_orth_.resize(static_cast<size_t>(mu_) * l_);
     dev_poly_matrix(model_.poly_degree, d_pts_.get(), mu_, p_orth_.get(), s_);
@@ -653,7 +654,7 @@ HermiteSolver::HermiteSolver(const Model& model, const std::vector<double>& poin
       ro.use_level_cfg = true;
     }
     ro.level_cfg = fp64_at_least(ro.level_cfg, 8);
-    pc_ = std::make_unique<RasPreconditioner>(model_, points, grad_points, s_, ro);
+    pc_ = std::make_unique<RasPreconditioner>(model_, points, grad_points, s_, ro, d_pts_.get());
   }
   if (l_ > 0) {
     p_orth_.resize(static_cast<size_t>(n_) * l_);
