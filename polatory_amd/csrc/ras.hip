@@ -1558,16 +1558,10 @@ std::vector<int32_t> choose_coarse_points(const std::vector<double>& pts, const 
     c.axes = bb.axes_by_width();
   };
   auto split_mid = [](int64_t sz) {
-    // median split (ties broken towards an even index, domain_divider.hpp:82-88)
-    int64_t mid = 0, bestd = INT64_MAX;
-    for (int64_t i = 0; i < sz; i++) {
-      int64_t dd = std::llabs(2 * i - sz);
-      if (dd < bestd || (dd == bestd && i % 2 == 0)) {
-        bestd = dd;
-        mid = i;
-      }
-    }
-    return mid;
+    // median split (ties broken towards an even index, domain_divider.hpp:82-88):
+    // the index i minimising |2 i - sz|, the even one of the two when sz is odd
+    if (sz % 2 == 0) return sz / 2;
+    return ((sz - 1) / 2) % 2 == 0 ? (sz - 1) / 2 : (sz + 1) / 2;
   };
   std::vector<int32_t> out(poly_idcs);
   if (all.empty()) return out;
