@@ -236,14 +236,22 @@ def fit_section(pa):
     """1,000,002-point ellipsoid fit, biharmonic3d degree 1, absolute tolerance 1e-5:
     mixed-precision operator (FP32 P2P, FP32-MFMA M2L, FP64 refinement) vs FP64
     everywhere (operator and RAS level updates as FP64 FMM of the same order).
-    The final residual is recomputed in FP64 at every site by the solver."""
+    The final residual is recomputed in FP64 at every site by the solver.
+    Each form is fitted once untimed first, so that one-time costs of the
+    process (code objects, M2L operators, cached device buffers and streams)
+    fall on neither timed fit."""
     pts, vals = ellipsoid_points(333_334)
     out = {"workload": "ellipsoid surface fit, 1,000,002 points, bh3 degree 1, tol 1e-5 absolute"}
     sub = np.random.default_rng(18).permutation(len(pts))[:10_000]
-    for name, opt in (("mixed", pa.FitOptions()),
-                      ("fp64", pa.FitOptions(op=pa.FmmConfig("fp64", 8, -1, "auto"),
-                                             level=pa.FmmConfig("fp64", 8, -1, "auto"),
-                                             refine_rtol=0.0, ras_fp32_factors=False))):
+    forms = (("mixed", pa.FitOptions()),
+             ("fp64", pa.FitOptions(op=pa.FmmConfig("fp64", 8, -1, "auto"),
+                                    level=pa.FmmConfig("fp64", 8, -1, "auto"),
+                                    refine_rtol=0.0, ras_fp32_factors=False)))
+    for name, opt in forms:
+        warm = pa.Interpolant(pa.Model(pa.Biharmonic3D(), poly_degree=1))
+        warm.fit(pts, vals, 1e-5, max_iter=100, options=opt)
+        del warm
+    for name, opt in forms:
         ip = pa.Interpolant(pa.Model(pa.Biharmonic3D(), poly_degree=1))
         t0 = time.perf_counter()
         ip.fit(pts, vals, 1e-5, max_iter=100, options=opt)
