@@ -604,7 +604,8 @@ def main():
     # L2P: FP64 locals of every target leaf + FP64 tree-order targets read, FP64 far field written
     l2p_bytes = m["n_trg_leaves"] * p**3 * 8 + N_TRG * (32 + 8)
     stage_roofs = {
-        "m2l": {"kernel": "m2l_pair_kernel<float,float> + m2l_pair_reduce_kernel<float>", "bound": "mfma",
+        "m2l": {"kernel": ("m2l_batch_kernel<double,double> + " if r > 0 else "")
+                + "m2l_pair_kernel<float,float> + m2l_pair_reduce_kernel<float>", "bound": "mfma",
                 "achieved": m2l_ach / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
                 "frac": m2l_ach / PEAK_FP32, "flops": m2l_flops, "cell_pairs": m["m2l_pairs"]},
         "p2p": {"kernel": "p2p_jobs_kernel<float,0,false>", "bound": "valu (fp32)", "achieved": p2p_ach / 1e12,

@@ -155,8 +155,12 @@ typedef struct {
   int order;      /* Chebyshev points per axis, 3..12 */
   int leaf_level; /* octree leaf level 2..18, or -1 for automatic */
   int mode;       /* PAM_MODE_* */
-  double svd_tolerance; /* M2L compression cutoff sigma_r/sigma_0 when > 0;
-                           0 or negative = dense M2L (the default) */
+  double svd_tolerance; /* M2L compression: > 0 = compressed transfers whose
+                           basis leaves an estimated relative residual within
+                           this value (rank a multiple of 64, at most 256);
+                           0 = dense M2L operators; negative = the default
+                           (compressed at 2e-7 for biharmonic and triharmonic
+                           kernels at orders 7 and 8, dense otherwise) */
 } pam_fmm_config;
 int pam_fmm_evaluator_set_config(pam_fmm_evaluator* h, const pam_fmm_config* cfg);
 int pam_fmm_evaluator_get_config(pam_fmm_evaluator* h, pam_fmm_config* cfg);

@@ -1506,6 +1506,7 @@ m2l_pair_kernel(M2LPairArgs<TM, PT> b) {
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const int r0 = rb * kM2LR + wr * 64 + i * 16;
+        if (r0 >= rpad) continue;  // the half block past a 64-row padding
         if constexpr (sizeof(TM) == 4 && sizeof(PT) == 4) {
           using f32x4 = float __attribute__((ext_vector_type(4)));
           f32x4 v = {static_cast<float>(d[i][j][0]), static_cast<float>(d[i][j][1]),
@@ -1685,7 +1686,9 @@ void build_m2l_schedule(const std::vector<const M2LLevel*>& lvls, int nrowblk, M
 template <class TM>
 void launch_m2l_dense(const M2LOperator& op, const std::vector<M2LDenseLevel<TM>>& levels,
                       const int32_t* vlist, M2LSchedule& sched, DevBuf<double>& partial, hipStream_t s) {
-  if (op.rpad % kM2LR != 0) throw_runtime("M2L operator rows are not padded to 128");
+  // Rows padded to 64; a final half block of 64 rows reads into the operator's
+  // slack (64 zero rows after its last transfer vector) and writes no partials.
+  if (op.rpad % 64 != 0) throw_runtime("M2L operator rows are not padded to 64");
   if (op.kpad % M2LTile<TM>::kK != 0) throw_runtime("M2L reduction length is not a multiple of the chunk");
   M2LDenseArgs<TM> b{};
   b.r = op.r;
@@ -1793,7 +1796,9 @@ void build_m2l_pair_schedule(const std::vector<const M2LLevel*>& lvls, int nrowb
 template <class TM, class PT>
 void launch_m2l_pairs(const M2LOperator& op, const std::vector<M2LDenseLevel<TM>>& levels,
                       M2LPairSchedule& sched, DevBuf<PT>& partial, hipStream_t s) {
-  if (op.rpad % kM2LR != 0) throw_runtime("M2L operator rows are not padded to 128");
+  // Rows padded to 64; a final half block of 64 rows reads into the operator's
+  // slack (64 zero rows after its last transfer vector) and writes no partials.
+  if (op.rpad % 64 != 0) throw_runtime("M2L operator rows are not padded to 64");
   if (op.kpad % M2LTile<TM>::kK != 0) throw_runtime("M2L reduction length is not a multiple of the chunk");
   M2LPairArgs<TM, PT> b{};
   M2LPairReduceArgs<PT> rb{};
@@ -1830,7 +1835,7 @@ void launch_m2l_pairs(const M2LOperator& op, const std::vector<M2LDenseLevel<TM>
   if (lvls.empty()) return;
   rb.n = static_cast<int>(lvls.size());
   rb.cell_off[rb.n] = cells;
-  if (!sched.valid) build_m2l_pair_schedule(lvls, op.rpad / kM2LR, sched, s);
+  if (!sched.valid) build_m2l_pair_schedule(lvls, (op.rpad + kM2LR - 1) / kM2LR, sched, s);
   b.sched = reinterpret_cast<const int4*>(sched.d.get());
   if (sched.n_wg > 0) {
     m2l_pair_kernel<TM, PT><<<static_cast<unsigned>(sched.n_wg), kThreads, 0, s>>>(b);

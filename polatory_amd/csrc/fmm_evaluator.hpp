@@ -26,10 +26,14 @@ struct FmmConfig {
   int order = 6;
   int leaf_level = -1;  // -1: chosen from the point counts
   int mode = kModeAuto;
-  // M2L compression: relative singular-value cutoff > 0 selects the
-  // SVD-compressed M2L; 0 or negative = dense M2L (the default).
+  // M2L compression: > 0 selects compressed transfers whose basis leaves an
+  // estimated relative residual within this value (rank a multiple of 64, at
+  // most 256); 0 = dense operators; negative = the default (kDefaultSvdTol for
+  // the homogeneous kernels at orders 7 and 8, dense otherwise).
   double svd_tol = -1.0;
 };
+
+constexpr double kDefaultSvdTol = 2e-7;
 
 // Compression tolerance actually used for a configuration and kernel.
 double effective_svd_tol(const FmmConfig& c, const RbfDesc& rbf);
@@ -50,6 +54,7 @@ struct FmmState {
   std::vector<DevBuf<double>> m, l;    // per level, P^3 node values
   std::vector<DevBuf<float>> m32;      // per level, FP32 copy of m (mixed-form M2L operand)
   std::vector<DevBuf<double>> mt, lt;  // per level, compressed coefficients
+  std::vector<DevBuf<float>> mt32;     // per level, FP32 copy of mt (mixed-form transfer operand)
   DevBuf<double> y_far;
   DevBuf<double> m2l_partial;          // split-reduction partial sums of the dense M2L
   M2LSchedule m2l_sched;

@@ -78,10 +78,13 @@ void gemm(const GemmArgs& g, int batch, const double* a, const double* b, double
 // launch_m2l: ((b * (kpad/16) + k/16) * rpad + row) * 16 + k % 16, zero padded.
 template <class T>
 __global__ void pack_fragment_kernel(const double* __restrict__ src, int batch, int rows, int cols,
-                                     int rpad, int kpad, T* __restrict__ dst) {
+                                     int rpad, int kpad, int64_t slack, T* __restrict__ dst) {
   int64_t per = static_cast<int64_t>(rpad) * kpad;
   int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (idx >= batch * per) return;
+  if (idx >= batch * per) {
+    if (idx < batch * per + slack) dst[idx] = T(0);
+    return;
+  }
   int b = static_cast<int>(idx / per);
   int64_t rem = idx % per;
   int kb = static_cast<int>(rem / (static_cast<int64_t>(rpad) * 16));
@@ -91,145 +94,146 @@ __global__ void pack_fragment_kernel(const double* __restrict__ src, int batch, 
   dst[idx] = static_cast<T>(v);
 }
 
+// row_pad: row padding of the layout (128 for launch_m2l; 64 for the pair-major
+// transfer, whose workgroups read 128 rows: `slack` zero rows past the last
+// batch keep the reads of a final half block in bounds).
 template <class T>
-void pack(const double* src, int batch, int rows, int cols, M2LOperator& op, DevBuf<T>& dst, hipStream_t s) {
+void pack(const double* src, int batch, int rows, int cols, M2LOperator& op, DevBuf<T>& dst, hipStream_t s,
+          int row_pad = 128, int64_t slack = 0) {
   op.r = rows;
-  op.rpad = rpad_of(rows);
+  op.rpad = (rows + row_pad - 1) / row_pad * row_pad;
   op.kpad = (cols + 15) / 16 * 16;
   op.per_v = static_cast<size_t>(op.rpad) * op.kpad;
-  dst.resize(static_cast<size_t>(batch) * op.per_v);
   int64_t total = static_cast<int64_t>(batch) * op.per_v;
-  pack_fragment_kernel<T><<<ceil_div(total, 256), 256, 0, s>>>(src, batch, rows, cols, op.rpad, op.kpad, dst.get());
+  dst.resize(static_cast<size_t>(total + slack));
+  pack_fragment_kernel<T><<<ceil_div(total + slack, 256), 256, 0, s>>>(src, batch, rows, cols, op.rpad, op.kpad,
+                                                                       slack, dst.get());
   PAM_LAUNCH_CHECK();
 }
 
-}  // namespace
+// Standard normal samples from a counter hash (splitmix64 finaliser, Box-Muller),
+// reproducible for a given seed.
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x ^= x >> 30;
+  x *= 0xbf58476d1ce4e5b9ULL;
+  x ^= x >> 27;
+  x *= 0x94d049bb133111ebULL;
+  x ^= x >> 31;
+  return x;
+}
 
-void jacobi_eigen(std::vector<double>& a, int n, std::vector<double>& evals, std::vector<double>& vecs) {
-  // Pad to an even size with a zero row/column so that every round pairs all indices.
-  int m = n + (n & 1);
-  std::vector<double> A(static_cast<size_t>(m) * m, 0.0), V(static_cast<size_t>(m) * m, 0.0);
-  for (int i = 0; i < n; i++)
-    for (int j = 0; j < n; j++) A[static_cast<size_t>(i) * m + j] = a[static_cast<size_t>(i) * n + j];
-  for (int i = 0; i < m; i++) V[static_cast<size_t>(i) * m + i] = 1.0;
-  std::vector<int> pos(m);
-  for (int i = 0; i < m; i++) pos[i] = i;
-  std::vector<double> cs(m / 2), sn(m / 2);
-  std::vector<int> pp(m / 2), qq(m / 2);
-  double total = 0.0;
-  for (double x : A) total += x * x;
-  for (int sweep = 0; sweep < 40; sweep++) {
-    double off = 0.0;
-    for (int i = 0; i < m; i++)
-      for (int j = 0; j < m; j++)
-        if (i != j) off += A[static_cast<size_t>(i) * m + j] * A[static_cast<size_t>(i) * m + j];
-    if (off <= 1e-30 * total) break;
-    for (int round = 0; round < m - 1; round++) {
-      for (int k = 0; k < m / 2; k++) {
-        int p = std::min(pos[k], pos[m - 1 - k]), q = std::max(pos[k], pos[m - 1 - k]);
-        pp[k] = p;
-        qq[k] = q;
-        double apq = A[static_cast<size_t>(p) * m + q];
-        if (std::abs(apq) < 1e-300) {
-          cs[k] = 1.0;
-          sn[k] = 0.0;
-          continue;
-        }
-        double tau = (A[static_cast<size_t>(q) * m + q] - A[static_cast<size_t>(p) * m + p]) / (2.0 * apq);
-        double t = (tau >= 0 ? 1.0 : -1.0) / (std::abs(tau) + std::sqrt(1.0 + tau * tau));
-        cs[k] = 1.0 / std::sqrt(1.0 + t * t);
-        sn[k] = t * cs[k];
+__global__ void gaussian_kernel(uint64_t seed, int64_t n, double* __restrict__ out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t a = mix64(seed + 2 * static_cast<uint64_t>(i)), b = mix64(seed + 2 * static_cast<uint64_t>(i) + 1);
+  const double u1 = (static_cast<double>(a >> 11) + 1.0) * 0x1.0p-53;  // (0, 1]
+  const double u2 = static_cast<double>(b >> 11) * 0x1.0p-53;
+  out[i] = sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
+}
+
+void gaussian(uint64_t seed, int64_t n, double* out, hipStream_t s) {
+  gaussian_kernel<<<ceil_div(n, 256), 256, 0, s>>>(seed, n, out);
+  PAM_LAUNCH_CHECK();
+}
+
+// Orthonormal columns of the row-major n x k matrix y, in place: modified
+// Gram-Schmidt applied twice per column ("twice is enough", Giraud et al. 2005).
+void orthonormalize_columns(std::vector<double>& y, int n, int k) {
+  std::vector<double> col(n);
+  for (int j = 0; j < k; j++) {
+    for (int i = 0; i < n; i++) col[i] = y[static_cast<size_t>(i) * k + j];
+    for (int pass = 0; pass < 2; pass++)
+      for (int q = 0; q < j; q++) {
+        double d = 0.0;
+        for (int i = 0; i < n; i++) d += y[static_cast<size_t>(i) * k + q] * col[i];
+        for (int i = 0; i < n; i++) col[i] -= d * y[static_cast<size_t>(i) * k + q];
       }
-      // columns of A and V (all rotations of the round touch disjoint columns)
-#pragma omp parallel for schedule(static)
-      for (int row = 0; row < m; row++) {
-        double* ar = &A[static_cast<size_t>(row) * m];
-        double* vr = &V[static_cast<size_t>(row) * m];
-        for (int k = 0; k < m / 2; k++) {
-          double c = cs[k], s = sn[k];
-          if (s == 0.0) continue;
-          int p = pp[k], q = qq[k];
-          double x = ar[p], y = ar[q];
-          ar[p] = c * x - s * y;
-          ar[q] = s * x + c * y;
-          x = vr[p];
-          y = vr[q];
-          vr[p] = c * x - s * y;
-          vr[q] = s * x + c * y;
-        }
-      }
-      // rows of A
-#pragma omp parallel for schedule(static)
-      for (int k = 0; k < m / 2; k++) {
-        double c = cs[k], s = sn[k];
-        if (s == 0.0) continue;
-        double* ap = &A[static_cast<size_t>(pp[k]) * m];
-        double* aq = &A[static_cast<size_t>(qq[k]) * m];
-        for (int col = 0; col < m; col++) {
-          double x = ap[col], y = aq[col];
-          ap[col] = c * x - s * y;
-          aq[col] = s * x + c * y;
-        }
-      }
-      int last = pos[m - 1];
-      for (int i = m - 1; i > 1; i--) pos[i] = pos[i - 1];
-      pos[1] = last;
-    }
-  }
-  std::vector<int> order(m);
-  for (int i = 0; i < m; i++) order[i] = i;
-  std::sort(order.begin(), order.end(), [&](int x, int y) {
-    return A[static_cast<size_t>(x) * m + x] > A[static_cast<size_t>(y) * m + y];
-  });
-  evals.assign(n, 0.0);
-  vecs.assign(static_cast<size_t>(n) * n, 0.0);
-  int k = 0;
-  for (int oi = 0; oi < m && k < n; oi++) {
-    int c = order[oi];
-    // skip the padding direction (a unit vector on the added index)
-    if (m != n && std::abs(V[static_cast<size_t>(n) * m + c]) > 0.5) continue;
-    evals[k] = A[static_cast<size_t>(c) * m + c];
-    for (int i = 0; i < n; i++) vecs[static_cast<size_t>(i) * n + k] = V[static_cast<size_t>(i) * m + c];
-    k++;
+    double nrm = 0.0;
+    for (int i = 0; i < n; i++) nrm += col[i] * col[i];
+    nrm = std::sqrt(nrm);
+    if (!(nrm > 0.0)) throw_runtime("M2L compression: sample matrix lost rank");
+    for (int i = 0; i < n; i++) y[static_cast<size_t>(i) * k + j] = col[i] / nrm;
   }
 }
+
+}  // namespace
 
 template <class T>
 void build_m2l_compressed(int p, const RbfDesc& rbf, double h, double rel_tol, CompressedM2L<T>& out,
                           hipStream_t s) {
   const int p3 = p * p * p;
   out.p = p;
+  const int kmax = std::min(kMaxM2LRank, p3 / 64 * 64);
+  if (kmax < 64) throw_invalid("M2L compression needs an order of at least 4");
   PhiConsts pc = phi_consts(rbf, h);
-  int64_t per_v = static_cast<int64_t>(p3) * p3;
+  const int64_t per_v = static_cast<int64_t>(p3) * p3;
+  const int64_t n_wide = 343 * static_cast<int64_t>(p3);  // columns of the stacked [K_v]_v
   DevBuf<double> kv(static_cast<size_t>(343 * per_v));
   dense_kv_kernel<<<ceil_div(343 * per_v, 256), 256, 0, s>>>(p, rbf.kind, pc.a, pc.b, kv.get());
   PAM_LAUNCH_CHECK();
-  // G = sum_v K_v K_v^T
-  DevBuf<double> g(static_cast<size_t>(p3) * p3);
-  GemmArgs ga{p3, p3, p3, 343, 0, per_v, p3, 1, 0, per_v, 1, p3, 0, p3, 1};
-  gemm(ga, 1, kv.get(), kv.get(), g.get(), s);
-  std::vector<double> hg(static_cast<size_t>(p3) * p3);
-  PAM_HIP(hipMemcpyAsync(hg.data(), g.get(), hg.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  // Range of K = [K_v]_v by randomized subspace iteration (Halko, Martinsson
+  // and Tropp 2011, algorithms 4.1 and 4.4): Q1 = orth(K W), Q = orth(K K^T Q1)
+  // with W an n_wide x kmax Gaussian matrix. Unlike the eigenvectors of
+  // sum_v K_v K_v^T, the basis resolves directions down to rounding of K itself.
+  DevBuf<double> w(static_cast<size_t>(n_wide) * kmax), y(static_cast<size_t>(p3) * kmax),
+      z(static_cast<size_t>(n_wide) * kmax);
+  gaussian(0x5eed0001ULL, n_wide * kmax, w.get(), s);
+  // Y = sum_v K_v W_v (W_v: rows v p3 .. v p3 + p3 of W)
+  const GemmArgs stack{p3, kmax, p3, 343, 0, per_v, p3, 1, 0, static_cast<int64_t>(p3) * kmax, kmax, 1, 0, kmax, 1};
+  gemm(stack, 1, kv.get(), w.get(), y.get(), s);
+  std::vector<double> hq(static_cast<size_t>(p3) * kmax);
+  PAM_HIP(hipMemcpyAsync(hq.data(), y.get(), hq.size() * sizeof(double), hipMemcpyDeviceToHost, s));
   PAM_HIP(hipStreamSynchronize(s));
-  std::vector<double> evals, vecs;
-  jacobi_eigen(hg, p3, evals, vecs);
-  out.sv.resize(p3);
-  for (int i = 0; i < p3; i++) out.sv[i] = std::sqrt(std::max(evals[i], 0.0));
-  int r = p3;
-  for (int k = 16; k < p3; k += 16) {
-    if (out.sv[k] <= rel_tol * out.sv[0]) {
-      r = k;
+  orthonormalize_columns(hq, p3, kmax);
+  DevBuf<double> dq;
+  dq.upload(hq.data(), hq.size(), s);
+  // Z_v = K_v^T Q1, then Y = sum_v K_v Z_v
+  const GemmArgs kt{p3, kmax, p3, 1, per_v, 0, 1, p3, 0, 0, kmax, 1, static_cast<int64_t>(p3) * kmax, kmax, 1};
+  gemm(kt, 343, kv.get(), dq.get(), z.get(), s);
+  gemm(stack, 1, kv.get(), z.get(), y.get(), s);
+  PAM_HIP(hipMemcpyAsync(hq.data(), y.get(), hq.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  PAM_HIP(hipStreamSynchronize(s));
+  orthonormalize_columns(hq, p3, kmax);
+  // Rank: the first multiple of 64 whose leading columns Q_r leave
+  // max |E - Q_r Q_r^T E| <= rel_tol max |E| for a fresh sample E = K W'
+  // of 8 columns (the a-posteriori estimate of Halko et al., section 4.3).
+  constexpr int kProbe = 8;
+  gaussian(0x5eed0002ULL, n_wide * kProbe, w.get(), s);
+  const GemmArgs probe{p3, kProbe, p3, 343, 0, per_v, p3, 1, 0, static_cast<int64_t>(p3) * kProbe, kProbe, 1, 0,
+                       kProbe, 1};
+  gemm(probe, 1, kv.get(), w.get(), y.get(), s);
+  std::vector<double> he(static_cast<size_t>(p3) * kProbe);
+  PAM_HIP(hipMemcpyAsync(he.data(), y.get(), he.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  PAM_HIP(hipStreamSynchronize(s));
+  double emax = 0.0;
+  for (double e : he) emax = std::max(emax, std::abs(e));
+  out.sv.clear();
+  int r = kmax;
+  for (int rr = 64; rr <= kmax; rr += 64) {
+    double res = 0.0;
+    for (int c = 0; c < kProbe; c++) {
+      std::vector<double> proj(rr, 0.0);
+      for (int i = 0; i < p3; i++)
+        for (int q = 0; q < rr; q++) proj[q] += hq[static_cast<size_t>(i) * kmax + q] * he[static_cast<size_t>(i) * kProbe + c];
+      for (int i = 0; i < p3; i++) {
+        double v = he[static_cast<size_t>(i) * kProbe + c];
+        for (int q = 0; q < rr; q++) v -= hq[static_cast<size_t>(i) * kmax + q] * proj[q];
+        res = std::max(res, std::abs(v));
+      }
+    }
+    out.sv.push_back(emax > 0.0 ? res / emax : 0.0);
+    if (emax == 0.0 || res <= rel_tol * emax) {
+      r = rr;
       break;
     }
   }
-  r = std::min(r, p3);
   out.r = r;
-  out.kpad_r = (r + 15) / 16 * 16;
+  out.kpad_r = r;
   std::vector<double> hu(static_cast<size_t>(p3) * r), hut(static_cast<size_t>(r) * p3);
   for (int i = 0; i < p3; i++)
     for (int k = 0; k < r; k++) {
-      hu[static_cast<size_t>(i) * r + k] = vecs[static_cast<size_t>(i) * p3 + k];
-      hut[static_cast<size_t>(k) * p3 + i] = vecs[static_cast<size_t>(i) * p3 + k];
+      hu[static_cast<size_t>(i) * r + k] = hq[static_cast<size_t>(i) * kmax + k];
+      hut[static_cast<size_t>(k) * p3 + i] = hq[static_cast<size_t>(i) * kmax + k];
     }
   DevBuf<double> du, dut, tv(static_cast<size_t>(343) * p3 * r), cv(static_cast<size_t>(343) * r * r);
   du.upload(hu.data(), hu.size(), s);
@@ -239,9 +243,10 @@ void build_m2l_compressed(int p, const RbfDesc& rbf, double h, double rel_tol, C
   gemm(g1, 343, kv.get(), du.get(), tv.get(), s);
   GemmArgs g2{r, r, p3, 1, 0, 0, p3, 1, static_cast<int64_t>(p3) * r, 0, r, 1, static_cast<int64_t>(r) * r, r, 1};
   gemm(g2, 343, dut.get(), tv.get(), cv.get(), s);
-  pack<T>(cv.get(), 343, r, r, out.c_op, out.c, s);
-  pack<T>(dut.get(), 1, r, p3, out.ut_op, out.ut, s);
-  pack<T>(du.get(), 1, p3, r, out.u_op, out.u, s);
+  // C_v in the pair-major layout (rows padded to 64, 64 zero rows of slack x 16).
+  pack<T>(cv.get(), 343, r, r, out.c_op, out.c, s, 64, 64 * 16);
+  pack<double>(dut.get(), 1, r, p3, out.ut_op, out.ut, s);
+  pack<double>(du.get(), 1, p3, r, out.u_op, out.u, s);
   PAM_HIP(hipStreamSynchronize(s));  // temporaries go out of scope
 }
 

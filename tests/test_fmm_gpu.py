@@ -174,6 +174,47 @@ def test_fmm_matches_direct(fmm_problem, kind, prec, order, tol):
     assert e < tol
 
 
+def run_fmm(kind, cfg, src, w, trg):
+    ev = pa.make_fmm_evaluator(make(kind), ((0, 0, 0), (1, 1, 1)))
+    ev.set_config(cfg)
+    ev.set_source_points(src)
+    ev.set_weights(w)
+    ev.set_target_points(trg)
+    y = ev.evaluate()
+    return y, ev.stats()
+
+
+@pytest.mark.parametrize("kind", ["bh3", "th3"])
+def test_compressed_m2l(fmm_problem, kind):
+    """Compressed transfers (the default at orders 7 and 8 for the homogeneous
+    kernels): rank 128, the dense form's accuracy bound against the FP64 direct
+    sum, and, at the largest rank (256), the FP64 dense result to 5e-9 (measured
+    1.4e-9 for bh3, 9e-11 for th3): the basis is orthonormal and
+    C_v = U^T K_v U is assembled correctly."""
+    src, trg, w, refs = fmm_problem
+    log("compressed m2l", kind)
+    y_dense, st = run_fmm(kind, pa.FmmConfig("fp64", 7, 3, "fmm", svd_tol=0.0), src, w, trg)
+    assert st["m2l_rank"] == 0
+    y_full, st = run_fmm(kind, pa.FmmConfig("fp64", 7, 3, "fmm", svd_tol=1e-14), src, w, trg)
+    assert st["m2l_rank"] == 256
+    d = rel_err(y_full, y_dense)
+    log(f"  rank 256 vs dense (FP64): {d:.2e}")
+    assert d <= 5e-9
+    for prec, tol in (("fp64", 1e-6), ("mixed", 2e-6)):
+        y, st = run_fmm(kind, pa.FmmConfig(prec, 7, 3, "fmm"), src, w, trg)
+        e = rel_err(y, refs[kind])
+        log(f"  default {prec}: rank {st['m2l_rank']}, rel err {e:.2e} (dense FP64 {rel_err(y_dense, refs[kind]):.2e})")
+        assert st["m2l_rank"] == 128
+        assert e <= tol
+
+
+def test_exponential_kernel_keeps_dense_m2l(fmm_problem):
+    src, trg, w, refs = fmm_problem
+    y, st = run_fmm("exp", pa.FmmConfig("mixed", 7, 3, "fmm"), src, w, trg)
+    assert st["m2l_rank"] == 0
+    assert rel_err(y, refs["exp"]) <= 1e-5
+
+
 @pytest.mark.parametrize("prec,order,tol", [("mixed", 6, 1e-5), ("fp64", 10, 1e-8)])
 def test_symmetric_fmm_matches_direct(orc, prec, order, tol):
     log("symmetric", prec, order)
@@ -341,7 +382,8 @@ def test_headline_config_within_1e6_relative(headline_problem, prec):
     st = ev.stats()
     per_leaf = len(src) / st["n_src_leaves"]
     e = rel_err(y[sub], ref)
-    log(f"  leaf level {st['leaf_level']}, {per_leaf:.0f} sources per leaf, rel err {e:.3e}")
+    log(f"  leaf level {st['leaf_level']}, {per_leaf:.0f} sources per leaf, M2L rank {st['m2l_rank']}, "
+        f"rel err {e:.3e}")
     assert st["direct"] == 0
     assert per_leaf >= 150
     assert e <= 1e-6
