@@ -678,18 +678,12 @@ void FmmEvaluator::run_fmm(double* d_out) {
     sp2p = sst.spts64.get();
     tp2p = tpts64;
   }
-  // Compressed transfers need lists with columns; they are rebuilt before
-  // the fork, as the near field reads the lists on the second stream.
-  const double tol = effective_svd_tol(cfg_, rbf_);
-  if (tol > 0.0 && !lists_.has_columns) {
-    build_lists(tt, stree, lists_, ws_, stream_, true);
-    lists_gen_++;
-  }
+  if (!so_) prepare_source_multipoles<TP>();
+  record(2);
   // Fork: the near field (VALU work) writes the output on the second stream
-  // while the upward pass, M2L (MFMA work) and L2L run here; L2P adds the far
-  // field after the join (near + far, the same sum in the same order as one
-  // stream would do). Without overlap the near field runs on this stream
-  // after L2L.
+  // while M2L (MFMA work) and L2L run here; L2P adds the far field after the
+  // join (near + far, the same sum in the same order as one stream would do).
+  // Without overlap the near field runs on this stream after L2L.
   auto near_field = [&](hipStream_t ps) {
     if (defer_) {
       // Recorded for a launch shared with other evaluations (value kernel).
@@ -711,14 +705,17 @@ void FmmEvaluator::run_fmm(double* d_out) {
   };
   const bool overlap = overlap_ && !defer_;
   if (overlap) near_field(p2p_stream_);
-  if (!so_) prepare_source_multipoles<TP>();
-  record(2);
   const int deg = homogeneity(rbf_);
   const int32_t* vl = octant_vlist_device(stream_);
+  const double tol = effective_svd_tol(cfg_, rbf_);
   stats_.m2l_rank = 0;
   for (int l = 2; l <= L; l++) st.l[l].resize(static_cast<size_t>(tt.levels[l].n) * kp);
   if (tol > 0.0) {
     // SVD-compressed M2L: compress multipoles, transfer in rank r, expand locals.
+    if (!lists_.has_columns) {
+      build_lists(tt, stree, lists_, ws_, stream_, true);
+      lists_gen_++;
+    }
     if (st.mt.size() < static_cast<size_t>(L) + 1) st.mt.resize(L + 1);
     if (st.lt.size() < static_cast<size_t>(L) + 1) st.lt.resize(L + 1);
     if (sizeof(TP) == 4 && st.mt32.size() < static_cast<size_t>(L) + 1) st.mt32.resize(L + 1);
