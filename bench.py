@@ -509,6 +509,8 @@ def main():
     ap.add_argument("--order64", type=int, default=0,
                     help="FP64-form order (0: the accuracy search's choice at 1e-6 x max|y|)")
     ap.add_argument("--leaf-level", type=int, default=-1)
+    ap.add_argument("--m2l-dense", action="store_true",
+                    help="dense M2L operators instead of the default compressed transfers (svd_tol 0)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fit", action="store_true", help="skip the 1M-point fit comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
@@ -558,6 +560,8 @@ def main():
              else search_config(pa, src, w, "mixed", acc))
     cfg64 = (pa.FmmConfig("fp64", args.order64, args.leaf_level, "fmm") if args.order64 > 0
              else search_config(pa, src, w, "fp64", acc))
+    if args.m2l_dense:
+        cfg32.svd_tol = cfg64.svd_tol = 0.0
     log(f"configurations: mixed {cfg32}, fp64 {cfg64} (accuracy {acc:.3e})")
 
     results = {}
