@@ -1542,7 +1542,7 @@ struct M2LPairReduceArgs {  // TM: partial type
 
 constexpr int kPairReduceThreads = 128;
 
-template <class TM>
+template <class TM, int kRows>
 __global__ void __launch_bounds__(kPairReduceThreads) m2l_pair_reduce_kernel(M2LPairReduceArgs<TM> b) {
   __shared__ int32_t pos[kM2LSlots + 7];
   const int64_t g = blockIdx.x;
@@ -1553,11 +1553,14 @@ __global__ void __launch_bounds__(kPairReduceThreads) m2l_pair_reduce_kernel(M2L
     pos[sl] = sl < kM2LSlots ? b.pair_pos[k][cell * kM2LSlots + sl] : -1;
   __syncthreads();
   const TM* __restrict__ partial = b.partial[k];
-  // Three rows per thread and eight slots per trip keep 24 loads in flight;
-  // the additions still run in slot order.
-  constexpr int kRows = 3, kSl = 8;
+  // kRows rows per thread (three for dense operators, one for compressed
+  // ones of at most 128 rows) and eight slots per trip keep up to 24 loads in
+  // flight; the additions still run in slot order.
+  constexpr int kSl = 8;
   for (int row0 = 0; row0 < b.r; row0 += kRows * kPairReduceThreads) {
-    double acc[kRows] = {0.0, 0.0, 0.0};
+    double acc[kRows];
+#pragma unroll
+    for (int q = 0; q < kRows; q++) acc[q] = 0.0;
     int row[kRows];
 #pragma unroll
     for (int q = 0; q < kRows; q++) row[q] = min(row0 + q * kPairReduceThreads + static_cast<int>(threadIdx.x), b.r - 1);
@@ -1841,7 +1844,10 @@ void launch_m2l_pairs(const M2LOperator& op, const std::vector<M2LDenseLevel<TM>
     m2l_pair_kernel<TM, PT><<<static_cast<unsigned>(sched.n_wg), kThreads, 0, s>>>(b);
     PAM_LAUNCH_CHECK();
   }
-  m2l_pair_reduce_kernel<PT><<<static_cast<unsigned>(cells), kPairReduceThreads, 0, s>>>(rb);
+  if (op.r <= kPairReduceThreads)
+    m2l_pair_reduce_kernel<PT, 1><<<static_cast<unsigned>(cells), kPairReduceThreads, 0, s>>>(rb);
+  else
+    m2l_pair_reduce_kernel<PT, 3><<<static_cast<unsigned>(cells), kPairReduceThreads, 0, s>>>(rb);
   PAM_LAUNCH_CHECK();
 }
 
